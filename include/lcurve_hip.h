@@ -1,0 +1,158 @@
+/*
+ * lcurve_hip.h — C ABI of the MI355X (gfx950) light-curve flux engine.
+ *
+ * Drop-in replacement for the GPU backend of lcurve-rs and for the CPU
+ * flux loop behind its light-curve entry points. Every entry point returns
+ * an int status (LCH_OK = 0, negative LCH_ERR_* on failure; HIP error codes
+ * are never passed through raw) and records a message retrievable with
+ * lch_last_error() on the calling thread. Host buffers are caller-owned;
+ * device memory is owned by the context / batch and freed by its destroy
+ * call. All pointers are plain host pointers; no HIP or torch types appear.
+ *
+ * Reference interfaces replaced (paths relative to the lcurve-rs tree):
+ *   lch_device_count   <- lcuda_device_count      crates/lcurve-cuda/kernels/flux_kernel.cu:557-562
+ *   lch_ctx_create     <- CudaContext::new        crates/lcurve-cuda/src/context.rs:261-276
+ *                         lcuda_init               crates/lcurve-cuda/kernels/flux_kernel.cu:494-496
+ *   lch_ctx_destroy    <- Drop for CudaContext    crates/lcurve-cuda/src/context.rs:148-252
+ *   lch_chisq_batch    <- chisq_batch_gpu         crates/lcurve-cuda/src/batch.rs:73-195
+ *                         chisq_batch (CPU)        crates/lcurve/src/orchestration.rs:329-357
+ *   lch_light_curve    <- light_curve_comp        crates/lcurve/src/orchestration.rs:36-318
+ *   lch_batch_build    <- prepare_model + upload  crates/lcurve-cuda/src/batch.rs:129-156, 227-531
+ *   lch_batch_launch   <- lcuda_flux_eval_stream  crates/lcurve-cuda/kernels/flux_kernel.cu:594-615
+ *   lch_batch_run      <- launch + D2H + compute_chisq  crates/lcurve-cuda/src/batch.rs:174-222, 533-568
+ *   lch_ctx_sync       <- lcuda_stream_sync       crates/lcurve-cuda/kernels/flux_kernel.cu:577-579
+ *   lch_model_*        <- Model::from_file / set_param_value / get_param_value
+ *                                                 crates/lcurve/src/model.rs:186-532
+ *   lch_data_*         <- read_data               crates/lcurve/src/types.rs:229-254
+ * The reference's lcuda_malloc / memcpy / stream / host_alloc exports exist
+ * because its Rust host manages device buffers; here the library owns all
+ * device memory, so they have no counterpart.
+ */
+#ifndef LCURVE_HIP_H
+#define LCURVE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LCH_OK 0
+#define LCH_ERR_INVALID (-1)     /* bad argument */
+#define LCH_ERR_PARSE (-2)       /* model or data file could not be parsed */
+#define LCH_ERR_SETUP (-3)       /* model setup failed (e.g. star larger than its Roche lobe) */
+#define LCH_ERR_DEVICE (-4)      /* no usable HIP device */
+#define LCH_ERR_RUNTIME (-5)     /* HIP runtime failure during a call */
+#define LCH_ERR_UNSUPPORTED (-6) /* model feature the engine does not implement */
+#define LCH_ERR_NOMEM (-7)       /* host allocation failed */
+
+typedef struct lch_ctx lch_ctx;
+typedef struct lch_model lch_model;
+typedef struct lch_batch lch_batch;
+
+/* Scalars of one light curve (orchestration.rs:9-28 LcResult). */
+typedef struct lch_lc_result {
+  double wdwarf, chisq, wnok, logg1, logg2, rv1, rv2;
+  double sfac[5];
+} lch_lc_result;
+
+typedef struct lch_batch_stats {
+  long long n_sets, n_valid, n_times, n_lanes, n_items;
+  long long visits;       /* element-visits in the reference's loop order */
+  long long record_bytes; /* resident element records in HBM */
+  double setup_seconds;   /* CPU setup + packing wall time */
+  double upload_seconds;  /* host-to-device copies and table builds */
+  int chunk;              /* elements per work item */
+  int persistent;         /* 1 if the persistent launch form is selected */
+} lch_batch_stats;
+
+int lch_version(void);
+const char* lch_last_error(void);
+
+/* ---- device context ---- */
+int lch_device_count(void);
+int lch_ctx_create(int device, lch_ctx** out);
+void lch_ctx_destroy(lch_ctx* ctx);
+/* Override the launch tuning (chunk > 0 and persistent in {0,1}). */
+int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
+int lch_ctx_sync(lch_ctx* ctx);
+
+/* ---- model parameter files ---- */
+int lch_model_load_file(const char* path, lch_model** out);
+int lch_model_load_text(const char* text, lch_model** out);
+int lch_model_copy(const lch_model* m, lch_model** out);
+void lch_model_free(lch_model* m);
+/* Physical parameters by name; LCH_ERR_INVALID for an unknown name. */
+int lch_model_set_param(lch_model* m, const char* name, double value);
+int lch_model_get_param(const lch_model* m, const char* name, double* value);
+int lch_model_get_pparam(const lch_model* m, const char* name, double* value, double* range,
+                         double* dstep, int* vary, int* defined);
+/* Null pointers leave the field unchanged. */
+int lch_model_set_pparam(lch_model* m, const char* name, const double* value, const double* range,
+                         const double* dstep, const int* vary);
+int lch_param_count(void);
+const char* lch_param_name(int i);
+
+/* ---- data files (time expose flux ferr weight ndiv) ---- */
+int lch_data_count(const char* path, int* n);
+int lch_data_read(const char* path, int n, double* time, double* expose, double* flux,
+                  double* ferr, double* weight, int* ndiv);
+
+/* ---- light curve of one model ----
+ * calc_out[n] receives the fluxes (autoscaled when scale != 0). rdata != 0
+ * computes chi-squared against flux/ferr/weight as light_curve_comp does. */
+int lch_light_curve(lch_ctx* ctx, const lch_model* m, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, int scale, int rdata, double* calc_out, lch_lc_result* res);
+
+/* ---- batched chi-squared ----
+ * values is row-major [nsets][ndim]; names[ndim] are parameter names (an
+ * unknown name is ignored, as set_param_value's result is in the reference).
+ * chisq_out[nsets]; NaN for a set whose setup fails. */
+int lch_chisq_batch(lch_ctx* ctx, const lch_model* base, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, const char* const* names, int ndim, const double* values, int nsets,
+                    int scale, double* chisq_out);
+
+/* ---- resident batches: build once, launch many times ----
+ * flux2d/ferr2d/weight2d, when non-null, give each set its own observations
+ * [nsets][n]. flags bit 0 keeps element maps for lch_batch_visibility. */
+#define LCH_BATCH_KEEP_MAPS 1
+int lch_batch_build(lch_ctx* ctx, const lch_model* base, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, const char* const* names, int ndim, const double* values, int nsets,
+                    const double* flux2d, const double* ferr2d, const double* weight2d, int flags,
+                    lch_batch** out);
+/* Same, with one fully specified model per set (e.g. per-set grid
+ * resolution, which is not a physical parameter). */
+int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nsets,
+                           const double* time, const double* expose, const double* flux,
+                           const double* ferr, const double* weight, const int* ndiv, int n,
+                           const double* flux2d, const double* ferr2d, const double* weight2d,
+                           int flags, lch_batch** out);
+void lch_batch_free(lch_batch* b);
+/* Enqueue one pass (flux + assembly) on the context stream; no copy back. */
+int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale);
+/* One pass, wait, copy back (any output may be null). flux is [nsets][n]. */
+int lch_batch_run(lch_ctx* ctx, lch_batch* b, int scale, double* chisq, double* wnok,
+                  double* sfac, double* wdwarf, double* flux);
+/* Mean per-pass milliseconds of the flux kernel and of the whole pass over
+ * `reps` passes, from HIP events on the context stream. */
+int lch_batch_time(lch_ctx* ctx, lch_batch* b, int scale, int reps, double* ms_flux,
+                   double* ms_total);
+/* Try the candidate chunk sizes / launch forms and keep the fastest. */
+int lch_batch_autotune(lch_ctx* ctx, lch_batch* b, int scale);
+int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out);
+/* Per-set setup status and per-curve scalars; err may be null. */
+int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, double* logg2,
+                       double* rv1, double* rv2, char* err, int errlen);
+/* Eclipse visibility computed on the GPU for one grid (0 star1f, 1 star1c,
+ * 2 star2f, 3 star2c, 4 disc, 5 edge, 6 spot) of one set at nphase phases:
+ * mask[p * nelem + e], element order of the CPU setup. mask may be null to
+ * query nelem. Needs LCH_BATCH_KEEP_MAPS. */
+int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
+                         int nphase, unsigned char* mask, int* nelem);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LCURVE_HIP_H */

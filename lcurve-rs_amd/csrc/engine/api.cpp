@@ -1,0 +1,428 @@
+// extern "C" entry points declared in include/lcurve_hip.h.
+#include "../../../include/lcurve_hip.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.h"
+
+struct lch_ctx {
+  lch::Context impl;
+  explicit lch_ctx(int dev) : impl(dev) {}
+};
+struct lch_model {
+  lc::Model impl;
+};
+struct lch_batch {
+  std::unique_ptr<lch::Batch> impl;
+  double setup_s = 0, upload_s = 0;
+  long long n_valid = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Runs f and maps exceptions to status codes.
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const lch::DeviceError& e) {
+    return fail(LCH_ERR_RUNTIME, e.what());
+  } catch (const lc::LcError& e) {
+    std::string m = e.what();
+    return fail(m.rfind("parse error", 0) == 0 || m.rfind("I/O error", 0) == 0 ? LCH_ERR_PARSE
+                                                                             : LCH_ERR_SETUP,
+                m);
+  } catch (const std::bad_alloc&) {
+    return fail(LCH_ERR_NOMEM, "host allocation failed");
+  } catch (const std::invalid_argument& e) {
+    return fail(LCH_ERR_INVALID, e.what());
+  } catch (const std::exception& e) {
+    return fail(LCH_ERR_RUNTIME, e.what());
+  }
+}
+
+lc::Data make_data(const double* time, const double* expose, const double* flux, const double* ferr,
+                   const double* weight, const int* ndiv, int n) {
+  if (n <= 0) throw std::invalid_argument("no data points");
+  if (!time || !expose || !flux || !ferr || !weight || !ndiv)
+    throw std::invalid_argument("null data array");
+  lc::Data d(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    d[i].time = time[i];
+    d[i].expose = expose[i];
+    d[i].flux = flux[i];
+    d[i].ferr = ferr[i];
+    d[i].weight = weight[i];
+    d[i].ndiv = ndiv[i];
+  }
+  return d;
+}
+
+void check_unsupported(const lc::Model& m) {
+  if (m.iscale)
+    throw lc::LcError("iscale=1 (per-component scale factors) is not supported by the HIP engine");
+}
+
+// Setup + pack + upload of a batch of models.
+lch_batch* build_batch(lch_ctx* ctx, const std::vector<lc::Model>& models,
+                       const std::vector<lc::Data>& data, bool keep_maps) {
+  for (const lc::Model& m : models) check_unsupported(m);
+  auto b = std::make_unique<lch_batch>();
+  double t0 = now_s();
+  auto packs = lch::build_packs(models, data, 0, keep_maps);
+  double t1 = now_s();
+  for (const auto& p : packs) b->n_valid += p->host.valid ? 1 : 0;
+  size_t nt = data[0].size();
+  b->impl = std::make_unique<lch::Batch>(ctx->impl, std::move(packs), nt);
+  b->setup_s = t1 - t0;
+  b->upload_s = now_s() - t1;
+  return b.release();
+}
+
+std::vector<lc::Model> apply_sets(const lc::Model& base, const char* const* names, int ndim,
+                                  const double* values, int nsets) {
+  if (nsets < 0 || ndim < 0) throw std::invalid_argument("negative batch size");
+  if (nsets > 0 && ndim > 0 && (!names || !values)) throw std::invalid_argument("null names/values");
+  std::vector<lc::Model> models(static_cast<size_t>(nsets), base);
+  for (int s = 0; s < nsets; ++s)
+    for (int j = 0; j < ndim; ++j)
+      models[s].set_param_value(names[j], values[static_cast<size_t>(s) * ndim + j]);
+  return models;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lch_version(void) { return 1; }
+const char* lch_last_error(void) { return g_err.c_str(); }
+
+int lch_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int lch_ctx_create(int device, lch_ctx** out) {
+  if (!out) return fail(LCH_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  try {
+    *out = new lch_ctx(device);
+    return LCH_OK;
+  } catch (const std::exception& e) {
+    return fail(LCH_ERR_DEVICE, e.what());
+  }
+}
+
+void lch_ctx_destroy(lch_ctx* ctx) { delete ctx; }
+
+int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent) {
+  if (!ctx || chunk <= 0 || (persistent != 0 && persistent != 1))
+    return fail(LCH_ERR_INVALID, "bad tuning");
+  ctx->impl.tuning.chunk = static_cast<uint32_t>(chunk);
+  ctx->impl.tuning.persistent = persistent;
+  return LCH_OK;
+}
+
+int lch_ctx_sync(lch_ctx* ctx) {
+  if (!ctx) return fail(LCH_ERR_INVALID, "null context");
+  hipError_t e = hipStreamSynchronize(ctx->impl.stream());
+  return e == hipSuccess ? LCH_OK : fail(LCH_ERR_RUNTIME, hipGetErrorString(e));
+}
+
+int lch_model_load_file(const char* path, lch_model** out) {
+  if (!path || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    *out = new lch_model{lc::Model::from_file(path)};
+    return LCH_OK;
+  });
+}
+
+int lch_model_load_text(const char* text, lch_model** out) {
+  if (!text || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    *out = new lch_model{lc::Model::from_text(text)};
+    return LCH_OK;
+  });
+}
+
+int lch_model_copy(const lch_model* m, lch_model** out) {
+  if (!m || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    *out = new lch_model{m->impl};
+    return LCH_OK;
+  });
+}
+
+void lch_model_free(lch_model* m) { delete m; }
+
+int lch_model_set_param(lch_model* m, const char* name, double value) {
+  if (!m || !name) return fail(LCH_ERR_INVALID, "null argument");
+  if (!m->impl.set_param_value(name, value))
+    return fail(LCH_ERR_INVALID, std::string("unknown parameter: '") + name + "'");
+  return LCH_OK;
+}
+
+int lch_model_get_param(const lch_model* m, const char* name, double* value) {
+  if (!m || !name || !value) return fail(LCH_ERR_INVALID, "null argument");
+  const lc::Pparam* p = m->impl.pparam(name);
+  if (!p) return fail(LCH_ERR_INVALID, std::string("unknown parameter: '") + name + "'");
+  *value = p->value;
+  return LCH_OK;
+}
+
+int lch_model_get_pparam(const lch_model* m, const char* name, double* value, double* range,
+                         double* dstep, int* vary, int* defined) {
+  if (!m || !name) return fail(LCH_ERR_INVALID, "null argument");
+  const lc::Pparam* p = m->impl.pparam(name);
+  if (!p) return fail(LCH_ERR_INVALID, std::string("unknown parameter: '") + name + "'");
+  if (value) *value = p->value;
+  if (range) *range = p->range;
+  if (dstep) *dstep = p->dstep;
+  if (vary) *vary = p->vary ? 1 : 0;
+  if (defined) *defined = p->defined ? 1 : 0;
+  return LCH_OK;
+}
+
+int lch_model_set_pparam(lch_model* m, const char* name, const double* value, const double* range,
+                         const double* dstep, const int* vary) {
+  if (!m || !name) return fail(LCH_ERR_INVALID, "null argument");
+  lc::Pparam* p = m->impl.pparam(name);
+  if (!p) return fail(LCH_ERR_INVALID, std::string("unknown parameter: '") + name + "'");
+  if (value) p->value = *value;
+  if (range) p->range = *range;
+  if (dstep) p->dstep = *dstep;
+  if (vary) p->vary = *vary != 0;
+  return LCH_OK;
+}
+
+int lch_param_count(void) { return static_cast<int>(lc::pparam_names().size()); }
+
+const char* lch_param_name(int i) {
+  const auto& v = lc::pparam_names();
+  if (i < 0 || static_cast<size_t>(i) >= v.size()) return nullptr;
+  return v[static_cast<size_t>(i)].c_str();
+}
+
+int lch_data_count(const char* path, int* n) {
+  if (!path || !n) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    *n = static_cast<int>(lc::read_data(path).size());
+    return LCH_OK;
+  });
+}
+
+int lch_data_read(const char* path, int n, double* time, double* expose, double* flux, double* ferr,
+                  double* weight, int* ndiv) {
+  if (!path) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    lc::Data d = lc::read_data(path);
+    if (static_cast<int>(d.size()) != n) throw std::invalid_argument("data size changed");
+    for (int i = 0; i < n; ++i) {
+      time[i] = d[i].time;
+      expose[i] = d[i].expose;
+      flux[i] = d[i].flux;
+      ferr[i] = d[i].ferr;
+      weight[i] = d[i].weight;
+      ndiv[i] = d[i].ndiv;
+    }
+    return LCH_OK;
+  });
+}
+
+int lch_light_curve(lch_ctx* ctx, const lch_model* m, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, int scale, int rdata, double* calc_out, lch_lc_result* res) {
+  if (!ctx || !m || !calc_out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    std::vector<lc::Data> data{make_data(time, expose, flux, ferr, weight, ndiv, n)};
+    std::unique_ptr<lch_batch> b(build_batch(ctx, {m->impl}, data, false));
+    const lch::SetHost& H = b->impl->set(0);
+    if (!H.valid) throw lc::LcError(H.error);
+    double chisq = 0, wnok = 0, sfac = 1, wd = 0;
+    b->impl->run(scale ? 1 : 0, &chisq, &wnok, &sfac, &wd, calc_out);
+    if (res) {
+      res->wdwarf = wd;
+      res->chisq = (scale || rdata) ? chisq : 0.0;
+      res->wnok = (scale || rdata) ? wnok : 0.0;
+      res->logg1 = H.logg1;
+      res->logg2 = H.logg2;
+      res->rv1 = H.rv1;
+      res->rv2 = H.rv2;
+      for (double& s : res->sfac) s = scale ? sfac : 1.0;
+    }
+    return LCH_OK;
+  });
+}
+
+int lch_chisq_batch(lch_ctx* ctx, const lch_model* base, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, const char* const* names, int ndim, const double* values, int nsets,
+                    int scale, double* chisq_out) {
+  if (!ctx || !base || (nsets > 0 && !chisq_out)) return fail(LCH_ERR_INVALID, "null argument");
+  if (ndim <= 0) return fail(LCH_ERR_INVALID, "ndim must be > 0");
+  return guarded([&] {
+    if (nsets == 0) return LCH_OK;
+    std::vector<lc::Data> data{make_data(time, expose, flux, ferr, weight, ndiv, n)};
+    auto models = apply_sets(base->impl, names, ndim, values, nsets);
+    std::unique_ptr<lch_batch> b(build_batch(ctx, models, data, false));
+    b->impl->run(scale ? 1 : 0, chisq_out, nullptr, nullptr, nullptr, nullptr);
+    return LCH_OK;
+  });
+}
+
+namespace {
+lch_batch* build_with_obs(lch_ctx* ctx, const std::vector<lc::Model>& models, const double* time,
+                          const double* expose, const double* flux, const double* ferr,
+                          const double* weight, const int* ndiv, int n, const double* flux2d,
+                          const double* ferr2d, const double* weight2d, int flags) {
+    const int nsets = static_cast<int>(models.size());
+    std::vector<lc::Data> data;
+    if (flux2d || ferr2d || weight2d) {
+      data.resize(static_cast<size_t>(nsets));
+      for (int s = 0; s < nsets; ++s) {
+        size_t o = static_cast<size_t>(s) * n;
+        data[s] = make_data(time, expose, flux2d ? flux2d + o : flux, ferr2d ? ferr2d + o : ferr,
+                            weight2d ? weight2d + o : weight, ndiv, n);
+      }
+    } else {
+      data.push_back(make_data(time, expose, flux, ferr, weight, ndiv, n));
+    }
+    return build_batch(ctx, models, data, (flags & LCH_BATCH_KEEP_MAPS) != 0);
+}
+}  // namespace
+
+int lch_batch_build(lch_ctx* ctx, const lch_model* base, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, const char* const* names, int ndim, const double* values, int nsets,
+                    const double* flux2d, const double* ferr2d, const double* weight2d, int flags,
+                    lch_batch** out) {
+  if (!ctx || !base || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    if (nsets <= 0) throw std::invalid_argument("nsets must be > 0");
+    auto models = apply_sets(base->impl, names, ndim, values, nsets);
+    *out = build_with_obs(ctx, models, time, expose, flux, ferr, weight, ndiv, n, flux2d, ferr2d,
+                          weight2d, flags);
+    return LCH_OK;
+  });
+}
+
+int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nsets,
+                           const double* time, const double* expose, const double* flux,
+                           const double* ferr, const double* weight, const int* ndiv, int n,
+                           const double* flux2d, const double* ferr2d, const double* weight2d,
+                           int flags, lch_batch** out) {
+  if (!ctx || !models || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    if (nsets <= 0) throw std::invalid_argument("nsets must be > 0");
+    std::vector<lc::Model> ms;
+    ms.reserve(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) {
+      if (!models[s]) throw std::invalid_argument("null model in batch");
+      ms.push_back(models[s]->impl);
+    }
+    *out = build_with_obs(ctx, ms, time, expose, flux, ferr, weight, ndiv, n, flux2d, ferr2d,
+                          weight2d, flags);
+    return LCH_OK;
+  });
+}
+
+void lch_batch_free(lch_batch* b) { delete b; }
+
+int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale) {
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->launch(scale ? 1 : 0);
+    return LCH_OK;
+  });
+}
+
+int lch_batch_run(lch_ctx* ctx, lch_batch* b, int scale, double* chisq, double* wnok, double* sfac,
+                  double* wdwarf, double* flux) {
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->run(scale ? 1 : 0, chisq, wnok, sfac, wdwarf, flux);
+    return LCH_OK;
+  });
+}
+
+int lch_batch_time(lch_ctx* ctx, lch_batch* b, int scale, int reps, double* ms_flux,
+                   double* ms_total) {
+  if (!ctx || !b || reps <= 0) return fail(LCH_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    b->impl->time(scale ? 1 : 0, reps, ms_flux, ms_total);
+    return LCH_OK;
+  });
+}
+
+int lch_batch_autotune(lch_ctx* ctx, lch_batch* b, int scale) {
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->autotune(scale ? 1 : 0);
+    return LCH_OK;
+  });
+}
+
+int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
+  if (!b || !out) return fail(LCH_ERR_INVALID, "null argument");
+  const lch::Batch& B = *b->impl;
+  out->n_sets = B.n_sets();
+  out->n_valid = b->n_valid;
+  out->n_times = static_cast<long long>(B.n_times());
+  out->n_lanes = static_cast<long long>(B.n_lanes());
+  out->n_items = static_cast<long long>(B.n_items());
+  out->visits = static_cast<long long>(B.visits());
+  out->record_bytes = static_cast<long long>(B.record_bytes());
+  out->setup_seconds = b->setup_s;
+  out->upload_seconds = b->upload_s;
+  out->chunk = static_cast<int>(B.chunk());
+  out->persistent = B.persistent();
+  return LCH_OK;
+}
+
+int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, double* logg2,
+                       double* rv1, double* rv2, char* err, int errlen) {
+  if (!b || set < 0 || static_cast<uint32_t>(set) >= b->impl->n_sets())
+    return fail(LCH_ERR_INVALID, "bad set index");
+  const lch::SetHost& H = b->impl->set(static_cast<uint32_t>(set));
+  if (valid) *valid = H.valid ? 1 : 0;
+  if (logg1) *logg1 = H.logg1;
+  if (logg2) *logg2 = H.logg2;
+  if (rv1) *rv1 = H.rv1;
+  if (rv2) *rv2 = H.rv2;
+  if (err && errlen > 0) {
+    std::strncpy(err, H.error.c_str(), static_cast<size_t>(errlen - 1));
+    err[errlen - 1] = 0;
+  }
+  return LCH_OK;
+}
+
+int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
+                         int nphase, unsigned char* mask, int* nelem) {
+  if (!ctx || !b || !nelem) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    uint32_t n = 0;
+    b->impl->visibility(set, grid, phases, nphase, mask, &n);
+    *nelem = static_cast<int>(n);
+    return LCH_OK;
+  });
+}
+
+}  // extern "C"

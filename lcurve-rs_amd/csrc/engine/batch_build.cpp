@@ -1,0 +1,303 @@
+// Packs CPU-setup output into the device layout of layout.h.
+//
+// Per set:
+//  * every grid is split into segments of equal eclipse-pair count (stable
+//    order within a segment), so the kernel's interval loop has a fixed trip
+//    count and segments with no eclipses skip the test;
+//  * one lane per exposure sub-phase (exposure_weight, flux.rs:378-387),
+//    then the calibration lanes at the Ginterp phases of
+//    orchestration.rs:93-131 and the white-dwarf lane at phase 0.5
+//    (orchestration.rs:256-259);
+//  * per-time smearing counts, polynomial factors and observations.
+#include "batch_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
+
+namespace lch {
+
+namespace {
+
+uint32_t mask_of(uint32_t cls, std::initializer_list<uint32_t> ptypes) {
+  uint32_t m = 0;
+  for (uint32_t p : ptypes) m |= cp_bit(cls, p);
+  return m;
+}
+
+const std::initializer_list<uint32_t> kAll = {1u, 2u, 3u};
+
+struct GridSpec {
+  uint32_t id;  // 0..6 visible grids, 7.. calibration copies
+  const std::vector<lc::Point>* pts;
+  uint32_t kind;
+  uint32_t use;
+  bool posn;
+};
+
+void push_lane(SetPack& S, double phi, uint32_t cls, const lc::Ginterp& gi) {
+  S.lane_phi.push_back(phi);
+  S.lane_meta.push_back(cls | (static_cast<uint32_t>(gi.grid_type(phi)) << 2) | META_VALID);
+}
+
+}  // namespace
+
+std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& data) {
+  auto sp = std::make_unique<SetPack>();
+  SetPack& S = *sp;
+  S.host.valid = false;
+  S.host.error = err;
+  S.host.n_times = static_cast<uint32_t>(data.size());
+  S.dev.flags = 0;
+  S.dev.n_times = static_cast<uint32_t>(data.size());
+  S.dev.n_lanes = 0;
+  S.t_lane0.assign(data.size(), 0u);
+  S.t_ndiv.assign(data.size(), 0);
+  S.t_slfac.assign(data.size(), 0.0);
+  S.t_y.assign(data.size(), 0.0);
+  S.t_ferr.assign(data.size(), 1.0);
+  S.t_w.assign(data.size(), 0.0);
+  return sp;
+}
+
+std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps) {
+  auto sp = std::make_unique<SetPack>();
+  SetPack& S = *sp;
+  SetDev& d = S.dev;
+  SetHost& H = S.host;
+  H.valid = true;
+  H.logg1 = P.logg1;
+  H.logg2 = P.logg2;
+  H.rv1 = P.rv1;
+  H.rv2 = P.rv2;
+
+  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0, lens = P.glens1;
+  d.flags = SET_VALID | (P.need_cal1 ? SET_CAL1 : 0u) | (P.need_cal2 ? SET_CAL2 : 0u) |
+            (beam1 ? SET_BEAM1 : 0u) | (beam2 ? SET_BEAM2 : 0u) | (lens ? SET_LENS : 0u);
+  d.ltype1 = static_cast<uint32_t>(P.ldc1.ltype);
+  d.ltype2 = static_cast<uint32_t>(P.ldc2.ltype);
+  const double l1[4] = {P.ldc1.ldc1, P.ldc1.ldc2, P.ldc1.ldc3, P.ldc1.ldc4};
+  const double l2[4] = {P.ldc2.ldc1, P.ldc2.ldc2, P.ldc2.ldc3, P.ldc2.ldc4};
+  for (int i = 0; i < 4; ++i) {
+    d.ldc1[i] = static_cast<float>(l1[i]);
+    d.ldc2[i] = static_cast<float>(l2[i]);
+  }
+  d.mucrit1 = static_cast<float>(P.ldc1.mucrit);
+  d.mucrit2 = static_cast<float>(P.ldc2.mucrit);
+  d.lin_disc = static_cast<float>(P.lin_limb_disc);
+  d.quad_disc = static_cast<float>(P.quad_limb_disc);
+  d.beam1 = static_cast<float>(P.beam_factor1);
+  d.beam2 = static_cast<float>(P.beam_factor2);
+  d.vfac = static_cast<float>(P.vfac);
+  d.xcofm = static_cast<float>(P.xcofm);
+  d.spin1 = static_cast<float>(P.spin1);
+  d.spin2 = static_cast<float>(P.spin2);
+  d.rlens1 = static_cast<float>(P.rlens1);
+  d.cosi_lc = P.cosi_lc;
+  d.sini_lc = P.sini_lc;
+  d.cosi_rad = P.cosi_rad;
+  d.sini_rad = P.sini_rad;
+  d.phase1 = P.phase1;
+  d.phase2 = P.phase2;
+  d.third = P.third;
+
+  // Which lanes add which grid (class x grid type; see LaneClass).
+  const bool cc = P.cal_copies;
+  std::vector<GridSpec> grids;
+  if (P.star1_alias) {
+    grids.push_back({0, &P.star1f, KIND_STAR1, mask_of(CLS_NORMAL, kAll) | mask_of(CLS_WD, kAll), beam1});
+  } else {
+    uint32_t cal_f = (P.need_cal1 && !cc) ? mask_of(CLS_CAL1, {1u}) : 0u;
+    uint32_t cal_c = (P.need_cal1 && !cc) ? mask_of(CLS_CAL1, {2u, 3u}) : 0u;
+    grids.push_back({0, &P.star1f, KIND_STAR1, mask_of(CLS_NORMAL, {1u}) | mask_of(CLS_WD, {1u}) | cal_f, beam1});
+    grids.push_back({1, &P.star1c, KIND_STAR1, mask_of(CLS_NORMAL, {2u, 3u}) | mask_of(CLS_WD, {2u, 3u}) | cal_c, beam1});
+  }
+  const bool posn2 = beam2 || lens;
+  if (P.star2_alias) {
+    grids.push_back({2, &P.star2f, KIND_STAR2, mask_of(CLS_NORMAL, kAll), posn2});
+  } else {
+    uint32_t cal_f = (P.need_cal2 && !cc) ? mask_of(CLS_CAL2, {3u}) : 0u;
+    uint32_t cal_c = (P.need_cal2 && !cc) ? mask_of(CLS_CAL2, {1u, 2u}) : 0u;
+    grids.push_back({2, &P.star2f, KIND_STAR2, mask_of(CLS_NORMAL, {3u}) | cal_f, posn2});
+    grids.push_back({3, &P.star2c, KIND_STAR2, mask_of(CLS_NORMAL, {1u, 2u}) | cal_c, posn2});
+  }
+  grids.push_back({4, &P.disc, KIND_DISC, mask_of(CLS_NORMAL, kAll), false});
+  grids.push_back({5, &P.edge, KIND_EDGE, mask_of(CLS_NORMAL, kAll), false});
+  grids.push_back({6, &P.spot, KIND_SPOT, mask_of(CLS_NORMAL, kAll), false});
+  if (cc) {
+    // Calibration grids without the opaque-disc eclipses.
+    if (P.need_cal1) {
+      grids.push_back({7, &P.star1f_cal, KIND_STAR1, mask_of(CLS_CAL1, {1u}), beam1});
+      grids.push_back({8, &P.star1c_cal, KIND_STAR1, mask_of(CLS_CAL1, {2u, 3u}), beam1});
+    }
+    if (P.need_cal2) {
+      grids.push_back({9, &P.star2f_cal, KIND_STAR2, mask_of(CLS_CAL2, {3u}), posn2});
+      grids.push_back({10, &P.star2c_cal, KIND_STAR2, mask_of(CLS_CAL2, {1u, 2u}), posn2});
+    }
+  }
+
+  // Records, one segment per (grid, eclipse count).
+  uint64_t off = 0;
+  for (const GridSpec& g : grids) {
+    const std::vector<lc::Point>& pts = *g.pts;
+    if (g.id < 7) H.grids[g.id].n = static_cast<uint32_t>(pts.size());
+    if (pts.empty()) continue;
+    size_t maxe = 0;
+    for (const lc::Point& pt : pts) maxe = std::max(maxe, pt.eclipse.size());
+    if (maxe > static_cast<size_t>(MAX_ECL))
+      throw lc::LcError("element with " + std::to_string(maxe) +
+                        " eclipse intervals exceeds the kernel limit of " + std::to_string(MAX_ECL));
+    for (uint32_t k = 0; k <= maxe; ++k) {
+      std::vector<uint32_t> idx;
+      for (uint32_t e = 0; e < pts.size(); ++e)
+        if (pts[e].eclipse.size() == k) idx.push_back(e);
+      if (idx.empty()) continue;
+      const uint32_t stride = rec_stride(k, g.posn);
+      const uint32_t eoff = g.posn ? 32u : 16u;
+      off = (off + 15) / 16 * 16;
+      SegDev sg{};
+      sg.rec_off = off;
+      sg.n = static_cast<uint32_t>(idx.size());
+      sg.use_mask = g.use;
+      sg.kind = g.kind;
+      sg.necl = k;
+      sg.flags = g.posn ? SEG_POSN : 0u;
+      S.recs.resize(off + static_cast<uint64_t>(stride) * idx.size());
+      uint8_t* base = S.recs.data() + off;
+      for (size_t i = 0; i < idx.size(); ++i) {
+        const lc::Point& pt = pts[idx[i]];
+        uint8_t* r = base + i * stride;
+        const float a[4] = {static_cast<float>(pt.dirn.x), static_cast<float>(pt.dirn.y),
+                            static_cast<float>(pt.dirn.z), pt.flux};
+        std::memcpy(r, a, 16);
+        if (g.posn) {
+          const float b[4] = {static_cast<float>(pt.posn.x), static_cast<float>(pt.posn.y),
+                              static_cast<float>(pt.posn.z), 0.0f};
+          std::memcpy(r + 16, b, 16);
+        }
+        for (uint32_t j = 0; j < k; ++j) {
+          const double e3[3] = {pt.eclipse[j].first, pt.eclipse[j].second, pt.eclipse[j].second - 1.0};
+          std::memcpy(r + eoff + 24 * j, e3, 24);
+        }
+      }
+      off += static_cast<uint64_t>(stride) * idx.size();
+      S.segs.push_back(sg);
+      S.seg_grid.push_back(g.id);
+      if (keep_maps) S.seg_orig.push_back(std::move(idx));
+    }
+  }
+
+  // Lanes.
+  lc::Ginterp gi;
+  gi.phase1 = P.phase1;
+  gi.phase2 = P.phase2;
+  const size_t nt = P.phase.size();
+  S.t_lane0.resize(nt);
+  S.t_ndiv.resize(nt);
+  for (size_t t = 0; t < nt; ++t) {
+    S.t_lane0[t] = static_cast<uint32_t>(S.lane_phi.size());
+    S.t_ndiv[t] = P.ndiv[t];
+    for (int nd = 0; nd < P.ndiv[t]; ++nd) {
+      double phi, wgt;
+      lc::exposure_weight(P.phase[t], P.expose[t], nd, P.ndiv[t], &phi, &wgt);
+      push_lane(S, phi, CLS_NORMAL, gi);
+    }
+  }
+  d.special0 = static_cast<uint32_t>(S.lane_phi.size());
+  if (P.need_cal1) {
+    push_lane(S, 0.9999999999 * P.phase1, CLS_CAL1, gi);
+    push_lane(S, 1.0000000001 * P.phase1, CLS_CAL1, gi);
+    push_lane(S, 1.0 - 0.9999999999 * P.phase1, CLS_CAL1, gi);
+    push_lane(S, 1.0 - 1.0000000001 * P.phase1, CLS_CAL1, gi);
+  }
+  if (P.need_cal2) {
+    push_lane(S, 1.0 - 1.0000000001 * P.phase2, CLS_CAL2, gi);
+    push_lane(S, 1.0 - 0.9999999999 * P.phase2, CLS_CAL2, gi);
+    push_lane(S, 1.0000000001 * P.phase2, CLS_CAL2, gi);
+    push_lane(S, 0.9999999999 * P.phase2, CLS_CAL2, gi);
+  }
+  push_lane(S, 0.5, CLS_WD, gi);
+  while (S.lane_phi.size() % TILE) {
+    S.lane_phi.push_back(0.0);
+    S.lane_meta.push_back(0u);
+  }
+  d.n_lanes = static_cast<uint32_t>(S.lane_phi.size());
+  d.n_times = static_cast<uint32_t>(nt);
+
+  // Element-visits in the reference's loop order (flux.rs:24-107 and the
+  // single-star calls of orchestration.rs), used for throughput figures.
+  const uint64_t n1f = P.star1f.size(), n1c = P.star1_alias ? n1f : P.star1c.size();
+  const uint64_t n2f = P.star2f.size(), n2c = P.star2_alias ? n2f : P.star2c.size();
+  const uint64_t nother = P.disc.size() + P.edge.size() + P.spot.size();
+  uint64_t visits = 0;
+  for (size_t l = 0; l < S.lane_meta.size(); ++l) {
+    const uint32_t m = S.lane_meta[l];
+    if (!(m & META_VALID)) continue;
+    const uint32_t cls = meta_class(m), pt = meta_ptype(m);
+    const uint64_t s1 = pt == 1 ? n1f : n1c, s2 = pt == 3 ? n2f : n2c;
+    if (cls == CLS_NORMAL) visits += s1 + s2 + nother;
+    else if (cls == CLS_CAL2) visits += s2;
+    else visits += s1;
+  }
+  H.visits = visits;
+  H.n_times = static_cast<uint32_t>(nt);
+
+  // Per-time terms and observations.
+  S.t_slfac = P.slfac;
+  S.t_y.resize(nt);
+  S.t_ferr.resize(nt);
+  S.t_w.resize(nt);
+  for (size_t t = 0; t < nt; ++t) {
+    S.t_y[t] = data[t].flux;
+    S.t_ferr[t] = data[t].ferr;
+    S.t_w[t] = data[t].weight;
+  }
+  return sp;
+}
+
+std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& models,
+                                                  const std::vector<lc::Data>& data, int nthreads,
+                                                  bool keep_maps) {
+  const size_t ns = models.size();
+  std::vector<std::unique_ptr<SetPack>> packs(ns);
+  if (data.empty()) throw lc::LcError("no data");
+  auto data_of = [&](size_t s) -> const lc::Data& { return data.size() == 1 ? data[0] : data[s]; };
+  int nt = nthreads > 0 ? nthreads : static_cast<int>(std::thread::hardware_concurrency());
+  if (nt < 1) nt = 1;
+  nt = static_cast<int>(std::min<size_t>(static_cast<size_t>(nt), std::max<size_t>(ns, 1)));
+  std::atomic<size_t> next{0};
+  std::exception_ptr fatal;
+  std::mutex mu;
+  auto worker = [&]() {
+    for (;;) {
+      size_t s = next.fetch_add(1);
+      if (s >= ns) break;
+      try {
+        try {
+          auto P = lc::prepare_model(models[s], data_of(s));
+          packs[s] = pack_set(*P, data_of(s), keep_maps);
+        } catch (const lc::LcError& e) {
+          packs[s] = invalid_set(e.what(), data_of(s));
+        }
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!fatal) fatal = std::current_exception();
+        next.store(ns);
+      }
+    }
+  };
+  if (nt == 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  if (fatal) std::rethrow_exception(fatal);
+  return packs;
+}
+
+}  // namespace lch

@@ -1,0 +1,57 @@
+// Packing of one parameter set's CPU-setup output into the device layout,
+// and the threaded setup + pack of a whole batch.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "layout.h"
+#include "../setup/prepare.h"
+
+namespace lch {
+
+// Host-side description of one set of a batch.
+struct SetHost {
+  bool valid = false;
+  std::string error;
+  double logg1 = 0, logg2 = 0, rv1 = 0, rv2 = 0;
+  uint32_t n_times = 0;
+  uint64_t visits = 0;  // element-visits in the reference's loop order
+  // Debug visibility map: per grid (0 star1f, 1 star1c, 2 star2f, 3 star2c,
+  // 4 disc, 5 edge, 6 spot) its segments and each segment element's index
+  // in the grid's original order.
+  struct GridMap {
+    std::vector<uint32_t> segs;
+    std::vector<std::vector<uint32_t>> orig;
+    uint32_t n = 0;
+  } grids[7];
+};
+
+// One set, packed on the host, offsets relative to the set.
+struct SetPack {
+  SetDev dev{};
+  SetHost host;
+  std::vector<uint8_t> recs;
+  std::vector<SegDev> segs;
+  std::vector<uint32_t> seg_grid;                 // grid id per segment (>= 7: calibration copy)
+  std::vector<std::vector<uint32_t>> seg_orig;    // optional element maps
+  std::vector<double> lane_phi;
+  std::vector<uint32_t> lane_meta;
+  std::vector<uint32_t> t_lane0;
+  std::vector<int> t_ndiv;
+  std::vector<double> t_slfac, t_y, t_ferr, t_w;
+};
+
+// Pack a prepared model; `data` supplies the observations of this set.
+std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps);
+// A set whose setup failed: contributes NaN results, no work.
+std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& data);
+
+// CPU setup + packing for every set on `nthreads` threads (0 = all cores).
+// `data` holds either one data set shared by all models or one per model.
+std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& models,
+                                                  const std::vector<lc::Data>& data, int nthreads,
+                                                  bool keep_maps);
+
+}  // namespace lch

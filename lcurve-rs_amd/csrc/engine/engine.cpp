@@ -1,0 +1,438 @@
+// Host side of the flux engine: packs CPU-setup output into the HBM layout
+// of layout.h, builds work items, launches (optionally through a captured
+// HIP graph), times, autotunes and reads back results.
+#include "engine.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "batch_build.h"
+
+namespace lch {
+
+namespace {
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace
+
+// ----------------------------------------------------------------- context
+Context::Context(int device) : device_(device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw DeviceError("no HIP device available");
+  if (device < 0 || device >= n)
+    throw DeviceError("device id " + std::to_string(device) + " out of range (" + std::to_string(n) +
+                      " devices)");
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  cu_count_ = prop.multiProcessorCount;
+  // Default priority on purpose: streams of other priorities are avoided.
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Context::~Context() {
+  if (stream_) {
+    (void)hipSetDevice(device_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+// ------------------------------------------------------------------- batch
+template <class T>
+T* Batch::dalloc(size_t n) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  allocs_.push_back(p);
+  return static_cast<T*>(p);
+}
+
+void Batch::dfree_all() {
+  for (void* p : allocs_) (void)hipFree(p);
+  allocs_.clear();
+}
+
+Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times)
+    : ctx_(ctx), n_times_(n_times) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  const size_t ns = packs.size();
+  host_.resize(ns);
+  sets_.resize(ns);
+
+  // Global offsets.
+  uint64_t rec_total = 0;
+  size_t nseg = 0, nlane = 0, ntime = 0;
+  std::vector<uint64_t> rec_base(ns);
+  for (size_t s = 0; s < ns; ++s) {
+    rec_base[s] = rec_total;
+    rec_total += (packs[s]->recs.size() + 255) / 256 * 256;
+    nseg += packs[s]->segs.size();
+    nlane += packs[s]->lane_phi.size();
+    ntime += packs[s]->t_lane0.size();
+  }
+  rec_bytes_ = rec_total;
+  n_lanes_ = nlane;
+
+  char* d_recs = dalloc<char>(rec_total);
+  segs_.reserve(nseg);
+  lane_phi_.reserve(nlane);
+  lane_meta_.reserve(nlane);
+  t_lane0_.reserve(ntime);
+  t_ndiv_.reserve(ntime);
+  t_slfac_.reserve(ntime);
+  t_y_.reserve(ntime);
+  t_ferr_.reserve(ntime);
+  t_w_.reserve(ntime);
+
+  for (size_t s = 0; s < ns; ++s) {
+    SetPack& P = *packs[s];
+    SetDev sd = P.dev;
+    sd.lane_off = static_cast<uint32_t>(lane_phi_.size());
+    sd.time_off = static_cast<uint32_t>(t_lane0_.size());
+    sets_[s] = sd;
+    host_[s] = P.host;
+    SetHost& H = host_[s];
+    for (size_t g = 0; g < P.segs.size(); ++g) {
+      SegDev sg = P.segs[g];
+      sg.rec_off += rec_base[s];
+      sg.set = static_cast<uint32_t>(s);
+      uint32_t gid = static_cast<uint32_t>(segs_.size());
+      if (P.seg_grid[g] < 7) {
+        H.grids[P.seg_grid[g]].segs.push_back(gid);
+        if (!P.seg_orig.empty()) H.grids[P.seg_grid[g]].orig.push_back(std::move(P.seg_orig[g]));
+      }
+      segs_.push_back(sg);
+    }
+    lane_phi_.insert(lane_phi_.end(), P.lane_phi.begin(), P.lane_phi.end());
+    lane_meta_.insert(lane_meta_.end(), P.lane_meta.begin(), P.lane_meta.end());
+    t_lane0_.insert(t_lane0_.end(), P.t_lane0.begin(), P.t_lane0.end());
+    t_ndiv_.insert(t_ndiv_.end(), P.t_ndiv.begin(), P.t_ndiv.end());
+    t_slfac_.insert(t_slfac_.end(), P.t_slfac.begin(), P.t_slfac.end());
+    t_y_.insert(t_y_.end(), P.t_y.begin(), P.t_y.end());
+    t_ferr_.insert(t_ferr_.end(), P.t_ferr.begin(), P.t_ferr.end());
+    t_w_.insert(t_w_.end(), P.t_w.begin(), P.t_w.end());
+    if (!P.recs.empty())
+      hip_check(hipMemcpy(d_recs + rec_base[s], P.recs.data(), P.recs.size(), hipMemcpyHostToDevice),
+                "hipMemcpy records");
+    packs[s].reset();  // release host staging as we go
+  }
+
+  // Host-side consistency checks of the per-set tables.
+  for (size_t s = 0; s < ns; ++s) {
+    const SetDev& sd = sets_[s];
+    if (sd.lane_off % TILE || sd.n_lanes % TILE || sd.lane_off + sd.n_lanes > lane_phi_.size())
+      throw DeviceError("set lanes out of range");
+    if (sd.time_off + sd.n_times > t_lane0_.size()) throw DeviceError("set times out of range");
+    if ((sd.flags & SET_VALID) && sd.special0 >= sd.n_lanes) throw DeviceError("special lanes out of range");
+    for (uint32_t t = 0; t < sd.n_times; ++t) {
+      const uint64_t end = static_cast<uint64_t>(t_lane0_[sd.time_off + t]) +
+                           static_cast<uint64_t>(std::max(0, t_ndiv_[sd.time_off + t]));
+      if ((sd.flags & SET_VALID) && end > sd.special0) throw DeviceError("time lanes out of range");
+    }
+  }
+
+  // Tile class/grid-type masks.
+  tile_cp_.assign(lane_meta_.size() / TILE, 0u);
+  for (size_t l = 0; l < lane_meta_.size(); ++l) {
+    uint32_t m = lane_meta_[l];
+    if (m & META_VALID) tile_cp_[l / TILE] |= cp_bit(meta_class(m), meta_ptype(m));
+  }
+
+  d_.n_sets = static_cast<uint32_t>(ns);
+  d_.recs = d_recs;
+  upload();
+  hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
+  build_items(ctx_.tuning.chunk);
+}
+
+Batch::~Batch() {
+  (void)hipSetDevice(ctx_.device());
+  release_graph();
+  for (hipEvent_t e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (d_items_) (void)hipFree(d_items_);
+  if (d_partials_) (void)hipFree(d_partials_);
+  if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
+  if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
+  dfree_all();
+}
+
+void Batch::upload() {
+  auto up = [&](auto& vec, auto** dst) {
+    using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+    T* p = dalloc<T>(vec.size());
+    if (!vec.empty())
+      hip_check(hipMemcpy(p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    *dst = p;
+  };
+  const SegDev* segs;
+  up(segs_, const_cast<SegDev**>(&segs));
+  d_.segs = segs;
+  const double* lp;
+  up(lane_phi_, const_cast<double**>(&lp));
+  d_.lane_phi = lp;
+  const uint32_t* u;
+  up(lane_meta_, const_cast<uint32_t**>(&u));
+  d_.lane_meta = u;
+  up(tile_cp_, const_cast<uint32_t**>(&u));
+  d_.tile_cp = u;
+  up(t_lane0_, const_cast<uint32_t**>(&u));
+  d_.t_lane0 = u;
+  const int* ip;
+  up(t_ndiv_, const_cast<int**>(&ip));
+  d_.t_ndiv = ip;
+  up(t_slfac_, const_cast<double**>(&lp));
+  d_.t_slfac = lp;
+  up(t_y_, const_cast<double**>(&lp));
+  d_.t_y = lp;
+  up(t_ferr_, const_cast<double**>(&lp));
+  d_.t_ferr = lp;
+  up(t_w_, const_cast<double**>(&lp));
+  d_.t_w = lp;
+  d_.lane_tot = dalloc<double>(lane_phi_.size());
+  d_.flux = dalloc<double>(t_lane0_.size());
+  d_.out = dalloc<SetOut>(sets_.size());
+  d_.counter = dalloc<uint32_t>(1);
+  // sets_ and chunk_info_ are (re)uploaded by build_items.
+}
+
+void Batch::release_graph() {
+  if (graph_) {
+    (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+  }
+  graph_scale_ = -1;
+}
+
+void Batch::build_items(uint32_t chunk) {
+  if (chunk == 0) chunk = 1;
+  release_graph();
+  chunk_ = chunk;
+  chunk_info_.clear();
+  std::vector<ItemDev> items;
+  uint64_t prow = 0;
+  size_t seg_i = 0;
+  for (size_t s = 0; s < sets_.size(); ++s) {
+    SetDev& sd = sets_[s];
+    sd.chunk_off = static_cast<uint32_t>(chunk_info_.size());
+    // Segments of this set are contiguous in segs_.
+    size_t g0 = seg_i;
+    while (seg_i < segs_.size() && segs_[seg_i].set == s) ++seg_i;
+    std::vector<uint32_t> chunk0(seg_i - g0);
+    uint32_t nch = 0;
+    for (size_t g = g0; g < seg_i; ++g) {
+      chunk0[g - g0] = nch;
+      uint32_t c = (segs_[g].n + chunk - 1) / chunk;
+      for (uint32_t k = 0; k < c; ++k) chunk_info_.push_back(segs_[g].use_mask | (segs_[g].kind << 16));
+      nch += c;
+    }
+    sd.n_chunks = nch;
+    sd.prow_base = prow;
+    const uint32_t ntile = sd.n_lanes / TILE;
+    if (sd.flags & SET_VALID) {
+      // Items ordered (segment, chunk, tile): the tiles of one chunk run
+      // back to back and share the chunk in cache.
+      for (size_t g = g0; g < seg_i; ++g) {
+        const SegDev& sg = segs_[g];
+        uint32_t c = (sg.n + chunk - 1) / chunk;
+        for (uint32_t k = 0; k < c; ++k) {
+          for (uint32_t t = 0; t < ntile; ++t) {
+            if (!(tile_cp_[sd.lane_off / TILE + t] & sg.use_mask)) continue;
+            ItemDev it;
+            it.seg = static_cast<uint32_t>(g);
+            it.e0 = k * chunk;
+            it.ne = std::min(chunk, sg.n - k * chunk);
+            it.tile = t;
+            it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
+            items.push_back(it);
+          }
+        }
+      }
+    }
+    prow += static_cast<uint64_t>(ntile) * nch;
+  }
+  n_prows_ = prow;
+  n_items_ = items.size();
+  // Host-side bounds check of every item against the buffers it indexes.
+  for (const ItemDev& it : items) {
+    if (it.seg >= segs_.size()) throw DeviceError("item segment out of range");
+    const SegDev& sg = segs_[it.seg];
+    const SetDev& sd = sets_[sg.set];
+    if (static_cast<uint64_t>(it.e0) + it.ne > sg.n || it.ne == 0) throw DeviceError("item elements out of range");
+    if (it.tile >= sd.n_lanes / TILE) throw DeviceError("item tile out of range");
+    if (it.prow >= n_prows_ || it.prow < sd.prow_base) throw DeviceError("item partial row out of range");
+    const uint64_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
+    if (sg.rec_off + (static_cast<uint64_t>(it.e0) + it.ne) * stride > rec_bytes_)
+      throw DeviceError("item records out of range");
+    if (sg.necl > static_cast<uint32_t>(MAX_ECL)) throw DeviceError("segment eclipse count out of range");
+  }
+  if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
+
+  if (d_items_) (void)hipFree(d_items_);
+  if (d_partials_) (void)hipFree(d_partials_);
+  d_items_ = nullptr;
+  d_partials_ = nullptr;
+  hip_check(hipMalloc(reinterpret_cast<void**>(&d_items_), std::max<size_t>(items.size(), 1) * sizeof(ItemDev)),
+            "hipMalloc items");
+  hip_check(hipMalloc(reinterpret_cast<void**>(&d_partials_), std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)),
+            "hipMalloc partials");
+  if (!items.empty())
+    hip_check(hipMemcpy(d_items_, items.data(), items.size() * sizeof(ItemDev), hipMemcpyHostToDevice),
+              "hipMemcpy items");
+  // Per-set table and chunk info (reallocated: sizes change with chunk).
+  SetDev* dsets = nullptr;
+  uint32_t* dci = nullptr;
+  hip_check(hipMalloc(reinterpret_cast<void**>(&dsets), std::max<size_t>(sets_.size(), 1) * sizeof(SetDev)), "hipMalloc");
+  hip_check(hipMalloc(reinterpret_cast<void**>(&dci), std::max<size_t>(chunk_info_.size(), 1) * sizeof(uint32_t)), "hipMalloc");
+  if (!sets_.empty())
+    hip_check(hipMemcpy(dsets, sets_.data(), sets_.size() * sizeof(SetDev), hipMemcpyHostToDevice), "hipMemcpy");
+  if (!chunk_info_.empty())
+    hip_check(hipMemcpy(dci, chunk_info_.data(), chunk_info_.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "hipMemcpy");
+  if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
+  if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
+  d_.sets = dsets;
+  d_.chunk_info = dci;
+  d_.items = d_items_;
+  d_.n_items = static_cast<uint32_t>(n_items_);
+  d_.partials = d_partials_;
+}
+
+uint64_t Batch::visits() const {
+  uint64_t v = 0;
+  for (const SetHost& h : host_) v += h.visits;
+  return v;
+}
+
+void Batch::launch(int scale) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  const Tuning& T = ctx_.tuning;
+  const uint32_t pwaves = static_cast<uint32_t>(ctx_.cu_count()) * T.waves_per_cu;
+  hipStream_t st = ctx_.stream();
+  if (use_graph) {
+    if (!graph_ || graph_scale_ != scale) {
+      release_graph();
+      hipGraph_t g;
+      hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      hipError_t e1 = launch_flux_items(d_, T.persistent, pwaves, T.grab, st);
+      hipError_t e2 = launch_assemble(d_, scale, st);
+      hipError_t e3 = hipStreamEndCapture(st, &g);
+      hip_check(e1, "capture flux kernel");
+      hip_check(e2, "capture assemble kernel");
+      hip_check(e3, "hipStreamEndCapture");
+      hipError_t e4 = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      hip_check(e4, "hipGraphInstantiate");
+      graph_scale_ = scale;
+    }
+    hip_check(hipGraphLaunch(graph_, st), "hipGraphLaunch");
+  } else {
+    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st), "flux kernel launch");
+    hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+  }
+}
+
+void Batch::run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux) {
+  launch(scale);
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  std::vector<SetOut> out(sets_.size());
+  if (!out.empty())
+    hip_check(hipMemcpy(out.data(), d_.out, out.size() * sizeof(SetOut), hipMemcpyDeviceToHost), "hipMemcpy out");
+  for (size_t s = 0; s < out.size(); ++s) {
+    if (chisq) chisq[s] = out[s].chisq;
+    if (wnok) wnok[s] = out[s].wnok;
+    if (sfac) sfac[s] = out[s].scale;
+    if (wdwarf) wdwarf[s] = out[s].wdwarf;
+  }
+  if (flux && !t_lane0_.empty())
+    hip_check(hipMemcpy(flux, d_.flux, t_lane0_.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
+}
+
+void Batch::time(int scale, int reps, double* ms_flux, double* ms_total) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  const Tuning& T = ctx_.tuning;
+  const uint32_t pwaves = static_cast<uint32_t>(ctx_.cu_count()) * T.waves_per_cu;
+  hipStream_t st = ctx_.stream();
+  double tf = 0.0;
+  hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
+  for (int r = 0; r < reps; ++r) {
+    hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
+    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st), "flux kernel launch");
+    hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
+    hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+    hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    tf += ms;
+  }
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  hipEvent_t end;
+  hip_check(hipEventCreate(&end), "hipEventCreate");
+  hip_check(hipEventRecord(end, st), "hipEventRecord");
+  hip_check(hipEventSynchronize(end), "hipEventSynchronize");
+  float total = 0.0f;
+  hip_check(hipEventElapsedTime(&total, ev_[2], end), "hipEventElapsedTime");
+  (void)hipEventDestroy(end);
+  if (ms_flux) *ms_flux = reps > 0 ? tf / reps : 0.0;
+  if (ms_total) *ms_total = reps > 0 ? total / reps : 0.0;
+}
+
+void Batch::visibility(int set, int grid, const double* phases, int nphase, uint8_t* mask,
+                       uint32_t* n_out) {
+  if (set < 0 || static_cast<size_t>(set) >= host_.size()) throw std::invalid_argument("set out of range");
+  if (grid < 0 || grid > 6) throw std::invalid_argument("grid out of range");
+  const SetHost::GridMap& gm = host_[set].grids[grid];
+  *n_out = gm.n;
+  if (!mask || nphase <= 0 || gm.n == 0) return;
+  if (gm.orig.size() != gm.segs.size()) throw std::invalid_argument("element maps were not kept for this batch");
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  double* dph = nullptr;
+  hip_check(hipMalloc(reinterpret_cast<void**>(&dph), nphase * sizeof(double)), "hipMalloc");
+  hip_check(hipMemcpy(dph, phases, nphase * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  for (size_t k = 0; k < gm.segs.size(); ++k) {
+    const SegDev& sg = segs_[gm.segs[k]];
+    size_t bytes = static_cast<size_t>(sg.n) * nphase;
+    uint8_t* dm = nullptr;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&dm), std::max<size_t>(bytes, 1)), "hipMalloc");
+    hip_check(launch_visibility(d_, sg, dph, static_cast<uint32_t>(nphase), dm, ctx_.stream()), "visibility launch");
+    std::vector<uint8_t> hm(bytes);
+    hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+    hip_check(hipMemcpy(hm.data(), dm, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    (void)hipFree(dm);
+    const std::vector<uint32_t>& orig = gm.orig[k];
+    for (int p = 0; p < nphase; ++p)
+      for (uint32_t e = 0; e < sg.n; ++e) mask[static_cast<size_t>(p) * gm.n + orig[e]] = hm[static_cast<size_t>(p) * sg.n + e];
+  }
+  (void)hipFree(dph);
+}
+
+void Batch::autotune(int scale) {
+  struct Cand { uint32_t chunk; int persistent; };
+  const Cand cands[] = {{256, 0}, {512, 0}, {1024, 0}, {2048, 0}, {4096, 0}, {1024, 1}, {2048, 1}, {4096, 1}};
+  Tuning best = ctx_.tuning;
+  double best_ms = 1e300;
+  for (const Cand& c : cands) {
+    ctx_.tuning.chunk = c.chunk;
+    ctx_.tuning.persistent = c.persistent;
+    build_items(c.chunk);
+    double ms_f = 0, ms_t = 0;
+    time(scale, 1, &ms_f, &ms_t);  // warm
+    time(scale, 3, &ms_f, &ms_t);
+    if (ms_f < best_ms) {
+      best_ms = ms_f;
+      best = ctx_.tuning;
+    }
+  }
+  best.tuned = true;
+  ctx_.tuning = best;
+  build_items(best.chunk);
+}
+
+}  // namespace lch

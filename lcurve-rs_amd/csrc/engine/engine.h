@@ -1,0 +1,111 @@
+// Host side of the flux engine: device context, batch packing (CPU setup
+// output -> HBM layout of layout.h), launch, graph replay and autotuning.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_build.h"
+#include "kernels.h"
+
+namespace lch {
+
+// Raised for HIP runtime failures; the C ABI maps it to LCH_ERR_RUNTIME.
+struct DeviceError : public std::runtime_error {
+  explicit DeviceError(const std::string& m) : std::runtime_error(m) {}
+};
+
+// Launch configuration of the flux kernel; chosen by the autotuner.
+struct Tuning {
+  uint32_t chunk = 1024;        // elements per work item
+  int persistent = 0;           // 0: one workgroup per item; 1: persistent grid
+  uint32_t waves_per_cu = 8;    // persistent grid size = CUs * waves_per_cu
+  uint32_t grab = 4;            // items taken per atomic (persistent)
+  bool tuned = false;
+};
+
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  int cu_count() const { return cu_count_; }
+  Tuning tuning;
+
+ private:
+  int device_ = 0;
+  int cu_count_ = 0;
+  hipStream_t stream_ = nullptr;
+};
+
+class Batch {
+ public:
+  // Takes the packed sets (host staging is released while uploading).
+  Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  // Rebuild the work-item table for a chunk size (keeps resident data).
+  void build_items(uint32_t chunk);
+  // Enqueue flux + assembly on the context stream (graph replay if enabled).
+  void launch(int scale);
+  // Launch, wait and copy results back (any pointer may be null).
+  void run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux);
+  // Time `reps` launches of the flux kernel alone and of the whole batch
+  // with HIP events on the context stream. Returns mean milliseconds.
+  void time(int scale, int reps, double* ms_flux, double* ms_total);
+  // GPU eclipse visibility of one grid of one set at `nphase` phases,
+  // written as mask[p * n + e] in the grid's original element order.
+  void visibility(int set, int grid, const double* phases, int nphase, uint8_t* mask,
+                  uint32_t* n_out);
+  // Pick the chunk size / launch form with the fastest flux kernel.
+  void autotune(int scale);
+
+  uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
+  const SetHost& set(uint32_t i) const { return host_[i]; }
+  uint64_t visits() const;
+  uint64_t record_bytes() const { return rec_bytes_; }
+  uint64_t n_items() const { return n_items_; }
+  uint64_t n_lanes() const { return n_lanes_; }
+  size_t n_times() const { return n_times_; }
+  uint32_t chunk() const { return chunk_; }
+  int persistent() const { return ctx_.tuning.persistent; }
+  bool use_graph = true;
+
+ private:
+  void upload();
+  void release_graph();
+  template <class T>
+  T* dalloc(size_t n);
+  void dfree_all();
+
+  Context& ctx_;
+  size_t n_times_;
+  std::vector<SetHost> host_;
+  // Host mirrors (kept small: records are uploaded set by set).
+  std::vector<SetDev> sets_;
+  std::vector<SegDev> segs_;
+  std::vector<double> lane_phi_;
+  std::vector<uint32_t> lane_meta_, tile_cp_;
+  std::vector<uint32_t> t_lane0_;
+  std::vector<int> t_ndiv_;
+  std::vector<double> t_slfac_, t_y_, t_ferr_, t_w_;
+  std::vector<uint32_t> chunk_info_;
+  uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0;
+  uint32_t chunk_ = 0;
+
+  DeviceBatch d_;
+  std::vector<void*> allocs_;
+  ItemDev* d_items_ = nullptr;
+  double* d_partials_ = nullptr;
+  hipGraphExec_t graph_ = nullptr;
+  int graph_scale_ = -1;
+  hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+};
+
+}  // namespace lch

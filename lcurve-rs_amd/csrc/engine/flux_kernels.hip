@@ -1,0 +1,545 @@
+// HIP kernels of the flux engine for CDNA4 (gfx950).
+//
+// flux_items_kernel — the hot path. One wavefront = one work item =
+//   64 sub-phases (one per lane) x one chunk of surface elements. Each lane
+//   owns one sub-phase and loops over the chunk; the element record address
+//   is wave-uniform, so records arrive through scalar loads and feed the
+//   VALU as scalar operands. There is no cross-lane reduction in the inner
+//   loop. Per element and lane: FP64 eclipse-interval test (bit-exact with
+//   the reference's Point::visible, types.rs:50-58), FP32 mu, limb
+//   darkening, beaming and lensing, Kahan-compensated FP32 accumulation.
+//   The lane's chunk sum is written as an FP64 partial.
+// assemble_kernel — one workgroup per set, all FP64: sums the partials per
+//   sub-phase and component, derives the Ginterp scale factors from the
+//   calibration lanes, applies the scale factors, smears exposures, adds the
+//   polynomial factor and third light, autoscales and forms chi-squared.
+// visibility_kernel — debug: eclipse visibility of every element of one
+//   segment at a list of phases, for the bit-exact mask check.
+//
+// Reference: crates/lcurve/src/flux.rs:7-279, orchestration.rs:85-131,
+// 181-310, 361-401; crates/lcurve-cuda/kernels/flux_kernel.cu:384-485 is
+// the kernel this replaces.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace lch {
+
+namespace {
+
+constexpr double K_TWOPI = 6.283185307179586476925286766559;
+
+// ------------------------------------------------------------ helpers
+struct LaneCtx {
+  double phif;          // sub-phase minus its floor: the visibility phase
+  float ex, ey, ez;     // earth vector
+  float kx, ky, k0;     // beaming: vx = kx*py, vy = ky*px + k0
+};
+
+struct StarPar {
+  float a0, a1, a2, a3, csum, mucrit, beam, rlens;
+};
+
+// Point::visible for NECL intervals (types.rs:50-58), in FP64 so the mask is
+// identical to the reference's. egm1 is egress - 1 rounded on the host, the
+// same value the reference computes inline.
+template <int NECL>
+__device__ __forceinline__ bool eclipse_free(const double* __restrict__ e, double phif) {
+  bool ecl = false;
+#pragma unroll
+  for (int k = 0; k < NECL; ++k) {
+    const double ing = e[3 * k], eg = e[3 * k + 1], egm1 = e[3 * k + 2];
+    ecl = ecl | ((phif >= ing) & (phif <= eg)) | (phif <= egm1);
+  }
+  return !ecl;
+}
+
+// LDC::imu (types.rs:101-119) in FP32.
+template <int LD>
+__device__ __forceinline__ float imu(float mu, const StarPar& P) {
+  float m = fminf(mu, 1.0f);
+  float im;
+  if (LD == 0) {
+    float o = 1.0f - m;
+    im = 1.0f - o * (P.a0 + o * (P.a1 + o * (P.a2 + o * P.a3)));
+  } else {
+    float s = sqrtf(m);
+    im = (1.0f - P.csum) + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
+  }
+  return mu > 0.0f ? im : 0.0f;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+  float y = x - c;
+  float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// lensing_mag (flux.rs:390-406) in FP32.
+__device__ __forceinline__ float lensing_mag(float px, float py, float pz, const LaneCtx& L,
+                                             float rlens) {
+  float d = -(px * L.ex + py * L.ey + pz * L.ez);
+  if (!(d > 0.0f)) return 1.0f;
+  float qx = px + d * L.ex, qy = py + d * L.ey, qz = pz + d * L.ez;
+  float p = sqrtf(qx * qx + qy * qy + qz * qz);
+  float ph = p * 0.5f;
+  float phsq = ph * ph;
+  float rd = rlens * d;
+  float pd = phsq > 25.0f * rd ? p + rd / p : ph + sqrtf(phsq + rd);
+  return pd * pd / (pd - ph) / ph * 0.25f;
+}
+
+// Star elements (flux.rs:36-80 / 115-200).
+template <int NECL, int LD, bool BEAM, bool LENS>
+__device__ __forceinline__ void star_chunk(const char* __restrict__ p, uint32_t ne,
+                                           const LaneCtx& L, const StarPar& P, float& s, float& c) {
+  constexpr bool POSN = BEAM || LENS;
+  constexpr uint32_t STRIDE = rec_stride(NECL, POSN);
+  constexpr uint32_t EOFF = POSN ? 32u : 16u;
+  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
+    const float4 d = *reinterpret_cast<const float4*>(p);
+    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + EOFF), L.phif);
+    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
+    float term;
+    if (POSN) {
+      const float4 q = *reinterpret_cast<const float4*>(p + 16);
+      float mag = LENS ? lensing_mag(q.x, q.y, q.z, L, P.rlens) : 1.0f;
+      if (BEAM) {
+        float vx = L.kx * q.y;
+        float vy = fmaf(L.ky, q.x, L.k0);
+        float vr = -(L.ex * vx + L.ey * vy);
+        float vn = d.x * vx + d.y * vy;
+        float mud = mu - mu * vr - vn;
+        term = mu * mag * d.w * (1.0f - P.beam * vr) * imu<LD>(mud, P);
+      } else {
+        term = mu * mag * d.w * imu<LD>(mu, P);
+      }
+    } else {
+      term = mu * d.w * imu<LD>(mu, P);
+    }
+    term = (vis & (mu > P.mucrit)) ? term : 0.0f;
+    kahan_add(s, c, term);
+  }
+}
+
+// Disc and disc-edge elements (flux.rs:83-100).
+template <int NECL>
+__device__ __forceinline__ void disc_chunk(const char* __restrict__ p, uint32_t ne, const LaneCtx& L,
+                                           float lin, float quad, float& s, float& c) {
+  constexpr uint32_t STRIDE = rec_stride(NECL, false);
+  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
+    const float4 d = *reinterpret_cast<const float4*>(p);
+    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + 16), L.phif);
+    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
+    float ommu = 1.0f - mu;
+    float term = mu * d.w * (1.0f - ommu * (lin + quad * ommu));
+    term = (vis & (mu > 0.0f)) ? term : 0.0f;
+    kahan_add(s, c, term);
+  }
+}
+
+// Bright-spot elements (flux.rs:102-106).
+template <int NECL>
+__device__ __forceinline__ void spot_chunk(const char* __restrict__ p, uint32_t ne, const LaneCtx& L,
+                                           float& s, float& c) {
+  constexpr uint32_t STRIDE = rec_stride(NECL, false);
+  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
+    const float4 d = *reinterpret_cast<const float4*>(p);
+    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + 16), L.phif);
+    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
+    float term = (vis & (mu > 0.0f)) ? mu * d.w : 0.0f;
+    kahan_add(s, c, term);
+  }
+}
+
+template <int NECL, int LD>
+__device__ __forceinline__ void star_dispatch_fx(uint32_t fx, const char* p, uint32_t ne,
+                                                 const LaneCtx& L, const StarPar& P, float& s,
+                                                 float& c) {
+  switch (fx) {
+    case 0: star_chunk<NECL, LD, false, false>(p, ne, L, P, s, c); break;
+    case 1: star_chunk<NECL, LD, true, false>(p, ne, L, P, s, c); break;
+    case 2: star_chunk<NECL, LD, false, true>(p, ne, L, P, s, c); break;
+    default: star_chunk<NECL, LD, true, true>(p, ne, L, P, s, c); break;
+  }
+}
+
+template <int NECL>
+__device__ __forceinline__ void chunk_dispatch(uint32_t kind, uint32_t ld, uint32_t fx,
+                                               const char* p, uint32_t ne, const LaneCtx& L,
+                                               const StarPar& P, float lin, float quad, float& s,
+                                               float& c) {
+  if (kind <= KIND_STAR2) {
+    if (ld == 0) star_dispatch_fx<NECL, 0>(fx, p, ne, L, P, s, c);
+    else star_dispatch_fx<NECL, 1>(fx, p, ne, L, P, s, c);
+  } else if (kind == KIND_SPOT) {
+    spot_chunk<NECL>(p, ne, L, s, c);
+  } else {
+    disc_chunk<NECL>(p, ne, L, lin, quad, s, c);
+  }
+}
+
+// Workgroup b of a launch lands on XCD b % 8. Give every XCD a contiguous
+// run of items so that the tiles of one chunk share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
+  uint32_t q = n / 8u, r = n % 8u;
+  uint32_t x = b % 8u, i = b / 8u;
+  return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
+                             const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
+                             const char* __restrict__ recs, const double* __restrict__ lane_phi,
+                             const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
+  const ItemDev it = items[id];
+  const SegDev sg = segs[it.seg];
+  const SetDev& sp = sets[sg.set];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t gl = sp.lane_off + it.tile * TILE + lane;
+  const double ph = lane_phi[gl];
+  const uint32_t meta = lane_meta[gl];
+  const uint32_t cls = meta_class(meta);
+  const bool active = (meta & META_VALID) && (sg.use_mask & cp_bit(cls, meta_ptype(meta)));
+
+  LaneCtx L;
+  L.phif = ph - floor(ph);
+  const bool lc_lane = cls == CLS_NORMAL;
+  const double cosi = lc_lane ? sp.cosi_lc : sp.cosi_rad;
+  const double sini = lc_lane ? sp.sini_lc : sp.sini_rad;
+  double sn, cs;
+  sincos(K_TWOPI * ph, &sn, &cs);
+  L.ex = static_cast<float>(sini * cs);
+  L.ey = static_cast<float>(-sini * sn);
+  L.ez = static_cast<float>(cosi);
+
+  const bool star1 = sg.kind == KIND_STAR1;
+  StarPar P;
+  P.a0 = star1 ? sp.ldc1[0] : sp.ldc2[0];
+  P.a1 = star1 ? sp.ldc1[1] : sp.ldc2[1];
+  P.a2 = star1 ? sp.ldc1[2] : sp.ldc2[2];
+  P.a3 = star1 ? sp.ldc1[3] : sp.ldc2[3];
+  P.csum = P.a0 + P.a1 + P.a2 + P.a3;
+  P.mucrit = star1 ? sp.mucrit1 : sp.mucrit2;
+  P.beam = star1 ? sp.beam1 : sp.beam2;
+  P.rlens = sp.rlens1;
+  const uint32_t ld = star1 ? sp.ltype1 : sp.ltype2;
+  uint32_t fx = 0;
+  if (sg.kind <= KIND_STAR2) {
+    const bool beam = (sp.flags & (star1 ? SET_BEAM1 : SET_BEAM2)) != 0;
+    const bool lens = !star1 && (sp.flags & SET_LENS);
+    fx = (beam ? 1u : 0u) | (lens ? 2u : 0u);
+    // Velocity of an element: comp_light lanes include the spin and the
+    // star's centre of mass (flux.rs:41-42, 68-69); comp_star1/2 lanes use
+    // the spin-free form of flux.rs:138-139 / 183-184.
+    const float spin = star1 ? sp.spin1 : sp.spin2;
+    if (lc_lane) {
+      L.kx = -sp.vfac * spin;
+      L.ky = sp.vfac * spin;
+      L.k0 = star1 ? sp.vfac * (-sp.xcofm) : sp.vfac * (1.0f - sp.xcofm - spin);
+    } else {
+      L.kx = -sp.vfac;
+      L.ky = sp.vfac;
+      L.k0 = -sp.vfac * sp.xcofm;
+    }
+  } else {
+    L.kx = L.ky = L.k0 = 0.0f;
+  }
+
+  float s = 0.0f, c = 0.0f;
+  const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
+  const char* p = recs + sg.rec_off + static_cast<uint64_t>(it.e0) * stride;
+  switch (sg.necl) {
+    case 0: chunk_dispatch<0>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+    case 1: chunk_dispatch<1>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+    case 2: chunk_dispatch<2>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+    case 3: chunk_dispatch<3>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+    default: chunk_dispatch<4>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+  }
+  const double sum = static_cast<double>(s) - static_cast<double>(c);
+  partials[it.prow * TILE + lane] = active ? sum : 0.0;
+}
+
+}  // namespace
+
+// One workgroup = one wavefront = one work item.
+__global__ __launch_bounds__(TILE) void flux_items_kernel(
+    const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
+    const SetDev* __restrict__ sets, const char* __restrict__ recs,
+    const double* __restrict__ lane_phi, const uint32_t* __restrict__ lane_meta,
+    double* __restrict__ partials) {
+  const uint32_t id = xcd_map(blockIdx.x, n_items);
+  if (id >= n_items) return;
+  process_item(id, items, segs, sets, recs, lane_phi, lane_meta, partials);
+}
+
+// Persistent form: a fixed grid of wavefronts pulls items from a counter.
+// Every wave exits once the counter passes n_items.
+__global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
+    const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
+    const SetDev* __restrict__ sets, const char* __restrict__ recs,
+    const double* __restrict__ lane_phi, const uint32_t* __restrict__ lane_meta,
+    double* __restrict__ partials, uint32_t* __restrict__ counter, uint32_t grab) {
+  for (;;) {
+    uint32_t base = 0;
+    if (threadIdx.x == 0) base = atomicAdd(counter, grab);
+    base = __shfl(base, 0);
+    if (base >= n_items) break;
+    const uint32_t end = base + grab < n_items ? base + grab : n_items;
+    for (uint32_t id = base; id < end; ++id)
+      process_item(id, items, segs, sets, recs, lane_phi, lane_meta, partials);
+  }
+}
+
+__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
+
+// ------------------------------------------------------------ assembly
+namespace {
+
+// Ginterp::scale1 / scale2 (types.rs:161-184) in FP64.
+__device__ __forceinline__ double gscale1(double phase, double p1, double s11, double s12) {
+  double pnorm = phase - floor(phase);
+  if (pnorm <= p1 || pnorm >= 1.0 - p1) return 1.0;
+  return (s11 * (1.0 - p1 - pnorm) + s12 * (pnorm - p1)) / (1.0 - 2.0 * p1);
+}
+__device__ __forceinline__ double gscale2(double phase, double p2, double s21, double s22) {
+  double pnorm = phase - floor(phase);
+  if (pnorm >= p2 && pnorm <= 1.0 - p2) return 1.0;
+  if (pnorm < 0.5) return (s22 * (p2 - pnorm) + s21 * (pnorm + p2)) / (2.0 * p2);
+  return (s21 * (1.0 + p2 - pnorm) + s22 * (pnorm - 1.0 + p2)) / (2.0 * p2);
+}
+
+// Sum of a lane's partial rows per component: S[0] star 1, S[1] star 2,
+// S[2] disc + edge + spot.
+__device__ __forceinline__ void lane_sums(const SetDev& sp, uint32_t l, const uint32_t* tile_cp,
+                                          const uint32_t* chunk_info, const double* partials,
+                                          double S[3]) {
+  const uint32_t t = l / TILE, j = l % TILE;
+  const uint32_t cp = tile_cp[sp.lane_off / TILE + t];
+  S[0] = S[1] = S[2] = 0.0;
+  const uint64_t row0 = sp.prow_base + static_cast<uint64_t>(t) * sp.n_chunks;
+  for (uint32_t c = 0; c < sp.n_chunks; ++c) {
+    const uint32_t info = chunk_info[sp.chunk_off + c];
+    if (info & cp & 0xFFFFu) {
+      const uint32_t kind = info >> 16;
+      const double v = partials[(row0 + c) * TILE + j];
+      S[kind < 2u ? kind : 2u] += v;
+    }
+  }
+}
+
+constexpr int ABLOCK = 256;
+
+__device__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < ABLOCK / 64; ++i) r += red[i];
+  return r;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(ABLOCK) void assemble_kernel(
+    const SetDev* __restrict__ sets, const double* __restrict__ lane_phi,
+    const uint32_t* __restrict__ tile_cp, const uint32_t* __restrict__ chunk_info,
+    const double* __restrict__ partials, const uint32_t* __restrict__ t_lane0,
+    const int* __restrict__ t_ndiv, const double* __restrict__ t_slfac,
+    const double* __restrict__ t_y, const double* __restrict__ t_ferr,
+    const double* __restrict__ t_w, double* __restrict__ lane_tot, double* __restrict__ flux_out,
+    SetOut* __restrict__ out, int scale) {
+  // FP64 assembly without contraction, in the reference's operation order.
+#pragma clang fp contract(off)
+  __shared__ double special[9];
+  __shared__ double scl[4];
+  __shared__ double red[ABLOCK / 64];
+  const SetDev sp = sets[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  if (!(sp.flags & SET_VALID)) {
+    const double nan = __builtin_nan("");
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) flux_out[sp.time_off + t] = nan;
+    if (tid == 0) out[blockIdx.x] = SetOut{nan, nan, nan, nan};
+    return;
+  }
+  const bool cal1 = (sp.flags & SET_CAL1) != 0, cal2 = (sp.flags & SET_CAL2) != 0;
+  const uint32_t n_special = (cal1 ? 4u : 0u) + (cal2 ? 4u : 0u) + 1u;
+
+  // 1. Calibration and white-dwarf lanes: raw component sums.
+  if (tid < n_special) {
+    double S[3];
+    lane_sums(sp, sp.special0 + tid, tile_cp, chunk_info, partials, S);
+    const bool star2_lane = cal2 && tid >= (cal1 ? 4u : 0u) && tid < n_special - 1u;
+    special[tid] = star2_lane ? S[1] : S[0];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // The reference updates one Ginterp in place, so each ratio sees the
+    // factors found before it: fc12 is weighted by scale1 with scale11 set,
+    // fc22 by scale2 with scale21 set (orchestration.rs:93-131,
+    // comp_star1/2 at flux.rs:151 / 197).
+    const uint32_t b = sp.lane_off + sp.special0;
+    double s11 = 1.0, s12 = 1.0, s21 = 1.0, s22 = 1.0;
+    uint32_t k = 0;
+    if (cal1) {
+      double ff = (1.0 * gscale1(lane_phi[b + 0], sp.phase1, s11, s12)) * special[0] / 1.0;
+      double fc = (1.0 * gscale1(lane_phi[b + 1], sp.phase1, s11, s12)) * special[1] / 1.0;
+      s11 = ff / fc;
+      ff = (1.0 * gscale1(lane_phi[b + 2], sp.phase1, s11, s12)) * special[2] / 1.0;
+      fc = (1.0 * gscale1(lane_phi[b + 3], sp.phase1, s11, s12)) * special[3] / 1.0;
+      s12 = ff / fc;
+      k = 4;
+    }
+    if (cal2) {
+      double ff = (1.0 * gscale2(lane_phi[b + k + 0], sp.phase2, s21, s22)) * special[k + 0] / 1.0;
+      double fc = (1.0 * gscale2(lane_phi[b + k + 1], sp.phase2, s21, s22)) * special[k + 1] / 1.0;
+      s21 = ff / fc;
+      ff = (1.0 * gscale2(lane_phi[b + k + 2], sp.phase2, s21, s22)) * special[k + 2] / 1.0;
+      fc = (1.0 * gscale2(lane_phi[b + k + 3], sp.phase2, s21, s22)) * special[k + 3] / 1.0;
+      s22 = ff / fc;
+    }
+    scl[0] = s11; scl[1] = s12; scl[2] = s21; scl[3] = s22;
+  }
+  __syncthreads();
+  const double s11 = scl[0], s12 = scl[1], s21 = scl[2], s22 = scl[3];
+
+  // 2. Sub-phase totals (flux.rs:74-107 with scale factors).
+  for (uint32_t l = tid; l < sp.special0; l += ABLOCK) {
+    double S[3];
+    lane_sums(sp, l, tile_cp, chunk_info, partials, S);
+    const double phi = lane_phi[sp.lane_off + l];
+    const double ssum = S[0] * gscale1(phi, sp.phase1, s11, s12) +
+                        gscale2(phi, sp.phase2, s21, s22) * S[1] + S[2];
+    lane_tot[sp.lane_off + l] = ssum;
+  }
+  __syncthreads();
+
+  // 3. Exposure smearing and the per-time factors (flux.rs:24-111,
+  //    orchestration.rs:243-251).
+  for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+    const uint32_t l0 = t_lane0[sp.time_off + t];
+    const int nd = t_ndiv[sp.time_off + t];
+    double sum = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      const double wgt = nd == 1 ? 1.0 : ((k == 0 || k == nd - 1) ? 0.5 : 1.0);
+      sum += wgt * lane_tot[sp.lane_off + l0 + k];
+    }
+    const double denom = static_cast<double>(nd - 1 > 1 ? nd - 1 : 1);
+    flux_out[sp.time_off + t] = t_slfac[sp.time_off + t] * (sum / denom) + sp.third;
+  }
+  __syncthreads();
+
+  // 4. Autoscale (re_scale, orchestration.rs:361-388) or plain
+  //    chi-squared (compute_chisq, orchestration.rs:391-401).
+  double wdwarf = (1.0 * gscale1(0.5, sp.phase1, s11, s12)) * special[n_special - 1u] / 1.0;
+  double sfac = 1.0, chisq = 0.0, wnok = 0.0;
+  if (scale) {
+    double sdy = 0.0, syy = 0.0, wn = 0.0;
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      const uint32_t g = sp.time_off + t;
+      if (t_w[g] > 0.0) {
+        const double f = flux_out[g];
+        const double wgt = t_w[g] / (t_ferr[g] * t_ferr[g]);
+        sdy += wgt * t_y[g] * f;
+        syy += wgt * f * f;
+        wn += t_w[g];
+      }
+    }
+    sdy = block_sum(sdy, red);
+    syy = block_sum(syy, red);
+    wn = block_sum(wn, red);
+    if (wn > 0.0 && syy > 0.0) {
+      sfac = sdy / syy;
+      double cs = 0.0;
+      for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+        const uint32_t g = sp.time_off + t;
+        const double f = flux_out[g] * sfac;
+        flux_out[g] = f;
+        if (t_w[g] > 0.0) {
+          const double r = (t_y[g] - f) / t_ferr[g];
+          cs += t_w[g] * (r * r);
+        }
+      }
+      chisq = block_sum(cs, red);
+      wnok = wn;
+      wdwarf *= sfac;
+    }
+  } else {
+    double cs = 0.0, wn = 0.0;
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      const uint32_t g = sp.time_off + t;
+      if (t_w[g] > 0.0) {
+        wn += t_w[g];
+        const double r = (t_y[g] - flux_out[g]) / t_ferr[g];
+        cs += t_w[g] * (r * r);
+      }
+    }
+    chisq = block_sum(cs, red);
+    wnok = block_sum(wn, red);
+  }
+  if (tid == 0) out[blockIdx.x] = SetOut{chisq, wnok, sfac, wdwarf};
+}
+
+// ------------------------------------------------------------ visibility
+__global__ void visibility_kernel(const char* __restrict__ recs, SegDev sg,
+                                  const double* __restrict__ phases, uint32_t nphase,
+                                  uint8_t* __restrict__ mask) {
+  const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t total = static_cast<uint64_t>(sg.n) * nphase;
+  if (idx >= total) return;
+  const uint32_t p = static_cast<uint32_t>(idx / sg.n);
+  const uint32_t e = static_cast<uint32_t>(idx % sg.n);
+  const double ph = phases[p];
+  const double phif = ph - floor(ph);
+  const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
+  const uint32_t eoff = (sg.flags & SEG_POSN) ? 32u : 16u;
+  const double* ecl = reinterpret_cast<const double*>(recs + sg.rec_off + static_cast<uint64_t>(e) * stride + eoff);
+  bool vis;
+  switch (sg.necl) {
+    case 0: vis = eclipse_free<0>(ecl, phif); break;
+    case 1: vis = eclipse_free<1>(ecl, phif); break;
+    case 2: vis = eclipse_free<2>(ecl, phif); break;
+    case 3: vis = eclipse_free<3>(ecl, phif); break;
+    default: vis = eclipse_free<4>(ecl, phif); break;
+  }
+  mask[static_cast<uint64_t>(p) * sg.n + e] = vis ? 1 : 0;
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
+                             uint32_t grab, hipStream_t stream) {
+  if (d.n_items == 0) return hipSuccess;
+  if (!persistent) {
+    hipLaunchKernelGGL(flux_items_kernel, dim3(d.n_items), dim3(TILE), 0, stream, d.items, d.n_items,
+                       d.segs, d.sets, d.recs, d.lane_phi, d.lane_meta, d.partials);
+  } else {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
+    hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
+                       d.items, d.n_items, d.segs, d.sets, d.recs, d.lane_phi, d.lane_meta,
+                       d.partials, d.counter, grab);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const DeviceBatch& d, int scale, hipStream_t stream) {
+  if (d.n_sets == 0) return hipSuccess;
+  hipLaunchKernelGGL(assemble_kernel, dim3(d.n_sets), dim3(ABLOCK), 0, stream, d.sets, d.lane_phi,
+                     d.tile_cp, d.chunk_info, d.partials, d.t_lane0, d.t_ndiv, d.t_slfac, d.t_y,
+                     d.t_ferr, d.t_w, d.lane_tot, d.flux, d.out, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_visibility(const DeviceBatch& d, const SegDev& sg, const double* phases,
+                             uint32_t nphase, uint8_t* mask, hipStream_t stream) {
+  uint64_t total = static_cast<uint64_t>(sg.n) * nphase;
+  if (total == 0) return hipSuccess;
+  uint32_t blocks = static_cast<uint32_t>((total + 255) / 256);
+  hipLaunchKernelGGL(visibility_kernel, dim3(blocks), dim3(256), 0, stream, d.recs, sg, phases,
+                     nphase, mask);
+  return hipGetLastError();
+}
+
+}  // namespace lch

@@ -1,0 +1,43 @@
+// Device buffers of a resident batch and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+
+namespace lch {
+
+struct DeviceBatch {
+  uint32_t n_sets = 0, n_items = 0;
+  const SetDev* sets = nullptr;
+  const SegDev* segs = nullptr;
+  const ItemDev* items = nullptr;
+  const char* recs = nullptr;
+  const double* lane_phi = nullptr;
+  const uint32_t* lane_meta = nullptr;
+  const uint32_t* tile_cp = nullptr;
+  const uint32_t* chunk_info = nullptr;  // use_mask | kind << 16, per chunk
+  const uint32_t* t_lane0 = nullptr;
+  const int* t_ndiv = nullptr;
+  const double* t_slfac = nullptr;
+  const double* t_y = nullptr;
+  const double* t_ferr = nullptr;
+  const double* t_w = nullptr;
+  double* partials = nullptr;
+  double* lane_tot = nullptr;
+  double* flux = nullptr;
+  SetOut* out = nullptr;
+  uint32_t* counter = nullptr;
+};
+
+// Work items: static grid (one workgroup per item) or persistent grid of
+// `persistent_waves` wavefronts taking `grab` items per atomic.
+hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
+                             uint32_t grab, hipStream_t stream);
+// FP64 assembly, scale factors, smearing, autoscale and chi-squared.
+hipError_t launch_assemble(const DeviceBatch& d, int scale, hipStream_t stream);
+// Eclipse visibility of one segment at `nphase` phases (debug / parity).
+hipError_t launch_visibility(const DeviceBatch& d, const SegDev& sg, const double* phases,
+                             uint32_t nphase, uint8_t* mask, hipStream_t stream);
+
+}  // namespace lch

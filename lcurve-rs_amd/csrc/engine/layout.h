@@ -1,0 +1,97 @@
+// Device-side data layout of a batch of light curves, shared by the host
+// packer (engine.cpp) and the kernels (flux_kernels.hip).
+//
+// A batch holds one or more parameter sets. Each set owns:
+//  * lanes   — one per exposure sub-phase (plus up to nine calibration /
+//              white-dwarf lanes), padded to tiles of 64 = one wavefront;
+//  * segments— its surface-element grids, split by eclipse-pair count so a
+//              segment's elements share one record layout;
+//  * chunks  — fixed-size slices of segments;
+//  * partial rows — one row of 64 FP64 partial sums per (tile, chunk).
+// A work item is one wavefront computing one (tile, chunk) partial row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lch {
+
+// Segment kinds. Star 1 and star 2 use their own limb darkening; disc and
+// edge use the disc's quadratic law and test mu > 0; the spot has none.
+enum SegKind : uint32_t { KIND_STAR1 = 0, KIND_STAR2 = 1, KIND_DISC = 2, KIND_EDGE = 3, KIND_SPOT = 4 };
+
+// Lane classes. NORMAL lanes are exposure sub-phases (flux.rs:7-111
+// conventions); CAL1 / CAL2 lanes evaluate comp_star1 / comp_star2 at the
+// Ginterp calibration phases (orchestration.rs:93-131); WD evaluates
+// comp_star1 at phase 0.5 (orchestration.rs:256-259).
+enum LaneClass : uint32_t { CLS_NORMAL = 0, CLS_CAL1 = 1, CLS_CAL2 = 2, CLS_WD = 3 };
+
+// lane_meta bits: [0,2) class, [2,4) grid type 1..3, bit 4 valid.
+constexpr uint32_t META_VALID = 1u << 4;
+__host__ __device__ inline uint32_t meta_class(uint32_t m) { return m & 3u; }
+__host__ __device__ inline uint32_t meta_ptype(uint32_t m) { return (m >> 2) & 3u; }
+// Bit of a (class, grid type) pair inside a segment use mask.
+__host__ __device__ inline uint32_t cp_bit(uint32_t cls, uint32_t ptype) { return 1u << (cls * 4u + ptype); }
+
+constexpr int TILE = 64;     // lanes per tile = wavefront width on CDNA
+constexpr int MAX_ECL = 4;   // eclipse pairs per element supported by the kernels
+
+// Element record, 16-byte aligned, stored back to back for a segment:
+//   float4 (dirn.x, dirn.y, dirn.z, flux)
+//   float4 (posn.x, posn.y, posn.z, 0)        only if SEG_POSN is set
+//   NECL x { double ingress, egress, egress - 1 }
+constexpr uint32_t SEG_POSN = 1u;
+
+__host__ __device__ constexpr uint32_t rec_stride(uint32_t necl, bool posn) {
+  return 16u + (posn ? 16u : 0u) + 24u * necl;
+}
+
+struct SegDev {
+  uint64_t rec_off;   // byte offset of the first record
+  uint32_t n;         // number of elements
+  uint32_t set;       // owning set
+  uint32_t use_mask;  // cp_bit()s of the lanes that add this segment
+  uint32_t kind;      // SegKind
+  uint32_t necl;      // eclipse pairs per element
+  uint32_t flags;     // SEG_POSN
+};
+
+struct ItemDev {
+  uint32_t seg;   // segment index (global)
+  uint32_t e0;    // first element
+  uint32_t ne;    // element count
+  uint32_t tile;  // tile within the set
+  uint64_t prow;  // partial row index
+};
+
+// Set flags.
+constexpr uint32_t SET_VALID = 1u, SET_CAL1 = 2u, SET_CAL2 = 4u, SET_BEAM1 = 8u, SET_BEAM2 = 16u,
+                   SET_LENS = 32u;
+
+struct SetDev {
+  uint32_t lane_off;   // first lane (global index, multiple of TILE)
+  uint32_t n_lanes;    // lanes incl. padding (multiple of TILE)
+  uint32_t special0;   // first calibration / white-dwarf lane, relative to lane_off
+  uint32_t flags;
+  uint32_t time_off;   // first time (global index)
+  uint32_t n_times;
+  uint32_t chunk_off;  // first entry of this set in chunk_info
+  uint32_t n_chunks;
+  uint64_t prow_base;  // first partial row; row = prow_base + tile * n_chunks + chunk
+  uint32_t ltype1, ltype2;  // 0 Poly, 1 Claret
+  float ldc1[4], ldc2[4];
+  float mucrit1, mucrit2, lin_disc, quad_disc;
+  float beam1, beam2, vfac, xcofm, spin1, spin2, rlens1, pad0;
+  double cosi_lc, sini_lc;    // cos/sin(TWOPI*iangle/360): comp_light lanes
+  double cosi_rad, sini_rad;  // iangle.to_radians(): comp_star1/2 lanes
+  double phase1, phase2;      // Ginterp switch phases
+  double third;
+};
+
+// Per-set results written by the assembly kernel.
+struct SetOut {
+  double chisq, wnok, scale, wdwarf;
+};
+
+}  // namespace lch

@@ -1,0 +1,885 @@
+// Roche geometry restated from crates/roche/src and crates/subs/src/numerical.
+// Operation order follows the reference line by line (see the file:line
+// notes) so that grids and eclipse phases come out identical.
+#include "roche.h"
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+
+namespace lc {
+
+namespace {
+constexpr int NMAX = 1000;
+constexpr double LEPS = 1.0e-12;
+
+// Newton iteration on the quintic for an L1-like point.
+// crates/roche/src/lagrange.rs:8-31 (same loop shape in xl11 / xl12)
+double quintic_root(double x, double a1, double a2, double a3, double a4, double a5, double a6,
+                    const char* who) {
+  double d1 = a2, d2 = 2.0 * a3, d3 = 3.0 * a4, d4 = 4.0 * a5, d5 = 5.0 * a6;
+  for (int i = 0; i < NMAX; ++i) {
+    double xold = x;
+    double f = x * (x * (x * (x * (x * a6 + a5) + a4) + a3) + a2) + a1;
+    double df = x * (x * (x * (x * d5 + d4) + d3) + d2) + d1;
+    x -= f / df;
+    if (std::fabs(x - xold) <= LEPS * std::fabs(x)) return x;
+  }
+  throw LcError(std::string(who) + ": exceeded maximum iterations");
+}
+
+void check_q(double q, const char* who) {
+  if (q <= 0.0) throw LcError(std::string(who) + ": q <= 0");
+}
+}  // namespace
+
+// crates/roche/src/lagrange.rs:8-31
+double xl1(double q) {
+  check_q(q, "xl1");
+  double mu = q / (1.0 + q);
+  return quintic_root(1.0 / (1.0 + q), -1.0 + mu, 2.0 - 2.0 * mu, -1.0 + mu, 1.0 + 2.0 * mu,
+                      -2.0 - mu, 1.0, "xl1");
+}
+
+// crates/roche/src/lagrange.rs:89-114
+double xl11(double q, double spin) {
+  check_q(q, "xl11");
+  double ssq = spin * spin;
+  double mu = q / (1.0 + q);
+  return quintic_root(1.0 / (1.0 + q), -1.0 + mu, 2.0 - 2.0 * mu, -1.0 + mu, ssq + 2.0 * mu,
+                      -2.0 * ssq - mu, ssq, "xl11");
+}
+
+// crates/roche/src/lagrange.rs:117-142
+double xl12(double q, double spin) {
+  check_q(q, "xl12");
+  double ssq = spin * spin;
+  double mu = q / (1.0 + q);
+  return quintic_root(1.0 / (1.0 + q), -1.0 + mu, 2.0 - 2.0 * mu, -ssq + mu,
+                      3.0 * ssq + 2.0 * mu - 2.0, 1.0 - mu - 3.0 * ssq, ssq, "xl12");
+}
+
+// crates/roche/src/potential.rs:8-19
+double rpot(double q, const Vec3& p) {
+  check_q(q, "rpot");
+  double mu = q / (1.0 + q);
+  double comp = 1.0 - mu;
+  double x2y2 = sqr(p.x) + sqr(p.y);
+  double r1sq = x2y2 + sqr(p.z);
+  double r1 = std::sqrt(r1sq);
+  double r2 = std::sqrt(r1sq + 1.0 - 2.0 * p.x);
+  return -comp / r1 - mu / r2 - (x2y2 + mu * (mu - 2.0 * p.x)) / 2.0;
+}
+
+// crates/roche/src/potential.rs:22-33
+double rpot1(double q, double spin, const Vec3& p) {
+  check_q(q, "rpot1");
+  double mu = q / (1.0 + q);
+  double comp = 1.0 - mu;
+  double x2y2 = sqr(p.x) + sqr(p.y);
+  double r1sq = x2y2 + sqr(p.z);
+  double r1 = std::sqrt(r1sq);
+  double r2 = std::sqrt(r1sq + 1.0 - 2.0 * p.x);
+  return -comp / r1 - mu / r2 - sqr(spin) * x2y2 / 2.0 + mu * p.x;
+}
+
+// crates/roche/src/potential.rs:36-47
+double rpot2(double q, double spin, const Vec3& p) {
+  check_q(q, "rpot2");
+  double mu = q / (1.0 + q);
+  double comp = 1.0 - mu;
+  double x2y2 = sqr(p.x) + sqr(p.y);
+  double r1sq = x2y2 + sqr(p.z);
+  double r1 = std::sqrt(r1sq);
+  double r2 = std::sqrt(r1sq + 1.0 - 2.0 * p.x);
+  return -comp / r1 - mu / r2 - sqr(spin) * (0.5 + 0.5 * x2y2 - p.x) - comp * p.x;
+}
+
+// crates/roche/src/potential.rs:68-86
+Vec3 drpot1(double q, double spin, const Vec3& p) {
+  check_q(q, "drpot1");
+  double r1sq = p.sqr();
+  double r2sq = r1sq + 1.0 - 2.0 * p.x;
+  double r1 = std::sqrt(r1sq);
+  double r2 = std::sqrt(r2sq);
+  double mu = q / (1.0 + q);
+  double mu1 = mu / r2 / r2sq;
+  double comp = (1.0 - mu) / r1 / r1sq;
+  double ssq = sqr(spin);
+  return {comp * p.x + mu1 * (p.x - 1.0) - ssq * p.x + mu, comp * p.y + mu1 * p.y - ssq * p.y,
+          comp * p.z + mu1 * p.z};
+}
+
+// crates/roche/src/potential.rs:88-106
+Vec3 drpot2(double q, double spin, const Vec3& p) {
+  check_q(q, "drpot2");
+  double r1sq = p.sqr();
+  double r2sq = r1sq + 1.0 - 2.0 * p.x;
+  double r1 = std::sqrt(r1sq);
+  double r2 = std::sqrt(r2sq);
+  double mu = q / (1.0 + q);
+  double mu1 = mu / r2 / r2sq;
+  double comp = (1.0 - mu) / r1 / r1sq;
+  double ssq = sqr(spin);
+  return {comp * p.x + mu1 * (p.x - 1.0) - ssq * (p.x - 1.0) + mu - 1.0,
+          comp * p.y + mu1 * p.y - ssq * p.y, comp * p.z + mu1 * p.z};
+}
+
+namespace {
+inline Vec3 star_cofm(Star s) { return s == Star::Primary ? Vec3(0, 0, 0) : Vec3(1.0, 0, 0); }
+
+inline double pot_star(double q, Star s, double spin, const Vec3& p) {
+  return s == Star::Primary ? rpot1(q, spin, p) : rpot2(q, spin, p);
+}
+
+// crates/roche/src/potential.rs:109-126
+inline double rpot_along_los(double q, Star s, double spin, const Vec3& earth, const Vec3& p,
+                             double lam) {
+  Vec3 point = p + lam * earth;
+  return pot_star(q, s, spin, point);
+}
+inline double drpot_along_los(double q, Star s, double spin, const Vec3& earth, const Vec3& p,
+                              double lam) {
+  Vec3 point = p + lam * earth;
+  Vec3 g = s == Star::Primary ? drpot1(q, spin, point) : drpot2(q, spin, point);
+  return dot(g, earth);
+}
+
+inline double sign_of(double a, double b) { return b >= 0.0 ? std::fabs(a) : -std::fabs(a); }
+
+// Brent minimisation with derivatives, early exit below fref.
+// crates/subs/src/numerical/dbrent.rs:20-165. Returns false on too many
+// iterations (the reference's Err).
+template <class F, class D>
+bool dbrent(double ax, double bx, double cx, F&& func, D&& dfunc, double acc, bool stopfast,
+            double fref, double* fmin_out, double* xmin_out) {
+  const int ITMAX = 100;
+  double a = fminr(ax, cx);
+  double b = fmaxr(ax, cx);
+  double x = bx, w = bx, v = bx;
+  double fx = func(x);
+  double fw = fx, fv = fx;
+  if (stopfast && fx < fref) { *fmin_out = fx; *xmin_out = x; return true; }
+  double dx = dfunc(x);
+  double dw = dx, dv = dx;
+  double e = 0.0, d = 0.0;
+  for (int it = 0; it < ITMAX; ++it) {
+    double xm = 0.5 * (a + b);
+    double tol1 = acc;
+    double tol2 = 2.0 * tol1;
+    if (std::fabs(x - xm) <= tol2 - 0.5 * (b - a)) { *fmin_out = fx; *xmin_out = x; return true; }
+    if (std::fabs(e) > tol1) {
+      double d1 = 2.0 * (b - a);
+      double d2 = d1;
+      if (dw != dx) d1 = (w - x) * dx / (dx - dw);
+      if (dv != dx) d2 = (v - x) * dx / (dx - dv);
+      double u1 = x + d1;
+      double u2 = x + d2;
+      bool ok1 = (a - u1) * (u1 - b) > 0.0 && dx * d1 <= 0.0;
+      bool ok2 = (a - u2) * (u2 - b) > 0.0 && dx * d2 <= 0.0;
+      double olde = e;
+      e = d;
+      if (ok1 || ok2) {
+        if (ok1 && ok2) d = std::fabs(d1) < std::fabs(d2) ? d1 : d2;
+        else if (ok1) d = d1;
+        else d = d2;
+        if (std::fabs(d) <= std::fabs(0.5 * olde)) {
+          double u = x + d;
+          if (u - a < tol2 || b - u < tol2) d = sign_of(tol1, xm - x);
+        } else {
+          e = dx >= 0.0 ? a - x : b - x;
+          d = 0.5 * e;
+        }
+      } else {
+        e = dx >= 0.0 ? a - x : b - x;
+        d = 0.5 * e;
+      }
+    } else {
+      e = dx >= 0.0 ? a - x : b - x;
+      d = 0.5 * e;
+    }
+    double u, fu;
+    if (std::fabs(d) >= tol1) {
+      u = x + d;
+      fu = func(u);
+      if (stopfast && fu < fref) { *fmin_out = fu; *xmin_out = u; return true; }
+    } else {
+      u = x + sign_of(tol1, d);
+      fu = func(u);
+      if (stopfast && fu < fref) { *fmin_out = fu; *xmin_out = u; return true; }
+      if (fu > fx) { *fmin_out = fx; *xmin_out = x; return true; }
+    }
+    double du = dfunc(u);
+    if (fu <= fx) {
+      if (u >= x) a = x; else b = x;
+      v = w; fv = fw; dv = dw;
+      w = x; fw = fx; dw = dx;
+      x = u; fx = fu; dx = du;
+    } else {
+      if (u < x) a = u; else b = u;
+      if (fu <= fw || w == x) {
+        v = w; fv = fw; dv = dw;
+        w = u; fw = fu; dw = du;
+      } else if (fu < fv || v == x || v == w) {
+        v = u; fv = fu; dv = du;
+      }
+    }
+  }
+  (void)fv;
+  return false;
+}
+
+// crates/roche/src/eclipse.rs:202-224
+bool sphere_eclipse_los(const Vec3& earth, const Vec3& r, const Vec3& c, double rsphere,
+                        double* lam1, double* lam2) {
+  Vec3 d = r - c;
+  double bquad = dot(earth, d);
+  if (bquad >= 0.0) return false;
+  double cquad = d.sqr() - sqr(rsphere);
+  double fac = sqr(bquad) - cquad;
+  if (fac <= 0.0) return false;
+  fac = std::sqrt(fac);
+  *lam2 = -bquad + fac;
+  *lam1 = fmaxr(cquad / *lam2, 0.0);
+  return true;
+}
+}  // namespace
+
+// crates/roche/src/surface.rs:14-39
+void ref_sphere(double q, Star star, double spin, double ffac, double* rref, double* pref) {
+  double rl1 = star == Star::Primary ? xl11(q, spin) : 1.0 - xl12(q, spin);
+  double tref = rl1;
+  double surf_r = ffac * tref;
+  Vec3 p = star == Star::Primary ? Vec3(surf_r, 0.0, 0.0) : Vec3(1.0 - surf_r, 0.0, 0.0);
+  *pref = star == Star::Primary ? rpot1(q, spin, p) : rpot2(q, spin, p);
+  *rref = fminr(1.001 * ffac, 1.0) * tref;
+}
+
+// crates/roche/src/surface.rs:54-121
+void face(double q, Star star, double spin, const Vec3& dirn, double rref, double pref, double acc,
+          Vec3* pvec_out, Vec3* dvec_out, double* radius_out, double* gravity_out) {
+  Vec3 cofm = star_cofm(star);
+  auto pot_fn = [&](double radius) {
+    Vec3 p = cofm + radius * dirn;
+    return pot_star(q, star, spin, p);
+  };
+  double lo = rref / 2.0;
+  double hi = rref;
+  double tref = pref + 1.0;
+  for (int i = 0; i < 30; ++i) {
+    if (tref <= pref) break;
+    lo = hi / 2.0;
+    tref = pot_fn(lo);
+    if (tref > pref) hi = lo;
+  }
+  for (int i = 0; i < 100; ++i) {
+    if (hi - lo <= acc) break;
+    double mid = (lo + hi) / 2.0;
+    double pot_mid = pot_fn(mid);
+    if (pot_mid < pref) lo = mid;
+    else hi = mid;
+  }
+  double radius = (lo + hi) / 2.0;
+  Vec3 pvec = cofm + radius * dirn;
+  Vec3 grad = star == Star::Primary ? drpot1(q, spin, pvec) : drpot2(q, spin, pvec);
+  double gravity = grad.length();
+  Vec3 dvec = grad;
+  dvec.unit();
+  *pvec_out = pvec;
+  *dvec_out = dvec;
+  *radius_out = radius;
+  *gravity_out = gravity;
+}
+
+// crates/roche/src/eclipse.rs:229-265
+bool sphere_eclipse_phase(double cosi, double sini, const Vec3& r, const Vec3& c, double rsphere,
+                          double* phi1, double* phi2, double* lam1, double* lam2) {
+  Vec3 d = r - c;
+  double pdist = std::sqrt(sqr(d.x) + sqr(d.y));
+  double bquad = d.z * cosi - pdist * sini;
+  if (bquad >= 0.0) return false;
+  double cquad = d.sqr() - sqr(rsphere);
+  double fac = sqr(bquad) - cquad;
+  if (fac <= 0.0) return false;
+  fac = std::sqrt(fac);
+  *lam2 = -bquad + fac;
+  *lam1 = fmaxr(cquad / *lam2, 0.0);
+  if (cquad < 0.0) {
+    *phi1 = 0.0;
+    *phi2 = 1.0;
+  } else {
+    double delta = std::acos((cosi * d.z + std::sqrt(cquad)) / (sini * pdist));
+    double phi = std::atan2(d.y, -d.x);
+    double p1 = (phi - delta) / TWOPI;
+    p1 -= std::floor(p1);
+    *phi1 = p1;
+    *phi2 = p1 + 2.0 * delta / TWOPI;
+  }
+  return true;
+}
+
+// crates/roche/src/eclipse.rs:123-197
+bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec3& earth,
+            const Vec3& p) {
+  double rref, pref;
+  ref_sphere(q, star, spin, ffac, &rref, &pref);
+  Vec3 cofm = star_cofm(star);
+  double lam1, lam2;
+  if (!sphere_eclipse_los(earth, p, cofm, rref, &lam1, &lam2)) return false;
+  if (lam1 == 0.0) return true;
+
+  auto func = [&](double lam) { return rpot_along_los(q, star, spin, earth, p, lam); };
+  size_t nstep = 1;
+  double step_size = lam2 - lam1;
+  double f1 = 0.0, f2 = 0.0, flam = 1.0;
+  double lam = lam1;
+  while (step_size > acc) {
+    lam = lam1 + step_size / 2.0;
+    bool bracketted = false;
+    for (size_t n = 0; n < nstep; ++n) {
+      flam = func(lam + static_cast<double>(n) * step_size);
+      if (flam <= pref) return true;
+      if (nstep == 1) {
+        f1 = func(lam1);
+        f2 = func(lam2);
+      }
+      if (flam < f1 && flam < f2) {
+        lam = lam + static_cast<double>(n) * step_size;
+        bracketted = true;
+        break;
+      }
+    }
+    if (bracketted) break;
+    step_size /= 2.0;
+    nstep *= 2;
+  }
+  if (flam < f1 && flam < f2) {
+    auto dfunc = [&](double x) { return drpot_along_los(q, star, spin, earth, p, x); };
+    double fmin, xmin;
+    if (dbrent(lam1, lam, lam2, func, dfunc, acc, true, pref, &fmin, &xmin)) return fmin < pref;
+    return false;
+  }
+  return false;
+}
+
+namespace {
+// crates/roche/src/eclipse.rs:347-373
+bool pot_min_search(double q, double cosi, double sini, Star star, double spin, double ffac,
+                    const Vec3& r, double phi1, double phi2, double acc, double* phi_out) {
+  size_t nsteps = as_usize(std::ceil((phi2 - phi1) / (acc / TWOPI)));
+  nsteps = std::min<size_t>(std::max<size_t>(nsteps, 10), 1000);
+  double dphi = (phi2 - phi1) / static_cast<double>(nsteps);
+  for (size_t i = 0; i <= nsteps; ++i) {
+    double phi = phi1 + static_cast<double>(i) * dphi;
+    Vec3 earth = set_earth(cosi, sini, phi);
+    if (fblink(q, star, spin, ffac, acc, earth, r)) {
+      *phi_out = phi;
+      return true;
+    }
+  }
+  return false;
+}
+}  // namespace
+
+// crates/roche/src/eclipse.rs:270-338
+bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,
+                    const Vec3& r, double* ingress_out, double* egress_out) {
+  double rref, pref;
+  ref_sphere(q, star, spin, ffac, &rref, &pref);
+  double irad = to_radians(iangle);
+  double cosi = std::cos(irad);
+  double sini = std::sin(irad);
+  Vec3 cofm = star_cofm(star);
+  double phi1, phi2, lam1, lam2;
+  if (!sphere_eclipse_phase(cosi, sini, r, cofm, rref, &phi1, &phi2, &lam1, &lam2)) return false;
+  double acc = 2.0 * std::sqrt(2.0 * TWOPI * (lam2 - lam1) * delta);
+  double phi;
+  if (!pot_min_search(q, cosi, sini, star, spin, ffac, r, phi1, phi2, acc, &phi)) return false;
+
+  double pin = phi, pout = phi1;
+  while (std::fabs(pin - pout) > delta) {
+    double pmid = (pin + pout) / 2.0;
+    Vec3 earth = set_earth(cosi, sini, pmid);
+    if (fblink(q, star, spin, ffac, acc, earth, r)) pin = pmid;
+    else pout = pmid;
+  }
+  double ingress = (pin + pout) / 2.0;
+  ingress -= std::floor(ingress);
+
+  pin = phi;
+  pout = phi2;
+  while (std::fabs(pin - pout) > delta) {
+    double pmid = (pin + pout) / 2.0;
+    Vec3 earth = set_earth(cosi, sini, pmid);
+    if (fblink(q, star, spin, ffac, acc, earth, r)) pin = pmid;
+    else pout = pmid;
+  }
+  double egress = (pin + pout) / 2.0;
+  egress -= std::floor(egress);
+  if (egress < ingress) egress += 1.0;
+  *ingress_out = ingress;
+  *egress_out = egress;
+  return true;
+}
+
+// ------------------------------------------------------------ disc eclipse
+namespace {
+enum class Circle { Above, Inside, Outside, Separate, Crossing };
+
+// crates/roche/src/disc_eclipse.rs:29-32
+double cut_phase(double rxy, double rcone, double radius) {
+  double cos_val = (sqr(rxy) + sqr(rcone) - sqr(radius)) / (2.0 * rcone * rxy);
+  return std::acos(clampd(cos_val, -1.0, 1.0)) / TWOPI;
+}
+
+// crates/roche/src/disc_eclipse.rs:44-70
+Circle circle_eclipse(double rxy, double z, double zcirc, double radius, double tani, double* phase) {
+  *phase = 0.0;
+  if (z >= zcirc) return Circle::Above;
+  double rcone = tani * (zcirc - z);
+  if (rcone >= rxy + radius) return Circle::Outside;
+  if (rxy >= rcone + radius) return Circle::Separate;
+  if (rxy + rcone <= radius) return Circle::Inside;
+  *phase = cut_phase(rxy, rcone, radius);
+  return Circle::Crossing;
+}
+
+// Applies one more rim circle to the running appear phase.
+// Shape shared by disc_eclipse.rs:226-253 / 266-305.
+void apply_rim(double rxy, double z, double zc, double rad, double tani, double* appear) {
+  if (*appear > 0.0) {
+    double ph;
+    Circle res = circle_eclipse(rxy, z, zc, rad, tani, &ph);
+    if (res == Circle::Crossing) *appear = fminr(*appear, ph);
+    else if (res != Circle::Inside) *appear = -1.0;
+  }
+}
+}  // namespace
+
+// crates/roche/src/disc_eclipse.rs:84-330
+EclipseVec disc_eclipse(double iangle, double rdisc1, double rdisc2, double beta, double height,
+                        const Vec3& r) {
+  EclipseVec temp;
+  double hout = height * std::pow(rdisc2, beta);
+  if (r.z >= hout) return temp;
+
+  double irad = iangle * PI / 180.0;
+  double sini = std::sin(irad);
+  double cosi = std::cos(irad);
+
+  if (cosi == 0.0) {
+    if (std::fabs(r.z) < hout) {
+      double rxy = std::sqrt(sqr(r.x) + sqr(r.y));
+      if (rxy <= rdisc2) {
+        temp.push_back({0.0, 1.0});
+      } else {
+        double subtend = std::asin(rdisc2 / rxy) / TWOPI;
+        double pcen = std::atan2(r.y, -r.x) / TWOPI;
+        double ingress = pcen - subtend;
+        ingress = ingress - std::floor(ingress);
+        double egress = ingress + 2.0 * subtend;
+        temp.push_back({ingress, egress});
+      }
+    }
+    return temp;
+  }
+
+  double rxy = std::sqrt(sqr(r.x) + sqr(r.y));
+  if (rdisc1 < rxy && rxy < rdisc2 && std::fabs(r.z) < height * std::pow(rxy, beta)) {
+    temp.push_back({0.0, 1.1});
+    return temp;
+  }
+
+  double tani = sini / cosi;
+
+  if (rxy < rdisc2 && r.z >= height * std::pow(fmaxr(rdisc1, rxy), beta)) {
+    double phase;
+    Circle result = circle_eclipse(rxy, r.z, hout, rdisc2, tani, &phase);
+    if (result == Circle::Outside) {
+      temp.push_back({0.0, 1.1});
+    } else if (result == Circle::Crossing) {
+      double phi0 = std::atan2(r.y, -r.x) / TWOPI;
+      double ingress = phi0 + phase;
+      ingress = ingress - std::floor(ingress);
+      double egress = ingress + 1.0 - 2.0 * phase;
+      temp.push_back({ingress, egress});
+    }
+    return temp;
+  }
+
+  double rcone_lo = fmaxr(tani * (-hout - r.z), 0.0);
+  if (rcone_lo >= rxy + rdisc2) return temp;
+  double rcone_hi = tani * (hout - r.z);
+  if (rxy >= rcone_hi + rdisc2) return temp;
+
+  double eclipse_phase;
+  if (rxy + rcone_lo <= rdisc2) {
+    eclipse_phase = 0.5;
+  } else if (rxy <= rdisc2) {
+    eclipse_phase = cut_phase(rxy, rcone_lo, rdisc2);
+  } else {
+    if (sqr(rcone_hi) + sqr(rdisc2) >= sqr(rxy) && sqr(rcone_lo) + sqr(rdisc2) <= sqr(rxy)) {
+      eclipse_phase = std::asin(rdisc2 / rxy) / TWOPI;
+    } else if (sqr(rcone_hi) + sqr(rdisc2) < sqr(rxy)) {
+      eclipse_phase = cut_phase(rxy, rcone_hi, rdisc2);
+    } else {
+      eclipse_phase = cut_phase(rxy, rcone_lo, rdisc2);
+    }
+  }
+
+  double hin = height * std::pow(rdisc1, beta);
+  double appear_phase = -1.0;
+
+  if (r.z < -hout) {
+    double ph;
+    Circle res = circle_eclipse(rxy, r.z, -hout, rdisc2, tani, &ph);
+    if (res == Circle::Inside) appear_phase = 0.5;
+    else if (res == Circle::Crossing) appear_phase = ph;
+    apply_rim(rxy, r.z, -hin, rdisc1, tani, &appear_phase);
+    apply_rim(rxy, r.z, hin, rdisc1, tani, &appear_phase);
+    apply_rim(rxy, r.z, hout, rdisc2, tani, &appear_phase);
+  } else if (rxy < rdisc1) {
+    if (r.z < -hin) {
+      double ph;
+      Circle res = circle_eclipse(rxy, r.z, -hin, rdisc1, tani, &ph);
+      if (res == Circle::Inside) appear_phase = 0.5;
+      else if (res == Circle::Crossing) appear_phase = ph;
+      apply_rim(rxy, r.z, hin, rdisc1, tani, &appear_phase);
+      apply_rim(rxy, r.z, hout, rdisc2, tani, &appear_phase);
+    } else if (r.z < hin) {
+      double ph;
+      Circle res = circle_eclipse(rxy, r.z, hin, rdisc1, tani, &ph);
+      if (res == Circle::Inside) appear_phase = 0.5;
+      else if (res == Circle::Crossing) appear_phase = ph;
+      apply_rim(rxy, r.z, hout, rdisc2, tani, &appear_phase);
+    }
+  }
+
+  double phi0 = std::atan2(r.y, -r.x) / TWOPI;
+  if (appear_phase <= 0.0) {
+    double ingress = phi0 - eclipse_phase;
+    ingress = ingress - std::floor(ingress);
+    double egress = ingress + 2.0 * eclipse_phase;
+    temp.push_back({ingress, egress});
+  } else if (appear_phase < eclipse_phase) {
+    double ingress = phi0 - eclipse_phase;
+    ingress = ingress - std::floor(ingress);
+    double egress = ingress + (eclipse_phase - appear_phase);
+    temp.push_back({ingress, egress});
+    ingress = phi0 + appear_phase;
+    ingress = ingress - std::floor(ingress);
+    egress = ingress + (eclipse_phase - appear_phase);
+    temp.push_back({ingress, egress});
+  }
+  return temp;
+}
+
+// ---------------------------------------------------------------- misc
+// crates/roche/src/misc.rs:4-19
+Vec3 rocacc(double q, const Vec3& r, const Vec3& v) {
+  double f1 = 1.0 / (1.0 + q);
+  double f2 = f1 * q;
+  double yzsq = sqr(r.y) + sqr(r.z);
+  double r1sq = sqr(r.x) + yzsq;
+  double r2sq = sqr(r.x - 1.0) + yzsq;
+  double fm1 = f1 / (r1sq * std::sqrt(r1sq));
+  double fm2 = f2 / (r2sq * std::sqrt(r2sq));
+  double fm3 = fm1 + fm2;
+  return {-fm3 * r.x + fm2 + 2.0 * v.y + r.x - f2, -fm3 * r.y - 2.0 * v.x + r.y, -fm3 * r.z};
+}
+
+// crates/roche/src/misc.rs:22-29
+double jacobi(double q, const Vec3& r, const Vec3& v) {
+  double f1 = 1.0 / (1.0 + q);
+  double f2 = f1 * q;
+  double yzsq = sqr(r.y) + sqr(r.z);
+  return (sqr(v.x) + sqr(v.y) + sqr(v.z) - sqr(r.y) - sqr(r.x - f2)) / 2.0 -
+         f1 / std::sqrt(sqr(r.x) + yzsq) - f2 / std::sqrt(sqr(r.x - 1.0) + yzsq);
+}
+
+// ------------------------------------------------- Bulirsch-Stoer stepper
+namespace {
+constexpr int KMAXX = 8;
+constexpr int IMAXX = KMAXX + 1;
+constexpr int NSEQ[IMAXX] = {2, 4, 6, 8, 10, 12, 14, 16, 18};
+constexpr double SAFE1 = 0.25, SAFE2 = 0.7, REDMAX = 1.0e-5, REDMIN = 0.7, TINY = 1.0e-30,
+                 SCALMX = 0.1;
+
+// crates/subs/src/numerical/bsstep.rs:15-41
+struct BsState {
+  bool first = true;
+  int kmax = 0;
+  int kopt = 0;
+  double epsold = -1.0;
+  double xnew = -1.0e29;
+  double a[IMAXX] = {0};
+  double alf[KMAXX][KMAXX] = {{0}};
+};
+
+constexpr int NV = 6;
+using Deriv = void (*)(double q, const double* y, double* dydt);
+
+// crates/roche/src/stream.rs:65-77
+void rocder(double q, const double* y, double* dydt) {
+  Vec3 r(y[0], y[1], y[2]);
+  Vec3 v(y[3], y[4], y[5]);
+  dydt[0] = v.x; dydt[1] = v.y; dydt[2] = v.z;
+  Vec3 a = rocacc(q, r, v);
+  dydt[3] = a.x; dydt[4] = a.y; dydt[5] = a.z;
+}
+
+// crates/subs/src/numerical/bsstep.rs:54-75
+void mmid(const double* y, const double* dydx, double xs, double htot, int nstep, double* yout,
+          double q) {
+  double h = htot / static_cast<double>(nstep);
+  double ym[NV], yn[NV];
+  for (int i = 0; i < NV; ++i) {
+    ym[i] = y[i];
+    yn[i] = y[i] + h * dydx[i];
+  }
+  double x = xs + h;
+  (void)x;
+  rocder(q, yn, yout);
+  double h2 = 2.0 * h;
+  for (int n = 1; n < nstep; ++n) {
+    for (int i = 0; i < NV; ++i) {
+      double swap = ym[i] + h2 * yout[i];
+      ym[i] = yn[i];
+      yn[i] = swap;
+    }
+    x += h;
+    rocder(q, yn, yout);
+  }
+  for (int i = 0; i < NV; ++i) yout[i] = 0.5 * (ym[i] + yn[i] + h * yout[i]);
+}
+
+// crates/subs/src/numerical/bsstep.rs:78-131
+void pzextr(int iest, double xest, const double* yest, double* yz, double* dy,
+            double d_tab[NV][KMAXX], double* x_tab) {
+  x_tab[iest] = xest;
+  for (int j = 0; j < NV; ++j) {
+    dy[j] = yest[j];
+    yz[j] = yest[j];
+  }
+  if (iest == 0) {
+    for (int j = 0; j < NV; ++j) d_tab[j][0] = yest[j];
+  } else {
+    double c[NV];
+    for (int j = 0; j < NV; ++j) c[j] = yest[j];
+    for (int k1 = 0; k1 < iest; ++k1) {
+      double delta = 1.0 / (x_tab[iest - k1 - 1] - xest);
+      double f1 = xest * delta;
+      double f2 = x_tab[iest - k1 - 1] * delta;
+      for (int j = 0; j < NV; ++j) {
+        double qv = d_tab[j][k1];
+        d_tab[j][k1] = dy[j];
+        double delta_val = c[j] - qv;
+        dy[j] = f1 * delta_val;
+        c[j] = f2 * delta_val;
+        yz[j] += dy[j];
+      }
+    }
+    for (int j = 0; j < NV; ++j) d_tab[j][iest] = dy[j];
+  }
+}
+
+inline void chk(bool ok) {
+  if (!ok) throw LcError("bsstep: extrapolation table index out of range");
+}
+
+// crates/subs/src/numerical/bsstep.rs:141-306 (including its quirks).
+void bsstep(double* y, const double* dydx, double* xx, double htry, double eps,
+            const double* yscal, BsState& st, double q, double* hdid_out, double* hnext_out,
+            bool* underflow) {
+  double yerr[NV], ysav[NV], yseq[NV];
+  double err[KMAXX] = {0};
+  double d_tab[NV][KMAXX] = {{0}};
+  double x_tab[KMAXX] = {0};
+  *underflow = false;
+
+  if (eps != st.epsold) {
+    st.xnew = -1.0e29;
+    double eps1 = SAFE1 * eps;
+    st.a[0] = static_cast<double>(NSEQ[0]) + 1.0;
+    for (int k = 0; k < KMAXX; ++k) st.a[k + 1] = st.a[k] + static_cast<double>(NSEQ[k + 1]);
+    for (int iq = 1; iq < KMAXX; ++iq) {
+      for (int k = 0; k < iq; ++k) {
+        st.alf[k][iq] = std::pow(eps1, (st.a[k + 1] - st.a[iq + 1]) /
+                                            ((st.a[iq + 1] - st.a[0] + 1.0) *
+                                             static_cast<double>(2 * k + 3)));
+      }
+    }
+    st.epsold = eps;
+    st.kopt = 1;
+    for (int kopt = 1; kopt < KMAXX - 1; ++kopt) {
+      if (st.a[kopt + 1] > st.a[kopt] * st.alf[kopt - 1][kopt]) {
+        st.kopt = kopt;
+        break;
+      }
+      st.kopt = kopt;
+    }
+    st.kmax = st.kopt;
+  }
+
+  double h = htry;
+  for (int i = 0; i < NV; ++i) ysav[i] = y[i];
+  if (*xx != st.xnew || h != st.xnew) {
+    st.first = true;
+    st.kopt = st.kmax;
+  }
+
+  bool reduct = false;
+  int km = 0;
+  for (;;) {
+    bool exitflag = false;
+    for (int k = 0; k <= st.kmax; ++k) {
+      st.xnew = *xx + h;
+      if (st.xnew == *xx) {
+        *hdid_out = 0.0;
+        *hnext_out = 0.0;
+        *underflow = true;
+        return;
+      }
+      mmid(ysav, dydx, *xx, h, NSEQ[k], yseq, q);
+      double xest = sqr(h / static_cast<double>(NSEQ[k]));
+      pzextr(k, xest, yseq, y, yerr, d_tab, x_tab);
+      if (k != 0) {
+        double errmax = TINY;
+        for (int i = 0; i < NV; ++i) errmax = fmaxr(errmax, std::fabs(yerr[i] / yscal[i]));
+        errmax /= eps;
+        km = k - 1;
+        err[km] = std::pow(errmax / SAFE1, 1.0 / static_cast<double>(2 * km + 3));
+      }
+      int koptm1 = st.kopt > 0 ? st.kopt - 1 : 0;
+      if (k != 0 && (k >= koptm1 || st.first)) {
+        if (err[km] != 0.0) {
+          double errmax = TINY;
+          for (int i = 0; i < NV; ++i) errmax = fmaxr(errmax, std::fabs(yerr[i] / yscal[i]));
+          errmax /= eps;
+          if (errmax < 1.0) {
+            exitflag = true;
+            break;
+          }
+        }
+        if (k == st.kmax || k == st.kopt + 1) break;
+        // Same short-circuit order as the reference; indices are checked
+        // only where the reference would index.
+        if (k == st.kopt) {
+          chk(st.kopt >= 1 && st.kopt < KMAXX);
+          if (st.alf[st.kopt - 1][st.kopt] < err[km]) break;
+        }
+        if (st.kopt == st.kmax) {
+          chk(st.kmax >= 1 && st.kmax - 1 < KMAXX);
+          if (st.alf[km][st.kmax - 1] < err[km]) break;
+        }
+        chk(st.kopt < KMAXX);
+        if (st.alf[km][st.kopt] < err[km]) break;
+      }
+    }
+    if (exitflag) break;
+    double red = err[km] > 0.0 ? fminr(fmaxr(SAFE2 / err[km], REDMAX), REDMIN) : REDMIN;
+    h *= red;
+    reduct = true;
+  }
+
+  *xx = st.xnew;
+  double hdid = h;
+  st.first = false;
+
+  double wrkmin = 1.0e35;
+  double scale = 1.0;
+  for (int kk = 0; kk <= km; ++kk) {
+    double fact = fmaxr(err[kk], SCALMX);
+    double work = fact * st.a[kk + 1];
+    if (work < wrkmin) {
+      scale = fact;
+      wrkmin = work;
+      st.kopt = kk + 1;
+    }
+  }
+  double hnext = h / scale;
+  int k_final = km + 1;
+  if (st.kopt >= k_final && st.kopt != st.kmax && !reduct) {
+    chk(st.kopt >= 1 && st.kopt + 1 < IMAXX);
+    double fact = fmaxr(scale / st.alf[st.kopt - 1][st.kopt], SCALMX);
+    if (st.a[st.kopt + 1] * fact <= wrkmin) {
+      hnext = h / fact;
+      st.kopt += 1;
+    }
+  }
+  *hdid_out = hdid;
+  *hnext_out = hnext;
+}
+
+// crates/roche/src/stream.rs:26-62
+void gsint(double q, Vec3* r, Vec3* v, double ttry, double* time, double eps, BsState& st,
+           double* tdid, double* tnext) {
+  double y[NV] = {r->x, r->y, r->z, v->x, v->y, v->z};
+  double dydt[NV];
+  rocder(q, y, dydt);
+  const double yscal[NV] = {1, 1, 1, 1, 1, 1};
+  bool underflow;
+  bsstep(y, dydt, time, ttry, eps, yscal, st, q, tdid, tnext, &underflow);
+  *r = Vec3(y[0], y[1], y[2]);
+  *v = Vec3(y[3], y[4], y[5]);
+}
+}  // namespace
+
+// crates/roche/src/stream.rs:8-20
+void strinit(double q, Vec3* r, Vec3* v) {
+  const double SMALL = 1.0e-5;
+  double rl1 = xl1(q);
+  double mu = q / (1.0 + q);
+  double a = (1.0 - mu) / powi3(rl1) + mu / powi3(1.0 - rl1);
+  double lambda1 = std::sqrt(((a - 2.0) + std::sqrt(a * (9.0 * a - 8.0))) / 2.0);
+  double m1 = (lambda1 * lambda1 - 2.0 * a - 1.0) / (2.0 * lambda1);
+  *r = Vec3(rl1 - SMALL, -m1 * SMALL, 0.0);
+  *v = Vec3(-lambda1 * SMALL, -lambda1 * m1 * SMALL, 0.0);
+}
+
+// crates/roche/src/stream.rs:82-145
+double stradv(double q, Vec3* r, Vec3* v, double rad, double acc, double smax) {
+  const double EPS = 1.0e-8;
+  const double TMAX = 10.0;
+  BsState st;
+  double rinit = r->length();
+  double rnow = rinit;
+  double time = 0.0;
+  double tnext = 1.0e-2;
+  // Guard against a stalled integrator (the reference loops until TMAX).
+  for (long iter = 0; iter < 100000000L; ++iter) {
+    Vec3 ro = *r, vo = *v;
+    double ttry = fminr(tnext, smax);
+    double tdid, tn;
+    gsint(q, r, v, ttry, &time, EPS, st, &tdid, &tn);
+    tnext = tn;
+    rnow = r->length();
+    if ((rinit > rad && rnow <= rad) || (rinit < rad && rnow >= rad)) {
+      double lo = 0.0, hi = tdid;
+      double rlo = ro.length(), rhi = rnow;
+      double to = time - tdid;
+      while (std::fabs(rhi - rlo) > acc) {
+        double tmid = (lo + hi) / 2.0;
+        *r = ro;
+        *v = vo;
+        double t = to;
+        BsState st2;
+        double d1, d2;
+        gsint(q, r, v, tmid, &t, EPS, st2, &d1, &d2);
+        rnow = r->length();
+        if ((rhi > rad && rnow > rad) || (rhi < rad && rnow < rad)) {
+          rhi = rnow;
+          hi = tmid;
+        } else {
+          rlo = rnow;
+          lo = tmid;
+        }
+      }
+      return time;
+    }
+    if (time > TMAX) throw LcError("stradv: took too long without crossing radius");
+  }
+  throw LcError("stradv: integrator stalled");
+}
+
+}  // namespace lc

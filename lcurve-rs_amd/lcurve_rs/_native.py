@@ -1,0 +1,120 @@
+"""ctypes binding of liblcurve_hip.so (declarations in include/lcurve_hip.h).
+
+The library is loaded from the package directory's parent
+(``lcurve-rs_amd/liblcurve_hip.so``) or from ``$LCURVE_HIP_LIB``. A missing
+library raises ImportError: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("LCURVE_HIP_LIB", os.path.join(os.path.dirname(_HERE), "liblcurve_hip.so"))
+
+LCH_OK = 0
+LCH_ERR_INVALID = -1
+LCH_ERR_PARSE = -2
+LCH_ERR_SETUP = -3
+LCH_ERR_DEVICE = -4
+LCH_ERR_RUNTIME = -5
+LCH_ERR_UNSUPPORTED = -6
+LCH_ERR_NOMEM = -7
+LCH_BATCH_KEEP_MAPS = 1
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+VP = ctypes.c_void_p
+CP = ctypes.c_char_p
+INT = ctypes.c_int
+
+
+class LcResultC(ctypes.Structure):
+    _fields_ = [("wdwarf", ctypes.c_double), ("chisq", ctypes.c_double), ("wnok", ctypes.c_double),
+                ("logg1", ctypes.c_double), ("logg2", ctypes.c_double), ("rv1", ctypes.c_double),
+                ("rv2", ctypes.c_double), ("sfac", ctypes.c_double * 5)]
+
+
+class BatchStatsC(ctypes.Structure):
+    _fields_ = [("n_sets", ctypes.c_longlong), ("n_valid", ctypes.c_longlong),
+                ("n_times", ctypes.c_longlong), ("n_lanes", ctypes.c_longlong),
+                ("n_items", ctypes.c_longlong), ("visits", ctypes.c_longlong),
+                ("record_bytes", ctypes.c_longlong), ("setup_seconds", ctypes.c_double),
+                ("upload_seconds", ctypes.c_double), ("chunk", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "lch_version": (INT, []),
+    "lch_last_error": (CP, []),
+    "lch_device_count": (INT, []),
+    "lch_ctx_create": (INT, [INT, ctypes.POINTER(VP)]),
+    "lch_ctx_destroy": (None, [VP]),
+    "lch_ctx_set_tuning": (INT, [VP, INT, INT]),
+    "lch_ctx_sync": (INT, [VP]),
+    "lch_model_load_file": (INT, [CP, ctypes.POINTER(VP)]),
+    "lch_model_load_text": (INT, [CP, ctypes.POINTER(VP)]),
+    "lch_model_copy": (INT, [VP, ctypes.POINTER(VP)]),
+    "lch_model_free": (None, [VP]),
+    "lch_model_set_param": (INT, [VP, CP, ctypes.c_double]),
+    "lch_model_get_param": (INT, [VP, CP, D]),
+    "lch_model_get_pparam": (INT, [VP, CP, D, D, D, I, I]),
+    "lch_model_set_pparam": (INT, [VP, CP, D, D, D, I]),
+    "lch_param_count": (INT, []),
+    "lch_param_name": (CP, [INT]),
+    "lch_data_count": (INT, [CP, I]),
+    "lch_data_read": (INT, [CP, INT, D, D, D, D, D, I]),
+    "lch_light_curve": (INT, [VP, VP, D, D, D, D, D, I, INT, INT, INT, D, ctypes.POINTER(LcResultC)]),
+    "lch_chisq_batch": (INT, [VP, VP, D, D, D, D, D, I, INT, ctypes.POINTER(CP), INT, D, INT, INT, D]),
+    "lch_batch_build": (INT, [VP, VP, D, D, D, D, D, I, INT, ctypes.POINTER(CP), INT, D, INT,
+                              D, D, D, INT, ctypes.POINTER(VP)]),
+    "lch_batch_build_models": (INT, [VP, ctypes.POINTER(VP), INT, D, D, D, D, D, I, INT, D, D, D, INT,
+                                     ctypes.POINTER(VP)]),
+    "lch_batch_free": (None, [VP]),
+    "lch_batch_launch": (INT, [VP, VP, INT]),
+    "lch_batch_run": (INT, [VP, VP, INT, D, D, D, D, D]),
+    "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
+    "lch_batch_autotune": (INT, [VP, VP, INT]),
+    "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
+    "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
+    "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises ImportError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"liblcurve_hip.so not found at {LIB_PATH}; build it with "
+                              f"`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    msg = lib().lch_last_error()
+    return msg.decode(errors="replace") if msg else ""
+
+
+class LcurveError(RuntimeError):
+    """A failing engine call; ``code`` is the LCH_ERR_* status."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def check(rc):
+    if rc != LCH_OK:
+        msg = last_error()
+        if rc in (LCH_ERR_INVALID, LCH_ERR_PARSE, LCH_ERR_SETUP, LCH_ERR_UNSUPPORTED):
+            raise ValueError(msg)
+        raise LcurveError(rc, msg)
+    return rc

@@ -1,0 +1,625 @@
+// ORACLE — test infrastructure only.
+//
+// FP64 CPU restatement of the lcurve-rs flux path, used as the checker for
+// the HIP engine: only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline may load it. It is never part of the product path.
+//
+// What it restates (reference file:line):
+//   comp_light            crates/lcurve/src/flux.rs:7-111
+//   comp_star1/comp_star2 crates/lcurve/src/flux.rs:115-200
+//   comp_disc/edge/spot   crates/lcurve/src/flux.rs:203-279
+//   exposure_weight       crates/lcurve/src/flux.rs:378-387 (lc_core.h)
+//   lensing_mag           crates/lcurve/src/flux.rs:390-406
+//   light_curve_comp      crates/lcurve/src/orchestration.rs:36-318 (iscale=0)
+//   chisq_batch           crates/lcurve/src/orchestration.rs:329-357
+//   re_scale              crates/lcurve/src/orchestration.rs:361-388
+//   compute_chisq         crates/lcurve/src/orchestration.rs:391-401
+// Grid / eclipse / brightness setup is the product's CPU setup code
+// (lcurve-rs_amd/csrc/setup), compiled into this library from the same
+// sources, so the oracle checks the flux sum on identical grids.
+//
+// Parallel decomposition mirrors the reference's Rayon use: over times in
+// one light curve (orchestration.rs:229) and over parameter sets in a batch
+// (orchestration.rs:343).
+//
+// Pinning: the test-model golden curve (test_data/test_lc.dat, C++ lroche),
+// batch == sequential, NaN on an unphysical set, and closed forms (uniform
+// sphere pi r^2 B, linear limb darkening pi r^2 B (1 - u/3), constant-curve
+// smearing, re_scale on an exact multiple). Disc, edge, bright spot, Claret,
+// beaming, lensing, ndiv>1 and pdot/deltat are pinned only by those closed
+// forms and invariants — parity unpinned against the Rust reference itself.
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "prepare.h"
+
+using namespace lc;
+
+namespace {
+
+using Grid = std::vector<Point>;
+
+// crates/lcurve/src/flux.rs:390-406
+double lensing_mag(const Vec3& posn, const Vec3& earth, double rlens1) {
+  Vec3 s = posn;
+  double d = -dot(s, earth);
+  if (d <= 0.0) return 1.0;
+  double p = (s + d * earth).length();
+  double ph = p / 2.0;
+  double phsq = ph * ph;
+  double rd = rlens1 * d;
+  double pd = phsq > 25.0 * rd ? p + rd / p : ph + std::sqrt(phsq + rd);
+  return pd * pd / (pd - ph) / ph / 4.0;
+}
+
+struct FluxCtx {
+  double iangle, q;
+  LDC ldc1, ldc2;
+  double lin_limb_disc, quad_limb_disc;
+  double beam_factor1, beam_factor2, spin1, spin2;
+  float vscale;
+  bool glens1;
+  double rlens1;
+};
+
+// crates/lcurve/src/flux.rs:7-111
+double comp_light(const FluxCtx& c, double phase, double expose, int ndiv, const Ginterp& gint,
+                  const Grid& star1f, const Grid& star2f, const Grid& star1c, const Grid& star2c,
+                  const Grid& disc, const Grid& edge, const Grid& spot) {
+  double xcofm = c.q / (1.0 + c.q);
+  double cosi = std::cos(TWOPI * c.iangle / 360.0);
+  double sini = std::sin(TWOPI * c.iangle / 360.0);
+  double vfac = static_cast<double>(c.vscale) / (C_LIGHT / 1e3);
+  double sum = 0.0;
+  for (int nd = 0; nd < ndiv; ++nd) {
+    double phi, wgt;
+    exposure_weight(phase, expose, nd, ndiv, &phi, &wgt);
+    Vec3 earth = set_earth(cosi, sini, phi);
+    int ptype = gint.grid_type(phi);
+    const Grid& star1 = ptype == 1 ? star1f : star1c;
+    const Grid& star2 = ptype == 3 ? star2f : star2c;
+
+    double ssum = 0.0;
+    for (const Point& pt : star1) {
+      if (!pt.visible(phi)) continue;
+      double mu = dot(earth, pt.dirn);
+      if (!c.ldc1.see(mu)) continue;
+      if (c.beam_factor1 != 0.0) {
+        double vx = -vfac * c.spin1 * pt.posn.y;
+        double vy = vfac * (c.spin1 * pt.posn.x - xcofm);
+        double vr = -(earth.x * vx + earth.y * vy);
+        double vn = pt.dirn.x * vx + pt.dirn.y * vy;
+        double mud = mu - mu * vr - vn;
+        ssum += mu * static_cast<double>(pt.flux) * (1.0 - c.beam_factor1 * vr) * c.ldc1.imu(mud);
+      } else {
+        ssum += mu * static_cast<double>(pt.flux) * c.ldc1.imu(mu);
+      }
+    }
+    ssum *= gint.scale1(phi);
+
+    double ssum2 = 0.0;
+    for (const Point& pt : star2) {
+      if (!pt.visible(phi)) continue;
+      double mu = dot(earth, pt.dirn);
+      if (!c.ldc2.see(mu)) continue;
+      double mag = c.glens1 ? lensing_mag(pt.posn, earth, c.rlens1) : 1.0;
+      if (c.beam_factor2 != 0.0) {
+        double vx = -vfac * c.spin2 * pt.posn.y;
+        double vy = vfac * (c.spin2 * (pt.posn.x - 1.0) + 1.0 - xcofm);
+        double vr = -(earth.x * vx + earth.y * vy);
+        double vn = pt.dirn.x * vx + pt.dirn.y * vy;
+        double mud = mu - mu * vr - vn;
+        ssum2 += mu * mag * static_cast<double>(pt.flux) * (1.0 - c.beam_factor2 * vr) *
+                 c.ldc2.imu(mud);
+      } else {
+        ssum2 += mu * mag * static_cast<double>(pt.flux) * c.ldc2.imu(mu);
+      }
+    }
+    ssum += gint.scale2(phi) * ssum2;
+
+    for (const Grid* g : {&disc, &edge}) {
+      for (const Point& pt : *g) {
+        double mu = dot(earth, pt.dirn);
+        if (mu > 0.0 && pt.visible(phi)) {
+          double ommu = 1.0 - mu;
+          ssum += mu * static_cast<double>(pt.flux) *
+                  (1.0 - ommu * (c.lin_limb_disc + c.quad_limb_disc * ommu));
+        }
+      }
+    }
+    for (const Point& pt : spot) {
+      double mu = dot(earth, pt.dirn);
+      if (mu > 0.0 && pt.visible(phi)) ssum += mu * static_cast<double>(pt.flux);
+    }
+    sum += wgt * ssum;
+  }
+  return sum / static_cast<double>(std::max(1, ndiv - 1));
+}
+
+// crates/lcurve/src/flux.rs:115-154
+double comp_star1(const FluxCtx& c, double phase, double expose, int ndiv, const Ginterp& gint,
+                  const Grid& star1f, const Grid& star1c) {
+  double xcofm = c.q / (1.0 + c.q);
+  double ri = to_radians(c.iangle);
+  double cosi = std::cos(ri), sini = std::sin(ri);
+  double vfac = static_cast<double>(c.vscale) / (C_LIGHT / 1e3);
+  double sum = 0.0;
+  for (int nd = 0; nd < ndiv; ++nd) {
+    double phi, wgt;
+    exposure_weight(phase, expose, nd, ndiv, &phi, &wgt);
+    Vec3 earth = set_earth(cosi, sini, phi);
+    const Grid& star1 = gint.grid_type(phi) == 1 ? star1f : star1c;
+    double ssum = 0.0;
+    for (const Point& pt : star1) {
+      if (!pt.visible(phi)) continue;
+      double mu = dot(earth, pt.dirn);
+      if (!c.ldc1.see(mu)) continue;
+      if (c.beam_factor1 != 0.0) {
+        double vx = -vfac * pt.posn.y;
+        double vy = vfac * (pt.posn.x - xcofm);
+        double vr = -(earth.x * vx + earth.y * vy);
+        double vn = pt.dirn.x * vx + pt.dirn.y * vy;
+        double mud = mu - mu * vr - vn;
+        ssum += mu * static_cast<double>(pt.flux) * (1.0 - c.beam_factor1 * vr) * c.ldc1.imu(mud);
+      } else {
+        double vr = vfac * (earth.x * pt.posn.y - earth.y * (pt.posn.x - xcofm));
+        ssum += mu * static_cast<double>(pt.flux) * (1.0 - c.beam_factor1 * vr) * c.ldc1.imu(mu);
+      }
+    }
+    sum += wgt * gint.scale1(phase) * ssum;
+  }
+  return sum / static_cast<double>(std::max(1, ndiv - 1));
+}
+
+// crates/lcurve/src/flux.rs:158-200
+double comp_star2(const FluxCtx& c, double phase, double expose, int ndiv, const Ginterp& gint,
+                  const Grid& star2f, const Grid& star2c) {
+  double xcofm = c.q / (1.0 + c.q);
+  double ri = to_radians(c.iangle);
+  double cosi = std::cos(ri), sini = std::sin(ri);
+  double vfac = static_cast<double>(c.vscale) / (C_LIGHT / 1e3);
+  double sum = 0.0;
+  for (int nd = 0; nd < ndiv; ++nd) {
+    double phi, wgt;
+    exposure_weight(phase, expose, nd, ndiv, &phi, &wgt);
+    Vec3 earth = set_earth(cosi, sini, phi);
+    const Grid& star2 = gint.grid_type(phi) == 3 ? star2f : star2c;
+    double ssum = 0.0;
+    for (const Point& pt : star2) {
+      if (!pt.visible(phi)) continue;
+      double mu = dot(earth, pt.dirn);
+      if (!c.ldc2.see(mu)) continue;
+      double mag = c.glens1 ? lensing_mag(pt.posn, earth, c.rlens1) : 1.0;
+      if (c.beam_factor2 != 0.0) {
+        double vx = -vfac * pt.posn.y;
+        double vy = vfac * (pt.posn.x - xcofm);
+        double vr = -(earth.x * vx + earth.y * vy);
+        double vn = pt.dirn.x * vx + pt.dirn.y * vy;
+        double mud = mu - mu * vr - vn;
+        ssum += mu * mag * static_cast<double>(pt.flux) * (1.0 - c.beam_factor2 * vr) *
+                c.ldc2.imu(mud);
+      } else {
+        double vr = vfac * (earth.x * pt.posn.y - earth.y * (pt.posn.x - xcofm));
+        ssum += mu * mag * static_cast<double>(pt.flux) * (1.0 - c.beam_factor2 * vr) *
+                c.ldc2.imu(mu);
+      }
+    }
+    sum += wgt * gint.scale2(phi) * ssum;
+  }
+  return sum / static_cast<double>(std::max(1, ndiv - 1));
+}
+
+// crates/lcurve/src/orchestration.rs:361-388
+void re_scale(const Data& data, std::vector<double>& fit, double* scale, double* chisq,
+              double* wnok) {
+  double sdy = 0.0, syy = 0.0, wn = 0.0;
+  for (size_t i = 0; i < data.size(); ++i) {
+    const Datum& d = data[i];
+    if (d.weight > 0.0) {
+      double wgt = d.weight / sqr(d.ferr);
+      sdy += wgt * d.flux * fit[i];
+      syy += wgt * fit[i] * fit[i];
+      wn += d.weight;
+    }
+  }
+  if (wn > 0.0 && syy > 0.0) {
+    double s = sdy / syy;
+    for (double& f : fit) f *= s;
+    double cs = 0.0;
+    for (size_t i = 0; i < data.size(); ++i) {
+      const Datum& d = data[i];
+      if (d.weight > 0.0) cs += d.weight * sqr((d.flux - fit[i]) / d.ferr);
+    }
+    *scale = s;
+    *chisq = cs;
+    *wnok = wn;
+  } else {
+    *scale = 1.0;
+    *chisq = 0.0;
+    *wnok = 0.0;
+  }
+}
+
+// crates/lcurve/src/orchestration.rs:391-401
+void compute_chisq(const Data& data, const std::vector<double>& calc, double* chisq, double* wnok) {
+  double cs = 0.0, wn = 0.0;
+  for (size_t i = 0; i < data.size(); ++i) {
+    const Datum& d = data[i];
+    if (d.weight > 0.0) {
+      wn += d.weight;
+      cs += d.weight * sqr((d.flux - calc[i]) / d.ferr);
+    }
+  }
+  *chisq = cs;
+  *wnok = wn;
+}
+
+struct LcOut {
+  std::vector<double> calc;
+  double wdwarf = 0, chisq = 0, wnok = 0, logg1 = 0, logg2 = 0, rv1 = 0, rv2 = 0;
+  double sfac[5] = {1, 1, 1, 1, 1};
+  double t_setup = 0, t_flux = 0;
+};
+
+FluxCtx make_ctx(const Model& mdl, const PreparedModel& P) {
+  FluxCtx c;
+  c.iangle = mdl.iangle.value;
+  c.q = mdl.q.value;
+  c.ldc1 = P.ldc1;
+  c.ldc2 = P.ldc2;
+  c.lin_limb_disc = P.lin_limb_disc;
+  c.quad_limb_disc = P.quad_limb_disc;
+  c.beam_factor1 = P.beam_factor1;
+  c.beam_factor2 = P.beam_factor2;
+  c.spin1 = P.spin1;
+  c.spin2 = P.spin2;
+  c.vscale = static_cast<float>(mdl.velocity_scale.value);
+  c.glens1 = P.glens1;
+  c.rlens1 = P.rlens1;
+  return c;
+}
+
+template <class F>
+void parallel_for(size_t n, int nthreads, F&& f) {
+  if (nthreads <= 1 || n <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> th;
+  int nt = static_cast<int>(std::min<size_t>(n, static_cast<size_t>(nthreads)));
+  for (int t = 0; t < nt; ++t) {
+    th.emplace_back([&]() {
+      for (;;) {
+        size_t i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i);
+      }
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+// Flux + scale/chi-squared on an already prepared model
+// (orchestration.rs:85-131 and 181-310, iscale = 0).
+void flux_and_chisq(const Model& mdl, const PreparedModel& P, const Data& data, bool scale,
+                    bool rdata, int nthreads, LcOut& out) {
+  if (mdl.iscale) throw LcError("oracle: iscale=1 is not restated");
+  FluxCtx c = make_ctx(mdl, P);
+  const Grid& s1f = P.star1f;
+  const Grid& s1c = P.star1_alias ? P.star1f : P.star1c;
+  const Grid& s2f = P.star2f;
+  const Grid& s2c = P.star2_alias ? P.star2f : P.star2c;
+
+  Ginterp gint;
+  gint.phase1 = P.phase1;
+  gint.phase2 = P.phase2;
+  if (P.need_cal1) {
+    const Grid& f = P.cal_copies ? P.star1f_cal : s1f;
+    const Grid& cg = P.cal_copies ? (P.star1_alias ? P.star1f_cal : P.star1c_cal) : s1c;
+    double ff = comp_star1(c, 0.9999999999 * P.phase1, 0.0, 1, gint, f, cg);
+    double fc = comp_star1(c, 1.0000000001 * P.phase1, 0.0, 1, gint, f, cg);
+    gint.scale11 = ff / fc;
+    ff = comp_star1(c, 1.0 - 0.9999999999 * P.phase1, 0.0, 1, gint, f, cg);
+    fc = comp_star1(c, 1.0 - 1.0000000001 * P.phase1, 0.0, 1, gint, f, cg);
+    gint.scale12 = ff / fc;
+  }
+  if (P.need_cal2) {
+    const Grid& f = P.cal_copies ? P.star2f_cal : s2f;
+    const Grid& cg = P.cal_copies ? (P.star2_alias ? P.star2f_cal : P.star2c_cal) : s2c;
+    double ff = comp_star2(c, 1.0 - 1.0000000001 * P.phase2, 0.0, 1, gint, f, cg);
+    double fc = comp_star2(c, 1.0 - 0.9999999999 * P.phase2, 0.0, 1, gint, f, cg);
+    gint.scale21 = ff / fc;
+    ff = comp_star2(c, 1.0000000001 * P.phase2, 0.0, 1, gint, f, cg);
+    fc = comp_star2(c, 0.9999999999 * P.phase2, 0.0, 1, gint, f, cg);
+    gint.scale22 = ff / fc;
+  }
+
+  size_t n = data.size();
+  out.calc.assign(n, 0.0);
+  parallel_for(n, nthreads, [&](size_t np) {
+    out.calc[np] = P.slfac[np] * comp_light(c, P.phase[np], P.expose[np], P.ndiv[np], gint, s1f, s2f,
+                                            s1c, s2c, P.disc, P.edge, P.spot) +
+                   P.third;
+  });
+
+  out.wdwarf = comp_star1(c, 0.5, 0.0, 1, gint, s1f, s1c);
+  for (double& s : out.sfac) s = 1.0;
+  out.chisq = 0.0;
+  out.wnok = 0.0;
+  if (scale) {
+    double s, cs, wn;
+    re_scale(data, out.calc, &s, &cs, &wn);
+    out.wdwarf *= s;
+    out.chisq = cs;
+    out.wnok = wn;
+    for (double& v : out.sfac) v = s;
+  } else {
+    out.wdwarf *= out.sfac[0];
+    for (double& v : out.calc) v *= out.sfac[0];
+    if (rdata) compute_chisq(data, out.calc, &out.chisq, &out.wnok);
+  }
+  out.logg1 = P.logg1;
+  out.logg2 = P.logg2;
+  out.rv1 = P.rv1;
+  out.rv2 = P.rv2;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void copy_err(const std::string& m, char* err, int errlen) {
+  if (err && errlen > 0) {
+    std::strncpy(err, m.c_str(), static_cast<size_t>(errlen - 1));
+    err[errlen - 1] = 0;
+  }
+}
+
+Data make_data(const double* time, const double* expose, const double* flux, const double* ferr,
+               const double* weight, const int* ndiv, int n) {
+  Data d(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    d[i].time = time[i];
+    d[i].expose = expose[i];
+    d[i].flux = flux[i];
+    d[i].ferr = ferr[i];
+    d[i].weight = weight[i];
+    d[i].ndiv = ndiv[i];
+  }
+  return d;
+}
+
+std::vector<std::string> split_names(const char* names) {
+  std::vector<std::string> v;
+  std::string cur;
+  for (const char* p = names; *p; ++p) {
+    if (*p == '\n') {
+      v.push_back(cur);
+      cur.clear();
+    } else {
+      cur += *p;
+    }
+  }
+  if (!cur.empty()) v.push_back(cur);
+  return v;
+}
+
+struct OracleBatch {
+  Model base;
+  std::vector<Model> models;
+  std::vector<Data> data;  // per set
+  std::vector<std::unique_ptr<PreparedModel>> prep;
+  double t_setup = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+// Light curve of one model (crates/lcurve/src/orchestration.rs:36-318).
+// scalars_out: wdwarf, chisq, wnok, logg1, logg2, rv1, rv2, sfac[0..4],
+//              t_setup_s, t_flux_s  (14 doubles)
+int orc_light_curve(const char* model_text, const double* time, const double* expose,
+                    const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                    int n, int scale, int rdata, int nthreads, double* calc_out,
+                    double* scalars_out, char* err, int errlen) {
+  try {
+    Model mdl = Model::from_text(model_text);
+    Data data = make_data(time, expose, flux, ferr, weight, ndiv, n);
+    double t0 = now_s();
+    auto P = prepare_model(mdl, data);
+    double t1 = now_s();
+    LcOut out;
+    flux_and_chisq(mdl, *P, data, scale != 0, rdata != 0, nthreads, out);
+    double t2 = now_s();
+    for (int i = 0; i < n; ++i) calc_out[i] = out.calc[i];
+    double s[14] = {out.wdwarf, out.chisq, out.wnok, out.logg1, out.logg2, out.rv1, out.rv2,
+                    out.sfac[0], out.sfac[1], out.sfac[2], out.sfac[3], out.sfac[4], t1 - t0, t2 - t1};
+    for (int i = 0; i < 14; ++i) scalars_out[i] = s[i];
+    return 0;
+  } catch (const std::exception& e) {
+    copy_err(e.what(), err, errlen);
+    return -1;
+  }
+}
+
+// Batch of parameter sets (orchestration.rs:329-357), split into a setup
+// call and a flux call so the flux + chi-squared part can be timed alone.
+// flux2d / ferr2d / weight2d, when non-null, give per-set observations
+// [nsets][n]; otherwise every set uses flux / ferr / weight.
+void* orc_batch_create(const char* model_text, const double* time, const double* expose,
+                       const double* flux, const double* ferr, const double* weight,
+                       const int* ndiv, int n, const char* names_nl, const double* values,
+                       int nsets, const double* flux2d, const double* ferr2d,
+                       const double* weight2d, int nthreads, char* err, int errlen) {
+  try {
+    auto B = std::make_unique<OracleBatch>();
+    B->base = Model::from_text(model_text);
+    auto names = split_names(names_nl);
+    size_t ndim = names.size();
+    B->models.assign(static_cast<size_t>(nsets), B->base);
+    B->data.resize(static_cast<size_t>(nsets));
+    B->prep.resize(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) {
+      for (size_t j = 0; j < ndim; ++j)
+        B->models[s].set_param_value(names[j], values[static_cast<size_t>(s) * ndim + j]);
+      const double* fx = flux2d ? flux2d + static_cast<size_t>(s) * n : flux;
+      const double* fe = ferr2d ? ferr2d + static_cast<size_t>(s) * n : ferr;
+      const double* wt = weight2d ? weight2d + static_cast<size_t>(s) * n : weight;
+      B->data[s] = make_data(time, expose, fx, fe, wt, ndiv, n);
+    }
+    double t0 = now_s();
+    parallel_for(static_cast<size_t>(nsets), nthreads, [&](size_t s) {
+      try {
+        B->prep[s] = prepare_model(B->models[s], B->data[s]);
+      } catch (const std::exception&) {
+        B->prep[s].reset();
+      }
+    });
+    B->t_setup = now_s() - t0;
+    return B.release();
+  } catch (const std::exception& e) {
+    copy_err(e.what(), err, errlen);
+    return nullptr;
+  }
+}
+
+// Same with one model text per set (e.g. per-set grid resolution).
+void* orc_batch_create_texts(const char* const* texts, int nsets, const double* time,
+                             const double* expose, const double* flux, const double* ferr,
+                             const double* weight, const int* ndiv, int n, const double* flux2d,
+                             const double* ferr2d, const double* weight2d, int nthreads, char* err,
+                             int errlen) {
+  try {
+    auto B = std::make_unique<OracleBatch>();
+    B->models.reserve(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) B->models.push_back(Model::from_text(texts[s]));
+    B->data.resize(static_cast<size_t>(nsets));
+    B->prep.resize(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) {
+      const double* fx = flux2d ? flux2d + static_cast<size_t>(s) * n : flux;
+      const double* fe = ferr2d ? ferr2d + static_cast<size_t>(s) * n : ferr;
+      const double* wt = weight2d ? weight2d + static_cast<size_t>(s) * n : weight;
+      B->data[s] = make_data(time, expose, fx, fe, wt, ndiv, n);
+    }
+    double t0 = now_s();
+    parallel_for(static_cast<size_t>(nsets), nthreads, [&](size_t s) {
+      try {
+        B->prep[s] = prepare_model(B->models[s], B->data[s]);
+      } catch (const std::exception&) {
+        B->prep[s].reset();
+      }
+    });
+    B->t_setup = now_s() - t0;
+    return B.release();
+  } catch (const std::exception& e) {
+    copy_err(e.what(), err, errlen);
+    return nullptr;
+  }
+}
+
+double orc_batch_setup_seconds(void* h) { return static_cast<OracleBatch*>(h)->t_setup; }
+
+// Flux + chi-squared for every set; NaN for sets whose setup failed.
+// flux_out (optional): [nsets][n] light curves.
+int orc_batch_flux(void* h, int scale, int nthreads, double* chisq_out, double* flux_out) {
+  auto* B = static_cast<OracleBatch*>(h);
+  size_t ns = B->models.size();
+  parallel_for(ns, nthreads, [&](size_t s) {
+    double nan = std::nan("");
+    if (!B->prep[s]) {
+      chisq_out[s] = nan;
+      if (flux_out)
+        for (size_t i = 0; i < B->data[s].size(); ++i) flux_out[s * B->data[s].size() + i] = nan;
+      return;
+    }
+    try {
+      LcOut out;
+      flux_and_chisq(B->models[s], *B->prep[s], B->data[s], scale != 0, true, 1, out);
+      chisq_out[s] = out.chisq;
+      if (flux_out)
+        for (size_t i = 0; i < out.calc.size(); ++i) flux_out[s * out.calc.size() + i] = out.calc[i];
+    } catch (const std::exception&) {
+      chisq_out[s] = nan;
+    }
+  });
+  return 0;
+}
+
+void orc_batch_free(void* h) { delete static_cast<OracleBatch*>(h); }
+
+// Eclipse visibility of one grid of a prepared model at the given phases:
+// mask_out[p * nelem + e] = Point::visible(phases[p]) (types.rs:50-58).
+// grid: 0 star1f, 1 star1c, 2 star2f, 3 star2c, 4 disc, 5 edge, 6 spot.
+int orc_batch_visibility(void* h, int set, int grid, const double* phases, int nphase,
+                         unsigned char* mask_out, long long* nelem_out) {
+  auto* B = static_cast<OracleBatch*>(h);
+  if (set < 0 || static_cast<size_t>(set) >= B->prep.size() || !B->prep[set]) return -1;
+  const PreparedModel& P = *B->prep[set];
+  const Grid* g = nullptr;
+  switch (grid) {
+    case 0: g = &P.star1f; break;
+    case 1: g = P.star1_alias ? &P.star1f : &P.star1c; break;
+    case 2: g = &P.star2f; break;
+    case 3: g = P.star2_alias ? &P.star2f : &P.star2c; break;
+    case 4: g = &P.disc; break;
+    case 5: g = &P.edge; break;
+    case 6: g = &P.spot; break;
+    default: return -2;
+  }
+  *nelem_out = static_cast<long long>(g->size());
+  if (!mask_out) return 0;
+  for (int p = 0; p < nphase; ++p)
+    for (size_t e = 0; e < g->size(); ++e)
+      mask_out[static_cast<size_t>(p) * g->size() + e] = (*g)[e].visible(phases[p]) ? 1 : 0;
+  return 0;
+}
+
+// Geometry of one grid of a prepared set, for closed-form and brute-force
+// checks: posn[3n], dirn[3n], area[n], flux[n], necl[n] (each may be null).
+// Returns the element count, or -1 for a bad set / grid.
+long long orc_batch_grid(void* h, int set, int grid, double* posn, double* dirn, double* area,
+                         double* flux, int* necl) {
+  auto* B = static_cast<OracleBatch*>(h);
+  if (set < 0 || static_cast<size_t>(set) >= B->prep.size() || !B->prep[set]) return -1;
+  const PreparedModel& P = *B->prep[set];
+  const Grid* g = nullptr;
+  switch (grid) {
+    case 0: g = &P.star1f; break;
+    case 1: g = P.star1_alias ? &P.star1f : &P.star1c; break;
+    case 2: g = &P.star2f; break;
+    case 3: g = P.star2_alias ? &P.star2f : &P.star2c; break;
+    case 4: g = &P.disc; break;
+    case 5: g = &P.edge; break;
+    case 6: g = &P.spot; break;
+    default: return -1;
+  }
+  for (size_t i = 0; i < g->size(); ++i) {
+    const Point& pt = (*g)[i];
+    if (posn) { posn[3 * i] = pt.posn.x; posn[3 * i + 1] = pt.posn.y; posn[3 * i + 2] = pt.posn.z; }
+    if (dirn) { dirn[3 * i] = pt.dirn.x; dirn[3 * i + 1] = pt.dirn.y; dirn[3 * i + 2] = pt.dirn.z; }
+    if (area) area[i] = pt.area;
+    if (flux) flux[i] = pt.flux;
+    if (necl) necl[i] = static_cast<int>(pt.eclipse.size());
+  }
+  return static_cast<long long>(g->size());
+}
+
+// Orbital phase of each time for a model (orchestration.rs:231-237).
+int orc_phases(const char* model_text, const double* time, int n, double* phase_out) {
+  try {
+    Model mdl = Model::from_text(model_text);
+    for (int i = 0; i < n; ++i) phase_out[i] = time_to_phase(mdl, time[i]);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+}  // extern "C"
