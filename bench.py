@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Throughput of the flux engine on one MI355X.
+
+Default workload: chisq_4096 — 4096 seeded parameter sets of the white-dwarf
++ Roche-donor binary with per-set star-2 resolution (10k-150k elements),
+each scored against its own 2000-time curve with 1% Gaussian noise,
+autoscaled. A step is one pass of the resident batch (flux kernel +
+FP64 assembly/chi-squared, one captured HIP graph); the CPU setup that
+builds the grids runs once before timing (it is the reference's CPU method
+and is reported separately). The metric is chi-squared evaluations per
+second.
+
+  python bench.py [--steps K] [--warmup W] [--workload chisq_4096|wd_donor_45k|disc_spot_300k]
+
+Prints one JSON line (plus progress lines on stderr).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lcurve-rs_amd"))
+
+import lcurve_rs as lc  # noqa: E402
+from lcurve_rs import workloads as W  # noqa: E402
+
+FLOP_PER_VISIT = 19.0        # FP32 flops per element-visit of the reference formulation
+VALU_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E peak
+HOST_THREADS = 16            # the GPU box's CPU share
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def chisq_models(nsets):
+    sets = W.chisq_sets(nsets)
+    base = W.chisq_base_text()
+    texts = [W.model_text({"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4],
+                           "nlat2f": int(s[5]), "nlat2c": int(s[5])}, base) for s in sets]
+    return texts, sets
+
+
+def build_workload(name, nsets):
+    """Returns (batch, info, cpu_builder) with observations in place."""
+    if name == "chisq_4096":
+        texts, sets = chisq_models(nsets)
+        times = W.linspace_times(-0.2, 1.2, 2000)
+        log(f"CPU setup of {nsets} sets (grids, eclipses, brightness) ...")
+        t0 = time.time()
+        batch = lc.Batch([lc.Model.from_text(x) for x in texts], times, 0.001, 1)
+        t_build = time.time() - t0
+        st = batch.stats()
+        log(f"setup {st['setup_seconds']:.1f} s, upload {st['upload_seconds']:.1f} s, "
+            f"{st['n_valid']}/{st['n_sets']} valid, {st['record_bytes'] / 1e9:.2f} GB records")
+        true = batch.run(scale=False)["flux"]
+        noise = W.chisq_noise(nsets, len(times))
+        obs = true * (1.0 + 0.01 * noise)
+        sig = 0.01 * np.abs(true)
+        batch.set_obs(obs, sig, np.ones_like(obs))
+        info = {"texts": texts, "times": times, "expose": 0.001, "ndiv": 1, "obs": obs, "sig": sig,
+                "scale": True, "unit_per_step": nsets, "build_s": t_build,
+                "config": {"workload": "chisq_4096", "sets": nsets, "times": len(times),
+                           "star2_nlat": "U{89..344}", "star2_elements": "10k-150k", "noise": 0.01,
+                           "autoscale": True, "seed": 42}}
+        return batch, info
+    if name == "wd_donor_45k":
+        text, times, expose, ndiv = W.wd_donor()
+        model = lc.Model.from_text(text)
+        batch = lc.Batch(model, times, expose, ndiv)
+        info = {"texts": [text], "times": times, "expose": expose, "ndiv": ndiv, "obs": None, "sig": None,
+                "scale": False, "unit_per_step": 1, "build_s": 0.0,
+                "config": {"workload": "wd_donor_45k", "star1_elements": 5067, "star2_elements": 39846,
+                           "times": 500, "ndiv": 1}}
+        return batch, info
+    if name == "disc_spot_300k":
+        text, times, expose, ndiv = W.disc_spot()
+        batch = lc.Batch(lc.Model.from_text(text), times, expose, ndiv)
+        info = {"texts": [text], "times": times, "expose": expose, "ndiv": ndiv, "obs": None, "sig": None,
+                "scale": False, "unit_per_step": 1, "build_s": 0.0,
+                "config": {"workload": "disc_spot_300k", "star2_elements": 200517, "disc_elements": 100713,
+                           "times": 5000, "ndiv": 5}}
+        return batch, info
+    raise SystemExit(f"unknown workload {name}")
+
+
+def parity_sample(batch, info, out, nsample):
+    """FP64 oracle on a few sets: max relative flux error and masks."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    texts = info["texts"]
+    idx = np.unique(np.linspace(0, len(texts) - 1, min(nsample, len(texts))).astype(int))
+    times = info["times"]
+    kw = {}
+    if info["obs"] is not None:
+        kw = dict(flux2d=info["obs"][idx], ferr2d=info["sig"][idx], weight2d=np.ones((len(idx), len(times))))
+    ob = pyoracle.OracleBatch([texts[i] for i in idx], times, info["expose"], info["ndiv"],
+                              nthreads=HOST_THREADS, **kw)
+    chi, flux = ob.flux(scale=info["scale"], nthreads=HOST_THREADS)
+    gflux = out["flux"][idx]
+    err = float(np.nanmax(np.abs(gflux - flux) / np.abs(flux)))
+    cerr = float(np.nanmax(np.abs(out["chisq"][idx] / chi - 1))) if info["scale"] else None
+    return err, cerr, len(idx)
+
+
+def cpu_baseline(info, budget_s):
+    """The FP64 oracle (CPU restatement of the reference's Rayon path,
+    parallel over sets) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    texts = info["texts"]
+    times = info["times"]
+    nsamp = min(len(texts), 4 * HOST_THREADS)
+    while True:
+        idx = np.arange(nsamp)
+        kw = {}
+        if info["obs"] is not None:
+            kw = dict(flux2d=info["obs"][idx], ferr2d=info["sig"][idx], weight2d=np.ones((nsamp, len(times))))
+        ob = pyoracle.OracleBatch([texts[i] for i in idx], times, info["expose"], info["ndiv"],
+                                  nthreads=HOST_THREADS, **kw)
+        t0 = time.time()
+        ob.flux(scale=info["scale"], nthreads=HOST_THREADS, want_flux=False)
+        dt = time.time() - t0
+        if dt > 2.0 or nsamp >= len(texts) or dt * 2 > budget_s:
+            break
+        nsamp = min(len(texts), nsamp * 2)
+    per_set_setup = ob.setup_seconds * HOST_THREADS / nsamp
+    return {"value": nsamp / dt if info["unit_per_step"] > 1 else 1.0 / dt,
+            "cores": HOST_THREADS, "kind": "port",
+            "sample": f"{nsamp} of {len(texts)} sets, all {len(times)} times, flux + chi^2 only "
+                      f"(grids already built); FP64 C++ restatement of the reference's CPU path",
+            "setup_core_seconds_per_set": round(per_set_setup, 4), "wall_s": round(dt, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="chisq_4096",
+                    choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k"])
+    ap.add_argument("--nsets", type=int, default=4096, help="sets of chisq_4096 (4096 is the workload)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--parity-sets", type=int, default=8)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    if not lc.has_hip():
+        raise SystemExit("no HIP device: the engine has no CPU path")
+    lc.set_device("hip:0")
+
+    batch, info = build_workload(args.workload, args.nsets)
+    scale = info["scale"]
+    if not args.no_autotune:
+        log("autotuning chunk size / launch form ...")
+        batch.autotune(scale=scale)
+    st = batch.stats()
+    log(f"tuning: chunk {st['chunk']}, persistent {st['persistent']}, items {st['n_items']}")
+
+    # correctness on a sample against the FP64 oracle
+    out = batch.run(scale=scale)
+    log("parity sample against the FP64 oracle ...")
+    flux_err, chi_err, nchk = parity_sample(batch, info, out, args.parity_sets)
+    log(f"max rel flux err {flux_err:.3e} over {nchk} sets")
+
+    for _ in range(args.warmup):
+        batch.launch(scale=scale)
+    batch.sync()
+    log(f"timing {args.steps} steps ...")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch(scale=scale)
+    batch.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / args.steps
+    value = info["unit_per_step"] * args.steps / elapsed
+
+    # Dominant kernel, timed with HIP events on its own stream.
+    ms_flux, ms_pass = batch.time(scale=scale, reps=max(3, min(args.steps, 10)))
+    visits = st["visits"]
+    tflops = visits * FLOP_PER_VISIT / (ms_flux * 1e-3) / 1e12
+    # Compulsory HBM bytes of one flux launch: element records once, lane
+    # tables, partial rows written.
+    lanes = st["n_lanes"]
+    alg_bytes = st["record_bytes"] + lanes * 12 + 8 * 64 * st["n_items"]
+    roof = {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
+            "algorithmic_flop_per_visit": FLOP_PER_VISIT, "visits_per_launch": visits,
+            "hbm_algorithmic_gbs": round(alg_bytes / (ms_flux * 1e-3) / 1e9, 2),
+            "hbm_frac": round(alg_bytes / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+
+    rec = {"metric": "chi2 evals/sec (1 GPU) at <=1e-6 rel flux err vs FP64" if args.workload == "chisq_4096"
+           else "light curves/sec (1 GPU)",
+           "value": round(value, 3), "unit": "chi2/s" if args.workload == "chisq_4096" else "curves/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+kahan/fp64",
+           "data": "synthetic (seeded parameter sets and noise; no external data)",
+           "config": info["config"],
+           "element_visits_per_step": visits, "visits_per_s": round(visits * args.steps / elapsed, 1),
+           "max_rel_flux_err_vs_fp64": flux_err, "parity_sets": nchk,
+           "max_rel_chisq_err_vs_fp64": chi_err,
+           "cpu_setup_s": round(st["setup_seconds"], 2), "upload_s": round(st["upload_seconds"], 2),
+           "tuning": {"chunk": st["chunk"], "persistent": st["persistent"], "items": st["n_items"]},
+           "roofline": roof}
+    if not args.no_cpu_baseline:
+        log("CPU baseline (FP64 oracle, bounded sample) ...")
+        rec["cpu_baseline"] = cpu_baseline(info, budget_s=30.0)
+        rec["cpu_baseline"]["unit"] = rec["unit"]
+        rec["speedup_vs_cpu_flux"] = round(value / rec["cpu_baseline"]["value"], 1)
+        # End to end: CPU setup per set is the same method on both sides.
+        s = rec["cpu_baseline"]["setup_core_seconds_per_set"]
+        if info["unit_per_step"] > 1:
+            rec["end_to_end"] = {
+                "gpu_chi2_per_s_incl_setup_16_cores": round(1.0 / (s / HOST_THREADS + 1.0 / value), 2),
+                "cpu_chi2_per_s_incl_setup_16_cores": round(
+                    1.0 / (s / HOST_THREADS + 1.0 / rec["cpu_baseline"]["value"]), 2)}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

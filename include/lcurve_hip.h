@@ -129,6 +129,10 @@ int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nse
                            const double* flux2d, const double* ferr2d, const double* weight2d,
                            int flags, lch_batch** out);
 void lch_batch_free(lch_batch* b);
+/* Replace the observations of every set: flux2d / ferr2d / weight2d are
+ * [nsets][n] (a null array keeps the current values). */
+int lch_batch_set_obs(lch_ctx* ctx, lch_batch* b, const double* flux2d, const double* ferr2d,
+                      const double* weight2d);
 /* Enqueue one pass (flux + assembly) on the context stream; no copy back. */
 int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale);
 /* One pass, wait, copy back (any output may be null). flux is [nsets][n]. */

@@ -346,6 +346,15 @@ int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nse
 
 void lch_batch_free(lch_batch* b) { delete b; }
 
+int lch_batch_set_obs(lch_ctx* ctx, lch_batch* b, const double* flux2d, const double* ferr2d,
+                      const double* weight2d) {
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->set_obs(flux2d, ferr2d, weight2d);
+    return LCH_OK;
+  });
+}
+
 int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale) {
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {

@@ -73,6 +73,8 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   H.logg2 = P.logg2;
   H.rv1 = P.rv1;
   H.rv2 = P.rv2;
+  H.star1_alias = P.star1_alias;
+  H.star2_alias = P.star2_alias;
 
   const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0, lens = P.glens1;
   d.flags = SET_VALID | (P.need_cal1 ? SET_CAL1 : 0u) | (P.need_cal2 ? SET_CAL2 : 0u) |

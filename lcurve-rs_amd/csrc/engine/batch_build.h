@@ -18,6 +18,7 @@ struct SetHost {
   double logg1 = 0, logg2 = 0, rv1 = 0, rv2 = 0;
   uint32_t n_times = 0;
   uint64_t visits = 0;  // element-visits in the reference's loop order
+  bool star1_alias = false, star2_alias = false;  // coarse grid = fine grid
   // Debug visibility map: per grid (0 star1f, 1 star1c, 2 star2f, 3 star2c,
   // 4 disc, 5 edge, 6 spot) its segments and each segment element's index
   // in the grid's original order.

@@ -388,6 +388,8 @@ void Batch::visibility(int set, int grid, const double* phases, int nphase, uint
                        uint32_t* n_out) {
   if (set < 0 || static_cast<size_t>(set) >= host_.size()) throw std::invalid_argument("set out of range");
   if (grid < 0 || grid > 6) throw std::invalid_argument("grid out of range");
+  // An aliased coarse grid is the fine grid (the reference clones it).
+  if ((grid == 1 && host_[set].star1_alias) || (grid == 3 && host_[set].star2_alias)) grid -= 1;
   const SetHost::GridMap& gm = host_[set].grids[grid];
   *n_out = gm.n;
   if (!mask || nphase <= 0 || gm.n == 0) return;
@@ -411,6 +413,22 @@ void Batch::visibility(int set, int grid, const double* phases, int nphase, uint
       for (uint32_t e = 0; e < sg.n; ++e) mask[static_cast<size_t>(p) * gm.n + orig[e]] = hm[static_cast<size_t>(p) * sg.n + e];
   }
   (void)hipFree(dph);
+}
+
+void Batch::set_obs(const double* flux2d, const double* ferr2d, const double* weight2d) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  const size_t n = t_y_.size();
+  if (n != sets_.size() * n_times_) throw std::invalid_argument("time tables do not match the set count");
+  auto put = [&](const double* src, std::vector<double>& host, const double* dev) {
+    if (!src) return;
+    host.assign(src, src + n);
+    hip_check(hipMemcpy(const_cast<double*>(dev), host.data(), n * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy observations");
+  };
+  put(flux2d, t_y_, d_.t_y);
+  put(ferr2d, t_ferr_, d_.t_ferr);
+  put(weight2d, t_w_, d_.t_w);
 }
 
 void Batch::autotune(int scale) {

@@ -65,6 +65,8 @@ class Batch {
                   uint32_t* n_out);
   // Pick the chunk size / launch form with the fastest flux kernel.
   void autotune(int scale);
+  // Replace observations, [n_sets][n_times] each (null keeps the current).
+  void set_obs(const double* flux2d, const double* ferr2d, const double* weight2d);
 
   uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
   const SetHost& set(uint32_t i) const { return host_[i]; }

@@ -71,9 +71,11 @@ double rpot(double q, const Vec3& p) {
   return -comp / r1 - mu / r2 - (x2y2 + mu * (mu - 2.0 * p.x)) / 2.0;
 }
 
+namespace {
+// Potentials without the q > 0 check: callers validate q once (every
+// setup path calls xl11 / xl12 first, which reject q <= 0).
 // crates/roche/src/potential.rs:22-33
-double rpot1(double q, double spin, const Vec3& p) {
-  check_q(q, "rpot1");
+inline double rpot1_u(double q, double spin, const Vec3& p) {
   double mu = q / (1.0 + q);
   double comp = 1.0 - mu;
   double x2y2 = sqr(p.x) + sqr(p.y);
@@ -84,8 +86,7 @@ double rpot1(double q, double spin, const Vec3& p) {
 }
 
 // crates/roche/src/potential.rs:36-47
-double rpot2(double q, double spin, const Vec3& p) {
-  check_q(q, "rpot2");
+inline double rpot2_u(double q, double spin, const Vec3& p) {
   double mu = q / (1.0 + q);
   double comp = 1.0 - mu;
   double x2y2 = sqr(p.x) + sqr(p.y);
@@ -96,8 +97,7 @@ double rpot2(double q, double spin, const Vec3& p) {
 }
 
 // crates/roche/src/potential.rs:68-86
-Vec3 drpot1(double q, double spin, const Vec3& p) {
-  check_q(q, "drpot1");
+inline Vec3 drpot1_u(double q, double spin, const Vec3& p) {
   double r1sq = p.sqr();
   double r2sq = r1sq + 1.0 - 2.0 * p.x;
   double r1 = std::sqrt(r1sq);
@@ -111,8 +111,7 @@ Vec3 drpot1(double q, double spin, const Vec3& p) {
 }
 
 // crates/roche/src/potential.rs:88-106
-Vec3 drpot2(double q, double spin, const Vec3& p) {
-  check_q(q, "drpot2");
+inline Vec3 drpot2_u(double q, double spin, const Vec3& p) {
   double r1sq = p.sqr();
   double r2sq = r1sq + 1.0 - 2.0 * p.x;
   double r1 = std::sqrt(r1sq);
@@ -124,24 +123,43 @@ Vec3 drpot2(double q, double spin, const Vec3& p) {
   return {comp * p.x + mu1 * (p.x - 1.0) - ssq * (p.x - 1.0) + mu - 1.0,
           comp * p.y + mu1 * p.y - ssq * p.y, comp * p.z + mu1 * p.z};
 }
+}  // namespace
+
+double rpot1(double q, double spin, const Vec3& p) {
+  check_q(q, "rpot1");
+  return rpot1_u(q, spin, p);
+}
+double rpot2(double q, double spin, const Vec3& p) {
+  check_q(q, "rpot2");
+  return rpot2_u(q, spin, p);
+}
+Vec3 drpot1(double q, double spin, const Vec3& p) {
+  check_q(q, "drpot1");
+  return drpot1_u(q, spin, p);
+}
+Vec3 drpot2(double q, double spin, const Vec3& p) {
+  check_q(q, "drpot2");
+  return drpot2_u(q, spin, p);
+}
 
 namespace {
 inline Vec3 star_cofm(Star s) { return s == Star::Primary ? Vec3(0, 0, 0) : Vec3(1.0, 0, 0); }
 
 inline double pot_star(double q, Star s, double spin, const Vec3& p) {
-  return s == Star::Primary ? rpot1(q, spin, p) : rpot2(q, spin, p);
+  return s == Star::Primary ? rpot1_u(q, spin, p) : rpot2_u(q, spin, p);
 }
 
-// crates/roche/src/potential.rs:109-126
-inline double rpot_along_los(double q, Star s, double spin, const Vec3& earth, const Vec3& p,
-                             double lam) {
+// Potential and its line-of-sight derivative at p + lam * earth
+// (crates/roche/src/potential.rs:109-126), specialised on the star.
+template <Star S>
+inline double rpot_along_los(double q, double spin, const Vec3& earth, const Vec3& p, double lam) {
   Vec3 point = p + lam * earth;
-  return pot_star(q, s, spin, point);
+  return S == Star::Primary ? rpot1_u(q, spin, point) : rpot2_u(q, spin, point);
 }
-inline double drpot_along_los(double q, Star s, double spin, const Vec3& earth, const Vec3& p,
-                              double lam) {
+template <Star S>
+inline double drpot_along_los(double q, double spin, const Vec3& earth, const Vec3& p, double lam) {
   Vec3 point = p + lam * earth;
-  Vec3 g = s == Star::Primary ? drpot1(q, spin, point) : drpot2(q, spin, point);
+  Vec3 g = S == Star::Primary ? drpot1_u(q, spin, point) : drpot2_u(q, spin, point);
   return dot(g, earth);
 }
 
@@ -318,17 +336,19 @@ bool sphere_eclipse_phase(double cosi, double sini, const Vec3& r, const Vec3& c
   return true;
 }
 
-// crates/roche/src/eclipse.rs:123-197
-bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec3& earth,
-            const Vec3& p) {
-  double rref, pref;
-  ref_sphere(q, star, spin, ffac, &rref, &pref);
-  Vec3 cofm = star_cofm(star);
+namespace {
+// fblink's body (crates/roche/src/eclipse.rs:123-197) with the reference
+// sphere passed in: ref_sphere is a pure function of (q, star, spin, ffac),
+// so ingress_egress computes it once for its many fblink calls.
+template <Star S>
+bool fblink_ref(double q, double spin, double acc, const Vec3& earth, const Vec3& p, double rref,
+                double pref) {
+  const Vec3 cofm = star_cofm(S);
   double lam1, lam2;
   if (!sphere_eclipse_los(earth, p, cofm, rref, &lam1, &lam2)) return false;
   if (lam1 == 0.0) return true;
 
-  auto func = [&](double lam) { return rpot_along_los(q, star, spin, earth, p, lam); };
+  auto func = [&](double lam) { return rpot_along_los<S>(q, spin, earth, p, lam); };
   size_t nstep = 1;
   double step_size = lam2 - lam1;
   double f1 = 0.0, f2 = 0.0, flam = 1.0;
@@ -354,7 +374,7 @@ bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec
     nstep *= 2;
   }
   if (flam < f1 && flam < f2) {
-    auto dfunc = [&](double x) { return drpot_along_los(q, star, spin, earth, p, x); };
+    auto dfunc = [&](double x) { return drpot_along_los<S>(q, spin, earth, p, x); };
     double fmin, xmin;
     if (dbrent(lam1, lam, lam2, func, dfunc, acc, true, pref, &fmin, &xmin)) return fmin < pref;
     return false;
@@ -362,45 +382,52 @@ bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec
   return false;
 }
 
-namespace {
+template <Star S>
+bool fblink_t(double q, double spin, double ffac, double acc, const Vec3& earth, const Vec3& p) {
+  double rref, pref;
+  ref_sphere(q, S, spin, ffac, &rref, &pref);
+  return fblink_ref<S>(q, spin, acc, earth, p, rref, pref);
+}
+
 // crates/roche/src/eclipse.rs:347-373
-bool pot_min_search(double q, double cosi, double sini, Star star, double spin, double ffac,
-                    const Vec3& r, double phi1, double phi2, double acc, double* phi_out) {
+template <Star S>
+bool pot_min_search(double q, double cosi, double sini, double spin, const Vec3& r, double phi1,
+                    double phi2, double acc, double rref, double pref, double* phi_out) {
   size_t nsteps = as_usize(std::ceil((phi2 - phi1) / (acc / TWOPI)));
   nsteps = std::min<size_t>(std::max<size_t>(nsteps, 10), 1000);
   double dphi = (phi2 - phi1) / static_cast<double>(nsteps);
   for (size_t i = 0; i <= nsteps; ++i) {
     double phi = phi1 + static_cast<double>(i) * dphi;
     Vec3 earth = set_earth(cosi, sini, phi);
-    if (fblink(q, star, spin, ffac, acc, earth, r)) {
+    if (fblink_ref<S>(q, spin, acc, earth, r, rref, pref)) {
       *phi_out = phi;
       return true;
     }
   }
   return false;
 }
-}  // namespace
 
 // crates/roche/src/eclipse.rs:270-338
-bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,
-                    const Vec3& r, double* ingress_out, double* egress_out) {
+template <Star S>
+bool ingress_egress_t(double q, double spin, double ffac, double iangle, double delta,
+                      const Vec3& r, double* ingress_out, double* egress_out) {
   double rref, pref;
-  ref_sphere(q, star, spin, ffac, &rref, &pref);
+  ref_sphere(q, S, spin, ffac, &rref, &pref);
   double irad = to_radians(iangle);
   double cosi = std::cos(irad);
   double sini = std::sin(irad);
-  Vec3 cofm = star_cofm(star);
+  Vec3 cofm = star_cofm(S);
   double phi1, phi2, lam1, lam2;
   if (!sphere_eclipse_phase(cosi, sini, r, cofm, rref, &phi1, &phi2, &lam1, &lam2)) return false;
   double acc = 2.0 * std::sqrt(2.0 * TWOPI * (lam2 - lam1) * delta);
   double phi;
-  if (!pot_min_search(q, cosi, sini, star, spin, ffac, r, phi1, phi2, acc, &phi)) return false;
+  if (!pot_min_search<S>(q, cosi, sini, spin, r, phi1, phi2, acc, rref, pref, &phi)) return false;
 
   double pin = phi, pout = phi1;
   while (std::fabs(pin - pout) > delta) {
     double pmid = (pin + pout) / 2.0;
     Vec3 earth = set_earth(cosi, sini, pmid);
-    if (fblink(q, star, spin, ffac, acc, earth, r)) pin = pmid;
+    if (fblink_ref<S>(q, spin, acc, earth, r, rref, pref)) pin = pmid;
     else pout = pmid;
   }
   double ingress = (pin + pout) / 2.0;
@@ -411,7 +438,7 @@ bool ingress_egress(double q, Star star, double spin, double ffac, double iangle
   while (std::fabs(pin - pout) > delta) {
     double pmid = (pin + pout) / 2.0;
     Vec3 earth = set_earth(cosi, sini, pmid);
-    if (fblink(q, star, spin, ffac, acc, earth, r)) pin = pmid;
+    if (fblink_ref<S>(q, spin, acc, earth, r, rref, pref)) pin = pmid;
     else pout = pmid;
   }
   double egress = (pin + pout) / 2.0;
@@ -420,6 +447,24 @@ bool ingress_egress(double q, Star star, double spin, double ffac, double iangle
   *ingress_out = ingress;
   *egress_out = egress;
   return true;
+}
+}  // namespace
+
+// crates/roche/src/eclipse.rs:123-197
+bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec3& earth,
+            const Vec3& p) {
+  check_q(q, "fblink");
+  return star == Star::Primary ? fblink_t<Star::Primary>(q, spin, ffac, acc, earth, p)
+                               : fblink_t<Star::Secondary>(q, spin, ffac, acc, earth, p);
+}
+
+bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,
+                    const Vec3& r, double* ingress_out, double* egress_out) {
+  check_q(q, "ingress_egress");
+  return star == Star::Primary
+             ? ingress_egress_t<Star::Primary>(q, spin, ffac, iangle, delta, r, ingress_out, egress_out)
+             : ingress_egress_t<Star::Secondary>(q, spin, ffac, iangle, delta, r, ingress_out,
+                                                 egress_out);
 }
 
 // ------------------------------------------------------------ disc eclipse

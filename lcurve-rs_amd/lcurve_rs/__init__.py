@@ -451,6 +451,13 @@ class Batch:
                                           _dptr(sfac), _dptr(wd), None if flux is None else _dptr(flux)))
         return {"chisq": chisq, "wnok": wnok, "sfac": sfac, "wdwarf": wd, "flux": flux}
 
+    def set_obs(self, flux2d=None, ferr2d=None, weight2d=None):
+        """Replace the observations of all sets ([n_sets, n_times] arrays)."""
+        shp = (self.n_sets, self.n_times)
+        arrs = [None if a is None else np.ascontiguousarray(np.broadcast_to(a, shp), dtype=np.float64)
+                for a in (flux2d, ferr2d, weight2d)]
+        check(_native.lib().lch_batch_set_obs(_ctx(), self._h, *[None if a is None else _dptr(a) for a in arrs]))
+
     def launch(self, scale=True):
         check(_native.lib().lch_batch_launch(_ctx(), self._h, int(bool(scale))))
 

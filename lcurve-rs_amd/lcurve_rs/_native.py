@@ -70,6 +70,7 @@ SIGNATURES = {
     "lch_batch_build_models": (INT, [VP, ctypes.POINTER(VP), INT, D, D, D, D, D, I, INT, D, D, D, INT,
                                      ctypes.POINTER(VP)]),
     "lch_batch_free": (None, [VP]),
+    "lch_batch_set_obs": (INT, [VP, VP, D, D, D]),
     "lch_batch_launch": (INT, [VP, VP, INT]),
     "lch_batch_run": (INT, [VP, VP, INT, D, D, D, D, D]),
     "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
