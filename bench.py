@@ -25,6 +25,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lcurve-rs_amd"))
+HOST_THREADS = 16            # the GPU box's CPU share
+os.environ.setdefault("LCURVE_HIP_THREADS", str(HOST_THREADS))
 
 import lcurve_rs as lc  # noqa: E402
 from lcurve_rs import workloads as W  # noqa: E402
@@ -32,7 +34,6 @@ from lcurve_rs import workloads as W  # noqa: E402
 FLOP_PER_VISIT = 19.0        # FP32 flops per element-visit of the reference formulation
 VALU_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E peak
-HOST_THREADS = 16            # the GPU box's CPU share
 
 
 def log(msg):

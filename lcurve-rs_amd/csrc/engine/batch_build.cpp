@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -267,7 +268,14 @@ std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& 
   std::vector<std::unique_ptr<SetPack>> packs(ns);
   if (data.empty()) throw lc::LcError("no data");
   auto data_of = [&](size_t s) -> const lc::Data& { return data.size() == 1 ? data[0] : data[s]; };
-  int nt = nthreads > 0 ? nthreads : static_cast<int>(std::thread::hardware_concurrency());
+  // Default: all cores, or LCURVE_HIP_THREADS when set (the CPU share a
+  // process may use on a shared node).
+  int nt = nthreads;
+  if (nt <= 0) {
+    const char* env = std::getenv("LCURVE_HIP_THREADS");
+    nt = env ? std::atoi(env) : 0;
+  }
+  if (nt <= 0) nt = static_cast<int>(std::thread::hardware_concurrency());
   if (nt < 1) nt = 1;
   nt = static_cast<int>(std::min<size_t>(static_cast<size_t>(nt), std::max<size_t>(ns, 1)));
   std::atomic<size_t> next{0};

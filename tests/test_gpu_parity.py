@@ -239,7 +239,8 @@ def test_launch_forms_and_autotune_agree(lcurve_rs):
     models = [lcurve_rs.Model.from_text(x) for x in texts]
     b = lcurve_rs.Batch(models, t, 0.001, 1)
     true = b.run(scale=False)["flux"]
-    b.set_obs(1.01 * true, 0.01 * true, np.ones_like(true))
+    obs = 1.01 * true * (1 + 0.01 * W.chisq_noise(len(texts), len(t), seed=3))
+    b.set_obs(obs, 0.01 * true, np.ones_like(true))
     ref = b.run(scale=True)
     b.autotune(scale=True)
     tuned = b.run(scale=True)
@@ -248,7 +249,7 @@ def test_launch_forms_and_autotune_agree(lcurve_rs):
     assert np.max(np.abs(b.run(scale=False)["flux"] / true - 1)) < 1e-6
     assert np.max(np.abs(tuned["flux"] / ref["flux"] - 1)) < 1e-6
     assert np.max(np.abs(tuned["chisq"] / ref["chisq"] - 1)) < 1e-5
-    assert np.allclose(ref["sfac"], 1.01, rtol=1e-6)
+    assert np.allclose(ref["sfac"], 1.01, rtol=3e-3)
 
 
 def test_light_curve_result_fields(lcurve_rs, golden_model_text):
