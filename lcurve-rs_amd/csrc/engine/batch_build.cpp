@@ -13,6 +13,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
+#include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -31,6 +34,8 @@ uint32_t mask_of(uint32_t cls, std::initializer_list<uint32_t> ptypes) {
 
 const std::initializer_list<uint32_t> kAll = {1u, 2u, 3u};
 
+constexpr double kConeMargin = 2e-3;
+
 struct GridSpec {
   uint32_t id;  // 0..6 visible grids, 7.. calibration copies
   const std::vector<lc::Point>* pts;
@@ -38,6 +43,67 @@ struct GridSpec {
   uint32_t use;
   bool posn;
 };
+
+// Morton code of a direction's octahedral map (16 bits per axis).
+uint64_t direction_key(const lc::Vec3& d) {
+  double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  double s = ax + ay + az;
+  if (!(s > 0.0)) return 0;
+  double u = d.x / s, v = d.y / s;
+  if (d.z < 0.0) {
+    double uu = (1.0 - std::fabs(v)) * (u >= 0.0 ? 1.0 : -1.0);
+    double vv = (1.0 - std::fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
+    u = uu;
+    v = vv;
+  }
+  auto q = [](double x) {
+    double t = (x + 1.0) * 0.5 * 65535.0;
+    return static_cast<uint64_t>(std::min(65535.0, std::max(0.0, t)));
+  };
+  auto spread = [](uint64_t x) {
+    x &= 0xFFFF;
+    x = (x | (x << 8)) & 0x00FF00FF;
+    x = (x | (x << 4)) & 0x0F0F0F0F;
+    x = (x | (x << 2)) & 0x33333333;
+    x = (x | (x << 1)) & 0x55555555;
+    return x;
+  };
+  return spread(q(u)) | (spread(q(v)) << 1);
+}
+
+// Bounding cone of a set of directions: axis and half-angle (pi = no bound).
+struct Cone {
+  float x, y, z, theta;
+};
+
+// Cone around the normalised mean direction holding all `dirs`, with the
+// half-angle rounded up. An empty set or a zero mean gives a full sphere
+// (half-angle pi), which never permits a skip.
+Cone bounding_cone(const std::vector<lc::Vec3>& dirs) {
+  Cone c{};
+  lc::Vec3 m(0, 0, 0);
+  for (const lc::Vec3& d : dirs) m = m + d;
+  double len = m.length();
+  if (dirs.empty() || !(len > 0.0)) {
+    c.theta = static_cast<float>(lc::PI);
+    return c;
+  }
+  m = (1.0 / len) * m;
+  double th = 0.0;
+  for (const lc::Vec3& d : dirs) {
+    double dl = d.length();
+    if (!(dl > 0.0)) continue;
+    double ca = lc::clampd(lc::dot(m, d) / dl, -1.0, 1.0);
+    th = std::max(th, std::acos(ca));
+  }
+  th = std::min(lc::PI, th * (1.0 + 1e-6) + 1e-6);
+  c.x = static_cast<float>(m.x);
+  c.y = static_cast<float>(m.y);
+  c.z = static_cast<float>(m.z);
+  // Rounded up to FP32.
+  c.theta = std::nextafter(static_cast<float>(th), 4.0f);
+  return c;
+}
 
 void push_lane(SetPack& S, double phi, uint32_t cls, const lc::Ginterp& gi) {
   S.lane_phi.push_back(phi);
@@ -144,6 +210,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
 
   // Records, one segment per (grid, eclipse count).
   uint64_t off = 0;
+  std::vector<Cone> cones;  // normal cone per patch
   for (const GridSpec& g : grids) {
     const std::vector<lc::Point>& pts = *g.pts;
     if (g.id < 7) H.grids[g.id].n = static_cast<uint32_t>(pts.size());
@@ -158,21 +225,39 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
       for (uint32_t e = 0; e < pts.size(); ++e)
         if (pts[e].eclipse.size() == k) idx.push_back(e);
       if (idx.empty()) continue;
+      // Spatial order: consecutive 64-element patches then have compact
+      // normal cones, which lets the kernel skip back-facing patches.
+      std::vector<uint64_t> key(pts.size());
+      for (uint32_t e : idx) key[e] = direction_key(pts[e].dirn);
+      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+      const uint32_t n_real = static_cast<uint32_t>(idx.size());
+      const uint32_t n_pad = (n_real + PATCH - 1) / PATCH * PATCH;
       const uint32_t stride = rec_stride(k, g.posn);
       const uint32_t eoff = g.posn ? 32u : 16u;
       off = (off + 15) / 16 * 16;
       SegDev sg{};
       sg.rec_off = off;
-      sg.n = static_cast<uint32_t>(idx.size());
+      sg.n = n_pad;
       sg.use_mask = g.use;
       sg.kind = g.kind;
       sg.necl = k;
       sg.flags = g.posn ? SEG_POSN : 0u;
-      S.recs.resize(off + static_cast<uint64_t>(stride) * idx.size());
+      sg.patch_off = static_cast<uint32_t>(cones.size());
+      S.recs.resize(off + static_cast<uint64_t>(stride) * n_pad);
       uint8_t* base = S.recs.data() + off;
-      for (size_t i = 0; i < idx.size(); ++i) {
+      for (uint32_t i = 0; i < n_pad; ++i) {
+        uint8_t* r = base + static_cast<size_t>(i) * stride;
+        if (i >= n_real) {
+          // Padding: zero normal and flux contribute exactly 0; its
+          // intervals can never contain a phase in [0, 1).
+          std::memset(r, 0, eoff);
+          for (uint32_t j = 0; j < k; ++j) {
+            const double e3[3] = {2.0, -1.0, -2.0};
+            std::memcpy(r + eoff + 24 * j, e3, 24);
+          }
+          continue;
+        }
         const lc::Point& pt = pts[idx[i]];
-        uint8_t* r = base + i * stride;
         const float a[4] = {static_cast<float>(pt.dirn.x), static_cast<float>(pt.dirn.y),
                             static_cast<float>(pt.dirn.z), pt.flux};
         std::memcpy(r, a, 16);
@@ -186,10 +271,18 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           std::memcpy(r + eoff + 24 * j, e3, 24);
         }
       }
-      off += static_cast<uint64_t>(stride) * idx.size();
+      for (uint32_t p0 = 0; p0 < n_pad; p0 += PATCH) {
+        std::vector<lc::Vec3> dirs;
+        for (uint32_t i = p0; i < std::min(p0 + PATCH, n_real); ++i) dirs.push_back(pts[idx[i]].dirn);
+        cones.push_back(bounding_cone(dirs));
+      }
+      off += static_cast<uint64_t>(stride) * n_pad;
       S.segs.push_back(sg);
       S.seg_grid.push_back(g.id);
-      if (keep_maps) S.seg_orig.push_back(std::move(idx));
+      if (keep_maps) {
+        idx.resize(n_pad, UINT32_MAX);
+        S.seg_orig.push_back(std::move(idx));
+      }
     }
   }
 
@@ -229,6 +322,106 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   }
   d.n_lanes = static_cast<uint32_t>(S.lane_phi.size());
   d.n_times = static_cast<uint32_t>(nt);
+
+  // Per (patch, tile) work codes. For each tile: the cone of its valid
+  // lanes' earth vectors (the vectors the kernel forms: comp_light lanes use
+  // the TWOPI*iangle/360 inclination, the single-star lanes
+  // iangle.to_radians()) and the range of their visibility phases.
+  const uint32_t ntile = d.n_lanes / TILE;
+  struct TileInfo {
+    bool any = false;
+    double lo = 0.0, hi = 0.0;  // min / max of phi - floor(phi)
+    lc::Vec3 axis;
+    double theta = lc::PI;
+  };
+  std::vector<TileInfo> tiles(ntile);
+  for (uint32_t t = 0; t < ntile; ++t) {
+    TileInfo& ti = tiles[t];
+    std::vector<lc::Vec3> es;
+    for (uint32_t l = t * TILE; l < (t + 1) * TILE; ++l) {
+      const uint32_t m = S.lane_meta[l];
+      if (!(m & META_VALID)) continue;
+      const bool lcl = meta_class(m) == CLS_NORMAL;
+      const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
+      es.push_back(lc::Vec3(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z)));
+      const double phif = S.lane_phi[l] - std::floor(S.lane_phi[l]);
+      ti.lo = ti.any ? std::min(ti.lo, phif) : phif;
+      ti.hi = ti.any ? std::max(ti.hi, phif) : phif;
+      ti.any = true;
+    }
+    const Cone c = bounding_cone(es);
+    ti.axis = lc::Vec3(c.x, c.y, c.z);
+    const double len = ti.axis.length();
+    if (len > 0.0) ti.axis = (1.0 / len) * ti.axis;
+    ti.theta = c.theta;
+  }
+  S.n_tiles = ntile;
+  S.ptcode.assign(cones.size() * ntile, PT_FULL);
+  for (size_t g = 0; g < S.segs.size(); ++g) {
+    const SegDev& sg = S.segs[g];
+    const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
+    const uint32_t eoff = (sg.flags & SEG_POSN) ? 32u : 16u;
+    // An element adds nothing unless mu > thr: the star's mucrit (mu <= 0
+    // also gives 0 through imu unless beaming evaluates imu at another mu),
+    // or 0 for disc, edge and spot (flux.rs:36-106).
+    double thr = 0.0;
+    if (sg.kind <= KIND_STAR2) {
+      const bool s1 = sg.kind == KIND_STAR1;
+      const double mc = s1 ? P.ldc1.mucrit : P.ldc2.mucrit;
+      thr = (s1 ? beam1 : beam2) ? mc : std::max(mc, 0.0);
+    }
+    const bool cull = std::fabs(thr) < 0.99;
+    const double beta = cull ? std::acos(thr) : lc::PI;
+    for (uint32_t q = 0; q < sg.n / PATCH; ++q) {
+      const Cone& pc = cones[sg.patch_off + q];
+      lc::Vec3 pa(pc.x, pc.y, pc.z);
+      const double plen = pa.length();
+      if (plen > 0.0) pa = (1.0 / plen) * pa;
+      const uint8_t* pr = S.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
+      for (uint32_t t = 0; t < ntile; ++t) {
+        const TileInfo& ti = tiles[t];
+        uint8_t& code = S.ptcode[static_cast<size_t>(sg.patch_off + q) * ntile + t];
+        if (!ti.any) {
+          code = PT_SKIP;
+          continue;
+        }
+        // Back-facing: every normal of the patch is more than beta + margin
+        // from every earth vector of the tile, so every FP32 mu <= thr. The
+        // margin (2e-3 rad) covers FP32 rounding of mu and of both cones.
+        const double gam = beta + ti.theta + pc.theta + kConeMargin;
+        if (cull && plen > 0.0 && gam < lc::PI - 1e-3 && lc::dot(ti.axis, pa) < std::cos(gam)) {
+          code = PT_SKIP;
+          continue;
+        }
+        if (sg.necl == 0) {
+          code = PT_FREE;
+          continue;
+        }
+        // Point::visible (types.rs:50-58) over the tile's phase range
+        // [lo, hi], with the same FP64 comparisons: no lane eclipsed when
+        // every interval misses the range; every lane eclipsed when some
+        // interval holds it.
+        bool all_vis = true, all_ecl = true;
+        for (uint32_t i = 0; i < PATCH; ++i) {
+          const uint8_t* r = pr + static_cast<size_t>(i) * stride;
+          float w;
+          std::memcpy(&w, r + 12, sizeof w);
+          if (w == 0.0f) continue;  // zero flux (padding) adds 0 either way
+          const uint8_t* e = r + eoff;
+          bool vis = true, ecl = false;
+          for (uint32_t k = 0; k < sg.necl; ++k) {
+            double e3[3];
+            std::memcpy(e3, e + 24 * k, sizeof e3);
+            if (!((ti.hi < e3[0] || ti.lo > e3[1]) && ti.lo > e3[2])) vis = false;
+            if ((ti.lo >= e3[0] && ti.hi <= e3[1]) || ti.hi <= e3[2]) ecl = true;
+          }
+          all_vis = all_vis && vis;
+          all_ecl = all_ecl && ecl;
+        }
+        code = all_ecl ? PT_SKIP : (all_vis ? PT_FREE : PT_FULL);
+      }
+    }
+  }
 
   // Element-visits in the reference's loop order (flux.rs:24-107 and the
   // single-star calls of orchestration.rs), used for throughput figures.

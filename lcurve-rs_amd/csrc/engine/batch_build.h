@@ -37,6 +37,10 @@ struct SetPack {
   std::vector<SegDev> segs;
   std::vector<uint32_t> seg_grid;                 // grid id per segment (>= 7: calibration copy)
   std::vector<std::vector<uint32_t>> seg_orig;    // optional element maps
+  // Work code per (patch, tile): ptcode[patch * n_tiles + tile], patches
+  // numbered in segment order (SegDev::patch_off).
+  std::vector<uint8_t> ptcode;
+  uint32_t n_tiles = 0;
   std::vector<double> lane_phi;
   std::vector<uint32_t> lane_meta;
   std::vector<uint32_t> t_lane0;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <climits>
 #include <cstring>
 #include <thread>
 
@@ -63,6 +64,8 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
   const size_t ns = packs.size();
   host_.resize(ns);
   sets_.resize(ns);
+  pt_base_.assign(ns, 0);
+  pt_tiles_.assign(ns, 0);
 
   // Global offsets.
   uint64_t rec_total = 0;
@@ -97,6 +100,10 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
     sets_[s] = sd;
     host_[s] = P.host;
     SetHost& H = host_[s];
+    if ((sd.flags & SET_VALID) && P.n_tiles != P.lane_phi.size() / TILE) throw DeviceError("patch codes out of step");
+    pt_base_[s] = ptcode_.size();
+    pt_tiles_[s] = P.n_tiles;
+    ptcode_.insert(ptcode_.end(), P.ptcode.begin(), P.ptcode.end());
     for (size_t g = 0; g < P.segs.size(); ++g) {
       SegDev sg = P.segs[g];
       sg.rec_off += rec_base[s];
@@ -212,7 +219,9 @@ void Batch::release_graph() {
 }
 
 void Batch::build_items(uint32_t chunk) {
-  if (chunk == 0) chunk = 1;
+  // Items cover whole element patches, at most 64 of them.
+  chunk = std::max<uint32_t>(PATCH, (chunk + PATCH - 1) / PATCH * PATCH);
+  chunk = std::min<uint32_t>(chunk, MAX_ITEM_PATCHES * PATCH);
   release_graph();
   chunk_ = chunk;
   chunk_info_.clear();
@@ -251,6 +260,15 @@ void Batch::build_items(uint32_t chunk) {
             it.ne = std::min(chunk, sg.n - k * chunk);
             it.tile = t;
             it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
+            it.todo = it.free = 0;
+            const uint32_t np = it.ne / PATCH;
+            for (uint32_t q = 0; q < np; ++q) {
+              const uint64_t ci = pt_base_[s] + static_cast<uint64_t>(sg.patch_off + it.e0 / PATCH + q) * pt_tiles_[s] + t;
+              if (ci >= ptcode_.size()) throw DeviceError("patch code out of range");
+              const uint8_t code = ptcode_[ci];
+              if (code != PT_SKIP) it.todo |= 1ull << q;
+              if (code == PT_FREE) it.free |= 1ull << q;
+            }
             items.push_back(it);
           }
         }
@@ -272,6 +290,10 @@ void Batch::build_items(uint32_t chunk) {
     if (sg.rec_off + (static_cast<uint64_t>(it.e0) + it.ne) * stride > rec_bytes_)
       throw DeviceError("item records out of range");
     if (sg.necl > static_cast<uint32_t>(MAX_ECL)) throw DeviceError("segment eclipse count out of range");
+    if (it.e0 % PATCH || it.ne % PATCH || it.ne / PATCH > MAX_ITEM_PATCHES)
+      throw DeviceError("item not aligned to patches");
+    const uint64_t live = it.ne / PATCH == 64 ? ~0ull : (1ull << (it.ne / PATCH)) - 1ull;
+    if ((it.todo & ~live) || (it.free & ~it.todo)) throw DeviceError("item patch masks out of range");
   }
   if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
 
@@ -410,7 +432,9 @@ void Batch::visibility(int set, int grid, const double* phases, int nphase, uint
     (void)hipFree(dm);
     const std::vector<uint32_t>& orig = gm.orig[k];
     for (int p = 0; p < nphase; ++p)
-      for (uint32_t e = 0; e < sg.n; ++e) mask[static_cast<size_t>(p) * gm.n + orig[e]] = hm[static_cast<size_t>(p) * sg.n + e];
+      for (uint32_t e = 0; e < sg.n; ++e)
+        if (orig[e] != UINT32_MAX)  // padding records have no grid element
+          mask[static_cast<size_t>(p) * gm.n + orig[e]] = hm[static_cast<size_t>(p) * sg.n + e];
   }
   (void)hipFree(dph);
 }

@@ -98,6 +98,10 @@ class Batch {
   std::vector<int> t_ndiv_;
   std::vector<double> t_slfac_, t_y_, t_ferr_, t_w_;
   std::vector<uint32_t> chunk_info_;
+  // Per-set patch codes (ptcode of SetPack), kept for rebuilding items.
+  std::vector<uint8_t> ptcode_;
+  std::vector<uint64_t> pt_base_;
+  std::vector<uint32_t> pt_tiles_;
   uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0;
   uint32_t chunk_ = 0;
 

@@ -1,14 +1,19 @@
 // HIP kernels of the flux engine for CDNA4 (gfx950).
 //
 // flux_items_kernel — the hot path. One wavefront = one work item =
-//   64 sub-phases (one per lane) x one chunk of surface elements. Each lane
-//   owns one sub-phase and loops over the chunk; the element record address
-//   is wave-uniform, so records arrive through scalar loads and feed the
-//   VALU as scalar operands. There is no cross-lane reduction in the inner
-//   loop. Per element and lane: FP64 eclipse-interval test (bit-exact with
+//   64 sub-phases (one per lane) x one chunk of surface elements, stored as
+//   patches of 64 elements. Each lane owns one sub-phase and loops over the
+//   item's patches; the element record address is wave-uniform, so records
+//   arrive through scalar loads and feed the VALU as scalar operands. There
+//   is no cross-lane reduction in the inner loop. The item carries two
+//   64-bit masks worked out exactly on the host: patches that add zero to
+//   every lane of the tile (back-facing or fully eclipsed) are not visited,
+//   and patches that no lane of the tile sees eclipsed skip the eclipse
+//   test. Per element and lane: FP64 eclipse-interval test (bit-exact with
 //   the reference's Point::visible, types.rs:50-58), FP32 mu, limb
-//   darkening, beaming and lensing, Kahan-compensated FP32 accumulation.
-//   The lane's chunk sum is written as an FP64 partial.
+//   darkening, beaming and lensing. Summation is cascaded: four FP32
+//   accumulators over a patch (16 terms each), FP64 across patches; the
+//   lane's item sum is written as an FP64 partial.
 // assemble_kernel — one workgroup per set, all FP64: sums the partials per
 //   sub-phase and component, derives the Ginterp scale factors from the
 //   calibration lanes, applies the scale factors, smears exposures, adds the
@@ -29,6 +34,10 @@ namespace {
 
 constexpr double K_TWOPI = 6.283185307179586476925286766559;
 
+// Limb-darkening forms: LD_POLY / LD_CLARET are LDC::imu (types.rs:101-119);
+// LD_LIN is the Poly law with only the linear coefficient set.
+enum : int { LD_POLY = 0, LD_CLARET = 1, LD_LIN = 2 };
+
 // ------------------------------------------------------------ helpers
 struct LaneCtx {
   double phif;          // sub-phase minus its floor: the visibility phase
@@ -37,7 +46,9 @@ struct LaneCtx {
 };
 
 struct StarPar {
-  float a0, a1, a2, a3, csum, mucrit, beam, rlens;
+  float a0, a1, a2, a3;
+  float c0;      // LD_LIN: 1 - a0; LD_CLARET: 1 - (a0 + a1 + a2 + a3)
+  float mucrit, beam, rlens;
 };
 
 // Point::visible for NECL intervals (types.rs:50-58), in FP64 so the mask is
@@ -54,26 +65,34 @@ __device__ __forceinline__ bool eclipse_free(const double* __restrict__ e, doubl
   return !ecl;
 }
 
-// LDC::imu (types.rs:101-119) in FP32.
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+// LDC::imu (types.rs:101-119) in FP32, including its mu <= 0 test.
 template <int LD>
 __device__ __forceinline__ float imu(float mu, const StarPar& P) {
   float m = fminf(mu, 1.0f);
   float im;
-  if (LD == 0) {
+  if (LD == LD_CLARET) {
+    float s = sqrtf(m);
+    im = P.c0 + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
+  } else {
     float o = 1.0f - m;
     im = 1.0f - o * (P.a0 + o * (P.a1 + o * (P.a2 + o * P.a3)));
-  } else {
-    float s = sqrtf(m);
-    im = (1.0f - P.csum) + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
   }
   return mu > 0.0f ? im : 0.0f;
 }
 
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-  float y = x - c;
-  float t = s + y;
-  c = (t - s) - y;
-  s = t;
+// LDC::imu for m = clamp(mu, 0, 1), for use where m == 0 already zeroes the
+// term (the element's mu factor is m too).
+template <int LD>
+__device__ __forceinline__ float imu_clamped(float m, const StarPar& P) {
+  if (LD == LD_LIN) return fmaf(P.a0, m, P.c0);
+  if (LD == LD_CLARET) {
+    float s = sqrtf(m);
+    return P.c0 + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
+  }
+  float o = 1.0f - m;
+  return 1.0f - o * (P.a0 + o * (P.a1 + o * (P.a2 + o * P.a3)));
 }
 
 // lensing_mag (flux.rs:390-406) in FP32.
@@ -90,94 +109,184 @@ __device__ __forceinline__ float lensing_mag(float px, float py, float pz, const
   return pd * pd / (pd - ph) / ph * 0.25f;
 }
 
-// Star elements (flux.rs:36-80 / 115-200).
-template <int NECL, int LD, bool BEAM, bool LENS>
-__device__ __forceinline__ void star_chunk(const char* __restrict__ p, uint32_t ne,
-                                           const LaneCtx& L, const StarPar& P, float& s, float& c) {
-  constexpr bool POSN = BEAM || LENS;
-  constexpr uint32_t STRIDE = rec_stride(NECL, POSN);
-  constexpr uint32_t EOFF = POSN ? 32u : 16u;
-  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
-    const float4 d = *reinterpret_cast<const float4*>(p);
-    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + EOFF), L.phif);
-    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
-    float term;
-    if (POSN) {
-      const float4 q = *reinterpret_cast<const float4*>(p + 16);
-      float mag = LENS ? lensing_mag(q.x, q.y, q.z, L, P.rlens) : 1.0f;
+__device__ __forceinline__ float mu_of(const float4& d, const LaneCtx& L) {
+  return fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
+}
+
+// ---------------------------------------------------- one patch of 64
+// Star elements without beaming or lensing (flux.rs:36-80 / 115-200):
+// mu * flux * imu(mu) where mu > mucrit and the element is visible. With
+// m = clamp(mu, 0, 1) the term m * flux * imu(m) equals the reference's for
+// 0 < mu <= 1 and is 0 for mu <= 0, so the mucrit test is only needed for
+// mucrit > 0 (MCRIT) and the select only with it or the eclipse test.
+template <int NECL, bool FREE, int LD, bool MCRIT>
+__device__ __forceinline__ float star_patch(const char* __restrict__ p, const LaneCtx& L,
+                                            const StarPar& P) {
+  constexpr uint32_t STRIDE = rec_stride(NECL, false);
+  constexpr bool ECL = NECL > 0 && !FREE;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (uint32_t i = 0; i < PATCH; i += 4) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const char* r = p + (i + j) * STRIDE;
+      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float mu = mu_of(d, L);
+      const float m = clamp01(mu);
+      float wm = m * d.w;
+      if (ECL || MCRIT) {
+        bool ok = true;
+        if (ECL) ok = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif);
+        if (MCRIT) ok = ok & (mu > P.mucrit);
+        wm = ok ? wm : 0.0f;
+      }
+      acc[j] = fmaf(wm, imu_clamped<LD>(m, P), acc[j]);
+    }
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+// Star elements with beaming and / or lensing (FX bit 0 beaming, bit 1
+// lensing): the reference's expression as written, imu with its own test.
+template <int NECL, bool FREE, int LD, int FX>
+__device__ __forceinline__ float star_patch_fx(const char* __restrict__ p, const LaneCtx& L,
+                                               const StarPar& P) {
+  constexpr bool BEAM = (FX & 1) != 0, LENS = (FX & 2) != 0;
+  constexpr uint32_t STRIDE = rec_stride(NECL, true);
+  constexpr bool ECL = NECL > 0 && !FREE;
+  float acc[2] = {0.0f, 0.0f};
+  for (uint32_t i = 0; i < PATCH; i += 2) {
+#pragma unroll
+    for (uint32_t j = 0; j < 2; ++j) {
+      const char* r = p + (i + j) * STRIDE;
+      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float4 q = *reinterpret_cast<const float4*>(r + 16);
+      const float mu = mu_of(d, L);
+      const float mag = LENS ? lensing_mag(q.x, q.y, q.z, L, P.rlens) : 1.0f;
+      float term;
       if (BEAM) {
-        float vx = L.kx * q.y;
-        float vy = fmaf(L.ky, q.x, L.k0);
-        float vr = -(L.ex * vx + L.ey * vy);
-        float vn = d.x * vx + d.y * vy;
-        float mud = mu - mu * vr - vn;
+        const float vx = L.kx * q.y;
+        const float vy = fmaf(L.ky, q.x, L.k0);
+        const float vr = -(L.ex * vx + L.ey * vy);
+        const float vn = d.x * vx + d.y * vy;
+        const float mud = mu - mu * vr - vn;
         term = mu * mag * d.w * (1.0f - P.beam * vr) * imu<LD>(mud, P);
       } else {
         term = mu * mag * d.w * imu<LD>(mu, P);
       }
-    } else {
-      term = mu * d.w * imu<LD>(mu, P);
+      bool ok = mu > P.mucrit;
+      if (ECL) ok = ok & eclipse_free<NECL>(reinterpret_cast<const double*>(r + 32), L.phif);
+      acc[j] += ok ? term : 0.0f;
     }
-    term = (vis & (mu > P.mucrit)) ? term : 0.0f;
-    kahan_add(s, c, term);
   }
+  return acc[0] + acc[1];
 }
 
-// Disc and disc-edge elements (flux.rs:83-100).
-template <int NECL>
-__device__ __forceinline__ void disc_chunk(const char* __restrict__ p, uint32_t ne, const LaneCtx& L,
-                                           float lin, float quad, float& s, float& c) {
+// Disc and disc-edge elements (flux.rs:83-100): mu > 0 and visible.
+template <int NECL, bool FREE>
+__device__ __forceinline__ float disc_patch(const char* __restrict__ p, const LaneCtx& L, float lin,
+                                            float quad) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
-  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
-    const float4 d = *reinterpret_cast<const float4*>(p);
-    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + 16), L.phif);
-    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
-    float ommu = 1.0f - mu;
-    float term = mu * d.w * (1.0f - ommu * (lin + quad * ommu));
-    term = (vis & (mu > 0.0f)) ? term : 0.0f;
-    kahan_add(s, c, term);
+  constexpr bool ECL = NECL > 0 && !FREE;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (uint32_t i = 0; i < PATCH; i += 4) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const char* r = p + (i + j) * STRIDE;
+      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float m = clamp01(mu_of(d, L));
+      float wm = m * d.w;
+      if (ECL) wm = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? wm : 0.0f;
+      const float o = 1.0f - m;
+      acc[j] = fmaf(wm, 1.0f - o * (lin + quad * o), acc[j]);
+    }
   }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
-// Bright-spot elements (flux.rs:102-106).
-template <int NECL>
-__device__ __forceinline__ void spot_chunk(const char* __restrict__ p, uint32_t ne, const LaneCtx& L,
-                                           float& s, float& c) {
+// Bright-spot elements (flux.rs:102-106): mu * flux where mu > 0 and visible.
+template <int NECL, bool FREE>
+__device__ __forceinline__ float spot_patch(const char* __restrict__ p, const LaneCtx& L) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
-  for (uint32_t i = 0; i < ne; ++i, p += STRIDE) {
-    const float4 d = *reinterpret_cast<const float4*>(p);
-    const bool vis = eclipse_free<NECL>(reinterpret_cast<const double*>(p + 16), L.phif);
-    float mu = fmaf(L.ez, d.z, fmaf(L.ey, d.y, L.ex * d.x));
-    float term = (vis & (mu > 0.0f)) ? mu * d.w : 0.0f;
-    kahan_add(s, c, term);
+  constexpr bool ECL = NECL > 0 && !FREE;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (uint32_t i = 0; i < PATCH; i += 4) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const char* r = p + (i + j) * STRIDE;
+      const float4 d = *reinterpret_cast<const float4*>(r);
+      float w = d.w;
+      if (ECL) w = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? w : 0.0f;
+      acc[j] = fmaf(clamp01(mu_of(d, L)), w, acc[j]);
+    }
   }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
-template <int NECL, int LD>
-__device__ __forceinline__ void star_dispatch_fx(uint32_t fx, const char* p, uint32_t ne,
-                                                 const LaneCtx& L, const StarPar& P, float& s,
-                                                 float& c) {
-  switch (fx) {
-    case 0: star_chunk<NECL, LD, false, false>(p, ne, L, P, s, c); break;
-    case 1: star_chunk<NECL, LD, true, false>(p, ne, L, P, s, c); break;
-    case 2: star_chunk<NECL, LD, false, true>(p, ne, L, P, s, c); break;
-    default: star_chunk<NECL, LD, true, true>(p, ne, L, P, s, c); break;
+// ----------------------------------------------------- one work item
+// Visits the patches in `todo` (wave-uniform), choosing the patch body by
+// the `fr` bit; each patch's FP32 sum is added in FP64.
+enum : int { PK_STAR = 0, PK_STARFX = 1, PK_DISC = 2, PK_SPOT = 3 };
+
+template <int PK, int NECL, int LD, int X>
+__device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t todo, uint64_t fr,
+                                           const LaneCtx& L, const StarPar& P, float lin,
+                                           float quad) {
+  constexpr uint32_t PB = PATCH * rec_stride(NECL, PK == PK_STARFX);
+  double acc = 0.0;
+  while (todo) {
+    const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(todo));
+    todo &= todo - 1ull;
+    const char* pp = p + static_cast<size_t>(q) * PB;
+    const bool free = NECL > 0 && ((fr >> q) & 1ull);
+    float s;
+    if (PK == PK_STAR) {
+      s = free ? star_patch<NECL, true, LD, X != 0>(pp, L, P) : star_patch<NECL, false, LD, X != 0>(pp, L, P);
+    } else if (PK == PK_STARFX) {
+      s = free ? star_patch_fx<NECL, true, LD, X>(pp, L, P) : star_patch_fx<NECL, false, LD, X>(pp, L, P);
+    } else if (PK == PK_DISC) {
+      s = free ? disc_patch<NECL, true>(pp, L, lin, quad) : disc_patch<NECL, false>(pp, L, lin, quad);
+    } else {
+      s = free ? spot_patch<NECL, true>(pp, L) : spot_patch<NECL, false>(pp, L);
+    }
+    acc += static_cast<double>(s);
   }
+  return acc;
 }
 
 template <int NECL>
-__device__ __forceinline__ void chunk_dispatch(uint32_t kind, uint32_t ld, uint32_t fx,
-                                               const char* p, uint32_t ne, const LaneCtx& L,
-                                               const StarPar& P, float lin, float quad, float& s,
-                                               float& c) {
+__device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t fx, bool mcrit,
+                                                const char* p, uint64_t todo, uint64_t fr,
+                                                const LaneCtx& L, const StarPar& P, float lin,
+                                                float quad) {
   if (kind <= KIND_STAR2) {
-    if (ld == 0) star_dispatch_fx<NECL, 0>(fx, p, ne, L, P, s, c);
-    else star_dispatch_fx<NECL, 1>(fx, p, ne, L, P, s, c);
-  } else if (kind == KIND_SPOT) {
-    spot_chunk<NECL>(p, ne, L, s, c);
-  } else {
-    disc_chunk<NECL>(p, ne, L, lin, quad, s, c);
+    if (fx == 0) {
+      if (ld == LD_LIN)
+        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad)
+                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad);
+      if (ld == LD_CLARET)
+        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad)
+                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad);
+      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad)
+                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad);
+    }
+    const bool cl = ld == LD_CLARET;
+    switch (fx) {
+      case 1:
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad);
+      case 2:
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad);
+      default:
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad);
+    }
   }
+  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
+  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
 }
 
 // Workgroup b of a launch lands on XCD b % 8. Give every XCD a contiguous
@@ -219,11 +328,12 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   P.a1 = star1 ? sp.ldc1[1] : sp.ldc2[1];
   P.a2 = star1 ? sp.ldc1[2] : sp.ldc2[2];
   P.a3 = star1 ? sp.ldc1[3] : sp.ldc2[3];
-  P.csum = P.a0 + P.a1 + P.a2 + P.a3;
   P.mucrit = star1 ? sp.mucrit1 : sp.mucrit2;
   P.beam = star1 ? sp.beam1 : sp.beam2;
   P.rlens = sp.rlens1;
-  const uint32_t ld = star1 ? sp.ltype1 : sp.ltype2;
+  int ld = (star1 ? sp.ltype1 : sp.ltype2) == 1u ? LD_CLARET : LD_POLY;
+  if (ld == LD_POLY && P.a1 == 0.0f && P.a2 == 0.0f && P.a3 == 0.0f) ld = LD_LIN;
+  P.c0 = ld == LD_LIN ? 1.0f - P.a0 : 1.0f - (P.a0 + P.a1 + P.a2 + P.a3);
   uint32_t fx = 0;
   if (sg.kind <= KIND_STAR2) {
     const bool beam = (sp.flags & (star1 ? SET_BEAM1 : SET_BEAM2)) != 0;
@@ -245,18 +355,19 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   } else {
     L.kx = L.ky = L.k0 = 0.0f;
   }
+  const bool mcrit = P.mucrit > 0.0f;
 
-  float s = 0.0f, c = 0.0f;
   const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
   const char* p = recs + sg.rec_off + static_cast<uint64_t>(it.e0) * stride;
+  const float lin = sp.lin_disc, quad = sp.quad_disc;
+  double sum;
   switch (sg.necl) {
-    case 0: chunk_dispatch<0>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
-    case 1: chunk_dispatch<1>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
-    case 2: chunk_dispatch<2>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
-    case 3: chunk_dispatch<3>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
-    default: chunk_dispatch<4>(sg.kind, ld, fx, p, it.ne, L, P, sp.lin_disc, sp.quad_disc, s, c); break;
+    case 0: sum = item_dispatch<0>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 1: sum = item_dispatch<1>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 2: sum = item_dispatch<2>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 3: sum = item_dispatch<3>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    default: sum = item_dispatch<4>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
   }
-  const double sum = static_cast<double>(s) - static_cast<double>(c);
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
 

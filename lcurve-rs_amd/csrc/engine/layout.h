@@ -47,22 +47,38 @@ __host__ __device__ constexpr uint32_t rec_stride(uint32_t necl, bool posn) {
   return 16u + (posn ? 16u : 0u) + 24u * necl;
 }
 
+// Elements of a segment are stored in patches of PATCH (padded with
+// zero-flux records), in an order that keeps each patch's normals close
+// together. For every (patch, lane tile) pair the host decides, exactly,
+// whether the patch must be evaluated with the eclipse test (PT_FULL),
+// needs no eclipse test because no lane of the tile is eclipsed by any
+// element of it (PT_FREE), or adds exactly zero to every lane of the tile
+// (PT_SKIP: all normals face away from all the tile's lines of sight, or
+// every element is eclipsed at every lane).
+constexpr uint32_t PATCH = 64;
+constexpr uint8_t PT_FULL = 0, PT_FREE = 1, PT_SKIP = 2;
+// At most 64 patches per work item: the item's patch codes fit 64-bit masks.
+constexpr uint32_t MAX_ITEM_PATCHES = 64;
+
 struct SegDev {
   uint64_t rec_off;   // byte offset of the first record
-  uint32_t n;         // number of elements
+  uint32_t n;         // number of elements, padded to a multiple of PATCH
   uint32_t set;       // owning set
   uint32_t use_mask;  // cp_bit()s of the lanes that add this segment
   uint32_t kind;      // SegKind
   uint32_t necl;      // eclipse pairs per element
   uint32_t flags;     // SEG_POSN
+  uint32_t patch_off; // first patch, numbered within the set (host only)
 };
 
 struct ItemDev {
   uint32_t seg;   // segment index (global)
-  uint32_t e0;    // first element
-  uint32_t ne;    // element count
+  uint32_t e0;    // first element (multiple of PATCH)
+  uint32_t ne;    // element count (multiple of PATCH)
   uint32_t tile;  // tile within the set
   uint64_t prow;  // partial row index
+  uint64_t todo;  // bit q: patch e0 / PATCH + q is evaluated (not PT_SKIP)
+  uint64_t free;  // bit q: that patch needs no eclipse test (PT_FREE)
 };
 
 // Set flags.
