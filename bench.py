@@ -139,6 +139,21 @@ def cpu_baseline(info, budget_s):
             "setup_core_seconds_per_set": round(per_set_setup, 4), "wall_s": round(dt, 3)}
 
 
+def copy_calibration(mib):
+    """A streaming kernel of known traffic (torch.neg over float32: reads and
+    writes `mib` MiB each), run under `rocprofv3 --pmc FETCH_SIZE` /
+    `WRITE_SIZE` to scale those counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "torch.neg float32", "read_bytes": n * 4,
+                                           "write_bytes": n * 4, "launches": 5}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,9 +163,16 @@ def main():
                     choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k"])
     ap.add_argument("--nsets", type=int, default=4096, help="sets of chisq_4096 (4096 is the workload)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="fix elements per work item (skips autotuning; used for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--parity-sets", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a streaming kernel of known traffic (for PMC scaling)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -159,10 +181,12 @@ def main():
     if not lc.has_hip():
         raise SystemExit("no HIP device: the engine has no CPU path")
     lc.set_device("hip:0")
+    if args.chunk > 0:
+        lc.set_tuning(args.chunk, 0)
 
     batch, info = build_workload(args.workload, args.nsets)
     scale = info["scale"]
-    if not args.no_autotune:
+    if not args.no_autotune and args.chunk <= 0:
         log("autotuning chunk size / launch form ...")
         batch.autotune(scale=scale)
     st = batch.stats()
@@ -198,6 +222,10 @@ def main():
             "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None,
             "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
             "algorithmic_flop_per_visit": FLOP_PER_VISIT, "visits_per_launch": visits,
+            # (element, lane) pairs the kernel actually evaluates; the rest are
+            # skipped because they add exactly zero.
+            "pairs_evaluated_per_launch": st["pairs_evaluated"],
+            "evaluated_pairs_per_s": round(st["pairs_evaluated"] / (ms_flux * 1e-3), 1),
             "hbm_algorithmic_gbs": round(alg_bytes / (ms_flux * 1e-3) / 1e9, 2),
             "hbm_frac": round(alg_bytes / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
 
@@ -205,7 +233,7 @@ def main():
            else "light curves/sec (1 GPU)",
            "value": round(value, 3), "unit": "chi2/s" if args.workload == "chisq_4096" else "curves/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+kahan/fp64",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
            "data": "synthetic (seeded parameter sets and noise; no external data)",
            "config": info["config"],
            "element_visits_per_step": visits, "visits_per_s": round(visits * args.steps / elapsed, 1),

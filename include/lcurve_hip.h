@@ -58,6 +58,11 @@ typedef struct lch_batch_stats {
   long long n_sets, n_valid, n_times, n_lanes, n_items;
   long long visits;       /* element-visits in the reference's loop order */
   long long record_bytes; /* resident element records in HBM */
+  long long pairs_evaluated; /* (element, lane) pairs the flux kernel evaluates one
+                                by one per launch, padding included */
+  long long pairs_by_moments; /* pairs summed in closed form from patch moments
+                                 (seen whole, no eclipse); the rest of the
+                                 reference's pairs are skipped as exactly zero */
   double setup_seconds;   /* CPU setup + packing wall time */
   double upload_seconds;  /* host-to-device copies and table builds */
   int chunk;              /* elements per work item */
@@ -154,6 +159,17 @@ int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, d
  * query nelem. Needs LCH_BATCH_KEEP_MAPS. */
 int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
                          int nphase, unsigned char* mask, int* nelem);
+
+/* ---- diagnostics (CPU only, no device needed) ---- */
+/* Packs one model as the engine does and re-evaluates, in FP64 on the CPU,
+ * every (element, lane) pair twice: as the per-(patch, tile) work codes
+ * direct (skip, closed-form patch moments, no eclipse test, full test) and
+ * by brute force. max_rel_diff = max over (lane, segment) of
+ * |coded - brute| / sum |terms|; counts[0..3] = (patch, tile) pairs coded
+ * full / no-eclipse / skip / moments (either output may be null). No
+ * counterpart in the reference: it checks this engine's own work skipping. */
+int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const double* expose,
+                              const int* ndiv, int n, double* max_rel_diff, long long* counts);
 
 #ifdef __cplusplus
 }

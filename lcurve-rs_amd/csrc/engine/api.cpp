@@ -8,6 +8,7 @@
 #include <string>
 
 #include "engine.h"
+#include "selfcheck.h"
 
 struct lch_ctx {
   lch::Context impl;
@@ -399,6 +400,8 @@ int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
   out->n_items = static_cast<long long>(B.n_items());
   out->visits = static_cast<long long>(B.visits());
   out->record_bytes = static_cast<long long>(B.record_bytes());
+  out->pairs_evaluated = static_cast<long long>(B.pairs_evaluated());
+  out->pairs_by_moments = static_cast<long long>(B.pairs_by_moments());
   out->setup_seconds = b->setup_s;
   out->upload_seconds = b->upload_s;
   out->chunk = static_cast<int>(B.chunk());
@@ -435,3 +438,17 @@ int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const do
 }
 
 }  // extern "C"
+
+int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const double* expose,
+                              const int* ndiv, int n, double* max_rel_diff, long long* counts) {
+  if (!m) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const std::vector<double> zero(static_cast<size_t>(std::max(n, 0)), 0.0), one(zero.size(), 1.0);
+    const lc::Data data = make_data(time, expose, zero.data(), one.data(), one.data(), ndiv, n);
+    const lch::SelfCheck c = lch::selfcheck_model(m->impl, data);
+    if (max_rel_diff) *max_rel_diff = c.max_rel;
+    if (counts)
+      for (int i = 0; i < 4; ++i) counts[i] = c.counts[i];
+    return LCH_OK;
+  });
+}

@@ -44,7 +44,9 @@ struct GridSpec {
   bool posn;
 };
 
-// Morton code of a direction's octahedral map (16 bits per axis).
+// Position of a direction along a Hilbert curve over its octahedral map
+// (16 bits per axis). Consecutive keys are close on the sphere, so runs of
+// 64 elements have narrow normal cones.
 uint64_t direction_key(const lc::Vec3& d) {
   double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
   double s = ax + ay + az;
@@ -56,19 +58,25 @@ uint64_t direction_key(const lc::Vec3& d) {
     u = uu;
     v = vv;
   }
+  constexpr int64_t N = 1 << 16;
   auto q = [](double x) {
-    double t = (x + 1.0) * 0.5 * 65535.0;
-    return static_cast<uint64_t>(std::min(65535.0, std::max(0.0, t)));
+    double t = (x + 1.0) * 0.5 * static_cast<double>(N - 1);
+    return static_cast<int64_t>(std::min(static_cast<double>(N - 1), std::max(0.0, t)));
   };
-  auto spread = [](uint64_t x) {
-    x &= 0xFFFF;
-    x = (x | (x << 8)) & 0x00FF00FF;
-    x = (x | (x << 4)) & 0x0F0F0F0F;
-    x = (x | (x << 2)) & 0x33333333;
-    x = (x | (x << 1)) & 0x55555555;
-    return x;
-  };
-  return spread(q(u)) | (spread(q(v)) << 1);
+  int64_t x = q(u), y = q(v);
+  uint64_t key = 0;
+  for (int64_t sq = N / 2; sq > 0; sq /= 2) {
+    const int64_t rx = (x & sq) ? 1 : 0, ry = (y & sq) ? 1 : 0;
+    key += static_cast<uint64_t>(sq) * static_cast<uint64_t>(sq) * static_cast<uint64_t>((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = sq - 1 - x;
+        y = sq - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return key;
 }
 
 // Bounding cone of a set of directions: axis and half-angle (pi = no bound).
@@ -103,6 +111,51 @@ Cone bounding_cone(const std::vector<lc::Vec3>& dirs) {
   // Rounded up to FP32.
   c.theta = std::nextafter(static_cast<float>(th), 4.0f);
   return c;
+}
+
+// Degree of a segment's contribution as a polynomial in mu for elements
+// that are visible with mu above the cut-off, or 0 where it is none
+// (beaming, lensing, Claret or cubic / quartic Poly limb darkening). The
+// FP32 coefficients are the ones the kernel uses.
+int moment_degree(const SegDev& sg, const SetDev& d) {
+  if (sg.kind == KIND_SPOT) return 1;
+  if (sg.kind == KIND_DISC || sg.kind == KIND_EDGE) return 3;
+  if (sg.flags & SEG_POSN) return 0;  // beaming / lensing
+  const bool s1 = sg.kind == KIND_STAR1;
+  if ((s1 ? d.ltype1 : d.ltype2) != 0u) return 0;
+  const float* a = s1 ? d.ldc1 : d.ldc2;
+  if (a[2] != 0.0f || a[3] != 0.0f) return 0;
+  return a[1] != 0.0f ? 3 : 2;
+}
+
+// Direction moments of one patch (layout of MOM_STRIDE) from its records.
+void patch_moments(const uint8_t* pr, uint32_t stride, double* M) {
+  for (uint32_t i = 0; i < MOM_STRIDE; ++i) M[i] = 0.0;
+  for (uint32_t i = 0; i < PATCH; ++i) {
+    float f[4];
+    std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
+    const double x = f[0], y = f[1], z = f[2], w = f[3];
+    if (w == 0.0) continue;
+    M[0] += w * x;
+    M[1] += w * y;
+    M[2] += w * z;
+    M[3] += w * x * x;
+    M[4] += w * y * y;
+    M[5] += w * z * z;
+    M[6] += 2.0 * w * x * y;
+    M[7] += 2.0 * w * x * z;
+    M[8] += 2.0 * w * y * z;
+    M[9] += w * x * x * x;
+    M[10] += w * y * y * y;
+    M[11] += w * z * z * z;
+    M[12] += 3.0 * w * x * x * y;
+    M[13] += 3.0 * w * x * x * z;
+    M[14] += 3.0 * w * y * y * x;
+    M[15] += 3.0 * w * y * y * z;
+    M[16] += 3.0 * w * z * z * x;
+    M[17] += 3.0 * w * z * z * y;
+    M[18] += 6.0 * w * x * y * z;
+  }
 }
 
 void push_lane(SetPack& S, double phi, uint32_t cls, const lc::Ginterp& gi) {
@@ -357,10 +410,12 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   }
   S.n_tiles = ntile;
   S.ptcode.assign(cones.size() * ntile, PT_FULL);
+  S.moms.assign(cones.size() * MOM_STRIDE, 0.0);
   for (size_t g = 0; g < S.segs.size(); ++g) {
     const SegDev& sg = S.segs[g];
     const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
     const uint32_t eoff = (sg.flags & SEG_POSN) ? 32u : 16u;
+    const bool mom_ok = moment_degree(sg, d) > 0;
     // An element adds nothing unless mu > thr: the star's mucrit (mu <= 0
     // also gives 0 through imu unless beaming evaluates imu at another mu),
     // or 0 for disc, edge and spot (flux.rs:36-106).
@@ -372,12 +427,18 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     }
     const bool cull = std::fabs(thr) < 0.99;
     const double beta = cull ? std::acos(thr) : lc::PI;
+    // Seen whole: mu > max(thr, 0) for every element and lane, so that the
+    // reference's term is the contribution polynomial itself.
+    const double thr_front = std::max(thr, 0.0);
+    const bool front_ok = mom_ok && thr_front < 0.99;
+    const double beta_front = front_ok ? std::acos(thr_front) : 0.0;
     for (uint32_t q = 0; q < sg.n / PATCH; ++q) {
       const Cone& pc = cones[sg.patch_off + q];
       lc::Vec3 pa(pc.x, pc.y, pc.z);
       const double plen = pa.length();
       if (plen > 0.0) pa = (1.0 / plen) * pa;
       const uint8_t* pr = S.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
+      if (mom_ok) patch_moments(pr, stride, &S.moms[static_cast<size_t>(sg.patch_off + q) * MOM_STRIDE]);
       for (uint32_t t = 0; t < ntile; ++t) {
         const TileInfo& ti = tiles[t];
         uint8_t& code = S.ptcode[static_cast<size_t>(sg.patch_off + q) * ntile + t];
@@ -393,8 +454,10 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           code = PT_SKIP;
           continue;
         }
+        const double alpha = plen > 0.0 ? std::acos(lc::clampd(lc::dot(ti.axis, pa), -1.0, 1.0)) : lc::PI;
+        const bool front = front_ok && plen > 0.0 && alpha + ti.theta + pc.theta + kConeMargin < beta_front;
         if (sg.necl == 0) {
-          code = PT_FREE;
+          code = front ? PT_MOM : PT_FREE;
           continue;
         }
         // Point::visible (types.rs:50-58) over the tile's phase range
@@ -418,7 +481,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           all_vis = all_vis && vis;
           all_ecl = all_ecl && ecl;
         }
-        code = all_ecl ? PT_SKIP : (all_vis ? PT_FREE : PT_FULL);
+        code = all_ecl ? PT_SKIP : (all_vis ? (front ? PT_MOM : PT_FREE) : PT_FULL);
       }
     }
   }

@@ -41,6 +41,7 @@ struct SetPack {
   // numbered in segment order (SegDev::patch_off).
   std::vector<uint8_t> ptcode;
   uint32_t n_tiles = 0;
+  std::vector<double> moms;  // MOM_STRIDE per patch (zero where unused)
   std::vector<double> lane_phi;
   std::vector<uint32_t> lane_meta;
   std::vector<uint32_t> t_lane0;

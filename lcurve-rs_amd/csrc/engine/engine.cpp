@@ -104,10 +104,15 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
     pt_base_[s] = ptcode_.size();
     pt_tiles_[s] = P.n_tiles;
     ptcode_.insert(ptcode_.end(), P.ptcode.begin(), P.ptcode.end());
+    if (P.moms.size() % MOM_STRIDE) throw DeviceError("patch moments out of step");
+    const uint64_t mom_base = moms_.size() / MOM_STRIDE;
+    moms_.insert(moms_.end(), P.moms.begin(), P.moms.end());
     for (size_t g = 0; g < P.segs.size(); ++g) {
       SegDev sg = P.segs[g];
       sg.rec_off += rec_base[s];
       sg.set = static_cast<uint32_t>(s);
+      if (mom_base + sg.patch_off > UINT32_MAX) throw DeviceError("too many patches");
+      sg.mom_off = static_cast<uint32_t>(mom_base + sg.patch_off);
       uint32_t gid = static_cast<uint32_t>(segs_.size());
       if (P.seg_grid[g] < 7) {
         H.grids[P.seg_grid[g]].segs.push_back(gid);
@@ -203,6 +208,8 @@ void Batch::upload() {
   d_.t_ferr = lp;
   up(t_w_, const_cast<double**>(&lp));
   d_.t_w = lp;
+  up(moms_, const_cast<double**>(&lp));
+  d_.moms = lp;
   d_.lane_tot = dalloc<double>(lane_phi_.size());
   d_.flux = dalloc<double>(t_lane0_.size());
   d_.out = dalloc<SetOut>(sets_.size());
@@ -260,14 +267,15 @@ void Batch::build_items(uint32_t chunk) {
             it.ne = std::min(chunk, sg.n - k * chunk);
             it.tile = t;
             it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
-            it.todo = it.free = 0;
+            it.todo = it.free = it.mom = 0;
             const uint32_t np = it.ne / PATCH;
             for (uint32_t q = 0; q < np; ++q) {
               const uint64_t ci = pt_base_[s] + static_cast<uint64_t>(sg.patch_off + it.e0 / PATCH + q) * pt_tiles_[s] + t;
               if (ci >= ptcode_.size()) throw DeviceError("patch code out of range");
               const uint8_t code = ptcode_[ci];
-              if (code != PT_SKIP) it.todo |= 1ull << q;
+              if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
               if (code == PT_FREE) it.free |= 1ull << q;
+              if (code == PT_MOM) it.mom |= 1ull << q;
             }
             items.push_back(it);
           }
@@ -278,6 +286,11 @@ void Batch::build_items(uint32_t chunk) {
   }
   n_prows_ = prow;
   n_items_ = items.size();
+  pairs_eval_ = pairs_mom_ = 0;
+  for (const ItemDev& it : items) {
+    pairs_eval_ += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
+    pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom)) * PATCH * TILE;
+  }
   // Host-side bounds check of every item against the buffers it indexes.
   for (const ItemDev& it : items) {
     if (it.seg >= segs_.size()) throw DeviceError("item segment out of range");
@@ -293,7 +306,10 @@ void Batch::build_items(uint32_t chunk) {
     if (it.e0 % PATCH || it.ne % PATCH || it.ne / PATCH > MAX_ITEM_PATCHES)
       throw DeviceError("item not aligned to patches");
     const uint64_t live = it.ne / PATCH == 64 ? ~0ull : (1ull << (it.ne / PATCH)) - 1ull;
-    if ((it.todo & ~live) || (it.free & ~it.todo)) throw DeviceError("item patch masks out of range");
+    if ((it.todo & ~live) || (it.free & ~it.todo) || (it.mom & ~live) || (it.mom & it.todo))
+      throw DeviceError("item patch masks out of range");
+    if (it.mom && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
+      throw DeviceError("item moments out of range");
   }
   if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
 

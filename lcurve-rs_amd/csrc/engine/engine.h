@@ -72,6 +72,8 @@ class Batch {
   const SetHost& set(uint32_t i) const { return host_[i]; }
   uint64_t visits() const;
   uint64_t record_bytes() const { return rec_bytes_; }
+  uint64_t pairs_evaluated() const { return pairs_eval_; }
+  uint64_t pairs_by_moments() const { return pairs_mom_; }
   uint64_t n_items() const { return n_items_; }
   uint64_t n_lanes() const { return n_lanes_; }
   size_t n_times() const { return n_times_; }
@@ -100,9 +102,10 @@ class Batch {
   std::vector<uint32_t> chunk_info_;
   // Per-set patch codes (ptcode of SetPack), kept for rebuilding items.
   std::vector<uint8_t> ptcode_;
+  std::vector<double> moms_;
   std::vector<uint64_t> pt_base_;
   std::vector<uint32_t> pt_tiles_;
-  uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0;
+  uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0;
   uint32_t chunk_ = 0;
 
   DeviceBatch d_;

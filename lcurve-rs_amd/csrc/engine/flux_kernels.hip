@@ -289,6 +289,39 @@ __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t 
   return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
 }
 
+// Patches seen whole by every lane (PT_MOM): the sum over the patch of
+// w * mu * (c1 + c2 mu + c3 mu^2) from its direction moments, in FP64.
+__device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint64_t mm,
+                                             const LaneCtx& L, double c1, double c2, double c3) {
+  const double x = L.ex, y = L.ey, z = L.ez;
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  double acc = 0.0;
+  if (c3 != 0.0) {
+    const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
+                 t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
+    while (mm) {
+      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
+      mm &= mm - 1ull;
+      const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
+      const double s1 = M[0] * x + M[1] * y + M[2] * z;
+      const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
+      const double s3 = M[9] * t0 + M[10] * t1 + M[11] * t2 + M[12] * t3 + M[13] * t4 +
+                        M[14] * t5 + M[15] * t6 + M[16] * t7 + M[17] * t8 + M[18] * t9;
+      acc += c1 * s1 + c2 * s2 + c3 * s3;
+    }
+  } else {
+    while (mm) {
+      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
+      mm &= mm - 1ull;
+      const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
+      const double s1 = M[0] * x + M[1] * y + M[2] * z;
+      const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
+      acc += c1 * s1 + c2 * s2;
+    }
+  }
+  return acc;
+}
+
 // Workgroup b of a launch lands on XCD b % 8. Give every XCD a contiguous
 // run of items so that the tiles of one chunk share that XCD's L2.
 __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
@@ -299,7 +332,8 @@ __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
 
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
-                             const char* __restrict__ recs, const double* __restrict__ lane_phi,
+                             const char* __restrict__ recs, const double* __restrict__ moms,
+                             const double* __restrict__ lane_phi,
                              const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
   const ItemDev it = items[id];
   const SegDev sg = segs[it.seg];
@@ -360,13 +394,23 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
   const char* p = recs + sg.rec_off + static_cast<uint64_t>(it.e0) * stride;
   const float lin = sp.lin_disc, quad = sp.quad_disc;
-  double sum;
-  switch (sg.necl) {
-    case 0: sum = item_dispatch<0>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 1: sum = item_dispatch<1>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 2: sum = item_dispatch<2>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 3: sum = item_dispatch<3>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    default: sum = item_dispatch<4>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+  double sum = 0.0;
+  if (it.mom) {
+    // w mu (1 - A o - B o^2) with o = 1 - mu, as c1 mu + c2 mu^2 + c3 mu^3:
+    // stars (A, B) = (a0, a1) (the host sends only Poly laws with a2 = a3 = 0
+    // and no beaming or lensing here), disc and edge (lin, quad), spot (0, 0).
+    const bool star = sg.kind <= KIND_STAR2;
+    const double A = star ? P.a0 : (sg.kind == KIND_SPOT ? 0.0 : lin);
+    const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
+    sum = moment_sum(moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE, it.mom, L,
+                     1.0 - A - B, A + 2.0 * B, -B);
+  }
+  if (it.todo) switch (sg.necl) {
+    case 0: sum += item_dispatch<0>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 1: sum += item_dispatch<1>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 2: sum += item_dispatch<2>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 3: sum += item_dispatch<3>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    default: sum += item_dispatch<4>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
   }
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
@@ -377,11 +421,11 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
 __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ lane_phi, const uint32_t* __restrict__ lane_meta,
-    double* __restrict__ partials) {
+    const double* __restrict__ moms, const double* __restrict__ lane_phi,
+    const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item(id, items, segs, sets, recs, lane_phi, lane_meta, partials);
+  process_item(id, items, segs, sets, recs, moms, lane_phi, lane_meta, partials);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -389,8 +433,9 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
 __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ lane_phi, const uint32_t* __restrict__ lane_meta,
-    double* __restrict__ partials, uint32_t* __restrict__ counter, uint32_t grab) {
+    const double* __restrict__ moms, const double* __restrict__ lane_phi,
+    const uint32_t* __restrict__ lane_meta, double* __restrict__ partials,
+    uint32_t* __restrict__ counter, uint32_t grab) {
   for (;;) {
     uint32_t base = 0;
     if (threadIdx.x == 0) base = atomicAdd(counter, grab);
@@ -398,7 +443,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item(id, items, segs, sets, recs, lane_phi, lane_meta, partials);
+      process_item(id, items, segs, sets, recs, moms, lane_phi, lane_meta, partials);
   }
 }
 
@@ -625,12 +670,12 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
   if (d.n_items == 0) return hipSuccess;
   if (!persistent) {
     hipLaunchKernelGGL(flux_items_kernel, dim3(d.n_items), dim3(TILE), 0, stream, d.items, d.n_items,
-                       d.segs, d.sets, d.recs, d.lane_phi, d.lane_meta, d.partials);
+                       d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_meta, d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
-                       d.items, d.n_items, d.segs, d.sets, d.recs, d.lane_phi, d.lane_meta,
-                       d.partials, d.counter, grab);
+                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.lane_phi,
+                       d.lane_meta, d.partials, d.counter, grab);
   }
   return hipGetLastError();
 }

@@ -52,11 +52,22 @@ __host__ __device__ constexpr uint32_t rec_stride(uint32_t necl, bool posn) {
 // together. For every (patch, lane tile) pair the host decides, exactly,
 // whether the patch must be evaluated with the eclipse test (PT_FULL),
 // needs no eclipse test because no lane of the tile is eclipsed by any
-// element of it (PT_FREE), or adds exactly zero to every lane of the tile
+// element of it (PT_FREE), adds exactly zero to every lane of the tile
 // (PT_SKIP: all normals face away from all the tile's lines of sight, or
-// every element is eclipsed at every lane).
+// every element is eclipsed at every lane), or is seen whole by every lane
+// (PT_MOM: every element visible with mu above the cut-off), when a
+// contribution polynomial in mu of degree <= 3 is a polynomial in the earth
+// vector whose coefficients are the patch's direction moments.
 constexpr uint32_t PATCH = 64;
-constexpr uint8_t PT_FULL = 0, PT_FREE = 1, PT_SKIP = 2;
+constexpr uint8_t PT_FULL = 0, PT_FREE = 1, PT_SKIP = 2, PT_MOM = 3;
+
+// Direction moments of a patch, FP64, MOM_STRIDE doubles per patch, from the
+// FP32 normals d and fluxes w the element records hold:
+//   [0..2]  sum w d                       (x, y, z)
+//   [3..8]  sum w (e.d)^2 coefficients of  x2 y2 z2 xy xz yz
+//   [9..18] sum w (e.d)^3 coefficients of  x3 y3 z3 x2y x2z y2x y2z z2x z2y xyz
+// so that sum_w mu^k = the k-th block dotted with the same monomials of e.
+constexpr uint32_t MOM_STRIDE = 20;
 // At most 64 patches per work item: the item's patch codes fit 64-bit masks.
 constexpr uint32_t MAX_ITEM_PATCHES = 64;
 
@@ -69,6 +80,7 @@ struct SegDev {
   uint32_t necl;      // eclipse pairs per element
   uint32_t flags;     // SEG_POSN
   uint32_t patch_off; // first patch, numbered within the set (host only)
+  uint32_t mom_off;   // first patch's moments (global patch index)
 };
 
 struct ItemDev {
@@ -77,8 +89,9 @@ struct ItemDev {
   uint32_t ne;    // element count (multiple of PATCH)
   uint32_t tile;  // tile within the set
   uint64_t prow;  // partial row index
-  uint64_t todo;  // bit q: patch e0 / PATCH + q is evaluated (not PT_SKIP)
+  uint64_t todo;  // bit q: patch e0 / PATCH + q is evaluated element by element
   uint64_t free;  // bit q: that patch needs no eclipse test (PT_FREE)
+  uint64_t mom;   // bit q: that patch is evaluated from its moments (PT_MOM)
 };
 
 // Set flags.

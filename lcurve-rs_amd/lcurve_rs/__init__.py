@@ -85,6 +85,14 @@ def _ctx():
         return h
 
 
+def set_tuning(chunk, persistent=0):
+    """Fix the flux-kernel launch form of batches built from now on on the
+    current device: `chunk` elements per work item (rounded up to whole
+    64-element patches, at most 4096), `persistent` 0 (one workgroup per
+    item) or 1 (persistent grid). Batch.autotune() overrides it."""
+    check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
+
+
 def _dptr(a):
     return a.ctypes.data_as(_native.D)
 
@@ -361,6 +369,20 @@ class Model:
             _dptr(cols["ferr"]), _dptr(cols["weight"]), _iptr(cols["ndiv"]), n, int(bool(scale)),
             int(rdata), _dptr(calc), ctypes.byref(res)))
         return LcResult(cols["time"].copy(), calc, res)
+
+    def selfcheck_patch_codes(self, times, expose=0.001, ndiv=1):
+        """CPU check (no device) of the engine's work skipping for this model
+        at these times: returns (max relative difference between the coded
+        and the brute-force FP64 sums, {code: (patch, tile) pairs})."""
+        t = _f64(times)
+        n = len(t)
+        e = np.full(n, float(expose))
+        nd = np.full(n, int(ndiv), dtype=np.int32)
+        r = ctypes.c_double()
+        c = (ctypes.c_longlong * 4)()
+        check(_native.lib().lch_selfcheck_patch_codes(self._h, _dptr(t), _dptr(e), _iptr(nd), n,
+                                                      ctypes.byref(r), c))
+        return r.value, {"full": c[0], "no_eclipse": c[1], "skip": c[2], "moments": c[3]}
 
     def chisq_batch(self, data, names, values, scale=True):
         """Chi-squared of N parameter sets (python/src/lib.rs:696-734).

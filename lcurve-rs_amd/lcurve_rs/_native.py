@@ -37,7 +37,9 @@ class BatchStatsC(ctypes.Structure):
     _fields_ = [("n_sets", ctypes.c_longlong), ("n_valid", ctypes.c_longlong),
                 ("n_times", ctypes.c_longlong), ("n_lanes", ctypes.c_longlong),
                 ("n_items", ctypes.c_longlong), ("visits", ctypes.c_longlong),
-                ("record_bytes", ctypes.c_longlong), ("setup_seconds", ctypes.c_double),
+                ("record_bytes", ctypes.c_longlong), ("pairs_evaluated", ctypes.c_longlong),
+                ("pairs_by_moments", ctypes.c_longlong),
+                ("setup_seconds", ctypes.c_double),
                 ("upload_seconds", ctypes.c_double), ("chunk", ctypes.c_int),
                 ("persistent", ctypes.c_int)]
 
@@ -76,6 +78,7 @@ SIGNATURES = {
     "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
     "lch_batch_autotune": (INT, [VP, VP, INT]),
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
+    "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong)]),
     "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
     "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),
 }
