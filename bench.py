@@ -32,8 +32,16 @@ import lcurve_rs as lc  # noqa: E402
 from lcurve_rs import workloads as W  # noqa: E402
 
 FLOP_PER_VISIT = 19.0        # FP32 flops per element-visit of the reference formulation
+# Work the flux kernel performs, per unit, on the linear limb-darkening star
+# path of the chisq / wd_donor workloads: an element-lane pair evaluated one
+# by one costs mu (mul + 2 fma = 5), w*m (1), I(m) (fma = 2), accumulate
+# (fma = 2) = 10 FP32 flops; a patch-lane summed from its moments costs
+# sum w mu (5), sum w mu^2 (11), c1*s1 + c2*s2 (3), accumulate (1) = 20 FP64.
+FLOP_PER_PAIR = 10.0
+FLOP_PER_MOMENT_LANE = 20.0
 VALU_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E peak
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(msg):
@@ -139,6 +147,44 @@ def cpu_baseline(info, budget_s):
             "setup_core_seconds_per_set": round(per_set_setup, 4), "wall_s": round(dt, 3)}
 
 
+def pmc_summary(workload, chunk):
+    """HBM traffic and VALU issue figures of the flux kernel from the
+    committed rocprofv3 --pmc summary of this workload, when it was taken at
+    the same work-item size (else None)."""
+    path = os.path.join(PROFILES, f"flux_items_kernel_{workload}_pmc.json")
+    try:
+        with open(path) as f:
+            pm = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return pm if pm.get("chunk") == chunk else None
+
+
+def roofline(workload, st, ms_flux):
+    pairs = st["pairs_evaluated"]
+    mom_lanes = st["pairs_by_moments"] // 64
+    flop = pairs * FLOP_PER_PAIR + mom_lanes * FLOP_PER_MOMENT_LANE
+    tflops = flop / (ms_flux * 1e-3) / 1e12
+    # Compulsory HBM bytes of one flux launch: element records and patch
+    # moments once, lane tables, partial rows written.
+    alg_bytes = st["record_bytes"] + st["n_lanes"] * 12 + 8 * 64 * st["n_items"]
+    pm = pmc_summary(workload, st["chunk"])
+    traffic = pm["hbm_bytes_per_launch"] if pm else None
+    return {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": traffic,
+            "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
+            "flop_per_evaluated_pair": FLOP_PER_PAIR, "flop_per_moment_patch_lane": FLOP_PER_MOMENT_LANE,
+            # the per-pair figure is that of the linear limb-darkening star path; the
+            # Claret, disc and spot terms of disc_spot_300k cost more, so there it
+            # understates the work done
+            "flop_counts_exact": workload != "disc_spot_300k",
+            "pairs_evaluated_per_launch": pairs, "moment_patch_lanes_per_launch": mom_lanes,
+            "reference_visits_per_launch": st["visits"],
+            "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
+            "hbm_algorithmic_bytes": alg_bytes,
+            "hbm_frac": round((traffic or alg_bytes) / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+
+
 def copy_calibration(mib):
     """A streaming kernel of known traffic (torch.neg over float32: reads and
     writes `mib` MiB each), run under `rocprofv3 --pmc FETCH_SIZE` /
@@ -212,22 +258,7 @@ def main():
 
     # Dominant kernel, timed with HIP events on its own stream.
     ms_flux, ms_pass = batch.time(scale=scale, reps=max(3, min(args.steps, 10)))
-    visits = st["visits"]
-    tflops = visits * FLOP_PER_VISIT / (ms_flux * 1e-3) / 1e12
-    # Compulsory HBM bytes of one flux launch: element records once, lane
-    # tables, partial rows written.
-    lanes = st["n_lanes"]
-    alg_bytes = st["record_bytes"] + lanes * 12 + 8 * 64 * st["n_items"]
-    roof = {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
-            "algorithmic_flop_per_visit": FLOP_PER_VISIT, "visits_per_launch": visits,
-            # (element, lane) pairs the kernel actually evaluates; the rest are
-            # skipped because they add exactly zero.
-            "pairs_evaluated_per_launch": st["pairs_evaluated"],
-            "evaluated_pairs_per_s": round(st["pairs_evaluated"] / (ms_flux * 1e-3), 1),
-            "hbm_algorithmic_gbs": round(alg_bytes / (ms_flux * 1e-3) / 1e9, 2),
-            "hbm_frac": round(alg_bytes / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    roof = roofline(args.workload, st, ms_flux)
 
     rec = {"metric": "chi2 evals/sec (1 GPU) at <=1e-6 rel flux err vs FP64" if args.workload == "chisq_4096"
            else "light curves/sec (1 GPU)",
@@ -236,7 +267,10 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
            "data": "synthetic (seeded parameter sets and noise; no external data)",
            "config": info["config"],
-           "element_visits_per_step": visits, "visits_per_s": round(visits * args.steps / elapsed, 1),
+           "element_visits_per_step": st["visits"],
+           "visits_per_s": round(st["visits"] * args.steps / elapsed, 1),
+           # rate the reference's per-visit arithmetic would need for this throughput
+           "reference_equivalent_tflop_s": round(st["visits"] * FLOP_PER_VISIT / (ms_flux * 1e-3) / 1e12, 2),
            "max_rel_flux_err_vs_fp64": flux_err, "parity_sets": nchk,
            "max_rel_chisq_err_vs_fp64": chi_err,
            "cpu_setup_s": round(st["setup_seconds"], 2), "upload_s": round(st["upload_seconds"], 2),

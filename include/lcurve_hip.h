@@ -57,7 +57,7 @@ typedef struct lch_lc_result {
 typedef struct lch_batch_stats {
   long long n_sets, n_valid, n_times, n_lanes, n_items;
   long long visits;       /* element-visits in the reference's loop order */
-  long long record_bytes; /* resident element records in HBM */
+  long long record_bytes; /* resident element records and patch moments in HBM */
   long long pairs_evaluated; /* (element, lane) pairs the flux kernel evaluates one
                                 by one per launch, padding included */
   long long pairs_by_moments; /* pairs summed in closed form from patch moments

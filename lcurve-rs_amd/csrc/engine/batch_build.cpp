@@ -388,15 +388,21 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     double theta = lc::PI;
   };
   std::vector<TileInfo> tiles(ntile);
+  S.lane_e.resize(S.lane_phi.size());
+  for (size_t l = 0; l < S.lane_phi.size(); ++l) {
+    const uint32_t m = S.lane_meta[l];
+    const bool lcl = meta_class(m) == CLS_NORMAL;
+    const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
+    S.lane_e[l] = make_float4(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z), 0.0f);
+  }
   for (uint32_t t = 0; t < ntile; ++t) {
     TileInfo& ti = tiles[t];
     std::vector<lc::Vec3> es;
     for (uint32_t l = t * TILE; l < (t + 1) * TILE; ++l) {
       const uint32_t m = S.lane_meta[l];
       if (!(m & META_VALID)) continue;
-      const bool lcl = meta_class(m) == CLS_NORMAL;
-      const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
-      es.push_back(lc::Vec3(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z)));
+      const float4 e = S.lane_e[l];
+      es.push_back(lc::Vec3(e.x, e.y, e.z));
       const double phif = S.lane_phi[l] - std::floor(S.lane_phi[l]);
       ti.lo = ti.any ? std::min(ti.lo, phif) : phif;
       ti.hi = ti.any ? std::max(ti.hi, phif) : phif;

@@ -43,6 +43,7 @@ struct SetPack {
   uint32_t n_tiles = 0;
   std::vector<double> moms;  // MOM_STRIDE per patch (zero where unused)
   std::vector<double> lane_phi;
+  std::vector<float4> lane_e;  // earth vector per lane (x, y, z, 0), FP32
   std::vector<uint32_t> lane_meta;
   std::vector<uint32_t> t_lane0;
   std::vector<int> t_ndiv;

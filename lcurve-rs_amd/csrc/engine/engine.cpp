@@ -120,7 +120,9 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
       }
       segs_.push_back(sg);
     }
+    if (P.lane_e.size() != P.lane_phi.size()) throw DeviceError("lane earth vectors out of step");
     lane_phi_.insert(lane_phi_.end(), P.lane_phi.begin(), P.lane_phi.end());
+    lane_e_.insert(lane_e_.end(), P.lane_e.begin(), P.lane_e.end());
     lane_meta_.insert(lane_meta_.end(), P.lane_meta.begin(), P.lane_meta.end());
     t_lane0_.insert(t_lane0_.end(), P.t_lane0.begin(), P.t_lane0.end());
     t_ndiv_.insert(t_ndiv_.end(), P.t_ndiv.begin(), P.t_ndiv.end());
@@ -190,6 +192,9 @@ void Batch::upload() {
   const double* lp;
   up(lane_phi_, const_cast<double**>(&lp));
   d_.lane_phi = lp;
+  const float4* fp;
+  up(lane_e_, const_cast<float4**>(&fp));
+  d_.lane_e = fp;
   const uint32_t* u;
   up(lane_meta_, const_cast<uint32_t**>(&u));
   d_.lane_meta = u;
@@ -265,7 +270,7 @@ void Batch::build_items(uint32_t chunk) {
             it.seg = static_cast<uint32_t>(g);
             it.e0 = k * chunk;
             it.ne = std::min(chunk, sg.n - k * chunk);
-            it.tile = t;
+            it.lane0 = sd.lane_off + t * TILE;
             it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
             it.todo = it.free = it.mom = 0;
             const uint32_t np = it.ne / PATCH;
@@ -277,6 +282,9 @@ void Batch::build_items(uint32_t chunk) {
               if (code == PT_FREE) it.free |= 1ull << q;
               if (code == PT_MOM) it.mom |= 1ull << q;
             }
+            // An item adding zero to every lane is not launched: its
+            // partial row stays at the zero written once below.
+            if (it.todo == 0 && it.mom == 0) continue;
             items.push_back(it);
           }
         }
@@ -297,7 +305,9 @@ void Batch::build_items(uint32_t chunk) {
     const SegDev& sg = segs_[it.seg];
     const SetDev& sd = sets_[sg.set];
     if (static_cast<uint64_t>(it.e0) + it.ne > sg.n || it.ne == 0) throw DeviceError("item elements out of range");
-    if (it.tile >= sd.n_lanes / TILE) throw DeviceError("item tile out of range");
+    if (it.lane0 % TILE || it.lane0 < sd.lane_off || it.lane0 + TILE > sd.lane_off + sd.n_lanes ||
+        it.lane0 + TILE > lane_phi_.size())
+      throw DeviceError("item lanes out of range");
     if (it.prow >= n_prows_ || it.prow < sd.prow_base) throw DeviceError("item partial row out of range");
     const uint64_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
     if (sg.rec_off + (static_cast<uint64_t>(it.e0) + it.ne) * stride > rec_bytes_)
@@ -321,6 +331,9 @@ void Batch::build_items(uint32_t chunk) {
             "hipMalloc items");
   hip_check(hipMalloc(reinterpret_cast<void**>(&d_partials_), std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)),
             "hipMalloc partials");
+  // Rows of items that are not launched must read as zero; launched items
+  // rewrite theirs on every launch. Cleared here, outside any graph.
+  hip_check(hipMemset(d_partials_, 0, std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)), "hipMemset partials");
   if (!items.empty())
     hip_check(hipMemcpy(d_items_, items.data(), items.size() * sizeof(ItemDev), hipMemcpyHostToDevice),
               "hipMemcpy items");

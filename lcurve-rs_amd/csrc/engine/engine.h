@@ -71,7 +71,8 @@ class Batch {
   uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
   const SetHost& set(uint32_t i) const { return host_[i]; }
   uint64_t visits() const;
-  uint64_t record_bytes() const { return rec_bytes_; }
+  // Resident element records plus patch moments.
+  uint64_t record_bytes() const { return rec_bytes_ + moms_.size() * sizeof(double); }
   uint64_t pairs_evaluated() const { return pairs_eval_; }
   uint64_t pairs_by_moments() const { return pairs_mom_; }
   uint64_t n_items() const { return n_items_; }
@@ -95,6 +96,7 @@ class Batch {
   std::vector<SetDev> sets_;
   std::vector<SegDev> segs_;
   std::vector<double> lane_phi_;
+  std::vector<float4> lane_e_;
   std::vector<uint32_t> lane_meta_, tile_cp_;
   std::vector<uint32_t> t_lane0_;
   std::vector<int> t_ndiv_;

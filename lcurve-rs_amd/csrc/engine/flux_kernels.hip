@@ -32,7 +32,6 @@ namespace lch {
 
 namespace {
 
-constexpr double K_TWOPI = 6.283185307179586476925286766559;
 
 // Limb-darkening forms: LD_POLY / LD_CLARET are LDC::imu (types.rs:101-119);
 // LD_LIN is the Poly law with only the linear coefficient set.
@@ -333,28 +332,29 @@ __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
-                             const double* __restrict__ lane_phi,
+                             const double* __restrict__ lane_phi, const float4* __restrict__ lane_e,
                              const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
   const ItemDev it = items[id];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t gl = it.lane0 + lane;
+  // Lane data first: it depends on the item only.
+  const double ph = lane_phi[gl];
+  const float4 e = lane_e[gl];
+  const uint32_t meta = lane_meta[gl];
   const SegDev sg = segs[it.seg];
   const SetDev& sp = sets[sg.set];
-  const uint32_t lane = threadIdx.x;
-  const uint32_t gl = sp.lane_off + it.tile * TILE + lane;
-  const double ph = lane_phi[gl];
-  const uint32_t meta = lane_meta[gl];
   const uint32_t cls = meta_class(meta);
   const bool active = (meta & META_VALID) && (sg.use_mask & cp_bit(cls, meta_ptype(meta)));
 
   LaneCtx L;
   L.phif = ph - floor(ph);
   const bool lc_lane = cls == CLS_NORMAL;
-  const double cosi = lc_lane ? sp.cosi_lc : sp.cosi_rad;
-  const double sini = lc_lane ? sp.sini_lc : sp.sini_rad;
-  double sn, cs;
-  sincos(K_TWOPI * ph, &sn, &cs);
-  L.ex = static_cast<float>(sini * cs);
-  L.ey = static_cast<float>(-sini * sn);
-  L.ez = static_cast<float>(cosi);
+  // Earth vector (set_earth, flux.rs:7-22 conventions) made on the host in
+  // FP64 and rounded to FP32: comp_light lanes use the TWOPI*iangle/360
+  // inclination, the single-star lanes iangle.to_radians().
+  L.ex = e.x;
+  L.ey = e.y;
+  L.ez = e.z;
 
   const bool star1 = sg.kind == KIND_STAR1;
   StarPar P;
@@ -422,10 +422,11 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
     const double* __restrict__ moms, const double* __restrict__ lane_phi,
-    const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
+    const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
+    double* __restrict__ partials) {
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item(id, items, segs, sets, recs, moms, lane_phi, lane_meta, partials);
+  process_item(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -434,8 +435,8 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
     const double* __restrict__ moms, const double* __restrict__ lane_phi,
-    const uint32_t* __restrict__ lane_meta, double* __restrict__ partials,
-    uint32_t* __restrict__ counter, uint32_t grab) {
+    const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
+    double* __restrict__ partials, uint32_t* __restrict__ counter, uint32_t grab) {
   for (;;) {
     uint32_t base = 0;
     if (threadIdx.x == 0) base = atomicAdd(counter, grab);
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item(id, items, segs, sets, recs, moms, lane_phi, lane_meta, partials);
+      process_item(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
   }
 }
 
@@ -670,11 +671,11 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
   if (d.n_items == 0) return hipSuccess;
   if (!persistent) {
     hipLaunchKernelGGL(flux_items_kernel, dim3(d.n_items), dim3(TILE), 0, stream, d.items, d.n_items,
-                       d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_meta, d.partials);
+                       d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
-                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.lane_phi,
+                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_e,
                        d.lane_meta, d.partials, d.counter, grab);
   }
   return hipGetLastError();

@@ -14,6 +14,7 @@ struct DeviceBatch {
   const ItemDev* items = nullptr;
   const char* recs = nullptr;
   const double* lane_phi = nullptr;
+  const float4* lane_e = nullptr;        // earth vector per lane
   const uint32_t* lane_meta = nullptr;
   const uint32_t* tile_cp = nullptr;
   const uint32_t* chunk_info = nullptr;  // use_mask | kind << 16, per chunk

@@ -87,7 +87,7 @@ struct ItemDev {
   uint32_t seg;   // segment index (global)
   uint32_t e0;    // first element (multiple of PATCH)
   uint32_t ne;    // element count (multiple of PATCH)
-  uint32_t tile;  // tile within the set
+  uint32_t lane0; // first lane of the item's tile (global index, multiple of TILE)
   uint64_t prow;  // partial row index
   uint64_t todo;  // bit q: patch e0 / PATCH + q is evaluated element by element
   uint64_t free;  // bit q: that patch needs no eclipse test (PT_FREE)

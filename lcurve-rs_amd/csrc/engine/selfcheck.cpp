@@ -96,14 +96,11 @@ SelfCheck selfcheck_codes(const SetPack& S) {
         const uint32_t m = S.lane_meta[l];
         if (!(m & META_VALID) || !(sg.use_mask & cp_bit(meta_class(m), meta_ptype(m)))) continue;
         const double ph = S.lane_phi[l];
-        const bool lcl = meta_class(m) == CLS_NORMAL;
-        const double cosi = lcl ? d.cosi_lc : d.cosi_rad, sini = lcl ? d.sini_lc : d.sini_rad;
-        const double a = 6.283185307179586476925286766559 * ph;
         Lane L;
         L.phif = ph - std::floor(ph);
-        L.ex = static_cast<float>(sini * std::cos(a));
-        L.ey = static_cast<float>(-sini * std::sin(a));
-        L.ez = static_cast<float>(cosi);
+        L.ex = S.lane_e[l].x;
+        L.ey = S.lane_e[l].y;
+        L.ez = S.lane_e[l].z;
         double coded = 0.0, brute = 0.0, scale = 0.0;
         for (uint32_t q = 0; q < sg.n / PATCH; ++q) {
           const uint8_t code = S.ptcode[static_cast<size_t>(sg.patch_off + q) * ntile + t];
