@@ -150,6 +150,11 @@ int lch_batch_time(lch_ctx* ctx, lch_batch* b, int scale, int reps, double* ms_f
 /* Try the candidate chunk sizes / launch forms and keep the fastest. */
 int lch_batch_autotune(lch_ctx* ctx, lch_batch* b, int scale);
 int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out);
+/* Per-set scale factors of the last lch_batch_run, sfac5[nsets * 5]: the
+ * per-component factors of iscale sets (star 1, disc, edge, spot, star 2;
+ * orchestration.rs:262-283), else the autoscale factor (or 1) five times
+ * (orchestration.rs:284-289), as LcResult.sfac. */
+int lch_batch_sfac(const lch_batch* b, double* sfac5);
 /* Per-set setup status and per-curve scalars; err may be null. */
 int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, double* logg2,
                        double* rv1, double* rv2, char* err, int errlen);

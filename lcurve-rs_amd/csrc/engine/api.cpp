@@ -74,15 +74,11 @@ lc::Data make_data(const double* time, const double* expose, const double* flux,
   return d;
 }
 
-void check_unsupported(const lc::Model& m) {
-  if (m.iscale)
-    throw lc::LcError("iscale=1 (per-component scale factors) is not supported by the HIP engine");
-}
+using SetOutArr = std::vector<lch::SetOut>;
 
 // Setup + pack + upload of a batch of models.
 lch_batch* build_batch(lch_ctx* ctx, const std::vector<lc::Model>& models,
                        const std::vector<lc::Data>& data, bool keep_maps) {
-  for (const lc::Model& m : models) check_unsupported(m);
   auto b = std::make_unique<lch_batch>();
   double t0 = now_s();
   auto packs = lch::build_packs(models, data, 0, keep_maps);
@@ -265,7 +261,8 @@ int lch_light_curve(lch_ctx* ctx, const lch_model* m, const double* time, const 
       res->logg2 = H.logg2;
       res->rv1 = H.rv1;
       res->rv2 = H.rv2;
-      for (double& s : res->sfac) s = scale ? sfac : 1.0;
+      const SetOutArr& o = b->impl->last_results();
+      for (int j = 0; j < 5; ++j) res->sfac[j] = o.empty() ? 1.0 : o[0].sfac[j];
     }
     return LCH_OK;
   });
@@ -451,4 +448,13 @@ int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const doub
       for (int i = 0; i < 4; ++i) counts[i] = c.counts[i];
     return LCH_OK;
   });
+}
+
+int lch_batch_sfac(const lch_batch* b, double* sfac5) {
+  if (!b || !sfac5) return fail(LCH_ERR_INVALID, "null argument");
+  const SetOutArr& o = b->impl->last_results();
+  if (o.size() != b->impl->n_sets()) return fail(LCH_ERR_INVALID, "no results yet: call lch_batch_run first");
+  for (size_t s = 0; s < o.size(); ++s)
+    for (int j = 0; j < 5; ++j) sfac5[s * 5 + j] = o[s].sfac[j];
+  return LCH_OK;
 }

@@ -177,6 +177,7 @@ std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& dat
   S.t_lane0.assign(data.size(), 0u);
   S.t_ndiv.assign(data.size(), 0);
   S.t_slfac.assign(data.size(), 0.0);
+  S.t_phase.assign(data.size(), 0.0);
   S.t_y.assign(data.size(), 0.0);
   S.t_ferr.assign(data.size(), 1.0);
   S.t_w.assign(data.size(), 0.0);
@@ -198,7 +199,8 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
 
   const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0, lens = P.glens1;
   d.flags = SET_VALID | (P.need_cal1 ? SET_CAL1 : 0u) | (P.need_cal2 ? SET_CAL2 : 0u) |
-            (beam1 ? SET_BEAM1 : 0u) | (beam2 ? SET_BEAM2 : 0u) | (lens ? SET_LENS : 0u);
+            (beam1 ? SET_BEAM1 : 0u) | (beam2 ? SET_BEAM2 : 0u) | (lens ? SET_LENS : 0u) |
+            (P.iscale ? SET_ISCALE : 0u) | (P.ncol == 5 ? SET_NCOL5 : 0u);
   d.ltype1 = static_cast<uint32_t>(P.ldc1.ltype);
   d.ltype2 = static_cast<uint32_t>(P.ldc2.ltype);
   const double l1[4] = {P.ldc1.ldc1, P.ldc1.ldc2, P.ldc1.ldc3, P.ldc1.ldc4};
@@ -391,7 +393,9 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   S.lane_e.resize(S.lane_phi.size());
   for (size_t l = 0; l < S.lane_phi.size(); ++l) {
     const uint32_t m = S.lane_meta[l];
-    const bool lcl = meta_class(m) == CLS_NORMAL;
+    // iscale time lanes follow the single-component functions, which use
+    // iangle.to_radians() (flux.rs:121, 167, 211, 238, 264).
+    const bool lcl = meta_class(m) == CLS_NORMAL && !P.iscale;
     const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
     S.lane_e[l] = make_float4(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z), 0.0f);
   }
@@ -512,6 +516,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
 
   // Per-time terms and observations.
   S.t_slfac = P.slfac;
+  S.t_phase = P.phase;
   S.t_y.resize(nt);
   S.t_ferr.resize(nt);
   S.t_w.resize(nt);

@@ -47,7 +47,7 @@ struct SetPack {
   std::vector<uint32_t> lane_meta;
   std::vector<uint32_t> t_lane0;
   std::vector<int> t_ndiv;
-  std::vector<double> t_slfac, t_y, t_ferr, t_w;
+  std::vector<double> t_slfac, t_y, t_ferr, t_w, t_phase;
 };
 
 // Pack a prepared model; `data` supplies the observations of this set.

@@ -130,6 +130,9 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
     t_y_.insert(t_y_.end(), P.t_y.begin(), P.t_y.end());
     t_ferr_.insert(t_ferr_.end(), P.t_ferr.begin(), P.t_ferr.end());
     t_w_.insert(t_w_.end(), P.t_w.begin(), P.t_w.end());
+    if (P.t_phase.size() != P.t_w.size()) throw DeviceError("time phases out of step");
+    t_phase_.insert(t_phase_.end(), P.t_phase.begin(), P.t_phase.end());
+    if (sd.flags & SET_ISCALE) any_iscale_ = true;
     if (!P.recs.empty())
       hip_check(hipMemcpy(d_recs + rec_base[s], P.recs.data(), P.recs.size(), hipMemcpyHostToDevice),
                 "hipMemcpy records");
@@ -213,6 +216,13 @@ void Batch::upload() {
   d_.t_ferr = lp;
   up(t_w_, const_cast<double**>(&lp));
   d_.t_w = lp;
+  up(t_phase_, const_cast<double**>(&lp));
+  d_.t_phase = lp;
+  if (any_iscale_) {
+    d_.lane_comp = dalloc<double>(lane_phi_.size() * 5);
+    d_.comp = dalloc<double>(t_lane0_.size() * 5);
+    d_.lsw = dalloc<double>(t_lane0_.size() * 6);
+  }
   up(moms_, const_cast<double**>(&lp));
   d_.moms = lp;
   d_.lane_tot = dalloc<double>(lane_phi_.size());
@@ -393,7 +403,8 @@ void Batch::launch(int scale) {
 void Batch::run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux) {
   launch(scale);
   hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
-  std::vector<SetOut> out(sets_.size());
+  std::vector<SetOut>& out = last_out_;
+  out.resize(sets_.size());
   if (!out.empty())
     hip_check(hipMemcpy(out.data(), d_.out, out.size() * sizeof(SetOut), hipMemcpyDeviceToHost), "hipMemcpy out");
   for (size_t s = 0; s < out.size(); ++s) {

@@ -69,6 +69,8 @@ class Batch {
   void set_obs(const double* flux2d, const double* ferr2d, const double* weight2d);
 
   uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
+  // Per-set results of the last run() (empty before the first).
+  const std::vector<SetOut>& last_results() const { return last_out_; }
   const SetHost& set(uint32_t i) const { return host_[i]; }
   uint64_t visits() const;
   // Resident element records plus patch moments.
@@ -100,7 +102,9 @@ class Batch {
   std::vector<uint32_t> lane_meta_, tile_cp_;
   std::vector<uint32_t> t_lane0_;
   std::vector<int> t_ndiv_;
-  std::vector<double> t_slfac_, t_y_, t_ferr_, t_w_;
+  std::vector<double> t_slfac_, t_y_, t_ferr_, t_w_, t_phase_;
+  bool any_iscale_ = false;
+  std::vector<SetOut> last_out_;
   std::vector<uint32_t> chunk_info_;
   // Per-set patch codes (ptcode of SetPack), kept for rebuilding items.
   std::vector<uint8_t> ptcode_;

@@ -348,7 +348,8 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
 
   LaneCtx L;
   L.phif = ph - floor(ph);
-  const bool lc_lane = cls == CLS_NORMAL;
+  // comp_light lanes; the time lanes of iscale sets follow comp_star1/2.
+  const bool lc_lane = cls == CLS_NORMAL && !(sp.flags & SET_ISCALE);
   // Earth vector (set_earth, flux.rs:7-22 conventions) made on the host in
   // FP64 and rounded to FP32: comp_light lanes use the TWOPI*iangle/360
   // inclination, the single-star lanes iangle.to_radians().
@@ -466,26 +467,24 @@ __device__ __forceinline__ double gscale2(double phase, double p2, double s21, d
   return (s21 * (1.0 + p2 - pnorm) + s22 * (pnorm - 1.0 + p2)) / (2.0 * p2);
 }
 
-// Sum of a lane's partial rows per component: S[0] star 1, S[1] star 2,
-// S[2] disc + edge + spot.
-__device__ __forceinline__ void lane_sums(const SetDev& sp, uint32_t l, const uint32_t* tile_cp,
-                                          const uint32_t* chunk_info, const double* partials,
-                                          double S[3]) {
+// Sum of a lane's partial rows per segment kind: S[KIND_STAR1 .. KIND_SPOT].
+__device__ __forceinline__ void lane_sums(const SetDev& sp, uint32_t l, const DeviceBatch& d,
+                                          double S[5]) {
   const uint32_t t = l / TILE, j = l % TILE;
-  const uint32_t cp = tile_cp[sp.lane_off / TILE + t];
-  S[0] = S[1] = S[2] = 0.0;
+  const uint32_t cp = d.tile_cp[sp.lane_off / TILE + t];
+  S[0] = S[1] = S[2] = S[3] = S[4] = 0.0;
   const uint64_t row0 = sp.prow_base + static_cast<uint64_t>(t) * sp.n_chunks;
   for (uint32_t c = 0; c < sp.n_chunks; ++c) {
-    const uint32_t info = chunk_info[sp.chunk_off + c];
+    const uint32_t info = d.chunk_info[sp.chunk_off + c];
     if (info & cp & 0xFFFFu) {
       const uint32_t kind = info >> 16;
-      const double v = partials[(row0 + c) * TILE + j];
-      S[kind < 2u ? kind : 2u] += v;
+      S[kind < 5u ? kind : 4u] += d.partials[(row0 + c) * TILE + j];
     }
   }
 }
 
 constexpr int ABLOCK = 256;
+constexpr int NWAVE = ABLOCK / 64;
 
 __device__ double block_sum(double v, double* red) {
 #pragma unroll
@@ -495,31 +494,165 @@ __device__ double block_sum(double v, double* red) {
   if (l == 0) red[w] = v;
   __syncthreads();
   double r = 0.0;
-  for (int i = 0; i < ABLOCK / 64; ++i) r += red[i];
+  for (int i = 0; i < NWAVE; ++i) r += red[i];
   return r;
+}
+
+// Block sums of n <= 6 values at once; red holds NWAVE * 6 doubles.
+__device__ void block_sum_n(double* v, int n, double* red) {
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  for (int k = 0; k < n; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[k] = x;
+  }
+  __syncthreads();
+  if (l == 0)
+    for (int k = 0; k < n; ++k) red[w * 6 + k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    double r = 0.0;
+    for (int i = 0; i < NWAVE; ++i) r += red[i * 6 + k];
+    v[k] = r;
+  }
+}
+
+// Minimum-norm solution of R x ~ c for an upper-triangular n x n R (n <= 5)
+// through a one-sided Jacobi SVD of R, singular values <= eps treated as
+// zero. R's singular values are those of the matrix it factors, so this is
+// the reference's svd.solve(&b, 1e-12) (orchestration.rs:431-432).
+__device__ void jacobi_solve(double U[5][5], const double* c, int n, double eps, double* x) {
+  double V[5][5];
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) V[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < n; ++p) {
+      for (int q = p + 1; q < n; ++q) {
+        double alpha = 0.0, beta = 0.0, gamma = 0.0;
+        for (int i = 0; i < n; ++i) {
+          alpha += U[i][p] * U[i][p];
+          beta += U[i][q] * U[i][q];
+          gamma += U[i][p] * U[i][q];
+        }
+        if (gamma == 0.0 || fabs(gamma) <= 1e-15 * sqrt(alpha * beta)) continue;
+        rotated = true;
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < n; ++i) {
+          const double up = U[i][p], uq = U[i][q];
+          U[i][p] = cs * up - sn * uq;
+          U[i][q] = sn * up + cs * uq;
+          const double vp = V[i][p], vq = V[i][q];
+          V[i][p] = cs * vp - sn * vq;
+          V[i][q] = sn * vp + cs * vq;
+        }
+      }
+    }
+    if (!rotated) break;
+  }
+  for (int k = 0; k < n; ++k) x[k] = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s2 = 0.0, uc = 0.0;
+    for (int i = 0; i < n; ++i) {
+      s2 += U[i][j] * U[i][j];
+      uc += U[i][j] * c[i];
+    }
+    if (!(sqrt(s2) > eps)) continue;
+    const double f = uc / s2;
+    for (int k = 0; k < n; ++k) x[k] += V[k][j] * f;
+  }
+}
+
+// Per-component scale factors of one iscale set (orchestration.rs:405-436):
+// least squares over the rows with weight > 0, each scaled by
+// sqrt(weight) / ferr, by modified Gram-Schmidt on [A | b] (stable for least
+// squares) and an SVD of the small triangular factor.
+__device__ void iscale_fit(const SetDev& sp, const DeviceBatch& d, int ncol, double* red,
+                           double* sf_sh) {
+  const uint32_t tid = threadIdx.x;
+  __shared__ double R[5][6];
+  double cnt[1] = {0.0};
+  for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+    const uint32_t g = sp.time_off + t;
+    const bool ok = d.t_w[g] > 0.0;
+    cnt[0] += ok ? 1.0 : 0.0;
+    const double wt = ok ? sqrt(d.t_w[g]) / d.t_ferr[g] : 0.0;
+    for (int j = 0; j < 5; ++j) d.lsw[g * 6 + j] = j < ncol ? d.comp[g * 5 + j] * wt : 0.0;
+    d.lsw[g * 6 + 5] = d.t_y[g] * wt;
+  }
+  block_sum_n(cnt, 1, red);
+  if (cnt[0] < ncol || cnt[0] == 0.0) {
+    if (tid == 0)
+      for (int j = 0; j < 5; ++j) sf_sh[j] = j < ncol ? 1.0 : 0.0;
+    __syncthreads();
+    return;
+  }
+  for (int j = 0; j < ncol; ++j) {
+    double nrm[1] = {0.0};
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      const double a = d.lsw[(sp.time_off + t) * 6 + j];
+      nrm[0] += a * a;
+    }
+    block_sum_n(nrm, 1, red);
+    const double rjj = sqrt(nrm[0]);
+    const double inv = rjj > 0.0 ? 1.0 / rjj : 0.0;
+    // q_j and the projections of the later columns and b on it.
+    double dots[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      double* row = d.lsw + (sp.time_off + t) * 6;
+      const double q = row[j] * inv;
+      row[j] = q;
+      for (int k = j + 1; k < 6; ++k) dots[k] += q * row[k];
+    }
+    block_sum_n(dots, 6, red);
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      double* row = d.lsw + (sp.time_off + t) * 6;
+      const double q = row[j];
+      for (int k = j + 1; k < 6; ++k) row[k] -= dots[k] * q;
+    }
+    if (tid == 0) {
+      for (int k = 0; k < 6; ++k) R[j][k] = k < j ? 0.0 : (k == j ? rjj : dots[k]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double U[5][5], c[5], x[5];
+    for (int i = 0; i < 5; ++i) {
+      c[i] = i < ncol ? R[i][5] : 0.0;
+      for (int j = 0; j < 5; ++j) U[i][j] = (i < ncol && j < ncol) ? R[i][j] : 0.0;
+    }
+    jacobi_solve(U, c, ncol, 1e-12, x);
+    for (int j = 0; j < 5; ++j) sf_sh[j] = j < ncol ? x[j] : 0.0;
+  }
+  __syncthreads();
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(ABLOCK) void assemble_kernel(
-    const SetDev* __restrict__ sets, const double* __restrict__ lane_phi,
-    const uint32_t* __restrict__ tile_cp, const uint32_t* __restrict__ chunk_info,
-    const double* __restrict__ partials, const uint32_t* __restrict__ t_lane0,
-    const int* __restrict__ t_ndiv, const double* __restrict__ t_slfac,
-    const double* __restrict__ t_y, const double* __restrict__ t_ferr,
-    const double* __restrict__ t_w, double* __restrict__ lane_tot, double* __restrict__ flux_out,
-    SetOut* __restrict__ out, int scale) {
+__global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, int scale) {
   // FP64 assembly without contraction, in the reference's operation order.
 #pragma clang fp contract(off)
   __shared__ double special[9];
   __shared__ double scl[4];
-  __shared__ double red[ABLOCK / 64];
-  const SetDev sp = sets[blockIdx.x];
+  __shared__ double red[NWAVE * 6];
+  __shared__ double sf_sh[5];
+  const SetDev sp = d.sets[blockIdx.x];
   const uint32_t tid = threadIdx.x;
+  const double* lane_phi = d.lane_phi;
+  const uint32_t* t_lane0 = d.t_lane0;
+  const int* t_ndiv = d.t_ndiv;
+  const double* t_slfac = d.t_slfac;
+  const double* t_y = d.t_y;
+  const double* t_ferr = d.t_ferr;
+  const double* t_w = d.t_w;
+  double* flux_out = d.flux;
   if (!(sp.flags & SET_VALID)) {
     const double nan = __builtin_nan("");
     for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) flux_out[sp.time_off + t] = nan;
-    if (tid == 0) out[blockIdx.x] = SetOut{nan, nan, nan, nan};
+    if (tid == 0) d.out[blockIdx.x] = SetOut{nan, nan, nan, nan, {nan, nan, nan, nan, nan}};
     return;
   }
   const bool cal1 = (sp.flags & SET_CAL1) != 0, cal2 = (sp.flags & SET_CAL2) != 0;
@@ -527,8 +660,8 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(
 
   // 1. Calibration and white-dwarf lanes: raw component sums.
   if (tid < n_special) {
-    double S[3];
-    lane_sums(sp, sp.special0 + tid, tile_cp, chunk_info, partials, S);
+    double S[5];
+    lane_sums(sp, sp.special0 + tid, d, S);
     const bool star2_lane = cal2 && tid >= (cal1 ? 4u : 0u) && tid < n_special - 1u;
     special[tid] = star2_lane ? S[1] : S[0];
   }
@@ -562,15 +695,75 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(
   }
   __syncthreads();
   const double s11 = scl[0], s12 = scl[1], s21 = scl[2], s22 = scl[3];
+  double wdwarf = (1.0 * gscale1(0.5, sp.phase1, s11, s12)) * special[n_special - 1u] / 1.0;
+
+  if (sp.flags & SET_ISCALE) {
+    // Per-component columns (orchestration.rs:181-227): comp_star1 with
+    // scale1(phase) (flux.rs:151), disc, edge, spot, comp_star2 with
+    // scale2(phi) (flux.rs:197), each times slfac, no third light.
+    const int ncol = (sp.flags & SET_NCOL5) ? 5 : 4;
+    for (uint32_t l = tid; l < sp.special0; l += ABLOCK) {
+      double S[5];
+      lane_sums(sp, l, d, S);
+      double* o = d.lane_comp + static_cast<uint64_t>(sp.lane_off + l) * 5;
+      for (int k = 0; k < 5; ++k) o[k] = S[k];
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      const uint32_t g = sp.time_off + t;
+      const uint32_t l0 = t_lane0[g];
+      const int nd = t_ndiv[g];
+      const double s1 = gscale1(d.t_phase[g], sp.phase1, s11, s12);
+      double c[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int k = 0; k < nd; ++k) {
+        const double wgt = nd == 1 ? 1.0 : ((k == 0 || k == nd - 1) ? 0.5 : 1.0);
+        const double* S = d.lane_comp + static_cast<uint64_t>(sp.lane_off + l0 + k) * 5;
+        c[0] += wgt * s1 * S[KIND_STAR1];
+        c[1] += wgt * S[KIND_DISC];
+        c[2] += wgt * S[KIND_EDGE];
+        c[3] += wgt * S[KIND_SPOT];
+        c[4] += wgt * gscale2(lane_phi[sp.lane_off + l0 + k], sp.phase2, s21, s22) * S[KIND_STAR2];
+      }
+      const double denom = static_cast<double>(nd - 1 > 1 ? nd - 1 : 1);
+      for (int j = 0; j < 5; ++j) d.comp[static_cast<uint64_t>(g) * 5 + j] = j < ncol ? t_slfac[g] * (c[j] / denom) : 0.0;
+    }
+    __syncthreads();
+    if (scale) {
+      iscale_fit(sp, d, ncol, red, sf_sh);
+    } else {
+      if (tid == 0)
+        for (int j = 0; j < 5; ++j) sf_sh[j] = 1.0;
+      __syncthreads();
+    }
+    const double sf[5] = {sf_sh[0], sf_sh[1], sf_sh[2], sf_sh[3], sf_sh[4]};
+    double cs = 0.0, wn = 0.0;
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
+      const uint32_t g = sp.time_off + t;
+      const double* c = d.comp + static_cast<uint64_t>(g) * 5;
+      double v = sf[0] * c[0] + sf[1] * c[1] + sf[2] * c[2] + sf[3] * c[3];
+      if (ncol == 5) v += sf[4] * c[4];
+      flux_out[g] = v;
+      if (t_w[g] > 0.0) {
+        wn += t_w[g];
+        const double r = (t_y[g] - v) / t_ferr[g];
+        cs += t_w[g] * (r * r);
+      }
+    }
+    const double chisq = block_sum(cs, red);
+    const double wnok = block_sum(wn, red);
+    if (tid == 0)
+      d.out[blockIdx.x] = SetOut{chisq, wnok, sf[0], wdwarf * sf[0], {sf[0], sf[1], sf[2], sf[3], sf[4]}};
+    return;
+  }
 
   // 2. Sub-phase totals (flux.rs:74-107 with scale factors).
   for (uint32_t l = tid; l < sp.special0; l += ABLOCK) {
-    double S[3];
-    lane_sums(sp, l, tile_cp, chunk_info, partials, S);
+    double S[5];
+    lane_sums(sp, l, d, S);
     const double phi = lane_phi[sp.lane_off + l];
     const double ssum = S[0] * gscale1(phi, sp.phase1, s11, s12) +
-                        gscale2(phi, sp.phase2, s21, s22) * S[1] + S[2];
-    lane_tot[sp.lane_off + l] = ssum;
+                        gscale2(phi, sp.phase2, s21, s22) * S[1] + (S[2] + S[3] + S[4]);
+    d.lane_tot[sp.lane_off + l] = ssum;
   }
   __syncthreads();
 
@@ -582,7 +775,7 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(
     double sum = 0.0;
     for (int k = 0; k < nd; ++k) {
       const double wgt = nd == 1 ? 1.0 : ((k == 0 || k == nd - 1) ? 0.5 : 1.0);
-      sum += wgt * lane_tot[sp.lane_off + l0 + k];
+      sum += wgt * d.lane_tot[sp.lane_off + l0 + k];
     }
     const double denom = static_cast<double>(nd - 1 > 1 ? nd - 1 : 1);
     flux_out[sp.time_off + t] = t_slfac[sp.time_off + t] * (sum / denom) + sp.third;
@@ -591,7 +784,6 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(
 
   // 4. Autoscale (re_scale, orchestration.rs:361-388) or plain
   //    chi-squared (compute_chisq, orchestration.rs:391-401).
-  double wdwarf = (1.0 * gscale1(0.5, sp.phase1, s11, s12)) * special[n_special - 1u] / 1.0;
   double sfac = 1.0, chisq = 0.0, wnok = 0.0;
   if (scale) {
     double sdy = 0.0, syy = 0.0, wn = 0.0;
@@ -637,7 +829,7 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(
     chisq = block_sum(cs, red);
     wnok = block_sum(wn, red);
   }
-  if (tid == 0) out[blockIdx.x] = SetOut{chisq, wnok, sfac, wdwarf};
+  if (tid == 0) d.out[blockIdx.x] = SetOut{chisq, wnok, sfac, wdwarf, {sfac, sfac, sfac, sfac, sfac}};
 }
 
 // ------------------------------------------------------------ visibility
@@ -683,9 +875,7 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
 
 hipError_t launch_assemble(const DeviceBatch& d, int scale, hipStream_t stream) {
   if (d.n_sets == 0) return hipSuccess;
-  hipLaunchKernelGGL(assemble_kernel, dim3(d.n_sets), dim3(ABLOCK), 0, stream, d.sets, d.lane_phi,
-                     d.tile_cp, d.chunk_info, d.partials, d.t_lane0, d.t_ndiv, d.t_slfac, d.t_y,
-                     d.t_ferr, d.t_w, d.lane_tot, d.flux, d.out, scale);
+  hipLaunchKernelGGL(assemble_kernel, dim3(d.n_sets), dim3(ABLOCK), 0, stream, d, scale);
   return hipGetLastError();
 }
 

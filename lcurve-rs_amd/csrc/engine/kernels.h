@@ -25,6 +25,12 @@ struct DeviceBatch {
   const double* t_y = nullptr;
   const double* t_ferr = nullptr;
   const double* t_w = nullptr;
+  const double* t_phase = nullptr;       // central phase per time
+  // iscale work space (null when no set uses iscale): per-lane component
+  // sums [lane][5], per-time columns [time][5], least-squares work [time][6].
+  double* lane_comp = nullptr;
+  double* comp = nullptr;
+  double* lsw = nullptr;
   double* partials = nullptr;
   double* lane_tot = nullptr;
   double* flux = nullptr;

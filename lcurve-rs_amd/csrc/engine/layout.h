@@ -97,6 +97,11 @@ struct ItemDev {
 // Set flags.
 constexpr uint32_t SET_VALID = 1u, SET_CAL1 = 2u, SET_CAL2 = 4u, SET_BEAM1 = 8u, SET_BEAM2 = 16u,
                    SET_LENS = 32u;
+// Per-component scale factors (iscale): the set's time lanes follow
+// comp_star1 / comp_star2 / comp_disc / comp_edge / comp_spot (flux.rs:115-279)
+// and the assembly fits one factor per component (orchestration.rs:181-297).
+// SET_NCOL5: star 2 is a component (t2 > 0).
+constexpr uint32_t SET_ISCALE = 64u, SET_NCOL5 = 128u;
 
 struct SetDev {
   uint32_t lane_off;   // first lane (global index, multiple of TILE)
@@ -121,6 +126,7 @@ struct SetDev {
 // Per-set results written by the assembly kernel.
 struct SetOut {
   double chisq, wnok, scale, wdwarf;
+  double sfac[5];  // per-component factors (all equal to scale without iscale)
 };
 
 }  // namespace lch

@@ -115,6 +115,8 @@ std::unique_ptr<PreparedModel> prepare_model(const Model& mdl, const Data& data)
   P.cosi_rad = std::cos(ri);
   P.sini_rad = std::sin(ri);
   P.third = mdl.third.value;
+  P.iscale = mdl.iscale;
+  P.ncol = mdl.t2.value > 0.0 ? 5 : 4;
 
   // Per-time terms (orchestration.rs:172-241).
   double xmin = data[0].time, xmax = data[0].time;

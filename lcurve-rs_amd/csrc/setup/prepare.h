@@ -39,6 +39,11 @@ struct PreparedModel {
   double cosi_lc = 0, sini_lc = 0;    // cos/sin(TWOPI*iangle/360), flux.rs:20-21
   double cosi_rad = 0, sini_rad = 0;  // iangle.to_radians(), flux.rs:121-123
   double third = 0;
+  // Per-component scale factors (orchestration.rs:181-227): the curve is
+  // sum_j sfac[j] * column j over star 1, disc, edge, spot and, when
+  // t2 > 0 (ncol = 5), star 2.
+  bool iscale = false;
+  int ncol = 5;
 
   // Per-curve scalars computed on the CPU (flux.rs:282-375).
   double logg1 = 0, logg2 = 0, rv1 = 0, rv2 = 0;

@@ -464,14 +464,18 @@ class Batch:
             self._h = None
 
     def run(self, scale=True, want_flux=True):
-        """One pass; returns dict with chisq, wnok, sfac, wdwarf (per set)
-        and flux [n_sets, n_times]."""
+        """One pass; returns dict with chisq, wnok, sfac, wdwarf (per set),
+        sfac5 [n_sets, 5] (per-component factors for iscale sets) and
+        flux [n_sets, n_times]."""
         ns, nt = self.n_sets, self.n_times
         chisq, wnok, sfac, wd = (np.zeros(ns) for _ in range(4))
         flux = np.zeros((ns, nt)) if want_flux else None
         check(_native.lib().lch_batch_run(_ctx(), self._h, int(bool(scale)), _dptr(chisq), _dptr(wnok),
                                           _dptr(sfac), _dptr(wd), None if flux is None else _dptr(flux)))
-        return {"chisq": chisq, "wnok": wnok, "sfac": sfac, "wdwarf": wd, "flux": flux}
+        sfac5 = np.zeros((ns, 5))
+        if ns:
+            check(_native.lib().lch_batch_sfac(self._h, _dptr(sfac5)))
+        return {"chisq": chisq, "wnok": wnok, "sfac": sfac, "wdwarf": wd, "sfac5": sfac5, "flux": flux}
 
     def set_obs(self, flux2d=None, ferr2d=None, weight2d=None):
         """Replace the observations of all sets ([n_sets, n_times] arrays)."""

@@ -78,6 +78,7 @@ SIGNATURES = {
     "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
     "lch_batch_autotune": (INT, [VP, VP, INT]),
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
+    "lch_batch_sfac": (INT, [VP, D]),
     "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong)]),
     "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
     "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),

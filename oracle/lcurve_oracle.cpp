@@ -10,7 +10,11 @@
 //   comp_disc/edge/spot   crates/lcurve/src/flux.rs:203-279
 //   exposure_weight       crates/lcurve/src/flux.rs:378-387 (lc_core.h)
 //   lensing_mag           crates/lcurve/src/flux.rs:390-406
-//   light_curve_comp      crates/lcurve/src/orchestration.rs:36-318 (iscale=0)
+//   light_curve_comp      crates/lcurve/src/orchestration.rs:36-318
+//   svd_scale             crates/lcurve/src/orchestration.rs:405-436 (the
+//                         reference calls nalgebra 0.33.2's SVD; restated
+//                         here as a one-sided Jacobi SVD, pinned against
+//                         numpy.linalg.lstsq in the tests)
 //   chisq_batch           crates/lcurve/src/orchestration.rs:329-357
 //   re_scale              crates/lcurve/src/orchestration.rs:361-388
 //   compute_chisq         crates/lcurve/src/orchestration.rs:391-401
@@ -214,6 +218,101 @@ double comp_star2(const FluxCtx& c, double phase, double expose, int ndiv, const
   return sum / static_cast<double>(std::max(1, ndiv - 1));
 }
 
+// crates/lcurve/src/flux.rs:203-279: comp_disc / comp_edge (same law) and
+// comp_spot, with the to_radians inclination.
+double comp_disc_like(const FluxCtx& c, double phase, double expose, int ndiv, const Grid& g,
+                      bool spot) {
+  double ri = to_radians(c.iangle);
+  double cosi = std::cos(ri), sini = std::sin(ri);
+  double sum = 0.0;
+  for (int nd = 0; nd < ndiv; ++nd) {
+    double phi, wgt;
+    exposure_weight(phase, expose, nd, ndiv, &phi, &wgt);
+    Vec3 earth = set_earth(cosi, sini, phi);
+    double ssum = 0.0;
+    for (const Point& pt : g) {
+      double mu = dot(earth, pt.dirn);
+      if (mu > 0.0 && pt.visible(phi)) {
+        if (spot) {
+          ssum += mu * static_cast<double>(pt.flux);
+        } else {
+          double ommu = 1.0 - mu;
+          ssum += mu * static_cast<double>(pt.flux) * (1.0 - ommu * (c.lin_limb_disc + c.quad_limb_disc * ommu));
+        }
+      }
+    }
+    sum += wgt * ssum;
+  }
+  return sum / static_cast<double>(std::max(1, ndiv - 1));
+}
+
+// Least squares data ~ sum_j sfac[j] * fcomp[i][j] over rows with weight > 0,
+// rows weighted by sqrt(weight) / ferr (orchestration.rs:405-436). The
+// minimum-norm solution through a one-sided Jacobi (Hestenes) SVD of the
+// weighted matrix, singular values <= 1e-12 treated as zero, as the
+// reference's svd.solve(&b, 1e-12) does.
+std::vector<double> svd_scale(const Data& data, const std::vector<std::vector<double>>& fcomp,
+                              int ncol) {
+  size_t nok = 0;
+  for (const Datum& d : data)
+    if (d.weight > 0.0) ++nok;
+  if (nok < static_cast<size_t>(ncol) || nok == 0) return std::vector<double>(ncol, 1.0);
+  std::vector<std::vector<double>> U(ncol, std::vector<double>(nok));
+  std::vector<double> b(nok);
+  size_t row = 0;
+  for (size_t i = 0; i < data.size(); ++i) {
+    const Datum& d = data[i];
+    if (!(d.weight > 0.0)) continue;
+    double w = std::sqrt(d.weight) / d.ferr;
+    for (int j = 0; j < ncol; ++j) U[j][row] = fcomp[i][j] * w;
+    b[row] = d.flux * w;
+    ++row;
+  }
+  std::vector<std::vector<double>> V(ncol, std::vector<double>(ncol, 0.0));
+  for (int j = 0; j < ncol; ++j) V[j][j] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < ncol; ++p) {
+      for (int q = p + 1; q < ncol; ++q) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (size_t i = 0; i < nok; ++i) {
+          alpha += U[p][i] * U[p][i];
+          beta += U[q][i] * U[q][i];
+          gamma += U[p][i] * U[q][i];
+        }
+        if (gamma == 0.0 || std::fabs(gamma) <= 1e-15 * std::sqrt(alpha * beta)) continue;
+        rotated = true;
+        double zeta = (beta - alpha) / (2.0 * gamma);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (size_t i = 0; i < nok; ++i) {
+          double up = U[p][i], uq = U[q][i];
+          U[p][i] = cs * up - sn * uq;
+          U[q][i] = sn * up + cs * uq;
+        }
+        for (int k = 0; k < ncol; ++k) {
+          double vp = V[k][p], vq = V[k][q];
+          V[k][p] = cs * vp - sn * vq;
+          V[k][q] = sn * vp + cs * vq;
+        }
+      }
+    }
+    if (!rotated) break;
+  }
+  std::vector<double> x(ncol, 0.0);
+  for (int j = 0; j < ncol; ++j) {
+    double sig = 0.0;
+    for (size_t i = 0; i < nok; ++i) sig += U[j][i] * U[j][i];
+    sig = std::sqrt(sig);
+    if (!(sig > 1e-12)) continue;
+    double ub = 0.0;
+    for (size_t i = 0; i < nok; ++i) ub += U[j][i] * b[i];
+    ub /= sig * sig;  // (u_j . b) / sigma_j with u_j = U[:, j] / sigma_j
+    for (int k = 0; k < ncol; ++k) x[k] += V[k][j] * ub;
+  }
+  return x;
+}
+
 // crates/lcurve/src/orchestration.rs:361-388
 void re_scale(const Data& data, std::vector<double>& fit, double* scale, double* chisq,
               double* wnok) {
@@ -261,6 +360,7 @@ void compute_chisq(const Data& data, const std::vector<double>& calc, double* ch
 
 struct LcOut {
   std::vector<double> calc;
+  std::vector<std::vector<double>> fcomp;  // iscale columns [n][ncol]
   double wdwarf = 0, chisq = 0, wnok = 0, logg1 = 0, logg2 = 0, rv1 = 0, rv2 = 0;
   double sfac[5] = {1, 1, 1, 1, 1};
   double t_setup = 0, t_flux = 0;
@@ -306,10 +406,9 @@ void parallel_for(size_t n, int nthreads, F&& f) {
 }
 
 // Flux + scale/chi-squared on an already prepared model
-// (orchestration.rs:85-131 and 181-310, iscale = 0).
+// (orchestration.rs:85-131 and 181-310).
 void flux_and_chisq(const Model& mdl, const PreparedModel& P, const Data& data, bool scale,
                     bool rdata, int nthreads, LcOut& out) {
-  if (mdl.iscale) throw LcError("oracle: iscale=1 is not restated");
   FluxCtx c = make_ctx(mdl, P);
   const Grid& s1f = P.star1f;
   const Grid& s1c = P.star1_alias ? P.star1f : P.star1c;
@@ -342,6 +441,43 @@ void flux_and_chisq(const Model& mdl, const PreparedModel& P, const Data& data, 
 
   size_t n = data.size();
   out.calc.assign(n, 0.0);
+  if (mdl.iscale) {
+    // orchestration.rs:181-227 and 262-297: per-component columns, each
+    // times slfac, no third light; SVD scale factors when scaling.
+    const int ncol = P.ncol;
+    out.fcomp.assign(n, std::vector<double>(ncol, 0.0));
+    parallel_for(n, nthreads, [&](size_t np) {
+      std::vector<double>& r = out.fcomp[np];
+      const double ph = P.phase[np], ex = P.expose[np], sl = P.slfac[np];
+      const int nd = P.ndiv[np];
+      r[0] = sl * comp_star1(c, ph, ex, nd, gint, s1f, s1c);
+      r[1] = sl * comp_disc_like(c, ph, ex, nd, P.disc, false);
+      r[2] = sl * comp_disc_like(c, ph, ex, nd, P.edge, false);
+      r[3] = sl * comp_disc_like(c, ph, ex, nd, P.spot, true);
+      if (ncol == 5) r[4] = sl * comp_star2(c, ph, ex, nd, gint, s2f, s2c);
+    });
+    out.wdwarf = comp_star1(c, 0.5, 0.0, 1, gint, s1f, s1c);
+    for (double& s : out.sfac) s = 1.0;
+    if (scale) {
+      std::vector<double> x = svd_scale(data, out.fcomp, ncol);
+      for (int j = 0; j < 5; ++j) out.sfac[j] = j < ncol ? x[j] : 0.0;
+    }
+    out.wdwarf *= out.sfac[0];
+    for (size_t np = 0; np < n; ++np) {
+      const std::vector<double>& r = out.fcomp[np];
+      double v = out.sfac[0] * r[0] + out.sfac[1] * r[1] + out.sfac[2] * r[2] + out.sfac[3] * r[3];
+      if (ncol == 5) v += out.sfac[4] * r[4];
+      out.calc[np] = v;
+    }
+    out.chisq = 0.0;
+    out.wnok = 0.0;
+    if (scale || rdata) compute_chisq(data, out.calc, &out.chisq, &out.wnok);
+    out.logg1 = P.logg1;
+    out.logg2 = P.logg2;
+    out.rv1 = P.rv1;
+    out.rv2 = P.rv2;
+    return;
+  }
   parallel_for(n, nthreads, [&](size_t np) {
     out.calc[np] = P.slfac[np] * comp_light(c, P.phase[np], P.expose[np], P.ndiv[np], gint, s1f, s2f,
                                             s1c, s2c, P.disc, P.edge, P.spot) +
@@ -443,6 +579,27 @@ int orc_light_curve(const char* model_text, const double* time, const double* ex
                     out.sfac[0], out.sfac[1], out.sfac[2], out.sfac[3], out.sfac[4], t1 - t0, t2 - t1};
     for (int i = 0; i < 14; ++i) scalars_out[i] = s[i];
     return 0;
+  } catch (const std::exception& e) {
+    copy_err(e.what(), err, errlen);
+    return -1;
+  }
+}
+
+// iscale columns of one model: fcomp_out[n * 5] (column 4 zero when
+// t2 <= 0), the unscaled inputs of svd_scale. Returns ncol, or -1.
+int orc_components(const char* model_text, const double* time, const double* expose,
+                   const double* flux, const double* ferr, const double* weight, const int* ndiv,
+                   int n, int nthreads, double* fcomp_out, char* err, int errlen) {
+  try {
+    Model mdl = Model::from_text(model_text);
+    mdl.iscale = true;
+    Data data = make_data(time, expose, flux, ferr, weight, ndiv, n);
+    auto P = prepare_model(mdl, data);
+    LcOut out;
+    flux_and_chisq(mdl, *P, data, false, false, nthreads, out);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < 5; ++j) fcomp_out[i * 5 + j] = j < P->ncol ? out.fcomp[i][j] : 0.0;
+    return P->ncol;
   } catch (const std::exception& e) {
     copy_err(e.what(), err, errlen);
     return -1;

@@ -28,6 +28,9 @@ def lib():
         L.orc_light_curve.restype = ctypes.c_int
         L.orc_light_curve.argtypes = [ctypes.c_char_p, _D, _D, _D, _D, _D, _I, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_char_p, ctypes.c_int]
+        L.orc_components.restype = ctypes.c_int
+        L.orc_components.argtypes = [ctypes.c_char_p, _D, _D, _D, _D, _D, _I, ctypes.c_int, ctypes.c_int, _D,
+                                     ctypes.c_char_p, ctypes.c_int]
         L.orc_batch_create.restype = ctypes.c_void_p
         L.orc_batch_create.argtypes = [ctypes.c_char_p, _D, _D, _D, _D, _D, _I, ctypes.c_int, ctypes.c_char_p,
                                        _D, ctypes.c_int, _D, _D, _D, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
@@ -88,6 +91,26 @@ def light_curve(model_text, times, expose=0.001, ndiv=1, flux=None, ferr=None, w
     out["t_setup"] = sc[12]
     out["t_flux"] = sc[13]
     return calc, out
+
+
+def components(model_text, times, expose=0.001, ndiv=1, flux=None, ferr=None, weight=None,
+               nthreads=1):
+    """The iscale columns (orchestration.rs:181-227) of a model, unscaled:
+    returns (fcomp [n, 5], ncol); column 4 is zero when t2 <= 0."""
+    t = _f(times)
+    n = len(t)
+    ex = _f(expose, n)
+    nd = np.ascontiguousarray(np.broadcast_to(ndiv, (n,)), dtype=np.int32)
+    fl = _f(np.zeros(n) if flux is None else flux, n)
+    fe = _f(np.ones(n) if ferr is None else ferr, n)
+    wt = _f(np.ones(n) if weight is None else weight, n)
+    out = np.zeros((n, 5))
+    err = ctypes.create_string_buffer(1024)
+    ncol = lib().orc_components(model_text.encode(), _d(t), _d(ex), _d(fl), _d(fe), _d(wt),
+                                nd.ctypes.data_as(_I), n, int(nthreads), _d(out), err, 1024)
+    if ncol < 0:
+        raise ValueError(err.value.decode())
+    return out, ncol
 
 
 class OracleBatch:

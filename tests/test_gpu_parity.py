@@ -15,7 +15,7 @@ import pytest
 import pyoracle
 from conftest import GOLDEN
 from lcurve_rs import workloads as W
-from variants import VARIANTS
+from variants import ISCALE_DISC, VARIANTS
 
 pytestmark = pytest.mark.gpu
 
@@ -260,10 +260,82 @@ def test_light_curve_scale_with_data(lcurve_rs, golden_model_text, tmp_path):
     assert res.wnok == len(t)
 
 
-def test_unsupported_iscale_raises(lcurve_rs, golden_model_text):
-    m = lcurve_rs.Model.from_text(W.model_text({"iscale": 1}, golden_model_text))
-    with pytest.raises(ValueError, match="iscale"):
-        m.light_curve(ntime=10)
+# ------------------------------------------------------------ iscale
+def _iscale_obs(text, t, truth, seed):
+    fcomp, ncol = pyoracle.components(text, t, nthreads=8)
+    clean = fcomp @ np.asarray(truth)
+    ferr = 0.003 * np.abs(clean)
+    y = clean + ferr * np.random.default_rng(seed).standard_normal(len(t))
+    w = np.ones(len(t))
+    w[::9] = 0.0
+    return y, ferr, w
+
+
+def _iscale_check(lc, text, t, expose=0.001, ndiv=1, scale=True, truth=(1.3, 0.7, 2.0, 0.5, 0.9)):
+    y, ferr, w = _iscale_obs(text, t, truth, 11)
+    r, sc = pyoracle.light_curve(text, t, expose=expose, ndiv=ndiv, flux=y, ferr=ferr, weight=w,
+                                 scale=scale, rdata=True, nthreads=8)
+    nd = np.full(len(t), ndiv, dtype=np.int32)
+    m = lc.Model.from_text(text)
+    calc = np.zeros(len(t))
+    res = lc._native.LcResultC()
+    lc._native.check(lc._native.lib().lch_light_curve(
+        lc._ctx(), m._h, lc._dptr(np.ascontiguousarray(t, dtype=np.float64)),
+        lc._dptr(np.full(len(t), expose)), lc._dptr(y), lc._dptr(ferr), lc._dptr(w), lc._iptr(nd),
+        len(t), int(scale), 1, lc._dptr(calc), lc.ctypes.byref(res)))
+    assert max_rel(calc, r) <= FLUX_TOL
+    assert res.chisq == pytest.approx(sc["chisq"], rel=CHISQ_TOL)
+    assert res.wnok == sc["wnok"]
+    assert np.allclose(list(res.sfac), sc["sfac"], rtol=1e-5, atol=1e-12)
+    assert res.wdwarf == pytest.approx(sc["wdwarf"], rel=FLUX_TOL)
+    return res, sc
+
+
+@pytest.mark.parametrize("scale", [True, False])
+def test_iscale_components_and_svd_factors(lcurve_rs, golden_model_text, scale):
+    """Per-component columns and their SVD least-squares factors
+    (orchestration.rs:181-297, 405-436) against the oracle, whose SVD is
+    pinned to numpy.linalg.lstsq."""
+    text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    res, sc = _iscale_check(lcurve_rs, text, W.linspace_times(-0.2, 1.2, 240), scale=scale)
+    if scale:
+        assert np.allclose(sc["sfac"], [1.3, 0.7, 2.0, 0.5, 0.9], rtol=0.05)
+    else:
+        assert list(res.sfac) == [1.0] * 5
+
+
+def test_iscale_smeared_exposures(lcurve_rs, golden_model_text):
+    """ndiv > 1: comp_star1 weights by scale1(phase), comp_star2 by scale2(phi)."""
+    text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    _iscale_check(lcurve_rs, text, W.linspace_times(-0.2, 1.2, 120), expose=0.01, ndiv=5)
+
+
+def test_iscale_absent_components(lcurve_rs, golden_model_text):
+    """No disc, edge or spot: zero columns get factor exactly 0."""
+    text = W.model_text({"iscale": 1}, golden_model_text)
+    res, _ = _iscale_check(lcurve_rs, text, W.linspace_times(-0.2, 1.2, 200), truth=(1.1, 0, 0, 0, 0.8))
+    assert list(res.sfac)[1:4] == [0.0, 0.0, 0.0]
+
+
+def test_iscale_in_chisq_batch(lcurve_rs, golden_model_text):
+    """Mixed batch: iscale sets fit per-component factors, the others autoscale."""
+    t = W.linspace_times(-0.2, 1.2, 160)
+    base = W.model_text(ISCALE_DISC, golden_model_text)
+    texts = [W.model_text({"iscale": k % 2, "q": 0.1 + 0.02 * k}, base) for k in range(6)]
+    y, ferr, w = _iscale_obs(W.model_text({"iscale": 1}, base), t, (1.2, 0.8, 1.5, 0.6, 1.1), 3)
+    b = lcurve_rs.Batch([lcurve_rs.Model.from_text(x) for x in texts], t, 0.001, 1)
+    b.set_obs(np.tile(y, (6, 1)), np.tile(ferr, (6, 1)), np.tile(w, (6, 1)))
+    out = b.run(scale=True)
+    ob = pyoracle.OracleBatch(texts, t, 0.001, 1, flux2d=np.tile(y, (6, 1)), ferr2d=np.tile(ferr, (6, 1)),
+                              weight2d=np.tile(w, (6, 1)), nthreads=8)
+    chi, flux = ob.flux(scale=True, nthreads=8)
+    assert np.allclose(out["chisq"], chi, rtol=CHISQ_TOL)
+    assert np.max(np.abs(out["flux"] - flux) / np.abs(flux)) <= FLUX_TOL
+    for k in range(6):
+        if k % 2 == 0:
+            assert np.all(out["sfac5"][k] == out["sfac"][k])
+        else:
+            assert len(set(out["sfac5"][k])) > 1
 
 
 def test_setup_error_raises(lcurve_rs, golden_model_text):

@@ -316,3 +316,80 @@ def test_phase_newton_with_pdot_inverts_ephemeris():
     phi_true = np.linspace(-3.0, 40.0, 50)
     t = 0.3 + phi_true * (0.2 + 1e-6 * phi_true)
     assert np.allclose(pyoracle.phases(txt, t), phi_true, atol=1e-10)
+
+
+# ------------------------------------------------------------ iscale (SVD)
+from variants import ISCALE_DISC  # noqa: E402
+
+
+def _weighted_lstsq(fcomp, ncol, y, ferr, w):
+    ok = w > 0
+    sw = np.sqrt(w[ok]) / ferr[ok]
+    A = fcomp[ok, :ncol] * sw[:, None]
+    b = y[ok] * sw
+    return np.linalg.lstsq(A, b, rcond=None)[0]
+
+
+def test_iscale_svd_matches_numpy_lstsq(golden_model_text):
+    """svd_scale (orchestration.rs:405-436) against LAPACK's SVD least squares:
+    observations built from known per-component factors plus noise."""
+    text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    t = W.linspace_times(-0.2, 1.2, 240)
+    fcomp, ncol = pyoracle.components(text, t, nthreads=4)
+    assert ncol == 5 and np.all(np.abs(fcomp).sum(0) > 0)
+    truth = np.array([1.3, 0.7, 2.0, 0.5, 0.9])
+    rng = np.random.default_rng(7)
+    clean = fcomp @ truth
+    ferr = 0.003 * np.abs(clean)
+    y = clean + ferr * rng.standard_normal(len(t))
+    w = np.ones(len(t))
+    w[::7] = 0.0  # masked points take no part
+    calc, sc = pyoracle.light_curve(text, t, flux=y, ferr=ferr, weight=w, scale=True, nthreads=4)
+    ref = _weighted_lstsq(fcomp, ncol, y, ferr, w)
+    assert np.allclose(sc["sfac"], ref, rtol=1e-9, atol=0)
+    assert np.allclose(calc, fcomp @ sc["sfac"], rtol=1e-12)
+    r = (y - calc) / ferr
+    assert sc["chisq"] == pytest.approx(np.sum(w * r * r), rel=1e-12)
+    assert sc["wnok"] == pytest.approx(w.sum())
+
+
+def test_iscale_absent_components_get_zero_factor(golden_model_text):
+    """No disc, edge or spot: those columns are zero, their singular values
+    are 0 and the minimum-norm solution gives them factor 0, as LAPACK does."""
+    text = W.model_text({"iscale": 1}, golden_model_text)
+    t = W.linspace_times(-0.2, 1.2, 200)
+    fcomp, ncol = pyoracle.components(text, t, nthreads=4)
+    assert np.all(fcomp[:, 1:4] == 0.0)
+    y = fcomp @ np.array([1.1, 0.0, 0.0, 0.0, 0.8])
+    ferr = 0.01 * y
+    calc, sc = pyoracle.light_curve(text, t, flux=y, ferr=ferr, weight=np.ones(200), scale=True, nthreads=4)
+    ref = _weighted_lstsq(fcomp, ncol, y, ferr, np.ones(200))
+    assert np.allclose(sc["sfac"], ref, rtol=1e-9, atol=1e-12)
+    assert sc["sfac"][0] == pytest.approx(1.1, rel=1e-9) and sc["sfac"][4] == pytest.approx(0.8, rel=1e-9)
+    assert np.all(sc["sfac"][1:4] == 0.0)
+    assert sc["chisq"] < 1e-12 * len(t)
+
+
+def test_iscale_unscaled_columns_sum_to_the_light_curve(golden_model_text):
+    """With ndiv = 1, no beaming and no third light the unscaled columns add
+    up to comp_light; the only differences are the inclination's
+    TWOPI*i/360 vs to_radians (last-bit) and Ginterp's scale1(phase) vs
+    scale1(phi), equal when ndiv = 1."""
+    text = W.model_text(ISCALE_DISC, golden_model_text)
+    t = W.linspace_times(-0.2, 1.2, 150)
+    plain, _ = pyoracle.light_curve(text, t, nthreads=4)
+    fcomp, ncol = pyoracle.components(text, t, nthreads=4)
+    assert np.allclose(fcomp.sum(1), plain, rtol=1e-12)
+    text_i = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    calc, sc = pyoracle.light_curve(text_i, t, nthreads=4)
+    assert np.array_equal(calc, fcomp[:, 0] + fcomp[:, 1] + fcomp[:, 2] + fcomp[:, 3] + fcomp[:, 4])
+    assert np.all(sc["sfac"] == 1.0)
+
+
+def test_iscale_too_few_points_gives_unit_factors(golden_model_text):
+    text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    t = W.linspace_times(-0.2, 1.2, 8)
+    w = np.zeros(8)
+    w[:4] = 1.0  # 4 usable points < 5 columns
+    calc, sc = pyoracle.light_curve(text, t, flux=np.ones(8), ferr=np.ones(8), weight=w, scale=True)
+    assert np.all(sc["sfac"] == 1.0)

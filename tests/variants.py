@@ -29,3 +29,9 @@ VARIANTS = {
     "roche_primary": {"roche1": 1, "r1": 0.2, "nlat1f": 16, "nlat1c": 10, "nlat2f": 24, "nlat2c": 16},
     "no_eclipses": {"eclipse1": 0, "eclipse2": 0},
 }
+
+
+# Disc, edge and bright spot for the per-component (iscale) fits.
+ISCALE_DISC = {"add_disc": 1, "opaque": 0, "rdisc1": 0.1, "rdisc2": 0.3, "nrad": 30, "temp_disc": 6000,
+               "texp_disc": -0.75, "temp_edge": 5000, "absorb_edge": 0.5, "add_spot": 1, "nspot": 50,
+               "radius_spot": 0.3, "expon_spot": 1.0}
