@@ -165,6 +165,37 @@ int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, d
 int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
                          int nphase, unsigned char* mask, int* nelem);
 
+/* ---- learned surrogate ----
+ * No counterpart in the reference. A multilayer perceptron (GELU hidden
+ * layers, linear output) trained and evaluated on the GPU by BF16 MFMA
+ * GEMMs with FP32 accumulation, Adam on FP32 master weights. It maps model
+ * parameters to phase-binned light curves and screens MCMC proposals;
+ * the exact flux path above stays the judge of every accepted sample.
+ * dims = {in, h1, ..., out}: in <= 64, the others multiples of 64;
+ * max_batch a multiple of 64. Parameters are ordered W0 [out x in], b0,
+ * W1, b1, ... (row-major, FP32). */
+typedef struct lch_mlp lch_mlp;
+int lch_mlp_create(lch_ctx* ctx, const int* dims, int ndims, unsigned long long seed, int max_batch,
+                   lch_mlp** out);
+void lch_mlp_free(lch_mlp* m);
+long long lch_mlp_param_count(const lch_mlp* m);
+int lch_mlp_get_params(const lch_mlp* m, float* out);
+int lch_mlp_set_params(lch_mlp* m, const float* in);
+/* Resident training set, x [n x in] and y [n x out]. */
+int lch_mlp_set_data(lch_mlp* m, const float* x, const float* y, int n);
+/* One Adam step per `batch` indices of perm[n] (a trailing partial batch is
+ * dropped); mean_loss = mean over steps of the batch MSE. */
+int lch_mlp_train_epoch(lch_mlp* m, const int* perm, int n, int batch, float lr, double* mean_loss);
+/* MSE loss and its gradient (parameter order) for n rows (n a multiple of
+ * 64, at most max_batch), without updating; grads may be null. */
+int lch_mlp_loss_grad(lch_mlp* m, const float* x, const float* y, int n, double* loss, float* grads);
+/* Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradients. */
+int lch_mlp_adam_step(lch_mlp* m, float lr);
+int lch_mlp_predict(lch_mlp* m, const float* x, int n, float* y);
+/* Mean ms of `reps` training steps on the resident set and of their GEMMs
+ * alone (HIP events), and the GEMM flops of one step. */
+int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop);
+
 /* ---- diagnostics (CPU only, no device needed) ---- */
 /* Packs one model as the engine does and re-evaluates, in FP64 on the CPU,
  * every (element, lane) pair twice: as the per-(patch, tile) work codes

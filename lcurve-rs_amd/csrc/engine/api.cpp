@@ -9,6 +9,7 @@
 
 #include "engine.h"
 #include "selfcheck.h"
+#include "../surrogate/mlp.h"
 
 struct lch_ctx {
   lch::Context impl;
@@ -16,6 +17,9 @@ struct lch_ctx {
 };
 struct lch_model {
   lc::Model impl;
+};
+struct lch_mlp {
+  std::unique_ptr<lch::Mlp> impl;
 };
 struct lch_batch {
   std::unique_ptr<lch::Batch> impl;
@@ -457,4 +461,88 @@ int lch_batch_sfac(const lch_batch* b, double* sfac5) {
   for (size_t s = 0; s < o.size(); ++s)
     for (int j = 0; j < 5; ++j) sfac5[s * 5 + j] = o[s].sfac[j];
   return LCH_OK;
+}
+
+// ---------------------------------------------------------------- surrogate
+int lch_mlp_create(lch_ctx* ctx, const int* dims, int ndims, unsigned long long seed, int max_batch,
+                   lch_mlp** out) {
+  if (!ctx || !dims || !out || ndims < 2) return fail(LCH_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    auto m = std::make_unique<lch_mlp>();
+    m->impl = std::make_unique<lch::Mlp>(ctx->impl, std::vector<int>(dims, dims + ndims), seed, max_batch);
+    *out = m.release();
+    return LCH_OK;
+  });
+}
+
+void lch_mlp_free(lch_mlp* m) { delete m; }
+
+long long lch_mlp_param_count(const lch_mlp* m) {
+  return m ? static_cast<long long>(m->impl->param_count()) : -1;
+}
+
+int lch_mlp_get_params(const lch_mlp* m, float* out) {
+  if (!m || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    m->impl->get_params(out);
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_set_params(lch_mlp* m, const float* in) {
+  if (!m || !in) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    m->impl->set_params(in);
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_set_data(lch_mlp* m, const float* x, const float* y, int n) {
+  if (!m || !x || !y) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    m->impl->set_data(x, y, n);
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_train_epoch(lch_mlp* m, const int* perm, int n, int batch, float lr, double* mean_loss) {
+  if (!m || !perm || n < 0) return fail(LCH_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    const double l = m->impl->train_epoch(perm, n, batch, lr);
+    if (mean_loss) *mean_loss = l;
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_loss_grad(lch_mlp* m, const float* x, const float* y, int n, double* loss, float* grads) {
+  if (!m || !x || !y) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const double l = m->impl->loss_and_grad(x, y, n, grads);
+    if (loss) *loss = l;
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_adam_step(lch_mlp* m, float lr) {
+  if (!m) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    m->impl->adam_step(lr);
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_predict(lch_mlp* m, const float* x, int n, float* y) {
+  if (!m || (n > 0 && (!x || !y))) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    m->impl->predict(x, n, y);
+    return LCH_OK;
+  });
+}
+
+int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop) {
+  if (!m || reps <= 0 || !ms_step || !ms_gemm || !gemm_flop) return fail(LCH_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    m->impl->time_step(batch, reps, ms_step, ms_gemm, gemm_flop);
+    return LCH_OK;
+  });
 }

@@ -244,6 +244,8 @@ void Batch::build_items(uint32_t chunk) {
   // Items cover whole element patches, at most 64 of them.
   chunk = std::max<uint32_t>(PATCH, (chunk + PATCH - 1) / PATCH * PATCH);
   chunk = std::min<uint32_t>(chunk, MAX_ITEM_PATCHES * PATCH);
+  // Launches in flight still read the tables replaced below.
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
   release_graph();
   chunk_ = chunk;
   chunk_info_.clear();
@@ -457,6 +459,7 @@ void Batch::visibility(int set, int grid, const double* phases, int nphase, uint
   if (!mask || nphase <= 0 || gm.n == 0) return;
   if (gm.orig.size() != gm.segs.size()) throw std::invalid_argument("element maps were not kept for this batch");
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
   double* dph = nullptr;
   hip_check(hipMalloc(reinterpret_cast<void**>(&dph), nphase * sizeof(double)), "hipMalloc");
   hip_check(hipMemcpy(dph, phases, nphase * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");

@@ -1,0 +1,238 @@
+// HIP kernels of the light-curve surrogate MLP for CDNA4 (gfx950).
+//
+// gemm_kernel — C[M x N] = A[M x K] . B[K x N] on BF16 MFMA
+//   (v_mfma_f32_32x32x16_bf16) with FP32 accumulation. A workgroup of four
+//   wavefronts computes a 64 x 64 tile, each wave a 32 x 32 quarter; K is
+//   staged through LDS in steps of 32 with both operands stored K-contiguous
+//   there, so every lane's MFMA operand is one 16-byte LDS read. Operands
+//   may arrive K-contiguous or transposed in global memory (the three GEMMs
+//   of a layer's forward and backward passes need both), and the epilogue is
+//   fused: bias + GELU for hidden layers, bias only for the output layer,
+//   GELU' for the backward pass, plain FP32 for weight gradients.
+// loss_kernel / bias_grad_kernel / adam_kernel / pack_kernel — the MSE
+//   loss and its gradient, bias gradients (column sums), Adam on FP32 master
+//   weights writing the BF16 copy, and FP32 -> BF16 packing.
+//
+// Shapes are checked on the host: M, N multiples of 64, K a multiple of 32,
+// leading dimensions multiples of 8 elements.
+#include <hip/hip_runtime.h>
+
+#include "mlp_kernels.h"
+
+namespace lch {
+namespace mlp {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row; the pad breaks bank conflicts
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Round-to-nearest-even FP32 -> BF16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+// GELU with the error function, and its derivative.
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// Stage one BM x BK (or BN x BK) operand tile into LDS as [row][k].
+// KC: global element (row, k) at g[row * ld + k]; else at g[k * ld + row].
+template <bool KC>
+__device__ __forceinline__ void stage(uint16_t* __restrict__ s, const uint16_t* __restrict__ g,
+                                      int ld, int row0, int k0) {
+  const int t = threadIdx.x;
+  if (KC) {
+    const int r = t >> 2, kk = (t & 3) * 8;
+    const uint4 v = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(row0 + r) * ld + k0 + kk);
+    *reinterpret_cast<uint4*>(s + r * LDS_STRIDE + kk) = v;
+  } else {
+    const int kk = t >> 3, r = (t & 7) * 8;
+    const uint4 v = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(k0 + kk) * ld + row0 + r);
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[(r + i) * LDS_STRIDE + kk] = h[i];
+  }
+}
+
+}  // namespace
+
+template <bool AKC, bool BKC, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < a.K; k0 += BK) {
+    stage<AKC>(As, a.A, a.lda, m0, k0);
+    stage<BKC>(Bs, a.B, a.ldb, n0, k0);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < BK / 16; ++s) {
+      const int kk = s * 16 + (l >> 5) * 8;
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(As + (wm + (l & 31)) * LDS_STRIDE + kk);
+      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs + (wn + (l & 31)) * LDS_STRIDE + kk);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // Lane l holds C[(r / 4) * 8 + (l / 32) * 4 + r % 4][l % 32] of the wave's
+  // 32 x 32 quarter in acc[r].
+  const int n = n0 + wn + (l & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + wm + (r >> 2) * 8 + (l >> 5) * 4 + (r & 3);
+    const size_t idx = static_cast<size_t>(m) * a.ldc + n;
+    const float v = acc[r];
+    if (EPI == EPI_BIAS_GELU) {
+      const float z = v + a.bias[n];
+      a.Cf[idx] = z;
+      a.Cb[idx] = f2bf(gelu(z));
+    } else if (EPI == EPI_BIAS) {
+      a.Cf[idx] = v + a.bias[n];
+    } else if (EPI == EPI_GELU_GRAD) {
+      a.Cb[idx] = f2bf(v * gelu_grad(a.Z[idx]));
+    } else {
+      a.Cf[idx] = v;
+    }
+  }
+}
+
+// MSE over the batch and outputs: loss = mean((y - t)^2); dy = 2 (y - t) / count
+// in BF16 for the backward GEMMs. Per-block partial sums in partial[].
+__global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                                                   uint16_t* __restrict__ dy, size_t count,
+                                                   double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0.0;
+  const float g = 2.0f / static_cast<float>(count);
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x) {
+    const float d = y[i] - t[i];
+    s += static_cast<double>(d) * d;
+    if (dy) dy[i] = f2bf(g * d);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// db[n] = sum over rows of dz[row][n] (BF16 in, FP32 out); one thread per column.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ dz, int rows, int cols,
+                                                        float* __restrict__ db) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= cols) return;
+  float s = 0.0f;
+  for (int r = 0; r < rows; ++r) s += bf2f(dz[static_cast<size_t>(r) * cols + n]);
+  db[n] = s;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; writes the BF16 copy when
+// wb is non-null. mask, when non-null, freezes entries whose mask is 0
+// (padding columns of the first layer).
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v,
+                                                   uint16_t* __restrict__ wb,
+                                                   const uint8_t* __restrict__ mask, size_t n,
+                                                   AdamParams p) {
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i >= n) return;
+  if (mask && !mask[i]) return;
+  const float gi = g[i];
+  const float mi = p.beta1 * m[i] + (1.0f - p.beta1) * gi;
+  const float vi = p.beta2 * v[i] + (1.0f - p.beta2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / p.bc1, vh = vi / p.bc2;
+  const float wi = w[i] - p.lr * mh / (sqrtf(vh) + p.eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+// Minibatch gather: xb[b][0..din_pad) = bf16(X[idx[b]][0..din)), zero
+// padded; yb[b][:] = Y[idx[b]][:].
+__global__ __launch_bounds__(256) void gather_kernel(const float* __restrict__ X, const float* __restrict__ Y,
+                                                     const int* __restrict__ idx, int batch, int din,
+                                                     int din_pad, int dout, uint16_t* __restrict__ xb,
+                                                     float* __restrict__ yb) {
+  const int b = blockIdx.x;
+  if (b >= batch) return;
+  const size_t r = static_cast<size_t>(idx[b]);
+  for (int j = threadIdx.x; j < din_pad; j += 256)
+    xb[static_cast<size_t>(b) * din_pad + j] = j < din ? f2bf(X[r * din + j]) : static_cast<uint16_t>(0);
+  if (Y)
+    for (int j = threadIdx.x; j < dout; j += 256) yb[static_cast<size_t>(b) * dout + j] = Y[r * dout + j];
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, uint16_t* __restrict__ y,
+                                                   size_t n) {
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// ------------------------------------------------------------ launchers
+template <bool AKC, bool BKC, int EPI>
+static hipError_t launch_gemm_t(const GemmArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL((gemm_kernel<AKC, BKC, EPI>), dim3(a.N / BN, a.M / BM), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(const GemmArgs& a, bool a_kc, bool b_kc, int epi, hipStream_t st) {
+  if (a.M % BM || a.N % BN || a.K % BK || a.M <= 0 || a.N <= 0 || a.K <= 0) return hipErrorInvalidValue;
+  if (a.lda % 8 || a.ldb % 8) return hipErrorInvalidValue;
+#define LCH_GEMM_CASE(AK, BK_, E) \
+  if (a_kc == AK && b_kc == BK_ && epi == E) return launch_gemm_t<AK, BK_, E>(a, st);
+  LCH_GEMM_CASE(true, true, EPI_BIAS_GELU)
+  LCH_GEMM_CASE(true, true, EPI_BIAS)
+  LCH_GEMM_CASE(true, true, EPI_PLAIN)
+  LCH_GEMM_CASE(true, false, EPI_GELU_GRAD)
+  LCH_GEMM_CASE(true, false, EPI_PLAIN)
+  LCH_GEMM_CASE(false, false, EPI_PLAIN)
+  LCH_GEMM_CASE(false, true, EPI_PLAIN)
+#undef LCH_GEMM_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_loss(const float* y, const float* t, uint16_t* dy, size_t count, double* partial,
+                       int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(loss_kernel, dim3(blocks), dim3(256), 0, st, y, t, dy, count, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* db, hipStream_t st) {
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, dz, rows, cols, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, const uint8_t* mask,
+                       size_t n, const AdamParams& p, hipStream_t st) {
+  hipLaunchKernelGGL(adam_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, w, g, m, v,
+                     wb, mask, n, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const float* X, const float* Y, const int* idx, int batch, int din, int din_pad,
+                         int dout, uint16_t* xb, float* yb, hipStream_t st) {
+  hipLaunchKernelGGL(gather_kernel, dim3(batch), dim3(256), 0, st, X, Y, idx, batch, din, din_pad, dout, xb, yb);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* x, uint16_t* y, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(pack_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, x, y, n);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace lch

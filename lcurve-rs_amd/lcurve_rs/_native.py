@@ -21,6 +21,7 @@ LCH_ERR_NOMEM = -7
 LCH_BATCH_KEEP_MAPS = 1
 
 D = ctypes.POINTER(ctypes.c_double)
+F = ctypes.POINTER(ctypes.c_float)
 I = ctypes.POINTER(ctypes.c_int)
 VP = ctypes.c_void_p
 CP = ctypes.c_char_p
@@ -80,6 +81,17 @@ SIGNATURES = {
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
     "lch_batch_sfac": (INT, [VP, D]),
     "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong)]),
+    "lch_mlp_create": (INT, [VP, I, INT, ctypes.c_ulonglong, INT, ctypes.POINTER(VP)]),
+    "lch_mlp_free": (None, [VP]),
+    "lch_mlp_param_count": (ctypes.c_longlong, [VP]),
+    "lch_mlp_get_params": (INT, [VP, F]),
+    "lch_mlp_set_params": (INT, [VP, F]),
+    "lch_mlp_set_data": (INT, [VP, F, F, INT]),
+    "lch_mlp_train_epoch": (INT, [VP, I, INT, INT, ctypes.c_float, D]),
+    "lch_mlp_loss_grad": (INT, [VP, F, F, INT, D, F]),
+    "lch_mlp_adam_step": (INT, [VP, ctypes.c_float]),
+    "lch_mlp_predict": (INT, [VP, F, INT, F]),
+    "lch_mlp_time_step": (INT, [VP, INT, INT, D, D, D]),
     "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
     "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),
 }
