@@ -11,6 +11,8 @@ and is reported separately). The metric is chi-squared evaluations per
 second.
 
   python bench.py [--steps K] [--warmup W] [--workload chisq_4096|wd_donor_45k|disc_spot_300k]
+  python bench.py --workload surrogate_200k   # learned surrogate: BF16 MFMA training
+  python bench.py --workload da_mcmc          # delayed-acceptance vs plain exact MCMC
 
 Prints one JSON line (plus progress lines on stderr).
 """
@@ -185,6 +187,139 @@ def roofline(workload, st, ms_flux):
             "hbm_frac": round((traffic or alg_bytes) / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
+
+
+def bench_surrogate(args):
+    """Surrogate training: curves from the exact GPU method over the
+    parameter box, folded to 256 phase bins (log flux / wdwarf); the
+    5-4x512-256 GELU MLP trained by the BF16 MFMA kernels; held-out error
+    against the exact method; MFMA utilisation of the training GEMMs."""
+    from lcurve_rs import mcmc
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    base = mcmc.surrogate_base_text()
+    n = args.ncurves
+    params = mcmc.sample_box(n, 42)
+    log(f"generating {n} curves with the exact GPU method ...")
+    t0 = time.perf_counter()
+    curves = mcmc.make_training_set(base, params, 256, log=log)
+    t_gen = time.perf_counter() - t0
+    ok = np.all(np.isfinite(curves), axis=1)
+    P, C = params[ok], curves[ok]
+    nh = max(1024, len(P) // 20)
+    s = mcmc.Surrogate(256, seed=1)
+    log(f"training on {len(P) - nh} curves, {args.epochs} epochs ...")
+    fit = s.fit(P[nh:], C[nh:], epochs=args.epochs, log=log)
+    err = s.predict(P[:nh]) - C[:nh]
+    ae = np.abs(err)
+    tm = s.net.time_step(4096, reps=50)
+    tflops = tm["gemm_flop"] / (tm["ms_gemm"] * 1e-3) / 1e12
+    value = 4096.0 / (tm["ms_step"] * 1e-3)
+    rec = {"metric": "surrogate training samples/s (1 GPU)", "value": round(value, 1), "unit": "samples/s",
+           "n_gpus": 1, "steps": 50, "warmup": 1, "ms_per_step": round(tm["ms_step"], 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f32",
+           "data": "synthetic: curves from the exact GPU method over the parameter box",
+           "config": {"workload": "surrogate_200k", "curves": n, "bins": 256, "mlp": "5-512x4-256 GELU",
+                      "batch": 4096, "epochs": args.epochs,
+                      "box": {k: list(v) for k, v in zip(mcmc.PARAM_NAMES, mcmc.BOX.tolist())},
+                      "grid_nlat": [20, 20, 40, 40]},
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None,
+                        "kernel": "gemm_kernel", "gemm_ms_per_step": round(tm["ms_gemm"], 4),
+                        "gemm_flop_per_step": tm["gemm_flop"]},
+           "heldout_vs_exact": {"curves": int(nh), "rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
+                                "median_abs": float(np.median(ae)), "p99_abs": float(np.percentile(ae, 99)),
+                                "max_abs": float(ae.max())},
+           "data_generation_s": round(t_gen, 1), "train_s": round(fit["seconds"], 2),
+           "final_train_loss": fit["loss"][-1], "failed_setups": int((~ok).sum())}
+    if not args.no_cpu_baseline:
+        import mlp_oracle
+        dims = (5, 512, 512, 512, 512, 256)
+        p = s.net.get_params().astype(np.float64)
+        xs = mcmc.to_unit(P[nh:nh + 256]).astype(np.float64)
+        ys = (C[nh:nh + 256] - mcmc.quad_features(mcmc.to_unit(P[nh:nh + 256])) @ s.coef) / s.y_std
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            mlp_oracle.loss_and_grad(p, dims, xs, ys, bf16_mode=False)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        rec["cpu_baseline"] = {"value": round(256 / dt, 1), "unit": "samples/s", "cores": HOST_THREADS,
+                               "kind": "port",
+                               "sample": f"{reps} x 256-row forward+backward in FP64 numpy (oracle/mlp_oracle.py)"}
+    print(json.dumps(rec), flush=True)
+
+
+def bench_da_mcmc(args):
+    """Delayed acceptance (surrogate transition) vs plain exact ensemble
+    MCMC on a synthetic 3000-point, 0.5%-noise curve of the one-binary
+    geometry: pilot run, surrogate trained on the pilot posterior +-10 sigma,
+    then both samplers from the same equilibrated walkers."""
+    from lcurve_rs import mcmc
+    base = mcmc.surrogate_base_text()
+    truth = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
+    times = np.linspace(-0.2, 1.2, 3000)
+    tg = mcmc.Target(base, truth, times, args.noise, seed=42)
+    rng = np.random.default_rng(1)
+    nw = args.walkers
+    p0 = truth * (1.0 + 1e-4 * rng.standard_normal((nw, 5)))
+    log(f"pilot: {nw} walkers x {args.pilot_steps} exact steps ...")
+    pilot = mcmc.ensemble_mcmc(tg, p0, args.pilot_steps, seed=1)
+    tail = pilot["chain"][args.pilot_steps // 2:].reshape(-1, 5)
+    box = mcmc.local_box(tail.mean(0), tail.std(0), 10.0)
+    log(f"training set: {args.ncurves} curves over the pilot posterior +-10 sigma ...")
+    t0 = time.perf_counter()
+    params = mcmc.sample_box(args.ncurves, 7, box)
+    phases = mcmc.eclipse_phases(256, 192, 0.06)
+    curves = mcmc.make_training_set(base, params, log=log, phases=phases)
+    t_gen = time.perf_counter() - t0
+    ok = np.all(np.isfinite(curves), axis=1)
+    P, C = params[ok], curves[ok]
+    nh = max(512, len(P) // 20)
+    s = mcmc.Surrogate(seed=1, box=box, phases=phases)
+    fit = s.fit(P[nh:], C[nh:], epochs=args.epochs, log=log)
+    err = s.predict(P[:nh]) - C[:nh]
+    start = pilot["chain"][-1].copy()
+    log(f"plain exact MCMC: {args.steps} steps ...")
+    plain = mcmc.ensemble_mcmc(tg, start, args.steps, seed=2)
+    log(f"delayed acceptance, {args.inner} surrogate moves per exact check: {args.steps} steps ...")
+    da = mcmc.ensemble_mcmc(tg, start, args.steps, seed=3, surrogate=s, inner_steps=args.inner)
+    lp_s = tg.logp_surrogate(s, da["chain"][-1])
+    lp_e = tg.logp_exact(da["chain"][-1])
+    ess_p, tau_p = mcmc.ess(plain["chain"], burn=0.1)
+    ess_d, tau_d = mcmc.ess(da["chain"], burn=0.1)
+    sp, sd = mcmc.summarize(plain["chain"], 0.1), mcmc.summarize(da["chain"], 0.1)
+    sig = 0.5 * (sp[2] - sp[0])
+    value = ess_d / da["seconds"]
+    base_v = ess_p / plain["seconds"]
+    rec = {"metric": "MCMC effective samples/s (1 GPU), delayed acceptance", "value": round(value, 3),
+           "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.pilot_steps,
+           "ms_per_step": round(1e3 * da["seconds"] / args.steps, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32/f64 exact, bf16 surrogate",
+           "data": "synthetic target curve (3000 times, 0.5% noise) of the one-binary geometry",
+           "config": {"workload": "da_mcmc", "walkers": nw, "steps": args.steps, "inner_steps": args.inner,
+                      "free": list(mcmc.PARAM_NAMES), "truth": truth.tolist(), "grid_nlat": [20, 20, 40, 40],
+                      "surrogate_curves": args.ncurves, "epochs": args.epochs, "noise": args.noise,
+                      "output_phases": "192 in |phase| < 0.06, 64 elsewhere"},
+           "plain": {"ess_per_s": round(base_v, 3), "ess": round(float(ess_p), 1), "tau": tau_p.round(2).tolist(),
+                     "seconds": round(plain["seconds"], 1), "n_exact": plain["n_exact"],
+                     "acceptance": round(plain["accepted"] / plain["proposals"], 3)},
+           "delayed_acceptance": {"ess_per_s": round(value, 3), "ess": round(float(ess_d), 1),
+                                  "tau": tau_d.round(2).tolist(), "seconds": round(da["seconds"], 1),
+                                  "n_exact": da["n_exact"], "acceptance": round(da["accepted"] / da["proposals"], 3),
+                                  "moved_to_exact_check": da["stage1_passed"], "fallback_at": da["fallback_at"]},
+           "speedup_ess_per_s": round(value / base_v, 2),
+           "posterior": {"names": list(mcmc.PARAM_NAMES), "plain_p16_p50_p84": sp.tolist(),
+                         "da_p16_p50_p84": sd.tolist(),
+                         "max_median_shift_in_sigma": float(np.max(np.abs(sd[1] - sp[1]) / sig)),
+                         "width_ratio": (0.5 * (sd[2] - sd[0]) / sig).round(3).tolist()},
+           "surrogate": {"box": box.tolist(), "heldout_rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
+                         "loglike_error_spread_at_end": float(np.std(lp_s - lp_e)),
+                         "train_s": round(fit["seconds"], 2), "data_generation_s": round(t_gen, 1)},
+           "pilot": {"seconds": round(pilot["seconds"], 1), "n_exact": pilot["n_exact"]}}
+    print(json.dumps(rec), flush=True)
+
+
 def copy_calibration(mib):
     """A streaming kernel of known traffic (torch.neg over float32: reads and
     writes `mib` MiB each), run under `rocprofv3 --pmc FETCH_SIZE` /
@@ -206,7 +341,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="chisq_4096",
-                    choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k"])
+                    choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k", "surrogate_200k", "da_mcmc"])
+    ap.add_argument("--ncurves", type=int, default=None,
+                    help="training curves (surrogate_200k: 200000; da_mcmc: 30000)")
+    ap.add_argument("--epochs", type=int, default=None, help="surrogate training epochs")
+    ap.add_argument("--walkers", type=int, default=64)
+    ap.add_argument("--pilot-steps", type=int, default=120)
+    ap.add_argument("--inner", type=int, default=10, help="surrogate moves per exact check (da_mcmc)")
+    ap.add_argument("--noise", type=float, default=0.005, help="relative noise of the da_mcmc target curve")
     ap.add_argument("--nsets", type=int, default=4096, help="sets of chisq_4096 (4096 is the workload)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--chunk", type=int, default=0,
@@ -227,6 +369,18 @@ def main():
     if not lc.has_hip():
         raise SystemExit("no HIP device: the engine has no CPU path")
     lc.set_device("hip:0")
+    if args.workload == "surrogate_200k":
+        args.ncurves = args.ncurves or 200000
+        args.epochs = args.epochs or 60
+        bench_surrogate(args)
+        return
+    if args.workload == "da_mcmc":
+        args.ncurves = args.ncurves or 30000
+        args.epochs = args.epochs or 300
+        if args.steps == 10:
+            args.steps = 400
+        bench_da_mcmc(args)
+        return
     if args.chunk > 0:
         lc.set_tuning(args.chunk, 0)
 

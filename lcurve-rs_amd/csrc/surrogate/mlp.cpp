@@ -99,6 +99,9 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
   }
   yb_ = alloc<float>(static_cast<size_t>(max_batch) * out_dim());
   xin_ = alloc<float>(static_cast<size_t>(max_batch) * in_dim());
+  int wmax = 0;
+  for (int d : pdims_) wmax = std::max(wmax, d);
+  bpart_ = alloc<float>(static_cast<size_t>(max_batch / 64) * wmax);
   idx_ = alloc<int>(max_batch);
   partial_ = alloc<double>(loss_blocks_);
   for (hipEvent_t& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
@@ -206,7 +209,7 @@ void Mlp::backward(int batch) {
     g.ldc = din;
     g.Cf = gW_[l];
     gemm(g, false, false, mlp::EPI_PLAIN);
-    check(mlp::launch_bias_grad(dZ_[l], batch, dout, gb_[l], ctx_.stream()), "bias grad");
+    if (!gemm_only_) check(mlp::launch_bias_grad(dZ_[l], batch, dout, bpart_, gb_[l], ctx_.stream()), "bias grad");
     if (l > 0) {
       // dZ_{l-1} = (dZ_l . W_l) * gelu'(Z_{l-1}) : (k = out, n = in) at W[out * din + in].
       mlp::GemmArgs d;
@@ -243,13 +246,17 @@ void Mlp::adam_step(float lr) {
   }
 }
 
+// gemm_only (timing): the GEMMs of a step without loss, bias gradients
+// and Adam.
 void Mlp::run_step(int batch, float lr, bool adam, bool gemm_only) {
+  gemm_only_ = gemm_only;
   forward(batch);
   if (!gemm_only) {
     const size_t count = static_cast<size_t>(batch) * out_dim();
     check(mlp::launch_loss(Z_.back(), yb_, dZ_.back(), count, partial_, loss_blocks_, ctx_.stream()), "loss");
   }
   backward(batch);
+  gemm_only_ = false;
   if (adam) adam_step(lr);
 }
 
@@ -343,8 +350,6 @@ void Mlp::time_step(int batch, int reps, double* ms_step, double* ms_gemm, doubl
   check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "hipEventElapsedTime");
   check(hipEventElapsedTime(&b, ev_[1], ev_[2]), "hipEventElapsedTime");
   *ms_step = a / reps;
-  // GEMM-only pass still includes the bias-gradient reductions; count them
-  // as GEMM time (they are < 2% of it).
   *ms_gemm = b / reps;
   double f = 0.0;
   for (int l = 0; l < n_layers(); ++l) {

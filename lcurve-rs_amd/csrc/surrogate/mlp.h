@@ -71,6 +71,8 @@ class Mlp {
   float* yb_ = nullptr;      // targets of the batch [B x out]
   float* xin_ = nullptr;     // host-supplied inputs [max_batch x in]
   double* partial_ = nullptr;
+  float* bpart_ = nullptr;   // bias-gradient partial sums
+  bool gemm_only_ = false;
   int loss_blocks_ = 256;
   // Resident training set.
   float* X_ = nullptr;

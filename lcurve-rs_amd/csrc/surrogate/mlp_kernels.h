@@ -37,7 +37,8 @@ struct AdamParams {
 hipError_t launch_gemm(const GemmArgs& a, bool a_kc, bool b_kc, int epi, hipStream_t st);
 hipError_t launch_loss(const float* y, const float* t, uint16_t* dy, size_t count, double* partial,
                        int blocks, hipStream_t st);
-hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* db, hipStream_t st);
+// partial: (rows / 64) x cols floats of work space.
+hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* partial, float* db, hipStream_t st);
 hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, const uint8_t* mask,
                        size_t n, const AdamParams& p, hipStream_t st);
 hipError_t launch_gather(const float* X, const float* Y, const int* idx, int batch, int din, int din_pad,

@@ -3,8 +3,10 @@
 // gemm_kernel — C[M x N] = A[M x K] . B[K x N] on BF16 MFMA
 //   (v_mfma_f32_32x32x16_bf16) with FP32 accumulation. A workgroup of four
 //   wavefronts computes a 64 x 64 tile, each wave a 32 x 32 quarter; K is
-//   staged through LDS in steps of 32 with both operands stored K-contiguous
-//   there, so every lane's MFMA operand is one 16-byte LDS read. Operands
+//   staged through two LDS buffers in steps of 64 with both operands stored
+//   K-contiguous there, so every lane's MFMA operand is one 16-byte LDS
+//   read, and the next step's global loads are in flight (in registers)
+//   while the current step's MFMAs run. Operands
 //   may arrive K-contiguous or transposed in global memory (the three GEMMs
 //   of a layer's forward and backward passes need both), and the epilogue is
 //   fused: bias + GELU for hidden layers, bias only for the output layer,
@@ -13,8 +15,8 @@
 //   loss and its gradient, bias gradients (column sums), Adam on FP32 master
 //   weights writing the BF16 copy, and FP32 -> BF16 packing.
 //
-// Shapes are checked on the host: M, N multiples of 64, K a multiple of 32,
-// leading dimensions multiples of 8 elements.
+// Shapes are checked on the host: M, N and K multiples of 64, leading
+// dimensions multiples of 8 elements.
 #include <hip/hip_runtime.h>
 
 #include "mlp_kernels.h"
@@ -27,7 +29,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int BM = 64, BN = 64, BK = 64;
 constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row; the pad breaks bank conflicts
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
@@ -46,22 +48,45 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// Stage one BM x BK (or BN x BK) operand tile into LDS as [row][k].
-// KC: global element (row, k) at g[row * ld + k]; else at g[k * ld + row].
+// One BM x BK (or BN x BK) operand tile: each of the 256 threads fetches
+// two 16-byte pieces into registers (fetch), later written to LDS as
+// [row][k] (put). KC: global element (row, k) at g[row * ld + k]; else at
+// g[k * ld + row].
+struct Piece {
+  uint4 v[2];
+};
+
 template <bool KC>
-__device__ __forceinline__ void stage(uint16_t* __restrict__ s, const uint16_t* __restrict__ g,
-                                      int ld, int row0, int k0) {
+__device__ __forceinline__ Piece fetch(const uint16_t* __restrict__ g, int ld, int row0, int k0) {
   const int t = threadIdx.x;
-  if (KC) {
-    const int r = t >> 2, kk = (t & 3) * 8;
-    const uint4 v = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(row0 + r) * ld + k0 + kk);
-    *reinterpret_cast<uint4*>(s + r * LDS_STRIDE + kk) = v;
-  } else {
-    const int kk = t >> 3, r = (t & 7) * 8;
-    const uint4 v = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(k0 + kk) * ld + row0 + r);
-    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+  Piece p;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) s[(r + i) * LDS_STRIDE + kk] = h[i];
+  for (int h = 0; h < 2; ++h) {
+    if (KC) {
+      const int r = t >> 2, kk = (t & 3) * 8 + h * 32;
+      p.v[h] = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(row0 + r) * ld + k0 + kk);
+    } else {
+      const int kk = (t >> 3) + h * 32, r = (t & 7) * 8;
+      p.v[h] = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(k0 + kk) * ld + row0 + r);
+    }
+  }
+  return p;
+}
+
+template <bool KC>
+__device__ __forceinline__ void put(uint16_t* __restrict__ s, const Piece& p) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (KC) {
+      const int r = t >> 2, kk = (t & 3) * 8 + h * 32;
+      *reinterpret_cast<uint4*>(s + r * LDS_STRIDE + kk) = p.v[h];
+    } else {
+      const int kk = (t >> 3) + h * 32, r = (t & 7) * 8;
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&p.v[h]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s[(r + i) * LDS_STRIDE + kk] = e[i];
+    }
   }
 }
 
@@ -69,22 +94,36 @@ __device__ __forceinline__ void stage(uint16_t* __restrict__ s, const uint16_t* 
 
 template <bool AKC, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS_STRIDE];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   f32x16 acc = {};
-  for (int k0 = 0; k0 < a.K; k0 += BK) {
-    stage<AKC>(As, a.A, a.lda, m0, k0);
-    stage<BKC>(Bs, a.B, a.ldb, n0, k0);
-    __syncthreads();
+  Piece pa = fetch<AKC>(a.A, a.lda, m0, 0);
+  Piece pb = fetch<BKC>(a.B, a.ldb, n0, 0);
+  put<AKC>(As[0], pa);
+  put<BKC>(Bs[0], pb);
+  __syncthreads();
+  const int nk = a.K / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      pa = fetch<AKC>(a.A, a.lda, m0, (kt + 1) * BK);
+      pb = fetch<BKC>(a.B, a.ldb, n0, (kt + 1) * BK);
+    }
 #pragma unroll
     for (int s = 0; s < BK / 16; ++s) {
       const int kk = s * 16 + (l >> 5) * 8;
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(As + (wm + (l & 31)) * LDS_STRIDE + kk);
-      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs + (wn + (l & 31)) * LDS_STRIDE + kk);
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(As[cur] + (wm + (l & 31)) * LDS_STRIDE + kk);
+      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs[cur] + (wn + (l & 31)) * LDS_STRIDE + kk);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    if (kt + 1 < nk) {
+      // The other buffer was last read in step kt - 1, before the barrier
+      // that ended it.
+      put<AKC>(As[cur ^ 1], pa);
+      put<BKC>(Bs[cur ^ 1], pb);
     }
     __syncthreads();
   }
@@ -129,13 +168,29 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// db[n] = sum over rows of dz[row][n] (BF16 in, FP32 out); one thread per column.
-__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ dz, int rows, int cols,
-                                                        float* __restrict__ db) {
+// db[n] = sum over rows of dz[row][n] (BF16 in, FP32 out), in two passes
+// with a fixed order: block (x, y) sums 64 rows of 64 columns into
+// partial[y][col]; the second pass adds the partial rows.
+__global__ __launch_bounds__(256) void bias_partial_kernel(const uint16_t* __restrict__ dz, int cols,
+                                                           float* __restrict__ partial) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int col = blockIdx.x * 64 + c;
+  const size_t r0 = static_cast<size_t>(blockIdx.y) * 64;
+  float s = 0.0f;
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) s += bf2f(dz[(r0 + g + 4 * i) * cols + col]);
+  red[g][c] = s;
+  __syncthreads();
+  if (g == 0) partial[static_cast<size_t>(blockIdx.y) * cols + col] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+}
+
+__global__ __launch_bounds__(256) void bias_final_kernel(const float* __restrict__ partial, int nchunk, int cols,
+                                                         float* __restrict__ db) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= cols) return;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += bf2f(dz[static_cast<size_t>(r) * cols + n]);
+  for (int k = 0; k < nchunk; ++k) s += partial[static_cast<size_t>(k) * cols + n];
   db[n] = s;
 }
 
@@ -211,8 +266,10 @@ hipError_t launch_loss(const float* y, const float* t, uint16_t* dy, size_t coun
   return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* db, hipStream_t st) {
-  hipLaunchKernelGGL(bias_grad_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, dz, rows, cols, db);
+hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* partial, float* db, hipStream_t st) {
+  if (rows % 64 || cols % 64 || rows <= 0 || cols <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bias_partial_kernel, dim3(cols / 64, rows / 64), dim3(256), 0, st, dz, cols, partial);
+  hipLaunchKernelGGL(bias_final_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, partial, rows / 64, cols, db);
   return hipGetLastError();
 }
 
