@@ -378,16 +378,15 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   d.n_lanes = static_cast<uint32_t>(S.lane_phi.size());
   d.n_times = static_cast<uint32_t>(nt);
 
-  // Per (patch, tile) work codes. For each tile: the cone of its valid
-  // lanes' earth vectors (the vectors the kernel forms: comp_light lanes use
-  // the TWOPI*iangle/360 inclination, the single-star lanes
-  // iangle.to_radians()) and the range of their visibility phases.
+  // Per (patch, tile) work codes. For each tile: its valid lanes' earth
+  // vectors (the FP32 vectors the kernel reads: comp_light lanes use the
+  // TWOPI*iangle/360 inclination, the single-star lanes iangle.to_radians()),
+  // normalised, and the range of their visibility phases.
   const uint32_t ntile = d.n_lanes / TILE;
   struct TileInfo {
     bool any = false;
     double lo = 0.0, hi = 0.0;  // min / max of phi - floor(phi)
-    lc::Vec3 axis;
-    double theta = lc::PI;
+    std::vector<lc::Vec3> es;
   };
   std::vector<TileInfo> tiles(ntile);
   S.lane_e.resize(S.lane_phi.size());
@@ -401,22 +400,18 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   }
   for (uint32_t t = 0; t < ntile; ++t) {
     TileInfo& ti = tiles[t];
-    std::vector<lc::Vec3> es;
     for (uint32_t l = t * TILE; l < (t + 1) * TILE; ++l) {
       const uint32_t m = S.lane_meta[l];
       if (!(m & META_VALID)) continue;
       const float4 e = S.lane_e[l];
-      es.push_back(lc::Vec3(e.x, e.y, e.z));
+      lc::Vec3 v(e.x, e.y, e.z);
+      const double len = v.length();
+      ti.es.push_back(len > 0.0 ? (1.0 / len) * v : v);
       const double phif = S.lane_phi[l] - std::floor(S.lane_phi[l]);
       ti.lo = ti.any ? std::min(ti.lo, phif) : phif;
       ti.hi = ti.any ? std::max(ti.hi, phif) : phif;
       ti.any = true;
     }
-    const Cone c = bounding_cone(es);
-    ti.axis = lc::Vec3(c.x, c.y, c.z);
-    const double len = ti.axis.length();
-    if (len > 0.0) ti.axis = (1.0 / len) * ti.axis;
-    ti.theta = c.theta;
   }
   S.n_tiles = ntile;
   S.ptcode.assign(cones.size() * ntile, PT_FULL);
@@ -456,16 +451,27 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           code = PT_SKIP;
           continue;
         }
-        // Back-facing: every normal of the patch is more than beta + margin
-        // from every earth vector of the tile, so every FP32 mu <= thr. The
-        // margin (2e-3 rad) covers FP32 rounding of mu and of both cones.
-        const double gam = beta + ti.theta + pc.theta + kConeMargin;
-        if (cull && plen > 0.0 && gam < lc::PI - 1e-3 && lc::dot(ti.axis, pa) < std::cos(gam)) {
+        // Angles between the patch axis and the tile's lines of sight span
+        // [acos(emax), acos(emin)]; a normal of the patch lies within
+        // theta_p of the axis.
+        double emin = 1.0, emax = -1.0;
+        for (const lc::Vec3& e : ti.es) {
+          const double c = lc::dot(e, pa);
+          emin = std::min(emin, c);
+          emax = std::max(emax, c);
+        }
+        // Back-facing: every normal is more than beta + margin from every
+        // line of sight, so every FP32 mu <= thr. The margin (2e-3 rad)
+        // covers FP32 rounding of mu and of the patch cone.
+        const double gam = beta + pc.theta + kConeMargin;
+        if (cull && plen > 0.0 && gam < lc::PI - 1e-3 && emax < std::cos(gam)) {
           code = PT_SKIP;
           continue;
         }
-        const double alpha = plen > 0.0 ? std::acos(lc::clampd(lc::dot(ti.axis, pa), -1.0, 1.0)) : lc::PI;
-        const bool front = front_ok && plen > 0.0 && alpha + ti.theta + pc.theta + kConeMargin < beta_front;
+        // Seen whole: every normal is less than beta_front - margin from
+        // every line of sight.
+        const double gf = beta_front - pc.theta - kConeMargin;
+        const bool front = front_ok && plen > 0.0 && gf > 1e-3 && emin > std::cos(gf);
         if (sg.necl == 0) {
           code = front ? PT_MOM : PT_FREE;
           continue;
