@@ -215,6 +215,17 @@ def bench_surrogate(args):
     tm = s.net.time_step(4096, reps=50)
     tflops = tm["gemm_flop"] / (tm["ms_gemm"] * 1e-3) / 1e12
     value = 4096.0 / (tm["ms_step"] * 1e-3)
+    # The same network's step at larger batches (synthetic rows): how far the
+    # GEMMs are from MFMA-bound at batch 4096.
+    from lcurve_rs.surrogate import Mlp
+    scaling = {}
+    for bb in (16384, 65536):
+        net = Mlp((5, 512, 512, 512, 512, 256), seed=2, max_batch=bb)
+        rng = np.random.default_rng(0)
+        net.set_data(rng.uniform(-1, 1, (bb, 5)).astype(np.float32), rng.standard_normal((bb, 256)).astype(np.float32))
+        tb = net.time_step(bb, reps=10)
+        scaling[str(bb)] = {"ms_step": round(tb["ms_step"], 4), "gemm_tflops": round(tb["gemm_flop"] / (tb["ms_gemm"] * 1e-3) / 1e12, 2)}
+        del net
     rec = {"metric": "surrogate training samples/s (1 GPU)", "value": round(value, 1), "unit": "samples/s",
            "n_gpus": 1, "steps": 50, "warmup": 1, "ms_per_step": round(tm["ms_step"], 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f32",
@@ -226,7 +237,7 @@ def bench_surrogate(args):
            "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None,
                         "kernel": "gemm_kernel", "gemm_ms_per_step": round(tm["ms_gemm"], 4),
-                        "gemm_flop_per_step": tm["gemm_flop"]},
+                        "gemm_flop_per_step": tm["gemm_flop"], "larger_batches": scaling},
            "heldout_vs_exact": {"curves": int(nh), "rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
                                 "median_abs": float(np.median(ae)), "p99_abs": float(np.percentile(ae, 99)),
                                 "max_abs": float(ae.max())},

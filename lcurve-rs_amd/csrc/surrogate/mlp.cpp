@@ -67,6 +67,7 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
     for (int o = 0; o < dout; ++o) b[o] = static_cast<float>((2.0 * rng.uniform() - 1.0) * bound);
     W_.push_back(alloc<float>(nw));
     Wb_.push_back(alloc<uint16_t>(nw));
+    Wbt_.push_back(alloc<uint16_t>(nw));
     gW_.push_back(alloc<float>(nw));
     mW_.push_back(alloc<float>(nw));
     vW_.push_back(alloc<float>(nw));
@@ -83,6 +84,7 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
     check(hipMemset(gW_[l], 0, nw * sizeof(float)), "hipMemset");
     check(hipMemset(gb_[l], 0, dout * sizeof(float)), "hipMemset");
     check(mlp::launch_pack(W_[l], Wb_[l], nw, ctx_.stream()), "pack");
+    check(mlp::launch_transpose(Wb_[l], dout, din, Wbt_[l], ctx_.stream()), "transpose");
   }
   {
     const size_t nw = static_cast<size_t>(pdims_[1]) * 64;
@@ -101,7 +103,10 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
   xin_ = alloc<float>(static_cast<size_t>(max_batch) * in_dim());
   int wmax = 0;
   for (int d : pdims_) wmax = std::max(wmax, d);
-  bpart_ = alloc<float>(static_cast<size_t>(max_batch / 64) * wmax);
+  dZt_ = alloc<uint16_t>(static_cast<size_t>(max_batch) * wmax);
+  splitk_ = alloc<float>(static_cast<size_t>(kMaxSplit) * wmax * wmax);
+  At_ = alloc<uint16_t>(static_cast<size_t>(max_batch) * wmax);
+  if (2 * L > mlp::ADAM_MAX_TENSORS) throw std::invalid_argument("too many layers");
   idx_ = alloc<int>(max_batch);
   partial_ = alloc<double>(loss_blocks_);
   for (hipEvent_t& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
@@ -150,6 +155,7 @@ void Mlp::set_params(const float* in) {
     copy(b_[l], in + o, dout * sizeof(float), hipMemcpyHostToDevice);
     o += dout;
     check(mlp::launch_pack(W_[l], Wb_[l], w.size(), ctx_.stream()), "pack");
+    check(mlp::launch_transpose(Wb_[l], dout, din, Wbt_[l], ctx_.stream()), "transpose");
   }
   check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
 }
@@ -192,38 +198,51 @@ void Mlp::forward(int batch) {
   }
 }
 
-// dZ_[L-1] holds dLoss/dY (BF16).
+// dZ_[L-1] holds dLoss/dY (BF16). Every GEMM reads K-contiguous operands:
+// the weight gradient dZ^T . A uses transposed copies of dZ and A, the
+// input gradient dZ . W the transposed BF16 weights Adam keeps.
 void Mlp::backward(int batch) {
   const int L = n_layers();
   for (int l = L - 1; l >= 0; --l) {
     const int din = pdims_[l], dout = pdims_[l + 1];
-    // gW = dZ^T . A_l : (m = out, k = row) at dZ[row * dout + out].
+    if (!gemm_only_) {
+      check(mlp::launch_transpose(dZ_[l], batch, dout, dZt_, ctx_.stream()), "transpose");
+      check(mlp::launch_rowsum(dZt_, dout, batch, gb_[l], ctx_.stream()), "bias grad");
+      check(mlp::launch_transpose(A_[l], batch, din, At_, ctx_.stream()), "transpose");
+    }
+    // gW[out][in] = sum over rows of dZ[row][out] A[row][in]. M x N is
+    // small and K = batch long: K is split so that the launch fills the
+    // chip, the slices' FP32 partial products are summed in fixed order.
     mlp::GemmArgs g;
-    g.A = dZ_[l];
-    g.B = A_[l];
+    g.A = dZt_;
+    g.B = At_;
     g.M = dout;
     g.N = din;
     g.K = batch;
-    g.lda = dout;
-    g.ldb = din;
+    g.lda = batch;
+    g.ldb = batch;
     g.ldc = din;
-    g.Cf = gW_[l];
-    gemm(g, false, false, mlp::EPI_PLAIN);
-    if (!gemm_only_) check(mlp::launch_bias_grad(dZ_[l], batch, dout, bpart_, gb_[l], ctx_.stream()), "bias grad");
+    int ks = 1;
+    while (ks < kMaxSplit && (dout / 64) * (din / 64) * ks < 512 && (batch / 64) % (2 * ks) == 0) ks *= 2;
+    g.ksplit = ks;
+    g.Cf = ks > 1 ? splitk_ : gW_[l];
+    gemm(g, true, true, mlp::EPI_PLAIN);
+    if (ks > 1)
+      check(mlp::launch_splitk_sum(splitk_, ks, static_cast<size_t>(dout) * din, gW_[l], ctx_.stream()), "split-K sum");
     if (l > 0) {
-      // dZ_{l-1} = (dZ_l . W_l) * gelu'(Z_{l-1}) : (k = out, n = in) at W[out * din + in].
+      // dZ_{l-1} = (dZ_l . W_l) * gelu'(Z_{l-1}); B (k = out, n = in) = Wt[in][out].
       mlp::GemmArgs d;
       d.A = dZ_[l];
-      d.B = Wb_[l];
+      d.B = Wbt_[l];
       d.M = batch;
       d.N = din;
       d.K = dout;
       d.lda = dout;
-      d.ldb = din;
+      d.ldb = dout;
       d.ldc = din;
       d.Z = Z_[l - 1];
       d.Cb = dZ_[l - 1];
-      gemm(d, true, false, mlp::EPI_GELU_GRAD);
+      gemm(d, true, true, mlp::EPI_GELU_GRAD);
     }
   }
 }
@@ -237,17 +256,41 @@ void Mlp::adam_step(float lr) {
   p.eps = 1e-8f;
   p.bc1 = static_cast<float>(1.0 - std::pow(0.9, t_));
   p.bc2 = static_cast<float>(1.0 - std::pow(0.999, t_));
+  mlp::AdamTable tab{};
+  size_t off = 0;
+  int k = 0;
   for (int l = 0; l < n_layers(); ++l) {
-    const size_t nw = static_cast<size_t>(pdims_[l + 1]) * pdims_[l];
-    check(mlp::launch_adam(W_[l], gW_[l], mW_[l], vW_[l], Wb_[l], l == 0 ? mask0_ : nullptr, nw, p,
-                           ctx_.stream()), "adam");
-    check(mlp::launch_adam(b_[l], gb_[l], mb_[l], vb_[l], nullptr, nullptr, pdims_[l + 1], p, ctx_.stream()),
-          "adam");
+    const int din = pdims_[l], dout = pdims_[l + 1];
+    tab.w[k] = W_[l];
+    tab.g[k] = gW_[l];
+    tab.m[k] = mW_[l];
+    tab.v[k] = vW_[l];
+    tab.wb[k] = Wb_[l];
+    tab.wbt[k] = Wbt_[l];
+    tab.mask[k] = l == 0 ? mask0_ : nullptr;
+    tab.rows[k] = dout;
+    tab.cols[k] = din;
+    tab.start[k] = off;
+    off += static_cast<size_t>(dout) * din;
+    ++k;
+    tab.w[k] = b_[l];
+    tab.g[k] = gb_[l];
+    tab.m[k] = mb_[l];
+    tab.v[k] = vb_[l];
+    tab.wb[k] = nullptr;
+    tab.wbt[k] = nullptr;
+    tab.mask[k] = nullptr;
+    tab.rows[k] = 1;
+    tab.cols[k] = dout;
+    tab.start[k] = off;
+    off += dout;
+    ++k;
   }
+  tab.start[k] = off;
+  tab.count = k;
+  check(mlp::launch_adam_multi(tab, p, ctx_.stream()), "adam");
 }
 
-// gemm_only (timing): the GEMMs of a step without loss, bias gradients
-// and Adam.
 void Mlp::run_step(int batch, float lr, bool adam, bool gemm_only) {
   gemm_only_ = gemm_only;
   forward(batch);

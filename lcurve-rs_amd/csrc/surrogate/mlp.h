@@ -62,7 +62,7 @@ class Mlp {
   int max_batch_;
   // Parameters per layer.
   std::vector<float*> W_, b_, gW_, gb_, mW_, vW_, mb_, vb_;
-  std::vector<uint16_t*> Wb_;
+  std::vector<uint16_t*> Wb_, Wbt_;  // BF16 weights [out x in] and transposed [in x out]
   uint8_t* mask0_ = nullptr;  // layer-0 weight mask (padding columns frozen)
   // Activations: A_[l] BF16 input of layer l [B x pdims[l]]; Z_[l] FP32
   // pre-activation of layer l [B x pdims[l+1]]; dZ_[l] BF16 gradient wrt Z_[l].
@@ -71,7 +71,10 @@ class Mlp {
   float* yb_ = nullptr;      // targets of the batch [B x out]
   float* xin_ = nullptr;     // host-supplied inputs [max_batch x in]
   double* partial_ = nullptr;
-  float* bpart_ = nullptr;   // bias-gradient partial sums
+  uint16_t* dZt_ = nullptr;  // transposed output gradient of the layer in work
+  uint16_t* At_ = nullptr;   // transposed input activations of that layer
+  static constexpr int kMaxSplit = 16;
+  float* splitk_ = nullptr;  // split-K partial weight gradients [kMaxSplit x wmax x wmax]
   bool gemm_only_ = false;
   int loss_blocks_ = 256;
   // Resident training set.

@@ -28,22 +28,42 @@ struct GemmArgs {
   const float* Z = nullptr;     // EPI_GELU_GRAD: pre-activations [M x ldc]
   float* Cf = nullptr;
   uint16_t* Cb = nullptr;
+  // EPI_PLAIN only: K split into ksplit slices whose partial products go to
+  // ksplit consecutive M x ldc planes of Cf (reduced by launch_splitk_sum).
+  int ksplit = 1;
 };
 
 struct AdamParams {
   float lr, beta1, beta2, eps, bc1, bc2;  // bc = 1 - beta^t
 };
 
+// Parameter tensors updated by one Adam launch. Tensor k holds elements
+// [start[k], start[k+1]) of the launch; wb / wbt (BF16 copy and its
+// transpose, rows x cols) are null for bias vectors.
+constexpr int ADAM_MAX_TENSORS = 24;
+struct AdamTable {
+  float* w[ADAM_MAX_TENSORS];
+  const float* g[ADAM_MAX_TENSORS];
+  float* m[ADAM_MAX_TENSORS];
+  float* v[ADAM_MAX_TENSORS];
+  uint16_t* wb[ADAM_MAX_TENSORS];
+  uint16_t* wbt[ADAM_MAX_TENSORS];
+  const uint8_t* mask[ADAM_MAX_TENSORS];
+  int rows[ADAM_MAX_TENSORS], cols[ADAM_MAX_TENSORS];
+  size_t start[ADAM_MAX_TENSORS + 1];
+  int count;
+};
+
 hipError_t launch_gemm(const GemmArgs& a, bool a_kc, bool b_kc, int epi, hipStream_t st);
 hipError_t launch_loss(const float* y, const float* t, uint16_t* dy, size_t count, double* partial,
                        int blocks, hipStream_t st);
-// partial: (rows / 64) x cols floats of work space.
-hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* partial, float* db, hipStream_t st);
-hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, const uint8_t* mask,
-                       size_t n, const AdamParams& p, hipStream_t st);
 hipError_t launch_gather(const float* X, const float* Y, const int* idx, int batch, int din, int din_pad,
                          int dout, uint16_t* xb, float* yb, hipStream_t st);
 hipError_t launch_pack(const float* x, uint16_t* y, size_t n, hipStream_t st);
+hipError_t launch_transpose(const uint16_t* in, int rows, int cols, uint16_t* out, hipStream_t st);
+hipError_t launch_rowsum(const uint16_t* x, int rows, int cols, float* out, hipStream_t st);
+hipError_t launch_splitk_sum(const float* part, int nsplit, size_t n, float* out, hipStream_t st);
+hipError_t launch_adam_multi(const AdamTable& t, const AdamParams& p, hipStream_t st);
 
 }  // namespace mlp
 }  // namespace lch

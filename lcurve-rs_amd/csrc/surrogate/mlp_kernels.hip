@@ -11,9 +11,11 @@
 //   of a layer's forward and backward passes need both), and the epilogue is
 //   fused: bias + GELU for hidden layers, bias only for the output layer,
 //   GELU' for the backward pass, plain FP32 for weight gradients.
-// loss_kernel / bias_grad_kernel / adam_kernel / pack_kernel — the MSE
-//   loss and its gradient, bias gradients (column sums), Adam on FP32 master
-//   weights writing the BF16 copy, and FP32 -> BF16 packing.
+// loss_kernel / rowsum_kernel / transpose_kernel / adam_multi_kernel /
+//   pack_kernel — the MSE loss and its gradient, bias gradients (row sums of
+//   the transposed output gradient), BF16 transposes, one Adam launch over
+//   all parameter tensors on FP32 master weights writing the BF16 copies,
+//   and FP32 -> BF16 packing.
 //
 // Shapes are checked on the host: M, N and K multiples of 64, leading
 // dimensions multiples of 8 elements.
@@ -99,18 +101,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  // Split K: slice blockIdx.z covers [kb, kb + K / gridDim.z).
+  const int kspan = a.K / static_cast<int>(gridDim.z);
+  const int kb = static_cast<int>(blockIdx.z) * kspan;
   f32x16 acc = {};
-  Piece pa = fetch<AKC>(a.A, a.lda, m0, 0);
-  Piece pb = fetch<BKC>(a.B, a.ldb, n0, 0);
+  Piece pa = fetch<AKC>(a.A, a.lda, m0, kb);
+  Piece pb = fetch<BKC>(a.B, a.ldb, n0, kb);
   put<AKC>(As[0], pa);
   put<BKC>(Bs[0], pb);
   __syncthreads();
-  const int nk = a.K / BK;
+  const int nk = kspan / BK;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) {
-      pa = fetch<AKC>(a.A, a.lda, m0, (kt + 1) * BK);
-      pb = fetch<BKC>(a.B, a.ldb, n0, (kt + 1) * BK);
+      pa = fetch<AKC>(a.A, a.lda, m0, kb + (kt + 1) * BK);
+      pb = fetch<BKC>(a.B, a.ldb, n0, kb + (kt + 1) * BK);
     }
 #pragma unroll
     for (int s = 0; s < BK / 16; ++s) {
@@ -130,10 +135,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
   // Lane l holds C[(r / 4) * 8 + (l / 32) * 4 + r % 4][l % 32] of the wave's
   // 32 x 32 quarter in acc[r].
   const int n = n0 + wn + (l & 31);
+  // Split-K slices write FP32 partial products to consecutive M x ldc planes.
+  const size_t plane = static_cast<size_t>(blockIdx.z) * a.M * a.ldc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = m0 + wm + (r >> 2) * 8 + (l >> 5) * 4 + (r & 3);
-    const size_t idx = static_cast<size_t>(m) * a.ldc + n;
+    const size_t idx = static_cast<size_t>(m) * a.ldc + n + (EPI == EPI_PLAIN ? plane : 0);
     const float v = acc[r];
     if (EPI == EPI_BIAS_GELU) {
       const float z = v + a.bias[n];
@@ -168,52 +175,90 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ y, 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// db[n] = sum over rows of dz[row][n] (BF16 in, FP32 out), in two passes
-// with a fixed order: block (x, y) sums 64 rows of 64 columns into
-// partial[y][col]; the second pass adds the partial rows.
-__global__ __launch_bounds__(256) void bias_partial_kernel(const uint16_t* __restrict__ dz, int cols,
-                                                           float* __restrict__ partial) {
-  __shared__ float red[4][64];
-  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int col = blockIdx.x * 64 + c;
-  const size_t r0 = static_cast<size_t>(blockIdx.y) * 64;
-  float s = 0.0f;
-#pragma unroll 4
-  for (int i = 0; i < 16; ++i) s += bf2f(dz[(r0 + g + 4 * i) * cols + col]);
-  red[g][c] = s;
+// BF16 transpose out[c][r] = in[r][c] of a rows x cols matrix (both
+// multiples of 64) through a 64 x 64 LDS tile: coalesced 16-byte loads and
+// stores, the transpose done in LDS.
+__global__ __launch_bounds__(256) void transpose_kernel(const uint16_t* __restrict__ in, int rows, int cols,
+                                                        uint16_t* __restrict__ out) {
+  __shared__ uint16_t tile[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int t = threadIdx.x;
+  {
+    const int r = t >> 2, c = (t & 3) * 16;
+    const uint4* src = reinterpret_cast<const uint4*>(in + static_cast<size_t>(r0 + r) * cols + c0 + c);
+    const uint4 a = src[0], b = src[1];
+    const uint16_t* ha = reinterpret_cast<const uint16_t*>(&a);
+    const uint16_t* hb = reinterpret_cast<const uint16_t*>(&b);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      tile[r][c + i] = ha[i];
+      tile[r][c + 8 + i] = hb[i];
+    }
+  }
   __syncthreads();
-  if (g == 0) partial[static_cast<size_t>(blockIdx.y) * cols + col] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+  {
+    const int oc = t >> 2, orr = (t & 3) * 16;  // output row = input column oc
+    uint4 a, b;
+    uint16_t* ha = reinterpret_cast<uint16_t*>(&a);
+    uint16_t* hb = reinterpret_cast<uint16_t*>(&b);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      ha[i] = tile[orr + i][oc];
+      hb[i] = tile[orr + 8 + i][oc];
+    }
+    uint4* dst = reinterpret_cast<uint4*>(out + static_cast<size_t>(c0 + oc) * rows + r0 + orr);
+    dst[0] = a;
+    dst[1] = b;
+  }
 }
 
-__global__ __launch_bounds__(256) void bias_final_kernel(const float* __restrict__ partial, int nchunk, int cols,
-                                                         float* __restrict__ db) {
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= cols) return;
+// Row sums of a BF16 matrix (FP32 out; cols a multiple of 8), one
+// wavefront per row with 16-byte loads: the bias gradient from the
+// transposed output gradient.
+__global__ __launch_bounds__(256) void rowsum_kernel(const uint16_t* __restrict__ x, int rows, int cols,
+                                                     float* __restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (row >= rows) return;
+  const uint4* p = reinterpret_cast<const uint4*>(x + static_cast<size_t>(row) * cols);
   float s = 0.0f;
-  for (int k = 0; k < nchunk; ++k) s += partial[static_cast<size_t>(k) * cols + n];
-  db[n] = s;
+  for (int c = l; c < cols / 8; c += 64) {
+    const uint4 v = p[c];
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += bf2f(h[i]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (l == 0) out[row] = s;
 }
 
-// Adam (Kingma & Ba 2015) on FP32 master weights; writes the BF16 copy when
-// wb is non-null. mask, when non-null, freezes entries whose mask is 0
-// (padding columns of the first layer).
-__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
-                                                   float* __restrict__ m, float* __restrict__ v,
-                                                   uint16_t* __restrict__ wb,
-                                                   const uint8_t* __restrict__ mask, size_t n,
-                                                   AdamParams p) {
+// Adam (Kingma & Ba 2015) over every parameter tensor in one launch: FP32
+// master weights, BF16 copy (and its transpose for weight matrices).
+// Entries of a tensor's mask that are 0 stay frozen (padding columns of the
+// first layer).
+__global__ __launch_bounds__(256) void adam_multi_kernel(AdamTable tab, AdamParams p) {
   const size_t i = blockIdx.x * 256ull + threadIdx.x;
-  if (i >= n) return;
-  if (mask && !mask[i]) return;
-  const float gi = g[i];
-  const float mi = p.beta1 * m[i] + (1.0f - p.beta1) * gi;
-  const float vi = p.beta2 * v[i] + (1.0f - p.beta2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float mh = mi / p.bc1, vh = vi / p.bc2;
-  const float wi = w[i] - p.lr * mh / (sqrtf(vh) + p.eps);
-  w[i] = wi;
-  if (wb) wb[i] = f2bf(wi);
+  if (i >= tab.start[tab.count]) return;
+  int k = 0;
+  while (i >= tab.start[k + 1]) ++k;
+  const size_t j = i - tab.start[k];
+  if (tab.mask[k] && !tab.mask[k][j]) return;
+  float* w = tab.w[k];
+  float* m = tab.m[k];
+  float* v = tab.v[k];
+  const float gi = tab.g[k][j];
+  const float mi = p.beta1 * m[j] + (1.0f - p.beta1) * gi;
+  const float vi = p.beta2 * v[j] + (1.0f - p.beta2) * gi * gi;
+  m[j] = mi;
+  v[j] = vi;
+  const float wi = w[j] - p.lr * (mi / p.bc1) / (sqrtf(vi / p.bc2) + p.eps);
+  w[j] = wi;
+  if (tab.wb[k]) {
+    const uint16_t h = f2bf(wi);
+    tab.wb[k][j] = h;
+    const int cols = tab.cols[k], rows = tab.rows[k];
+    tab.wbt[k][static_cast<size_t>(j % cols) * rows + j / cols] = h;
+  }
 }
 
 // Minibatch gather: xb[b][0..din_pad) = bf16(X[idx[b]][0..din)), zero
@@ -240,22 +285,23 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
 // ------------------------------------------------------------ launchers
 template <bool AKC, bool BKC, int EPI>
 static hipError_t launch_gemm_t(const GemmArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL((gemm_kernel<AKC, BKC, EPI>), dim3(a.N / BN, a.M / BM), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((gemm_kernel<AKC, BKC, EPI>), dim3(a.N / BN, a.M / BM, a.ksplit), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_gemm(const GemmArgs& a, bool a_kc, bool b_kc, int epi, hipStream_t st) {
   if (a.M % BM || a.N % BN || a.K % BK || a.M <= 0 || a.N <= 0 || a.K <= 0) return hipErrorInvalidValue;
+  if (a.ksplit < 1 || (a.K / BK) % a.ksplit || (a.ksplit > 1 && epi != EPI_PLAIN)) return hipErrorInvalidValue;
   if (a.lda % 8 || a.ldb % 8) return hipErrorInvalidValue;
 #define LCH_GEMM_CASE(AK, BK_, E) \
   if (a_kc == AK && b_kc == BK_ && epi == E) return launch_gemm_t<AK, BK_, E>(a, st);
+  // The MLP passes K-contiguous operands only (transposed copies are made
+  // by transpose_kernel); the strided staging path serves other callers.
   LCH_GEMM_CASE(true, true, EPI_BIAS_GELU)
   LCH_GEMM_CASE(true, true, EPI_BIAS)
   LCH_GEMM_CASE(true, true, EPI_PLAIN)
-  LCH_GEMM_CASE(true, false, EPI_GELU_GRAD)
-  LCH_GEMM_CASE(true, false, EPI_PLAIN)
+  LCH_GEMM_CASE(true, true, EPI_GELU_GRAD)
   LCH_GEMM_CASE(false, false, EPI_PLAIN)
-  LCH_GEMM_CASE(false, true, EPI_PLAIN)
 #undef LCH_GEMM_CASE
   return hipErrorInvalidValue;
 }
@@ -266,23 +312,44 @@ hipError_t launch_loss(const float* y, const float* t, uint16_t* dy, size_t coun
   return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const uint16_t* dz, int rows, int cols, float* partial, float* db, hipStream_t st) {
-  if (rows % 64 || cols % 64 || rows <= 0 || cols <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(bias_partial_kernel, dim3(cols / 64, rows / 64), dim3(256), 0, st, dz, cols, partial);
-  hipLaunchKernelGGL(bias_final_kernel, dim3((cols + 255) / 256), dim3(256), 0, st, partial, rows / 64, cols, db);
-  return hipGetLastError();
-}
-
-hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, const uint8_t* mask,
-                       size_t n, const AdamParams& p, hipStream_t st) {
-  hipLaunchKernelGGL(adam_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, w, g, m, v,
-                     wb, mask, n, p);
-  return hipGetLastError();
-}
-
 hipError_t launch_gather(const float* X, const float* Y, const int* idx, int batch, int din, int din_pad,
                          int dout, uint16_t* xb, float* yb, hipStream_t st) {
   hipLaunchKernelGGL(gather_kernel, dim3(batch), dim3(256), 0, st, X, Y, idx, batch, din, din_pad, dout, xb, yb);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const uint16_t* in, int rows, int cols, uint16_t* out, hipStream_t st) {
+  if (rows % 64 || cols % 64 || rows <= 0 || cols <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(transpose_kernel, dim3(cols / 64, rows / 64), dim3(256), 0, st, in, rows, cols, out);
+  return hipGetLastError();
+}
+
+// out[i] = sum over the nsplit planes of part[plane * n + i].
+__global__ __launch_bounds__(256) void splitk_sum_kernel(const float* __restrict__ part, int nsplit, size_t n,
+                                                         float* __restrict__ out) {
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int p = 0; p < nsplit; ++p) s += part[static_cast<size_t>(p) * n + i];
+  out[i] = s;
+}
+
+hipError_t launch_splitk_sum(const float* part, int nsplit, size_t n, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(splitk_sum_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, part,
+                     nsplit, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsum(const uint16_t* x, int rows, int cols, float* out, hipStream_t st) {
+  if (cols % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rowsum_kernel, dim3((rows + 3) / 4), dim3(256), 0, st, x, rows, cols, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam_multi(const AdamTable& t, const AdamParams& p, hipStream_t st) {
+  if (t.count <= 0 || t.count > ADAM_MAX_TENSORS) return hipErrorInvalidValue;
+  const size_t n = t.start[t.count];
+  hipLaunchKernelGGL(adam_multi_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, t, p);
   return hipGetLastError();
 }
 
