@@ -305,6 +305,14 @@ void Batch::build_items(uint32_t chunk) {
     prow += static_cast<uint64_t>(ntile) * nch;
   }
   n_prows_ = prow;
+  // Items of beamed / lensed star segments go last: the kernel for the
+  // others is compiled without those paths.
+  auto needs_fx = [&](const ItemDev& it) {
+    const SegDev& sg = segs_[it.seg];
+    return sg.kind <= KIND_STAR2 && (sg.flags & SEG_POSN) != 0;
+  };
+  const auto mid = std::stable_partition(items.begin(), items.end(), [&](const ItemDev& it) { return !needs_fx(it); });
+  n_items_plain_ = static_cast<uint64_t>(mid - items.begin());
   n_items_ = items.size();
   pairs_eval_ = pairs_mom_ = 0;
   for (const ItemDev& it : items) {
@@ -365,6 +373,7 @@ void Batch::build_items(uint32_t chunk) {
   d_.chunk_info = dci;
   d_.items = d_items_;
   d_.n_items = static_cast<uint32_t>(n_items_);
+  d_.n_items_plain = static_cast<uint32_t>(n_items_plain_);
   d_.partials = d_partials_;
 }
 

@@ -111,7 +111,8 @@ class Batch {
   std::vector<double> moms_;
   std::vector<uint64_t> pt_base_;
   std::vector<uint32_t> pt_tiles_;
-  uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0;
+  uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0,
+           n_items_plain_ = 0;
   uint32_t chunk_ = 0;
 
   DeviceBatch d_;

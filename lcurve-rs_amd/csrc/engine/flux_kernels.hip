@@ -255,13 +255,17 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
   return acc;
 }
 
-template <int NECL>
+// FX: whether the beaming / lensing star paths are compiled in. The kernel
+// without them needs fewer registers (more waves per SIMD); the host sends
+// it no item of a beamed or lensed star segment.
+template <int NECL, bool FX>
 __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t fx, bool mcrit,
                                                 const char* p, uint64_t todo, uint64_t fr,
                                                 const LaneCtx& L, const StarPar& P, float lin,
                                                 float quad) {
   if (kind <= KIND_STAR2) {
-    if (fx == 0) {
+    if (!FX && fx != 0) return __builtin_nan("");  // never sent here: fail loudly
+    if (!FX || fx == 0) {
       if (ld == LD_LIN)
         return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad)
                      : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad);
@@ -329,6 +333,7 @@ __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
   return x * q + (x < r ? x : r) + i;
 }
 
+template <bool FX>
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
@@ -407,11 +412,11 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
                      1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
-    case 0: sum += item_dispatch<0>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 1: sum += item_dispatch<1>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 2: sum += item_dispatch<2>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 3: sum += item_dispatch<3>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    default: sum += item_dispatch<4>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
   }
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
@@ -419,6 +424,7 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
 }  // namespace
 
 // One workgroup = one wavefront = one work item.
+template <bool FX>
 __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
@@ -427,7 +433,7 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
     double* __restrict__ partials) {
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
+  process_item<FX>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -445,7 +451,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
+      process_item<true>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
   }
 }
 
@@ -862,8 +868,14 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
                              uint32_t grab, hipStream_t stream) {
   if (d.n_items == 0) return hipSuccess;
   if (!persistent) {
-    hipLaunchKernelGGL(flux_items_kernel, dim3(d.n_items), dim3(TILE), 0, stream, d.items, d.n_items,
-                       d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
+    // Items without beaming / lensing come first (host order).
+    const uint32_t nf = d.n_items_plain, nx = d.n_items - d.n_items_plain;
+    if (nf)
+      hipLaunchKernelGGL(flux_items_kernel<false>, dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs, d.sets,
+                         d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
+    if (nx)
+      hipLaunchKernelGGL(flux_items_kernel<true>, dim3(nx), dim3(TILE), 0, stream, d.items + nf, nx, d.segs,
+                         d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,

@@ -9,6 +9,7 @@ namespace lch {
 
 struct DeviceBatch {
   uint32_t n_sets = 0, n_items = 0;
+  uint32_t n_items_plain = 0;  // leading items with no beaming / lensing
   const SetDev* sets = nullptr;
   const SegDev* segs = nullptr;
   const ItemDev* items = nullptr;
