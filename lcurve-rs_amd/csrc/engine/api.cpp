@@ -454,6 +454,22 @@ int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const doub
   });
 }
 
+int lch_selfcheck_eclipse_prescreen(const lch_model* m, long long* out, double* ms) {
+  if (!m || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    const lch::PrescreenCheck c = lch::selfcheck_prescreen(m->impl);
+    out[0] = c.elements;
+    out[1] = c.mismatches;
+    out[2] = c.skipped;
+    out[3] = c.evaluated;
+    if (ms) {
+      ms[0] = c.ms_plain;
+      ms[1] = c.ms_screened;
+    }
+    return LCH_OK;
+  });
+}
+
 int lch_batch_sfac(const lch_batch* b, double* sfac5) {
   if (!b || !sfac5) return fail(LCH_ERR_INVALID, "null argument");
   const SetOutArr& o = b->impl->last_results();

@@ -9,9 +9,13 @@
 // some element is eclipsed, or a wrong moment shows up as a difference.
 // Beamed / lensed star segments use an indicator term (nonzero exactly when
 // the reference adds the element) since only the skip decisions apply there.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
+
+#include "../setup/grid.h"
+#include "../setup/roche.h"
 
 #include "batch_build.h"
 #include "selfcheck.h"
@@ -133,6 +137,65 @@ SelfCheck selfcheck_model(const lc::Model& model, const lc::Data& data) {
   auto P = lc::prepare_model(model, data);
   auto S = pack_set(*P, data, false);
   return selfcheck_codes(*S);
+}
+
+namespace {
+bool same_bits(const void* a, const void* b, size_t n) { return std::memcmp(a, b, n) == 0; }
+
+bool same_point(const lc::Point& a, const lc::Point& b) {
+  if (!same_bits(&a.posn, &b.posn, sizeof a.posn) || !same_bits(&a.dirn, &b.dirn, sizeof a.dirn) ||
+      !same_bits(&a.area, &b.area, sizeof a.area) ||
+      !same_bits(&a.gravity, &b.gravity, sizeof a.gravity) || a.eclipse.size() != b.eclipse.size())
+    return false;
+  for (size_t k = 0; k < a.eclipse.size(); ++k)
+    if (!same_bits(&a.eclipse[k], &b.eclipse[k], sizeof a.eclipse[k])) return false;
+  return true;
+}
+
+struct PrescreenMode {
+  bool old;
+  explicit PrescreenMode(bool on) : old(lc::set_eclipse_prescreen(on)) {}
+  ~PrescreenMode() { lc::set_eclipse_prescreen(old); }
+};
+}  // namespace
+
+PrescreenCheck selfcheck_prescreen(const lc::Model& model) {
+  using clock = std::chrono::steady_clock;
+  auto ms = [](clock::time_point a, clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  PrescreenCheck out;
+  for (lc::Star star : {lc::Star::Primary, lc::Star::Secondary}) {
+    for (bool fine : {true, false}) {
+      std::vector<lc::Point> plain, screened;
+      long long s0, e0, s1, e1;
+      const auto t0 = clock::now();
+      {
+        PrescreenMode mode(false);
+        plain = lc::set_star_grid(model, star, fine);
+      }
+      const auto t1 = clock::now();
+      lc::eclipse_prescreen_counts(&s0, &e0);
+      {
+        PrescreenMode mode(true);
+        screened = lc::set_star_grid(model, star, fine);
+      }
+      const auto t2 = clock::now();
+      lc::eclipse_prescreen_counts(&s1, &e1);
+      out.ms_plain += ms(t0, t1);
+      out.ms_screened += ms(t1, t2);
+      out.skipped += s1 - s0;
+      out.evaluated += e1 - e0;
+      out.elements += static_cast<long long>(plain.size());
+      if (plain.size() != screened.size()) {
+        out.mismatches += static_cast<long long>(std::max(plain.size(), screened.size()));
+        continue;
+      }
+      for (size_t i = 0; i < plain.size(); ++i)
+        if (!same_point(plain[i], screened[i])) ++out.mismatches;
+    }
+  }
+  return out;
 }
 
 }  // namespace lch

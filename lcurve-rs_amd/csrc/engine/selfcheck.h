@@ -15,4 +15,15 @@ struct SelfCheck {
 SelfCheck selfcheck_codes(const SetPack& S);
 SelfCheck selfcheck_model(const lc::Model& model, const lc::Data& data);
 
+// Star grids (fine and coarse, both stars) built with and without the
+// certified pre-screen of the eclipse phase scan, compared bit for bit.
+struct PrescreenCheck {
+  long long elements = 0;    // grid elements compared
+  long long mismatches = 0;  // elements whose position, normal, area, gravity or eclipses differ
+  long long skipped = 0;     // scan steps the pre-screen certified
+  long long evaluated = 0;   // scan steps run through fblink
+  double ms_plain = 0.0, ms_screened = 0.0;
+};
+PrescreenCheck selfcheck_prescreen(const lc::Model& model);
+
 }  // namespace lch

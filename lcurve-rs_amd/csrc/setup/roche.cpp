@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdio>
 
 namespace lc {
@@ -337,16 +338,20 @@ bool sphere_eclipse_phase(double cosi, double sini, const Vec3& r, const Vec3& c
 }
 
 namespace {
-// fblink's body (crates/roche/src/eclipse.rs:123-197) with the reference
-// sphere passed in: ref_sphere is a pure function of (q, star, spin, ffac),
-// so ingress_egress computes it once for its many fblink calls.
+// fblink's body up to its Brent search (crates/roche/src/eclipse.rs:123-181),
+// with the reference sphere passed in: ref_sphere is a pure function of
+// (q, star, spin, ffac), so ingress_egress computes it once for its many
+// fblink calls. Returns 0 or 1 when this part decides; 2 when a minimum of
+// the potential along the line of sight is bracketed at *lam (potential
+// *flam) inside the chord [*lam1, *lam2] through the reference sphere.
 template <Star S>
-bool fblink_ref(double q, double spin, double acc, const Vec3& earth, const Vec3& p, double rref,
-                double pref) {
+int fblink_bracket(double q, double spin, double acc, const Vec3& earth, const Vec3& p,
+                   double rref, double pref, double* lam1_out, double* lam2_out, double* lam_out,
+                   double* flam_out) {
   const Vec3 cofm = star_cofm(S);
   double lam1, lam2;
-  if (!sphere_eclipse_los(earth, p, cofm, rref, &lam1, &lam2)) return false;
-  if (lam1 == 0.0) return true;
+  if (!sphere_eclipse_los(earth, p, cofm, rref, &lam1, &lam2)) return 0;
+  if (lam1 == 0.0) return 1;
 
   auto func = [&](double lam) { return rpot_along_los<S>(q, spin, earth, p, lam); };
   size_t nstep = 1;
@@ -358,7 +363,7 @@ bool fblink_ref(double q, double spin, double acc, const Vec3& earth, const Vec3
     bool bracketted = false;
     for (size_t n = 0; n < nstep; ++n) {
       flam = func(lam + static_cast<double>(n) * step_size);
-      if (flam <= pref) return true;
+      if (flam <= pref) return 1;
       if (nstep == 1) {
         f1 = func(lam1);
         f2 = func(lam2);
@@ -374,12 +379,34 @@ bool fblink_ref(double q, double spin, double acc, const Vec3& earth, const Vec3
     nstep *= 2;
   }
   if (flam < f1 && flam < f2) {
-    auto dfunc = [&](double x) { return drpot_along_los<S>(q, spin, earth, p, x); };
-    double fmin, xmin;
-    if (dbrent(lam1, lam, lam2, func, dfunc, acc, true, pref, &fmin, &xmin)) return fmin < pref;
-    return false;
+    *lam1_out = lam1;
+    *lam2_out = lam2;
+    *lam_out = lam;
+    *flam_out = flam;
+    return 2;
   }
+  return 0;
+}
+
+// fblink's Brent search from the bracket (eclipse.rs:182-196).
+template <Star S>
+bool fblink_min(double q, double spin, double acc, const Vec3& earth, const Vec3& p, double pref,
+                double lam1, double lam, double lam2) {
+  auto func = [&](double x) { return rpot_along_los<S>(q, spin, earth, p, x); };
+  auto dfunc = [&](double x) { return drpot_along_los<S>(q, spin, earth, p, x); };
+  double fmin, xmin;
+  if (dbrent(lam1, lam, lam2, func, dfunc, acc, true, pref, &fmin, &xmin)) return fmin < pref;
   return false;
+}
+
+// crates/roche/src/eclipse.rs:123-197
+template <Star S>
+bool fblink_ref(double q, double spin, double acc, const Vec3& earth, const Vec3& p, double rref,
+                double pref) {
+  double lam1, lam2, lam, flam;
+  const int k = fblink_bracket<S>(q, spin, acc, earth, p, rref, pref, &lam1, &lam2, &lam, &flam);
+  if (k != 2) return k == 1;
+  return fblink_min<S>(q, spin, acc, earth, p, pref, lam1, lam, lam2);
 }
 
 template <Star S>
@@ -389,17 +416,226 @@ bool fblink_t(double q, double spin, double ffac, double acc, const Vec3& earth,
   return fblink_ref<S>(q, spin, acc, earth, p, rref, pref);
 }
 
-// crates/roche/src/eclipse.rs:347-373
+thread_local bool tl_prescreen = true;
+thread_local long long tl_skipped = 0, tl_evaluated = 0;
+
+// Certified short cut for the phase scan of ingress_egress.
+//
+// Most scan steps' lines of sight pass beside the eclipsing lobe, and for
+// each fblink runs its grid search, brackets the potential minimum along the
+// chord [lam1, lam2] through the reference sphere and then runs dbrent, the
+// costly part, only to find the minimum above pref. The scan runs
+// fblink_bracket exactly as fblink does and skips dbrent when its result is
+// certain to be false:
+//
+//  (A) a rigorous lower bound of the potential over the part of the line of
+//      sight inside the ball B(c, rx), rx = rref + 1e-6, exceeds pref by far
+//      more than the rounding of one potential evaluation, so no point of
+//      the chord gives a potential <= pref; and
+//  (B) the bound over [lam1, lam1 + acc] and over [lam2 - acc, lam2] exceeds
+//      flam, the value at the bracketing point, likewise.
+//
+// Why that suffices: dbrent's best point x starts at the bracketing point
+// and only moves to trial points with a potential <= its own, so by (B) it
+// never lies within acc of a chord end. Its bracket starts as [lam1, lam2]
+// and only takes values of x and of trial points. Its trial points are
+// parabolic steps strictly inside the bracket, midpoints between x and a
+// bracket end, or steps of acc from x; with x at least acc inside the
+// chord, all of them lie in the chord. So dbrent evaluates the potential
+// only on the chord, where by (A) every value is above pref: it returns a
+// minimum above pref (or fails), and fblink is false either way. Steps not
+// certified run dbrent as before, so the scan finds the same first step and
+// ingress and egress are unchanged to the bit.
+//
+// (A) covers a block of consecutive phases at once: every line of sight of
+// the block lies within |lam| * de of the line for the block's centre phase
+// (de bounds |earth(phi) - earth(phi_c)| <= 2 pi sin(i) |phi - phi_c|).
+// Along a piece [la, lb] of that central line the terms are bounded
+// separately: -comp/r1 and -mu/r2 by the closest approach of the widened
+// piece to each star, and the polynomial part -ssq/2 rho^2 + linear, which
+// is concave along a line, by its smaller end value less a Lipschitz
+// widening term. Pieces that do not clear pref are halved, up to a budget.
+template <Star S>
+struct ScanBound {
+  double mu, comp, ssq, pref, acc, rx, lam_lo, lam_hi, cosi, sini, phi1, dphi;
+  Vec3 r, c;
+  bool usable;
+
+  ScanBound(double q, double spin, double cosi_, double sini_, const Vec3& r_, double phi1_,
+            double dphi_, double acc_, double rref, double pref_)
+      : cosi(cosi_), sini(sini_), phi1(phi1_), dphi(dphi_), r(r_), c(star_cofm(S)) {
+    mu = q / (1.0 + q);
+    comp = 1.0 - mu;
+    ssq = spin * spin;
+    pref = pref_;
+    acc = acc_;
+    rx = rref + 2.0 * acc + 1e-6;
+    const double dist = (r - c).length();
+    lam_lo = (dist - rx) * (1.0 - 1e-12) - 1e-12;
+    lam_hi = (dist + rx) * (1.0 + 1e-12) + 1e-12;
+    // An element inside (or touching) the ball is left to the plain scan.
+    usable = lam_lo > 1e-9 && std::isfinite(pref) && std::isfinite(lam_hi);
+  }
+
+  double seg_dist(const Vec3& o, const Vec3& e, double la, double lb) const {
+    Vec3 d = r - o;
+    double t = -dot(d, e);
+    t = t < la ? la : (t > lb ? lb : t);
+    return (d + t * e).length();
+  }
+
+  // Lower bound of the potential over the points within `fat` of
+  // r + lam * e, lam in [la, lb]; *mag bounds the size of its terms.
+  double piece_bound(const Vec3& e, double fat, double la, double lb, double* mag) const {
+    const double d1 = seg_dist(Vec3(0.0, 0.0, 0.0), e, la, lb) - fat - 1e-12;
+    const double d2 = seg_dist(Vec3(1.0, 0.0, 0.0), e, la, lb) - fat - 1e-12;
+    if (!(d1 > 0.0 && d2 > 0.0)) {
+      *mag = 1.0;
+      return -HUGE_VAL;
+    }
+    const Vec3 A = r + la * e, B = r + lb * e;
+    const double ax = S == Star::Primary ? 0.0 : 1.0;  // spin axis x
+    const double rhoA = std::sqrt(sqr(A.x - ax) + sqr(A.y));
+    const double rhoB = std::sqrt(sqr(B.x - ax) + sqr(B.y));
+    // Linear term: +mu x about star 1, -comp x about star 2.
+    const double lin = S == Star::Primary ? mu : -comp;
+    const double PA = -0.5 * ssq * sqr(rhoA) + lin * A.x;
+    const double PB = -0.5 * ssq * sqr(rhoB) + lin * B.x;
+    const double rho = std::fmax(rhoA, rhoB);
+    const double widen = ssq * (rho * fat + 0.5 * fat * fat) + std::fabs(lin) * fat;
+    const double t1 = comp / d1, t2 = mu / d2;
+    *mag = 1.0 + t1 + t2 + 0.5 * ssq * sqr(rho + fat) +
+           std::fabs(lin) * (std::fmax(std::fabs(A.x), std::fabs(B.x)) + fat);
+    return std::fmin(PA, PB) - widen - t1 - t2;
+  }
+
+  // (A) for one piece: no point within `fat` of it inside B(c, rx) has a
+  // potential <= pref.
+  bool piece_clear(const Vec3& e, double de, double la, double lb) const {
+    const double fat = std::fmax(std::fabs(la), std::fabs(lb)) * de + 1e-12;
+    if (seg_dist(c, e, la, lb) - fat > rx) return true;
+    double mag;
+    const double bound = piece_bound(e, fat, la, lb, &mag);
+    return bound - 1e-9 * mag > pref;
+  }
+
+  // (A) for scan steps i0..i1 (inclusive).
+  bool block_clear(size_t i0, size_t i1, int budget) const {
+    const double ic = 0.5 * (static_cast<double>(i0) + static_cast<double>(i1));
+    const Vec3 e = set_earth(cosi, sini, phi1 + ic * dphi);
+    const double hw = 0.5 * static_cast<double>(i1 - i0) * dphi + 1e-14;
+    const double de = TWOPI * std::fabs(sini) * hw + 1e-14;
+    double stk[48][2];
+    int n = 0;
+    // Split first at the closest approach to the eclipsing star.
+    const double tc = dot(c - r, e);
+    if (tc > lam_lo && tc < lam_hi) {
+      stk[n][0] = tc; stk[n][1] = lam_hi; ++n;
+      stk[n][0] = lam_lo; stk[n][1] = tc; ++n;
+    } else {
+      stk[n][0] = lam_lo; stk[n][1] = lam_hi; ++n;
+    }
+    while (n > 0) {
+      --n;
+      const double a = stk[n][0], b = stk[n][1];
+      if (piece_clear(e, de, a, b)) continue;
+      if (--budget <= 0 || n + 2 > 48) return false;
+      const double m = 0.5 * (a + b);
+      stk[n][0] = m; stk[n][1] = b; ++n;
+      stk[n][0] = a; stk[n][1] = m; ++n;
+    }
+    return true;
+  }
+
+  // (B) at one chord end `end` (inward direction in = +1 at lam1, -1 at
+  // lam2; fend the potential fblink computes there; fm an upper bound of
+  // dbrent's best value, fm <= fend). Along the line,
+  // Phi(end + in t) >= fend + in D t - K t^2 / 2, with D the derivative at
+  // the end and -K a lower bound of the second derivative on the zone (for
+  // a point mass m, e.H.e = m (1 - 3 cos^2) / r^3 >= -2 m / r^3; the
+  // centrifugal part contributes >= -ssq). Required: that bound exceeds fm
+  // for t in [dl, acc + dl], where dl = 0 if fend > fm and dl = acc / 1000
+  // otherwise. No dbrent trial point lands within dl of the end while its
+  // best point stays out of the zone (trial points sit 2 acc inside the
+  // bracket, halfway to an end, or acc from the best point), so the best
+  // point, whose value is below fm, never enters the zone.
+  bool end_clear(const Vec3& e, double q, double spin, double end, double in, double fend,
+                 double fm) const {
+    const double dl = fend > fm ? 0.0 : 1e-3 * acc;
+    const double T = acc + dl;
+    const double la = in > 0.0 ? end : end - T, lb = la + T;
+    const double r1 = seg_dist(Vec3(0.0, 0.0, 0.0), e, la, lb) * (1.0 - 1e-12) - 1e-12;
+    const double r2 = seg_dist(Vec3(1.0, 0.0, 0.0), e, la, lb) * (1.0 - 1e-12) - 1e-12;
+    if (!(r1 > 0.0 && r2 > 0.0)) return false;
+    const double K = 2.0 * comp / (r1 * r1 * r1) + 2.0 * mu / (r2 * r2 * r2) + ssq;
+    const double D = in * drpot_along_los<S>(q, spin, e, r, end);
+    // Rounding of potentials and of D, with a wide margin.
+    const double tol = 1e-10 * (1.0 + comp / r1 + mu / r2 + ssq + std::fabs(fend));
+    auto g = [&](double t) { return (fend - fm) + D * t - 0.5 * K * t * t; };
+    if (g(dl) > tol && g(T) > tol) return true;
+    // Or the potential rises outward over [end - acc, end + acc] (|Phi''| <=
+    // K): a step of acc from a best point within acc of the end to beyond
+    // it is then worse by more than rounding, and dbrent returns at once.
+    return -(D + K * acc) * acc > tol;
+  }
+
+  // (B) for one step that has (A): true when fblink is certain to be false.
+  bool step_clear(double q, double spin, const Vec3& e, double rref) const {
+    double lam1, lam2;
+    if (!sphere_eclipse_los(e, r, c, rref, &lam1, &lam2)) return true;
+    if (lam1 == 0.0) return false;
+    if (lam2 - lam1 <= acc) return true;
+    // fblink's first three potentials. A minimum bracketed by the chord's
+    // midpoint keeps dbrent's best point at or below fmid; one bracketed
+    // deeper in the grid search, below min(f1, f2).
+    const double f1 = rpot_along_los<S>(q, spin, e, r, lam1);
+    const double f2 = rpot_along_los<S>(q, spin, e, r, lam2);
+    const double fmid = rpot_along_los<S>(q, spin, e, r, lam1 + (lam2 - lam1) / 2.0);
+    const double v = fmid < f1 && fmid < f2 ? fmid : std::fmin(f1, f2);
+    return end_clear(e, q, spin, lam1, 1.0, f1, v) && end_clear(e, q, spin, lam2, -1.0, f2, v);
+  }
+};
+
+// crates/roche/src/eclipse.rs:347-373. The steps are visited in the
+// reference's order and the first one whose fblink is true is returned;
+// for steps that ScanBound certifies, dbrent's false is taken without
+// running it.
 template <Star S>
 bool pot_min_search(double q, double cosi, double sini, double spin, const Vec3& r, double phi1,
                     double phi2, double acc, double rref, double pref, double* phi_out) {
   size_t nsteps = as_usize(std::ceil((phi2 - phi1) / (acc / TWOPI)));
   nsteps = std::min<size_t>(std::max<size_t>(nsteps, 10), 1000);
   double dphi = (phi2 - phi1) / static_cast<double>(nsteps);
+  const ScanBound<S> sb(q, spin, cosi, sini, r, phi1, dphi, acc, rref, pref);
+  const bool screen = tl_prescreen && sb.usable;
+  // Steps < clear_end have (A). Blocks double while they clear, up to 16
+  // steps (wider ones rarely clear: the fan of lines of sight gets too
+  // wide); a failed block halves the size and caps it for the rest of the
+  // scan, which runs toward the lobe.
+  size_t blk = 1, cap = 16, clear_end = 0;
   for (size_t i = 0; i <= nsteps; ++i) {
+    if (screen && i >= clear_end) {
+      for (;;) {
+        const size_t j = std::min(i + blk - 1, nsteps);
+        if (sb.block_clear(i, j, blk == 1 ? 4 : 12)) {
+          clear_end = j + 1;
+          blk = std::min<size_t>(blk * 2, cap);
+          break;
+        }
+        if (blk == 1) break;
+        blk /= 2;
+        cap = blk;
+      }
+    }
     double phi = phi1 + static_cast<double>(i) * dphi;
     Vec3 earth = set_earth(cosi, sini, phi);
-    if (fblink_ref<S>(q, spin, acc, earth, r, rref, pref)) {
+    if (screen && i < clear_end && sb.step_clear(q, spin, earth, rref)) {
+      ++tl_skipped;
+      continue;
+    }
+    ++tl_evaluated;
+    const bool hit = fblink_ref<S>(q, spin, acc, earth, r, rref, pref);
+    if (hit) {
       *phi_out = phi;
       return true;
     }
@@ -456,6 +692,17 @@ bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec
   check_q(q, "fblink");
   return star == Star::Primary ? fblink_t<Star::Primary>(q, spin, ffac, acc, earth, p)
                                : fblink_t<Star::Secondary>(q, spin, ffac, acc, earth, p);
+}
+
+bool set_eclipse_prescreen(bool on) {
+  bool old = tl_prescreen;
+  tl_prescreen = on;
+  return old;
+}
+
+void eclipse_prescreen_counts(long long* skipped, long long* evaluated) {
+  *skipped = tl_skipped;
+  *evaluated = tl_evaluated;
 }
 
 bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,

@@ -37,6 +37,14 @@ bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec
 bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,
                     const Vec3& r, double* ingress, double* egress);
 
+// The phase scan of ingress_egress skips steps whose line of sight provably
+// misses the eclipsing lobe (a certified lower bound of the potential lies
+// above the lobe's potential). Results are identical with and without it;
+// the switch is per thread, for self-checks. Returns the previous setting.
+bool set_eclipse_prescreen(bool on);
+// Scan steps skipped / scan steps run through fblink on this thread.
+void eclipse_prescreen_counts(long long* skipped, long long* evaluated);
+
 // crates/roche/src/disc_eclipse.rs
 EclipseVec disc_eclipse(double iangle, double rdisc1, double rdisc2, double beta, double height,
                         const Vec3& r);

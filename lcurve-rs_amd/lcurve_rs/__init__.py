@@ -384,6 +384,17 @@ class Model:
                                                       ctypes.byref(r), c))
         return r.value, {"full": c[0], "no_eclipse": c[1], "skip": c[2], "moments": c[3]}
 
+    def selfcheck_eclipse_prescreen(self):
+        """CPU check (no device): both stars' grids built with and without the
+        certified pre-screen of the eclipse phase scan. Returns a dict with
+        elements compared, mismatches (must be 0), scan steps skipped and
+        evaluated, and the build times in ms without and with it."""
+        out = (ctypes.c_longlong * 4)()
+        ms = np.zeros(2)
+        check(_native.lib().lch_selfcheck_eclipse_prescreen(self._h, out, _dptr(ms)))
+        return {"elements": out[0], "mismatches": out[1], "skipped": out[2], "evaluated": out[3],
+                "ms_plain": float(ms[0]), "ms_screened": float(ms[1])}
+
     def chisq_batch(self, data, names, values, scale=True):
         """Chi-squared of N parameter sets (python/src/lib.rs:696-734).
 

@@ -81,6 +81,7 @@ SIGNATURES = {
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
     "lch_batch_sfac": (INT, [VP, D]),
     "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong)]),
+    "lch_selfcheck_eclipse_prescreen": (INT, [VP, ctypes.POINTER(ctypes.c_longlong), D]),
     "lch_mlp_create": (INT, [VP, I, INT, ctypes.c_ulonglong, INT, ctypes.POINTER(VP)]),
     "lch_mlp_free": (None, [VP]),
     "lch_mlp_param_count": (ctypes.c_longlong, [VP]),
