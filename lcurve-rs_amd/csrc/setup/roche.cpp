@@ -457,13 +457,13 @@ thread_local long long tl_skipped = 0, tl_evaluated = 0;
 // widening term. Pieces that do not clear pref are halved, up to a budget.
 template <Star S>
 struct ScanBound {
-  double mu, comp, ssq, pref, acc, rx, lam_lo, lam_hi, cosi, sini, phi1, dphi;
+  double mu, comp, ssq, pref, acc, rx, lam_lo, lam_hi, cosi, sini;
   Vec3 r, c;
   bool usable;
 
-  ScanBound(double q, double spin, double cosi_, double sini_, const Vec3& r_, double phi1_,
-            double dphi_, double acc_, double rref, double pref_)
-      : cosi(cosi_), sini(sini_), phi1(phi1_), dphi(dphi_), r(r_), c(star_cofm(S)) {
+  ScanBound(double q, double spin, double cosi_, double sini_, const Vec3& r_, double acc_,
+            double rref, double pref_)
+      : cosi(cosi_), sini(sini_), r(r_), c(star_cofm(S)) {
     mu = q / (1.0 + q);
     comp = 1.0 - mu;
     ssq = spin * spin;
@@ -519,12 +519,10 @@ struct ScanBound {
     return bound - 1e-9 * mag > pref;
   }
 
-  // (A) for scan steps i0..i1 (inclusive).
-  bool block_clear(size_t i0, size_t i1, int budget) const {
-    const double ic = 0.5 * (static_cast<double>(i0) + static_cast<double>(i1));
-    const Vec3 e = set_earth(cosi, sini, phi1 + ic * dphi);
-    const double hw = 0.5 * static_cast<double>(i1 - i0) * dphi + 1e-14;
-    const double de = TWOPI * std::fabs(sini) * hw + 1e-14;
+  // (A) for the phases within hw of phi_c.
+  bool phases_clear(double phi_c, double hw, int budget) const {
+    const Vec3 e = set_earth(cosi, sini, phi_c);
+    const double de = TWOPI * std::fabs(sini) * (hw + 1e-14) + 1e-14;
     double stk[48][2];
     int n = 0;
     // Split first at the closest approach to the eclipsing star.
@@ -602,27 +600,36 @@ struct ScanBound {
 // running it.
 template <Star S>
 bool pot_min_search(double q, double cosi, double sini, double spin, const Vec3& r, double phi1,
-                    double phi2, double acc, double rref, double pref, double* phi_out) {
+                    double phi2, double acc, double rref, double pref, const ScanBound<S>& sb,
+                    bool screen, double* phi_out) {
   size_t nsteps = as_usize(std::ceil((phi2 - phi1) / (acc / TWOPI)));
   nsteps = std::min<size_t>(std::max<size_t>(nsteps, 10), 1000);
   double dphi = (phi2 - phi1) / static_cast<double>(nsteps);
-  const ScanBound<S> sb(q, spin, cosi, sini, r, phi1, dphi, acc, rref, pref);
-  const bool screen = tl_prescreen && sb.usable;
   // Steps < clear_end have (A). Blocks double while they clear, up to 16
   // steps (wider ones rarely clear: the fan of lines of sight gets too
   // wide); a failed block halves the size and caps it for the rest of the
   // scan, which runs toward the lobe.
-  size_t blk = 1, cap = 16, clear_end = 0;
+  // After a single step fails, the next (A) attempts wait 1, 2, 4, ...
+  // steps: near the lobe they keep failing.
+  size_t blk = 1, cap = 16, clear_end = 0, retry_at = 0, wait = 1;
   for (size_t i = 0; i <= nsteps; ++i) {
-    if (screen && i >= clear_end) {
+    if (screen && i >= clear_end && i >= retry_at) {
       for (;;) {
         const size_t j = std::min(i + blk - 1, nsteps);
-        if (sb.block_clear(i, j, blk == 1 ? 4 : 12)) {
+        // Steps i..j: phases phi1 + k dphi, within hw of the centre.
+        const double ic = 0.5 * (static_cast<double>(i) + static_cast<double>(j));
+        const double hw = 0.5 * static_cast<double>(j - i) * dphi;
+        if (sb.phases_clear(phi1 + ic * dphi, hw, blk == 1 ? 4 : 12)) {
           clear_end = j + 1;
           blk = std::min<size_t>(blk * 2, cap);
+          wait = 1;
           break;
         }
-        if (blk == 1) break;
+        if (blk == 1) {
+          retry_at = i + wait;
+          wait = std::min<size_t>(wait * 2, 16);
+          break;
+        }
         blk /= 2;
         cap = blk;
       }
@@ -656,9 +663,14 @@ bool ingress_egress_t(double q, double spin, double ffac, double iangle, double 
   double phi1, phi2, lam1, lam2;
   if (!sphere_eclipse_phase(cosi, sini, r, cofm, rref, &phi1, &phi2, &lam1, &lam2)) return false;
   double acc = 2.0 * std::sqrt(2.0 * TWOPI * (lam2 - lam1) * delta);
+  const ScanBound<S> sb(q, spin, cosi, sini, r, acc, rref, pref);
+  const bool screen = tl_prescreen && sb.usable;
   double phi;
-  if (!pot_min_search<S>(q, cosi, sini, spin, r, phi1, phi2, acc, rref, pref, &phi)) return false;
+  if (!pot_min_search<S>(q, cosi, sini, spin, r, phi1, phi2, acc, rref, pref, sb, screen, &phi))
+    return false;
 
+  // The binary chops (eclipse.rs:302-332): their midpoints close in on the
+  // lobe's limb, where nothing can be certified, so fblink runs on each.
   double pin = phi, pout = phi1;
   while (std::fabs(pin - pout) > delta) {
     double pmid = (pin + pout) / 2.0;
