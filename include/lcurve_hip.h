@@ -208,13 +208,14 @@ int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const doub
                               const int* ndiv, int n, double* max_rel_diff, long long* counts);
 
 /* Builds both stars' grids (fine and coarse) twice, with and without the
- * certified pre-screen that lets the eclipse phase scan skip steps whose
- * line of sight provably misses the eclipsing lobe, and compares them bit
- * for bit. out[0] = elements compared, out[1] = elements that differ (must
+ * certified short cuts of the CPU setup (the eclipse phase scan skips steps
+ * whose line of sight provably misses the eclipsing lobe; the surface-radius
+ * chop decides midpoints from a certified Taylor bound), and compares them
+ * bit for bit. out[0] = elements compared, out[1] = elements that differ (must
  * be 0), out[2] = scan steps skipped, out[3] = scan steps evaluated;
  * ms[0..1] = build time without / with the pre-screen (ms may be null).
- * No counterpart in the reference: it checks this engine's setup speed-up
- * of crates/roche/src/eclipse.rs:347-373. */
+ * No counterpart in the reference: it checks this engine's setup speed-ups
+ * of crates/roche/src/eclipse.rs:347-373 and surface.rs:82-102. */
 int lch_selfcheck_eclipse_prescreen(const lch_model* m, long long* out, double* ms);
 
 #ifdef __cplusplus

@@ -14,6 +14,11 @@ namespace {
 constexpr int NMAX = 1000;
 constexpr double LEPS = 1.0e-12;
 
+// Certified short cuts in face() and in ingress_egress's phase scan: on by
+// default, switchable per thread for self-checks; counters of the scan.
+thread_local bool tl_prescreen = true;
+thread_local long long tl_skipped = 0, tl_evaluated = 0;
+
 // Newton iteration on the quintic for an L1-like point.
 // crates/roche/src/lagrange.rs:8-31 (same loop shape in xl11 / xl12)
 double quintic_root(double x, double a1, double a2, double a3, double a4, double a5, double a6,
@@ -275,12 +280,36 @@ void ref_sphere(double q, Star star, double spin, double ffac, double* rref, dou
 }
 
 // crates/roche/src/surface.rs:54-121
+//
+// The binary chop for the surface radius compares the potential at each
+// midpoint with pref. Once the bracket is narrower than kFaceTaylor, the
+// chop takes a Newton step to a centre c near the root and evaluates the
+// potential F and its radial derivative D there; after that a midpoint's
+// comparison is decided without evaluating the potential whenever
+//   F + D h + K h^2 / 2 (h = mid - c, K the bound of the radial second
+// derivative in the direction that matters) clears pref by far more than
+// the rounding of one potential evaluation. Along the ray from the star's
+// own centre the second derivative is exactly -2 m_own / rho^3 from its own
+// mass, -ssq (dx^2 + dy^2) from the rotation and within
+// [-2 m_oth, m_oth] / r_oth^3 from the other star (r_oth >= 1 - hi). A
+// decided comparison is the one the evaluation would give, so the chop
+// visits the same midpoints and returns the same radius to the bit;
+// undecided midpoints (within ~1e-11 of the root) are evaluated.
+namespace {
+constexpr double kFaceTaylor = 1e-2;
+}
+
 void face(double q, Star star, double spin, const Vec3& dirn, double rref, double pref, double acc,
           Vec3* pvec_out, Vec3* dvec_out, double* radius_out, double* gravity_out) {
   Vec3 cofm = star_cofm(star);
   auto pot_fn = [&](double radius) {
     Vec3 p = cofm + radius * dirn;
     return pot_star(q, star, spin, p);
+  };
+  auto dpot_fn = [&](double radius) {
+    Vec3 p = cofm + radius * dirn;
+    Vec3 g = star == Star::Primary ? drpot1_u(q, spin, p) : drpot2_u(q, spin, p);
+    return dot(g, dirn);
   };
   double lo = rref / 2.0;
   double hi = rref;
@@ -291,11 +320,44 @@ void face(double q, Star star, double spin, const Vec3& dirn, double rref, doubl
     tref = pot_fn(lo);
     if (tref > pref) hi = lo;
   }
+  // Taylor centre state: 0 not set, 1 set, -1 unusable.
+  int taylor = tl_prescreen ? 0 : -1;
+  double c0 = 0.0, F0 = 0.0, D0 = 0.0, klo = 0.0, khi = 0.0, tol = 0.0;
   for (int i = 0; i < 100; ++i) {
     if (hi - lo <= acc) break;
     double mid = (lo + hi) / 2.0;
-    double pot_mid = pot_fn(mid);
-    if (pot_mid < pref) lo = mid;
+    if (taylor == 0 && hi - lo <= kFaceTaylor && lo > 0.0 && hi < 1.0) {
+      const double Fm = pot_fn(mid), Dm = dpot_fn(mid);
+      double c = Dm > 0.0 ? mid - (Fm - pref) / Dm : mid;
+      c = c < lo ? lo : (c > hi ? hi : c);
+      F0 = pot_fn(c);
+      D0 = dpot_fn(c);
+      c0 = c;
+      const double mu = q / (1.0 + q);
+      const double mown = star == Star::Primary ? 1.0 - mu : mu;
+      const double moth = 1.0 - mown;
+      const double roth = 1.0 - hi;
+      const double cen = sqr(spin) * (sqr(dirn.x) + sqr(dirn.y));
+      klo = -2.0 * mown / (lo * lo * lo) - 2.0 * moth / (roth * roth * roth) - cen;
+      khi = -2.0 * mown / (hi * hi * hi) + moth / (roth * roth * roth) - cen;
+      tol = 1e-12 * (1.0 + mown / lo + moth / roth + sqr(spin) + std::fabs(pref));
+      taylor = D0 > 0.0 && std::isfinite(F0) && std::isfinite(klo) ? 1 : -1;
+    }
+    bool below;
+    int known = -1;
+    if (taylor == 1) {
+      const double h = mid - c0;
+      const double lin = F0 + D0 * h;
+      if (lin + 0.5 * khi * h * h + tol < pref) known = 1;
+      else if (lin + 0.5 * klo * h * h - tol > pref) known = 0;
+    }
+    if (known >= 0) {
+      below = known == 1;
+    } else {
+      double pot_mid = pot_fn(mid);
+      below = pot_mid < pref;
+    }
+    if (below) lo = mid;
     else hi = mid;
   }
   double radius = (lo + hi) / 2.0;
@@ -415,9 +477,6 @@ bool fblink_t(double q, double spin, double ffac, double acc, const Vec3& earth,
   ref_sphere(q, S, spin, ffac, &rref, &pref);
   return fblink_ref<S>(q, spin, acc, earth, p, rref, pref);
 }
-
-thread_local bool tl_prescreen = true;
-thread_local long long tl_skipped = 0, tl_evaluated = 0;
 
 // Certified short cut for the phase scan of ingress_egress.
 //

@@ -37,10 +37,12 @@ bool fblink(double q, Star star, double spin, double ffac, double acc, const Vec
 bool ingress_egress(double q, Star star, double spin, double ffac, double iangle, double delta,
                     const Vec3& r, double* ingress, double* egress);
 
-// The phase scan of ingress_egress skips steps whose line of sight provably
-// misses the eclipsing lobe (a certified lower bound of the potential lies
-// above the lobe's potential). Results are identical with and without it;
-// the switch is per thread, for self-checks. Returns the previous setting.
+// Certified short cuts: the phase scan of ingress_egress skips steps whose
+// line of sight provably misses the eclipsing lobe (a certified lower bound
+// of the potential lies above the lobe's potential), and face()'s chop
+// decides midpoints from a certified Taylor bound. Results are identical
+// with and without them; the switch is per thread, for self-checks.
+// Returns the previous setting.
 bool set_eclipse_prescreen(bool on);
 // Scan steps skipped / scan steps run through fblink on this thread.
 void eclipse_prescreen_counts(long long* skipped, long long* evaluated);

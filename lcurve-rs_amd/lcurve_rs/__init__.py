@@ -386,7 +386,8 @@ class Model:
 
     def selfcheck_eclipse_prescreen(self):
         """CPU check (no device): both stars' grids built with and without the
-        certified pre-screen of the eclipse phase scan. Returns a dict with
+        certified setup short cuts (eclipse phase scan pre-screen, Taylor-
+        decided surface chop). Returns a dict with
         elements compared, mismatches (must be 0), scan steps skipped and
         evaluated, and the build times in ms without and with it."""
         out = (ctypes.c_longlong * 4)()

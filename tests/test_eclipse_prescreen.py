@@ -1,10 +1,12 @@
-"""CPU checks of the certified pre-screen in the eclipse phase scan.
+"""CPU checks of the certified short cuts in the CPU setup.
 
 ingress_egress (crates/roche/src/eclipse.rs:270-373) scans up to 1000 phases
 per element with fblink until the line of sight first dips into the
-eclipsing star's Roche lobe. The setup skips a scan step only when a rigorous
-bound proves fblink false there, so the grids must come out bit-identical
-with and without it. Model.selfcheck_eclipse_prescreen builds both stars'
+eclipsing star's Roche lobe; the setup skips a scan step only when a rigorous
+bound proves fblink false there. face() (surface.rs:54-121) chops for the
+surface radius; the setup decides a midpoint's comparison without the
+potential only when a certified Taylor bound fixes it. The grids must come
+out bit-identical with and without both. Model.selfcheck_eclipse_prescreen builds both stars'
 fine and coarse grids both ways and counts elements whose position, normal,
 area, gravity or eclipse intervals differ in any bit.
 """
