@@ -6,9 +6,9 @@ eclipsing star's Roche lobe; the setup skips a scan step only when a rigorous
 bound proves fblink false there. face() (surface.rs:54-121) chops for the
 surface radius; the setup decides a midpoint's comparison without the
 potential only when a certified Taylor bound fixes it. The grids must come
-out bit-identical with and without both. Model.selfcheck_eclipse_prescreen builds both stars'
-fine and coarse grids both ways and counts elements whose position, normal,
-area, gravity or eclipse intervals differ in any bit.
+out bit-identical with and without both. Model.selfcheck_eclipse_prescreen
+builds both stars' fine and coarse grids both ways and counts elements whose
+position, normal, area, gravity or eclipse intervals differ in any bit.
 """
 import numpy as np
 import pytest
