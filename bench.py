@@ -174,13 +174,15 @@ def roofline(workload, st, ms_flux):
     traffic = pm["hbm_bytes_per_launch"] if pm else None
     return {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": traffic,
-            "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
+            # the flux pass: flux_items_kernel, then limb_kernel on the limb lanes
+            "kernel": "flux_items_kernel+limb_kernel", "kernel_ms": round(ms_flux, 4),
             "flop_per_evaluated_pair": FLOP_PER_PAIR, "flop_per_moment_patch_lane": FLOP_PER_MOMENT_LANE,
             # the per-pair figure is that of the linear limb-darkening star path; the
             # Claret, disc and spot terms of disc_spot_300k cost more, so there it
             # understates the work done
             "flop_counts_exact": workload != "disc_spot_300k",
             "pairs_evaluated_per_launch": pairs, "moment_patch_lanes_per_launch": mom_lanes,
+            "limb_pairs_per_launch": st["pairs_limb"],
             "reference_visits_per_launch": st["visits"],
             "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
             "hbm_algorithmic_bytes": alg_bytes,
@@ -365,6 +367,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0,
                     help="fix elements per work item (skips autotuning; used for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-limb", action="store_true",
+                    help="evaluate no-eclipse patches on all lanes of a tile (no limb lanes)")
     ap.add_argument("--parity-sets", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a streaming kernel of known traffic (for PMC scaling)")
@@ -394,6 +398,7 @@ def main():
         return
     if args.chunk > 0:
         lc.set_tuning(args.chunk, 0)
+    lc.set_limb(not args.no_limb)
 
     batch, info = build_workload(args.workload, args.nsets)
     scale = info["scale"]
@@ -439,7 +444,8 @@ def main():
            "max_rel_flux_err_vs_fp64": flux_err, "parity_sets": nchk,
            "max_rel_chisq_err_vs_fp64": chi_err,
            "cpu_setup_s": round(st["setup_seconds"], 2), "upload_s": round(st["upload_seconds"], 2),
-           "tuning": {"chunk": st["chunk"], "persistent": st["persistent"], "items": st["n_items"]},
+           "tuning": {"chunk": st["chunk"], "persistent": st["persistent"], "items": st["n_items"],
+                      "limb": not args.no_limb, "limb_chunks": st["limb_chunks"]},
            "roofline": roof}
     if not args.no_cpu_baseline:
         log("CPU baseline (FP64 oracle, bounded sample) ...")

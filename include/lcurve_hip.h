@@ -67,6 +67,9 @@ typedef struct lch_batch_stats {
   double upload_seconds;  /* host-to-device copies and table builds */
   int chunk;              /* elements per work item */
   int persistent;         /* 1 if the persistent launch form is selected */
+  long long pairs_limb;   /* of pairs_evaluated: pairs of limb lanes (lanes that see
+                             a no-eclipse patch only in part, regrouped across tiles) */
+  long long limb_chunks;  /* workgroups of the limb pass */
 } lch_batch_stats;
 
 int lch_version(void);
@@ -78,6 +81,11 @@ int lch_ctx_create(int device, lch_ctx** out);
 void lch_ctx_destroy(lch_ctx* ctx);
 /* Override the launch tuning (chunk > 0 and persistent in {0,1}). */
 int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
+/* Limb lanes on (1, the default) or off (0) for batches built from now on:
+ * no-eclipse patches split per lane into moments / limb lanes / nothing,
+ * versus evaluated element by element on all 64 lanes of the tile. Results
+ * agree to FP32 rounding either way. No counterpart in the reference. */
+int lch_ctx_set_limb(lch_ctx* ctx, int on);
 int lch_ctx_sync(lch_ctx* ctx);
 
 /* ---- model parameter files ---- */
@@ -202,10 +210,13 @@ int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* 
  * direct (skip, closed-form patch moments, no eclipse test, full test) and
  * by brute force. max_rel_diff = max over (lane, segment) of
  * |coded - brute| / sum |terms|; counts[0..3] = (patch, tile) pairs coded
- * full / no-eclipse / skip / moments (either output may be null). No
- * counterpart in the reference: it checks this engine's own work skipping. */
+ * full / no-eclipse / skip / moments; split[0..2] = lanes of no-eclipse
+ * pairs split per lane into seen whole (moments) / in part (element by
+ * element) / not at all (any output may be null). No counterpart in the
+ * reference: it checks this engine's own work skipping. */
 int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const double* expose,
-                              const int* ndiv, int n, double* max_rel_diff, long long* counts);
+                              const int* ndiv, int n, double* max_rel_diff, long long* counts,
+                              long long* split);
 
 /* Builds both stars' grids (fine and coarse) twice, with and without the
  * certified short cuts of the CPU setup (the eclipse phase scan skips steps

@@ -140,6 +140,12 @@ int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent) {
   return LCH_OK;
 }
 
+int lch_ctx_set_limb(lch_ctx* ctx, int on) {
+  if (!ctx || (on != 0 && on != 1)) return fail(LCH_ERR_INVALID, "bad limb setting");
+  ctx->impl.tuning.limb = on;
+  return LCH_OK;
+}
+
 int lch_ctx_sync(lch_ctx* ctx) {
   if (!ctx) return fail(LCH_ERR_INVALID, "null context");
   hipError_t e = hipStreamSynchronize(ctx->impl.stream());
@@ -407,6 +413,8 @@ int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
   out->upload_seconds = b->upload_s;
   out->chunk = static_cast<int>(B.chunk());
   out->persistent = B.persistent();
+  out->pairs_limb = static_cast<long long>(B.pairs_limb());
+  out->limb_chunks = static_cast<long long>(B.n_limb_chunks());
   return LCH_OK;
 }
 
@@ -441,7 +449,8 @@ int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const do
 }  // extern "C"
 
 int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const double* expose,
-                              const int* ndiv, int n, double* max_rel_diff, long long* counts) {
+                              const int* ndiv, int n, double* max_rel_diff, long long* counts,
+                              long long* split) {
   if (!m) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     const std::vector<double> zero(static_cast<size_t>(std::max(n, 0)), 0.0), one(zero.size(), 1.0);
@@ -450,6 +459,8 @@ int lch_selfcheck_patch_codes(const lch_model* m, const double* time, const doub
     if (max_rel_diff) *max_rel_diff = c.max_rel;
     if (counts)
       for (int i = 0; i < 4; ++i) counts[i] = c.counts[i];
+    if (split)
+      for (int i = 0; i < 3; ++i) split[i] = c.split_lanes[i];
     return LCH_OK;
   });
 }

@@ -155,6 +155,7 @@ void patch_moments(const uint8_t* pr, uint32_t stride, double* M) {
     M[16] += 3.0 * w * z * z * x;
     M[17] += 3.0 * w * z * z * y;
     M[18] += 6.0 * w * x * y * z;
+    M[19] += std::fabs(w);  // size bound for the limb lanes' fixed point
   }
 }
 
@@ -414,6 +415,13 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     }
   }
   S.n_tiles = ntile;
+  // Normalised FP32 earth vector per lane, for the per-lane split.
+  std::vector<lc::Vec3> lane_en(S.lane_e.size());
+  for (size_t l = 0; l < S.lane_e.size(); ++l) {
+    lc::Vec3 v(S.lane_e[l].x, S.lane_e[l].y, S.lane_e[l].z);
+    const double len = v.length();
+    lane_en[l] = len > 0.0 ? (1.0 / len) * v : v;
+  }
   S.ptcode.assign(cones.size() * ntile, PT_FULL);
   S.moms.assign(cones.size() * MOM_STRIDE, 0.0);
   for (size_t g = 0; g < S.segs.size(); ++g) {
@@ -437,6 +445,67 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     const double thr_front = std::max(thr, 0.0);
     const bool front_ok = mom_ok && thr_front < 0.99;
     const double beta_front = front_ok ? std::acos(thr_front) : 0.0;
+    // Per-lane version of the skip / seen-whole tests for a PT_FREE pair:
+    // every element of the patch is free of eclipses at every lane of the
+    // tile, so a lane whose line of sight is within gf of the patch axis
+    // sees it whole and one beyond gam sees none of it.
+    auto split_lanes = [&](uint32_t patch, uint32_t t, const lc::Vec3& pa, double plen, double gam,
+                           double gf) {
+      if (!front_ok || !(plen > 0.0)) return;
+      const bool can_all = gf > 1e-3, can_none = cull && gam < lc::PI - 1e-3;
+      const double cf = can_all ? std::cos(gf) : 2.0, cg = can_none ? std::cos(gam) : -2.0;
+      uint64_t all = 0, some = 0;
+      for (uint32_t j = 0; j < TILE; ++j) {
+        const uint32_t l = t * TILE + j;
+        const uint32_t m = S.lane_meta[l];
+        if (!(m & META_VALID) || !(sg.use_mask & cp_bit(meta_class(m), meta_ptype(m)))) continue;
+        const double c = lc::dot(lane_en[l], pa);
+        if (c > cf) all |= 1ull << j;
+        else if (!(c < cg)) some |= 1ull << j;
+      }
+      // The cone leaves a band of lanes undecided; decide them from the
+      // elements themselves: mu of every element against the cut-offs with
+      // a 1e-6 margin, which covers the FP32 rounding of mu here and in the
+      // kernel (a few 1e-7).
+      if (some) {
+        float lx[TILE], ly[TILE], lz[TILE], mn[TILE], mx[TILE];
+        uint32_t js[TILE];
+        uint32_t n = 0;
+        for (uint64_t b = some; b; b &= b - 1ull) {
+          const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(b));
+          const float4 e = S.lane_e[t * TILE + j];
+          js[n] = j;
+          lx[n] = e.x;
+          ly[n] = e.y;
+          lz[n] = e.z;
+          mn[n] = 4.0f;
+          mx[n] = -4.0f;
+          ++n;
+        }
+        const uint8_t* pr = S.recs.data() + sg.rec_off + static_cast<uint64_t>(patch - sg.patch_off) * PATCH * stride;
+        for (uint32_t i = 0; i < PATCH; ++i) {
+          float f[4];
+          std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
+          if (f[3] == 0.0f) continue;  // zero flux adds 0 either way
+          for (uint32_t k = 0; k < n; ++k) {
+            const float mu = lx[k] * f[0] + ly[k] * f[1] + lz[k] * f[2];
+            mn[k] = std::min(mn[k], mu);
+            mx[k] = std::max(mx[k], mu);
+          }
+        }
+        const float whole = static_cast<float>(thr_front + 1e-6), none = static_cast<float>(thr - 1e-6);
+        for (uint32_t k = 0; k < n; ++k) {
+          const uint64_t bit = 1ull << js[k];
+          if (mn[k] > whole) {
+            all |= bit;
+            some &= ~bit;
+          } else if (mx[k] < none) {
+            some &= ~bit;
+          }
+        }
+      }
+      S.lsplit.push_back({patch * ntile + t, all, some});
+    };
     for (uint32_t q = 0; q < sg.n / PATCH; ++q) {
       const Cone& pc = cones[sg.patch_off + q];
       lc::Vec3 pa(pc.x, pc.y, pc.z);
@@ -474,6 +543,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
         const bool front = front_ok && plen > 0.0 && gf > 1e-3 && emin > std::cos(gf);
         if (sg.necl == 0) {
           code = front ? PT_MOM : PT_FREE;
+          if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, gam, gf);
           continue;
         }
         // Point::visible (types.rs:50-58) over the tile's phase range
@@ -498,6 +568,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           all_ecl = all_ecl && ecl;
         }
         code = all_ecl ? PT_SKIP : (all_vis ? (front ? PT_MOM : PT_FREE) : PT_FULL);
+        if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, gam, gf);
       }
     }
   }

@@ -42,6 +42,16 @@ struct SetPack {
   std::vector<uint8_t> ptcode;
   uint32_t n_tiles = 0;
   std::vector<double> moms;  // MOM_STRIDE per patch (zero where unused)
+  // Per-lane split of PT_FREE (patch, tile) pairs of moment-capable
+  // segments, over the lanes that add the segment: `all` lanes see every
+  // element of the patch with mu above the cut-off (the moments give the
+  // exact sum), `some` lanes see part of it (element by element); the
+  // other lanes see none of it. Sorted by pt = patch * n_tiles + tile.
+  struct LaneSplit {
+    uint32_t pt;
+    uint64_t all, some;
+  };
+  std::vector<LaneSplit> lsplit;
   std::vector<double> lane_phi;
   std::vector<float4> lane_e;  // earth vector per lane (x, y, z, 0), FP32
   std::vector<uint32_t> lane_meta;

@@ -19,6 +19,18 @@ void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess)
     throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+// A no-eclipse patch goes to limb lanes when at most this many of a tile's
+// lanes see part of it (more, and the item's own loop costs no more).
+constexpr int kLimbMaxPart = 48;
+
+const SetPack::LaneSplit* find_split(const std::vector<SetPack::LaneSplit>& v, uint32_t pt) {
+  auto it = std::lower_bound(v.begin(), v.end(), pt,
+                             [](const SetPack::LaneSplit& a, uint32_t x) { return a.pt < x; });
+  return it != v.end() && it->pt == pt ? &*it : nullptr;
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 }  // namespace
 
 // ----------------------------------------------------------------- context
@@ -66,6 +78,7 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
   sets_.resize(ns);
   pt_base_.assign(ns, 0);
   pt_tiles_.assign(ns, 0);
+  lsplit_.resize(ns);
 
   // Global offsets.
   uint64_t rec_total = 0;
@@ -104,6 +117,7 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
     pt_base_[s] = ptcode_.size();
     pt_tiles_[s] = P.n_tiles;
     ptcode_.insert(ptcode_.end(), P.ptcode.begin(), P.ptcode.end());
+    lsplit_[s] = std::move(P.lsplit);
     if (P.moms.size() % MOM_STRIDE) throw DeviceError("patch moments out of step");
     const uint64_t mom_base = moms_.size() / MOM_STRIDE;
     moms_.insert(moms_.end(), P.moms.begin(), P.moms.end());
@@ -176,6 +190,7 @@ Batch::~Batch() {
     if (e) (void)hipEventDestroy(e);
   if (d_items_) (void)hipFree(d_items_);
   if (d_partials_) (void)hipFree(d_partials_);
+  if (d_limb_) (void)hipFree(d_limb_);
   if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
   if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
   dfree_all();
@@ -249,6 +264,12 @@ void Batch::build_items(uint32_t chunk) {
   release_graph();
   chunk_ = chunk;
   chunk_info_.clear();
+  limb_chunks_.clear();
+  limb_items_.clear();
+  limb_idx_.clear();
+  limb_lanes_.clear();
+  lane_masks_.clear();
+  pairs_limb_ = 0;
   std::vector<ItemDev> items;
   uint64_t prow = 0;
   size_t seg_i = 0;
@@ -276,28 +297,122 @@ void Batch::build_items(uint32_t chunk) {
         const SegDev& sg = segs_[g];
         uint32_t c = (sg.n + chunk - 1) / chunk;
         for (uint32_t k = 0; k < c; ++k) {
+          const uint32_t e0 = k * chunk, ne = std::min(chunk, sg.n - k * chunk), np = ne / PATCH;
+          // Tentative items of this chunk, their pmom lane masks, and the
+          // limb lanes of each patch (global lane indices, tile order).
+          struct Tent {
+            ItemDev it;
+            std::vector<uint64_t> masks;
+            bool limb_row = false;
+          };
+          std::vector<Tent> tent;
+          std::vector<std::vector<uint32_t>> plan(np);
           for (uint32_t t = 0; t < ntile; ++t) {
             if (!(tile_cp_[sd.lane_off / TILE + t] & sg.use_mask)) continue;
-            ItemDev it;
+            Tent tt;
+            ItemDev& it = tt.it;
             it.seg = static_cast<uint32_t>(g);
-            it.e0 = k * chunk;
-            it.ne = std::min(chunk, sg.n - k * chunk);
+            it.e0 = e0;
+            it.ne = ne;
             it.lane0 = sd.lane_off + t * TILE;
             it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
-            it.todo = it.free = it.mom = 0;
-            const uint32_t np = it.ne / PATCH;
+            it.todo = it.free = it.mom = it.pmom = 0;
+            it.lm_off = it.pad = 0;
             for (uint32_t q = 0; q < np; ++q) {
-              const uint64_t ci = pt_base_[s] + static_cast<uint64_t>(sg.patch_off + it.e0 / PATCH + q) * pt_tiles_[s] + t;
+              const uint32_t pt = (sg.patch_off + e0 / PATCH + q) * pt_tiles_[s] + t;
+              const uint64_t ci = pt_base_[s] + pt;
               if (ci >= ptcode_.size()) throw DeviceError("patch code out of range");
               const uint8_t code = ptcode_[ci];
+              if (ctx_.tuning.limb && code == PT_FREE) {
+                const SetPack::LaneSplit* spl = find_split(lsplit_[s], pt);
+                if (spl && __builtin_popcountll(spl->some) <= kLimbMaxPart) {
+                  it.pmom |= 1ull << q;
+                  tt.masks.push_back(spl->all);
+                  for (uint64_t m = spl->some; m; m &= m - 1ull)
+                    plan[q].push_back(it.lane0 + static_cast<uint32_t>(__builtin_ctzll(m)));
+                  if (spl->some) tt.limb_row = true;
+                  continue;
+                }
+              }
               if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
               if (code == PT_FREE) it.free |= 1ull << q;
               if (code == PT_MOM) it.mom |= 1ull << q;
             }
+            tent.push_back(std::move(tt));
+          }
+          // The chunk's limb lanes; too many for one workgroup's LDS: the
+          // patches go back to the items' element loops.
+          std::vector<uint32_t> uni;
+          for (const auto& v : plan) uni.insert(uni.end(), v.begin(), v.end());
+          std::sort(uni.begin(), uni.end());
+          uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+          if (uni.size() > LIMB_UNION_MAX) {
+            for (Tent& tt : tent) {
+              tt.it.todo |= tt.it.pmom;
+              tt.it.free |= tt.it.pmom;
+              tt.it.pmom = 0;
+              tt.masks.clear();
+              tt.limb_row = false;
+            }
+            uni.clear();
+          }
+          if (!uni.empty()) {
+            LimbChunk lc;
+            lc.seg = static_cast<uint32_t>(g);
+            lc.chunk = chunk0[g - g0] + k;
+            // Fixed-point scale: a lane's total over the chunk's limb
+            // patches is at most sum |w| * max |I(m)|, m in [0, 1].
+            {
+              const bool s1 = sg.kind == KIND_STAR1;
+              double imax = 1.0;
+              if (sg.kind <= KIND_STAR2) {
+                const float* a = s1 ? sd.ldc1 : sd.ldc2;
+                imax += std::fabs(a[0]) + std::fabs(a[1]) + std::fabs(a[2]) + std::fabs(a[3]);
+              } else if (sg.kind != KIND_SPOT) {
+                imax += std::fabs(sd.lin_disc) + std::fabs(sd.quad_disc);
+              }
+              double wsum = 0.0;
+              for (uint32_t q = 0; q < np; ++q) {
+                if (plan[q].empty()) continue;
+                const size_t mi = static_cast<size_t>(sg.mom_off + e0 / PATCH + q) * MOM_STRIDE + 19;
+                if (mi >= moms_.size()) throw DeviceError("limb patch moments out of range");
+                wsum += moms_[mi];
+              }
+              const double bound = 1.01 * imax * wsum;
+              int ex = 0;
+              if (bound > 0.0) (void)std::frexp(bound, &ex);  // bound < 2^ex
+              lc.shift = bound > 0.0 ? 61 - ex : 0;
+              lc.pad = 0;
+            }
+            lc.item0 = static_cast<uint32_t>(limb_items_.size());
+            lc.lane_off = static_cast<uint32_t>(limb_lanes_.size());
+            lc.n_lanes = static_cast<uint32_t>(uni.size());
+            limb_lanes_.insert(limb_lanes_.end(), uni.begin(), uni.end());
+            for (uint32_t q = 0; q < np; ++q) {
+              const std::vector<uint32_t>& v = plan[q];
+              for (size_t i0 = 0; i0 < v.size(); i0 += TILE) {
+                limb_items_.push_back(LimbItem{e0 + q * PATCH, 0u});
+                for (size_t i = i0; i < i0 + TILE; ++i) {
+                  uint16_t u = 0xFFFFu;
+                  if (i < v.size()) {
+                    u = static_cast<uint16_t>(std::lower_bound(uni.begin(), uni.end(), v[i]) - uni.begin());
+                    pairs_limb_ += PATCH;
+                  }
+                  limb_idx_.push_back(u);
+                }
+              }
+            }
+            lc.n_items = static_cast<uint32_t>(limb_items_.size()) - lc.item0;
+            limb_chunks_.push_back(lc);
+          }
+          for (Tent& tt : tent) {
             // An item adding zero to every lane is not launched: its
-            // partial row stays at the zero written once below.
-            if (it.todo == 0 && it.mom == 0) continue;
-            items.push_back(it);
+            // partial row stays at the zero written once below. A row that
+            // limb lanes add to is rewritten by its item on every launch.
+            if (tt.it.todo == 0 && tt.it.mom == 0 && tt.it.pmom == 0 && !tt.limb_row) continue;
+            tt.it.lm_off = static_cast<uint32_t>(lane_masks_.size());
+            lane_masks_.insert(lane_masks_.end(), tt.masks.begin(), tt.masks.end());
+            items.push_back(tt.it);
           }
         }
       }
@@ -319,6 +434,7 @@ void Batch::build_items(uint32_t chunk) {
     pairs_eval_ += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
     pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom)) * PATCH * TILE;
   }
+  pairs_eval_ += pairs_limb_;
   // Host-side bounds check of every item against the buffers it indexes.
   for (const ItemDev& it : items) {
     if (it.seg >= segs_.size()) throw DeviceError("item segment out of range");
@@ -338,17 +454,77 @@ void Batch::build_items(uint32_t chunk) {
     const uint64_t live = it.ne / PATCH == 64 ? ~0ull : (1ull << (it.ne / PATCH)) - 1ull;
     if ((it.todo & ~live) || (it.free & ~it.todo) || (it.mom & ~live) || (it.mom & it.todo))
       throw DeviceError("item patch masks out of range");
-    if (it.mom && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
+    if ((it.mom || it.pmom) && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
       throw DeviceError("item moments out of range");
+    if ((it.pmom & ~live) || (it.pmom & (it.todo | it.mom)) ||
+        static_cast<uint64_t>(it.lm_off) + __builtin_popcountll(it.pmom) > lane_masks_.size())
+      throw DeviceError("item lane masks out of range");
   }
+  for (const LimbChunk& lc : limb_chunks_) {
+    if (lc.seg >= segs_.size()) throw DeviceError("limb chunk segment out of range");
+    const SegDev& sg = segs_[lc.seg];
+    const SetDev& sd = sets_[sg.set];
+    if (!(sd.flags & SET_VALID) || lc.chunk >= sd.n_chunks) throw DeviceError("limb chunk out of range");
+    if ((sg.flags & SEG_POSN) || sg.necl > static_cast<uint32_t>(MAX_ECL)) throw DeviceError("limb chunk segment kind");
+    if (lc.n_lanes == 0 || lc.n_lanes > LIMB_UNION_MAX ||
+        static_cast<uint64_t>(lc.lane_off) + lc.n_lanes > limb_lanes_.size())
+      throw DeviceError("limb chunk lanes out of range");
+    if (static_cast<uint64_t>(lc.item0) + lc.n_items > limb_items_.size()) throw DeviceError("limb items out of range");
+    if (lc.shift < -1000 || lc.shift > 1000) throw DeviceError("limb fixed-point scale out of range");
+    for (uint32_t u = 0; u < lc.n_lanes; ++u) {
+      const uint32_t gl = limb_lanes_[lc.lane_off + u];
+      if (gl < sd.lane_off || gl >= sd.lane_off + sd.n_lanes) throw DeviceError("limb lane out of range");
+      const uint64_t row = sd.prow_base + static_cast<uint64_t>((gl - sd.lane_off) / TILE) * sd.n_chunks + lc.chunk;
+      if (row >= n_prows_) throw DeviceError("limb partial row out of range");
+    }
+    const uint64_t stride = rec_stride(sg.necl, false);
+    for (uint32_t i = lc.item0; i < lc.item0 + lc.n_items; ++i) {
+      const LimbItem& li = limb_items_[i];
+      if (li.e0 % PATCH || static_cast<uint64_t>(li.e0) + PATCH > sg.n ||
+          sg.rec_off + (static_cast<uint64_t>(li.e0) + PATCH) * stride > rec_bytes_)
+        throw DeviceError("limb item records out of range");
+      for (uint32_t j = 0; j < TILE; ++j) {
+        const uint16_t u = limb_idx_[static_cast<size_t>(i) * TILE + j];
+        if (u != 0xFFFFu && u >= lc.n_lanes) throw DeviceError("limb item lane out of range");
+      }
+    }
+  }
+  if (limb_idx_.size() != limb_items_.size() * TILE) throw DeviceError("limb index table out of step");
   if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
 
   if (d_items_) (void)hipFree(d_items_);
   if (d_partials_) (void)hipFree(d_partials_);
+  if (d_limb_) (void)hipFree(d_limb_);
   d_items_ = nullptr;
   d_partials_ = nullptr;
+  d_limb_ = nullptr;
   hip_check(hipMalloc(reinterpret_cast<void**>(&d_items_), std::max<size_t>(items.size(), 1) * sizeof(ItemDev)),
             "hipMalloc items");
+  // Limb tables and item lane masks, one allocation.
+  {
+    const size_t b_ch = align256(limb_chunks_.size() * sizeof(LimbChunk));
+    const size_t b_it = align256(limb_items_.size() * sizeof(LimbItem));
+    const size_t b_ix = align256(limb_idx_.size() * sizeof(uint16_t));
+    const size_t b_ln = align256(limb_lanes_.size() * sizeof(uint32_t));
+    const size_t b_lm = align256(lane_masks_.size() * sizeof(uint64_t));
+    const size_t total = std::max<size_t>(b_ch + b_it + b_ix + b_ln + b_lm, 256);
+    hip_check(hipMalloc(&d_limb_, total), "hipMalloc limb tables");
+    char* base = static_cast<char*>(d_limb_);
+    auto put = [&](const void* src, size_t bytes, size_t at) {
+      if (bytes) hip_check(hipMemcpy(base + at, src, bytes, hipMemcpyHostToDevice), "hipMemcpy limb tables");
+    };
+    put(limb_chunks_.data(), limb_chunks_.size() * sizeof(LimbChunk), 0);
+    put(limb_items_.data(), limb_items_.size() * sizeof(LimbItem), b_ch);
+    put(limb_idx_.data(), limb_idx_.size() * sizeof(uint16_t), b_ch + b_it);
+    put(limb_lanes_.data(), limb_lanes_.size() * sizeof(uint32_t), b_ch + b_it + b_ix);
+    put(lane_masks_.data(), lane_masks_.size() * sizeof(uint64_t), b_ch + b_it + b_ix + b_ln);
+    d_.n_limb_chunks = static_cast<uint32_t>(limb_chunks_.size());
+    d_.limb_chunks = reinterpret_cast<const LimbChunk*>(base);
+    d_.limb_items = reinterpret_cast<const LimbItem*>(base + b_ch);
+    d_.limb_idx = reinterpret_cast<const uint16_t*>(base + b_ch + b_it);
+    d_.limb_lanes = reinterpret_cast<const uint32_t*>(base + b_ch + b_it + b_ix);
+    d_.lane_masks = reinterpret_cast<const uint64_t*>(base + b_ch + b_it + b_ix + b_ln);
+  }
   hip_check(hipMalloc(reinterpret_cast<void**>(&d_partials_), std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)),
             "hipMalloc partials");
   // Rows of items that are not launched must read as zero; launched items

@@ -24,6 +24,7 @@ struct Tuning {
   int persistent = 0;           // 0: one workgroup per item; 1: persistent grid
   uint32_t waves_per_cu = 8;    // persistent grid size = CUs * waves_per_cu
   uint32_t grab = 4;            // items taken per atomic (persistent)
+  int limb = 1;                 // 1: split no-eclipse patches per lane (limb chunks)
   bool tuned = false;
 };
 
@@ -77,6 +78,9 @@ class Batch {
   uint64_t record_bytes() const { return rec_bytes_ + moms_.size() * sizeof(double); }
   uint64_t pairs_evaluated() const { return pairs_eval_; }
   uint64_t pairs_by_moments() const { return pairs_mom_; }
+  // Of pairs_evaluated(): (element, lane) pairs of limb lanes.
+  uint64_t pairs_limb() const { return pairs_limb_; }
+  uint64_t n_limb_chunks() const { return limb_chunks_.size(); }
   uint64_t n_items() const { return n_items_; }
   uint64_t n_lanes() const { return n_lanes_; }
   size_t n_times() const { return n_times_; }
@@ -111,8 +115,16 @@ class Batch {
   std::vector<double> moms_;
   std::vector<uint64_t> pt_base_;
   std::vector<uint32_t> pt_tiles_;
+  std::vector<std::vector<SetPack::LaneSplit>> lsplit_;  // per set
+  // Limb lanes of the current item build.
+  std::vector<LimbChunk> limb_chunks_;
+  std::vector<LimbItem> limb_items_;
+  std::vector<uint16_t> limb_idx_;
+  std::vector<uint32_t> limb_lanes_;
+  std::vector<uint64_t> lane_masks_;
   uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0,
-           n_items_plain_ = 0;
+           n_items_plain_ = 0, pairs_limb_ = 0;
+  void* d_limb_ = nullptr;  // one allocation for the limb tables
   uint32_t chunk_ = 0;
 
   DeviceBatch d_;

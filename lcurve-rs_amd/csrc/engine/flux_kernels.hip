@@ -325,6 +325,60 @@ __device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint
   return acc;
 }
 
+// Patches split per lane (pmom): the same sums, added only on the lanes of
+// each patch's mask (wave-uniform, one per patch in bit order).
+__device__ __forceinline__ double moment_sum_masked(const double* __restrict__ mb, uint64_t pm,
+                                                    const uint64_t* __restrict__ masks,
+                                                    uint32_t lane, const LaneCtx& L, double c1,
+                                                    double c2, double c3) {
+  const double x = L.ex, y = L.ey, z = L.ez;
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
+               t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
+  double acc = 0.0;
+  uint32_t i = 0;
+  while (pm) {
+    const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(pm));
+    pm &= pm - 1ull;
+    const uint64_t mk = masks[i++];
+    const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
+    const double s1 = M[0] * x + M[1] * y + M[2] * z;
+    const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
+    double v = c1 * s1 + c2 * s2;
+    if (c3 != 0.0) {
+      const double s3 = M[9] * t0 + M[10] * t1 + M[11] * t2 + M[12] * t3 + M[13] * t4 +
+                        M[14] * t5 + M[15] * t6 + M[16] * t7 + M[17] * t8 + M[18] * t9;
+      v += c3 * s3;
+    }
+    acc += ((mk >> lane) & 1ull) ? v : 0.0;
+  }
+  return acc;
+}
+
+// Star limb darkening and cut-off of a segment, for the paths without
+// beaming or lensing.
+struct SegLaw {
+  StarPar P;
+  int ld;
+  bool mcrit;
+};
+__device__ __forceinline__ SegLaw seg_law(const SegDev& sg, const SetDev& sp) {
+  SegLaw w;
+  const bool star1 = sg.kind == KIND_STAR1;
+  w.P.a0 = star1 ? sp.ldc1[0] : sp.ldc2[0];
+  w.P.a1 = star1 ? sp.ldc1[1] : sp.ldc2[1];
+  w.P.a2 = star1 ? sp.ldc1[2] : sp.ldc2[2];
+  w.P.a3 = star1 ? sp.ldc1[3] : sp.ldc2[3];
+  w.P.mucrit = star1 ? sp.mucrit1 : sp.mucrit2;
+  w.P.beam = star1 ? sp.beam1 : sp.beam2;
+  w.P.rlens = sp.rlens1;
+  w.ld = (star1 ? sp.ltype1 : sp.ltype2) == 1u ? LD_CLARET : LD_POLY;
+  if (w.ld == LD_POLY && w.P.a1 == 0.0f && w.P.a2 == 0.0f && w.P.a3 == 0.0f) w.ld = LD_LIN;
+  w.P.c0 = w.ld == LD_LIN ? 1.0f - w.P.a0 : 1.0f - (w.P.a0 + w.P.a1 + w.P.a2 + w.P.a3);
+  w.mcrit = w.P.mucrit > 0.0f;
+  return w;
+}
+
 // Workgroup b of a launch lands on XCD b % 8. Give every XCD a contiguous
 // run of items so that the tiles of one chunk share that XCD's L2.
 __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
@@ -338,7 +392,8 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
                              const double* __restrict__ lane_phi, const float4* __restrict__ lane_e,
-                             const uint32_t* __restrict__ lane_meta, double* __restrict__ partials) {
+                             const uint32_t* __restrict__ lane_meta, const uint64_t* __restrict__ lane_masks,
+                             double* __restrict__ partials) {
   const ItemDev it = items[id];
   const uint32_t lane = threadIdx.x;
   const uint32_t gl = it.lane0 + lane;
@@ -363,17 +418,9 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   L.ez = e.z;
 
   const bool star1 = sg.kind == KIND_STAR1;
-  StarPar P;
-  P.a0 = star1 ? sp.ldc1[0] : sp.ldc2[0];
-  P.a1 = star1 ? sp.ldc1[1] : sp.ldc2[1];
-  P.a2 = star1 ? sp.ldc1[2] : sp.ldc2[2];
-  P.a3 = star1 ? sp.ldc1[3] : sp.ldc2[3];
-  P.mucrit = star1 ? sp.mucrit1 : sp.mucrit2;
-  P.beam = star1 ? sp.beam1 : sp.beam2;
-  P.rlens = sp.rlens1;
-  int ld = (star1 ? sp.ltype1 : sp.ltype2) == 1u ? LD_CLARET : LD_POLY;
-  if (ld == LD_POLY && P.a1 == 0.0f && P.a2 == 0.0f && P.a3 == 0.0f) ld = LD_LIN;
-  P.c0 = ld == LD_LIN ? 1.0f - P.a0 : 1.0f - (P.a0 + P.a1 + P.a2 + P.a3);
+  const SegLaw law = seg_law(sg, sp);
+  const StarPar& P = law.P;
+  const int ld = law.ld;
   uint32_t fx = 0;
   if (sg.kind <= KIND_STAR2) {
     const bool beam = (sp.flags & (star1 ? SET_BEAM1 : SET_BEAM2)) != 0;
@@ -395,21 +442,23 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   } else {
     L.kx = L.ky = L.k0 = 0.0f;
   }
-  const bool mcrit = P.mucrit > 0.0f;
+  const bool mcrit = law.mcrit;
 
   const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
   const char* p = recs + sg.rec_off + static_cast<uint64_t>(it.e0) * stride;
   const float lin = sp.lin_disc, quad = sp.quad_disc;
   double sum = 0.0;
-  if (it.mom) {
+  if (it.mom | it.pmom) {
     // w mu (1 - A o - B o^2) with o = 1 - mu, as c1 mu + c2 mu^2 + c3 mu^3:
     // stars (A, B) = (a0, a1) (the host sends only Poly laws with a2 = a3 = 0
     // and no beaming or lensing here), disc and edge (lin, quad), spot (0, 0).
     const bool star = sg.kind <= KIND_STAR2;
     const double A = star ? P.a0 : (sg.kind == KIND_SPOT ? 0.0 : lin);
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
-    sum = moment_sum(moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE, it.mom, L,
-                     1.0 - A - B, A + 2.0 * B, -B);
+    const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
+    if (it.mom) sum = moment_sum(mb, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
+    if (it.pmom)
+      sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
     case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
@@ -430,10 +479,10 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
     const double* __restrict__ moms, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
-    double* __restrict__ partials) {
+    const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item<FX>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
+  process_item<FX>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, lane_masks, partials);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -443,7 +492,8 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
     const double* __restrict__ moms, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
-    double* __restrict__ partials, uint32_t* __restrict__ counter, uint32_t grab) {
+    const uint64_t* __restrict__ lane_masks, double* __restrict__ partials,
+    uint32_t* __restrict__ counter, uint32_t grab) {
   for (;;) {
     uint32_t base = 0;
     if (threadIdx.x == 0) base = atomicAdd(counter, grab);
@@ -451,7 +501,74 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item<true>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, partials);
+      process_item<true>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, lane_masks,
+                         partials);
+  }
+}
+
+// Limb lanes: one workgroup of LIMB_WAVES wavefronts per limb chunk. The
+// chunk's limb lanes' earth vectors are staged in LDS. Each wavefront takes
+// the chunk's limb items w, w + LIMB_WAVES, ...: every lane sums one patch
+// element by element (no eclipse test: the patches are PT_FREE) for its
+// listed lane and adds the sum, as a 64-bit fixed-point integer, to that
+// lane's LDS slot. Integer addition is exact and associative, so the total
+// does not depend on the order of the adds. The totals are then added to
+// the partial rows the work items wrote earlier in the stream.
+__global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBatch d) {
+  __shared__ unsigned long long acc[LIMB_UNION_MAX];
+  __shared__ float4 elane[LIMB_UNION_MAX];
+  const uint32_t ci = xcd_map(blockIdx.x, d.n_limb_chunks);
+  if (ci >= d.n_limb_chunks) return;  // whole workgroup, before any barrier
+  const LimbChunk c = d.limb_chunks[ci];
+  for (uint32_t u = threadIdx.x; u < c.n_lanes; u += blockDim.x) {
+    acc[u] = 0ull;
+    elane[u] = d.lane_e[d.limb_lanes[c.lane_off + u]];
+  }
+  __syncthreads();
+  // The wavefront index is uniform across the wave; saying so keeps item
+  // and record addresses in SGPRs (records then arrive by scalar loads).
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE), lane = threadIdx.x % TILE;
+  const SegDev sg = d.segs[c.seg];
+  const SetDev& sp = d.sets[sg.set];
+  const SegLaw law = seg_law(sg, sp);
+  const float lin = sp.lin_disc, quad = sp.quad_disc;
+  const uint32_t stride = rec_stride(sg.necl, false);
+  const double scale = ldexp(1.0, c.shift);
+  const uint32_t iend = c.item0 + c.n_items;
+  uint32_t i = c.item0 + w;
+  // Each item's lane slots are fetched one item ahead.
+  uint32_t u_next = i < iend ? d.limb_idx[static_cast<size_t>(i) * TILE + lane] : 0xFFFFu;
+  for (; i < iend; i += LIMB_WAVES) {
+    const uint32_t u = u_next;
+    const uint32_t inext = i + LIMB_WAVES;
+    u_next = inext < iend ? d.limb_idx[static_cast<size_t>(inext) * TILE + lane] : 0xFFFFu;
+    const LimbItem li = d.limb_items[i];
+    const bool valid = u != 0xFFFFu;
+    const float4 e = elane[valid ? u : 0u];
+    LaneCtx L;
+    L.phif = 0.0;
+    L.ex = e.x;
+    L.ey = e.y;
+    L.ez = e.z;
+    L.kx = L.ky = L.k0 = 0.0f;
+    const char* p = d.recs + sg.rec_off + static_cast<uint64_t>(li.e0) * stride;
+    double s;
+    switch (sg.necl) {
+      case 0: s = item_dispatch<0, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+      case 1: s = item_dispatch<1, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+      case 2: s = item_dispatch<2, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+      case 3: s = item_dispatch<3, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+      default: s = item_dispatch<4, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+    }
+    if (valid) atomicAdd(&acc[u], static_cast<unsigned long long>(__builtin_llrint(s * scale)));
+  }
+  __syncthreads();
+  const double inv = ldexp(1.0, -c.shift);
+  for (uint32_t u = threadIdx.x; u < c.n_lanes; u += blockDim.x) {
+    const double v = static_cast<double>(static_cast<long long>(acc[u])) * inv;
+    const uint32_t sl = d.limb_lanes[c.lane_off + u] - sp.lane_off;
+    const uint64_t row = sp.prow_base + static_cast<uint64_t>(sl / TILE) * sp.n_chunks + c.chunk;
+    d.partials[row * TILE + sl % TILE] += v;
   }
 }
 
@@ -872,16 +989,19 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
     const uint32_t nf = d.n_items_plain, nx = d.n_items - d.n_items_plain;
     if (nf)
       hipLaunchKernelGGL(flux_items_kernel<false>, dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs, d.sets,
-                         d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
+                         d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
     if (nx)
       hipLaunchKernelGGL(flux_items_kernel<true>, dim3(nx), dim3(TILE), 0, stream, d.items + nf, nx, d.segs,
-                         d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.partials);
+                         d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
                        d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_e,
-                       d.lane_meta, d.partials, d.counter, grab);
+                       d.lane_meta, d.lane_masks, d.partials, d.counter, grab);
   }
+  // After the items in stream order: limb chunks add to their rows.
+  if (d.n_limb_chunks)
+    hipLaunchKernelGGL(limb_kernel, dim3(d.n_limb_chunks), dim3(LIMB_WAVES * TILE), 0, stream, d);
   return hipGetLastError();
 }
 

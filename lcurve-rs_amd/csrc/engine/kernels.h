@@ -37,10 +37,18 @@ struct DeviceBatch {
   double* flux = nullptr;
   SetOut* out = nullptr;
   uint32_t* counter = nullptr;
+  // Limb lanes (none when n_limb_chunks == 0).
+  uint32_t n_limb_chunks = 0;
+  const LimbChunk* limb_chunks = nullptr;
+  const LimbItem* limb_items = nullptr;
+  const uint16_t* limb_idx = nullptr;    // 64 per limb item
+  const uint32_t* limb_lanes = nullptr;  // global lane per chunk-list entry
+  const uint64_t* lane_masks = nullptr;  // per pmom patch of an item
 };
 
 // Work items: static grid (one workgroup per item) or persistent grid of
-// `persistent_waves` wavefronts taking `grab` items per atomic.
+// `persistent_waves` wavefronts taking `grab` items per atomic; then the
+// limb chunks, which add to the partial rows the items wrote.
 hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
                              uint32_t grab, hipStream_t stream);
 // FP64 assembly, scale factors, smearing, autoscale and chi-squared.

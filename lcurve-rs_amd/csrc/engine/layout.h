@@ -66,6 +66,7 @@ constexpr uint8_t PT_FULL = 0, PT_FREE = 1, PT_SKIP = 2, PT_MOM = 3;
 //   [0..2]  sum w d                       (x, y, z)
 //   [3..8]  sum w (e.d)^2 coefficients of  x2 y2 z2 xy xz yz
 //   [9..18] sum w (e.d)^3 coefficients of  x3 y3 z3 x2y x2z y2x y2z z2x z2y xyz
+//   [19]    sum |w|                        (a bound for fixed-point sums)
 // so that sum_w mu^k = the k-th block dotted with the same monomials of e.
 constexpr uint32_t MOM_STRIDE = 20;
 // At most 64 patches per work item: the item's patch codes fit 64-bit masks.
@@ -92,6 +93,36 @@ struct ItemDev {
   uint64_t todo;  // bit q: patch e0 / PATCH + q is evaluated element by element
   uint64_t free;  // bit q: that patch needs no eclipse test (PT_FREE)
   uint64_t mom;   // bit q: that patch is evaluated from its moments (PT_MOM)
+  // bit q: that patch (PT_FREE) is evaluated from its moments on the lanes
+  // of lane_masks[lm_off + rank of q in pmom] only; its limb lanes, which see
+  // part of it, are summed by a limb chunk, and its other lanes see none of it.
+  uint64_t pmom;
+  uint32_t lm_off;
+  uint32_t pad;
+};
+
+// Limb lanes. For the PT_FREE patches of one segment chunk, the lanes (of
+// any tile) that see only part of a patch are evaluated element by element
+// by one workgroup per chunk: its wavefronts take limb items (one patch and
+// up to 64 such lanes) and add each lane's patch sum to the lane's LDS slot
+// as a 64-bit fixed-point integer (2^shift units; integer addition makes the
+// total independent of the order of the adds). After the work items have
+// written the chunk's partial rows, the totals are added to them. A chunk's
+// limb lanes are listed once (`lanes`, global lane indices); a limb item
+// names them by index into that list (16-bit, 0xFFFF = unused).
+constexpr uint32_t LIMB_UNION_MAX = 2048;  // limb lanes per chunk (LDS)
+constexpr uint32_t LIMB_WAVES = 8;         // wavefronts per limb workgroup
+struct LimbChunk {
+  uint32_t seg;         // segment (global index)
+  uint32_t chunk;       // chunk index within the set (partial row column)
+  uint32_t item0, n_items;  // limb items
+  uint32_t lane_off, n_lanes;  // entries of the chunk's lane list
+  int32_t shift;        // fixed-point scale 2^shift: |lane total| * 2^shift < 2^62
+  uint32_t pad;
+};
+struct LimbItem {
+  uint32_t e0;  // first element of the patch within the segment
+  uint32_t pad;
 };
 
 // Set flags.

@@ -9,6 +9,7 @@
 // some element is eclipsed, or a wrong moment shows up as a difference.
 // Beamed / lensed star segments use an indicator term (nonzero exactly when
 // the reference adds the element) since only the skip decisions apply there.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -85,6 +86,12 @@ double moment_term(const SegDev& sg, const SetDev& d, const double* M, const Lan
   return (1.0 - A - B) * s1 + (A + 2.0 * B) * s2 - B * s3;
 }
 
+const SetPack::LaneSplit* find_split(const SetPack& S, uint32_t pt) {
+  auto it = std::lower_bound(S.lsplit.begin(), S.lsplit.end(), pt,
+                             [](const SetPack::LaneSplit& a, uint32_t v) { return a.pt < v; });
+  return it != S.lsplit.end() && it->pt == pt ? &*it : nullptr;
+}
+
 }  // namespace
 
 SelfCheck selfcheck_codes(const SetPack& S) {
@@ -116,7 +123,20 @@ SelfCheck selfcheck_codes(const SetPack& S) {
             scale += std::fabs(v);
           }
           brute += pb;
-          if (code == PT_MOM) {
+          const SetPack::LaneSplit* sp = code == PT_FREE ? find_split(S, (sg.patch_off + q) * ntile + t) : nullptr;
+          if (sp) {
+            const uint64_t bit = 1ull << (l - t * TILE);
+            if (sp->all & bit) {
+              ++out.split_lanes[0];
+              coded += moment_term(sg, d, &S.moms[static_cast<size_t>(sg.patch_off + q) * MOM_STRIDE], L);
+            } else if (sp->some & bit) {
+              ++out.split_lanes[1];
+              for (uint32_t i = 0; i < PATCH; ++i)
+                coded += term(sg, d, pr + static_cast<size_t>(i) * stride, L, false);
+            } else {
+              ++out.split_lanes[2];
+            }
+          } else if (code == PT_MOM) {
             coded += moment_term(sg, d, &S.moms[static_cast<size_t>(sg.patch_off + q) * MOM_STRIDE], L);
           } else if (code != PT_SKIP) {
             for (uint32_t i = 0; i < PATCH; ++i)

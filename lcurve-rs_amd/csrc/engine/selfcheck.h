@@ -10,6 +10,9 @@ struct SelfCheck {
   double max_rel = 0.0;          // max |coded - brute| / sum |terms| per (lane, segment)
   long long counts[4] = {0, 0, 0, 0};  // (patch, tile) pairs per code PT_FULL..PT_MOM
   long long lane_segments = 0;
+  // Lanes of split PT_FREE pairs: seen whole (moments), in part (element
+  // by element), not at all.
+  long long split_lanes[3] = {0, 0, 0};
 };
 
 SelfCheck selfcheck_codes(const SetPack& S);

@@ -93,6 +93,14 @@ def set_tuning(chunk, persistent=0):
     check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
 
 
+def set_limb(on):
+    """Limb lanes for batches built from now on (default on): no-eclipse
+    patches are split per lane into closed-form moments, limb lanes evaluated
+    element by element (regrouped across tiles) and lanes that see none of
+    the patch. Off: every lane of the tile runs the element loop."""
+    check(_native.lib().lch_ctx_set_limb(_ctx(), int(bool(on))))
+
+
 def _dptr(a):
     return a.ctypes.data_as(_native.D)
 
@@ -373,16 +381,20 @@ class Model:
     def selfcheck_patch_codes(self, times, expose=0.001, ndiv=1):
         """CPU check (no device) of the engine's work skipping for this model
         at these times: returns (max relative difference between the coded
-        and the brute-force FP64 sums, {code: (patch, tile) pairs})."""
+        and the brute-force FP64 sums, {code: (patch, tile) pairs, and the
+        per-lane split of no-eclipse pairs: lanes seen whole / in part /
+        not at all})."""
         t = _f64(times)
         n = len(t)
         e = np.full(n, float(expose))
         nd = np.full(n, int(ndiv), dtype=np.int32)
         r = ctypes.c_double()
         c = (ctypes.c_longlong * 4)()
+        sp = (ctypes.c_longlong * 3)()
         check(_native.lib().lch_selfcheck_patch_codes(self._h, _dptr(t), _dptr(e), _iptr(nd), n,
-                                                      ctypes.byref(r), c))
-        return r.value, {"full": c[0], "no_eclipse": c[1], "skip": c[2], "moments": c[3]}
+                                                      ctypes.byref(r), c, sp))
+        return r.value, {"full": c[0], "no_eclipse": c[1], "skip": c[2], "moments": c[3],
+                         "split_whole": sp[0], "split_part": sp[1], "split_none": sp[2]}
 
     def selfcheck_eclipse_prescreen(self):
         """CPU check (no device): both stars' grids built with and without the

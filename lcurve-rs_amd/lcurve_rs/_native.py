@@ -42,7 +42,8 @@ class BatchStatsC(ctypes.Structure):
                 ("pairs_by_moments", ctypes.c_longlong),
                 ("setup_seconds", ctypes.c_double),
                 ("upload_seconds", ctypes.c_double), ("chunk", ctypes.c_int),
-                ("persistent", ctypes.c_int)]
+                ("persistent", ctypes.c_int), ("pairs_limb", ctypes.c_longlong),
+                ("limb_chunks", ctypes.c_longlong)]
 
 
 # name -> (restype, argtypes)
@@ -52,6 +53,7 @@ SIGNATURES = {
     "lch_device_count": (INT, []),
     "lch_ctx_create": (INT, [INT, ctypes.POINTER(VP)]),
     "lch_ctx_destroy": (None, [VP]),
+    "lch_ctx_set_limb": (INT, [VP, INT]),
     "lch_ctx_set_tuning": (INT, [VP, INT, INT]),
     "lch_ctx_sync": (INT, [VP]),
     "lch_model_load_file": (INT, [CP, ctypes.POINTER(VP)]),
@@ -80,7 +82,8 @@ SIGNATURES = {
     "lch_batch_autotune": (INT, [VP, VP, INT]),
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
     "lch_batch_sfac": (INT, [VP, D]),
-    "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong)]),
+    "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong),
+                                        ctypes.POINTER(ctypes.c_longlong)]),
     "lch_selfcheck_eclipse_prescreen": (INT, [VP, ctypes.POINTER(ctypes.c_longlong), D]),
     "lch_mlp_create": (INT, [VP, I, INT, ctypes.c_ulonglong, INT, ctypes.POINTER(VP)]),
     "lch_mlp_free": (None, [VP]),

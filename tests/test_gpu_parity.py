@@ -219,6 +219,42 @@ def test_graph_replay_is_stable(lcurve_rs):
         assert np.array_equal(first["chisq"], again["chisq"])
 
 
+def _chisq_texts(n, nlat_lo=89, nlat_hi=344, seed=42):
+    base = W.chisq_base_text()
+    return [W.model_text({"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4],
+                          "nlat2f": int(s[5]), "nlat2c": int(s[5])}, base)
+            for s in W.chisq_sets(n, seed=seed, nlat_lo=nlat_lo, nlat_hi=nlat_hi)]
+
+
+def test_limb_lanes_match_full_tiles_and_oracle(lcurve_rs):
+    """No-eclipse patches split per lane (moments / limb lanes regrouped
+    across tiles / nothing) against the same batch with every lane of the
+    tile running the element loop, and against the FP64 oracle."""
+    texts = _chisq_texts(6)
+    t = W.linspace_times(-0.2, 1.2, 2000)
+    models = [lcurve_rs.Model.from_text(x) for x in texts]
+    try:
+        lcurve_rs.set_limb(False)
+        off = lcurve_rs.Batch(models, t, 0.001, 1)
+        f_off = off.run(scale=False)["flux"]
+        assert off.stats()["limb_chunks"] == 0
+    finally:
+        lcurve_rs.set_limb(True)
+    on = lcurve_rs.Batch(models, t, 0.001, 1)
+    st = on.stats()
+    assert st["limb_chunks"] > 0 and 0 < st["pairs_limb"] < st["pairs_evaluated"]
+    assert st["pairs_evaluated"] < 0.8 * off.stats()["pairs_evaluated"]
+    f_on = on.run(scale=False)["flux"]
+    assert np.max(np.abs(f_on / f_off - 1)) <= 3e-7
+    idx = np.linspace(0, len(t) - 1, 60).astype(int)
+    ob = pyoracle.OracleBatch(texts, t[idx], 0.001, 1, nthreads=16)
+    _, rf = ob.flux(scale=False, nthreads=16)
+    assert np.max(np.abs(f_on[:, idx] / rf - 1)) <= FLUX_TOL
+    # fixed summation order: replays are identical
+    again = on.run(scale=False)["flux"]
+    assert np.array_equal(f_on, again)
+
+
 def test_launch_forms_and_autotune_agree(lcurve_rs):
     texts, t = uneven_batch(6, 200)
     models = [lcurve_rs.Model.from_text(x) for x in texts]

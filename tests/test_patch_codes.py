@@ -16,6 +16,11 @@ from lcurve_rs import workloads as W
 from variants import VARIANTS
 
 DENSE = W.linspace_times(-0.2, 1.2, 2000)
+CODES = ("full", "no_eclipse", "skip", "moments")
+
+
+def _pairs(counts):
+    return sum(counts[k] for k in CODES)
 
 
 @pytest.fixture(scope="module")
@@ -30,6 +35,8 @@ def test_golden_model_codes(lc, golden_model_text):
     assert rel < 1e-10
     assert counts["skip"] > 0 and counts["moments"] > 0 and counts["no_eclipse"] > 0
     assert counts["full"] > 0  # the white dwarf's eclipse by the donor
+    # No-eclipse pairs are split per lane: whole (moments), part, none.
+    assert counts["split_whole"] > 0 and counts["split_part"] > 0 and counts["split_none"] > 0
 
 
 @pytest.mark.parametrize("name", sorted(VARIANTS))
@@ -37,7 +44,7 @@ def test_variant_codes(lc, golden_model_text, name):
     m = lc.Model.from_text(W.model_text(VARIANTS[name], golden_model_text))
     rel, counts = m.selfcheck_patch_codes(DENSE)
     assert rel < 1e-10, (name, rel, counts)
-    assert sum(counts.values()) > 0
+    assert _pairs(counts) > 0
 
 
 def test_sparse_times_skip_less(lc, golden_model_text):
@@ -60,13 +67,16 @@ def test_chisq_workload_set_codes(lc):
                          "nlat2f": int(s[5]), "nlat2c": int(s[5])}, W.chisq_base_text())
     rel, counts = lc.Model.from_text(text).selfcheck_patch_codes(DENSE)
     assert rel < 1e-10
-    tot = sum(counts.values())
+    tot = _pairs(counts)
     # most patch-tile pairs avoid the element-by-element loop
     assert (counts["skip"] + counts["moments"]) / tot > 0.5
+    # and of the no-eclipse pairs' lanes, a minority straddles the limb
+    split = counts["split_whole"] + counts["split_part"] + counts["split_none"]
+    assert 0 < counts["split_part"] < 0.5 * split
 
 
 def test_disc_spot_codes(lc):
     text, _, _, _ = W.disc_spot(0.3, 100)
     rel, counts = lc.Model.from_text(text).selfcheck_patch_codes(DENSE[::2], expose=0.005, ndiv=5)
     assert rel < 1e-10
-    assert counts["moments"] > 0
+    assert counts["moments"] > 0 and counts["split_whole"] > 0
