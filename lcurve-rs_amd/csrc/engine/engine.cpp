@@ -127,6 +127,34 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
       sg.set = static_cast<uint32_t>(s);
       if (mom_base + sg.patch_off > UINT32_MAX) throw DeviceError("too many patches");
       sg.mom_off = static_cast<uint32_t>(mom_base + sg.patch_off);
+      // Moment sums of aligned runs of 2, 4, ..., 64 patches, each level
+      // from pairs of the one below.
+      {
+        const uint32_t np = sg.n / PATCH;
+        if (mblk_.size() / MOM_STRIDE > UINT32_MAX) throw DeviceError("too many moment blocks");
+        sg.blk_off = static_cast<uint32_t>(mblk_.size() / MOM_STRIDE);
+        sg.pad = 0;
+        size_t prev = static_cast<size_t>(sg.mom_off) * MOM_STRIDE;  // level 0 in moms_
+        uint32_t nprev = np;
+        bool from_moms = true;
+        for (uint32_t k = 1; k <= MOM_LEVELS; ++k) {
+          const uint32_t nb = (np + (1u << k) - 1) >> k;
+          const size_t base = mblk_.size();
+          mblk_.resize(base + static_cast<size_t>(nb) * MOM_STRIDE, 0.0);
+          for (uint32_t j = 0; j < nb; ++j) {
+            for (uint32_t h = 0; h < 2; ++h) {
+              const uint32_t src = 2 * j + h;
+              if (src >= nprev) break;
+              const double* a = (from_moms ? moms_.data() : mblk_.data()) + prev + static_cast<size_t>(src) * MOM_STRIDE;
+              double* o = mblk_.data() + base + static_cast<size_t>(j) * MOM_STRIDE;
+              for (uint32_t c = 0; c < MOM_STRIDE; ++c) o[c] += a[c];
+            }
+          }
+          prev = base;
+          nprev = nb;
+          from_moms = false;
+        }
+      }
       uint32_t gid = static_cast<uint32_t>(segs_.size());
       if (P.seg_grid[g] < 7) {
         H.grids[P.seg_grid[g]].segs.push_back(gid);
@@ -240,6 +268,9 @@ void Batch::upload() {
   }
   up(moms_, const_cast<double**>(&lp));
   d_.moms = lp;
+  if (mblk_.empty()) mblk_.assign(MOM_STRIDE, 0.0);
+  up(mblk_, const_cast<double**>(&lp));
+  d_.mblk = lp;
   d_.lane_tot = dalloc<double>(lane_phi_.size());
   d_.flux = dalloc<double>(t_lane0_.size());
   d_.out = dalloc<SetOut>(sets_.size());
@@ -456,6 +487,12 @@ void Batch::build_items(uint32_t chunk) {
       throw DeviceError("item patch masks out of range");
     if ((it.mom || it.pmom) && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
       throw DeviceError("item moments out of range");
+    if (it.mom) {
+      uint64_t nblk = 0;
+      for (uint32_t k = 1; k <= MOM_LEVELS; ++k) nblk += ((sg.n / PATCH) + (1u << k) - 1) >> k;
+      if (static_cast<uint64_t>(sg.blk_off) + nblk > mblk_.size() / MOM_STRIDE)
+        throw DeviceError("item moment blocks out of range");
+    }
     if ((it.pmom & ~live) || (it.pmom & (it.todo | it.mom)) ||
         static_cast<uint64_t>(it.lm_off) + __builtin_popcountll(it.pmom) > lane_masks_.size())
       throw DeviceError("item lane masks out of range");

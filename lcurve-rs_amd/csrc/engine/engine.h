@@ -113,6 +113,7 @@ class Batch {
   // Per-set patch codes (ptcode of SetPack), kept for rebuilding items.
   std::vector<uint8_t> ptcode_;
   std::vector<double> moms_;
+  std::vector<double> mblk_;  // moment sums of aligned patch runs
   std::vector<uint64_t> pt_base_;
   std::vector<uint32_t> pt_tiles_;
   std::vector<std::vector<SetPack::LaneSplit>> lsplit_;  // per set

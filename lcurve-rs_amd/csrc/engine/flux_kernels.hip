@@ -325,6 +325,55 @@ __device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint
   return acc;
 }
 
+// PT_MOM patches of an item by aligned runs: a run of 2^k patches starting
+// at a multiple of 2^k (within the segment) uses its precomputed moment sum
+// from the block table, so a visible region costs a few evaluations.
+// pbase: segment-local index of the item's first patch (a multiple of the
+// item's patch count, a power of two).
+__device__ __forceinline__ double moment_sum_runs(const double* __restrict__ m0,
+                                                  const double* __restrict__ blk, uint32_t np_seg,
+                                                  uint32_t pbase, uint64_t mm, const LaneCtx& L,
+                                                  double c1, double c2, double c3) {
+  const double x = L.ex, y = L.ey, z = L.ez;
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
+               t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
+  double acc = 0.0;
+  while (mm) {
+    const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
+    // Longest aligned run of set bits starting at q.
+    uint32_t k = 0;
+    uint64_t run = 1ull << q;
+    while (k < MOM_LEVELS) {
+      const uint32_t len = 2u << k;
+      if (((pbase + q) & (len - 1u)) != 0u || q + len > 64u) break;
+      const uint64_t r2 = (len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q;
+      if ((mm & r2) != r2) break;
+      run = r2;
+      ++k;
+    }
+    mm &= ~run;
+    const double* M;
+    if (k == 0) {
+      M = m0 + static_cast<size_t>(q) * MOM_STRIDE;
+    } else {
+      uint32_t off = 0;
+      for (uint32_t j = 1; j < k; ++j) off += (np_seg + (1u << j) - 1u) >> j;
+      M = blk + static_cast<size_t>(off + ((pbase + q) >> k)) * MOM_STRIDE;
+    }
+    const double s1 = M[0] * x + M[1] * y + M[2] * z;
+    const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
+    double v = c1 * s1 + c2 * s2;
+    if (c3 != 0.0) {
+      const double s3 = M[9] * t0 + M[10] * t1 + M[11] * t2 + M[12] * t3 + M[13] * t4 +
+                        M[14] * t5 + M[15] * t6 + M[16] * t7 + M[17] * t8 + M[18] * t9;
+      v += c3 * s3;
+    }
+    acc += v;
+  }
+  return acc;
+}
+
 // Patches split per lane (pmom): the same sums, added only on the lanes of
 // each patch's mask (wave-uniform, one per patch in bit order).
 __device__ __forceinline__ double moment_sum_masked(const double* __restrict__ mb, uint64_t pm,
@@ -391,6 +440,7 @@ template <bool FX>
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
+                             const double* __restrict__ mblk,
                              const double* __restrict__ lane_phi, const float4* __restrict__ lane_e,
                              const uint32_t* __restrict__ lane_meta, const uint64_t* __restrict__ lane_masks,
                              double* __restrict__ partials) {
@@ -456,7 +506,9 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
     const double A = star ? P.a0 : (sg.kind == KIND_SPOT ? 0.0 : lin);
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
     const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
-    if (it.mom) sum = moment_sum(mb, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
+    if (it.mom)
+      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, sg.n / PATCH,
+                            it.e0 / PATCH, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
     if (it.pmom)
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
@@ -477,12 +529,12 @@ template <bool FX>
 __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ moms, const double* __restrict__ lane_phi,
+    const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item<FX>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, lane_masks, partials);
+  process_item<FX>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -490,7 +542,7 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
 __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ moms, const double* __restrict__ lane_phi,
+    const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials,
     uint32_t* __restrict__ counter, uint32_t grab) {
@@ -501,7 +553,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item<true>(id, items, segs, sets, recs, moms, lane_phi, lane_e, lane_meta, lane_masks,
+      process_item<true>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks,
                          partials);
   }
 }
@@ -989,14 +1041,14 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
     const uint32_t nf = d.n_items_plain, nx = d.n_items - d.n_items_plain;
     if (nf)
       hipLaunchKernelGGL(flux_items_kernel<false>, dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs, d.sets,
-                         d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
+                         d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
     if (nx)
       hipLaunchKernelGGL(flux_items_kernel<true>, dim3(nx), dim3(TILE), 0, stream, d.items + nf, nx, d.segs,
-                         d.sets, d.recs, d.moms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
+                         d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
-                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.lane_phi, d.lane_e,
+                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e,
                        d.lane_meta, d.lane_masks, d.partials, d.counter, grab);
   }
   // After the items in stream order: limb chunks add to their rows.
