@@ -118,6 +118,18 @@ __device__ __forceinline__ float mu_of(const float4& d, const LaneCtx& L) {
 // m = clamp(mu, 0, 1) the term m * flux * imu(m) equals the reference's for
 // 0 < mu <= 1 and is 0 for mu <= 0, so the mucrit test is only needed for
 // mucrit > 0 (MCRIT) and the select only with it or the eclipse test.
+// Four consecutive 16-byte records (segments without eclipse intervals):
+// one 64-byte scalar load instead of four.
+struct Rec4 {
+  float4 r[4];
+};
+
+template <int NECL>
+__device__ __forceinline__ float4 rec_dirn(const char* __restrict__ p, uint32_t i, uint32_t j) {
+  if (NECL == 0) return reinterpret_cast<const Rec4*>(p + i * 16u)->r[j];
+  return *reinterpret_cast<const float4*>(p + (i + j) * rec_stride(NECL, false));
+}
+
 template <int NECL, bool FREE, int LD, bool MCRIT>
 __device__ __forceinline__ float star_patch(const char* __restrict__ p, const LaneCtx& L,
                                             const StarPar& P) {
@@ -129,7 +141,7 @@ __device__ __forceinline__ float star_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float4 d = rec_dirn<NECL>(p, i, j);
       const float mu = mu_of(d, L);
       const float m = clamp01(mu);
       float wm = m * d.w;
@@ -193,7 +205,7 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float4 d = rec_dirn<NECL>(p, i, j);
       const float m = clamp01(mu_of(d, L));
       float wm = m * d.w;
       if (ECL) wm = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? wm : 0.0f;
@@ -215,7 +227,7 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = *reinterpret_cast<const float4*>(r);
+      const float4 d = rec_dirn<NECL>(p, i, j);
       float w = d.w;
       if (ECL) w = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? w : 0.0f;
       acc[j] = fmaf(clamp01(mu_of(d, L)), w, acc[j]);
@@ -588,12 +600,14 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
   const double scale = ldexp(1.0, c.shift);
   const uint32_t iend = c.item0 + c.n_items;
   uint32_t i = c.item0 + w;
-  // Each item's lane slots are fetched one item ahead.
-  uint32_t u_next = i < iend ? d.limb_idx[static_cast<size_t>(i) * TILE + lane] : 0xFFFFu;
+  // Each item's lane slots are fetched one item ahead (the index is clamped
+  // to the chunk's last item: no branch around the load).
+  const uint32_t ilast = iend - 1u;
+  uint32_t u_next = d.limb_idx[static_cast<size_t>(i < ilast ? i : ilast) * TILE + lane];
   for (; i < iend; i += LIMB_WAVES) {
     const uint32_t u = u_next;
     const uint32_t inext = i + LIMB_WAVES;
-    u_next = inext < iend ? d.limb_idx[static_cast<size_t>(inext) * TILE + lane] : 0xFFFFu;
+    u_next = d.limb_idx[static_cast<size_t>(inext < ilast ? inext : ilast) * TILE + lane];
     const LimbItem li = d.limb_items[i];
     const bool valid = u != 0xFFFFu;
     const float4 e = elane[valid ? u : 0u];
