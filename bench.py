@@ -369,6 +369,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-limb", action="store_true",
                     help="evaluate no-eclipse patches on all lanes of a tile (no limb lanes)")
+    ap.add_argument("--limb-max-part", type=int, default=48,
+                    help="split a no-eclipse patch per lane when at most this many lanes see it in part")
     ap.add_argument("--parity-sets", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a streaming kernel of known traffic (for PMC scaling)")
@@ -398,7 +400,7 @@ def main():
         return
     if args.chunk > 0:
         lc.set_tuning(args.chunk, 0)
-    lc.set_limb(not args.no_limb)
+    lc.set_limb(not args.no_limb, args.limb_max_part)
 
     batch, info = build_workload(args.workload, args.nsets)
     scale = info["scale"]
@@ -445,7 +447,7 @@ def main():
            "max_rel_chisq_err_vs_fp64": chi_err,
            "cpu_setup_s": round(st["setup_seconds"], 2), "upload_s": round(st["upload_seconds"], 2),
            "tuning": {"chunk": st["chunk"], "persistent": st["persistent"], "items": st["n_items"],
-                      "limb": not args.no_limb, "limb_chunks": st["limb_chunks"]},
+                      "limb": 0 if args.no_limb else args.limb_max_part, "limb_chunks": st["limb_chunks"]},
            "roofline": roof}
     if not args.no_cpu_baseline:
         log("CPU baseline (FP64 oracle, bounded sample) ...")

@@ -81,11 +81,12 @@ int lch_ctx_create(int device, lch_ctx** out);
 void lch_ctx_destroy(lch_ctx* ctx);
 /* Override the launch tuning (chunk > 0 and persistent in {0,1}). */
 int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
-/* Limb lanes on (1, the default) or off (0) for batches built from now on:
- * no-eclipse patches split per lane into moments / limb lanes / nothing,
- * versus evaluated element by element on all 64 lanes of the tile. Results
+/* Limb lanes for batches built from now on: a no-eclipse (patch, tile)
+ * pair is split per lane into moments / limb lanes / nothing when at most
+ * max_part (1..64, default 48) of its lanes see the patch in part; 0 turns
+ * the split off (all 64 lanes of the tile run the element loop). Results
  * agree to FP32 rounding either way. No counterpart in the reference. */
-int lch_ctx_set_limb(lch_ctx* ctx, int on);
+int lch_ctx_set_limb(lch_ctx* ctx, int max_part);
 int lch_ctx_sync(lch_ctx* ctx);
 
 /* ---- model parameter files ---- */

@@ -140,9 +140,9 @@ int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent) {
   return LCH_OK;
 }
 
-int lch_ctx_set_limb(lch_ctx* ctx, int on) {
-  if (!ctx || (on != 0 && on != 1)) return fail(LCH_ERR_INVALID, "bad limb setting");
-  ctx->impl.tuning.limb = on;
+int lch_ctx_set_limb(lch_ctx* ctx, int max_part) {
+  if (!ctx || max_part < 0 || max_part > 64) return fail(LCH_ERR_INVALID, "bad limb setting");
+  ctx->impl.tuning.limb = max_part;
   return LCH_OK;
 }
 

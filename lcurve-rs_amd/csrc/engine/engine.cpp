@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -18,16 +19,6 @@ namespace {
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess)
     throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// A no-eclipse patch goes to limb lanes when at most this many of a tile's
-// lanes see part of it (more, and the item's own loop costs no more).
-constexpr int kLimbMaxPart = 48;
-
-const SetPack::LaneSplit* find_split(const std::vector<SetPack::LaneSplit>& v, uint32_t pt) {
-  auto it = std::lower_bound(v.begin(), v.end(), pt,
-                             [](const SetPack::LaneSplit& a, uint32_t x) { return a.pt < x; });
-  return it != v.end() && it->pt == pt ? &*it : nullptr;
 }
 
 size_t align256(size_t n) { return (n + 255) / 256 * 256; }
@@ -140,6 +131,7 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
         for (uint32_t k = 1; k <= MOM_LEVELS; ++k) {
           const uint32_t nb = (np + (1u << k) - 1) >> k;
           const size_t base = mblk_.size();
+          sg.lvl_off[k - 1] = static_cast<uint32_t>(base / MOM_STRIDE - sg.blk_off);
           mblk_.resize(base + static_cast<size_t>(nb) * MOM_STRIDE, 0.0);
           for (uint32_t j = 0; j < nb; ++j) {
             for (uint32_t h = 0; h < 2; ++h) {
@@ -301,156 +293,250 @@ void Batch::build_items(uint32_t chunk) {
   limb_lanes_.clear();
   lane_masks_.clear();
   pairs_limb_ = 0;
-  std::vector<ItemDev> items;
+  // Sequential pass: chunk tables and partial-row bases per set.
+  struct SetPlan {
+    size_t g0 = 0, g1 = 0;
+    std::vector<uint32_t> chunk0;
+    uint32_t nch = 0;
+  };
+  std::vector<SetPlan> plans(sets_.size());
   uint64_t prow = 0;
   size_t seg_i = 0;
   for (size_t s = 0; s < sets_.size(); ++s) {
     SetDev& sd = sets_[s];
+    SetPlan& pl = plans[s];
     sd.chunk_off = static_cast<uint32_t>(chunk_info_.size());
     // Segments of this set are contiguous in segs_.
-    size_t g0 = seg_i;
+    pl.g0 = seg_i;
     while (seg_i < segs_.size() && segs_[seg_i].set == s) ++seg_i;
-    std::vector<uint32_t> chunk0(seg_i - g0);
-    uint32_t nch = 0;
-    for (size_t g = g0; g < seg_i; ++g) {
-      chunk0[g - g0] = nch;
+    pl.g1 = seg_i;
+    pl.chunk0.resize(pl.g1 - pl.g0);
+    for (size_t g = pl.g0; g < pl.g1; ++g) {
+      pl.chunk0[g - pl.g0] = pl.nch;
       uint32_t c = (segs_[g].n + chunk - 1) / chunk;
       for (uint32_t k = 0; k < c; ++k) chunk_info_.push_back(segs_[g].use_mask | (segs_[g].kind << 16));
-      nch += c;
+      pl.nch += c;
     }
-    sd.n_chunks = nch;
+    sd.n_chunks = pl.nch;
     sd.prow_base = prow;
+    prow += static_cast<uint64_t>(sd.n_lanes / TILE) * pl.nch;
+  }
+  n_prows_ = prow;
+
+  // Per-set items and limb tables, built in parallel (offsets set-relative
+  // where they index the tables being built).
+  struct SetItems {
+    std::vector<ItemDev> items;
+    std::vector<uint64_t> masks;
+    std::vector<LimbChunk> lchunks;
+    std::vector<LimbItem> litems;
+    std::vector<uint16_t> lidx;
+    std::vector<uint32_t> llanes;
+    uint64_t pairs_limb = 0;
+    std::string error;
+  };
+  std::vector<SetItems> out(sets_.size());
+  const int limb = ctx_.tuning.limb;
+  auto build_set = [&](size_t s) {
+    const SetDev& sd = sets_[s];
+    if (!(sd.flags & SET_VALID)) return;
+    const SetPlan& pl = plans[s];
+    SetItems& R = out[s];
     const uint32_t ntile = sd.n_lanes / TILE;
-    if (sd.flags & SET_VALID) {
-      // Items ordered (segment, chunk, tile): the tiles of one chunk run
-      // back to back and share the chunk in cache.
-      for (size_t g = g0; g < seg_i; ++g) {
-        const SegDev& sg = segs_[g];
-        uint32_t c = (sg.n + chunk - 1) / chunk;
-        for (uint32_t k = 0; k < c; ++k) {
-          const uint32_t e0 = k * chunk, ne = std::min(chunk, sg.n - k * chunk), np = ne / PATCH;
-          // Tentative items of this chunk, their pmom lane masks, and the
-          // limb lanes of each patch (global lane indices, tile order).
-          struct Tent {
-            ItemDev it;
-            std::vector<uint64_t> masks;
-            bool limb_row = false;
-          };
-          std::vector<Tent> tent;
-          std::vector<std::vector<uint32_t>> plan(np);
-          for (uint32_t t = 0; t < ntile; ++t) {
-            if (!(tile_cp_[sd.lane_off / TILE + t] & sg.use_mask)) continue;
-            Tent tt;
-            ItemDev& it = tt.it;
-            it.seg = static_cast<uint32_t>(g);
-            it.e0 = e0;
-            it.ne = ne;
-            it.lane0 = sd.lane_off + t * TILE;
-            it.prow = prow + static_cast<uint64_t>(t) * nch + chunk0[g - g0] + k;
-            it.todo = it.free = it.mom = it.pmom = 0;
-            it.lm_off = it.pad = 0;
-            for (uint32_t q = 0; q < np; ++q) {
-              const uint32_t pt = (sg.patch_off + e0 / PATCH + q) * pt_tiles_[s] + t;
-              const uint64_t ci = pt_base_[s] + pt;
-              if (ci >= ptcode_.size()) throw DeviceError("patch code out of range");
-              const uint8_t code = ptcode_[ci];
-              if (ctx_.tuning.limb && code == PT_FREE) {
-                const SetPack::LaneSplit* spl = find_split(lsplit_[s], pt);
-                if (spl && __builtin_popcountll(spl->some) <= kLimbMaxPart) {
-                  it.pmom |= 1ull << q;
-                  tt.masks.push_back(spl->all);
-                  for (uint64_t m = spl->some; m; m &= m - 1ull)
-                    plan[q].push_back(it.lane0 + static_cast<uint32_t>(__builtin_ctzll(m)));
-                  if (spl->some) tt.limb_row = true;
-                  continue;
-                }
-              }
-              if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
-              if (code == PT_FREE) it.free |= 1ull << q;
-              if (code == PT_MOM) it.mom |= 1ull << q;
+    const std::vector<SetPack::LaneSplit>& spl_all = lsplit_[s];
+    // Scratch reused across chunks.
+    std::vector<const SetPack::LaneSplit*> tbl;
+    std::vector<std::vector<uint32_t>> plan(MAX_ITEM_PATCHES);
+    std::vector<uint32_t> uni;
+    std::vector<int32_t> uidx(sd.n_lanes, -1);
+    struct Tent {
+      ItemDev it;
+      uint32_t m0, mn;  // masks of this item in `tmasks`
+      bool limb_row;
+    };
+    std::vector<Tent> tent;
+    std::vector<uint64_t> tmasks;
+    for (size_t g = pl.g0; g < pl.g1; ++g) {
+      const SegDev& sg = segs_[g];
+      const uint32_t c = (sg.n + chunk - 1) / chunk;
+      for (uint32_t k = 0; k < c; ++k) {
+        const uint32_t e0 = k * chunk, ne = std::min(chunk, sg.n - k * chunk), np = ne / PATCH;
+        const uint32_t p0 = sg.patch_off + e0 / PATCH;
+        // Lane splits of this chunk's (patch, tile) pairs.
+        tbl.assign(static_cast<size_t>(np) * ntile, nullptr);
+        if (limb) {
+          const uint32_t lo = p0 * ntile, hi = (p0 + np) * ntile;
+          auto it0 = std::lower_bound(spl_all.begin(), spl_all.end(), lo,
+                                      [](const SetPack::LaneSplit& a, uint32_t x) { return a.pt < x; });
+          for (auto it = it0; it != spl_all.end() && it->pt < hi; ++it) tbl[it->pt - lo] = &*it;
+        }
+        for (uint32_t q = 0; q < np; ++q) plan[q].clear();
+        tent.clear();
+        tmasks.clear();
+        for (uint32_t t = 0; t < ntile; ++t) {
+          if (!(tile_cp_[sd.lane_off / TILE + t] & sg.use_mask)) continue;
+          Tent tt{};
+          ItemDev& it = tt.it;
+          it.seg = static_cast<uint32_t>(g);
+          it.e0 = e0;
+          it.ne = ne;
+          it.lane0 = sd.lane_off + t * TILE;
+          it.prow = sd.prow_base + static_cast<uint64_t>(t) * pl.nch + pl.chunk0[g - pl.g0] + k;
+          tt.m0 = static_cast<uint32_t>(tmasks.size());
+          for (uint32_t q = 0; q < np; ++q) {
+            const uint64_t ci = pt_base_[s] + static_cast<uint64_t>(p0 + q) * pt_tiles_[s] + t;
+            if (ci >= ptcode_.size()) {
+              R.error = "patch code out of range";
+              return;
             }
-            tent.push_back(std::move(tt));
+            const uint8_t code = ptcode_[ci];
+            if (code == PT_FREE) {
+              const SetPack::LaneSplit* sp = tbl[static_cast<size_t>(q) * ntile + t];
+              if (sp && __builtin_popcountll(sp->some) <= limb) {
+                it.pmom |= 1ull << q;
+                tmasks.push_back(sp->all);
+                for (uint64_t m = sp->some; m; m &= m - 1ull)
+                  plan[q].push_back(it.lane0 + static_cast<uint32_t>(__builtin_ctzll(m)));
+                if (sp->some) tt.limb_row = true;
+                continue;
+              }
+            }
+            if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
+            if (code == PT_FREE) it.free |= 1ull << q;
+            if (code == PT_MOM) it.mom |= 1ull << q;
           }
-          // The chunk's limb lanes; too many for one workgroup's LDS: the
-          // patches go back to the items' element loops.
-          std::vector<uint32_t> uni;
-          for (const auto& v : plan) uni.insert(uni.end(), v.begin(), v.end());
-          std::sort(uni.begin(), uni.end());
-          uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
-          if (uni.size() > LIMB_UNION_MAX) {
-            for (Tent& tt : tent) {
-              tt.it.todo |= tt.it.pmom;
-              tt.it.free |= tt.it.pmom;
-              tt.it.pmom = 0;
-              tt.masks.clear();
-              tt.limb_row = false;
-            }
-            uni.clear();
-          }
-          if (!uni.empty()) {
-            LimbChunk lc;
-            lc.seg = static_cast<uint32_t>(g);
-            lc.chunk = chunk0[g - g0] + k;
-            // Fixed-point scale: a lane's total over the chunk's limb
-            // patches is at most sum |w| * max |I(m)|, m in [0, 1].
-            {
-              const bool s1 = sg.kind == KIND_STAR1;
-              double imax = 1.0;
-              if (sg.kind <= KIND_STAR2) {
-                const float* a = s1 ? sd.ldc1 : sd.ldc2;
-                imax += std::fabs(a[0]) + std::fabs(a[1]) + std::fabs(a[2]) + std::fabs(a[3]);
-              } else if (sg.kind != KIND_SPOT) {
-                imax += std::fabs(sd.lin_disc) + std::fabs(sd.quad_disc);
-              }
-              double wsum = 0.0;
-              for (uint32_t q = 0; q < np; ++q) {
-                if (plan[q].empty()) continue;
-                const size_t mi = static_cast<size_t>(sg.mom_off + e0 / PATCH + q) * MOM_STRIDE + 19;
-                if (mi >= moms_.size()) throw DeviceError("limb patch moments out of range");
-                wsum += moms_[mi];
-              }
-              const double bound = 1.01 * imax * wsum;
-              int ex = 0;
-              if (bound > 0.0) (void)std::frexp(bound, &ex);  // bound < 2^ex
-              lc.shift = bound > 0.0 ? 61 - ex : 0;
-              lc.pad = 0;
-            }
-            lc.item0 = static_cast<uint32_t>(limb_items_.size());
-            lc.lane_off = static_cast<uint32_t>(limb_lanes_.size());
-            lc.n_lanes = static_cast<uint32_t>(uni.size());
-            limb_lanes_.insert(limb_lanes_.end(), uni.begin(), uni.end());
-            for (uint32_t q = 0; q < np; ++q) {
-              const std::vector<uint32_t>& v = plan[q];
-              for (size_t i0 = 0; i0 < v.size(); i0 += TILE) {
-                limb_items_.push_back(LimbItem{e0 + q * PATCH, 0u});
-                for (size_t i = i0; i < i0 + TILE; ++i) {
-                  uint16_t u = 0xFFFFu;
-                  if (i < v.size()) {
-                    u = static_cast<uint16_t>(std::lower_bound(uni.begin(), uni.end(), v[i]) - uni.begin());
-                    pairs_limb_ += PATCH;
-                  }
-                  limb_idx_.push_back(u);
-                }
-              }
-            }
-            lc.n_items = static_cast<uint32_t>(limb_items_.size()) - lc.item0;
-            limb_chunks_.push_back(lc);
-          }
+          tt.mn = static_cast<uint32_t>(tmasks.size()) - tt.m0;
+          tent.push_back(tt);
+        }
+        // The chunk's limb lanes; too many for one workgroup's LDS: the
+        // patches go back to the items' element loops.
+        uni.clear();
+        for (uint32_t q = 0; q < np; ++q) uni.insert(uni.end(), plan[q].begin(), plan[q].end());
+        std::sort(uni.begin(), uni.end());
+        uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+        if (uni.size() > LIMB_UNION_MAX) {
           for (Tent& tt : tent) {
-            // An item adding zero to every lane is not launched: its
-            // partial row stays at the zero written once below. A row that
-            // limb lanes add to is rewritten by its item on every launch.
-            if (tt.it.todo == 0 && tt.it.mom == 0 && tt.it.pmom == 0 && !tt.limb_row) continue;
-            tt.it.lm_off = static_cast<uint32_t>(lane_masks_.size());
-            lane_masks_.insert(lane_masks_.end(), tt.masks.begin(), tt.masks.end());
-            items.push_back(tt.it);
+            tt.it.todo |= tt.it.pmom;
+            tt.it.free |= tt.it.pmom;
+            tt.it.pmom = 0;
+            tt.mn = 0;
+            tt.limb_row = false;
           }
+          uni.clear();
+        }
+        if (!uni.empty()) {
+          LimbChunk lc{};
+          lc.seg = static_cast<uint32_t>(g);
+          lc.chunk = pl.chunk0[g - pl.g0] + k;
+          // Fixed-point scale: a lane's total over the chunk's limb
+          // patches is at most sum |w| * max |I(m)|, m in [0, 1].
+          double imax = 1.0;
+          if (sg.kind <= KIND_STAR2) {
+            const float* a = sg.kind == KIND_STAR1 ? sd.ldc1 : sd.ldc2;
+            imax += std::fabs(a[0]) + std::fabs(a[1]) + std::fabs(a[2]) + std::fabs(a[3]);
+          } else if (sg.kind != KIND_SPOT) {
+            imax += std::fabs(sd.lin_disc) + std::fabs(sd.quad_disc);
+          }
+          double wsum = 0.0;
+          for (uint32_t q = 0; q < np; ++q) {
+            if (plan[q].empty()) continue;
+            const size_t mi = static_cast<size_t>(sg.mom_off + e0 / PATCH + q) * MOM_STRIDE + 19;
+            if (mi >= moms_.size()) {
+              R.error = "limb patch moments out of range";
+              return;
+            }
+            wsum += moms_[mi];
+          }
+          const double bound = 1.01 * imax * wsum;
+          int ex = 0;
+          if (bound > 0.0) (void)std::frexp(bound, &ex);  // bound < 2^ex
+          lc.shift = bound > 0.0 ? 61 - ex : 0;
+          lc.item0 = static_cast<uint32_t>(R.litems.size());
+          lc.lane_off = static_cast<uint32_t>(R.llanes.size());
+          lc.n_lanes = static_cast<uint32_t>(uni.size());
+          for (size_t u = 0; u < uni.size(); ++u) uidx[uni[u] - sd.lane_off] = static_cast<int32_t>(u);
+          R.llanes.insert(R.llanes.end(), uni.begin(), uni.end());
+          for (uint32_t q = 0; q < np; ++q) {
+            const std::vector<uint32_t>& v = plan[q];
+            for (size_t i0 = 0; i0 < v.size(); i0 += TILE) {
+              R.litems.push_back(LimbItem{e0 + q * PATCH, 0u});
+              for (size_t i = i0; i < i0 + TILE; ++i) {
+                uint16_t u = 0xFFFFu;
+                if (i < v.size()) {
+                  u = static_cast<uint16_t>(uidx[v[i] - sd.lane_off]);
+                  R.pairs_limb += PATCH;
+                }
+                R.lidx.push_back(u);
+              }
+            }
+          }
+          lc.n_items = static_cast<uint32_t>(R.litems.size()) - lc.item0;
+          R.lchunks.push_back(lc);
+        }
+        for (Tent& tt : tent) {
+          // An item adding zero to every lane is not launched: its partial
+          // row stays at the zero written once below. A row that limb
+          // lanes add to is rewritten by its item on every launch.
+          if (tt.it.todo == 0 && tt.it.mom == 0 && tt.it.pmom == 0 && !tt.limb_row) continue;
+          tt.it.lm_off = static_cast<uint32_t>(R.masks.size());
+          R.masks.insert(R.masks.end(), tmasks.begin() + tt.m0, tmasks.begin() + tt.m0 + tt.mn);
+          R.items.push_back(tt.it);
         }
       }
     }
-    prow += static_cast<uint64_t>(ntile) * nch;
+  };
+  {
+    int nthreads = static_cast<int>(std::thread::hardware_concurrency());
+    if (const char* env = std::getenv("LCURVE_HIP_THREADS")) nthreads = std::max(1, std::atoi(env));
+    nthreads = std::max(1, std::min<int>(nthreads, static_cast<int>(sets_.size())));
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&] {
+        for (size_t s; (s = next.fetch_add(1)) < sets_.size();) build_set(s);
+      });
+    for (auto& th : pool) th.join();
   }
-  n_prows_ = prow;
+  // Concatenate, moving the set-relative offsets.
+  std::vector<ItemDev> items;
+  {
+    size_t ni = 0, nm = 0, nc = 0, nli = 0, nll = 0;
+    for (const SetItems& R : out) {
+      if (!R.error.empty()) throw DeviceError(R.error);
+      ni += R.items.size();
+      nm += R.masks.size();
+      nc += R.lchunks.size();
+      nli += R.litems.size();
+      nll += R.llanes.size();
+    }
+    if (nm > UINT32_MAX || nli > UINT32_MAX / TILE || nll > UINT32_MAX) throw DeviceError("limb tables too large");
+    items.reserve(ni);
+    lane_masks_.reserve(nm);
+    limb_chunks_.reserve(nc);
+    limb_items_.reserve(nli);
+    limb_idx_.reserve(nli * TILE);
+    limb_lanes_.reserve(nll);
+    for (SetItems& R : out) {
+      const uint32_t mbase = static_cast<uint32_t>(lane_masks_.size());
+      const uint32_t ibase = static_cast<uint32_t>(limb_items_.size());
+      const uint32_t lbase = static_cast<uint32_t>(limb_lanes_.size());
+      for (ItemDev& it : R.items) {
+        it.lm_off += mbase;
+        items.push_back(it);
+      }
+      for (LimbChunk& lc : R.lchunks) {
+        lc.item0 += ibase;
+        lc.lane_off += lbase;
+        limb_chunks_.push_back(lc);
+      }
+      lane_masks_.insert(lane_masks_.end(), R.masks.begin(), R.masks.end());
+      limb_items_.insert(limb_items_.end(), R.litems.begin(), R.litems.end());
+      limb_idx_.insert(limb_idx_.end(), R.lidx.begin(), R.lidx.end());
+      limb_lanes_.insert(limb_lanes_.end(), R.llanes.begin(), R.llanes.end());
+      pairs_limb_ += R.pairs_limb;
+      R = SetItems();
+    }
+  }
   // Items of beamed / lensed star segments go last: the kernel for the
   // others is compiled without those paths.
   auto needs_fx = [&](const ItemDev& it) {

@@ -24,7 +24,9 @@ struct Tuning {
   int persistent = 0;           // 0: one workgroup per item; 1: persistent grid
   uint32_t waves_per_cu = 8;    // persistent grid size = CUs * waves_per_cu
   uint32_t grab = 4;            // items taken per atomic (persistent)
-  int limb = 1;                 // 1: split no-eclipse patches per lane (limb chunks)
+  // Limb lanes: 0 off; otherwise a no-eclipse (patch, tile) pair is split
+  // per lane when at most `limb` of its lanes see the patch in part.
+  int limb = 48;
   bool tuned = false;
 };
 

@@ -343,36 +343,30 @@ __device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint
 // pbase: segment-local index of the item's first patch (a multiple of the
 // item's patch count, a power of two).
 __device__ __forceinline__ double moment_sum_runs(const double* __restrict__ m0,
-                                                  const double* __restrict__ blk, uint32_t np_seg,
-                                                  uint32_t pbase, uint64_t mm, const LaneCtx& L,
-                                                  double c1, double c2, double c3) {
+                                                  const double* __restrict__ blk,
+                                                  const uint32_t* __restrict__ lvl, uint32_t pbase,
+                                                  uint64_t mm, const LaneCtx& L, double c1,
+                                                  double c2, double c3) {
   const double x = L.ex, y = L.ey, z = L.ez;
   const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
   const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
                t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
   double acc = 0.0;
   while (mm) {
+    // Longest aligned run of set bits at q: limited by the ones that
+    // follow q, by the alignment of the segment-local patch index and by
+    // the block levels.
     const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
-    // Longest aligned run of set bits starting at q.
-    uint32_t k = 0;
-    uint64_t run = 1ull << q;
-    while (k < MOM_LEVELS) {
-      const uint32_t len = 2u << k;
-      if (((pbase + q) & (len - 1u)) != 0u || q + len > 64u) break;
-      const uint64_t r2 = (len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q;
-      if ((mm & r2) != r2) break;
-      run = r2;
-      ++k;
-    }
-    mm &= ~run;
-    const double* M;
-    if (k == 0) {
-      M = m0 + static_cast<size_t>(q) * MOM_STRIDE;
-    } else {
-      uint32_t off = 0;
-      for (uint32_t j = 1; j < k; ++j) off += (np_seg + (1u << j) - 1u) >> j;
-      M = blk + static_cast<size_t>(off + ((pbase + q) >> k)) * MOM_STRIDE;
-    }
+    const uint64_t inv = ~(mm >> q);
+    const uint32_t ones = inv ? static_cast<uint32_t>(__builtin_ctzll(inv)) : 64u;
+    const uint32_t pa = pbase + q;
+    uint32_t k = 31u - static_cast<uint32_t>(__builtin_clz(ones));
+    if (pa) k = min(k, static_cast<uint32_t>(__builtin_ctz(pa)));
+    k = min(k, MOM_LEVELS);
+    const uint32_t len = 1u << k;
+    mm &= ~((len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q);
+    const double* M = k == 0 ? m0 + static_cast<size_t>(q) * MOM_STRIDE
+                             : blk + static_cast<size_t>(lvl[k - 1] + (pa >> k)) * MOM_STRIDE;
     const double s1 = M[0] * x + M[1] * y + M[2] * z;
     const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
     double v = c1 * s1 + c2 * s2;
@@ -519,7 +513,7 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
     const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
     if (it.mom)
-      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, sg.n / PATCH,
+      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, segs[it.seg].lvl_off,
                             it.e0 / PATCH, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
     if (it.pmom)
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);

@@ -86,6 +86,7 @@ struct SegDev {
   // holds ceil(np / 2^k) entries (np = n / PATCH), levels back to back from
   // entry blk_off (MOM_STRIDE doubles each) of the block table.
   uint32_t blk_off;
+  uint32_t lvl_off[6];  // entry of level k + 1, relative to blk_off
   uint32_t pad;
 };
 constexpr uint32_t MOM_LEVELS = 6;  // up to 64 patches, one work item

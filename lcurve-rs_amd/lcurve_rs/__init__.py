@@ -93,12 +93,13 @@ def set_tuning(chunk, persistent=0):
     check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
 
 
-def set_limb(on):
+def set_limb(on, max_part=48):
     """Limb lanes for batches built from now on (default on): no-eclipse
     patches are split per lane into closed-form moments, limb lanes evaluated
     element by element (regrouped across tiles) and lanes that see none of
-    the patch. Off: every lane of the tile runs the element loop."""
-    check(_native.lib().lch_ctx_set_limb(_ctx(), int(bool(on))))
+    the patch, where at most `max_part` of a tile's lanes see it in part.
+    Off: every lane of the tile runs the element loop."""
+    check(_native.lib().lch_ctx_set_limb(_ctx(), int(max_part) if on else 0))
 
 
 def _dptr(a):
