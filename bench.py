@@ -149,28 +149,31 @@ def cpu_baseline(info, budget_s):
             "setup_core_seconds_per_set": round(per_set_setup, 4), "wall_s": round(dt, 3)}
 
 
-def pmc_summary(workload, chunk):
-    """HBM traffic and VALU issue figures of the flux kernel from the
-    committed rocprofv3 --pmc summary of this workload, when it was taken at
-    the same work-item size (else None)."""
-    path = os.path.join(PROFILES, f"flux_items_kernel_{workload}_pmc.json")
+def pmc_summary(workload, chunk, limb):
+    """HBM traffic and VALU issue figures of the flux pass (work items + limb
+    lanes) from the committed rocprofv3 --pmc summary of this workload, when
+    it was taken with the same work-item size and limb setting (else None)."""
+    path = os.path.join(PROFILES, f"flux_pass_{workload}_pmc.json")
     try:
         with open(path) as f:
             pm = json.load(f)
     except (OSError, ValueError):
         return None
-    return pm if pm.get("chunk") == chunk else None
+    return pm if pm.get("chunk") == chunk and pm.get("limb_max_part") == limb else None
 
 
-def roofline(workload, st, ms_flux):
+def roofline(workload, st, ms_flux, limb):
     pairs = st["pairs_evaluated"]
-    mom_lanes = st["pairs_by_moments"] // 64
+    # moment sums as evaluated (aligned runs of patches count once per lane)
+    mom_lanes = st["moment_evals"]
     flop = pairs * FLOP_PER_PAIR + mom_lanes * FLOP_PER_MOMENT_LANE
     tflops = flop / (ms_flux * 1e-3) / 1e12
     # Compulsory HBM bytes of one flux launch: element records and patch
     # moments once, lane tables, partial rows written.
+    # (+ the limb lanes' partial-row updates, 16 B per lane slot, counted
+    # within the partial rows)
     alg_bytes = st["record_bytes"] + st["n_lanes"] * 12 + 8 * 64 * st["n_items"]
-    pm = pmc_summary(workload, st["chunk"])
+    pm = pmc_summary(workload, st["chunk"], limb)
     traffic = pm["hbm_bytes_per_launch"] if pm else None
     return {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": traffic,
@@ -181,7 +184,8 @@ def roofline(workload, st, ms_flux):
             # Claret, disc and spot terms of disc_spot_300k cost more, so there it
             # understates the work done
             "flop_counts_exact": workload != "disc_spot_300k",
-            "pairs_evaluated_per_launch": pairs, "moment_patch_lanes_per_launch": mom_lanes,
+            "pairs_evaluated_per_launch": pairs, "moment_evals_per_launch": mom_lanes,
+            "pairs_by_moments_per_launch": st["pairs_by_moments"],
             "limb_pairs_per_launch": st["pairs_limb"],
             "reference_visits_per_launch": st["visits"],
             "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
@@ -430,7 +434,7 @@ def main():
 
     # Dominant kernel, timed with HIP events on its own stream.
     ms_flux, ms_pass = batch.time(scale=scale, reps=max(3, min(args.steps, 10)))
-    roof = roofline(args.workload, st, ms_flux)
+    roof = roofline(args.workload, st, ms_flux, 0 if args.no_limb else args.limb_max_part)
 
     rec = {"metric": "chi2 evals/sec (1 GPU) at <=1e-6 rel flux err vs FP64" if args.workload == "chisq_4096"
            else "light curves/sec (1 GPU)",

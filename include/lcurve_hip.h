@@ -61,8 +61,9 @@ typedef struct lch_batch_stats {
   long long pairs_evaluated; /* (element, lane) pairs the flux kernel evaluates one
                                 by one per launch, padding included */
   long long pairs_by_moments; /* pairs summed in closed form from patch moments
-                                 (seen whole, no eclipse); the rest of the
-                                 reference's pairs are skipped as exactly zero */
+                                 (seen whole, no eclipse, on the whole tile or
+                                 on single lanes); the rest of the reference's
+                                 pairs are skipped as exactly zero */
   double setup_seconds;   /* CPU setup + packing wall time */
   double upload_seconds;  /* host-to-device copies and table builds */
   int chunk;              /* elements per work item */
@@ -70,6 +71,8 @@ typedef struct lch_batch_stats {
   long long pairs_limb;   /* of pairs_evaluated: pairs of limb lanes (lanes that see
                              a no-eclipse patch only in part, regrouped across tiles) */
   long long limb_chunks;  /* workgroups of the limb pass */
+  long long moment_evals; /* moment sums evaluated per launch, counted per lane (a run of
+                             patches summed from the block table counts once) */
 } lch_batch_stats;
 
 int lch_version(void);

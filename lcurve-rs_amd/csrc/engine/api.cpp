@@ -415,6 +415,7 @@ int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
   out->persistent = B.persistent();
   out->pairs_limb = static_cast<long long>(B.pairs_limb());
   out->limb_chunks = static_cast<long long>(B.n_limb_chunks());
+  out->moment_evals = static_cast<long long>(B.moment_evals());
   return LCH_OK;
 }
 

@@ -547,9 +547,26 @@ void Batch::build_items(uint32_t chunk) {
   n_items_plain_ = static_cast<uint64_t>(mid - items.begin());
   n_items_ = items.size();
   pairs_eval_ = pairs_mom_ = 0;
+  mom_evals_ = 0;
   for (const ItemDev& it : items) {
     pairs_eval_ += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
-    pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom)) * PATCH * TILE;
+    pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom) + __builtin_popcountll(it.pmom)) * PATCH * TILE;
+    // Moment evaluations as the kernel does them: aligned runs of PT_MOM
+    // patches, one per split patch.
+    uint64_t runs = 0;
+    for (uint64_t mm = it.mom; mm;) {
+      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
+      const uint64_t inv = ~(mm >> q);
+      const uint32_t ones = inv ? static_cast<uint32_t>(__builtin_ctzll(inv)) : 64u;
+      const uint32_t pa = it.e0 / PATCH + q;
+      uint32_t k = 31u - static_cast<uint32_t>(__builtin_clz(ones));
+      if (pa) k = std::min<uint32_t>(k, static_cast<uint32_t>(__builtin_ctz(pa)));
+      k = std::min<uint32_t>(k, MOM_LEVELS);
+      const uint32_t len = 1u << k;
+      mm &= ~((len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q);
+      ++runs;
+    }
+    mom_evals_ += (runs + static_cast<uint64_t>(__builtin_popcountll(it.pmom))) * TILE;
   }
   pairs_eval_ += pairs_limb_;
   // Host-side bounds check of every item against the buffers it indexes.

@@ -565,7 +565,8 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
 }
 
 // Limb lanes: one workgroup of LIMB_WAVES wavefronts per limb chunk. The
-// chunk's limb lanes' earth vectors are staged in LDS. Each wavefront takes
+// chunk's limb lanes' earth vectors are staged in LDS (40 KB with the sums:
+// four workgroups per CU). Each wavefront takes
 // the chunk's limb items w, w + LIMB_WAVES, ...: every lane sums one patch
 // element by element (no eclipse test: the patches are PT_FREE) for its
 // listed lane and adds the sum, as a 64-bit fixed-point integer, to that
@@ -574,13 +575,16 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
 // the partial rows the work items wrote earlier in the stream.
 __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBatch d) {
   __shared__ unsigned long long acc[LIMB_UNION_MAX];
-  __shared__ float4 elane[LIMB_UNION_MAX];
+  __shared__ float elx[LIMB_UNION_MAX], ely[LIMB_UNION_MAX], elz[LIMB_UNION_MAX];
   const uint32_t ci = xcd_map(blockIdx.x, d.n_limb_chunks);
   if (ci >= d.n_limb_chunks) return;  // whole workgroup, before any barrier
   const LimbChunk c = d.limb_chunks[ci];
   for (uint32_t u = threadIdx.x; u < c.n_lanes; u += blockDim.x) {
     acc[u] = 0ull;
-    elane[u] = d.lane_e[d.limb_lanes[c.lane_off + u]];
+    const float4 e = d.lane_e[d.limb_lanes[c.lane_off + u]];
+    elx[u] = e.x;
+    ely[u] = e.y;
+    elz[u] = e.z;
   }
   __syncthreads();
   // The wavefront index is uniform across the wave; saying so keeps item
@@ -604,12 +608,12 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
     u_next = d.limb_idx[static_cast<size_t>(inext < ilast ? inext : ilast) * TILE + lane];
     const LimbItem li = d.limb_items[i];
     const bool valid = u != 0xFFFFu;
-    const float4 e = elane[valid ? u : 0u];
+    const uint32_t uu = valid ? u : 0u;
     LaneCtx L;
     L.phif = 0.0;
-    L.ex = e.x;
-    L.ey = e.y;
-    L.ez = e.z;
+    L.ex = elx[uu];
+    L.ey = ely[uu];
+    L.ez = elz[uu];
     L.kx = L.ky = L.k0 = 0.0f;
     const char* p = d.recs + sg.rec_off + static_cast<uint64_t>(li.e0) * stride;
     double s;
