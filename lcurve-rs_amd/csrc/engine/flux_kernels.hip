@@ -124,15 +124,20 @@ struct Rec4 {
   float4 r[4];
 };
 
-template <int NECL>
-__device__ __forceinline__ float4 rec_dirn(const char* __restrict__ p, uint32_t i, uint32_t j) {
+// Record i + j of a patch: from global memory (wave-uniform address, scalar
+// loads) or, with LDSR, from a copy of the patch staged in LDS (a broadcast
+// read; segments without eclipse intervals only).
+template <int NECL, bool LDSR = false>
+__device__ __forceinline__ float4 rec_dirn(const char* __restrict__ p, uint32_t i, uint32_t j,
+                                           const float4* lp = nullptr) {
+  if (LDSR) return lp[i + j];
   if (NECL == 0) return reinterpret_cast<const Rec4*>(p + i * 16u)->r[j];
   return *reinterpret_cast<const float4*>(p + (i + j) * rec_stride(NECL, false));
 }
 
-template <int NECL, bool FREE, int LD, bool MCRIT>
+template <int NECL, bool FREE, int LD, bool MCRIT, bool LDSR = false>
 __device__ __forceinline__ float star_patch(const char* __restrict__ p, const LaneCtx& L,
-                                            const StarPar& P) {
+                                            const StarPar& P, const float4* lp = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -141,7 +146,7 @@ __device__ __forceinline__ float star_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = rec_dirn<NECL>(p, i, j);
+      const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
       const float mu = mu_of(d, L);
       const float m = clamp01(mu);
       float wm = m * d.w;
@@ -194,9 +199,9 @@ __device__ __forceinline__ float star_patch_fx(const char* __restrict__ p, const
 }
 
 // Disc and disc-edge elements (flux.rs:83-100): mu > 0 and visible.
-template <int NECL, bool FREE>
+template <int NECL, bool FREE, bool LDSR = false>
 __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const LaneCtx& L, float lin,
-                                            float quad) {
+                                            float quad, const float4* lp = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -205,7 +210,7 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = rec_dirn<NECL>(p, i, j);
+      const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
       const float m = clamp01(mu_of(d, L));
       float wm = m * d.w;
       if (ECL) wm = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? wm : 0.0f;
@@ -217,8 +222,9 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
 }
 
 // Bright-spot elements (flux.rs:102-106): mu * flux where mu > 0 and visible.
-template <int NECL, bool FREE>
-__device__ __forceinline__ float spot_patch(const char* __restrict__ p, const LaneCtx& L) {
+template <int NECL, bool FREE, bool LDSR = false>
+__device__ __forceinline__ float spot_patch(const char* __restrict__ p, const LaneCtx& L,
+                                            const float4* lp = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -227,7 +233,7 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
-      const float4 d = rec_dirn<NECL>(p, i, j);
+      const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
       float w = d.w;
       if (ECL) w = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? w : 0.0f;
       acc[j] = fmaf(clamp01(mu_of(d, L)), w, acc[j]);
@@ -302,6 +308,25 @@ __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t 
   }
   if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
   return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
+}
+
+// One patch staged in LDS (segments without eclipse intervals, no beaming
+// or lensing, no eclipse test), by segment kind and limb-darkening law.
+__device__ __forceinline__ float patch_sum_lds(uint32_t kind, int ld, bool mcrit, const float4* lp,
+                                               const LaneCtx& L, const StarPar& P, float lin,
+                                               float quad) {
+  if (kind <= KIND_STAR2) {
+    if (ld == LD_LIN)
+      return mcrit ? star_patch<0, true, LD_LIN, true, true>(nullptr, L, P, lp)
+                   : star_patch<0, true, LD_LIN, false, true>(nullptr, L, P, lp);
+    if (ld == LD_CLARET)
+      return mcrit ? star_patch<0, true, LD_CLARET, true, true>(nullptr, L, P, lp)
+                   : star_patch<0, true, LD_CLARET, false, true>(nullptr, L, P, lp);
+    return mcrit ? star_patch<0, true, LD_POLY, true, true>(nullptr, L, P, lp)
+                 : star_patch<0, true, LD_POLY, false, true>(nullptr, L, P, lp);
+  }
+  if (kind == KIND_SPOT) return spot_patch<0, true, true>(nullptr, L, lp);
+  return disc_patch<0, true, true>(nullptr, L, lin, quad, lp);
 }
 
 // Patches seen whole by every lane (PT_MOM): the sum over the patch of
@@ -565,8 +590,8 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
 }
 
 // Limb lanes: one workgroup of LIMB_WAVES wavefronts per limb chunk. The
-// chunk's limb lanes' earth vectors are staged in LDS (40 KB with the sums:
-// four workgroups per CU). Each wavefront takes
+// chunk's limb lanes' earth vectors are staged in LDS (48 KB with the sums
+// and the patch buffers: three workgroups per CU). Each wavefront takes
 // the chunk's limb items w, w + LIMB_WAVES, ...: every lane sums one patch
 // element by element (no eclipse test: the patches are PT_FREE) for its
 // listed lane and adds the sum, as a 64-bit fixed-point integer, to that
@@ -576,6 +601,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
 __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBatch d) {
   __shared__ unsigned long long acc[LIMB_UNION_MAX];
   __shared__ float elx[LIMB_UNION_MAX], ely[LIMB_UNION_MAX], elz[LIMB_UNION_MAX];
+  __shared__ float4 lrec[LIMB_WAVES][PATCH];
   const uint32_t ci = xcd_map(blockIdx.x, d.n_limb_chunks);
   if (ci >= d.n_limb_chunks) return;  // whole workgroup, before any barrier
   const LimbChunk c = d.limb_chunks[ci];
@@ -602,6 +628,15 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
   // to the chunk's last item: no branch around the load).
   const uint32_t ilast = iend - 1u;
   uint32_t u_next = d.limb_idx[static_cast<size_t>(i < ilast ? i : ilast) * TILE + lane];
+  // Segments without eclipse intervals: lane j fetches record j of the next
+  // item's patch (one coalesced 1 KB load) while the current patch, staged
+  // in LDS, is summed from broadcast reads.
+  const bool staged = sg.necl == 0;
+  float4 rec_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (staged) {
+    const uint32_t e = d.limb_items[i < ilast ? i : ilast].e0;
+    rec_next = reinterpret_cast<const float4*>(d.recs + sg.rec_off)[e + lane];
+  }
   for (; i < iend; i += LIMB_WAVES) {
     const uint32_t u = u_next;
     const uint32_t inext = i + LIMB_WAVES;
@@ -617,7 +652,14 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
     L.kx = L.ky = L.k0 = 0.0f;
     const char* p = d.recs + sg.rec_off + static_cast<uint64_t>(li.e0) * stride;
     double s;
-    switch (sg.necl) {
+    if (staged) {
+      lrec[w][lane] = rec_next;
+      __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+      const uint32_t e = d.limb_items[inext < ilast ? inext : ilast].e0;
+      rec_next = reinterpret_cast<const float4*>(d.recs + sg.rec_off)[e + lane];
+      s = patch_sum_lds(sg.kind, law.ld, law.mcrit, &lrec[w][0], L, law.P, lin, quad);
+      __builtin_amdgcn_wave_barrier();  // reads done before the next write
+    } else switch (sg.necl) {
       case 0: s = item_dispatch<0, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
       case 1: s = item_dispatch<1, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
       case 2: s = item_dispatch<2, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
