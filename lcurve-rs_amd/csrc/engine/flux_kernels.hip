@@ -250,9 +250,29 @@ enum : int { PK_STAR = 0, PK_STARFX = 1, PK_DISC = 2, PK_SPOT = 3 };
 template <int PK, int NECL, int LD, int X>
 __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t todo, uint64_t fr,
                                            const LaneCtx& L, const StarPar& P, float lin,
-                                           float quad) {
+                                           float quad, float4* lp) {
   constexpr uint32_t PB = PATCH * rec_stride(NECL, PK == PK_STARFX);
   double acc = 0.0;
+  if (NECL == 0 && PK != PK_STARFX && lp) {
+    // Records staged in LDS: lane j fetches record j of the next patch (one
+    // coalesced 1 KB load) while the current patch is summed from LDS.
+    const uint32_t lane = __lane_id();
+    const float4* rp = reinterpret_cast<const float4*>(p);
+    float4 nxt = rp[static_cast<size_t>(__builtin_ctzll(todo)) * PATCH + lane];
+    while (todo) {
+      todo &= todo - 1ull;
+      lp[lane] = nxt;
+      __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+      if (todo) nxt = rp[static_cast<size_t>(__builtin_ctzll(todo)) * PATCH + lane];
+      float s;
+      if (PK == PK_STAR) s = star_patch<0, true, LD, X != 0, true>(nullptr, L, P, lp);
+      else if (PK == PK_DISC) s = disc_patch<0, true, true>(nullptr, L, lin, quad, lp);
+      else s = spot_patch<0, true, true>(nullptr, L, lp);
+      __builtin_amdgcn_wave_barrier();  // reads done before the next write
+      acc += static_cast<double>(s);
+    }
+    return acc;
+  }
   while (todo) {
     const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(todo));
     todo &= todo - 1ull;
@@ -280,34 +300,34 @@ template <int NECL, bool FX>
 __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t fx, bool mcrit,
                                                 const char* p, uint64_t todo, uint64_t fr,
                                                 const LaneCtx& L, const StarPar& P, float lin,
-                                                float quad) {
+                                                float quad, float4* lp = nullptr) {
   if (kind <= KIND_STAR2) {
     if (!FX && fx != 0) return __builtin_nan("");  // never sent here: fail loudly
     if (!FX || fx == 0) {
       if (ld == LD_LIN)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad)
-                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad, lp);
       if (ld == LD_CLARET)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad)
-                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad);
-      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad)
-                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad, lp);
+      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp)
+                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad, lp);
     }
     const bool cl = ld == LD_CLARET;
     switch (fx) {
       case 1:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp);
       case 2:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad, lp);
       default:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad, lp);
     }
   }
-  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
-  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad);
+  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
+  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
 }
 
 // One patch staged in LDS (segments without eclipse intervals, no beaming
@@ -474,7 +494,7 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
                              const double* __restrict__ mblk,
                              const double* __restrict__ lane_phi, const float4* __restrict__ lane_e,
                              const uint32_t* __restrict__ lane_meta, const uint64_t* __restrict__ lane_masks,
-                             double* __restrict__ partials) {
+                             double* __restrict__ partials, float4* lp) {
   const ItemDev it = items[id];
   const uint32_t lane = threadIdx.x;
   const uint32_t gl = it.lane0 + lane;
@@ -544,11 +564,11 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
-    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
-    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad); break;
+    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
   }
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
@@ -563,9 +583,11 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
+  __shared__ float4 pbuf[PATCH];  // one patch of records (one wavefront per workgroup)
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item<FX>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials);
+  process_item<FX>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials,
+                   pbuf);
 }
 
 // Persistent form: a fixed grid of wavefronts pulls items from a counter.
@@ -577,6 +599,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials,
     uint32_t* __restrict__ counter, uint32_t grab) {
+  __shared__ float4 pbuf[PATCH];
   for (;;) {
     uint32_t base = 0;
     if (threadIdx.x == 0) base = atomicAdd(counter, grab);
@@ -585,7 +608,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
       process_item<true>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks,
-                         partials);
+                         partials, pbuf);
   }
 }
 
