@@ -250,17 +250,15 @@ enum : int { PK_STAR = 0, PK_STARFX = 1, PK_DISC = 2, PK_SPOT = 3 };
 template <int PK, int NECL, int LD, int X>
 __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t todo, uint64_t fr,
                                            const LaneCtx& L, const StarPar& P, float lin,
-                                           float quad, float4* lp, float4 first) {
+                                           float quad, float4* lp) {
   constexpr uint32_t PB = PATCH * rec_stride(NECL, PK == PK_STARFX);
   double acc = 0.0;
   if (NECL == 0 && PK != PK_STARFX && lp) {
     // Records staged in LDS: lane j fetches record j of the next patch (one
-    // coalesced 1 KB load) while the current patch is summed from LDS. The
-    // first patch's records were fetched by the caller (`first`), before
-    // the item's moment sums.
+    // coalesced 1 KB load) while the current patch is summed from LDS.
     const uint32_t lane = __lane_id();
     const float4* rp = reinterpret_cast<const float4*>(p);
-    float4 nxt = first;
+    float4 nxt = rp[static_cast<size_t>(__builtin_ctzll(todo)) * PATCH + lane];
     while (todo) {
       todo &= todo - 1ull;
       lp[lane] = nxt;
@@ -302,35 +300,34 @@ template <int NECL, bool FX>
 __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t fx, bool mcrit,
                                                 const char* p, uint64_t todo, uint64_t fr,
                                                 const LaneCtx& L, const StarPar& P, float lin,
-                                                float quad, float4* lp = nullptr,
-                                                float4 first = float4{}) {
+                                                float quad, float4* lp = nullptr) {
   if (kind <= KIND_STAR2) {
     if (!FX && fx != 0) return __builtin_nan("");  // never sent here: fail loudly
     if (!FX || fx == 0) {
       if (ld == LD_LIN)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad, lp, first)
-                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad, lp, first);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad, lp);
       if (ld == LD_CLARET)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp, first)
-                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad, lp, first);
-      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp, first)
-                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad, lp, first);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad, lp);
+      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp)
+                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad, lp);
     }
     const bool cl = ld == LD_CLARET;
     switch (fx) {
       case 1:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp, first)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp, first);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp);
       case 2:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad, lp, first)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad, lp, first);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad, lp);
       default:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad, lp, first)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad, lp, first);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad, lp);
     }
   }
-  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp, first);
-  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp, first);
+  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
+  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
 }
 
 // One patch staged in LDS (segments without eclipse intervals, no beaming
@@ -551,11 +548,6 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
   const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
   const char* p = recs + sg.rec_off + static_cast<uint64_t>(it.e0) * stride;
   const float lin = sp.lin_disc, quad = sp.quad_disc;
-  // The first element-by-element patch of a segment without eclipse
-  // intervals is fetched now, so the load overlaps the moment sums.
-  float4 first = float4{};
-  if (sg.necl == 0 && fx == 0 && it.todo)
-    first = reinterpret_cast<const float4*>(p)[static_cast<size_t>(__builtin_ctzll(it.todo)) * PATCH + lane];
   double sum = 0.0;
   if (it.mom | it.pmom) {
     // w mu (1 - A o - B o^2) with o = 1 - mu, as c1 mu + c2 mu^2 + c3 mu^3:
@@ -572,11 +564,11 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
-    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp, first); break;
-    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp, first); break;
-    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp, first); break;
-    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp, first); break;
-    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp, first); break;
+    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
   }
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
