@@ -434,7 +434,8 @@ def main():
 
     # Dominant kernel, timed with HIP events on its own stream.
     ms_flux, ms_pass = batch.time(scale=scale, reps=max(3, min(args.steps, 10)))
-    roof = roofline(args.workload, st, ms_flux, 0 if args.no_limb else args.limb_max_part)
+    roof = roofline(args.workload, st, ms_flux,
+                    (0 if args.no_limb else args.limb_max_part) if st["limb_chunks"] else 0)
 
     rec = {"metric": "chi2 evals/sec (1 GPU) at <=1e-6 rel flux err vs FP64" if args.workload == "chisq_4096"
            else "light curves/sec (1 GPU)",
@@ -451,7 +452,8 @@ def main():
            "max_rel_chisq_err_vs_fp64": chi_err,
            "cpu_setup_s": round(st["setup_seconds"], 2), "upload_s": round(st["upload_seconds"], 2),
            "tuning": {"chunk": st["chunk"], "persistent": st["persistent"], "items": st["n_items"],
-                      "limb": 0 if args.no_limb else args.limb_max_part, "limb_chunks": st["limb_chunks"]},
+                      "limb": (0 if args.no_limb else args.limb_max_part) if st["limb_chunks"] else 0,
+                      "limb_chunks": st["limb_chunks"]},
            "roofline": roof}
     if not args.no_cpu_baseline:
         log("CPU baseline (FP64 oracle, bounded sample) ...")

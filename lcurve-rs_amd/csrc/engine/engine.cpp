@@ -824,25 +824,36 @@ void Batch::set_obs(const double* flux2d, const double* ferr2d, const double* we
 }
 
 void Batch::autotune(int scale) {
-  struct Cand { uint32_t chunk; int persistent; };
-  const Cand cands[] = {{256, 0}, {512, 0}, {1024, 0}, {2048, 0}, {4096, 0}, {1024, 1}, {2048, 1}, {4096, 1}};
+  // Limb lanes at the context's threshold, or off (a small batch may not
+  // repay the extra launch).
+  struct Cand { uint32_t chunk; int persistent; bool limb; };
+  const Cand cands[] = {{256, 0, true},  {512, 0, true},  {1024, 0, true},  {2048, 0, true},  {4096, 0, true},
+                        {256, 0, false}, {512, 0, false}, {1024, 0, false}, {2048, 0, false}, {4096, 0, false},
+                        {1024, 1, true}, {2048, 1, true}, {4096, 1, true}};
+  const int limb = ctx_.tuning.limb;
   Tuning best = ctx_.tuning;
   double best_ms = 1e300;
   for (const Cand& c : cands) {
+    if (!c.limb && limb == 0) continue;  // already covered
     ctx_.tuning.chunk = c.chunk;
     ctx_.tuning.persistent = c.persistent;
+    ctx_.tuning.limb = c.limb ? limb : 0;
     build_items(c.chunk);
     double ms_f = 0, ms_t = 0;
     time(scale, 1, &ms_f, &ms_t);  // warm
     time(scale, 3, &ms_f, &ms_t);
-    if (ms_f < best_ms) {
-      best_ms = ms_f;
+    // The whole pass counts: smaller chunks can speed the flux kernels but
+    // give the assembly more partial rows to sum.
+    if (ms_t < best_ms) {
+      best_ms = ms_t;
       best = ctx_.tuning;
     }
   }
   best.tuned = true;
   ctx_.tuning = best;
   build_items(best.chunk);
+  // The limb threshold stays the caller's for batches built later.
+  ctx_.tuning.limb = limb;
 }
 
 }  // namespace lch
