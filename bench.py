@@ -253,8 +253,8 @@ def bench_surrogate(args):
         import mlp_oracle
         dims = (5, 512, 512, 512, 512, 256)
         p = s.net.get_params().astype(np.float64)
-        xs = mcmc.to_unit(P[nh:nh + 256]).astype(np.float64)
-        ys = (C[nh:nh + 256] - mcmc.quad_features(mcmc.to_unit(P[nh:nh + 256])) @ s.coef) / s.y_std
+        xs = s.frame.to_u(P[nh:nh + 256])
+        ys = (C[nh:nh + 256] - s.baseline(P[nh:nh + 256])) / s.y_std
         t0 = time.perf_counter()
         reps = 0
         while time.perf_counter() - t0 < 10.0:
@@ -267,72 +267,107 @@ def bench_surrogate(args):
     print(json.dumps(rec), flush=True)
 
 
+DA_NLAT = (40, 20, 40, 40)
+DA_EXPOSE, DA_NDIV = 0.001, 5
+
+
 def bench_da_mcmc(args):
     """Delayed acceptance (surrogate transition) vs plain exact ensemble
     MCMC on a synthetic 3000-point, 0.5%-noise curve of the one-binary
-    geometry: pilot run, surrogate trained on the pilot posterior +-10 sigma,
-    then both samplers from the same equilibrated walkers."""
+    geometry (exposure 0.001 in phase, smeared over 5 sub-times): a pilot
+    run, a surrogate trained on a ball of ``--radius`` posterior standard
+    deviations around the pilot posterior (whitened coordinates), predicting
+    the curve at the data times, then both samplers from the same
+    equilibrated walkers."""
     from lcurve_rs import mcmc
-    base = mcmc.surrogate_base_text()
+    base = mcmc.surrogate_base_text(DA_NLAT)
     truth = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
     times = np.linspace(-0.2, 1.2, 3000)
-    tg = mcmc.Target(base, truth, times, args.noise, seed=42)
+    tg = mcmc.Target(base, truth, times, args.noise, seed=42, expose=DA_EXPOSE, ndiv=DA_NDIV)
     rng = np.random.default_rng(1)
     nw = args.walkers
     p0 = truth * (1.0 + 1e-4 * rng.standard_normal((nw, 5)))
     log(f"pilot: {nw} walkers x {args.pilot_steps} exact steps ...")
-    pilot = mcmc.ensemble_mcmc(tg, p0, args.pilot_steps, seed=1)
+    pilot = mcmc.ensemble_mcmc(tg, p0, args.pilot_steps, seed=1, log=log)
     tail = pilot["chain"][args.pilot_steps // 2:].reshape(-1, 5)
-    box = mcmc.local_box(tail.mean(0), tail.std(0), 10.0)
-    log(f"training set: {args.ncurves} curves over the pilot posterior +-10 sigma ...")
+    frame = mcmc.Frame.local(tail, args.radius)
+    ncurves = args.ncurves
+    log(f"training set: {ncurves} curves over the pilot posterior ball of radius {args.radius} sigma ...")
     t0 = time.perf_counter()
-    params = mcmc.sample_box(args.ncurves, 7, box)
-    phases = mcmc.eclipse_phases(256, 192, 0.06)
-    curves = mcmc.make_training_set(base, params, log=log, phases=phases)
+    params = frame.sample(ncurves, 7)
+    curves = mcmc.make_training_set(base, params, chunk=4000, log=log, times=times, expose=DA_EXPOSE,
+                                   ndiv=DA_NDIV)
     t_gen = time.perf_counter() - t0
     ok = np.all(np.isfinite(curves), axis=1)
     P, C = params[ok], curves[ok]
     nh = max(512, len(P) // 20)
-    s = mcmc.Surrogate(seed=1, box=box, phases=phases)
+    s = mcmc.Surrogate(seed=1, frame=frame, times=times, degree=args.degree, tail=True)
     fit = s.fit(P[nh:], C[nh:], epochs=args.epochs, log=log)
     err = s.predict(P[:nh]) - C[:nh]
+    err_poly = s.baseline(P[:nh]) - C[:nh]
     start = pilot["chain"][-1].copy()
     log(f"plain exact MCMC: {args.steps} steps ...")
-    plain = mcmc.ensemble_mcmc(tg, start, args.steps, seed=2)
-    log(f"delayed acceptance, {args.inner} surrogate moves per exact check: {args.steps} steps ...")
-    da = mcmc.ensemble_mcmc(tg, start, args.steps, seed=3, surrogate=s, inner_steps=args.inner)
-    lp_s = tg.logp_surrogate(s, da["chain"][-1])
-    lp_e = tg.logp_exact(da["chain"][-1])
+    plain = mcmc.ensemble_mcmc(tg, start, args.steps, seed=2, log=log)
+    inners = [int(v) for v in str(args.inner).split(",")]
+    sweep = []
+    for inner in inners:
+        log(f"delayed acceptance, {inner} surrogate moves per exact check: {args.steps} steps ...")
+        r = mcmc.ensemble_mcmc(tg, start, args.steps, seed=3, surrogate=s, inner_steps=inner, log=log)
+        e, t = mcmc.ess(r["chain"], burn=0.1)
+        sweep.append({"inner_steps": inner, "ess_per_s": round(e / r["seconds"], 3), "ess": round(float(e), 1),
+                      "tau": t.round(2).tolist(), "seconds": round(r["seconds"], 1),
+                      "acceptance": round(r["accepted"] / r["proposals"], 3), "n_exact": r["n_exact"]})
+        if len(sweep) == 1:
+            da = r
+    # Log-likelihood error of the surrogate (and of its polynomial alone) over
+    # the plain chain's last states: exact values from one more batch.
+    probe = plain["chain"][-1]
+    lp_e = tg.logp_exact(probe)
+    lp_s = tg.logp_surrogate(s, probe)
+    lp_poly = -0.5 * mcmc.autoscaled_chisq(np.exp(s.baseline(probe)), tg.y, tg.ferr, tg.w)
     ess_p, tau_p = mcmc.ess(plain["chain"], burn=0.1)
     ess_d, tau_d = mcmc.ess(da["chain"], burn=0.1)
     sp, sd = mcmc.summarize(plain["chain"], 0.1), mcmc.summarize(da["chain"], 0.1)
     sig = 0.5 * (sp[2] - sp[0])
     value = ess_d / da["seconds"]
     base_v = ess_p / plain["seconds"]
+    build_s = t_gen + fit["seconds"]
+    u_plain = np.sqrt(np.max(np.sum(frame.to_u(plain["chain"][len(plain["chain"]) // 10:].reshape(-1, 5)) ** 2, 1)))
+    out_da = float(np.mean(~frame.contains(da["chain"][len(da["chain"]) // 10:].reshape(-1, 5))))
     rec = {"metric": "MCMC effective samples/s (1 GPU), delayed acceptance", "value": round(value, 3),
            "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.pilot_steps,
            "ms_per_step": round(1e3 * da["seconds"] / args.steps, 2), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32/f64 exact, bf16 surrogate",
-           "data": "synthetic target curve (3000 times, 0.5% noise) of the one-binary geometry",
-           "config": {"workload": "da_mcmc", "walkers": nw, "steps": args.steps, "inner_steps": args.inner,
-                      "free": list(mcmc.PARAM_NAMES), "truth": truth.tolist(), "grid_nlat": [20, 20, 40, 40],
-                      "surrogate_curves": args.ncurves, "epochs": args.epochs, "noise": args.noise,
-                      "output_phases": "192 in |phase| < 0.06, 64 elsewhere"},
+           "data": f"synthetic target curve (3000 times, {args.noise:g} relative noise) of the one-binary geometry",
+           "config": {"workload": "da_mcmc", "walkers": nw, "steps": args.steps, "inner_steps": inners[0],
+                      "free": list(mcmc.PARAM_NAMES), "truth": truth.tolist(), "grid_nlat": list(DA_NLAT),
+                      "expose": DA_EXPOSE, "ndiv": DA_NDIV, "surrogate_curves": ncurves, "epochs": args.epochs,
+                      "noise": args.noise,
+                      "frame": f"ball of {args.radius} pilot sigma, whitened; Gaussian tail outside",
+                      "baseline_degree": args.degree, "outputs": "the 3000 data times (smeared)"},
            "plain": {"ess_per_s": round(base_v, 3), "ess": round(float(ess_p), 1), "tau": tau_p.round(2).tolist(),
                      "seconds": round(plain["seconds"], 1), "n_exact": plain["n_exact"],
-                     "acceptance": round(plain["accepted"] / plain["proposals"], 3)},
+                     "acceptance": round(plain["accepted"] / plain["proposals"], 3),
+                     "max_radius_in_frame": round(float(u_plain), 3)},
            "delayed_acceptance": {"ess_per_s": round(value, 3), "ess": round(float(ess_d), 1),
                                   "tau": tau_d.round(2).tolist(), "seconds": round(da["seconds"], 1),
                                   "n_exact": da["n_exact"], "acceptance": round(da["accepted"] / da["proposals"], 3),
-                                  "moved_to_exact_check": da["stage1_passed"], "fallback_at": da["fallback_at"]},
+                                  "moved_to_exact_check": da["stage1_passed"], "fallback_at": da["fallback_at"],
+                                  "fraction_outside_ball": round(out_da, 6),
+                                  "ess_per_s_incl_surrogate_build": round(ess_d / (da["seconds"] + build_s), 3)},
            "speedup_ess_per_s": round(value / base_v, 2),
+           "inner_sweep": [dict(v, speedup=round(v["ess_per_s"] / base_v, 2)) for v in sweep],
+           "speedup_incl_surrogate_build": round(ess_d / (da["seconds"] + build_s) / base_v, 2),
            "posterior": {"names": list(mcmc.PARAM_NAMES), "plain_p16_p50_p84": sp.tolist(),
                          "da_p16_p50_p84": sd.tolist(),
                          "max_median_shift_in_sigma": float(np.max(np.abs(sd[1] - sp[1]) / sig)),
                          "width_ratio": (0.5 * (sd[2] - sd[0]) / sig).round(3).tolist()},
-           "surrogate": {"box": box.tolist(), "heldout_rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
-                         "loglike_error_spread_at_end": float(np.std(lp_s - lp_e)),
-                         "train_s": round(fit["seconds"], 2), "data_generation_s": round(t_gen, 1)},
+           "surrogate": {"heldout_rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
+                         "heldout_rms_log_flux_polynomial_only": float(np.sqrt(np.mean(err_poly ** 2))),
+                         "loglike_error_spread": float(np.std(lp_s - lp_e)),
+                         "loglike_error_spread_polynomial_only": float(np.std(lp_poly - lp_e)),
+                         "train_s": round(fit["seconds"], 2), "data_generation_s": round(t_gen, 1),
+                         "center": frame.center.tolist()},
            "pilot": {"seconds": round(pilot["seconds"], 1), "n_exact": pilot["n_exact"]}}
     print(json.dumps(rec), flush=True)
 
@@ -360,11 +395,16 @@ def main():
     ap.add_argument("--workload", default="chisq_4096",
                     choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k", "surrogate_200k", "da_mcmc"])
     ap.add_argument("--ncurves", type=int, default=None,
-                    help="training curves (surrogate_200k: 200000; da_mcmc: 30000)")
+                    help="training curves (surrogate_200k: 200000; da_mcmc: 16000)")
     ap.add_argument("--epochs", type=int, default=None, help="surrogate training epochs")
     ap.add_argument("--walkers", type=int, default=64)
-    ap.add_argument("--pilot-steps", type=int, default=120)
-    ap.add_argument("--inner", type=int, default=10, help="surrogate moves per exact check (da_mcmc)")
+    ap.add_argument("--pilot-steps", type=int, default=200)
+    ap.add_argument("--inner", default="25",
+                    help="surrogate moves per exact check (da_mcmc); a comma list runs one sampler per value "
+                         "after the same plain run, the first giving the reported value")
+    ap.add_argument("--radius", type=float, default=6.5,
+                    help="da_mcmc surrogate domain: pilot posterior standard deviations (whitened ball)")
+    ap.add_argument("--degree", type=int, default=3, help="da_mcmc surrogate polynomial baseline degree")
     ap.add_argument("--noise", type=float, default=0.005, help="relative noise of the da_mcmc target curve")
     ap.add_argument("--nsets", type=int, default=4096, help="sets of chisq_4096 (4096 is the workload)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -396,8 +436,8 @@ def main():
         bench_surrogate(args)
         return
     if args.workload == "da_mcmc":
-        args.ncurves = args.ncurves or 30000
-        args.epochs = args.epochs or 300
+        args.ncurves = args.ncurves or 16000
+        args.epochs = args.epochs or 200
         if args.steps == 10:
             args.steps = 400
         bench_da_mcmc(args)

@@ -208,6 +208,29 @@ int lch_mlp_predict(lch_mlp* m, const float* x, int n, float* y);
  * alone (HIP events), and the GEMM flops of one step. */
 int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop);
 
+/* ---- surrogate log-likelihood ----
+ * No counterpart in the reference (whose likelihood is the exact
+ * chi-squared of re_scale, crates/lcurve/src/orchestration.rs:361-388).
+ * The autoscaled chi-squared of a surrogate curve at the data times, for
+ * many parameter rows, in one call on the GPU: frame coordinates
+ * u = (p - center) . M (M ndim x ndim row-major), the polynomial baseline
+ * sum_f phi_f(u) coef[f][j] (phi: 1, then the monomials of each degree up
+ * to `degree` in lexicographic index order; coef [nfeat x nbins]
+ * row-major, nfeat from lch_surr_lik_n_features), plus y_std times the
+ * network's output j, exponentiated; then wt = w / ferr^2,
+ * s = sum(m wt y) / sum(m^2 wt) (1 if not positive) and
+ * logl = -0.5 sum(w ((y - s m) / ferr)^2), all FP64 but the network.
+ * The network must outlive the likelihood and have in == ndim and
+ * out >= nbins; training it afterwards changes the likelihood with it.
+ * Not thread-safe. */
+typedef struct lch_surr_lik lch_surr_lik;
+int lch_surr_lik_n_features(int ndim, int degree);
+int lch_surr_lik_create(lch_mlp* net, int ndim, const double* center, const double* M, int degree,
+                        const double* coef, int nbins, double y_std, const double* y, const double* ferr,
+                        const double* w, lch_surr_lik** out);
+int lch_surr_lik_eval(lch_surr_lik* s, const double* params, int n, double* logl);
+void lch_surr_lik_free(lch_surr_lik* s);
+
 /* ---- diagnostics (CPU only, no device needed) ---- */
 /* Packs one model as the engine does and re-evaluates, in FP64 on the CPU,
  * every (element, lane) pair twice: as the per-(patch, tile) work codes

@@ -10,6 +10,7 @@
 #include "engine.h"
 #include "selfcheck.h"
 #include "../surrogate/mlp.h"
+#include "../surrogate/surr_lik.h"
 
 struct lch_ctx {
   lch::Context impl;
@@ -20,6 +21,9 @@ struct lch_model {
 };
 struct lch_mlp {
   std::unique_ptr<lch::Mlp> impl;
+};
+struct lch_surr_lik {
+  std::unique_ptr<lch::SurrogateLik> impl;
 };
 struct lch_batch {
   std::unique_ptr<lch::Batch> impl;
@@ -566,6 +570,34 @@ int lch_mlp_predict(lch_mlp* m, const float* x, int n, float* y) {
     return LCH_OK;
   });
 }
+
+int lch_surr_lik_create(lch_mlp* net, int ndim, const double* center, const double* M, int degree,
+                        const double* coef, int nbins, double y_std, const double* y, const double* ferr,
+                        const double* w, lch_surr_lik** out) {
+  if (!net || !center || !M || !coef || !y || !ferr || !w || !out) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    auto s = std::make_unique<lch_surr_lik>();
+    s->impl = std::make_unique<lch::SurrogateLik>(*net->impl, ndim, center, M, degree, coef, nbins, y_std, y, ferr,
+                                                  w);
+    *out = s.release();
+    return LCH_OK;
+  });
+}
+
+int lch_surr_lik_n_features(int ndim, int degree) {
+  if (ndim <= 0 || ndim > 64 || degree < 0 || degree > 6) return -1;
+  return lch::SurrogateLik::n_features(ndim, degree);
+}
+
+int lch_surr_lik_eval(lch_surr_lik* s, const double* params, int n, double* logl) {
+  if (!s || n < 0 || (n > 0 && (!params || !logl))) return fail(LCH_ERR_INVALID, "bad argument");
+  return guarded([&] {
+    s->impl->eval(params, n, logl);
+    return LCH_OK;
+  });
+}
+
+void lch_surr_lik_free(lch_surr_lik* s) { delete s; }
 
 int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop) {
   if (!m || reps <= 0 || !ms_step || !ms_gemm || !gemm_flop) return fail(LCH_ERR_INVALID, "bad argument");

@@ -108,6 +108,12 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
   At_ = alloc<uint16_t>(static_cast<size_t>(max_batch) * wmax);
   if (2 * L > mlp::ADAM_MAX_TENSORS) throw std::invalid_argument("too many layers");
   idx_ = alloc<int>(max_batch);
+  ident_ = alloc<int>(max_batch);
+  {
+    std::vector<int> ident(max_batch);
+    for (int i = 0; i < max_batch; ++i) ident[i] = i;
+    copy(ident_, ident.data(), max_batch * sizeof(int), hipMemcpyHostToDevice);
+  }
   partial_ = alloc<double>(loss_blocks_);
   for (hipEvent_t& e : ev_) check(hipEventCreate(&e), "hipEventCreate");
   check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
@@ -354,21 +360,23 @@ double Mlp::loss_and_grad(const float* x, const float* y, int n, float* grads) {
   return sum / (static_cast<double>(n) * out_dim());
 }
 
+void Mlp::forward_device(const float* x, int rows) {
+  if (rows <= 0 || rows % 64 || rows > max_batch_) throw std::invalid_argument("rows must be a positive multiple of 64 <= max_batch");
+  check(mlp::launch_gather(x, nullptr, ident_, rows, in_dim(), 64, out_dim(), A_[0], nullptr, ctx_.stream()),
+        "gather");
+  forward(rows);
+}
+
 void Mlp::predict(const float* x, int n, float* y) {
   if (n < 0) throw std::invalid_argument("negative row count");
   const int ch = max_batch_;
-  std::vector<int> ident(ch);
-  for (int i = 0; i < ch; ++i) ident[i] = i;
-  copy(idx_, ident.data(), ch * sizeof(int), hipMemcpyHostToDevice);
   for (int r0 = 0; r0 < n; r0 += ch) {
     const int rows = std::min(ch, n - r0);
     const int padded = (rows + 63) / 64 * 64;
     std::vector<float> xs(static_cast<size_t>(padded) * in_dim(), 0.0f);
     std::memcpy(xs.data(), x + static_cast<size_t>(r0) * in_dim(), static_cast<size_t>(rows) * in_dim() * sizeof(float));
     copy(xin_, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice);
-    check(mlp::launch_gather(xin_, nullptr, idx_, padded, in_dim(), 64, out_dim(), A_[0], nullptr, ctx_.stream()),
-          "gather");
-    forward(padded);
+    forward_device(xin_, padded);
     copy(y + static_cast<size_t>(r0) * out_dim(), Z_.back(),
                     static_cast<size_t>(rows) * out_dim() * sizeof(float), hipMemcpyDeviceToHost);
   }

@@ -44,6 +44,13 @@ class Mlp {
   void adam_step(float lr);
   // Forward pass of n host rows (any n), y [n x out].
   void predict(const float* x, int n, float* y);
+  // Forward pass of `rows` device rows x [rows x in] (rows a multiple of 64,
+  // at most max_batch), queued on the context stream without waiting; the
+  // outputs are then in output() [rows x out].
+  void forward_device(const float* x, int rows);
+  const float* output() const { return Z_.back(); }
+  int max_batch() const { return max_batch_; }
+  Context& context() const { return ctx_; }
   // Mean time of `reps` training steps on resident data, and of their GEMMs
   // alone (HIP events); flops per step of the GEMMs.
   void time_step(int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop);
@@ -81,6 +88,7 @@ class Mlp {
   float* X_ = nullptr;
   float* Y_ = nullptr;
   int* idx_ = nullptr;
+  int* ident_ = nullptr;  // 0, 1, ..., max_batch - 1
   int n_data_ = 0;
   int t_ = 0;  // Adam step count
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};

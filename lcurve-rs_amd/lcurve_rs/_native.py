@@ -96,6 +96,10 @@ SIGNATURES = {
     "lch_mlp_adam_step": (INT, [VP, ctypes.c_float]),
     "lch_mlp_predict": (INT, [VP, F, INT, F]),
     "lch_mlp_time_step": (INT, [VP, INT, INT, D, D, D]),
+    "lch_surr_lik_n_features": (INT, [INT, INT]),
+    "lch_surr_lik_create": (INT, [VP, INT, D, D, INT, D, INT, ctypes.c_double, D, D, D, ctypes.POINTER(VP)]),
+    "lch_surr_lik_eval": (INT, [VP, D, INT, D]),
+    "lch_surr_lik_free": (None, [VP]),
     "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
     "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),
 }

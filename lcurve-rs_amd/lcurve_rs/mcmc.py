@@ -1,10 +1,15 @@
 """Surrogate-assisted MCMC over light-curve models.
 
 * ``make_training_set`` — curves from the exact GPU method over a parameter
-  box, folded to phase bins: log(flux / wdwarf) at the bin centres.
-* ``Surrogate`` — the GPU MLP (lcurve_rs.surrogate.Mlp) trained on them,
-  with the input / output normalisation and the phase interpolation that
-  turns its bins into a curve at arbitrary phases.
+  region: log(flux / wdwarf) at phase-bin centres, or at a target's own
+  times with its exposure smearing.
+* ``Frame`` — the affine coordinates a surrogate works in: the unit box of a
+  parameter box, or a ball in whitened coordinates around a posterior (mean
+  and covariance of pilot samples), which follows the parameter
+  correlations instead of wasting the fit on the corners of a box.
+* ``Surrogate`` — an FP64 polynomial baseline in the frame's coordinates
+  plus the GPU MLP (lcurve_rs.surrogate.Mlp) trained on its residual, with
+  the phase interpolation that turns bins into a curve at arbitrary phases.
 * ``ensemble_mcmc`` — the affine-invariant stretch-move ensemble sampler
   (Goodman & Weare 2010) with exact likelihoods, or with delayed
   acceptance (Christen & Fox 2005) generalised to a surrogate transition
@@ -30,6 +35,7 @@
 Likelihoods are exp(-chi^2 / 2) with the autoscale factor fitted
 (re_scale, orchestration.rs:361-388), flat priors on the box.
 """
+import itertools
 import math
 import time
 
@@ -37,7 +43,7 @@ import numpy as np
 
 from . import Batch, Model
 from . import workloads as W
-from .surrogate import Mlp
+from .surrogate import Mlp, SurrogateLikelihood
 
 PARAM_NAMES = ("q", "iangle", "r1", "t1", "t2")
 # The batched-chi-squared box: q, iangle (deg), r1 (a), t1, t2 (K).
@@ -89,17 +95,23 @@ def eclipse_phases(n=256, n_dense=192, half_width=0.06):
     return np.sort(np.mod(np.concatenate([d, o]), 1.0))
 
 
-def make_training_set(base_text, params, nbins=256, chunk=20000, log=None, phases=None):
-    """log(flux / wdwarf) at the output phases (default: nbins bin centres)
-    for every parameter row, from the exact GPU method. Rows whose setup
-    fails come back as NaN."""
+def make_training_set(base_text, params, nbins=256, chunk=20000, log=None, phases=None, times=None, expose=0.0,
+                      ndiv=1):
+    """log(flux / wdwarf) for every parameter row, from the exact GPU
+    method: at the output phases (default: nbins bin centres), or, with
+    ``times``, at those times with exposure ``expose`` smeared over ``ndiv``
+    sub-times. Rows whose setup fails come back as NaN."""
     model = Model.from_text(base_text)
-    ph = bin_phases(nbins) if phases is None else np.asarray(phases, dtype=np.float64)
+    if times is not None:
+        ph = np.asarray(times, dtype=np.float64)
+    else:
+        ph = bin_phases(nbins) if phases is None else np.asarray(phases, dtype=np.float64)
+        expose, ndiv = 0.0, 1
     nbins = len(ph)
     out = np.full((len(params), nbins), np.nan)
     for c0 in range(0, len(params), chunk):
         rows = params[c0:c0 + chunk]
-        b = Batch(model, ph, 0.0, 1, names=PARAM_NAMES, values=rows)
+        b = Batch(model, ph, expose, ndiv, names=PARAM_NAMES, values=rows)
         r = b.run(scale=False)
         with np.errstate(invalid="ignore", divide="ignore"):
             out[c0:c0 + len(rows)] = np.log(r["flux"] / r["wdwarf"][:, None])
@@ -116,43 +128,144 @@ def local_box(center, width, nsig=10.0, box=BOX):
     return np.stack([lo, hi], axis=1)
 
 
-def quad_features(u):
-    """1, u_i, u_i u_j (i <= j) for unit-box coordinates u [n, d]."""
-    u = np.atleast_2d(u)
+def poly_features(u, degree):
+    """Every monomial of the coordinates u [n, d] up to ``degree``: 1, u_i,
+    u_i u_j (i <= j), u_i u_j u_k (i <= j <= k), ..."""
+    u = np.atleast_2d(np.asarray(u, dtype=np.float64))
     n, d = u.shape
-    cols = [np.ones(n)] + [u[:, i] for i in range(d)]
-    cols += [u[:, i] * u[:, j] for i in range(d) for j in range(i, d)]
+    cols = [np.ones(n)]
+    for k in range(1, degree + 1):
+        cols += [np.prod(u[:, list(c)], axis=1) for c in itertools.combinations_with_replacement(range(d), k)]
     return np.stack(cols, axis=1)
 
 
+def quad_features(u):
+    """1, u_i, u_i u_j (i <= j) for unit-box coordinates u [n, d]."""
+    return poly_features(u, 2)
+
+
+class Frame:
+    """Affine coordinates u = (p - center) @ M with a domain: the unit cube
+    |u_i| <= 1 (``Frame.from_box``) or the unit ball |u| <= 1
+    (``Frame.local``)."""
+
+    def __init__(self, center, M, kind, radius=1.0):
+        if kind not in ("box", "ball"):
+            raise ValueError("kind must be 'box' or 'ball'")
+        self.center = np.asarray(center, dtype=np.float64)
+        self.M = np.asarray(M, dtype=np.float64)
+        self.Minv = np.linalg.inv(self.M)
+        self.kind = kind
+        self.radius = float(radius)  # ball: standard deviations per unit of u
+
+    @classmethod
+    def from_box(cls, box):
+        box = np.asarray(box, dtype=np.float64)
+        return cls(0.5 * (box[:, 0] + box[:, 1]), np.diag(2.0 / (box[:, 1] - box[:, 0])), "box")
+
+    @classmethod
+    def local(cls, samples, radius=6.0):
+        """The ball of ``radius`` posterior standard deviations (Mahalanobis
+        distance under the samples' covariance) around the samples' mean."""
+        s = np.asarray(samples, dtype=np.float64)
+        L = np.linalg.cholesky(np.cov(s, rowvar=False))
+        return cls(s.mean(0), np.linalg.inv(L).T / radius, "ball", radius)
+
+    def to_u(self, p):
+        return (np.atleast_2d(np.asarray(p, dtype=np.float64)) - self.center) @ self.M
+
+    def from_u(self, u):
+        return self.center + np.atleast_2d(u) @ self.Minv
+
+    def contains(self, p):
+        u = self.to_u(p)
+        if self.kind == "box":
+            return np.all(np.abs(u) <= 1.0, axis=1)
+        return np.einsum("ij,ij->i", u, u) <= 1.0
+
+    def sample(self, n, seed, core=1.0 / 3.0):
+        """n points of the domain. Box: uniform (splitmix64 stream). Ball:
+        half from a normal of standard deviation ``core`` (in units of the
+        radius) cut to the ball, where the posterior mass lies, and the
+        rest uniform in the ball."""
+        d = len(self.center)
+        if self.kind == "box":
+            lo, hi = self.from_u(-np.ones(d))[0], self.from_u(np.ones(d))[0]
+            return sample_box(n, seed, np.stack([lo, hi], axis=1))
+        rng = np.random.default_rng(seed)
+        g = core * rng.standard_normal((2 * (n // 2) + 8, d))
+        g = g[np.einsum("ij,ij->i", g, g) <= 1.0][: n // 2]
+        m = n - len(g)
+        v = rng.standard_normal((m, d))
+        v *= (rng.random((m, 1)) ** (1.0 / d)) / np.linalg.norm(v, axis=1)[:, None]
+        return self.from_u(np.concatenate([g, v]))
+
+
 class Surrogate:
-    """Parameters -> log(flux / wdwarf) at nbins phase bins, trained over
-    ``box``; its log-likelihood is -inf outside the box.
+    """Parameters -> log(flux / wdwarf) at its output points, trained over
+    a ``Frame``; its log-likelihood is -inf outside the frame's domain.
 
-    The curve is a quadratic polynomial in the box coordinates, fitted by
-    FP64 least squares on the host, plus the MLP's correction, which is
-    trained on the normalised residual. The network thus carries only what
-    the polynomial misses, so its BF16 precision applies to a much smaller
-    quantity."""
+    The outputs are phase bins (``phases``, interpolated to any phase by
+    ``curve``) or a target's own times (``times``: ``curve`` then returns
+    exactly those points, exposure smearing included when the training
+    curves had it). The curve is a polynomial of ``degree`` in the frame
+    coordinates, fitted by FP64 least squares on the host, plus the MLP's
+    correction, which is trained on the normalised residual. The network
+    thus carries only what the polynomial misses, so its BF16 precision
+    applies to a much smaller quantity. ``box`` gives a box frame when no
+    ``frame`` is passed."""
 
-    def __init__(self, nbins=256, hidden=(512, 512, 512, 512), seed=1, max_batch=4096, box=BOX, phases=None):
-        self.box = np.array(box, dtype=np.float64)
-        self.phases = bin_phases(nbins) if phases is None else np.sort(np.asarray(phases, dtype=np.float64))
-        nbins = len(self.phases)
+    def __init__(self, nbins=256, hidden=(512, 512, 512, 512), seed=1, max_batch=4096, box=BOX, phases=None,
+                 frame=None, times=None, degree=2, tail=False):
+        self.frame = Frame.from_box(box) if frame is None else frame
+        if tail and self.frame.kind != "ball":
+            raise ValueError("a Gaussian tail needs a ball frame")
+        # Outside a ball frame's domain the log-likelihood is -inf, or with
+        # ``tail`` the Gaussian of the frame's covariance: the value at the
+        # centre minus half the squared Mahalanobis distance. A positive
+        # surrogate density everywhere keeps delayed acceptance exact on
+        # the whole prior, not only inside the ball.
+        self.tail = bool(tail)
+        self.degree = int(degree)
+        self.times = None if times is None else np.array(times, dtype=np.float64)
+        if self.times is not None:
+            self.phases = None
+            nbins = len(self.times)
+        else:
+            self.phases = bin_phases(nbins) if phases is None else np.sort(np.asarray(phases, dtype=np.float64))
+            nbins = len(self.phases)
         self.nbins = nbins
-        self.net = Mlp((len(PARAM_NAMES),) + tuple(hidden) + (nbins,), seed=seed, max_batch=max_batch)
-        self.coef = np.zeros((1 + 5 + 15, self.nbins))
+        # The MLP's layer widths are multiples of 64; extra outputs train to 0.
+        self.nout = -(-nbins // 64) * 64
+        nd = len(self.frame.center)
+        self.net = Mlp((nd,) + tuple(hidden) + (self.nout,), seed=seed, max_batch=max_batch)
+        self.coef = np.zeros((poly_features(np.zeros((1, nd)), self.degree).shape[1], self.nbins))
         self.y_std = 1.0
+        self.version = 0  # bumped by every fit
+
+    @property
+    def box(self):
+        """The parameter box of a box frame."""
+        if self.frame.kind != "box":
+            raise AttributeError("a ball frame has no box")
+        d = len(self.frame.center)
+        return np.stack([self.frame.from_u(-np.ones(d))[0], self.frame.from_u(np.ones(d))[0]], axis=1)
+
+    def contains(self, params):
+        return self.frame.contains(params)
 
     def fit(self, params, logcurves, epochs=60, batch=4096, lr=1e-3, lr_min=1e-5, seed=0, log=None):
         ok = np.all(np.isfinite(logcurves), axis=1)
-        u = to_unit(params[ok], self.box)
+        u = self.frame.to_u(params[ok])
         y = logcurves[ok]
-        F = quad_features(u)
+        F = poly_features(u, self.degree)
+        self.version += 1
         self.coef = np.linalg.lstsq(F, y, rcond=None)[0]
         r = y - F @ self.coef
         self.y_std = float(r.std()) or 1.0
-        self.net.set_data(u.astype(np.float32), (r / self.y_std).astype(np.float32))
+        t = np.zeros((len(u), self.nout), dtype=np.float32)
+        t[:, :self.nbins] = r / self.y_std
+        self.net.set_data(u.astype(np.float32), t)
         rng = np.random.default_rng(seed)
         hist = []
         t0 = time.perf_counter()
@@ -163,16 +276,34 @@ class Surrogate:
                 log(f"epoch {e}: loss {hist[-1]:.3e}")
         return {"loss": hist, "seconds": time.perf_counter() - t0, "n_train": int(ok.sum())}
 
+    def likelihood(self, y, ferr, w):
+        """The autoscaled log-likelihood of observed fluxes y (at this
+        surrogate's times) on the GPU: a callable params [n, d] -> [n]. It
+        ignores the domain; callers mask rows outside it."""
+        if self.times is None:
+            raise ValueError("the GPU likelihood needs a surrogate of the data times")
+        return SurrogateLikelihood(self.net, self.frame.center, self.frame.M, self.degree, self.coef, self.y_std,
+                                   y, ferr, w)
+
+    def baseline(self, params):
+        """The polynomial alone [n, nbins]."""
+        return poly_features(self.frame.to_u(params), self.degree) @ self.coef
+
     def predict(self, params):
         """log(flux / wdwarf) [n, nbins]."""
-        u = to_unit(params, self.box)
-        y = self.net.predict(u.astype(np.float32)).astype(np.float64)
-        return quad_features(u) @ self.coef + y * self.y_std
+        u = self.frame.to_u(params)
+        y = self.net.predict(u.astype(np.float32))[:, :self.nbins].astype(np.float64)
+        return poly_features(u, self.degree) @ self.coef + y * self.y_std
 
     def curve(self, params, phases):
-        """flux / wdwarf at arbitrary phases (periodic linear interpolation
-        in log flux between the output phases) [n, len(phases)]."""
+        """flux / wdwarf [n, len(phases)]: at the output times themselves
+        (a time-output surrogate only serves its own times), or by periodic
+        linear interpolation in log flux between the output phases."""
         lg = self.predict(params)
+        if self.times is not None:
+            if not np.array_equal(np.asarray(phases, dtype=np.float64), self.times):
+                raise ValueError("this surrogate predicts its training times only")
+            return np.exp(lg)
         n = self.nbins
         ext = np.concatenate([[self.phases[-1] - 1.0], self.phases, [self.phases[0] + 1.0]])
         x = np.mod(np.asarray(phases, dtype=np.float64), 1.0)
@@ -209,6 +340,7 @@ class Target:
         self.w = np.ones_like(f)
         self.n_exact = 0
         self.t_exact = 0.0
+        self._lik = None  # (surrogate, its version, GPU likelihood)
 
     def logp_exact(self, params):
         params = np.atleast_2d(params)
@@ -227,19 +359,52 @@ class Target:
         return out
 
     def logp_surrogate(self, surr, params):
+        """Surrogate log-likelihood: -inf outside its domain, or its
+        Gaussian tail there (Surrogate ``tail``). A surrogate of this
+        target's own times is evaluated on the GPU in one call
+        (Surrogate.likelihood); others through their interpolated curve."""
+        return self._logp_surrogate(surr, params, gpu=True)
+
+    def logp_surrogate_host(self, surr, params):
+        """The same on the host from the surrogate's curve (reference for
+        the GPU likelihood)."""
+        return self._logp_surrogate(surr, params, gpu=False)
+
+    def _logp_surrogate(self, surr, params, gpu):
         params = np.atleast_2d(params)
         out = np.full(len(params), -np.inf)
-        ins = in_box(params, surr.box)
+        ins = surr.contains(params)
+        own_times = getattr(surr, "times", None) is not None and np.array_equal(surr.times, self.times)
+
+        def inner(p):
+            if gpu and own_times:
+                if self._lik is None or self._lik[0] is not surr or self._lik[1] != surr.version:
+                    c = surr.frame.center[None]
+                    lik = surr.likelihood(self.y, self.ferr, self.w)
+                    self._lik = (surr, surr.version, lik, float(lik(c)[0]))
+                return self._lik[2](p)
+            return -0.5 * autoscaled_chisq(surr.curve(p, self.times), self.y, self.ferr, self.w)
+
         if ins.any():
-            out[ins] = -0.5 * autoscaled_chisq(surr.curve(params[ins], self.times), self.y, self.ferr, self.w)
+            out[ins] = inner(params[ins])
+        if getattr(surr, "tail", False) and not ins.all():
+            if gpu and own_times:
+                inner(params[:0])  # make sure the centre value is cached
+                l0 = self._lik[3]
+            else:
+                l0 = float(inner(surr.frame.center[None])[0])
+            u = surr.frame.to_u(params[~ins])
+            out[~ins] = l0 - 0.5 * surr.frame.radius ** 2 * np.einsum("ij,ij->i", u, u)
         return out
 
 
 def ensemble_mcmc(target, p0, nsteps, seed=0, surrogate=None, a=2.0, surr_tol=25.0, inner_steps=1,
-                  box=BOX):
+                  box=BOX, log=None, log_every=50):
     """Stretch-move ensemble; delayed acceptance / surrogate transition when
     ``surrogate`` is given. Returns dict with chain [nsteps, nwalkers,
-    ndim], acceptance counts, exact-evaluation count and wall time."""
+    ndim], acceptance counts, exact-evaluation count and wall time.
+    ``log`` (a callable) receives a progress line every ``log_every``
+    steps."""
     rng = np.random.default_rng(seed)
     x = np.array(p0, dtype=np.float64)
     nw, nd = x.shape
@@ -302,6 +467,9 @@ def ensemble_mcmc(target, p0, nsteps, seed=0, surrogate=None, a=2.0, surr_tol=25
         if use_surr and np.all(np.isfinite(lp)) and np.std(ls - lp) > surr_tol:
             use_surr = False
             fallback_at = step
+        if log and (step + 1) % log_every == 0:
+            log(f"step {step + 1}/{nsteps}: acceptance {acc / ((step + 1) * nw):.3f}, "
+                f"exact evaluations {target.n_exact - n_exact0}, {time.perf_counter() - t0:.1f} s")
     return {"chain": chain, "accepted": acc, "proposals": nsteps * nw, "stage1_passed": stage1,
             "fallback_at": fallback_at, "n_exact": target.n_exact - n_exact0,
             "seconds": time.perf_counter() - t0}

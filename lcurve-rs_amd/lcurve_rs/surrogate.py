@@ -14,10 +14,15 @@ from . import _ctx, _native
 from ._native import check
 
 _F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
 
 
 def _fp(a):
     return a.ctypes.data_as(_F)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
 
 
 def _f32(a, shape=None):
@@ -96,3 +101,53 @@ class Mlp:
         check(_native.lib().lch_mlp_time_step(self._h, int(batch), int(reps), ctypes.byref(a), ctypes.byref(b),
                                               ctypes.byref(f)))
         return {"ms_step": a.value, "ms_gemm": b.value, "gemm_flop": f.value}
+
+
+def n_poly_features(ndim, degree):
+    """Columns of the polynomial baseline: 1 and every monomial up to
+    ``degree`` (the engine's count)."""
+    n = _native.lib().lch_surr_lik_n_features(int(ndim), int(degree))
+    if n < 0:
+        raise ValueError("bad ndim or degree")
+    return n
+
+
+class SurrogateLikelihood:
+    """The autoscaled chi-squared log-likelihood of a time-output surrogate
+    against one observed curve, evaluated on the GPU (lch_surr_lik_*):
+    curve = exp(poly(u) . coef + y_std * net(u)), u = (p - center) . M.
+    Keeps the network alive; retraining the network changes it."""
+
+    def __init__(self, net, center, M, degree, coef, y_std, y, ferr, w):
+        center = np.ascontiguousarray(center, dtype=np.float64)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        ferr = np.ascontiguousarray(ferr, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        nd = len(center)
+        nb = len(y)
+        if M.shape != (nd, nd) or coef.shape != (n_poly_features(nd, degree), nb) or ferr.shape != (nb,) \
+                or w.shape != (nb,):
+            raise ValueError("inconsistent shapes")
+        self.net = net
+        self.ndim = nd
+        h = ctypes.c_void_p()
+        check(_native.lib().lch_surr_lik_create(net._h, nd, _dp(center), _dp(M), int(degree), _dp(coef), nb,
+                                                ctypes.c_double(y_std), _dp(y), _dp(ferr), _dp(w), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _native.lib().lch_surr_lik_free(h)
+            self._h = None
+
+    def __call__(self, params):
+        p = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        if p.shape[1] != self.ndim:
+            raise ValueError("parameter rows have the wrong width")
+        out = np.zeros(len(p))
+        if len(p):
+            check(_native.lib().lch_surr_lik_eval(self._h, _dp(p), len(p), _dp(out)))
+        return out

@@ -132,3 +132,20 @@ def test_compute_without_device_fails_loudly():
     m = lc.Model(os.path.join(GOLDEN, "test_model.dat"))
     with pytest.raises(_native.LcurveError):
         m.light_curve(ntime=10)
+
+
+def test_surrogate_likelihood_feature_count():
+    """The GPU surrogate likelihood's polynomial has 1 + every monomial of
+    degree 1..d in n variables: C(n + d, d) columns, as
+    lcurve_rs.mcmc.poly_features builds them."""
+    from math import comb
+    from lcurve_rs import mcmc
+    from lcurve_rs.surrogate import n_poly_features
+    for nd in (1, 3, 5):
+        for deg in range(5):
+            assert n_poly_features(nd, deg) == comb(nd + deg, deg)
+            assert n_poly_features(nd, deg) == mcmc.poly_features(np.zeros((1, nd)), deg).shape[1]
+    with pytest.raises(ValueError):
+        n_poly_features(0, 2)
+    with pytest.raises(ValueError):
+        n_poly_features(5, 7)

@@ -58,3 +58,74 @@ def test_delayed_acceptance_runs_on_the_exact_likelihood(mc):
     assert r["n_exact"] <= 16 + 20 * 16
     assert np.all(mc.in_box(r["chain"].reshape(-1, 5), box))
     assert 0 < r["accepted"] <= r["proposals"]
+
+
+def test_time_output_surrogate_in_a_local_frame(lcurve_rs, mc):
+    """A surrogate of a smeared target's own curve: training curves at the
+    data times with exposure smearing equal single light curves, the cubic
+    baseline in whitened coordinates plus the network beats the baseline
+    alone on held-out curves, and the surrogate's likelihood is finite
+    inside its ball and -inf outside."""
+    base = mc.surrogate_base_text(NLAT)
+    truth = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
+    times = np.linspace(-0.2, 1.2, 300)
+    expose, ndiv = 0.002, 3
+    rng = np.random.default_rng(4)
+    sd = np.array([0.004, 0.2, 0.0002, 150.0, 15.0])
+    corr = np.eye(5)
+    corr[0, 1] = corr[1, 0] = -0.8
+    pilot = rng.multivariate_normal(truth, corr * np.outer(sd, sd), size=400)
+    frame = mc.Frame.local(pilot, radius=5.0)
+    p = frame.sample(2500, 6)
+    lg = mc.make_training_set(base, p, times=times, expose=expose, ndiv=ndiv)
+    m = lcurve_rs.Model.from_text(W.model_text(dict(zip(mc.PARAM_NAMES, p[0])), base))
+    r = m.light_curve(times=times, expose=expose, ndiv=ndiv)
+    assert np.allclose(lg[0], np.log(r.calc / r.wdwarf), rtol=0, atol=1e-12)
+    s = mc.Surrogate(hidden=(256, 256), seed=2, frame=frame, times=times, degree=3)
+    assert s.nout == 320 and s.nbins == 300
+    s.fit(p[400:], lg[400:], epochs=150, batch=512)
+    err = s.predict(p[:400]) - lg[:400]
+    poly = s.baseline(p[:400]) - lg[:400]
+    assert np.sqrt(np.mean(err ** 2)) < 1e-3
+    assert np.sqrt(np.mean(err ** 2)) <= np.sqrt(np.mean(poly ** 2)) * 1.05
+    tg = mc.Target(base, truth, times, 0.02, seed=1, expose=expose, ndiv=ndiv)
+    far = truth + 20.0 * sd
+    ls = tg.logp_surrogate(s, np.stack([truth, far]))
+    assert np.isfinite(ls[0]) and ls[1] == -np.inf
+    le = tg.logp_exact(truth[None])
+    assert abs(ls[0] - le[0]) < 5.0
+    p0 = pilot[:16]
+    res = mc.ensemble_mcmc(tg, p0, 10, seed=1, surrogate=s, inner_steps=4, surr_tol=1e9)
+    assert np.all(frame.contains(res["chain"].reshape(-1, 5)))
+    assert 0 < res["accepted"] <= res["proposals"]
+
+
+def test_gpu_surrogate_likelihood_matches_host(mc):
+    """The one-call GPU likelihood of a time-output surrogate (baseline +
+    network + exp + autoscaled chi-squared on the device) equals the host's
+    evaluation of the same surrogate, over more rows than one device chunk,
+    with a row outside the domain, and follows a refit."""
+    base = mc.surrogate_base_text(NLAT)
+    truth = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
+    sd = np.array([0.004, 0.2, 0.0002, 150.0, 15.0])
+    times = np.linspace(-0.2, 1.2, 500)
+    expose, ndiv = 0.002, 3
+    rng = np.random.default_rng(5)
+    frame = mc.Frame.local(truth + rng.standard_normal((300, 5)) * sd, radius=5.0)
+    p = frame.sample(640, 8)
+    lg = mc.make_training_set(base, p, times=times, expose=expose, ndiv=ndiv)
+    s = mc.Surrogate(hidden=(128,), seed=4, frame=frame, times=times, degree=3)
+    assert s.nout == 512
+    s.fit(p, lg, epochs=5, batch=128)
+    tg = mc.Target(base, truth, times, 0.01, seed=2, expose=expose, ndiv=ndiv)
+    q = np.concatenate([frame.sample(299, 9), (truth + 30.0 * sd)[None]])
+    g = tg.logp_surrogate(s, q)
+    h = tg.logp_surrogate_host(s, q)
+    assert np.isinf(g[-1]) and np.isinf(h[-1])
+    assert np.all(np.isfinite(g[:-1])) and np.all(np.isfinite(h[:-1]))
+    assert np.allclose(g[:-1], h[:-1], rtol=1e-9, atol=1e-7)
+    assert np.all(np.isinf(tg.logp_surrogate(s, q[-1:])))
+    s.fit(p, lg, epochs=1, batch=128)
+    g2, h2 = tg.logp_surrogate(s, q[:5]), tg.logp_surrogate_host(s, q[:5])
+    assert not np.allclose(g2, g[:5], rtol=1e-12, atol=0)
+    assert np.allclose(g2, h2, rtol=1e-9, atol=1e-7)

@@ -120,6 +120,7 @@ def test_box_mapping_round_trip():
 def test_surrogate_curve_interpolates_periodically():
     s = mcmc.Surrogate.__new__(mcmc.Surrogate)
     s.nbins = 8
+    s.times = None
     s.phases = mcmc.bin_phases(8)
     lg = np.log(np.arange(1, 9, dtype=float))[None, :]
     s.predict = lambda p: lg
@@ -138,3 +139,134 @@ def test_eclipse_phase_grid():
     assert len(ph) == 256 and np.all(np.diff(ph) > 0) and ph[0] >= 0 and ph[-1] < 1
     near = np.minimum(ph, 1 - ph) < 0.06
     assert near.sum() == 192
+
+
+def test_time_output_surrogate_serves_its_own_times_only():
+    s = mcmc.Surrogate.__new__(mcmc.Surrogate)
+    s.times = np.array([-0.1, 0.0, 0.3, 1.1])
+    s.nbins = 4
+    lg = np.log(np.array([[1.0, 2.0, 3.0, 4.0]]))
+    s.predict = lambda p: lg
+    assert np.allclose(s.curve(np.zeros((1, 5)), s.times.copy()), [[1.0, 2.0, 3.0, 4.0]])
+    with pytest.raises(ValueError):
+        s.curve(np.zeros((1, 5)), s.times[:3])
+
+
+def test_box_frame_is_the_unit_box():
+    f = mcmc.Frame.from_box(mcmc.BOX)
+    p = mcmc.sample_box(200, 4)
+    assert np.allclose(f.to_u(p), mcmc.to_unit(p), rtol=0, atol=1e-12)
+    assert np.allclose(f.from_u(f.to_u(p)), p)
+    assert np.all(f.contains(p))
+    out = p.copy()
+    out[:, 1] = 95.0
+    assert not np.any(f.contains(out))
+    q = f.sample(300, 5)
+    assert np.all(mcmc.in_box(q))
+
+
+def test_local_frame_whitens_correlated_samples():
+    rng = np.random.default_rng(9)
+    x = rng.multivariate_normal(CENTER, COV, size=20000)
+    f = mcmc.Frame.local(x, radius=5.0)
+    u = f.to_u(x)
+    # Unit covariance in units of the radius, centred on the samples' mean.
+    assert np.allclose(np.cov(u, rowvar=False) * 25.0, np.eye(5), atol=1e-10)
+    assert np.allclose(u.mean(0), 0.0, atol=1e-12)
+    assert np.allclose(f.center, x.mean(0))
+    # The ball holds all but the far tail: P(chi2_5 > 25) = 1.4e-4.
+    assert 0.9995 < f.contains(x).mean() <= 1.0
+    # Mahalanobis distance under the true covariance tracks |u| * radius.
+    d_true = np.sqrt(np.einsum("ij,jk,ik->i", x - CENTER, PREC, x - CENTER))
+    assert np.corrcoef(d_true, 5.0 * np.linalg.norm(u, axis=1))[0, 1] > 0.999
+
+
+def test_ball_frame_sampling():
+    rng = np.random.default_rng(10)
+    f = mcmc.Frame.local(rng.multivariate_normal(CENTER, COV, size=5000), radius=6.0)
+    p = f.sample(4001, 11)
+    assert p.shape == (4001, 5)
+    r = np.linalg.norm(f.to_u(p), axis=1)
+    assert np.all(r <= 1.0 + 1e-12)
+    # Half from the core (a normal of sd 1/3 of the radius cut to the ball:
+    # 9 r^2 is chi-squared with 5 degrees of freedom), the rest uniform in
+    # the ball (P(r < x) = x^5).
+    from scipy.stats import chi2
+    cut = chi2.cdf(9.0, 5)
+    for x in (0.3, 0.5, 0.7, 0.9):
+        expect = 0.5 * chi2.cdf(9.0 * x * x, 5) / cut + 0.5 * x ** 5
+        assert abs(np.mean(r < x) - expect) < 0.03, (x, np.mean(r < x), expect)
+    assert np.array_equal(p, f.sample(4001, 11))
+
+
+def test_polynomial_features():
+    u = np.random.default_rng(12).uniform(-1, 1, (7, 5))
+    from math import comb
+    for deg in range(5):
+        assert mcmc.poly_features(u, deg).shape == (7, comb(5 + deg, deg))
+    assert np.array_equal(mcmc.poly_features(u, 2), mcmc.quad_features(u))
+    F3 = mcmc.poly_features(u, 3)
+    # Columns: 1, u0..u4, then u_i u_j (i <= j), then u_i u_j u_k.
+    assert np.allclose(F3[:, 0], 1.0)
+    assert np.allclose(F3[:, 1:6], u)
+    assert np.allclose(F3[:, 6], u[:, 0] ** 2)
+    assert np.allclose(F3[:, 21], u[:, 0] ** 3)
+    assert np.allclose(F3[:, -1], u[:, 4] ** 3)
+    # A cubic is fitted exactly.
+    x = np.random.default_rng(13).uniform(-1, 1, (200, 5))
+    y = 1.0 + x[:, 0] * x[:, 1] * x[:, 2] - 2.0 * x[:, 4] ** 3 + 0.5 * x[:, 3]
+    c = np.linalg.lstsq(mcmc.poly_features(x, 3), y, rcond=None)[0]
+    assert np.allclose(mcmc.poly_features(u, 3) @ c, 1.0 + u[:, 0] * u[:, 1] * u[:, 2] - 2.0 * u[:, 4] ** 3
+                       + 0.5 * u[:, 3])
+
+
+class _FakeTarget(mcmc.Target):
+    """A Target with observed data but no exact model (no GPU needed)."""
+
+    def __init__(self, times, y, ferr):
+        self.times, self.y, self.ferr, self.w = times, y, ferr, np.ones_like(y)
+        self._lik = None
+
+
+class _LinearSurrogate:
+    """flux = exp(0.01 * u . a) at every time: a stand-in with a frame."""
+    times = None
+    version = 0
+
+    def __init__(self, frame, tail):
+        self.frame, self.tail = frame, tail
+        self.a = np.linspace(0.5, 1.5, 5)
+
+    def contains(self, p):
+        return self.frame.contains(p)
+
+    def curve(self, p, t):
+        u = self.frame.to_u(p)
+        return np.exp(0.01 * (u @ self.a)[:, None] * np.linspace(-1.0, 1.0, len(t))[None, :])
+
+
+def test_surrogate_gaussian_tail_outside_the_ball():
+    rng = np.random.default_rng(14)
+    pilot = rng.multivariate_normal(CENTER, COV, size=4000)
+    frame = mcmc.Frame.local(pilot, radius=3.0)
+    t = np.linspace(0.0, 1.0, 40)
+    y = 1.0 + 0.01 * rng.standard_normal(40)
+    tg = _FakeTarget(t, y, 0.01 * np.ones(40))
+    p = np.concatenate([frame.sample(50, 15), rng.multivariate_normal(CENTER, 25.0 * COV, size=50)])
+    ins = frame.contains(p)
+    assert 10 < ins.sum() < 100
+    expect_in = -0.5 * mcmc.autoscaled_chisq(_LinearSurrogate(frame, False).curve(p[ins], t), y, tg.ferr, tg.w)
+    plain = tg.logp_surrogate(_LinearSurrogate(frame, False), p)
+    assert np.allclose(plain[ins], expect_in, rtol=1e-13) and np.all(plain[~ins] == -np.inf)
+    s = _LinearSurrogate(frame, True)
+    tailed = tg.logp_surrogate(s, p)
+    assert np.array_equal(tailed[ins], plain[ins])
+    l0 = -0.5 * mcmc.autoscaled_chisq(s.curve(frame.center[None], t), y, tg.ferr, tg.w)[0]
+    # Squared Mahalanobis distance under the pilot samples' covariance.
+    cov = np.cov(pilot, rowvar=False)
+    d2 = np.einsum("ij,jk,ik->i", p[~ins] - frame.center, np.linalg.inv(cov), p[~ins] - frame.center)
+    assert np.allclose(tailed[~ins], l0 - 0.5 * d2, rtol=1e-12)
+    # Past the ball the tail is below its value at the boundary.
+    assert np.all(tailed[~ins] < l0 - 0.5 * 9.0 + 1e-9)
+    with pytest.raises(ValueError):
+        mcmc.Surrogate.__init__(mcmc.Surrogate.__new__(mcmc.Surrogate), tail=True)
