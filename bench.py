@@ -399,7 +399,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=None, help="surrogate training epochs")
     ap.add_argument("--walkers", type=int, default=64)
     ap.add_argument("--pilot-steps", type=int, default=200)
-    ap.add_argument("--inner", default="25",
+    ap.add_argument("--inner", default="100,50,25",
                     help="surrogate moves per exact check (da_mcmc); a comma list runs one sampler per value "
                          "after the same plain run, the first giving the reported value")
     ap.add_argument("--radius", type=float, default=6.5,
