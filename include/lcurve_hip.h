@@ -100,6 +100,9 @@ void lch_model_free(lch_model* m);
 /* Physical parameters by name; LCH_ERR_INVALID for an unknown name. */
 int lch_model_set_param(lch_model* m, const char* name, double value);
 int lch_model_get_param(const lch_model* m, const char* name, double* value);
+/* The model's `iscale` setting (model.rs:181, 440): 1 if the light curve is
+ * fitted per component (lroche prints five scale factors then). */
+int lch_model_iscale(const lch_model* m, int* value);
 int lch_model_get_pparam(const lch_model* m, const char* name, double* value, double* range,
                          double* dstep, int* vary, int* defined);
 /* Null pointers leave the field unchanged. */

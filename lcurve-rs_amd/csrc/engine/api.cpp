@@ -197,6 +197,12 @@ int lch_model_get_param(const lch_model* m, const char* name, double* value) {
   return LCH_OK;
 }
 
+int lch_model_iscale(const lch_model* m, int* value) {
+  if (!m || !value) return fail(LCH_ERR_INVALID, "null argument");
+  *value = m->impl.iscale ? 1 : 0;
+  return LCH_OK;
+}
+
 int lch_model_get_pparam(const lch_model* m, const char* name, double* value, double* range,
                          double* dstep, int* vary, int* defined) {
   if (!m || !name) return fail(LCH_ERR_INVALID, "null argument");

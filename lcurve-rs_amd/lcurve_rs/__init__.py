@@ -298,6 +298,13 @@ class Model:
     def set_param(self, name, value):
         check(_native.lib().lch_model_set_param(self._h, name.encode(), float(value)))
 
+    @property
+    def iscale(self):
+        """Whether the curve is fitted per component (model.rs:181)."""
+        v = ctypes.c_int()
+        check(_native.lib().lch_model_iscale(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
     def get_pparam(self, name):
         v, r, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         vary, defined = ctypes.c_int(), ctypes.c_int()

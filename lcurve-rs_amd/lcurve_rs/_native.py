@@ -97,6 +97,7 @@ SIGNATURES = {
     "lch_mlp_predict": (INT, [VP, F, INT, F]),
     "lch_mlp_time_step": (INT, [VP, INT, INT, D, D, D]),
     "lch_surr_lik_n_features": (INT, [INT, INT]),
+    "lch_model_iscale": (INT, [VP, I]),
     "lch_surr_lik_create": (INT, [VP, INT, D, D, INT, D, INT, ctypes.c_double, D, D, D, ctypes.POINTER(VP)]),
     "lch_surr_lik_eval": (INT, [VP, D, INT, D]),
     "lch_surr_lik_free": (None, [VP]),
