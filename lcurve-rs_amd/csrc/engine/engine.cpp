@@ -195,6 +195,9 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
   }
 
   d_.n_sets = static_cast<uint32_t>(ns);
+  d_.max_set_lanes = 0;
+  for (const SetDev& sd : sets_)
+    if (sd.flags & SET_VALID) d_.max_set_lanes = std::max(d_.max_set_lanes, sd.special0);
   d_.recs = d_recs;
   upload();
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");

@@ -884,33 +884,15 @@ __device__ void iscale_fit(const SetDev& sp, const DeviceBatch& d, int ncol, dou
 
 }  // namespace
 
-__global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, int scale) {
-  // FP64 assembly without contraction, in the reference's operation order.
+// Calibration and white-dwarf lanes of a set: their raw component sums in
+// special[] and the Ginterp factors s11, s12, s21, s22 in scl[]. Every
+// thread of the workgroup calls it (it synchronises the workgroup).
+__device__ void set_scales(const SetDev& sp, const DeviceBatch& d, double* special, double* scl) {
 #pragma clang fp contract(off)
-  __shared__ double special[9];
-  __shared__ double scl[4];
-  __shared__ double red[NWAVE * 6];
-  __shared__ double sf_sh[5];
-  const SetDev sp = d.sets[blockIdx.x];
   const uint32_t tid = threadIdx.x;
   const double* lane_phi = d.lane_phi;
-  const uint32_t* t_lane0 = d.t_lane0;
-  const int* t_ndiv = d.t_ndiv;
-  const double* t_slfac = d.t_slfac;
-  const double* t_y = d.t_y;
-  const double* t_ferr = d.t_ferr;
-  const double* t_w = d.t_w;
-  double* flux_out = d.flux;
-  if (!(sp.flags & SET_VALID)) {
-    const double nan = __builtin_nan("");
-    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) flux_out[sp.time_off + t] = nan;
-    if (tid == 0) d.out[blockIdx.x] = SetOut{nan, nan, nan, nan, {nan, nan, nan, nan, nan}};
-    return;
-  }
   const bool cal1 = (sp.flags & SET_CAL1) != 0, cal2 = (sp.flags & SET_CAL2) != 0;
   const uint32_t n_special = (cal1 ? 4u : 0u) + (cal2 ? 4u : 0u) + 1u;
-
-  // 1. Calibration and white-dwarf lanes: raw component sums.
   if (tid < n_special) {
     double S[5];
     lane_sums(sp, sp.special0 + tid, d, S);
@@ -946,6 +928,69 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
     scl[0] = s11; scl[1] = s12; scl[2] = s21; scl[3] = s22;
   }
   __syncthreads();
+}
+
+// Per-lane sums of the partial rows, in parallel over (set, block of
+// ABLOCK lanes): a sub-phase's total with the Ginterp factors applied
+// (flux.rs:74-107), or for iscale sets its five raw component sums. Each
+// workgroup recomputes its set's factors from the special lanes. The
+// per-set assembly that follows reads these.
+__global__ __launch_bounds__(ABLOCK) void lane_kernel(const DeviceBatch d) {
+#pragma clang fp contract(off)
+  __shared__ double special[9];
+  __shared__ double scl[4];
+  const SetDev sp = d.sets[blockIdx.x];
+  // Both exits are uniform over the workgroup (before any barrier).
+  if (!(sp.flags & SET_VALID) || blockIdx.y * ABLOCK >= sp.special0) return;
+  const uint32_t l = blockIdx.y * ABLOCK + threadIdx.x;
+  if (sp.flags & SET_ISCALE) {
+    if (l >= sp.special0) return;
+    double S[5];
+    lane_sums(sp, l, d, S);
+    double* o = d.lane_comp + static_cast<uint64_t>(sp.lane_off + l) * 5;
+    for (int k = 0; k < 5; ++k) o[k] = S[k];
+    return;
+  }
+  set_scales(sp, d, special, scl);
+  if (l >= sp.special0) return;
+  const double s11 = scl[0], s12 = scl[1], s21 = scl[2], s22 = scl[3];
+  double S[5];
+  lane_sums(sp, l, d, S);
+  const double phi = d.lane_phi[sp.lane_off + l];
+  const double ssum = S[0] * gscale1(phi, sp.phase1, s11, s12) +
+                      gscale2(phi, sp.phase2, s21, s22) * S[1] + (S[2] + S[3] + S[4]);
+  d.lane_tot[sp.lane_off + l] = ssum;
+}
+
+__global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, int scale) {
+  // FP64 assembly without contraction, in the reference's operation order.
+#pragma clang fp contract(off)
+  __shared__ double special[9];
+  __shared__ double scl[4];
+  __shared__ double red[NWAVE * 6];
+  __shared__ double sf_sh[5];
+  const SetDev sp = d.sets[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  const double* lane_phi = d.lane_phi;
+  const uint32_t* t_lane0 = d.t_lane0;
+  const int* t_ndiv = d.t_ndiv;
+  const double* t_slfac = d.t_slfac;
+  const double* t_y = d.t_y;
+  const double* t_ferr = d.t_ferr;
+  const double* t_w = d.t_w;
+  double* flux_out = d.flux;
+  if (!(sp.flags & SET_VALID)) {
+    const double nan = __builtin_nan("");
+    for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) flux_out[sp.time_off + t] = nan;
+    if (tid == 0) d.out[blockIdx.x] = SetOut{nan, nan, nan, nan, {nan, nan, nan, nan, nan}};
+    return;
+  }
+  const bool cal1 = (sp.flags & SET_CAL1) != 0, cal2 = (sp.flags & SET_CAL2) != 0;
+  const uint32_t n_special = (cal1 ? 4u : 0u) + (cal2 ? 4u : 0u) + 1u;
+
+  // 1. Calibration and white-dwarf lanes: the Ginterp factors. The other
+  //    lanes' sums come from lane_kernel, launched before in the stream.
+  set_scales(sp, d, special, scl);
   const double s11 = scl[0], s12 = scl[1], s21 = scl[2], s22 = scl[3];
   double wdwarf = (1.0 * gscale1(0.5, sp.phase1, s11, s12)) * special[n_special - 1u] / 1.0;
 
@@ -954,13 +999,6 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
     // scale1(phase) (flux.rs:151), disc, edge, spot, comp_star2 with
     // scale2(phi) (flux.rs:197), each times slfac, no third light.
     const int ncol = (sp.flags & SET_NCOL5) ? 5 : 4;
-    for (uint32_t l = tid; l < sp.special0; l += ABLOCK) {
-      double S[5];
-      lane_sums(sp, l, d, S);
-      double* o = d.lane_comp + static_cast<uint64_t>(sp.lane_off + l) * 5;
-      for (int k = 0; k < 5; ++k) o[k] = S[k];
-    }
-    __syncthreads();
     for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
       const uint32_t g = sp.time_off + t;
       const uint32_t l0 = t_lane0[g];
@@ -1008,17 +1046,7 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
     return;
   }
 
-  // 2. Sub-phase totals (flux.rs:74-107 with scale factors).
-  for (uint32_t l = tid; l < sp.special0; l += ABLOCK) {
-    double S[5];
-    lane_sums(sp, l, d, S);
-    const double phi = lane_phi[sp.lane_off + l];
-    const double ssum = S[0] * gscale1(phi, sp.phase1, s11, s12) +
-                        gscale2(phi, sp.phase2, s21, s22) * S[1] + (S[2] + S[3] + S[4]);
-    d.lane_tot[sp.lane_off + l] = ssum;
-  }
-  __syncthreads();
-
+  // 2. Sub-phase totals: lane_tot, from lane_kernel.
   // 3. Exposure smearing and the per-time factors (flux.rs:24-111,
   //    orchestration.rs:243-251).
   for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) {
@@ -1136,6 +1164,13 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
 
 hipError_t launch_assemble(const DeviceBatch& d, int scale, hipStream_t stream) {
   if (d.n_sets == 0) return hipSuccess;
+  const uint32_t gy = (d.max_set_lanes + ABLOCK - 1) / ABLOCK;
+  if (gy > 65535u) return hipErrorInvalidValue;
+  if (gy > 0) {
+    hipLaunchKernelGGL(lane_kernel, dim3(d.n_sets, gy), dim3(ABLOCK), 0, stream, d);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(assemble_kernel, dim3(d.n_sets), dim3(ABLOCK), 0, stream, d, scale);
   return hipGetLastError();
 }

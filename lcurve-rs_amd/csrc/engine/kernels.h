@@ -9,6 +9,7 @@ namespace lch {
 
 struct DeviceBatch {
   uint32_t n_sets = 0, n_items = 0;
+  uint32_t max_set_lanes = 0;  // largest SetDev::special0 of a valid set
   uint32_t n_items_plain = 0;  // leading items with no beaming / lensing
   const SetDev* sets = nullptr;
   const SegDev* segs = nullptr;
@@ -52,7 +53,9 @@ struct DeviceBatch {
 // limb chunks, which add to the partial rows the items wrote.
 hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
                              uint32_t grab, hipStream_t stream);
-// FP64 assembly, scale factors, smearing, autoscale and chi-squared.
+// FP64 assembly: lane_kernel (per-lane sums, in parallel over sets and
+// lane blocks), then assemble_kernel (per set: scale factors, smearing,
+// autoscale and chi-squared).
 hipError_t launch_assemble(const DeviceBatch& d, int scale, hipStream_t stream);
 // Eclipse visibility of one segment at `nphase` phases (debug / parity).
 hipError_t launch_visibility(const DeviceBatch& d, const SegDev& sg, const double* phases,
