@@ -547,7 +547,16 @@ void Batch::build_items(uint32_t chunk) {
     return sg.kind <= KIND_STAR2 && (sg.flags & SEG_POSN) != 0;
   };
   const auto mid = std::stable_partition(items.begin(), items.end(), [&](const ItemDev& it) { return !needs_fx(it); });
-  n_items_plain_ = static_cast<uint64_t>(mid - items.begin());
+  // Of the others, items with element-loop patches of a segment with
+  // 1..LDS_ECL_MAX eclipse intervals go second: their kernel stages those
+  // intervals in LDS and needs more registers than the first one.
+  auto staged_ecl = [&](const ItemDev& it) {
+    const SegDev& sg = segs_[it.seg];
+    return it.todo != 0 && sg.necl >= 1 && sg.necl <= LDS_ECL_MAX;
+  };
+  const auto mid2 = std::stable_partition(items.begin(), mid, [&](const ItemDev& it) { return !staged_ecl(it); });
+  n_items_plain_ = static_cast<uint64_t>(mid2 - items.begin());
+  n_items_ecl_ = static_cast<uint64_t>(mid - mid2);
   n_items_ = items.size();
   pairs_eval_ = pairs_mom_ = 0;
   mom_evals_ = 0;
@@ -693,6 +702,7 @@ void Batch::build_items(uint32_t chunk) {
   d_.items = d_items_;
   d_.n_items = static_cast<uint32_t>(n_items_);
   d_.n_items_plain = static_cast<uint32_t>(n_items_plain_);
+  d_.n_items_ecl = static_cast<uint32_t>(n_items_ecl_);
   d_.partials = d_partials_;
 }
 

@@ -128,7 +128,7 @@ class Batch {
   std::vector<uint32_t> limb_lanes_;
   std::vector<uint64_t> lane_masks_;
   uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0,
-           n_items_plain_ = 0, pairs_limb_ = 0, mom_evals_ = 0;
+           n_items_plain_ = 0, n_items_ecl_ = 0, pairs_limb_ = 0, mom_evals_ = 0;
   void* d_limb_ = nullptr;  // one allocation for the limb tables
   uint32_t chunk_ = 0;
 

@@ -135,9 +135,19 @@ __device__ __forceinline__ float4 rec_dirn(const char* __restrict__ p, uint32_t 
   return *reinterpret_cast<const float4*>(p + (i + j) * rec_stride(NECL, false));
 }
 
+// Eclipse intervals of record k: from the record in global memory, or with
+// LDSR from the patch's interval block staged in LDS (3 NECL doubles per
+// record).
+template <int NECL, bool LDSR>
+__device__ __forceinline__ const double* rec_ecl(const char* __restrict__ r, uint32_t k, const double* le) {
+  if (LDSR) return le + k * 3u * NECL;
+  return reinterpret_cast<const double*>(r + 16);
+}
+
 template <int NECL, bool FREE, int LD, bool MCRIT, bool LDSR = false>
 __device__ __forceinline__ float star_patch(const char* __restrict__ p, const LaneCtx& L,
-                                            const StarPar& P, const float4* lp = nullptr) {
+                                            const StarPar& P, const float4* lp = nullptr,
+                                            const double* le = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -152,7 +162,7 @@ __device__ __forceinline__ float star_patch(const char* __restrict__ p, const La
       float wm = m * d.w;
       if (ECL || MCRIT) {
         bool ok = true;
-        if (ECL) ok = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif);
+        if (ECL) ok = eclipse_free<NECL>(rec_ecl<NECL, LDSR>(r, i + j, le), L.phif);
         if (MCRIT) ok = ok & (mu > P.mucrit);
         wm = ok ? wm : 0.0f;
       }
@@ -201,7 +211,7 @@ __device__ __forceinline__ float star_patch_fx(const char* __restrict__ p, const
 // Disc and disc-edge elements (flux.rs:83-100): mu > 0 and visible.
 template <int NECL, bool FREE, bool LDSR = false>
 __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const LaneCtx& L, float lin,
-                                            float quad, const float4* lp = nullptr) {
+                                            float quad, const float4* lp = nullptr, const double* le = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -213,7 +223,7 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
       const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
       const float m = clamp01(mu_of(d, L));
       float wm = m * d.w;
-      if (ECL) wm = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? wm : 0.0f;
+      if (ECL) wm = eclipse_free<NECL>(rec_ecl<NECL, LDSR>(r, i + j, le), L.phif) ? wm : 0.0f;
       const float o = 1.0f - m;
       acc[j] = fmaf(wm, 1.0f - o * (lin + quad * o), acc[j]);
     }
@@ -224,7 +234,7 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
 // Bright-spot elements (flux.rs:102-106): mu * flux where mu > 0 and visible.
 template <int NECL, bool FREE, bool LDSR = false>
 __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const LaneCtx& L,
-                                            const float4* lp = nullptr) {
+                                            const float4* lp = nullptr, const double* le = nullptr) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -235,7 +245,7 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
       const char* r = p + (i + j) * STRIDE;
       const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
       float w = d.w;
-      if (ECL) w = eclipse_free<NECL>(reinterpret_cast<const double*>(r + 16), L.phif) ? w : 0.0f;
+      if (ECL) w = eclipse_free<NECL>(rec_ecl<NECL, LDSR>(r, i + j, le), L.phif) ? w : 0.0f;
       acc[j] = fmaf(clamp01(mu_of(d, L)), w, acc[j]);
     }
   }
@@ -247,7 +257,12 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
 // the `fr` bit; each patch's FP32 sum is added in FP64.
 enum : int { PK_STAR = 0, PK_STARFX = 1, PK_DISC = 2, PK_SPOT = 3 };
 
-template <int PK, int NECL, int LD, int X>
+// Patches of segments with up to LDS_ECL_MAX eclipse intervals per element
+// are staged in LDS too: directions (PATCH float4) then intervals
+// (PATCH * 3 * LDS_ECL_MAX doubles), 4 KB per wavefront.
+constexpr int PBUF_F4 = PATCH + PATCH * 3 * LDS_ECL_MAX / 2;
+
+template <int PK, int NECL, int LD, int X, bool ECLS = false>
 __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t todo, uint64_t fr,
                                            const LaneCtx& L, const StarPar& P, float lin,
                                            float quad, float4* lp) {
@@ -268,6 +283,50 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
       if (PK == PK_STAR) s = star_patch<0, true, LD, X != 0, true>(nullptr, L, P, lp);
       else if (PK == PK_DISC) s = disc_patch<0, true, true>(nullptr, L, lin, quad, lp);
       else s = spot_patch<0, true, true>(nullptr, L, lp);
+      __builtin_amdgcn_wave_barrier();  // reads done before the next write
+      acc += static_cast<double>(s);
+    }
+    return acc;
+  }
+  if (ECLS && NECL > 0 && NECL <= LDS_ECL_MAX && PK != PK_STARFX && lp) {
+    // The same with eclipse intervals: lane j fetches record j's direction
+    // and its 3 NECL interval bounds (for eclipse-tested and free patches
+    // alike, so the staged values never cross a branch); the patch's
+    // directions go to lp[0..63], its intervals to the block after them.
+    constexpr uint32_t STRIDE = rec_stride(NECL, false);
+    constexpr int NE = 3 * (NECL > 0 ? NECL : 1);
+    const uint32_t lane = __lane_id();
+    double* le = reinterpret_cast<double*>(lp + PATCH);
+    uint32_t q = static_cast<uint32_t>(__builtin_ctzll(todo));
+    const char* rq = p + static_cast<size_t>(q) * PB + lane * STRIDE;
+    float4 nd = *reinterpret_cast<const float4*>(rq);
+    double ne[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) ne[k] = reinterpret_cast<const double*>(rq + 16)[k];
+    while (todo) {
+      const bool free = (fr >> q) & 1ull;
+      todo &= todo - 1ull;
+      lp[lane] = nd;
+#pragma unroll
+      for (int k = 0; k < NE; ++k) le[lane * NE + k] = ne[k];
+      __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+      if (todo) {
+        q = static_cast<uint32_t>(__builtin_ctzll(todo));
+        rq = p + static_cast<size_t>(q) * PB + lane * STRIDE;
+        nd = *reinterpret_cast<const float4*>(rq);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) ne[k] = reinterpret_cast<const double*>(rq + 16)[k];
+      }
+      float s;
+      if (PK == PK_STAR) {
+        s = free ? star_patch<NECL, true, LD, X != 0, true>(nullptr, L, P, lp, le)
+                 : star_patch<NECL, false, LD, X != 0, true>(nullptr, L, P, lp, le);
+      } else if (PK == PK_DISC) {
+        s = free ? disc_patch<NECL, true, true>(nullptr, L, lin, quad, lp, le)
+                 : disc_patch<NECL, false, true>(nullptr, L, lin, quad, lp, le);
+      } else {
+        s = free ? spot_patch<NECL, true, true>(nullptr, L, lp, le) : spot_patch<NECL, false, true>(nullptr, L, lp, le);
+      }
       __builtin_amdgcn_wave_barrier();  // reads done before the next write
       acc += static_cast<double>(s);
     }
@@ -296,7 +355,7 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
 // FX: whether the beaming / lensing star paths are compiled in. The kernel
 // without them needs fewer registers (more waves per SIMD); the host sends
 // it no item of a beamed or lensed star segment.
-template <int NECL, bool FX>
+template <int NECL, bool FX, bool ECLS = false>
 __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t fx, bool mcrit,
                                                 const char* p, uint64_t todo, uint64_t fr,
                                                 const LaneCtx& L, const StarPar& P, float lin,
@@ -305,29 +364,29 @@ __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t 
     if (!FX && fx != 0) return __builtin_nan("");  // never sent here: fail loudly
     if (!FX || fx == 0) {
       if (ld == LD_LIN)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1>(p, todo, fr, L, P, lin, quad, lp)
-                     : item_sum<PK_STAR, NECL, LD_LIN, 0>(p, todo, fr, L, P, lin, quad, lp);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_LIN, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
       if (ld == LD_CLARET)
-        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
-                     : item_sum<PK_STAR, NECL, LD_CLARET, 0>(p, todo, fr, L, P, lin, quad, lp);
-      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp)
-                   : item_sum<PK_STAR, NECL, LD_POLY, 0>(p, todo, fr, L, P, lin, quad, lp);
+        return mcrit ? item_sum<PK_STAR, NECL, LD_CLARET, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                     : item_sum<PK_STAR, NECL, LD_CLARET, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
+      return mcrit ? item_sum<PK_STAR, NECL, LD_POLY, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                   : item_sum<PK_STAR, NECL, LD_POLY, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
     }
     const bool cl = ld == LD_CLARET;
     switch (fx) {
       case 1:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1>(p, todo, fr, L, P, lin, quad, lp)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 1>(p, todo, fr, L, P, lin, quad, lp);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp);
       case 2:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2>(p, todo, fr, L, P, lin, quad, lp)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 2>(p, todo, fr, L, P, lin, quad, lp);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 2, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 2, ECLS>(p, todo, fr, L, P, lin, quad, lp);
       default:
-        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3>(p, todo, fr, L, P, lin, quad, lp)
-                  : item_sum<PK_STARFX, NECL, LD_POLY, 3>(p, todo, fr, L, P, lin, quad, lp);
+        return cl ? item_sum<PK_STARFX, NECL, LD_CLARET, 3, ECLS>(p, todo, fr, L, P, lin, quad, lp)
+                  : item_sum<PK_STARFX, NECL, LD_POLY, 3, ECLS>(p, todo, fr, L, P, lin, quad, lp);
     }
   }
-  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
-  return item_sum<PK_DISC, NECL, 0, 0>(p, todo, fr, L, P, lin, quad, lp);
+  if (kind == KIND_SPOT) return item_sum<PK_SPOT, NECL, 0, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
+  return item_sum<PK_DISC, NECL, 0, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
 }
 
 // One patch staged in LDS (segments without eclipse intervals, no beaming
@@ -487,7 +546,7 @@ __device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t n) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <bool FX>
+template <bool FX, bool ECLS = false>
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
@@ -564,29 +623,31 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
-    case 0: sum += item_dispatch<0, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
-    case 1: sum += item_dispatch<1, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
-    case 2: sum += item_dispatch<2, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
-    case 3: sum += item_dispatch<3, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
-    default: sum += item_dispatch<4, FX>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 0: sum += item_dispatch<0, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 1: sum += item_dispatch<1, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 2: sum += item_dispatch<2, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    case 3: sum += item_dispatch<3, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
+    default: sum += item_dispatch<4, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;
   }
   partials[it.prow * TILE + lane] = active ? sum : 0.0;
 }
 
 }  // namespace
 
-// One workgroup = one wavefront = one work item.
-template <bool FX>
+// One workgroup = one wavefront = one work item. ECLS: patches of segments
+// with 1..LDS_ECL_MAX eclipse intervals are staged in LDS (more registers;
+// the host launches this form only for items that have such patches).
+template <bool FX, bool ECLS>
 __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
     const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
-  __shared__ float4 pbuf[PATCH];  // one patch of records (one wavefront per workgroup)
+  __shared__ float4 pbuf[ECLS ? PBUF_F4 : PATCH];  // one patch of records (one wavefront per workgroup)
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item<FX>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials,
+  process_item<FX, ECLS>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials,
                    pbuf);
 }
 
@@ -1142,14 +1203,21 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
                              uint32_t grab, hipStream_t stream) {
   if (d.n_items == 0) return hipSuccess;
   if (!persistent) {
-    // Items without beaming / lensing come first (host order).
-    const uint32_t nf = d.n_items_plain, nx = d.n_items - d.n_items_plain;
+    // Host order: items without beaming / lensing and without staged
+    // eclipse intervals, then those with staged intervals, then the
+    // beamed / lensed ones.
+    const uint32_t nf = d.n_items_plain, ne = d.n_items_ecl, nx = d.n_items - nf - ne;
     if (nf)
-      hipLaunchKernelGGL(flux_items_kernel<false>, dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs, d.sets,
-                         d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
-    if (nx)
-      hipLaunchKernelGGL(flux_items_kernel<true>, dim3(nx), dim3(TILE), 0, stream, d.items + nf, nx, d.segs,
+      hipLaunchKernelGGL((flux_items_kernel<false, false>), dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs,
                          d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
+    if (ne)
+      hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, stream, d.items + nf, ne,
+                         d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                         d.partials);
+    if (nx)
+      hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, stream, d.items + nf + ne, nx,
+                         d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                         d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,

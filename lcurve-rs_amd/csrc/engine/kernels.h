@@ -10,7 +10,8 @@ namespace lch {
 struct DeviceBatch {
   uint32_t n_sets = 0, n_items = 0;
   uint32_t max_set_lanes = 0;  // largest SetDev::special0 of a valid set
-  uint32_t n_items_plain = 0;  // leading items with no beaming / lensing
+  uint32_t n_items_plain = 0;  // leading items with no beaming / lensing and no staged eclipse intervals
+  uint32_t n_items_ecl = 0;    // then items with staged eclipse intervals (no beaming / lensing)
   const SetDev* sets = nullptr;
   const SegDev* segs = nullptr;
   const ItemDev* items = nullptr;

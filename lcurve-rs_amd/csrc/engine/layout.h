@@ -43,6 +43,10 @@ constexpr int MAX_ECL = 4;   // eclipse pairs per element supported by the kerne
 //   NECL x { double ingress, egress, egress - 1 }
 constexpr uint32_t SEG_POSN = 1u;
 
+// Segments with 1..LDS_ECL_MAX eclipse intervals per element have their
+// element-loop patches staged in LDS by a kernel form of their own.
+constexpr uint32_t LDS_ECL_MAX = 2;
+
 __host__ __device__ constexpr uint32_t rec_stride(uint32_t necl, bool posn) {
   return 16u + (posn ? 16u : 0u) + 24u * necl;
 }
