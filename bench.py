@@ -187,6 +187,9 @@ def roofline(workload, st, ms_flux, limb):
             "pairs_evaluated_per_launch": pairs, "moment_evals_per_launch": mom_lanes,
             "pairs_by_moments_per_launch": st["pairs_by_moments"],
             "limb_pairs_per_launch": st["pairs_limb"],
+            # (element, lane) slots the limb pass computes, padding included:
+            # its lane fill is limb_pairs / limb_slots
+            "limb_slots_per_launch": st["limb_slots"],
             "reference_visits_per_launch": st["visits"],
             "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
             "hbm_algorithmic_bytes": alg_bytes,

@@ -73,6 +73,9 @@ typedef struct lch_batch_stats {
   long long limb_chunks;  /* workgroups of the limb pass */
   long long moment_evals; /* moment sums evaluated per launch, counted per lane (a run of
                              patches summed from the block table counts once) */
+  long long limb_items;   /* limb items (one patch and up to 64 limb lanes each) */
+  long long limb_slots;   /* (element, lane) slots the limb pass computes, padding
+                             included: pairs_limb / limb_slots is its lane fill */
 } lch_batch_stats;
 
 int lch_version(void);

@@ -426,6 +426,8 @@ int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
   out->pairs_limb = static_cast<long long>(B.pairs_limb());
   out->limb_chunks = static_cast<long long>(B.n_limb_chunks());
   out->moment_evals = static_cast<long long>(B.moment_evals());
+  out->limb_items = static_cast<long long>(B.n_limb_items());
+  out->limb_slots = static_cast<long long>(B.limb_slots());
   return LCH_OK;
 }
 

@@ -296,6 +296,7 @@ void Batch::build_items(uint32_t chunk) {
   limb_lanes_.clear();
   lane_masks_.clear();
   pairs_limb_ = 0;
+  limb_slots_ = 0;
   // Sequential pass: chunk tables and partial-row bases per set.
   struct SetPlan {
     size_t g0 = 0, g1 = 0;
@@ -336,6 +337,7 @@ void Batch::build_items(uint32_t chunk) {
     std::vector<uint16_t> lidx;
     std::vector<uint32_t> llanes;
     uint64_t pairs_limb = 0;
+    uint64_t limb_slots = 0;
     std::string error;
   };
   std::vector<SetItems> out(sets_.size());
@@ -461,16 +463,27 @@ void Batch::build_items(uint32_t chunk) {
           R.llanes.insert(R.llanes.end(), uni.begin(), uni.end());
           for (uint32_t q = 0; q < np; ++q) {
             const std::vector<uint32_t>& v = plan[q];
-            for (size_t i0 = 0; i0 < v.size(); i0 += TILE) {
-              R.litems.push_back(LimbItem{e0 + q * PATCH, 0u});
-              for (size_t i = i0; i < i0 + TILE; ++i) {
-                uint16_t u = 0xFFFFu;
-                if (i < v.size()) {
-                  u = static_cast<uint16_t>(uidx[v[i] - sd.lane_off]);
-                  R.pairs_limb += PATCH;
+            for (size_t i0 = 0; i0 < v.size();) {
+              // Up to 64 lanes per item. A remainder of at most 32 (16)
+              // lanes of a segment without eclipse intervals takes a split
+              // item: 2 (4) lane groups holding the same lanes, each
+              // summing a half (quarter) of the patch's elements.
+              const size_t rem = v.size() - i0;
+              uint32_t split = 0;
+              if (sg.necl == 0) split = rem <= TILE / 4 ? 2u : (rem <= TILE / 2 ? 1u : 0u);
+              const size_t gl = TILE >> split;
+              R.litems.push_back(LimbItem{e0 + q * PATCH, split});
+              for (uint32_t g = 0; g < (1u << split); ++g)
+                for (size_t i = i0; i < i0 + gl; ++i) {
+                  uint16_t u = 0xFFFFu;
+                  if (i < v.size()) {
+                    u = static_cast<uint16_t>(uidx[v[i] - sd.lane_off]);
+                    if (g == 0) R.pairs_limb += PATCH;
+                  }
+                  R.lidx.push_back(u);
                 }
-                R.lidx.push_back(u);
-              }
+              R.limb_slots += static_cast<uint64_t>(TILE) * (PATCH >> split);
+              i0 += gl;
             }
           }
           lc.n_items = static_cast<uint32_t>(R.litems.size()) - lc.item0;
@@ -537,6 +550,7 @@ void Batch::build_items(uint32_t chunk) {
       limb_idx_.insert(limb_idx_.end(), R.lidx.begin(), R.lidx.end());
       limb_lanes_.insert(limb_lanes_.end(), R.llanes.begin(), R.llanes.end());
       pairs_limb_ += R.pairs_limb;
+      limb_slots_ += R.limb_slots;
       R = SetItems();
     }
   }
@@ -632,6 +646,7 @@ void Batch::build_items(uint32_t chunk) {
     const uint64_t stride = rec_stride(sg.necl, false);
     for (uint32_t i = lc.item0; i < lc.item0 + lc.n_items; ++i) {
       const LimbItem& li = limb_items_[i];
+      if (li.split > 2 || (li.split && sg.necl != 0)) throw DeviceError("limb item split out of range");
       if (li.e0 % PATCH || static_cast<uint64_t>(li.e0) + PATCH > sg.n ||
           sg.rec_off + (static_cast<uint64_t>(li.e0) + PATCH) * stride > rec_bytes_)
         throw DeviceError("limb item records out of range");

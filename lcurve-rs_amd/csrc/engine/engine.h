@@ -83,6 +83,9 @@ class Batch {
   // Of pairs_evaluated(): (element, lane) pairs of limb lanes.
   uint64_t pairs_limb() const { return pairs_limb_; }
   uint64_t n_limb_chunks() const { return limb_chunks_.size(); }
+  uint64_t n_limb_items() const { return limb_items_.size(); }
+  // (element, lane) slots the limb pass computes, padding included.
+  uint64_t limb_slots() const { return limb_slots_; }
   // Moment sums evaluated per launch, per lane (runs and split patches).
   uint64_t moment_evals() const { return mom_evals_; }
   uint64_t n_items() const { return n_items_; }
@@ -128,7 +131,7 @@ class Batch {
   std::vector<uint32_t> limb_lanes_;
   std::vector<uint64_t> lane_masks_;
   uint64_t rec_bytes_ = 0, n_items_ = 0, n_lanes_ = 0, n_prows_ = 0, pairs_eval_ = 0, pairs_mom_ = 0,
-           n_items_plain_ = 0, n_items_ecl_ = 0, pairs_limb_ = 0, mom_evals_ = 0;
+           n_items_plain_ = 0, n_items_ecl_ = 0, pairs_limb_ = 0, limb_slots_ = 0, mom_evals_ = 0;
   void* d_limb_ = nullptr;  // one allocation for the limb tables
   uint32_t chunk_ = 0;
 

@@ -147,12 +147,12 @@ __device__ __forceinline__ const double* rec_ecl(const char* __restrict__ r, uin
 template <int NECL, bool FREE, int LD, bool MCRIT, bool LDSR = false>
 __device__ __forceinline__ float star_patch(const char* __restrict__ p, const LaneCtx& L,
                                             const StarPar& P, const float4* lp = nullptr,
-                                            const double* le = nullptr) {
+                                            const double* le = nullptr, uint32_t n = PATCH) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll(LDSR ? 1 : 2)
-  for (uint32_t i = 0; i < PATCH; i += 4) {
+  for (uint32_t i = 0; i < n; i += 4) {
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
@@ -211,12 +211,13 @@ __device__ __forceinline__ float star_patch_fx(const char* __restrict__ p, const
 // Disc and disc-edge elements (flux.rs:83-100): mu > 0 and visible.
 template <int NECL, bool FREE, bool LDSR = false>
 __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const LaneCtx& L, float lin,
-                                            float quad, const float4* lp = nullptr, const double* le = nullptr) {
+                                            float quad, const float4* lp = nullptr, const double* le = nullptr,
+                                            uint32_t n = PATCH) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-  for (uint32_t i = 0; i < PATCH; i += 4) {
+  for (uint32_t i = 0; i < n; i += 4) {
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
@@ -234,12 +235,13 @@ __device__ __forceinline__ float disc_patch(const char* __restrict__ p, const La
 // Bright-spot elements (flux.rs:102-106): mu * flux where mu > 0 and visible.
 template <int NECL, bool FREE, bool LDSR = false>
 __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const LaneCtx& L,
-                                            const float4* lp = nullptr, const double* le = nullptr) {
+                                            const float4* lp = nullptr, const double* le = nullptr,
+                                            uint32_t n = PATCH) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-  for (uint32_t i = 0; i < PATCH; i += 4) {
+  for (uint32_t i = 0; i < n; i += 4) {
 #pragma unroll
     for (uint32_t j = 0; j < 4; ++j) {
       const char* r = p + (i + j) * STRIDE;
@@ -393,19 +395,19 @@ __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t 
 // or lensing, no eclipse test), by segment kind and limb-darkening law.
 __device__ __forceinline__ float patch_sum_lds(uint32_t kind, int ld, bool mcrit, const float4* lp,
                                                const LaneCtx& L, const StarPar& P, float lin,
-                                               float quad) {
+                                               float quad, uint32_t n = PATCH) {
   if (kind <= KIND_STAR2) {
     if (ld == LD_LIN)
-      return mcrit ? star_patch<0, true, LD_LIN, true, true>(nullptr, L, P, lp)
-                   : star_patch<0, true, LD_LIN, false, true>(nullptr, L, P, lp);
+      return mcrit ? star_patch<0, true, LD_LIN, true, true>(nullptr, L, P, lp, nullptr, n)
+                   : star_patch<0, true, LD_LIN, false, true>(nullptr, L, P, lp, nullptr, n);
     if (ld == LD_CLARET)
-      return mcrit ? star_patch<0, true, LD_CLARET, true, true>(nullptr, L, P, lp)
-                   : star_patch<0, true, LD_CLARET, false, true>(nullptr, L, P, lp);
-    return mcrit ? star_patch<0, true, LD_POLY, true, true>(nullptr, L, P, lp)
-                 : star_patch<0, true, LD_POLY, false, true>(nullptr, L, P, lp);
+      return mcrit ? star_patch<0, true, LD_CLARET, true, true>(nullptr, L, P, lp, nullptr, n)
+                   : star_patch<0, true, LD_CLARET, false, true>(nullptr, L, P, lp, nullptr, n);
+    return mcrit ? star_patch<0, true, LD_POLY, true, true>(nullptr, L, P, lp, nullptr, n)
+                 : star_patch<0, true, LD_POLY, false, true>(nullptr, L, P, lp, nullptr, n);
   }
-  if (kind == KIND_SPOT) return spot_patch<0, true, true>(nullptr, L, lp);
-  return disc_patch<0, true, true>(nullptr, L, lin, quad, lp);
+  if (kind == KIND_SPOT) return spot_patch<0, true, true>(nullptr, L, lp, nullptr, n);
+  return disc_patch<0, true, true>(nullptr, L, lin, quad, lp, nullptr, n);
 }
 
 // Patches seen whole by every lane (PT_MOM): the sum over the patch of
@@ -448,7 +450,7 @@ __device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint
 // item's patch count, a power of two).
 __device__ __forceinline__ double moment_sum_runs(const double* __restrict__ m0,
                                                   const double* __restrict__ blk,
-                                                  const uint32_t* __restrict__ lvl, uint32_t pbase,
+                                                  const uint32_t (&lvl)[MOM_LEVELS], uint32_t pbase,
                                                   uint64_t mm, const LaneCtx& L, double c1,
                                                   double c2, double c3) {
   const double x = L.ex, y = L.ey, z = L.ez;
@@ -469,8 +471,13 @@ __device__ __forceinline__ double moment_sum_runs(const double* __restrict__ m0,
     k = min(k, MOM_LEVELS);
     const uint32_t len = 1u << k;
     mm &= ~((len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q);
+    // Level offset by selects over the segment's table (held in registers),
+    // not an indexed load: one memory round trip per run instead of two.
+    uint32_t lo = lvl[0];
+#pragma unroll
+    for (uint32_t v = 2; v <= MOM_LEVELS; ++v) lo = k == v ? lvl[v - 1] : lo;
     const double* M = k == 0 ? m0 + static_cast<size_t>(q) * MOM_STRIDE
-                             : blk + static_cast<size_t>(lvl[k - 1] + (pa >> k)) * MOM_STRIDE;
+                             : blk + static_cast<size_t>(lo + (pa >> k)) * MOM_STRIDE;
     const double s1 = M[0] * x + M[1] * y + M[2] * z;
     const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
     double v = c1 * s1 + c2 * s2;
@@ -617,7 +624,7 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
     const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
     if (it.mom)
-      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, segs[it.seg].lvl_off,
+      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, sg.lvl_off,
                             it.e0 / PATCH, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
     if (it.pmom)
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
@@ -726,6 +733,11 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
     const uint32_t inext = i + LIMB_WAVES;
     u_next = d.limb_idx[static_cast<size_t>(inext < ilast ? inext : ilast) * TILE + lane];
     const LimbItem li = d.limb_items[i];
+    // A split item (li.split = log2 of 2 or 4) gives each group of
+    // 64 >> split lanes (the same limb lanes in every group) its own share
+    // of the patch's elements; the groups' parts meet in the integer sums.
+    const uint32_t nel = PATCH >> li.split;
+    const uint32_t base = (lane / (TILE >> li.split)) * nel;
     const bool valid = u != 0xFFFFu;
     const uint32_t uu = valid ? u : 0u;
     LaneCtx L;
@@ -741,7 +753,7 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
       __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
       const uint32_t e = d.limb_items[inext < ilast ? inext : ilast].e0;
       rec_next = reinterpret_cast<const float4*>(d.recs + sg.rec_off)[e + lane];
-      s = patch_sum_lds(sg.kind, law.ld, law.mcrit, &lrec[w][0], L, law.P, lin, quad);
+      s = patch_sum_lds(sg.kind, law.ld, law.mcrit, &lrec[w][base], L, law.P, lin, quad, nel);
       __builtin_amdgcn_wave_barrier();  // reads done before the next write
     } else switch (sg.necl) {
       case 0: s = item_dispatch<0, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;

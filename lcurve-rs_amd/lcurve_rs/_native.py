@@ -43,7 +43,8 @@ class BatchStatsC(ctypes.Structure):
                 ("setup_seconds", ctypes.c_double),
                 ("upload_seconds", ctypes.c_double), ("chunk", ctypes.c_int),
                 ("persistent", ctypes.c_int), ("pairs_limb", ctypes.c_longlong),
-                ("limb_chunks", ctypes.c_longlong), ("moment_evals", ctypes.c_longlong)]
+                ("limb_chunks", ctypes.c_longlong), ("moment_evals", ctypes.c_longlong),
+                ("limb_items", ctypes.c_longlong), ("limb_slots", ctypes.c_longlong)]
 
 
 # name -> (restype, argtypes)
