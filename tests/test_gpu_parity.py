@@ -255,6 +255,31 @@ def test_limb_lanes_match_full_tiles_and_oracle(lcurve_rs):
     assert np.array_equal(f_on, again)
 
 
+@pytest.mark.parametrize("max_part", [16, 64])
+def test_limb_thresholds_and_split_items(lcurve_rs, max_part):
+    """Other limb thresholds give other mixes of whole and split limb items
+    (a patch's remainder of <= 32 or <= 16 limb lanes shares its elements
+    among 2 or 4 lane groups): every mix matches the tiles without limb
+    lanes, and the lane fill stays within [0, 1]."""
+    texts = _chisq_texts(4)
+    t = W.linspace_times(-0.2, 1.2, 1500)
+    models = [lcurve_rs.Model.from_text(x) for x in texts]
+    try:
+        lcurve_rs.set_limb(False)
+        f_off = lcurve_rs.Batch(models, t, 0.001, 1).run(scale=False)["flux"]
+        lcurve_rs.set_limb(True, max_part)
+        b = lcurve_rs.Batch(models, t, 0.001, 1)
+        st = b.stats()
+        f_on = b.run(scale=False)["flux"]
+    finally:
+        lcurve_rs.set_limb(True)
+    assert st["limb_chunks"] > 0 and 0 < st["pairs_limb"] <= st["limb_slots"]
+    # Split items take fewer slots than one whole item per patch would.
+    assert st["limb_slots"] < st["limb_items"] * 64 * 64
+    assert np.max(np.abs(f_on / f_off - 1)) <= 3e-7
+    assert np.array_equal(f_on, b.run(scale=False)["flux"])
+
+
 def test_launch_forms_and_autotune_agree(lcurve_rs):
     texts, t = uneven_batch(6, 200)
     models = [lcurve_rs.Model.from_text(x) for x in texts]
