@@ -71,3 +71,26 @@ def test_lroche_with_data_file_and_output(lcurve_rs, tmp_path):
     assert all(r[5] == "3" and r[1] == "0.002" for r in rows)
     calc = np.array([float(r[2]) for r in rows])
     assert np.allclose(calc, res.calc, rtol=6e-11, atol=0)
+
+
+def test_lroche_iscale_prints_five_factors(lcurve_rs, golden_model_text, tmp_path):
+    """An iscale model (per-component fit, orchestration.rs:181-227) prints
+    five scale factors, as src/main.rs does; they equal the LcResult's."""
+    from lcurve_rs import lroche
+    from lcurve_rs import workloads as W
+    model = tmp_path / "iscale_model.dat"
+    model.write_text(W.model_text({"iscale": 1}, golden_model_text))
+    assert lcurve_rs.Model(str(model)).iscale and not lcurve_rs.Model(MODEL).iscale
+    t = np.linspace(-0.2, 0.8, 150)
+    base = lcurve_rs.Model(MODEL).light_curve(times=t, expose=0.001, ndiv=1)
+    data = tmp_path / "obs.dat"
+    with open(data, "w") as f:
+        for i in range(len(t)):
+            f.write(f"{float(t[i])!r} 0.001 {float(2.0 * base.calc[i])!r} {float(0.01 * base.calc[i])!r} 1 1\n")
+    buf = io.StringIO()
+    res = lroche.run([str(model), str(data), "--scale"], out=buf)
+    line = buf.getvalue().splitlines()[1]
+    assert line.startswith("Scale factors = ")
+    vals = [float(v) for v in line.split(" = ")[1].split(", ")]
+    assert len(vals) == 5
+    assert np.allclose(vals, res.sfac[:5], rtol=1e-12, atol=0)
