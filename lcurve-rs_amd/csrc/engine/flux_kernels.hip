@@ -66,13 +66,20 @@ __device__ __forceinline__ bool eclipse_free(const double* __restrict__ e, doubl
 
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
+// sqrt(m) for the Claret law, m in [0, 1]: the hardware square root
+// (v_sqrt_f32, within 1 ulp). The correctly rounded sqrtf adds range
+// scaling and a Newton correction, about 13 instructions per element-lane
+// pair; the FP32 term does not need it (its error budget is 1e-6 of the
+// flux).
+__device__ __forceinline__ float claret_sqrt(float m) { return __builtin_amdgcn_sqrtf(m); }
+
 // LDC::imu (types.rs:101-119) in FP32, including its mu <= 0 test.
 template <int LD>
 __device__ __forceinline__ float imu(float mu, const StarPar& P) {
   float m = fminf(mu, 1.0f);
   float im;
   if (LD == LD_CLARET) {
-    float s = sqrtf(m);
+    float s = claret_sqrt(m);  // NaN for m < 0, dropped by the mu > 0 select below
     im = P.c0 + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
   } else {
     float o = 1.0f - m;
@@ -87,7 +94,7 @@ template <int LD>
 __device__ __forceinline__ float imu_clamped(float m, const StarPar& P) {
   if (LD == LD_LIN) return fmaf(P.a0, m, P.c0);
   if (LD == LD_CLARET) {
-    float s = sqrtf(m);
+    float s = claret_sqrt(m);
     return P.c0 + s * (P.a0 + s * (P.a1 + s * (P.a2 + s * P.a3)));
   }
   float o = 1.0f - m;
