@@ -118,35 +118,6 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
       sg.set = static_cast<uint32_t>(s);
       if (mom_base + sg.patch_off > UINT32_MAX) throw DeviceError("too many patches");
       sg.mom_off = static_cast<uint32_t>(mom_base + sg.patch_off);
-      // Moment sums of aligned runs of 2, 4, ..., 64 patches, each level
-      // from pairs of the one below.
-      {
-        const uint32_t np = sg.n / PATCH;
-        if (mblk_.size() / MOM_STRIDE > UINT32_MAX) throw DeviceError("too many moment blocks");
-        sg.blk_off = static_cast<uint32_t>(mblk_.size() / MOM_STRIDE);
-        sg.pad = 0;
-        size_t prev = static_cast<size_t>(sg.mom_off) * MOM_STRIDE;  // level 0 in moms_
-        uint32_t nprev = np;
-        bool from_moms = true;
-        for (uint32_t k = 1; k <= MOM_LEVELS; ++k) {
-          const uint32_t nb = (np + (1u << k) - 1) >> k;
-          const size_t base = mblk_.size();
-          sg.lvl_off[k - 1] = static_cast<uint32_t>(base / MOM_STRIDE - sg.blk_off);
-          mblk_.resize(base + static_cast<size_t>(nb) * MOM_STRIDE, 0.0);
-          for (uint32_t j = 0; j < nb; ++j) {
-            for (uint32_t h = 0; h < 2; ++h) {
-              const uint32_t src = 2 * j + h;
-              if (src >= nprev) break;
-              const double* a = (from_moms ? moms_.data() : mblk_.data()) + prev + static_cast<size_t>(src) * MOM_STRIDE;
-              double* o = mblk_.data() + base + static_cast<size_t>(j) * MOM_STRIDE;
-              for (uint32_t c = 0; c < MOM_STRIDE; ++c) o[c] += a[c];
-            }
-          }
-          prev = base;
-          nprev = nb;
-          from_moms = false;
-        }
-      }
       uint32_t gid = static_cast<uint32_t>(segs_.size());
       if (P.seg_grid[g] < 7) {
         H.grids[P.seg_grid[g]].segs.push_back(gid);
@@ -212,6 +183,7 @@ Batch::~Batch() {
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (d_items_) (void)hipFree(d_items_);
+  if (d_imoms_) (void)hipFree(d_imoms_);
   if (d_partials_) (void)hipFree(d_partials_);
   if (d_limb_) (void)hipFree(d_limb_);
   if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
@@ -263,9 +235,6 @@ void Batch::upload() {
   }
   up(moms_, const_cast<double**>(&lp));
   d_.moms = lp;
-  if (mblk_.empty()) mblk_.assign(MOM_STRIDE, 0.0);
-  up(mblk_, const_cast<double**>(&lp));
-  d_.mblk = lp;
   d_.lane_tot = dalloc<double>(lane_phi_.size());
   d_.flux = dalloc<double>(t_lane0_.size());
   d_.out = dalloc<SetOut>(sets_.size());
@@ -572,27 +541,48 @@ void Batch::build_items(uint32_t chunk) {
   n_items_plain_ = static_cast<uint64_t>(mid2 - items.begin());
   n_items_ecl_ = static_cast<uint64_t>(mid - mid2);
   n_items_ = items.size();
+  // Item moments: the sum, in patch order, of the moments of an item's
+  // PT_MOM patches, so that the kernel evaluates one moment sum per item and
+  // lane for them (FP64 on the host, across the setup threads).
+  {
+    uint64_t n_im = 0;
+    for (ItemDev& it : items) it.imom = it.mom ? static_cast<uint32_t>(n_im++) : 0u;
+    if (n_im > UINT32_MAX) throw DeviceError("too many item moments");
+    item_moms_.assign(std::max<uint64_t>(n_im, 1) * MOM_STRIDE, 0.0);
+    for (const ItemDev& it : items)
+      if (it.mom && (it.seg >= segs_.size() ||
+                     static_cast<uint64_t>(segs_[it.seg].mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE))
+        throw DeviceError("item moments out of range");
+    int nthreads = static_cast<int>(std::thread::hardware_concurrency());
+    if (const char* env = std::getenv("LCURVE_HIP_THREADS")) nthreads = std::max(1, std::atoi(env));
+    nthreads = std::max(1, nthreads);
+    const size_t blk = 4096;
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&] {
+        for (size_t b; (b = next.fetch_add(blk)) < items.size();)
+          for (size_t i = b; i < std::min(items.size(), b + blk); ++i) {
+            const ItemDev& it = items[i];
+            if (!it.mom) continue;
+            double* o = item_moms_.data() + static_cast<size_t>(it.imom) * MOM_STRIDE;
+            const double* m0 = moms_.data() + (static_cast<size_t>(segs_[it.seg].mom_off) + it.e0 / PATCH) * MOM_STRIDE;
+            for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) {
+              const double* a = m0 + static_cast<size_t>(__builtin_ctzll(mm)) * MOM_STRIDE;
+              for (uint32_t c = 0; c < MOM_STRIDE; ++c) o[c] += a[c];
+            }
+          }
+      });
+    for (auto& th : pool) th.join();
+  }
   pairs_eval_ = pairs_mom_ = 0;
   mom_evals_ = 0;
   for (const ItemDev& it : items) {
     pairs_eval_ += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
     pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom) + __builtin_popcountll(it.pmom)) * PATCH * TILE;
-    // Moment evaluations as the kernel does them: aligned runs of PT_MOM
+    // Moment evaluations as the kernel does them: one for the item's PT_MOM
     // patches, one per split patch.
-    uint64_t runs = 0;
-    for (uint64_t mm = it.mom; mm;) {
-      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
-      const uint64_t inv = ~(mm >> q);
-      const uint32_t ones = inv ? static_cast<uint32_t>(__builtin_ctzll(inv)) : 64u;
-      const uint32_t pa = it.e0 / PATCH + q;
-      uint32_t k = 31u - static_cast<uint32_t>(__builtin_clz(ones));
-      if (pa) k = std::min<uint32_t>(k, static_cast<uint32_t>(__builtin_ctz(pa)));
-      k = std::min<uint32_t>(k, MOM_LEVELS);
-      const uint32_t len = 1u << k;
-      mm &= ~((len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q);
-      ++runs;
-    }
-    mom_evals_ += (runs + static_cast<uint64_t>(__builtin_popcountll(it.pmom))) * TILE;
+    mom_evals_ += ((it.mom ? 1u : 0u) + static_cast<uint64_t>(__builtin_popcountll(it.pmom))) * TILE;
   }
   pairs_eval_ += pairs_limb_;
   // Host-side bounds check of every item against the buffers it indexes.
@@ -616,12 +606,8 @@ void Batch::build_items(uint32_t chunk) {
       throw DeviceError("item patch masks out of range");
     if ((it.mom || it.pmom) && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
       throw DeviceError("item moments out of range");
-    if (it.mom) {
-      uint64_t nblk = 0;
-      for (uint32_t k = 1; k <= MOM_LEVELS; ++k) nblk += ((sg.n / PATCH) + (1u << k) - 1) >> k;
-      if (static_cast<uint64_t>(sg.blk_off) + nblk > mblk_.size() / MOM_STRIDE)
-        throw DeviceError("item moment blocks out of range");
-    }
+    if (it.mom && (static_cast<uint64_t>(it.imom) + 1) * MOM_STRIDE > item_moms_.size())
+      throw DeviceError("item moment entry out of range");
     if ((it.pmom & ~live) || (it.pmom & (it.todo | it.mom)) ||
         static_cast<uint64_t>(it.lm_off) + __builtin_popcountll(it.pmom) > lane_masks_.size())
       throw DeviceError("item lane masks out of range");
@@ -660,13 +646,19 @@ void Batch::build_items(uint32_t chunk) {
   if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
 
   if (d_items_) (void)hipFree(d_items_);
+  if (d_imoms_) (void)hipFree(d_imoms_);
   if (d_partials_) (void)hipFree(d_partials_);
   if (d_limb_) (void)hipFree(d_limb_);
   d_items_ = nullptr;
+  d_imoms_ = nullptr;
   d_partials_ = nullptr;
   d_limb_ = nullptr;
   hip_check(hipMalloc(reinterpret_cast<void**>(&d_items_), std::max<size_t>(items.size(), 1) * sizeof(ItemDev)),
             "hipMalloc items");
+  hip_check(hipMalloc(reinterpret_cast<void**>(&d_imoms_), item_moms_.size() * sizeof(double)), "hipMalloc item moments");
+  hip_check(hipMemcpy(d_imoms_, item_moms_.data(), item_moms_.size() * sizeof(double), hipMemcpyHostToDevice),
+            "hipMemcpy item moments");
+  d_.imoms = d_imoms_;
   // Limb tables and item lane masks, one allocation.
   {
     const size_t b_ch = align256(limb_chunks_.size() * sizeof(LimbChunk));

@@ -120,7 +120,7 @@ class Batch {
   // Per-set patch codes (ptcode of SetPack), kept for rebuilding items.
   std::vector<uint8_t> ptcode_;
   std::vector<double> moms_;
-  std::vector<double> mblk_;  // moment sums of aligned patch runs
+  std::vector<double> item_moms_;  // summed moments of each item's PT_MOM patches
   std::vector<uint64_t> pt_base_;
   std::vector<uint32_t> pt_tiles_;
   std::vector<std::vector<SetPack::LaneSplit>> lsplit_;  // per set
@@ -138,6 +138,7 @@ class Batch {
   DeviceBatch d_;
   std::vector<void*> allocs_;
   ItemDev* d_items_ = nullptr;
+  double* d_imoms_ = nullptr;
   double* d_partials_ = nullptr;
   hipGraphExec_t graph_ = nullptr;
   int graph_scale_ = -1;

@@ -417,85 +417,22 @@ __device__ __forceinline__ float patch_sum_lds(uint32_t kind, int ld, bool mcrit
   return disc_patch<0, true, true>(nullptr, L, lin, quad, lp, nullptr, n);
 }
 
-// Patches seen whole by every lane (PT_MOM): the sum over the patch of
-// w * mu * (c1 + c2 mu + c3 mu^2) from its direction moments, in FP64.
-__device__ __forceinline__ double moment_sum(const double* __restrict__ mb, uint64_t mm,
-                                             const LaneCtx& L, double c1, double c2, double c3) {
+// An item's PT_MOM patches from their summed moments (one table entry per
+// item): w mu (c1 + c2 mu + c3 mu^2) summed over the patches, in FP64.
+__device__ __forceinline__ double moment_item(const double* __restrict__ M, const LaneCtx& L, double c1,
+                                              double c2, double c3) {
   const double x = L.ex, y = L.ey, z = L.ez;
   const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
-  double acc = 0.0;
+  const double s1 = M[0] * x + M[1] * y + M[2] * z;
+  const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
+  double v = c1 * s1 + c2 * s2;
   if (c3 != 0.0) {
-    const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
-                 t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
-    while (mm) {
-      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
-      mm &= mm - 1ull;
-      const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
-      const double s1 = M[0] * x + M[1] * y + M[2] * z;
-      const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
-      const double s3 = M[9] * t0 + M[10] * t1 + M[11] * t2 + M[12] * t3 + M[13] * t4 +
-                        M[14] * t5 + M[15] * t6 + M[16] * t7 + M[17] * t8 + M[18] * t9;
-      acc += c1 * s1 + c2 * s2 + c3 * s3;
-    }
-  } else {
-    while (mm) {
-      const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
-      mm &= mm - 1ull;
-      const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
-      const double s1 = M[0] * x + M[1] * y + M[2] * z;
-      const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
-      acc += c1 * s1 + c2 * s2;
-    }
+    const double s3 = M[9] * (xx * x) + M[10] * (yy * y) + M[11] * (zz * z) + M[12] * (xx * y) +
+                      M[13] * (xx * z) + M[14] * (yy * x) + M[15] * (yy * z) + M[16] * (zz * x) +
+                      M[17] * (zz * y) + M[18] * (xy * z);
+    v += c3 * s3;
   }
-  return acc;
-}
-
-// PT_MOM patches of an item by aligned runs: a run of 2^k patches starting
-// at a multiple of 2^k (within the segment) uses its precomputed moment sum
-// from the block table, so a visible region costs a few evaluations.
-// pbase: segment-local index of the item's first patch (a multiple of the
-// item's patch count, a power of two).
-__device__ __forceinline__ double moment_sum_runs(const double* __restrict__ m0,
-                                                  const double* __restrict__ blk,
-                                                  const uint32_t (&lvl)[MOM_LEVELS], uint32_t pbase,
-                                                  uint64_t mm, const LaneCtx& L, double c1,
-                                                  double c2, double c3) {
-  const double x = L.ex, y = L.ey, z = L.ez;
-  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
-  const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
-               t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
-  double acc = 0.0;
-  while (mm) {
-    // Longest aligned run of set bits at q: limited by the ones that
-    // follow q, by the alignment of the segment-local patch index and by
-    // the block levels.
-    const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(mm));
-    const uint64_t inv = ~(mm >> q);
-    const uint32_t ones = inv ? static_cast<uint32_t>(__builtin_ctzll(inv)) : 64u;
-    const uint32_t pa = pbase + q;
-    uint32_t k = 31u - static_cast<uint32_t>(__builtin_clz(ones));
-    if (pa) k = min(k, static_cast<uint32_t>(__builtin_ctz(pa)));
-    k = min(k, MOM_LEVELS);
-    const uint32_t len = 1u << k;
-    mm &= ~((len == 64u ? ~0ull : ((1ull << len) - 1ull)) << q);
-    // Level offset by selects over the segment's table (held in registers),
-    // not an indexed load: one memory round trip per run instead of two.
-    uint32_t lo = lvl[0];
-#pragma unroll
-    for (uint32_t v = 2; v <= MOM_LEVELS; ++v) lo = k == v ? lvl[v - 1] : lo;
-    const double* M = k == 0 ? m0 + static_cast<size_t>(q) * MOM_STRIDE
-                             : blk + static_cast<size_t>(lo + (pa >> k)) * MOM_STRIDE;
-    const double s1 = M[0] * x + M[1] * y + M[2] * z;
-    const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
-    double v = c1 * s1 + c2 * s2;
-    if (c3 != 0.0) {
-      const double s3 = M[9] * t0 + M[10] * t1 + M[11] * t2 + M[12] * t3 + M[13] * t4 +
-                        M[14] * t5 + M[15] * t6 + M[16] * t7 + M[17] * t8 + M[18] * t9;
-      v += c3 * s3;
-    }
-    acc += v;
-  }
-  return acc;
+  return v;
 }
 
 // Patches split per lane (pmom): the same sums, added only on the lanes of
@@ -564,7 +501,7 @@ template <bool FX, bool ECLS = false>
 __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restrict__ items,
                              const SegDev* __restrict__ segs, const SetDev* __restrict__ sets,
                              const char* __restrict__ recs, const double* __restrict__ moms,
-                             const double* __restrict__ mblk,
+                             const double* __restrict__ imoms,
                              const double* __restrict__ lane_phi, const float4* __restrict__ lane_e,
                              const uint32_t* __restrict__ lane_meta, const uint64_t* __restrict__ lane_masks,
                              double* __restrict__ partials, float4* lp) {
@@ -630,9 +567,7 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
     const double A = star ? P.a0 : (sg.kind == KIND_SPOT ? 0.0 : lin);
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
     const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
-    if (it.mom)
-      sum = moment_sum_runs(mb, mblk + static_cast<size_t>(sg.blk_off) * MOM_STRIDE, sg.lvl_off,
-                            it.e0 / PATCH, it.mom, L, 1.0 - A - B, A + 2.0 * B, -B);
+    if (it.mom) sum = moment_item(imoms + static_cast<size_t>(it.imom) * MOM_STRIDE, L, 1.0 - A - B, A + 2.0 * B, -B);
     if (it.pmom)
       sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
@@ -655,13 +590,13 @@ template <bool FX, bool ECLS>
 __global__ __launch_bounds__(TILE) void flux_items_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
+    const double* __restrict__ moms, const double* __restrict__ imoms, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
   __shared__ float4 pbuf[ECLS ? PBUF_F4 : PATCH];  // one patch of records (one wavefront per workgroup)
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
-  process_item<FX, ECLS>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks, partials,
+  process_item<FX, ECLS>(id, items, segs, sets, recs, moms, imoms, lane_phi, lane_e, lane_meta, lane_masks, partials,
                    pbuf);
 }
 
@@ -670,7 +605,7 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
 __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
     const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ moms, const double* __restrict__ mblk, const double* __restrict__ lane_phi,
+    const double* __restrict__ moms, const double* __restrict__ imoms, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials,
     uint32_t* __restrict__ counter, uint32_t grab) {
@@ -682,7 +617,7 @@ __global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
     if (base >= n_items) break;
     const uint32_t end = base + grab < n_items ? base + grab : n_items;
     for (uint32_t id = base; id < end; ++id)
-      process_item<true>(id, items, segs, sets, recs, moms, mblk, lane_phi, lane_e, lane_meta, lane_masks,
+      process_item<true>(id, items, segs, sets, recs, moms, imoms, lane_phi, lane_e, lane_meta, lane_masks,
                          partials, pbuf);
   }
 }
@@ -1228,19 +1163,19 @@ hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t pers
     const uint32_t nf = d.n_items_plain, ne = d.n_items_ecl, nx = d.n_items - nf - ne;
     if (nf)
       hipLaunchKernelGGL((flux_items_kernel<false, false>), dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs,
-                         d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
+                         d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
     if (ne)
       hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, stream, d.items + nf, ne,
-                         d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                         d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
                          d.partials);
     if (nx)
       hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, stream, d.items + nf + ne, nx,
-                         d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                         d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
                          d.partials);
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
-                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.mblk, d.lane_phi, d.lane_e,
+                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e,
                        d.lane_meta, d.lane_masks, d.partials, d.counter, grab);
   }
   // After the items in stream order: limb chunks add to their rows.

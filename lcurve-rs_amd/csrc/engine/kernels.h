@@ -22,7 +22,7 @@ struct DeviceBatch {
   const uint32_t* tile_cp = nullptr;
   const uint32_t* chunk_info = nullptr;  // use_mask | kind << 16, per chunk
   const double* moms = nullptr;          // MOM_STRIDE per patch
-  const double* mblk = nullptr;          // MOM_STRIDE per aligned run of patches (SegDev::blk_off)
+  const double* imoms = nullptr;         // MOM_STRIDE per item with PT_MOM patches (ItemDev::imom)
   const uint32_t* t_lane0 = nullptr;
   const int* t_ndiv = nullptr;
   const double* t_slfac = nullptr;

@@ -86,14 +86,7 @@ struct SegDev {
   uint32_t flags;     // SEG_POSN
   uint32_t patch_off; // first patch, numbered within the set (host only)
   uint32_t mom_off;   // first patch's moments (global patch index)
-  // Moment sums of aligned runs of 2^k patches, k = 1 .. MOM_LEVELS: level k
-  // holds ceil(np / 2^k) entries (np = n / PATCH), levels back to back from
-  // entry blk_off (MOM_STRIDE doubles each) of the block table.
-  uint32_t blk_off;
-  uint32_t lvl_off[6];  // entry of level k + 1, relative to blk_off
-  uint32_t pad;
 };
-constexpr uint32_t MOM_LEVELS = 6;  // up to 64 patches, one work item
 
 struct ItemDev {
   uint32_t seg;   // segment index (global)
@@ -109,7 +102,9 @@ struct ItemDev {
   // part of it, are summed by a limb chunk, and its other lanes see none of it.
   uint64_t pmom;
   uint32_t lm_off;
-  uint32_t pad;
+  // With mom != 0: entry of the item-moment table holding the sum, in patch
+  // order, of the moments of the patches in mom (MOM_STRIDE doubles).
+  uint32_t imom;
 };
 
 // Limb lanes. For the PT_FREE patches of one segment chunk, the lanes (of
