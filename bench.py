@@ -164,7 +164,8 @@ def pmc_summary(workload, chunk, limb):
 
 def roofline(workload, st, ms_flux, limb):
     pairs = st["pairs_evaluated"]
-    # moment sums as evaluated (aligned runs of patches count once per lane)
+    # moment sums as evaluated, per lane: one per work item for its summed
+    # moment patches, one per patch split per lane
     mom_lanes = st["moment_evals"]
     flop = pairs * FLOP_PER_PAIR + mom_lanes * FLOP_PER_MOMENT_LANE
     tflops = flop / (ms_flux * 1e-3) / 1e12

@@ -71,8 +71,9 @@ typedef struct lch_batch_stats {
   long long pairs_limb;   /* of pairs_evaluated: pairs of limb lanes (lanes that see
                              a no-eclipse patch only in part, regrouped across tiles) */
   long long limb_chunks;  /* workgroups of the limb pass */
-  long long moment_evals; /* moment sums evaluated per launch, counted per lane (a run of
-                             patches summed from the block table counts once) */
+  long long moment_evals; /* moment sums evaluated per launch, counted per lane: one per
+                             work item with moment patches (their summed moments),
+                             one per patch split per lane */
   long long limb_items;   /* limb items (one patch and up to 64 limb lanes each) */
   long long limb_slots;   /* (element, lane) slots the limb pass computes, padding
                              included: pairs_limb / limb_slots is its lane fill */

@@ -86,7 +86,8 @@ class Batch {
   uint64_t n_limb_items() const { return limb_items_.size(); }
   // (element, lane) slots the limb pass computes, padding included.
   uint64_t limb_slots() const { return limb_slots_; }
-  // Moment sums evaluated per launch, per lane (runs and split patches).
+  // Moment sums evaluated per launch, per lane (one per item with moment
+  // patches, one per split patch).
   uint64_t moment_evals() const { return mom_evals_; }
   uint64_t n_items() const { return n_items_; }
   uint64_t n_lanes() const { return n_lanes_; }
