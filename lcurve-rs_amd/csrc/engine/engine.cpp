@@ -434,12 +434,11 @@ void Batch::build_items(uint32_t chunk) {
             const std::vector<uint32_t>& v = plan[q];
             for (size_t i0 = 0; i0 < v.size();) {
               // Up to 64 lanes per item. A remainder of at most 32 (16)
-              // lanes of a segment without eclipse intervals takes a split
-              // item: 2 (4) lane groups holding the same lanes, each
-              // summing a half (quarter) of the patch's elements.
+              // lanes takes a split item: 2 (4) lane groups holding the
+              // same lanes, each summing a half (quarter) of the patch's
+              // elements.
               const size_t rem = v.size() - i0;
-              uint32_t split = 0;
-              if (sg.necl == 0) split = rem <= TILE / 4 ? 2u : (rem <= TILE / 2 ? 1u : 0u);
+              const uint32_t split = rem <= TILE / 4 ? 2u : (rem <= TILE / 2 ? 1u : 0u);
               const size_t gl = TILE >> split;
               R.litems.push_back(LimbItem{e0 + q * PATCH, split});
               for (uint32_t g = 0; g < (1u << split); ++g)
@@ -632,7 +631,7 @@ void Batch::build_items(uint32_t chunk) {
     const uint64_t stride = rec_stride(sg.necl, false);
     for (uint32_t i = lc.item0; i < lc.item0 + lc.n_items; ++i) {
       const LimbItem& li = limb_items_[i];
-      if (li.split > 2 || (li.split && sg.necl != 0)) throw DeviceError("limb item split out of range");
+      if (li.split > 2) throw DeviceError("limb item split out of range");
       if (li.e0 % PATCH || static_cast<uint64_t>(li.e0) + PATCH > sg.n ||
           sg.rec_off + (static_cast<uint64_t>(li.e0) + PATCH) * stride > rec_bytes_)
         throw DeviceError("limb item records out of range");

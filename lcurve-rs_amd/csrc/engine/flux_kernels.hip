@@ -661,14 +661,15 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
   // to the chunk's last item: no branch around the load).
   const uint32_t ilast = iend - 1u;
   uint32_t u_next = d.limb_idx[static_cast<size_t>(i < ilast ? i : ilast) * TILE + lane];
-  // Segments without eclipse intervals: lane j fetches record j of the next
-  // item's patch (one coalesced 1 KB load) while the current patch, staged
-  // in LDS, is summed from broadcast reads.
-  const bool staged = sg.necl == 0;
-  float4 rec_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (staged) {
+  // Lane j fetches the direction and weight of record j of the next item's
+  // patch while the current patch, staged in LDS, is summed from broadcast
+  // reads. Limb lanes come from no-eclipse (patch, tile) pairs only, so no
+  // eclipse test is made here and a segment's intervals are never read.
+  const char* rbase = d.recs + sg.rec_off;
+  float4 rec_next;
+  {
     const uint32_t e = d.limb_items[i < ilast ? i : ilast].e0;
-    rec_next = reinterpret_cast<const float4*>(d.recs + sg.rec_off)[e + lane];
+    rec_next = *reinterpret_cast<const float4*>(rbase + static_cast<uint64_t>(e + lane) * stride);
   }
   for (; i < iend; i += LIMB_WAVES) {
     const uint32_t u = u_next;
@@ -688,22 +689,14 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
     L.ey = ely[uu];
     L.ez = elz[uu];
     L.kx = L.ky = L.k0 = 0.0f;
-    const char* p = d.recs + sg.rec_off + static_cast<uint64_t>(li.e0) * stride;
-    double s;
-    if (staged) {
-      lrec[w][lane] = rec_next;
-      __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+    lrec[w][lane] = rec_next;
+    __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+    {
       const uint32_t e = d.limb_items[inext < ilast ? inext : ilast].e0;
-      rec_next = reinterpret_cast<const float4*>(d.recs + sg.rec_off)[e + lane];
-      s = patch_sum_lds(sg.kind, law.ld, law.mcrit, &lrec[w][base], L, law.P, lin, quad, nel);
-      __builtin_amdgcn_wave_barrier();  // reads done before the next write
-    } else switch (sg.necl) {
-      case 0: s = item_dispatch<0, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
-      case 1: s = item_dispatch<1, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
-      case 2: s = item_dispatch<2, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
-      case 3: s = item_dispatch<3, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
-      default: s = item_dispatch<4, false>(sg.kind, law.ld, 0u, law.mcrit, p, 1ull, 1ull, L, law.P, lin, quad); break;
+      rec_next = *reinterpret_cast<const float4*>(rbase + static_cast<uint64_t>(e + lane) * stride);
     }
+    const double s = patch_sum_lds(sg.kind, law.ld, law.mcrit, &lrec[w][base], L, law.P, lin, quad, nel);
+    __builtin_amdgcn_wave_barrier();  // reads done before the next write
     if (valid) atomicAdd(&acc[u], static_cast<unsigned long long>(__builtin_llrint(s * scale)));
   }
   __syncthreads();

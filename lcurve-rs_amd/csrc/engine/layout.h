@@ -129,7 +129,6 @@ struct LimbChunk {
 struct LimbItem {
   uint32_t e0;     // first element of the patch within the segment
   uint32_t split;  // log2 of the lane groups sharing the patch's elements: 0, 1 or 2
-                   // (1, 2: segments without eclipse intervals only)
 };
 
 // Set flags.
