@@ -529,12 +529,14 @@ void Batch::build_items(uint32_t chunk) {
     return sg.kind <= KIND_STAR2 && (sg.flags & SEG_POSN) != 0;
   };
   const auto mid = std::stable_partition(items.begin(), items.end(), [&](const ItemDev& it) { return !needs_fx(it); });
-  // Of the others, items with element-loop patches of a segment with
-  // 1..LDS_ECL_MAX eclipse intervals go second: their kernel stages those
-  // intervals in LDS and needs more registers than the first one.
+  // Of the others, items with an eclipse-tested element-loop patch of a
+  // segment with 1..LDS_ECL_MAX eclipse intervals go second: their kernel
+  // stages those intervals in LDS and needs more registers than the first
+  // one. (Items whose patches all skip the eclipse test stage records
+  // without intervals in the first kernel.)
   auto staged_ecl = [&](const ItemDev& it) {
     const SegDev& sg = segs_[it.seg];
-    return it.todo != 0 && sg.necl >= 1 && sg.necl <= LDS_ECL_MAX;
+    return (it.todo & ~it.free) != 0 && sg.necl >= 1 && sg.necl <= LDS_ECL_MAX;
   };
   const auto mid2 = std::stable_partition(items.begin(), mid, [&](const ItemDev& it) { return !staged_ecl(it); });
   n_items_plain_ = static_cast<uint64_t>(mid2 - items.begin());

@@ -277,17 +277,22 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
                                            float quad, float4* lp) {
   constexpr uint32_t PB = PATCH * rec_stride(NECL, PK == PK_STARFX);
   double acc = 0.0;
-  if (NECL == 0 && PK != PK_STARFX && lp) {
-    // Records staged in LDS: lane j fetches record j of the next patch (one
-    // coalesced 1 KB load) while the current patch is summed from LDS.
+  if (PK != PK_STARFX && lp && (NECL == 0 || (todo & ~fr) == 0)) {
+    // Records staged in LDS: lane j fetches the direction and weight of
+    // record j of the next patch while the current patch is summed from LDS.
+    // Also for segments with eclipse intervals when every patch of the item
+    // needs no eclipse test (the intervals are then not read).
+    constexpr uint32_t STRIDE = rec_stride(NECL, false);
     const uint32_t lane = __lane_id();
-    const float4* rp = reinterpret_cast<const float4*>(p);
-    float4 nxt = rp[static_cast<size_t>(__builtin_ctzll(todo)) * PATCH + lane];
+    auto rec = [&](uint64_t t) {
+      return *reinterpret_cast<const float4*>(p + (static_cast<size_t>(__builtin_ctzll(t)) * PATCH + lane) * STRIDE);
+    };
+    float4 nxt = rec(todo);
     while (todo) {
       todo &= todo - 1ull;
       lp[lane] = nxt;
       __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
-      if (todo) nxt = rp[static_cast<size_t>(__builtin_ctzll(todo)) * PATCH + lane];
+      if (todo) nxt = rec(todo);
       float s;
       if (PK == PK_STAR) s = star_patch<0, true, LD, X != 0, true>(nullptr, L, P, lp);
       else if (PK == PK_DISC) s = disc_patch<0, true, true>(nullptr, L, lin, quad, lp);
