@@ -211,8 +211,6 @@ void Batch::upload() {
   const uint32_t* u;
   up(lane_meta_, const_cast<uint32_t**>(&u));
   d_.lane_meta = u;
-  up(tile_cp_, const_cast<uint32_t**>(&u));
-  d_.tile_cp = u;
   up(t_lane0_, const_cast<uint32_t**>(&u));
   d_.t_lane0 = u;
   const int* ip;

@@ -736,15 +736,23 @@ __device__ __forceinline__ double gscale2(double phase, double p2, double s21, d
 __device__ __forceinline__ void lane_sums(const SetDev& sp, uint32_t l, const DeviceBatch& d,
                                           double S[5]) {
   const uint32_t t = l / TILE, j = l % TILE;
-  const uint32_t cp = d.tile_cp[sp.lane_off / TILE + t];
   S[0] = S[1] = S[2] = S[3] = S[4] = 0.0;
   const uint64_t row0 = sp.prow_base + static_cast<uint64_t>(t) * sp.n_chunks;
+  // Every row is read: a chunk whose segment the tile does not use has no
+  // item there, and its row stays zero, as do the slots of lanes an item
+  // does not use. Without a branch around them the loads overlap. Each row
+  // goes to its segment kind's sum, in chunk order, through selects (no
+  // indexed register array).
+#pragma unroll 4
   for (uint32_t c = 0; c < sp.n_chunks; ++c) {
     const uint32_t info = d.chunk_info[sp.chunk_off + c];
-    if (info & cp & 0xFFFFu) {
-      const uint32_t kind = info >> 16;
-      S[kind < 5u ? kind : 4u] += d.partials[(row0 + c) * TILE + j];
-    }
+    const double v = d.partials[(row0 + c) * TILE + j];
+    const uint32_t kind = info >> 16;
+    S[0] += kind == 0u ? v : 0.0;
+    S[1] += kind == 1u ? v : 0.0;
+    S[2] += kind == 2u ? v : 0.0;
+    S[3] += kind == 3u ? v : 0.0;
+    S[4] += kind >= 4u ? v : 0.0;
   }
 }
 

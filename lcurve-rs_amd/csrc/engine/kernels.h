@@ -19,7 +19,6 @@ struct DeviceBatch {
   const double* lane_phi = nullptr;
   const float4* lane_e = nullptr;        // earth vector per lane
   const uint32_t* lane_meta = nullptr;
-  const uint32_t* tile_cp = nullptr;
   const uint32_t* chunk_info = nullptr;  // use_mask | kind << 16, per chunk
   const double* moms = nullptr;          // MOM_STRIDE per patch
   const double* imoms = nullptr;         // MOM_STRIDE per item with PT_MOM patches (ItemDev::imom)
