@@ -174,6 +174,9 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
+  hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+  hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
   build_items(ctx_.tuning.chunk);
 }
 
@@ -182,6 +185,10 @@ Batch::~Batch() {
   release_graph();
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
+  if (side_.side) (void)hipStreamSynchronize(side_.side);
+  if (side_.fork) (void)hipEventDestroy(side_.fork);
+  if (side_.join) (void)hipEventDestroy(side_.join);
+  if (side_.side) (void)hipStreamDestroy(side_.side);
   if (d_items_) (void)hipFree(d_items_);
   if (d_imoms_) (void)hipFree(d_imoms_);
   if (d_partials_) (void)hipFree(d_partials_);
@@ -728,7 +735,7 @@ void Batch::launch(int scale) {
       release_graph();
       hipGraph_t g;
       hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      hipError_t e1 = launch_flux_items(d_, T.persistent, pwaves, T.grab, st);
+      hipError_t e1 = launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_);
       hipError_t e2 = launch_assemble(d_, scale, st);
       hipError_t e3 = hipStreamEndCapture(st, &g);
       hip_check(e1, "capture flux kernel");
@@ -741,7 +748,7 @@ void Batch::launch(int scale) {
     }
     hip_check(hipGraphLaunch(graph_, st), "hipGraphLaunch");
   } else {
-    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st), "flux kernel launch");
+    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_), "flux kernel launch");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
   }
 }
@@ -772,7 +779,7 @@ void Batch::time(int scale, int reps, double* ms_flux, double* ms_total) {
   hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
   for (int r = 0; r < reps; ++r) {
     hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
-    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st), "flux kernel launch");
+    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_), "flux kernel launch");
     hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
     hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");

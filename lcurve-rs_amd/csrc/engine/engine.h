@@ -144,6 +144,9 @@ class Batch {
   hipGraphExec_t graph_ = nullptr;
   int graph_scale_ = -1;
   hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+  // Side stream (default priority) and fork / join events for the item
+  // kernels that run beside the plain one.
+  SideStream side_;
 };
 
 }  // namespace lch

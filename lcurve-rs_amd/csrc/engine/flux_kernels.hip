@@ -1160,24 +1160,39 @@ __global__ void visibility_kernel(const char* __restrict__ recs, SegDev sg,
 
 // ------------------------------------------------------------ launchers
 hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
-                             uint32_t grab, hipStream_t stream) {
+                             uint32_t grab, hipStream_t stream, const SideStream* ss) {
   if (d.n_items == 0) return hipSuccess;
   if (!persistent) {
     // Host order: items without beaming / lensing and without staged
     // eclipse intervals, then those with staged intervals, then the
     // beamed / lensed ones.
     const uint32_t nf = d.n_items_plain, ne = d.n_items_ecl, nx = d.n_items - nf - ne;
+    // The three item kernels write disjoint partial rows, so the other two
+    // may run beside the plain one.
+    const bool fork = ss && ss->side && nf && (ne || nx);
+    hipStream_t s2 = fork ? ss->side : stream;
+    if (fork) {
+      hipError_t e = hipEventRecord(ss->fork, stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(ss->side, ss->fork, 0);
+      if (e != hipSuccess) return e;
+    }
     if (nf)
       hipLaunchKernelGGL((flux_items_kernel<false, false>), dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs,
                          d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
     if (ne)
-      hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, stream, d.items + nf, ne,
+      hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, s2, d.items + nf, ne,
                          d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
                          d.partials);
     if (nx)
-      hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, stream, d.items + nf + ne, nx,
+      hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, s2, d.items + nf + ne, nx,
                          d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
                          d.partials);
+    if (fork) {
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(ss->join, ss->side);
+      if (e == hipSuccess) e = hipStreamWaitEvent(stream, ss->join, 0);
+      if (e != hipSuccess) return e;
+    }
   } else {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
     hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
