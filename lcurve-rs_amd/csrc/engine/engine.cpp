@@ -63,6 +63,17 @@ void Batch::dfree_all() {
 
 Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times)
     : ctx_(ctx), n_times_(n_times) {
+  // A failure part-way (an allocation, an upload) frees what was created:
+  // the destructor does not run for a constructor that throws.
+  try {
+    init(std::move(packs));
+  } catch (...) {
+    release_all();
+    throw;
+  }
+}
+
+void Batch::init(std::vector<std::unique_ptr<SetPack>> packs) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   const size_t ns = packs.size();
   host_.resize(ns);
@@ -180,21 +191,32 @@ Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n
   build_items(ctx_.tuning.chunk);
 }
 
-Batch::~Batch() {
+Batch::~Batch() { release_all(); }
+
+void Batch::release_all() {
   (void)hipSetDevice(ctx_.device());
   release_graph();
-  for (hipEvent_t e : ev_)
+  for (hipEvent_t& e : ev_) {
     if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   if (side_.side) (void)hipStreamSynchronize(side_.side);
   if (side_.fork) (void)hipEventDestroy(side_.fork);
   if (side_.join) (void)hipEventDestroy(side_.join);
   if (side_.side) (void)hipStreamDestroy(side_.side);
+  side_ = SideStream{};
   if (d_items_) (void)hipFree(d_items_);
   if (d_imoms_) (void)hipFree(d_imoms_);
   if (d_partials_) (void)hipFree(d_partials_);
   if (d_limb_) (void)hipFree(d_limb_);
   if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
   if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
+  d_items_ = nullptr;
+  d_imoms_ = nullptr;
+  d_partials_ = nullptr;
+  d_limb_ = nullptr;
+  d_.sets = nullptr;
+  d_.chunk_info = nullptr;
   dfree_all();
 }
 

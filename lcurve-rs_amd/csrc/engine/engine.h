@@ -97,6 +97,9 @@ class Batch {
   bool use_graph = true;
 
  private:
+  void init(std::vector<std::unique_ptr<SetPack>> packs);
+  // Frees every device resource held; safe on a partly built batch.
+  void release_all();
   void upload();
   void release_graph();
   template <class T>

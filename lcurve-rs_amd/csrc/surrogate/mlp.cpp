@@ -52,6 +52,18 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
   for (size_t i = 1; i < dims.size(); ++i)
     if (dims[i] <= 0 || dims[i] % 64) throw std::invalid_argument("layer widths must be positive multiples of 64");
   check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  // A failure part-way frees what was allocated: the destructor does not
+  // run for a constructor that throws.
+  try {
+    init(seed);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void Mlp::init(uint64_t seed) {
+  const int max_batch = max_batch_;
   pdims_ = dims_;
   pdims_[0] = 64;
   const int L = n_layers();
@@ -119,14 +131,21 @@ Mlp::Mlp(Context& ctx, const std::vector<int>& dims, uint64_t seed, int max_batc
   check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
 }
 
-Mlp::~Mlp() {
+Mlp::~Mlp() { release(); }
+
+void Mlp::release() {
   (void)hipSetDevice(ctx_.device());
   (void)hipStreamSynchronize(ctx_.stream());
-  for (hipEvent_t e : ev_)
+  for (hipEvent_t& e : ev_) {
     if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   if (X_) (void)hipFree(X_);
   if (Y_) (void)hipFree(Y_);
+  X_ = nullptr;
+  Y_ = nullptr;
   for (void* p : allocs_) (void)hipFree(p);
+  allocs_.clear();
 }
 
 size_t Mlp::param_count() const {

@@ -56,6 +56,8 @@ class Mlp {
   void time_step(int batch, int reps, double* ms_step, double* ms_gemm, double* gemm_flop);
 
  private:
+  void init(uint64_t seed);
+  void release();  // frees every device resource; safe on a partly built network
   void forward(int batch);
   void backward(int batch);
   void gemm(const mlp::GemmArgs& a, bool a_kc, bool b_kc, int epi);

@@ -64,6 +64,16 @@ SurrogateLik::SurrogateLik(Mlp& net, int ndim, const double* center, const doubl
   const size_t xb = static_cast<size_t>(chunk_) * ndim * sizeof(float);
   phi_off_ = (xb + 255) / 256 * 256;
   stage_bytes_ = phi_off_ + static_cast<size_t>(chunk_) * nf_ * sizeof(double);
+  try {
+    init(coef, y, ferr, w);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void SurrogateLik::init(const double* coef, const double* y, const double* ferr, const double* w) {
+  Context& ctx = net_.context();
   auto dalloc = [&](void** p, size_t bytes) { check(hipMalloc(p, std::max<size_t>(bytes, 16)), "hipMalloc"); };
   dalloc(reinterpret_cast<void**>(&coef_), static_cast<size_t>(nf_) * nb_ * sizeof(double));
   dalloc(reinterpret_cast<void**>(&y_), nb_ * sizeof(double));
@@ -85,7 +95,9 @@ SurrogateLik::SurrogateLik(Mlp& net, int ndim, const double* center, const doubl
   check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
-SurrogateLik::~SurrogateLik() {
+SurrogateLik::~SurrogateLik() { release(); }
+
+void SurrogateLik::release() {
   Context& ctx = net_.context();
   (void)hipSetDevice(ctx.device());
   (void)hipStreamSynchronize(ctx.stream());
@@ -93,8 +105,12 @@ SurrogateLik::~SurrogateLik() {
                   static_cast<void*>(w_), static_cast<void*>(curve_), static_cast<void*>(out_),
                   static_cast<void*>(stage_)})
     if (p) (void)hipFree(p);
+  coef_ = y_ = ferr_ = w_ = curve_ = out_ = nullptr;
+  stage_ = nullptr;
   if (hstage_) (void)hipHostFree(hstage_);
   if (hout_) (void)hipHostFree(hout_);
+  hstage_ = nullptr;
+  hout_ = nullptr;
 }
 
 void SurrogateLik::eval(const double* params, int n, double* logl) {

@@ -37,6 +37,9 @@ class SurrogateLik {
   void eval(const double* params, int n, double* logl);
 
  private:
+  // Device buffers and the uploads of coef, y, ferr and w.
+  void init(const double* coef, const double* y, const double* ferr, const double* w);
+  void release();  // frees every buffer; safe on a partly built object
   void check(hipError_t e, const char* what) const;
 
   Mlp& net_;
