@@ -605,6 +605,46 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   return sp;
 }
 
+void scale_linear_records(SetPack& S) {
+  const SetDev& d = S.dev;
+  if (!(d.flags & SET_VALID)) return;
+  for (SegDev& sg : S.segs) {
+    // The kernel's linear form (seg_law): Poly law, a1 = a2 = a3 = 0 and
+    // mucrit <= 0 in the FP32 values it reads; no beaming or lensing.
+    if (sg.kind > KIND_STAR2 || (sg.flags & (SEG_POSN | SEG_LINW))) continue;
+    const bool s1 = sg.kind == KIND_STAR1;
+    if ((s1 ? d.ltype1 : d.ltype2) != 0u) continue;
+    const float* a = s1 ? d.ldc1 : d.ldc2;
+    if (a[1] != 0.0f || a[2] != 0.0f || a[3] != 0.0f || (s1 ? d.mucrit1 : d.mucrit2) > 0.0f) continue;
+    const uint32_t stride = rec_stride(sg.necl, false);
+    uint8_t* base = S.recs.data() + sg.rec_off;
+    if (sg.rec_off + static_cast<uint64_t>(stride) * sg.n > S.recs.size()) throw lc::LcError("segment records out of range");
+    float wmax = 0.0f;
+    bool ok = true;
+    for (uint32_t i = 0; i < sg.n && ok; ++i) {
+      float f[4];
+      std::memcpy(f, base + static_cast<size_t>(i) * stride, sizeof f);
+      ok = f[3] >= 0.0f && std::isfinite(f[3]);
+      if (ok) wmax = std::max(wmax, f[3]);
+    }
+    if (!ok || wmax == 0.0f) continue;
+    int k = 0;
+    std::frexp(std::sqrt(static_cast<double>(wmax)), &k);  // sqrt(wmax) < 2^k
+    if (k < -60 || k > 60) continue;                       // keep 4^k a normal float
+    const double inv = std::ldexp(1.0, -k);
+    for (uint32_t i = 0; i < sg.n; ++i) {
+      uint8_t* r = base + static_cast<size_t>(i) * stride;
+      float f[4];
+      std::memcpy(f, r, sizeof f);
+      const double sc = std::sqrt(static_cast<double>(f[3])) * inv;  // in [0, 1)
+      const float g[4] = {static_cast<float>(f[0] * sc), static_cast<float>(f[1] * sc),
+                          static_cast<float>(f[2] * sc), static_cast<float>(sc)};
+      std::memcpy(r, g, sizeof g);
+    }
+    sg.flags |= SEG_LINW | (static_cast<uint32_t>(k + 128) << 8);
+  }
+}
+
 std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& models,
                                                   const std::vector<lc::Data>& data, int nthreads,
                                                   bool keep_maps) {
@@ -633,6 +673,7 @@ std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& 
         try {
           auto P = lc::prepare_model(models[s], data_of(s));
           packs[s] = pack_set(*P, data_of(s), keep_maps);
+          scale_linear_records(*packs[s]);
         } catch (const lc::LcError& e) {
           packs[s] = invalid_set(e.what(), data_of(s));
         }

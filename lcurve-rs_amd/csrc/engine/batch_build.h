@@ -62,6 +62,9 @@ struct SetPack {
 
 // Pack a prepared model; `data` supplies the observations of this set.
 std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps);
+// Rescales the records of the set's linear-law star segments for the device
+// (SEG_LINW); call after every host table has been made from the records.
+void scale_linear_records(SetPack& S);
 // A set whose setup failed: contributes NaN results, no work.
 std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& data);
 

@@ -34,8 +34,9 @@ namespace {
 
 
 // Limb-darkening forms: LD_POLY / LD_CLARET are LDC::imu (types.rs:101-119);
-// LD_LIN is the Poly law with only the linear coefficient set.
-enum : int { LD_POLY = 0, LD_CLARET = 1, LD_LIN = 2 };
+// LD_LIN is the Poly law with only the linear coefficient set; LD_LINW the
+// same law on a segment whose records are rescaled by sqrt(w) (SEG_LINW).
+enum : int { LD_POLY = 0, LD_CLARET = 1, LD_LIN = 2, LD_LINW = 3 };
 
 // ------------------------------------------------------------ helpers
 struct LaneCtx {
@@ -46,8 +47,9 @@ struct LaneCtx {
 
 struct StarPar {
   float a0, a1, a2, a3;
-  float c0;      // LD_LIN: 1 - a0; LD_CLARET: 1 - (a0 + a1 + a2 + a3)
+  float c0;      // LD_LIN(W): 1 - a0; LD_CLARET: 1 - (a0 + a1 + a2 + a3)
   float mucrit, beam, rlens;
+  float w2;      // LD_LINW: 4^k, the records' weight scale
 };
 
 // Point::visible for NECL intervals (types.rs:50-58), in FP64 so the mask is
@@ -157,6 +159,25 @@ __device__ __forceinline__ float star_patch(const char* __restrict__ p, const La
                                             const double* le = nullptr, uint32_t n = PATCH) {
   constexpr uint32_t STRIDE = rec_stride(NECL, false);
   constexpr bool ECL = NECL > 0 && !FREE;
+  if constexpr (LD == LD_LINW) {
+    // Records (s d, s): v = clamp(s d.e) = s m, and w m (c0 + a0 m) =
+    // 4^k (c0 v s + a0 v^2). No cut-off on these segments.
+    static_assert(!MCRIT, "SEG_LINW segments have no mu cut-off");
+    float s1[2] = {0.0f, 0.0f}, s2[2] = {0.0f, 0.0f};
+#pragma unroll(LDSR ? 1 : 2)
+    for (uint32_t i = 0; i < n; i += 4) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const char* r = p + (i + j) * STRIDE;
+        const float4 d = rec_dirn<NECL, LDSR>(p, i, j, lp);
+        float v = clamp01(mu_of(d, L));
+        if (ECL) v = eclipse_free<NECL>(rec_ecl<NECL, LDSR>(r, i + j, le), L.phif) ? v : 0.0f;
+        s1[j & 1] = fmaf(v, d.w, s1[j & 1]);
+        s2[j & 1] = fmaf(v, v, s2[j & 1]);
+      }
+    }
+    return P.w2 * fmaf(P.c0, s1[0] + s1[1], P.a0 * (s2[0] + s2[1]));
+  }
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll(LDSR ? 1 : 2)
   for (uint32_t i = 0; i < n; i += 4) {
@@ -377,6 +398,7 @@ __device__ __forceinline__ double item_dispatch(uint32_t kind, int ld, uint32_t 
   if (kind <= KIND_STAR2) {
     if (!FX && fx != 0) return __builtin_nan("");  // never sent here: fail loudly
     if (!FX || fx == 0) {
+      if (ld == LD_LINW) return item_sum<PK_STAR, NECL, LD_LINW, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
       if (ld == LD_LIN)
         return mcrit ? item_sum<PK_STAR, NECL, LD_LIN, 1, ECLS>(p, todo, fr, L, P, lin, quad, lp)
                      : item_sum<PK_STAR, NECL, LD_LIN, 0, ECLS>(p, todo, fr, L, P, lin, quad, lp);
@@ -409,6 +431,7 @@ __device__ __forceinline__ float patch_sum_lds(uint32_t kind, int ld, bool mcrit
                                                const LaneCtx& L, const StarPar& P, float lin,
                                                float quad, uint32_t n = PATCH) {
   if (kind <= KIND_STAR2) {
+    if (ld == LD_LINW) return star_patch<0, true, LD_LINW, false, true>(nullptr, L, P, lp, nullptr, n);
     if (ld == LD_LIN)
       return mcrit ? star_patch<0, true, LD_LIN, true, true>(nullptr, L, P, lp, nullptr, n)
                    : star_patch<0, true, LD_LIN, false, true>(nullptr, L, P, lp, nullptr, n);
@@ -491,6 +514,12 @@ __device__ __forceinline__ SegLaw seg_law(const SegDev& sg, const SetDev& sp) {
   if (w.ld == LD_POLY && w.P.a1 == 0.0f && w.P.a2 == 0.0f && w.P.a3 == 0.0f) w.ld = LD_LIN;
   w.P.c0 = w.ld == LD_LIN ? 1.0f - w.P.a0 : 1.0f - (w.P.a0 + w.P.a1 + w.P.a2 + w.P.a3);
   w.mcrit = w.P.mucrit > 0.0f;
+  w.P.w2 = 1.0f;
+  // The host flags only linear-law segments without cut-off.
+  if (sg.flags & SEG_LINW) {
+    w.ld = LD_LINW;
+    w.P.w2 = ldexpf(1.0f, 2 * seg_linw_exp(sg.flags));
+  }
   return w;
 }
 

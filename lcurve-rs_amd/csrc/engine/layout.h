@@ -42,6 +42,17 @@ constexpr int MAX_ECL = 4;   // eclipse pairs per element supported by the kerne
 //   float4 (posn.x, posn.y, posn.z, 0)        only if SEG_POSN is set
 //   NECL x { double ingress, egress, egress - 1 }
 constexpr uint32_t SEG_POSN = 1u;
+// Star segments with the linear law (Poly with only the first coefficient
+// set), no mu cut-off and no beaming or lensing may carry SEG_LINW: their
+// device records hold (s d, s) in place of (d, w), s = sqrt(w) / 2^k, with k
+// in flags bits [8, 16) biased by 128. With v = clamp(s d.e, 0, 1) = s mu
+// (mu in [0, 1]), w mu (c0 + a0 mu) = 4^k (c0 v s + a0 v^2): the weight
+// multiply folds into the dot product. Host tables (moments, patch codes,
+// limb lanes) are made from the plain records before the rescaling.
+constexpr uint32_t SEG_LINW = 2u;
+__host__ __device__ inline int seg_linw_exp(uint32_t flags) {
+  return static_cast<int>((flags >> 8) & 0xffu) - 128;
+}
 
 // Segments with 1..LDS_ECL_MAX eclipse intervals per element have their
 // element-loop patches staged in LDS by a kernel form of their own.
