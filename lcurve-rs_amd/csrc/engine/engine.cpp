@@ -571,15 +571,36 @@ void Batch::build_items(uint32_t chunk) {
   n_items_ = items.size();
   // Item moments: the sum, in patch order, of the moments of an item's
   // PT_MOM patches, so that the kernel evaluates one moment sum per item and
-  // lane for them (FP64 on the host, across the setup threads).
+  // lane for them; then, for its split patches (pmom), one sum per distinct
+  // lane mask (in order of first appearance), whose masks replace the
+  // per-patch ones in place. FP64 on the host, across the setup threads.
   {
     uint64_t n_im = 0;
-    for (ItemDev& it : items) it.imom = it.mom ? static_cast<uint32_t>(n_im++) : 0u;
-    if (n_im > UINT32_MAX) throw DeviceError("too many item moments");
+    for (ItemDev& it : items) {
+      uint32_t g = 0;
+      if (it.pmom) {
+        const uint32_t np = static_cast<uint32_t>(__builtin_popcountll(it.pmom));
+        if (static_cast<uint64_t>(it.lm_off) + np > lane_masks_.size()) throw DeviceError("item lane masks out of range");
+        uint64_t seen[MAX_ITEM_PATCHES];
+        for (uint32_t i = 0; i < np; ++i) {
+          const uint64_t m = lane_masks_[it.lm_off + i];
+          uint32_t k = 0;
+          while (k < g && seen[k] != m) ++k;
+          if (k == g) seen[g++] = m;
+        }
+      }
+      it.ngrp = g;
+      it.pad = 0;
+      const uint64_t ne = (it.mom ? 1u : 0u) + g;
+      it.imom = ne ? static_cast<uint32_t>(n_im) : 0u;
+      n_im += ne;
+      if (n_im > UINT32_MAX) throw DeviceError("too many item moments");
+    }
     item_moms_.assign(std::max<uint64_t>(n_im, 1) * MOM_STRIDE, 0.0);
     for (const ItemDev& it : items)
-      if (it.mom && (it.seg >= segs_.size() ||
-                     static_cast<uint64_t>(segs_[it.seg].mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE))
+      if ((it.mom || it.pmom) &&
+          (it.seg >= segs_.size() ||
+           static_cast<uint64_t>(segs_[it.seg].mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE))
         throw DeviceError("item moments out of range");
     int nthreads = static_cast<int>(std::thread::hardware_concurrency());
     if (const char* env = std::getenv("LCURVE_HIP_THREADS")) nthreads = std::max(1, std::atoi(env));
@@ -592,12 +613,31 @@ void Batch::build_items(uint32_t chunk) {
         for (size_t b; (b = next.fetch_add(blk)) < items.size();)
           for (size_t i = b; i < std::min(items.size(), b + blk); ++i) {
             const ItemDev& it = items[i];
-            if (!it.mom) continue;
+            if (!it.mom && !it.pmom) continue;
             double* o = item_moms_.data() + static_cast<size_t>(it.imom) * MOM_STRIDE;
             const double* m0 = moms_.data() + (static_cast<size_t>(segs_[it.seg].mom_off) + it.e0 / PATCH) * MOM_STRIDE;
-            for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) {
-              const double* a = m0 + static_cast<size_t>(__builtin_ctzll(mm)) * MOM_STRIDE;
-              for (uint32_t c = 0; c < MOM_STRIDE; ++c) o[c] += a[c];
+            auto add = [&](double* dst, uint32_t q) {
+              const double* a = m0 + static_cast<size_t>(q) * MOM_STRIDE;
+              for (uint32_t c = 0; c < MOM_STRIDE; ++c) dst[c] += a[c];
+            };
+            if (it.mom) {
+              for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) add(o, static_cast<uint32_t>(__builtin_ctzll(mm)));
+              o += MOM_STRIDE;
+            }
+            if (it.pmom) {
+              // This item's masks, one per pmom patch in bit order; only
+              // this thread touches them.
+              uint64_t* mk = lane_masks_.data() + it.lm_off;
+              uint64_t gm[MAX_ITEM_PATCHES];
+              uint32_t g = 0, r = 0;
+              for (uint64_t mm = it.pmom; mm; mm &= mm - 1ull, ++r) {
+                const uint64_t m = mk[r];
+                uint32_t k = 0;
+                while (k < g && gm[k] != m) ++k;
+                if (k == g) gm[g++] = m;
+                add(o + static_cast<size_t>(k) * MOM_STRIDE, static_cast<uint32_t>(__builtin_ctzll(mm)));
+              }
+              for (uint32_t k = 0; k < g; ++k) mk[k] = gm[k];
             }
           }
       });
@@ -609,8 +649,8 @@ void Batch::build_items(uint32_t chunk) {
     pairs_eval_ += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
     pairs_mom_ += static_cast<uint64_t>(__builtin_popcountll(it.mom) + __builtin_popcountll(it.pmom)) * PATCH * TILE;
     // Moment evaluations as the kernel does them: one for the item's PT_MOM
-    // patches, one per split patch.
-    mom_evals_ += ((it.mom ? 1u : 0u) + static_cast<uint64_t>(__builtin_popcountll(it.pmom))) * TILE;
+    // patches, one per lane-mask group of its split patches.
+    mom_evals_ += ((it.mom ? 1u : 0u) + static_cast<uint64_t>(it.ngrp)) * TILE;
   }
   pairs_eval_ += pairs_limb_;
   // Host-side bounds check of every item against the buffers it indexes.
@@ -634,8 +674,11 @@ void Batch::build_items(uint32_t chunk) {
       throw DeviceError("item patch masks out of range");
     if ((it.mom || it.pmom) && static_cast<uint64_t>(sg.mom_off) + (it.e0 + it.ne) / PATCH > moms_.size() / MOM_STRIDE)
       throw DeviceError("item moments out of range");
-    if (it.mom && (static_cast<uint64_t>(it.imom) + 1) * MOM_STRIDE > item_moms_.size())
+    if ((it.mom || it.pmom) &&
+        (static_cast<uint64_t>(it.imom) + (it.mom ? 1u : 0u) + it.ngrp) * MOM_STRIDE > item_moms_.size())
       throw DeviceError("item moment entry out of range");
+    if ((it.pmom == 0) != (it.ngrp == 0) || it.ngrp > static_cast<uint32_t>(__builtin_popcountll(it.pmom)))
+      throw DeviceError("item moment groups out of range");
     if ((it.pmom & ~live) || (it.pmom & (it.todo | it.mom)) ||
         static_cast<uint64_t>(it.lm_off) + __builtin_popcountll(it.pmom) > lane_masks_.size())
       throw DeviceError("item lane masks out of range");

@@ -463,23 +463,18 @@ __device__ __forceinline__ double moment_item(const double* __restrict__ M, cons
   return v;
 }
 
-// Patches split per lane (pmom): the same sums, added only on the lanes of
-// each patch's mask (wave-uniform, one per patch in bit order).
-__device__ __forceinline__ double moment_sum_masked(const double* __restrict__ mb, uint64_t pm,
-                                                    const uint64_t* __restrict__ masks,
-                                                    uint32_t lane, const LaneCtx& L, double c1,
-                                                    double c2, double c3) {
+// Patches split per lane (pmom), summed by lane mask: group g's moments
+// (entry M + g * MOM_STRIDE) add only on the lanes of masks[g].
+__device__ __forceinline__ double moment_groups(const double* __restrict__ M, uint32_t ngrp,
+                                                const uint64_t* __restrict__ masks, uint32_t lane,
+                                                const LaneCtx& L, double c1, double c2, double c3) {
   const double x = L.ex, y = L.ey, z = L.ez;
   const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
   const double t0 = xx * x, t1 = yy * y, t2 = zz * z, t3 = xx * y, t4 = xx * z, t5 = yy * x,
                t6 = yy * z, t7 = zz * x, t8 = zz * y, t9 = xy * z;
   double acc = 0.0;
-  uint32_t i = 0;
-  while (pm) {
-    const uint32_t q = static_cast<uint32_t>(__builtin_ctzll(pm));
-    pm &= pm - 1ull;
-    const uint64_t mk = masks[i++];
-    const double* M = mb + static_cast<size_t>(q) * MOM_STRIDE;
+  for (uint32_t g = 0; g < ngrp; ++g, M += MOM_STRIDE) {
+    const uint64_t mk = masks[g];
     const double s1 = M[0] * x + M[1] * y + M[2] * z;
     const double s2 = M[3] * xx + M[4] * yy + M[5] * zz + M[6] * xy + M[7] * xz + M[8] * yz;
     double v = c1 * s1 + c2 * s2;
@@ -600,10 +595,12 @@ __device__ __forceinline__ void process_item(uint32_t id, const ItemDev* __restr
     const bool star = sg.kind <= KIND_STAR2;
     const double A = star ? P.a0 : (sg.kind == KIND_SPOT ? 0.0 : lin);
     const double B = star ? P.a1 : (sg.kind == KIND_SPOT ? 0.0 : quad);
-    const double* mb = moms + static_cast<size_t>(sg.mom_off + it.e0 / PATCH) * MOM_STRIDE;
-    if (it.mom) sum = moment_item(imoms + static_cast<size_t>(it.imom) * MOM_STRIDE, L, 1.0 - A - B, A + 2.0 * B, -B);
-    if (it.pmom)
-      sum += moment_sum_masked(mb, it.pmom, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
+    const double* im = imoms + static_cast<size_t>(it.imom) * MOM_STRIDE;
+    if (it.mom) {
+      sum = moment_item(im, L, 1.0 - A - B, A + 2.0 * B, -B);
+      im += MOM_STRIDE;
+    }
+    if (it.pmom) sum += moment_groups(im, it.ngrp, lane_masks + it.lm_off, lane, L, 1.0 - A - B, A + 2.0 * B, -B);
   }
   if (it.todo) switch (sg.necl) {
     case 0: sum += item_dispatch<0, FX, ECLS>(sg.kind, ld, fx, mcrit, p, it.todo, it.free, L, P, lin, quad, lp); break;

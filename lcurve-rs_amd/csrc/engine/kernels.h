@@ -45,7 +45,7 @@ struct DeviceBatch {
   const LimbItem* limb_items = nullptr;
   const uint16_t* limb_idx = nullptr;    // 64 per limb item
   const uint32_t* limb_lanes = nullptr;  // global lane per chunk-list entry
-  const uint64_t* lane_masks = nullptr;  // per pmom patch of an item
+  const uint64_t* lane_masks = nullptr;  // per pmom group of an item
 };
 
 // Work items: static grid (one workgroup per item) or persistent grid of

@@ -109,13 +109,18 @@ struct ItemDev {
   uint64_t free;  // bit q: that patch needs no eclipse test (PT_FREE)
   uint64_t mom;   // bit q: that patch is evaluated from its moments (PT_MOM)
   // bit q: that patch (PT_FREE) is evaluated from its moments on the lanes
-  // of lane_masks[lm_off + rank of q in pmom] only; its limb lanes, which see
-  // part of it, are summed by a limb chunk, and its other lanes see none of it.
+  // that see all of it only; its limb lanes, which see part of it, are
+  // summed by a limb chunk, and its other lanes see none of it. The pmom
+  // patches fall into ngrp groups by lane mask: group g adds on the lanes of
+  // lane_masks[lm_off + g].
   uint64_t pmom;
   uint32_t lm_off;
-  // With mom != 0: entry of the item-moment table holding the sum, in patch
-  // order, of the moments of the patches in mom (MOM_STRIDE doubles).
+  // First entry of the item-moment table (MOM_STRIDE doubles each): with
+  // mom != 0, the sum in patch order of the moments of the patches in mom;
+  // then one entry per pmom group, the sum in patch order of its patches'.
   uint32_t imom;
+  uint32_t ngrp;  // pmom groups (distinct lane masks), 0 without pmom
+  uint32_t pad;
 };
 
 // Limb lanes. For the PT_FREE patches of one segment chunk, the lanes (of
