@@ -417,7 +417,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-limb", action="store_true",
                     help="evaluate no-eclipse patches on all lanes of a tile (no limb lanes)")
-    ap.add_argument("--limb-max-part", type=int, default=32,
+    ap.add_argument("--limb-max-part", type=int, default=48,
                     help="split a no-eclipse patch per lane when at most this many lanes see it in part")
     ap.add_argument("--parity-sets", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -90,7 +90,7 @@ void lch_ctx_destroy(lch_ctx* ctx);
 int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
 /* Limb lanes for batches built from now on: a no-eclipse (patch, tile)
  * pair is split per lane into moments / limb lanes / nothing when at most
- * max_part (1..64, default 32) of its lanes see the patch in part; 0 turns
+ * max_part (1..64, default 48) of its lanes see the patch in part; 0 turns
  * the split off (all 64 lanes of the tile run the element loop). Results
  * agree to FP32 rounding either way. No counterpart in the reference. */
 int lch_ctx_set_limb(lch_ctx* ctx, int max_part);

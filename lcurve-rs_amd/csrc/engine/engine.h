@@ -26,7 +26,7 @@ struct Tuning {
   uint32_t grab = 4;            // items taken per atomic (persistent)
   // Limb lanes: 0 off; otherwise a no-eclipse (patch, tile) pair is split
   // per lane when at most `limb` of its lanes see the patch in part.
-  int limb = 32;
+  int limb = 48;
   bool tuned = false;
 };
 

@@ -93,7 +93,7 @@ def set_tuning(chunk, persistent=0):
     check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
 
 
-def set_limb(on, max_part=32):
+def set_limb(on, max_part=48):
     """Limb lanes for batches built from now on (default on): no-eclipse
     patches are split per lane into closed-form moments, limb lanes evaluated
     element by element (regrouped across tiles) and lanes that see none of
