@@ -26,12 +26,20 @@
 // one light curve (orchestration.rs:229) and over parameter sets in a batch
 // (orchestration.rs:343).
 //
-// Pinning: the test-model golden curve (test_data/test_lc.dat, C++ lroche),
-// batch == sequential, NaN on an unphysical set, and closed forms (uniform
-// sphere pi r^2 B, linear limb darkening pi r^2 B (1 - u/3), constant-curve
-// smearing, re_scale on an exact multiple). Disc, edge, bright spot, Claret,
-// beaming, lensing, ndiv>1 and pdot/deltat are pinned only by those closed
-// forms and invariants — parity unpinned against the Rust reference itself.
+// Pinning (tests/test_oracle.py, tests/test_oracle_pins.py): the test-model
+// golden curve (test_data/test_lc.dat, C++ lroche), batch == sequential, NaN
+// on an unphysical set, and mathematics that does not restate this file:
+//   star flux and limb darkening  uniform, polynomial and Claret spheres
+//   disc / edge / bright spot     flat and power-law discs, a cylindrical rim,
+//                                 the strip's geometric series and the stream
+//                                 integrated by scipy from L1
+//   beaming                       first-order Doppler derivative of spheres
+//   lensing_mag                   major-image point-lens magnification
+//   ndiv > 1                      convergence to the exposure-window average
+//   phase (pdot, deltat)          Newton inversion; Roemer delay at 0, 1/4, 1/2
+//   logg / rv (setup)             spheres; Roche lobe by quadrature, Eggleton
+//   svd_scale                     numpy.linalg.lstsq
+// None of this runs the Rust reference, which cannot be built here.
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -765,6 +773,29 @@ long long orc_batch_grid(void* h, int set, int grid, double* posn, double* dirn,
     if (flux) flux[i] = pt.flux;
     if (necl) necl[i] = static_cast<int>(pt.eclipse.size());
   }
+  return static_cast<long long>(g->size());
+}
+
+// Point-lens magnification of one star-2 element (flux.rs:390-406), for
+// checks of the function itself against the textbook formula.
+double orc_lensing_mag(const double* posn, const double* earth, double rlens1) {
+  return lensing_mag(Vec3(posn[0], posn[1], posn[2]), Vec3(earth[0], earth[1], earth[2]), rlens1);
+}
+
+// Gravity of each element of one grid (the normalised surface gravity the
+// setup stores, Point::gravity): gravity[n]. Returns the count or -1.
+long long orc_batch_gravity(void* h, int set, int grid, double* gravity) {
+  auto* B = static_cast<OracleBatch*>(h);
+  if (set < 0 || static_cast<size_t>(set) >= B->prep.size() || !B->prep[set]) return -1;
+  const PreparedModel& P = *B->prep[set];
+  const Grid* g = nullptr;
+  switch (grid) {
+    case 0: g = &P.star1f; break;
+    case 2: g = &P.star2f; break;
+    default: return -1;
+  }
+  if (gravity)
+    for (size_t i = 0; i < g->size(); ++i) gravity[i] = (*g)[i].gravity;
   return static_cast<long long>(g->size());
 }
 

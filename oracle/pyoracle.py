@@ -47,6 +47,10 @@ def lib():
         L.orc_batch_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _I]
         L.orc_phases.restype = ctypes.c_int
         L.orc_phases.argtypes = [ctypes.c_char_p, _D, ctypes.c_int, _D]
+        L.orc_lensing_mag.restype = ctypes.c_double
+        L.orc_lensing_mag.argtypes = [_D, _D, ctypes.c_double]
+        L.orc_batch_gravity.restype = ctypes.c_longlong
+        L.orc_batch_gravity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         _lib = L
     return _lib
 
@@ -187,6 +191,16 @@ class OracleBatch:
                              necl.ctypes.data_as(_I))
         return {"posn": posn, "dirn": dirn, "area": area, "flux": flux, "necl": necl}
 
+    def gravity(self, set_index, grid):
+        """Normalised surface gravity (Point::gravity) of grid 0 (star 1,
+        fine) or 2 (star 2, fine)."""
+        n = lib().orc_batch_gravity(self._h, set_index, grid, None)
+        if n < 0:
+            raise ValueError("bad set or grid")
+        g = np.zeros(n)
+        lib().orc_batch_gravity(self._h, set_index, grid, _d(g))
+        return g
+
     def visibility(self, set_index, grid, phases):
         ph = _f(phases)
         ne = ctypes.c_longlong()
@@ -207,3 +221,10 @@ def phases(model_text, times):
     if lib().orc_phases(model_text.encode(), _d(t), len(t), _d(out)) != 0:
         raise ValueError("bad model")
     return out
+
+
+def lensing_mag(posn, earth, rlens1):
+    """lensing_mag (flux.rs:390-406) of one element position and earth vector."""
+    p = _f(posn)
+    e = _f(earth)
+    return lib().orc_lensing_mag(_d(p), _d(e), float(rlens1))
