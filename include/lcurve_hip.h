@@ -5,7 +5,7 @@
  * flux loop behind its light-curve entry points. Every entry point returns
  * an int status (LCH_OK = 0, negative LCH_ERR_* on failure; HIP error codes
  * are never passed through raw) and records a message retrievable with
- * lch_last_error() on the calling thread. Host buffers are caller-owned;
+ * lch_last_error() (per context, and per calling thread). Host buffers are caller-owned;
  * device memory is owned by the context / batch and freed by its destroy
  * call. All pointers are plain host pointers; no HIP or torch types appear.
  *
@@ -24,6 +24,11 @@
  *   lch_model_*        <- Model::from_file / set_param_value / get_param_value
  *                                                 crates/lcurve/src/model.rs:186-532
  *   lch_data_*         <- read_data               crates/lcurve/src/types.rs:229-254
+ *   lch_model_upload   <- flatten_points + upload_model_async
+ *                                                 crates/lcurve-cuda/src/transfer.rs:134-159,
+ *                                                 crates/lcurve-cuda/src/batch.rs:198-231
+ *   lch_flux_eval      <- lcuda_flux_eval         crates/lcurve-cuda/kernels/flux_kernel.cu:534-555
+ *   lch_model_prepare  <- prepare_model (host)    crates/lcurve-cuda/src/batch.rs:233-501
  * The reference's lcuda_malloc / memcpy / stream / host_alloc exports exist
  * because its Rust host manages device buffers; here the library owns all
  * device memory, so they have no counterpart.
@@ -67,7 +72,7 @@ typedef struct lch_batch_stats {
   double setup_seconds;   /* CPU setup + packing wall time */
   double upload_seconds;  /* host-to-device copies and table builds */
   int chunk;              /* elements per work item */
-  int persistent;         /* 1 if the persistent launch form is selected */
+  int persistent;         /* always 0 (one workgroup per work item) */
   long long pairs_limb;   /* of pairs_evaluated: pairs of limb lanes (lanes that see
                              a no-eclipse patch only in part, regrouped across tiles) */
   long long limb_chunks;  /* workgroups of the limb pass */
@@ -80,13 +85,17 @@ typedef struct lch_batch_stats {
 } lch_batch_stats;
 
 int lch_version(void);
-const char* lch_last_error(void);
+/* Message of the last failing call: with a context, the last call made with
+ * that context that failed; with NULL, the last failing call of the calling
+ * thread (the only record for calls that take no context). */
+const char* lch_last_error(const lch_ctx* ctx);
 
 /* ---- device context ---- */
 int lch_device_count(void);
 int lch_ctx_create(int device, lch_ctx** out);
 void lch_ctx_destroy(lch_ctx* ctx);
-/* Override the launch tuning (chunk > 0 and persistent in {0,1}). */
+/* Override the work-item size (chunk > 0 elements); persistent must be 0
+ * (LCH_ERR_UNSUPPORTED otherwise: there is one launch form). */
 int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
 /* Limb lanes for batches built from now on: a no-eclipse (patch, tile)
  * pair is split per lane into moments / limb lanes / nothing when at most
@@ -183,6 +192,99 @@ int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, d
  * query nelem. Needs LCH_BATCH_KEEP_MAPS. */
 int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
                          int nphase, unsigned char* mask, int* nelem);
+
+/* ---- grid-level boundary: the caller's own surface-element grids ----
+ * The reference's GPU backend flattens lcurve's Point grids on the host and
+ * sums the flux over them on the device. These entry points take such grids
+ * directly, in struct-of-arrays form with every eclipse interval in FP64
+ * (CSR), keep them resident in HBM and evaluate comp_light
+ * (crates/lcurve/src/flux.rs:7-111) at the caller's phases. Grid order: */
+#define LCH_GRID_STAR1F 0
+#define LCH_GRID_STAR1C 1 /* n = 0: the coarse grid is the fine grid (the reference's clone, nlat1c == nlat1f) */
+#define LCH_GRID_STAR2F 2
+#define LCH_GRID_STAR2C 3 /* n = 0: likewise (orchestration.rs:72-79) */
+#define LCH_GRID_DISC 4
+#define LCH_GRID_EDGE 5
+#define LCH_GRID_SPOT 6
+/* Optional, read with gint_mode 1 only: the star grids before the opaque-disc
+ * eclipses were appended, which the reference's Ginterp calibration sums use
+ * (orchestration.rs:93-157); all empty = calibrate on the grids above. */
+#define LCH_GRID_STAR1F_CAL 7
+#define LCH_GRID_STAR1C_CAL 8
+#define LCH_GRID_STAR2F_CAL 9
+#define LCH_GRID_STAR2C_CAL 10
+#define LCH_NGRID 11
+
+/* One grid: the fields of Point (types.rs:7-21) as arrays. */
+typedef struct lch_grid_soa {
+  long long n;              /* elements (0: empty grid) */
+  const double* posn;       /* [n][3] positions, units of the binary separation */
+  const double* dirn;       /* [n][3] unit outward normals */
+  const float* area;        /* [n] Point::area; may be NULL (the flux sum does not read it) */
+  const float* gravity;     /* [n] Point::gravity; may be NULL */
+  const float* flux;        /* [n] Point::flux (area times surface brightness) */
+  const long long* ecl_off; /* [n + 1] CSR offsets into ingress / egress, ecl_off[0] = 0;
+                               NULL: no eclipses. At most 4 intervals per element. */
+  const double* ingress;    /* [ecl_off[n]] eclipse interval starts (Point::eclipse .0) */
+  const double* egress;     /* [ecl_off[n]] ends (.1) */
+} lch_grid_soa;
+
+/* comp_light's scalar arguments (flux.rs:7-19) and the Ginterp state
+ * (types.rs:129-202). */
+typedef struct lch_flux_params {
+  double iangle;               /* inclination, degrees */
+  double q;                    /* mass ratio M2 / M1 */
+  int limb1, limb2;            /* limb-darkening law per star: 0 Poly, 1 Claret */
+  double ldc1[4], ldc2[4];     /* its four coefficients per star */
+  double mucrit1, mucrit2;     /* LDC::see cut-offs */
+  double lin_limb_disc, quad_limb_disc;
+  double beam_factor1, beam_factor2;
+  double spin1, spin2;
+  double velocity_scale;       /* km/s; comp_light takes it rounded to f32 */
+  int glens1;                  /* nonzero: star-2 elements lensed by star 1 */
+  double rlens1;               /* lens radius (orchestration.rs:52-58) */
+  double phase1, phase2;       /* Ginterp switch phases */
+  /* 0: the Ginterp factors scale11..scale22 below; 1: found on the GPU from
+   * the fine and coarse grids as light_curve_comp finds them
+   * (orchestration.rs:85-131, comp_star1/2 at flux.rs:151 and 197) */
+  int gint_mode;
+  double scale11, scale12, scale21, scale22;
+} lch_flux_params;
+
+typedef struct lch_grids lch_grids;
+/* Validates and uploads grids[LCH_NGRID] (records sorted into patches of 64
+ * normals, patch moments); LCH_ERR_INVALID for a malformed grid. The grids
+ * stay resident until lch_grids_free. */
+int lch_model_upload(lch_ctx* ctx, const lch_grid_soa* grids, const lch_flux_params* params,
+                     lch_grids** out);
+void lch_grids_free(lch_grids* g);
+/* out[i] = slfac[i] * comp_light(phase[i], expose[i], ndiv[i]) + third for
+ * i < n: phases with the pdot and Roemer terms applied and exposures in
+ * phase units, as light_curve_comp passes them (orchestration.rs:231-251;
+ * the reference's GPU path takes the same host-computed values in
+ * GpuDatum / GpuFudge, batch.rs:460-492). A call with the same phase,
+ * expose and ndiv as the previous call on g reuses its work tables. Not
+ * thread-safe for one g. */
+int lch_flux_eval(lch_ctx* ctx, lch_grids* g, const double* phase, const double* expose, const int* ndiv,
+                  const double* slfac, double third, double* out, int n);
+
+/* The engine's CPU setup of a model (the reference's method: grids,
+ * eclipses, brightness; orchestration.rs:36-179) in the form
+ * lch_model_upload takes, with the per-time terms of `time`
+ * (orchestration.rs:172-241): the host half of the reference's GPU
+ * prepare_model. Needs no device. */
+typedef struct lch_setup lch_setup;
+int lch_model_prepare(const lch_model* m, const double* time, const double* expose, const int* ndiv, int n,
+                      lch_setup** out);
+/* A view of one grid; its pointers stay valid until lch_setup_free. */
+int lch_setup_grid(const lch_setup* s, int grid, lch_grid_soa* view);
+/* The flux parameters (gint_mode 1: the engine calibrates as lch_light_curve
+ * does). */
+int lch_setup_params(const lch_setup* s, lch_flux_params* out);
+/* The per-time terms, arrays of n = the setup's time count; any may be NULL. */
+int lch_setup_times(const lch_setup* s, int n, double* phase, double* expose, int* ndiv, double* slfac,
+                    double* third);
+void lch_setup_free(lch_setup* s);
 
 /* ---- learned surrogate ----
  * No counterpart in the reference. A multilayer perceptron (GELU hidden

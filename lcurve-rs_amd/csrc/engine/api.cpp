@@ -4,16 +4,20 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
 #include "engine.h"
+#include "grid_model.h"
 #include "selfcheck.h"
 #include "../surrogate/mlp.h"
 #include "../surrogate/surr_lik.h"
 
 struct lch_ctx {
   lch::Context impl;
+  std::mutex mu;    // guards err
+  std::string err;  // message of the last failing call made with this context
   explicit lch_ctx(int dev) : impl(dev) {}
 };
 struct lch_model {
@@ -25,6 +29,23 @@ struct lch_mlp {
 struct lch_surr_lik {
   std::unique_ptr<lch::SurrogateLik> impl;
 };
+struct lch_grids {
+  std::unique_ptr<lch::GridModel> impl;
+  int device = 0;
+};
+// CPU setup of one model exported as arrays (lch_model_prepare).
+struct lch_setup {
+  struct Flat {
+    std::vector<double> posn, dirn, ingress, egress;
+    std::vector<float> area, gravity, flux;
+    std::vector<long long> off;
+    long long n = 0;
+  } g[LCH_NGRID];
+  lch_flux_params params{};
+  std::vector<double> phase, expose, slfac;
+  std::vector<int> ndiv;
+  double third = 0.0;
+};
 struct lch_batch {
   std::unique_ptr<lch::Batch> impl;
   double setup_s = 0, upload_s = 0;
@@ -34,11 +55,24 @@ struct lch_batch {
 namespace {
 
 thread_local std::string g_err;
+// The context of the entry point running on this thread, if it takes one.
+thread_local lch_ctx* g_ctx = nullptr;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
+  if (g_ctx) {
+    std::lock_guard<std::mutex> lk(g_ctx->mu);
+    g_ctx->err = msg;
+  }
   return code;
 }
+
+// Makes fail() record the message in `ctx` too for the rest of the call.
+struct CtxScope {
+  lch_ctx* prev;
+  explicit CtxScope(lch_ctx* c) : prev(g_ctx) { g_ctx = c; }
+  ~CtxScope() { g_ctx = prev; }
+};
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -115,7 +149,13 @@ std::vector<lc::Model> apply_sets(const lc::Model& base, const char* const* name
 extern "C" {
 
 int lch_version(void) { return 1; }
-const char* lch_last_error(void) { return g_err.c_str(); }
+const char* lch_last_error(const lch_ctx* ctx) {
+  if (!ctx) return g_err.c_str();
+  // The pointer stays valid until the context's next failing call.
+  lch_ctx* c = const_cast<lch_ctx*>(ctx);
+  std::lock_guard<std::mutex> lk(c->mu);
+  return c->err.c_str();
+}
 
 int lch_device_count(void) {
   int n = 0;
@@ -137,20 +177,22 @@ int lch_ctx_create(int device, lch_ctx** out) {
 void lch_ctx_destroy(lch_ctx* ctx) { delete ctx; }
 
 int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent) {
-  if (!ctx || chunk <= 0 || (persistent != 0 && persistent != 1))
-    return fail(LCH_ERR_INVALID, "bad tuning");
+  CtxScope scope(ctx);
+  if (!ctx || chunk <= 0) return fail(LCH_ERR_INVALID, "bad tuning");
+  if (persistent != 0) return fail(LCH_ERR_UNSUPPORTED, "the persistent launch form was removed; use 0");
   ctx->impl.tuning.chunk = static_cast<uint32_t>(chunk);
-  ctx->impl.tuning.persistent = persistent;
   return LCH_OK;
 }
 
 int lch_ctx_set_limb(lch_ctx* ctx, int max_part) {
+  CtxScope scope(ctx);
   if (!ctx || max_part < 0 || max_part > 64) return fail(LCH_ERR_INVALID, "bad limb setting");
   ctx->impl.tuning.limb = max_part;
   return LCH_OK;
 }
 
 int lch_ctx_sync(lch_ctx* ctx) {
+  CtxScope scope(ctx);
   if (!ctx) return fail(LCH_ERR_INVALID, "null context");
   hipError_t e = hipStreamSynchronize(ctx->impl.stream());
   return e == hipSuccess ? LCH_OK : fail(LCH_ERR_RUNTIME, hipGetErrorString(e));
@@ -265,6 +307,7 @@ int lch_data_read(const char* path, int n, double* time, double* expose, double*
 int lch_light_curve(lch_ctx* ctx, const lch_model* m, const double* time, const double* expose,
                     const double* flux, const double* ferr, const double* weight, const int* ndiv,
                     int n, int scale, int rdata, double* calc_out, lch_lc_result* res) {
+  CtxScope scope(ctx);
   if (!ctx || !m || !calc_out) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     std::vector<lc::Data> data{make_data(time, expose, flux, ferr, weight, ndiv, n)};
@@ -292,6 +335,7 @@ int lch_chisq_batch(lch_ctx* ctx, const lch_model* base, const double* time, con
                     const double* flux, const double* ferr, const double* weight, const int* ndiv,
                     int n, const char* const* names, int ndim, const double* values, int nsets,
                     int scale, double* chisq_out) {
+  CtxScope scope(ctx);
   if (!ctx || !base || (nsets > 0 && !chisq_out)) return fail(LCH_ERR_INVALID, "null argument");
   if (ndim <= 0) return fail(LCH_ERR_INVALID, "ndim must be > 0");
   return guarded([&] {
@@ -330,6 +374,7 @@ int lch_batch_build(lch_ctx* ctx, const lch_model* base, const double* time, con
                     int n, const char* const* names, int ndim, const double* values, int nsets,
                     const double* flux2d, const double* ferr2d, const double* weight2d, int flags,
                     lch_batch** out) {
+  CtxScope scope(ctx);
   if (!ctx || !base || !out) return fail(LCH_ERR_INVALID, "null argument");
   *out = nullptr;
   return guarded([&] {
@@ -346,6 +391,7 @@ int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nse
                            const double* ferr, const double* weight, const int* ndiv, int n,
                            const double* flux2d, const double* ferr2d, const double* weight2d,
                            int flags, lch_batch** out) {
+  CtxScope scope(ctx);
   if (!ctx || !models || !out) return fail(LCH_ERR_INVALID, "null argument");
   *out = nullptr;
   return guarded([&] {
@@ -366,6 +412,7 @@ void lch_batch_free(lch_batch* b) { delete b; }
 
 int lch_batch_set_obs(lch_ctx* ctx, lch_batch* b, const double* flux2d, const double* ferr2d,
                       const double* weight2d) {
+  CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     b->impl->set_obs(flux2d, ferr2d, weight2d);
@@ -374,6 +421,7 @@ int lch_batch_set_obs(lch_ctx* ctx, lch_batch* b, const double* flux2d, const do
 }
 
 int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale) {
+  CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     b->impl->launch(scale ? 1 : 0);
@@ -383,6 +431,7 @@ int lch_batch_launch(lch_ctx* ctx, lch_batch* b, int scale) {
 
 int lch_batch_run(lch_ctx* ctx, lch_batch* b, int scale, double* chisq, double* wnok, double* sfac,
                   double* wdwarf, double* flux) {
+  CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     b->impl->run(scale ? 1 : 0, chisq, wnok, sfac, wdwarf, flux);
@@ -392,6 +441,7 @@ int lch_batch_run(lch_ctx* ctx, lch_batch* b, int scale, double* chisq, double* 
 
 int lch_batch_time(lch_ctx* ctx, lch_batch* b, int scale, int reps, double* ms_flux,
                    double* ms_total) {
+  CtxScope scope(ctx);
   if (!ctx || !b || reps <= 0) return fail(LCH_ERR_INVALID, "bad argument");
   return guarded([&] {
     b->impl->time(scale ? 1 : 0, reps, ms_flux, ms_total);
@@ -400,6 +450,7 @@ int lch_batch_time(lch_ctx* ctx, lch_batch* b, int scale, int reps, double* ms_f
 }
 
 int lch_batch_autotune(lch_ctx* ctx, lch_batch* b, int scale) {
+  CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     b->impl->autotune(scale ? 1 : 0);
@@ -422,7 +473,7 @@ int lch_batch_stats_get(const lch_batch* b, lch_batch_stats* out) {
   out->setup_seconds = b->setup_s;
   out->upload_seconds = b->upload_s;
   out->chunk = static_cast<int>(B.chunk());
-  out->persistent = B.persistent();
+  out->persistent = 0;
   out->pairs_limb = static_cast<long long>(B.pairs_limb());
   out->limb_chunks = static_cast<long long>(B.n_limb_chunks());
   out->moment_evals = static_cast<long long>(B.moment_evals());
@@ -450,6 +501,7 @@ int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, d
 
 int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
                          int nphase, unsigned char* mask, int* nelem) {
+  CtxScope scope(ctx);
   if (!ctx || !b || !nelem) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
     uint32_t n = 0;
@@ -506,6 +558,7 @@ int lch_batch_sfac(const lch_batch* b, double* sfac5) {
 // ---------------------------------------------------------------- surrogate
 int lch_mlp_create(lch_ctx* ctx, const int* dims, int ndims, unsigned long long seed, int max_batch,
                    lch_mlp** out) {
+  CtxScope scope(ctx);
   if (!ctx || !dims || !out || ndims < 2) return fail(LCH_ERR_INVALID, "bad argument");
   return guarded([&] {
     auto m = std::make_unique<lch_mlp>();
@@ -614,3 +667,156 @@ int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* 
     return LCH_OK;
   });
 }
+
+// ------------------------------------------------ grid-level boundary
+int lch_model_upload(lch_ctx* ctx, const lch_grid_soa* grids, const lch_flux_params* params,
+                     lch_grids** out) {
+  CtxScope scope(ctx);
+  if (!ctx || !grids || !params || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto g = std::make_unique<lch_grids>();
+    g->device = ctx->impl.device();
+    g->impl = std::make_unique<lch::GridModel>(ctx->impl, grids, *params);
+    *out = g.release();
+    return LCH_OK;
+  });
+}
+
+void lch_grids_free(lch_grids* g) { delete g; }
+
+int lch_flux_eval(lch_ctx* ctx, lch_grids* g, const double* phase, const double* expose, const int* ndiv,
+                  const double* slfac, double third, double* out, int n) {
+  CtxScope scope(ctx);
+  if (!ctx || !g) return fail(LCH_ERR_INVALID, "null argument");
+  if (g->device != ctx->impl.device()) return fail(LCH_ERR_INVALID, "grids were uploaded on another device");
+  return guarded([&] {
+    g->impl->eval(phase, expose, ndiv, slfac, third, out, n, nullptr);
+    return LCH_OK;
+  });
+}
+
+int lch_model_prepare(const lch_model* m, const double* time, const double* expose, const int* ndiv, int n,
+                      lch_setup** out) {
+  if (!m || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    if (!time || !expose || !ndiv) throw std::invalid_argument("null data array");
+    if (n <= 0) throw std::invalid_argument("no data points");
+    const std::vector<double> zero(static_cast<size_t>(n), 0.0), one(zero.size(), 1.0);
+    const lc::Data data = make_data(time, expose, zero.data(), one.data(), zero.data(), ndiv, n);
+    auto P = lc::prepare_model(m->impl, data);
+    if (P->iscale) throw std::invalid_argument("iscale models are fitted per component; the grid boundary sums comp_light");
+    auto S = std::make_unique<lch_setup>();
+    auto flat = [](const std::vector<lc::Point>& pts, lch_setup::Flat& f) {
+      f.n = static_cast<long long>(pts.size());
+      f.off.assign(pts.size() + 1, 0);
+      for (size_t i = 0; i < pts.size(); ++i) {
+        const lc::Point& p = pts[i];
+        f.posn.insert(f.posn.end(), {p.posn.x, p.posn.y, p.posn.z});
+        f.dirn.insert(f.dirn.end(), {p.dirn.x, p.dirn.y, p.dirn.z});
+        f.area.push_back(p.area);
+        f.gravity.push_back(p.gravity);
+        f.flux.push_back(p.flux);
+        for (const auto& e : p.eclipse) {
+          f.ingress.push_back(e.first);
+          f.egress.push_back(e.second);
+        }
+        f.off[i + 1] = static_cast<long long>(f.ingress.size());
+      }
+    };
+    flat(P->star1f, S->g[LCH_GRID_STAR1F]);
+    if (!P->star1_alias) flat(P->star1c, S->g[LCH_GRID_STAR1C]);
+    flat(P->star2f, S->g[LCH_GRID_STAR2F]);
+    if (!P->star2_alias) flat(P->star2c, S->g[LCH_GRID_STAR2C]);
+    flat(P->disc, S->g[LCH_GRID_DISC]);
+    flat(P->edge, S->g[LCH_GRID_EDGE]);
+    flat(P->spot, S->g[LCH_GRID_SPOT]);
+    if (P->cal_copies) {
+      if (P->need_cal1) {
+        flat(P->star1f_cal, S->g[LCH_GRID_STAR1F_CAL]);
+        flat(P->star1c_cal, S->g[LCH_GRID_STAR1C_CAL]);
+      }
+      if (P->need_cal2) {
+        flat(P->star2f_cal, S->g[LCH_GRID_STAR2F_CAL]);
+        flat(P->star2c_cal, S->g[LCH_GRID_STAR2C_CAL]);
+      }
+    }
+    lch_flux_params& fp = S->params;
+    const lc::Model& md = m->impl;
+    fp.iangle = md.iangle.value;
+    fp.q = md.q.value;
+    auto ld = [](const lc::LDC& l, int* law, double* a, double* mucrit) {
+      *law = l.ltype == lc::LdcType::Claret ? 1 : 0;
+      a[0] = l.ldc1;
+      a[1] = l.ldc2;
+      a[2] = l.ldc3;
+      a[3] = l.ldc4;
+      *mucrit = l.mucrit;
+    };
+    ld(P->ldc1, &fp.limb1, fp.ldc1, &fp.mucrit1);
+    ld(P->ldc2, &fp.limb2, fp.ldc2, &fp.mucrit2);
+    fp.lin_limb_disc = P->lin_limb_disc;
+    fp.quad_limb_disc = P->quad_limb_disc;
+    fp.beam_factor1 = P->beam_factor1;
+    fp.beam_factor2 = P->beam_factor2;
+    fp.spin1 = P->spin1;
+    fp.spin2 = P->spin2;
+    fp.velocity_scale = md.velocity_scale.value;
+    fp.glens1 = P->glens1 ? 1 : 0;
+    fp.rlens1 = P->rlens1;
+    fp.phase1 = P->phase1;
+    fp.phase2 = P->phase2;
+    fp.gint_mode = 1;
+    fp.scale11 = fp.scale12 = fp.scale21 = fp.scale22 = 1.0;
+    S->phase = P->phase;
+    S->expose = P->expose;
+    S->ndiv = P->ndiv;
+    S->slfac = P->slfac;
+    S->third = P->third;
+    *out = S.release();
+    return LCH_OK;
+  });
+}
+
+int lch_setup_grid(const lch_setup* s, int grid, lch_grid_soa* view) {
+  if (!s || !view) return fail(LCH_ERR_INVALID, "null argument");
+  if (grid < 0 || grid >= LCH_NGRID) return fail(LCH_ERR_INVALID, "grid out of range");
+  const lch_setup::Flat& f = s->g[grid];
+  lch_grid_soa v{};
+  v.n = f.n;
+  if (f.n > 0) {
+    v.posn = f.posn.data();
+    v.dirn = f.dirn.data();
+    v.area = f.area.data();
+    v.gravity = f.gravity.data();
+    v.flux = f.flux.data();
+    v.ecl_off = f.off.data();
+    v.ingress = f.ingress.data();
+    v.egress = f.egress.data();
+  }
+  *view = v;
+  return LCH_OK;
+}
+
+int lch_setup_params(const lch_setup* s, lch_flux_params* out) {
+  if (!s || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = s->params;
+  return LCH_OK;
+}
+
+int lch_setup_times(const lch_setup* s, int n, double* phase, double* expose, int* ndiv, double* slfac,
+                    double* third) {
+  if (!s) return fail(LCH_ERR_INVALID, "null argument");
+  if (n != static_cast<int>(s->phase.size())) return fail(LCH_ERR_INVALID, "n differs from the setup's time count");
+  for (int i = 0; i < n; ++i) {
+    if (phase) phase[i] = s->phase[i];
+    if (expose) expose[i] = s->expose[i];
+    if (ndiv) ndiv[i] = s->ndiv[i];
+    if (slfac) slfac[i] = s->slfac[i];
+  }
+  if (third) *third = s->third;
+  return LCH_OK;
+}
+
+void lch_setup_free(lch_setup* s) { delete s; }

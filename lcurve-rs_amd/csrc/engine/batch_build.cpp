@@ -79,11 +79,6 @@ uint64_t direction_key(const lc::Vec3& d) {
   return key;
 }
 
-// Bounding cone of a set of directions: axis and half-angle (pi = no bound).
-struct Cone {
-  float x, y, z, theta;
-};
-
 // Cone around the normalised mean direction holding all `dirs`, with the
 // half-angle rounded up. An empty set or a zero mean gives a full sphere
 // (half-angle pi), which never permits a skip.
@@ -185,7 +180,7 @@ std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& dat
   return sp;
 }
 
-std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps) {
+std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps) {
   auto sp = std::make_unique<SetPack>();
   SetPack& S = *sp;
   SetDev& d = S.dev;
@@ -228,6 +223,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
   d.phase1 = P.phase1;
   d.phase2 = P.phase2;
   d.third = P.third;
+  for (double& g : d.gscl) g = 1.0;
 
   // Which lanes add which grid (class x grid type; see LaneClass).
   const bool cc = P.cal_copies;
@@ -266,7 +262,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
 
   // Records, one segment per (grid, eclipse count).
   uint64_t off = 0;
-  std::vector<Cone> cones;  // normal cone per patch
+  std::vector<Cone>& cones = S.cones;  // normal cone per patch
   for (const GridSpec& g : grids) {
     const std::vector<lc::Point>& pts = *g.pts;
     if (g.id < 7) H.grids[g.id].n = static_cast<uint32_t>(pts.size());
@@ -342,6 +338,28 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     }
   }
 
+  // Direction moments of the patches of moment-capable segments.
+  S.moms.assign(cones.size() * MOM_STRIDE, 0.0);
+  for (const SegDev& sg : S.segs) {
+    if (moment_degree(sg, d) <= 0) continue;
+    const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
+    for (uint32_t q = 0; q < sg.n / PATCH; ++q)
+      patch_moments(S.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride, stride,
+                    &S.moms[static_cast<size_t>(sg.patch_off + q) * MOM_STRIDE]);
+  }
+  return sp;
+}
+
+void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data) {
+  SetDev& d = S.dev;
+  SetHost& H = S.host;
+  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0;
+  const std::vector<Cone>& cones = R.cones;
+  if (R.segs.size() != S.segs.size() || R.cones.size() * MOM_STRIDE != R.moms.size())
+    throw lc::LcError("records and segments out of step");
+  S.lane_phi.clear();
+  S.lane_meta.clear();
+  S.lsplit.clear();
   // Lanes.
   lc::Ginterp gi;
   gi.phase1 = P.phase1;
@@ -357,6 +375,12 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
       lc::exposure_weight(P.phase[t], P.expose[t], nd, P.ndiv[t], &phi, &wgt);
       push_lane(S, phi, CLS_NORMAL, gi);
     }
+  }
+  // The calibration and white-dwarf lanes take a tile of their own, so that
+  // the time lanes' tiles (and their work codes) do not depend on them.
+  while (S.lane_phi.size() % TILE) {
+    S.lane_phi.push_back(0.0);
+    S.lane_meta.push_back(0u);
   }
   d.special0 = static_cast<uint32_t>(S.lane_phi.size());
   if (P.need_cal1) {
@@ -423,7 +447,6 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     lane_en[l] = len > 0.0 ? (1.0 / len) * v : v;
   }
   S.ptcode.assign(cones.size() * ntile, PT_FULL);
-  S.moms.assign(cones.size() * MOM_STRIDE, 0.0);
   for (size_t g = 0; g < S.segs.size(); ++g) {
     const SegDev& sg = S.segs[g];
     const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
@@ -482,7 +505,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
           mx[n] = -4.0f;
           ++n;
         }
-        const uint8_t* pr = S.recs.data() + sg.rec_off + static_cast<uint64_t>(patch - sg.patch_off) * PATCH * stride;
+        const uint8_t* pr = R.recs.data() + sg.rec_off + static_cast<uint64_t>(patch - sg.patch_off) * PATCH * stride;
         for (uint32_t i = 0; i < PATCH; ++i) {
           float f[4];
           std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
@@ -511,8 +534,7 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
       lc::Vec3 pa(pc.x, pc.y, pc.z);
       const double plen = pa.length();
       if (plen > 0.0) pa = (1.0 / plen) * pa;
-      const uint8_t* pr = S.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
-      if (mom_ok) patch_moments(pr, stride, &S.moms[static_cast<size_t>(sg.patch_off + q) * MOM_STRIDE]);
+      const uint8_t* pr = R.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
       for (uint32_t t = 0; t < ntile; ++t) {
         const TileInfo& ti = tiles[t];
         uint8_t& code = S.ptcode[static_cast<size_t>(sg.patch_off + q) * ntile + t];
@@ -602,6 +624,11 @@ std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& da
     S.t_ferr[t] = data[t].ferr;
     S.t_w[t] = data[t].weight;
   }
+  }
+
+std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps) {
+  auto sp = pack_records(P, keep_maps);
+  pack_lanes(*sp, *sp, P, data);
   return sp;
 }
 

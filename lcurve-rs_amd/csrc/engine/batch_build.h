@@ -29,6 +29,11 @@ struct SetHost {
   } grids[7];
 };
 
+// Bounding cone of a set of directions: axis and half-angle (pi = no bound).
+struct Cone {
+  float x, y, z, theta;
+};
+
 // One set, packed on the host, offsets relative to the set.
 struct SetPack {
   SetDev dev{};
@@ -41,6 +46,7 @@ struct SetPack {
   // numbered in segment order (SegDev::patch_off).
   std::vector<uint8_t> ptcode;
   uint32_t n_tiles = 0;
+  std::vector<Cone> cones;   // normal cone per patch
   std::vector<double> moms;  // MOM_STRIDE per patch (zero where unused)
   // Per-lane split of PT_FREE (patch, tile) pairs of moment-capable
   // segments, over the lanes that add the segment: `all` lanes see every
@@ -61,7 +67,16 @@ struct SetPack {
 };
 
 // Pack a prepared model; `data` supplies the observations of this set.
+// pack_records then pack_lanes on the same pack.
 std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps);
+// The part of a pack that does not depend on the phases: flux-path scalars,
+// segments, element records in patch order, patch cones and moments.
+std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps);
+// The part that does: lanes for P's phases / exposures / sub-phase counts,
+// the per-(patch, tile) work codes and lane splits made from the plain
+// (not yet rescaled) records and cones of R, visit counts, per-time tables
+// and the observations of `data`. S carries R's segments (S may be R).
+void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data);
 // Rescales the records of the set's linear-law star segments for the device
 // (SEG_LINW); call after every host table has been made from the records.
 void scale_linear_records(SetPack& S);

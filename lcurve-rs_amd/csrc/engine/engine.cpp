@@ -49,8 +49,9 @@ Context::~Context() {
 // ------------------------------------------------------------------- batch
 template <class T>
 T* Batch::dalloc(size_t n) {
-  void* p = nullptr;
   size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+  if (mem_.arena) return static_cast<T*>(item_alloc(bytes, "batch tables"));
+  void* p = nullptr;
   hip_check(hipMalloc(&p, bytes), "hipMalloc");
   allocs_.push_back(p);
   return static_cast<T*>(p);
@@ -61,8 +62,30 @@ void Batch::dfree_all() {
   allocs_.clear();
 }
 
-Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times)
+void* Batch::item_alloc(size_t bytes, const char* what) {
+  bytes = std::max<size_t>(bytes, 16);
+  if (mem_.arena) {
+    const size_t at = align256(arena_used_);
+    if (at + bytes > mem_.arena_bytes)
+      throw ArenaFull(std::string(what) + ": the call needs more than the " +
+                              std::to_string(mem_.arena_bytes) + " bytes of device memory it was given");
+    arena_used_ = at + bytes;
+    return mem_.arena + at;
+  }
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, bytes), (std::string("hipMalloc ") + what).c_str());
+  return p;
+}
+
+void Batch::item_free(void* p) {
+  if (p && !mem_.arena) (void)hipFree(p);
+}
+
+Batch::Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times,
+             const BatchMemory* mem)
     : ctx_(ctx), n_times_(n_times) {
+  if (mem) mem_ = *mem;
+  if (mem_.arena) use_graph = false;
   // A failure part-way (an allocation, an upload) frees what was created:
   // the destructor does not run for a constructor that throws.
   try {
@@ -93,10 +116,14 @@ void Batch::init(std::vector<std::unique_ptr<SetPack>> packs) {
     nlane += packs[s]->lane_phi.size();
     ntime += packs[s]->t_lane0.size();
   }
+  if (mem_.recs) {
+    if (ns != 1 || !packs[0]->recs.empty()) throw DeviceError("resident records need a one-set batch without records");
+    rec_total = mem_.rec_bytes;
+  }
   rec_bytes_ = rec_total;
   n_lanes_ = nlane;
 
-  char* d_recs = dalloc<char>(rec_total);
+  char* d_recs = mem_.recs ? const_cast<char*>(mem_.recs) : dalloc<char>(rec_total);
   segs_.reserve(nseg);
   lane_phi_.reserve(nlane);
   lane_meta_.reserve(nlane);
@@ -182,12 +209,14 @@ void Batch::init(std::vector<std::unique_ptr<SetPack>> packs) {
     if (sd.flags & SET_VALID) d_.max_set_lanes = std::max(d_.max_set_lanes, sd.special0);
   d_.recs = d_recs;
   upload();
-  hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
-  hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
-  hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
-  hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-  hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
-  hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+  if (!mem_.arena) {
+    hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
+    hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
+    hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
+    hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+  }
   build_items(ctx_.tuning.chunk);
 }
 
@@ -205,12 +234,12 @@ void Batch::release_all() {
   if (side_.join) (void)hipEventDestroy(side_.join);
   if (side_.side) (void)hipStreamDestroy(side_.side);
   side_ = SideStream{};
-  if (d_items_) (void)hipFree(d_items_);
-  if (d_imoms_) (void)hipFree(d_imoms_);
-  if (d_partials_) (void)hipFree(d_partials_);
-  if (d_limb_) (void)hipFree(d_limb_);
-  if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
-  if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
+  item_free(d_items_);
+  item_free(d_imoms_);
+  item_free(d_partials_);
+  item_free(d_limb_);
+  item_free(const_cast<SetDev*>(d_.sets));
+  item_free(const_cast<uint32_t*>(d_.chunk_info));
   d_items_ = nullptr;
   d_imoms_ = nullptr;
   d_partials_ = nullptr;
@@ -265,7 +294,6 @@ void Batch::upload() {
   d_.lane_tot = dalloc<double>(lane_phi_.size());
   d_.flux = dalloc<double>(t_lane0_.size());
   d_.out = dalloc<SetOut>(sets_.size());
-  d_.counter = dalloc<uint32_t>(1);
   // sets_ and chunk_info_ are (re)uploaded by build_items.
 }
 
@@ -281,6 +309,8 @@ void Batch::build_items(uint32_t chunk) {
   // Items cover whole element patches, at most 64 of them.
   chunk = std::max<uint32_t>(PATCH, (chunk + PATCH - 1) / PATCH * PATCH);
   chunk = std::min<uint32_t>(chunk, MAX_ITEM_PATCHES * PATCH);
+  if (mem_.arena && items_built_) throw DeviceError("the work items of a batch in given memory are built once");
+  items_built_ = true;
   // Launches in flight still read the tables replaced below.
   hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
   release_graph();
@@ -716,17 +746,16 @@ void Batch::build_items(uint32_t chunk) {
   if (limb_idx_.size() != limb_items_.size() * TILE) throw DeviceError("limb index table out of step");
   if (n_items_ > 0xFFFFFFFFull / TILE) throw DeviceError("batch too large for one launch");
 
-  if (d_items_) (void)hipFree(d_items_);
-  if (d_imoms_) (void)hipFree(d_imoms_);
-  if (d_partials_) (void)hipFree(d_partials_);
-  if (d_limb_) (void)hipFree(d_limb_);
+  item_free(d_items_);
+  item_free(d_imoms_);
+  item_free(d_partials_);
+  item_free(d_limb_);
   d_items_ = nullptr;
   d_imoms_ = nullptr;
   d_partials_ = nullptr;
   d_limb_ = nullptr;
-  hip_check(hipMalloc(reinterpret_cast<void**>(&d_items_), std::max<size_t>(items.size(), 1) * sizeof(ItemDev)),
-            "hipMalloc items");
-  hip_check(hipMalloc(reinterpret_cast<void**>(&d_imoms_), item_moms_.size() * sizeof(double)), "hipMalloc item moments");
+  d_items_ = static_cast<ItemDev*>(item_alloc(std::max<size_t>(items.size(), 1) * sizeof(ItemDev), "items"));
+  d_imoms_ = static_cast<double*>(item_alloc(item_moms_.size() * sizeof(double), "item moments"));
   hip_check(hipMemcpy(d_imoms_, item_moms_.data(), item_moms_.size() * sizeof(double), hipMemcpyHostToDevice),
             "hipMemcpy item moments");
   d_.imoms = d_imoms_;
@@ -738,7 +767,7 @@ void Batch::build_items(uint32_t chunk) {
     const size_t b_ln = align256(limb_lanes_.size() * sizeof(uint32_t));
     const size_t b_lm = align256(lane_masks_.size() * sizeof(uint64_t));
     const size_t total = std::max<size_t>(b_ch + b_it + b_ix + b_ln + b_lm, 256);
-    hip_check(hipMalloc(&d_limb_, total), "hipMalloc limb tables");
+    d_limb_ = item_alloc(total, "limb tables");
     char* base = static_cast<char*>(d_limb_);
     auto put = [&](const void* src, size_t bytes, size_t at) {
       if (bytes) hip_check(hipMemcpy(base + at, src, bytes, hipMemcpyHostToDevice), "hipMemcpy limb tables");
@@ -755,26 +784,24 @@ void Batch::build_items(uint32_t chunk) {
     d_.limb_lanes = reinterpret_cast<const uint32_t*>(base + b_ch + b_it + b_ix);
     d_.lane_masks = reinterpret_cast<const uint64_t*>(base + b_ch + b_it + b_ix + b_ln);
   }
-  hip_check(hipMalloc(reinterpret_cast<void**>(&d_partials_), std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)),
-            "hipMalloc partials");
+  d_partials_ = static_cast<double*>(item_alloc(std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double), "partials"));
   // Rows of items that are not launched must read as zero; launched items
-  // rewrite theirs on every launch. Cleared here, outside any graph.
+  // rewrite theirs on every launch. Cleared here, outside any graph (given
+  // memory keeps whatever an earlier batch left in it).
   hip_check(hipMemset(d_partials_, 0, std::max<uint64_t>(n_prows_, 1) * TILE * sizeof(double)), "hipMemset partials");
   if (!items.empty())
     hip_check(hipMemcpy(d_items_, items.data(), items.size() * sizeof(ItemDev), hipMemcpyHostToDevice),
               "hipMemcpy items");
   // Per-set table and chunk info (reallocated: sizes change with chunk).
-  SetDev* dsets = nullptr;
-  uint32_t* dci = nullptr;
-  hip_check(hipMalloc(reinterpret_cast<void**>(&dsets), std::max<size_t>(sets_.size(), 1) * sizeof(SetDev)), "hipMalloc");
-  hip_check(hipMalloc(reinterpret_cast<void**>(&dci), std::max<size_t>(chunk_info_.size(), 1) * sizeof(uint32_t)), "hipMalloc");
+  SetDev* dsets = static_cast<SetDev*>(item_alloc(std::max<size_t>(sets_.size(), 1) * sizeof(SetDev), "set table"));
+  uint32_t* dci = static_cast<uint32_t*>(item_alloc(std::max<size_t>(chunk_info_.size(), 1) * sizeof(uint32_t), "chunk table"));
   if (!sets_.empty())
     hip_check(hipMemcpy(dsets, sets_.data(), sets_.size() * sizeof(SetDev), hipMemcpyHostToDevice), "hipMemcpy");
   if (!chunk_info_.empty())
     hip_check(hipMemcpy(dci, chunk_info_.data(), chunk_info_.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
               "hipMemcpy");
-  if (d_.sets) (void)hipFree(const_cast<SetDev*>(d_.sets));
-  if (d_.chunk_info) (void)hipFree(const_cast<uint32_t*>(d_.chunk_info));
+  item_free(const_cast<SetDev*>(d_.sets));
+  item_free(const_cast<uint32_t*>(d_.chunk_info));
   d_.sets = dsets;
   d_.chunk_info = dci;
   d_.items = d_items_;
@@ -792,15 +819,13 @@ uint64_t Batch::visits() const {
 
 void Batch::launch(int scale) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
-  const Tuning& T = ctx_.tuning;
-  const uint32_t pwaves = static_cast<uint32_t>(ctx_.cu_count()) * T.waves_per_cu;
   hipStream_t st = ctx_.stream();
   if (use_graph) {
     if (!graph_ || graph_scale_ != scale) {
       release_graph();
       hipGraph_t g;
       hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      hipError_t e1 = launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_);
+      hipError_t e1 = launch_flux_items(d_, st, &side_);
       hipError_t e2 = launch_assemble(d_, scale, st);
       hipError_t e3 = hipStreamEndCapture(st, &g);
       hip_check(e1, "capture flux kernel");
@@ -813,7 +838,7 @@ void Batch::launch(int scale) {
     }
     hip_check(hipGraphLaunch(graph_, st), "hipGraphLaunch");
   } else {
-    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_), "flux kernel launch");
+    hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
   }
 }
@@ -837,14 +862,12 @@ void Batch::run(int scale, double* chisq, double* wnok, double* sfac, double* wd
 
 void Batch::time(int scale, int reps, double* ms_flux, double* ms_total) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
-  const Tuning& T = ctx_.tuning;
-  const uint32_t pwaves = static_cast<uint32_t>(ctx_.cu_count()) * T.waves_per_cu;
   hipStream_t st = ctx_.stream();
   double tf = 0.0;
   hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
   for (int r = 0; r < reps; ++r) {
     hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
-    hip_check(launch_flux_items(d_, T.persistent, pwaves, T.grab, st, &side_), "flux kernel launch");
+    hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
     hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
     hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
@@ -914,20 +937,35 @@ void Batch::set_obs(const double* flux2d, const double* ferr2d, const double* we
   put(weight2d, t_w_, d_.t_w);
 }
 
+void Batch::set_curve_terms(const double* slfac, double third) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  const size_t n = t_slfac_.size();
+  if (n != sets_.size() * n_times_) throw std::invalid_argument("time tables do not match the set count");
+  if (slfac) {
+    t_slfac_.assign(slfac, slfac + n);
+    hip_check(hipMemcpy(const_cast<double*>(d_.t_slfac), t_slfac_.data(), n * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy polynomial factors");
+  }
+  for (SetDev& sd : sets_) sd.third = third;
+  if (!sets_.empty())
+    hip_check(hipMemcpy(const_cast<SetDev*>(d_.sets), sets_.data(), sets_.size() * sizeof(SetDev), hipMemcpyHostToDevice),
+              "hipMemcpy set table");
+}
+
 void Batch::autotune(int scale) {
+  if (mem_.arena) throw DeviceError("a batch in given memory is not autotuned");
   // Limb lanes at the context's threshold, or off (a small batch may not
   // repay the extra launch).
-  struct Cand { uint32_t chunk; int persistent; bool limb; };
-  const Cand cands[] = {{256, 0, true},  {512, 0, true},  {1024, 0, true},  {2048, 0, true},  {4096, 0, true},
-                        {256, 0, false}, {512, 0, false}, {1024, 0, false}, {2048, 0, false}, {4096, 0, false},
-                        {1024, 1, true}, {2048, 1, true}, {4096, 1, true}};
+  struct Cand { uint32_t chunk; bool limb; };
+  const Cand cands[] = {{256, true},  {512, true},  {1024, true},  {2048, true},  {4096, true},
+                        {256, false}, {512, false}, {1024, false}, {2048, false}, {4096, false}};
   const int limb = ctx_.tuning.limb;
   Tuning best = ctx_.tuning;
   double best_ms = 1e300;
   for (const Cand& c : cands) {
     if (!c.limb && limb == 0) continue;  // already covered
     ctx_.tuning.chunk = c.chunk;
-    ctx_.tuning.persistent = c.persistent;
     ctx_.tuning.limb = c.limb ? limb : 0;
     build_items(c.chunk);
     double ms_f = 0, ms_t = 0;

@@ -18,12 +18,14 @@ struct DeviceError : public std::runtime_error {
   explicit DeviceError(const std::string& m) : std::runtime_error(m) {}
 };
 
+// Raised when a batch's tables do not fit the device memory it was given.
+struct ArenaFull : public std::runtime_error {
+  explicit ArenaFull(const std::string& m) : std::runtime_error(m) {}
+};
+
 // Launch configuration of the flux kernel; chosen by the autotuner.
 struct Tuning {
   uint32_t chunk = 1024;        // elements per work item
-  int persistent = 0;           // 0: one workgroup per item; 1: persistent grid
-  uint32_t waves_per_cu = 8;    // persistent grid size = CUs * waves_per_cu
-  uint32_t grab = 4;            // items taken per atomic (persistent)
   // Limb lanes: 0 off; otherwise a no-eclipse (patch, tile) pair is split
   // per lane when at most `limb` of its lanes see the patch in part.
   int limb = 48;
@@ -45,10 +47,25 @@ class Context {
   hipStream_t stream_ = nullptr;
 };
 
+// Device memory a batch may use instead of allocating its own.
+//  * recs: element records already resident (a batch of one set whose
+//    pack carries no records; its segment offsets are relative to recs).
+//  * arena: a region every other device buffer of the batch is carved
+//    from, so that building and running the batch calls no hipMalloc. Such
+//    a batch creates no events, side stream or graph, and its work items
+//    are built once (no autotune, no rebuild).
+struct BatchMemory {
+  const char* recs = nullptr;
+  uint64_t rec_bytes = 0;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+};
+
 class Batch {
  public:
   // Takes the packed sets (host staging is released while uploading).
-  Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times);
+  Batch(Context& ctx, std::vector<std::unique_ptr<SetPack>> packs, size_t n_times,
+        const BatchMemory* mem = nullptr);
   ~Batch();
   Batch(const Batch&) = delete;
   Batch& operator=(const Batch&) = delete;
@@ -70,6 +87,11 @@ class Batch {
   void autotune(int scale);
   // Replace observations, [n_sets][n_times] each (null keeps the current).
   void set_obs(const double* flux2d, const double* ferr2d, const double* weight2d);
+  // Replace the per-time polynomial factors [n_sets][n_times] and each
+  // set's third light (a one-set batch of uploaded grids).
+  void set_curve_terms(const double* slfac, double third);
+  // Device bytes taken from the arena so far (0 without one).
+  size_t arena_used() const { return arena_used_; }
 
   uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
   // Per-set results of the last run() (empty before the first).
@@ -93,7 +115,6 @@ class Batch {
   uint64_t n_lanes() const { return n_lanes_; }
   size_t n_times() const { return n_times_; }
   uint32_t chunk() const { return chunk_; }
-  int persistent() const { return ctx_.tuning.persistent; }
   bool use_graph = true;
 
  private:
@@ -105,9 +126,16 @@ class Batch {
   template <class T>
   T* dalloc(size_t n);
   void dfree_all();
+  // Device memory of the work-item tables: from the arena when there is
+  // one (released with the batch), else hipMalloc (freed by item_free).
+  void* item_alloc(size_t bytes, const char* what);
+  void item_free(void* p);
 
   Context& ctx_;
   size_t n_times_;
+  BatchMemory mem_;
+  size_t arena_used_ = 0;
+  bool items_built_ = false;
   std::vector<SetHost> host_;
   // Host mirrors (kept small: records are uploaded set by set).
   std::vector<SetDev> sets_;

@@ -631,28 +631,6 @@ __global__ __launch_bounds__(TILE) void flux_items_kernel(
                    pbuf);
 }
 
-// Persistent form: a fixed grid of wavefronts pulls items from a counter.
-// Every wave exits once the counter passes n_items.
-__global__ __launch_bounds__(TILE) void flux_items_persistent_kernel(
-    const ItemDev* __restrict__ items, uint32_t n_items, const SegDev* __restrict__ segs,
-    const SetDev* __restrict__ sets, const char* __restrict__ recs,
-    const double* __restrict__ moms, const double* __restrict__ imoms, const double* __restrict__ lane_phi,
-    const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
-    const uint64_t* __restrict__ lane_masks, double* __restrict__ partials,
-    uint32_t* __restrict__ counter, uint32_t grab) {
-  __shared__ float4 pbuf[PATCH];
-  for (;;) {
-    uint32_t base = 0;
-    if (threadIdx.x == 0) base = atomicAdd(counter, grab);
-    base = __shfl(base, 0);
-    if (base >= n_items) break;
-    const uint32_t end = base + grab < n_items ? base + grab : n_items;
-    for (uint32_t id = base; id < end; ++id)
-      process_item<true>(id, items, segs, sets, recs, moms, imoms, lane_phi, lane_e, lane_meta, lane_masks,
-                         partials, pbuf);
-  }
-}
-
 // Limb lanes: one workgroup of LIMB_WAVES wavefronts per limb chunk. The
 // chunk's limb lanes' earth vectors are staged in LDS (48 KB with the sums
 // and the patch buffers: three workgroups per CU). Each wavefront takes
@@ -740,9 +718,12 @@ __global__ __launch_bounds__(LIMB_WAVES * TILE) void limb_kernel(const DeviceBat
   }
 }
 
-__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
-
 // ------------------------------------------------------------ assembly
+// FP64 from here on without FMA contraction (the reference's operation
+// order), helpers included: the file is built with
+// -ffp-contract=fast-honor-pragmas, which keeps contraction in the FP32
+// flux loops above and honours this pragma for every function below.
+#pragma clang fp contract(off)
 namespace {
 
 // Ginterp::scale1 / scale2 (types.rs:161-184) in FP64.
@@ -953,7 +934,7 @@ __device__ void set_scales(const SetDev& sp, const DeviceBatch& d, double* speci
     // fc22 by scale2 with scale21 set (orchestration.rs:93-131,
     // comp_star1/2 at flux.rs:151 / 197).
     const uint32_t b = sp.lane_off + sp.special0;
-    double s11 = 1.0, s12 = 1.0, s21 = 1.0, s22 = 1.0;
+    double s11 = sp.gscl[0], s12 = sp.gscl[1], s21 = sp.gscl[2], s22 = sp.gscl[3];
     uint32_t k = 0;
     if (cal1) {
       double ff = (1.0 * gscale1(lane_phi[b + 0], sp.phase1, s11, s12)) * special[0] / 1.0;
@@ -1029,7 +1010,7 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
   if (!(sp.flags & SET_VALID)) {
     const double nan = __builtin_nan("");
     for (uint32_t t = tid; t < sp.n_times; t += ABLOCK) flux_out[sp.time_off + t] = nan;
-    if (tid == 0) d.out[blockIdx.x] = SetOut{nan, nan, nan, nan, {nan, nan, nan, nan, nan}};
+    if (tid == 0) d.out[blockIdx.x] = SetOut{nan, nan, nan, nan, {nan, nan, nan, nan, nan}, {nan, nan, nan, nan}};
     return;
   }
   const bool cal1 = (sp.flags & SET_CAL1) != 0, cal2 = (sp.flags & SET_CAL2) != 0;
@@ -1089,7 +1070,8 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
     const double chisq = block_sum(cs, red);
     const double wnok = block_sum(wn, red);
     if (tid == 0)
-      d.out[blockIdx.x] = SetOut{chisq, wnok, sf[0], wdwarf * sf[0], {sf[0], sf[1], sf[2], sf[3], sf[4]}};
+      d.out[blockIdx.x] =
+          SetOut{chisq, wnok, sf[0], wdwarf * sf[0], {sf[0], sf[1], sf[2], sf[3], sf[4]}, {s11, s12, s21, s22}};
     return;
   }
 
@@ -1156,7 +1138,8 @@ __global__ __launch_bounds__(ABLOCK) void assemble_kernel(const DeviceBatch d, i
     chisq = block_sum(cs, red);
     wnok = block_sum(wn, red);
   }
-  if (tid == 0) d.out[blockIdx.x] = SetOut{chisq, wnok, sfac, wdwarf, {sfac, sfac, sfac, sfac, sfac}};
+  if (tid == 0)
+    d.out[blockIdx.x] = SetOut{chisq, wnok, sfac, wdwarf, {sfac, sfac, sfac, sfac, sfac}, {s11, s12, s21, s22}};
 }
 
 // ------------------------------------------------------------ visibility
@@ -1185,45 +1168,37 @@ __global__ void visibility_kernel(const char* __restrict__ recs, SegDev sg,
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
-                             uint32_t grab, hipStream_t stream, const SideStream* ss) {
+hipError_t launch_flux_items(const DeviceBatch& d, hipStream_t stream, const SideStream* ss) {
   if (d.n_items == 0) return hipSuccess;
-  if (!persistent) {
-    // Host order: items without beaming / lensing and without staged
-    // eclipse intervals, then those with staged intervals, then the
-    // beamed / lensed ones.
-    const uint32_t nf = d.n_items_plain, ne = d.n_items_ecl, nx = d.n_items - nf - ne;
-    // The three item kernels write disjoint partial rows, so the other two
-    // may run beside the plain one.
-    const bool fork = ss && ss->side && nf && (ne || nx);
-    hipStream_t s2 = fork ? ss->side : stream;
-    if (fork) {
-      hipError_t e = hipEventRecord(ss->fork, stream);
-      if (e == hipSuccess) e = hipStreamWaitEvent(ss->side, ss->fork, 0);
-      if (e != hipSuccess) return e;
-    }
-    if (nf)
-      hipLaunchKernelGGL((flux_items_kernel<false, false>), dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs,
-                         d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
-    if (ne)
-      hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, s2, d.items + nf, ne,
-                         d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
-                         d.partials);
-    if (nx)
-      hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, s2, d.items + nf + ne, nx,
-                         d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
-                         d.partials);
-    if (fork) {
-      hipError_t e = hipGetLastError();
-      if (e == hipSuccess) e = hipEventRecord(ss->join, ss->side);
-      if (e == hipSuccess) e = hipStreamWaitEvent(stream, ss->join, 0);
-      if (e != hipSuccess) return e;
-    }
-  } else {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d.counter);
-    hipLaunchKernelGGL(flux_items_persistent_kernel, dim3(persistent_waves), dim3(TILE), 0, stream,
-                       d.items, d.n_items, d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e,
-                       d.lane_meta, d.lane_masks, d.partials, d.counter, grab);
+  // Host order: items without beaming / lensing and without staged eclipse
+  // intervals, then those with staged intervals, then the beamed / lensed
+  // ones.
+  const uint32_t nf = d.n_items_plain, ne = d.n_items_ecl, nx = d.n_items - nf - ne;
+  // The three item kernels write disjoint partial rows, so the other two
+  // may run beside the plain one.
+  const bool fork = ss && ss->side && nf && (ne || nx);
+  hipStream_t s2 = fork ? ss->side : stream;
+  if (fork) {
+    hipError_t e = hipEventRecord(ss->fork, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ss->side, ss->fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  if (nf)
+    hipLaunchKernelGGL((flux_items_kernel<false, false>), dim3(nf), dim3(TILE), 0, stream, d.items, nf, d.segs,
+                       d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks, d.partials);
+  if (ne)
+    hipLaunchKernelGGL((flux_items_kernel<false, true>), dim3(ne), dim3(TILE), 0, s2, d.items + nf, ne,
+                       d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                       d.partials);
+  if (nx)
+    hipLaunchKernelGGL((flux_items_kernel<true, false>), dim3(nx), dim3(TILE), 0, s2, d.items + nf + ne, nx,
+                       d.segs, d.sets, d.recs, d.moms, d.imoms, d.lane_phi, d.lane_e, d.lane_meta, d.lane_masks,
+                       d.partials);
+  if (fork) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(ss->join, ss->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, ss->join, 0);
+    if (e != hipSuccess) return e;
   }
   // After the items in stream order: limb chunks add to their rows.
   if (d.n_limb_chunks)

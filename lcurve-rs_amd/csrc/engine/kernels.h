@@ -38,7 +38,6 @@ struct DeviceBatch {
   double* lane_tot = nullptr;
   double* flux = nullptr;
   SetOut* out = nullptr;
-  uint32_t* counter = nullptr;
   // Limb lanes (none when n_limb_chunks == 0).
   uint32_t n_limb_chunks = 0;
   const LimbChunk* limb_chunks = nullptr;
@@ -48,18 +47,16 @@ struct DeviceBatch {
   const uint64_t* lane_masks = nullptr;  // per pmom group of an item
 };
 
-// Work items: static grid (one workgroup per item) or persistent grid of
-// `persistent_waves` wavefronts taking `grab` items per atomic; then the
-// limb chunks, which add to the partial rows the items wrote.
-// With a side stream (static grid only), the plain items run on `stream`
-// and the other item kernels on `side`, forked after and joined before the
-// limb pass by the two events (also inside a graph capture of `stream`).
+// Work items, one workgroup (one wavefront) each; then the limb chunks,
+// which add to the partial rows the items wrote. With a side stream, the
+// plain items run on `stream` and the other item kernels on `side`, forked
+// after and joined before the limb pass by the two events (also inside a
+// graph capture of `stream`).
 struct SideStream {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
 };
-hipError_t launch_flux_items(const DeviceBatch& d, int persistent, uint32_t persistent_waves,
-                             uint32_t grab, hipStream_t stream, const SideStream* ss = nullptr);
+hipError_t launch_flux_items(const DeviceBatch& d, hipStream_t stream, const SideStream* ss = nullptr);
 // FP64 assembly: lane_kernel (per-lane sums, in parallel over sets and
 // lane blocks), then assemble_kernel (per set: scale factors, smearing,
 // autoscale and chi-squared).

@@ -174,12 +174,18 @@ struct SetDev {
   double cosi_rad, sini_rad;  // iangle.to_radians(): comp_star1/2 lanes
   double phase1, phase2;      // Ginterp switch phases
   double third;
+  // Ginterp factors s11, s12, s21, s22 (types.rs:129-202) the assembly
+  // starts from: 1 when the set's calibration lanes find them
+  // (SET_CAL1 / SET_CAL2, orchestration.rs:85-131), or the caller's own
+  // factors for a set uploaded as grids with no calibration lanes.
+  double gscl[4];
 };
 
 // Per-set results written by the assembly kernel.
 struct SetOut {
   double chisq, wnok, scale, wdwarf;
   double sfac[5];  // per-component factors (all equal to scale without iscale)
+  double gint[4];  // Ginterp factors s11, s12, s21, s22 the assembly used
 };
 
 }  // namespace lch

@@ -88,8 +88,9 @@ def _ctx():
 def set_tuning(chunk, persistent=0):
     """Fix the flux-kernel launch form of batches built from now on on the
     current device: `chunk` elements per work item (rounded up to whole
-    64-element patches, at most 4096), `persistent` 0 (one workgroup per
-    item) or 1 (persistent grid). Batch.autotune() overrides it."""
+    64-element patches, at most 4096); `persistent` must be 0 (one
+    workgroup per item, the only launch form). Batch.autotune() overrides
+    it."""
     check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
 
 

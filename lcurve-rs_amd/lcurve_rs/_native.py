@@ -47,10 +47,34 @@ class BatchStatsC(ctypes.Structure):
                 ("limb_items", ctypes.c_longlong), ("limb_slots", ctypes.c_longlong)]
 
 
+LL = ctypes.POINTER(ctypes.c_longlong)
+LCH_NGRID = 11
+GRID_NAMES = ("star1f", "star1c", "star2f", "star2c", "disc", "edge", "spot",
+              "star1f_cal", "star1c_cal", "star2f_cal", "star2c_cal")
+
+
+class GridSoaC(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_longlong), ("posn", D), ("dirn", D), ("area", F), ("gravity", F),
+                ("flux", F), ("ecl_off", LL), ("ingress", D), ("egress", D)]
+
+
+class FluxParamsC(ctypes.Structure):
+    _fields_ = [("iangle", ctypes.c_double), ("q", ctypes.c_double), ("limb1", ctypes.c_int),
+                ("limb2", ctypes.c_int), ("ldc1", ctypes.c_double * 4), ("ldc2", ctypes.c_double * 4),
+                ("mucrit1", ctypes.c_double), ("mucrit2", ctypes.c_double),
+                ("lin_limb_disc", ctypes.c_double), ("quad_limb_disc", ctypes.c_double),
+                ("beam_factor1", ctypes.c_double), ("beam_factor2", ctypes.c_double),
+                ("spin1", ctypes.c_double), ("spin2", ctypes.c_double),
+                ("velocity_scale", ctypes.c_double), ("glens1", ctypes.c_int), ("rlens1", ctypes.c_double),
+                ("phase1", ctypes.c_double), ("phase2", ctypes.c_double), ("gint_mode", ctypes.c_int),
+                ("scale11", ctypes.c_double), ("scale12", ctypes.c_double),
+                ("scale21", ctypes.c_double), ("scale22", ctypes.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "lch_version": (INT, []),
-    "lch_last_error": (CP, []),
+    "lch_last_error": (CP, [VP]),
     "lch_device_count": (INT, []),
     "lch_ctx_create": (INT, [INT, ctypes.POINTER(VP)]),
     "lch_ctx_destroy": (None, [VP]),
@@ -104,6 +128,14 @@ SIGNATURES = {
     "lch_surr_lik_free": (None, [VP]),
     "lch_batch_set_info": (INT, [VP, INT, I, D, D, D, D, CP, INT]),
     "lch_batch_visibility": (INT, [VP, VP, INT, INT, D, INT, ctypes.POINTER(ctypes.c_ubyte), I]),
+    "lch_model_upload": (INT, [VP, ctypes.POINTER(GridSoaC), ctypes.POINTER(FluxParamsC), ctypes.POINTER(VP)]),
+    "lch_grids_free": (None, [VP]),
+    "lch_flux_eval": (INT, [VP, VP, D, D, I, D, ctypes.c_double, D, INT]),
+    "lch_model_prepare": (INT, [VP, D, D, I, INT, ctypes.POINTER(VP)]),
+    "lch_setup_grid": (INT, [VP, INT, ctypes.POINTER(GridSoaC)]),
+    "lch_setup_params": (INT, [VP, ctypes.POINTER(FluxParamsC)]),
+    "lch_setup_times": (INT, [VP, INT, D, D, I, D, D]),
+    "lch_setup_free": (None, [VP]),
 }
 
 _lib = None
@@ -125,8 +157,10 @@ def lib():
     return _lib
 
 
-def last_error():
-    msg = lib().lch_last_error()
+def last_error(ctx=None):
+    """Message of the last failing call of this thread, or of the last
+    failing call made with context handle `ctx`."""
+    msg = lib().lch_last_error(ctx)
     return msg.decode(errors="replace") if msg else ""
 
 

@@ -149,3 +149,40 @@ def test_surrogate_likelihood_feature_count():
         n_poly_features(0, 2)
     with pytest.raises(ValueError):
         n_poly_features(5, 7)
+
+
+def test_setup_export_matches_the_cpu_setup(golden_model_text):
+    """lch_model_prepare (no device): the exported grids are the CPU setup's
+    (the oracle runs the same setup code), eclipse intervals in CSR form,
+    coarse grids present only where the reference keeps separate ones, the
+    phases those of orchestration.rs:231-237 and the flux parameters the
+    model's."""
+    import pyoracle
+    from lcurve_rs import grids as G
+    text = W.model_text({"deltat": 0.002, "pdot": 1e-5, "slope": 0.1, "third": 1e-12}, golden_model_text)
+    t = W.linspace_times(-0.2, 1.2, 40)
+    s = G.prepare(lc.Model.from_text(text), t, 0.002, 3)
+    assert set(s.grids) == {"star1f", "star1c", "star2f", "star2c"}
+    ob = pyoracle.OracleBatch(text, t, 0.002, 3)
+    for name, gi in (("star1f", 0), ("star1c", 1), ("star2f", 2), ("star2c", 3)):
+        o = ob.grid(0, gi)
+        g = s.grids[name]
+        assert np.array_equal(g["posn"], o["posn"]) and np.array_equal(g["dirn"], o["dirn"])
+        assert np.array_equal(g["flux"].astype(np.float64), o["flux"])
+        assert np.array_equal(np.diff(g["ecl_off"]), o["necl"])
+        assert g["ecl_off"][0] == 0 and len(g["ingress"]) == g["ecl_off"][-1] == len(g["egress"])
+        assert np.all(g["egress"] > g["ingress"])
+    assert np.array_equal(s.phase, pyoracle.phases(text, t))
+    assert np.all(s.expose == 0.002) and np.all(s.ndiv == 3) and s.third == 1e-12
+    mid, rng = (t.min() + t.max()) / 2, (t.max() - t.min()) / 2
+    assert np.allclose(s.slfac, 1 + 0.1 * (t - mid) / rng, rtol=1e-15)
+    p = s.params
+    assert p["q"] == 0.5 and p["iangle"] == 82.0 and p["gint_mode"] == 1
+    assert p["limb1"] == 0 and p["ldc1"][:2] == [0.4, 0.0] and p["ldc2"][0] == 0.6
+    # a model whose coarse grids are its fine ones exports no coarse grid
+    s2 = G.prepare(lc.Model.from_text(W.model_text({"nlat1c": 20, "nlat2c": 40}, golden_model_text)), t)
+    assert set(s2.grids) == {"star1f", "star2f"}
+    # iscale models are fitted per component: not this boundary
+    with pytest.raises(ValueError, match="iscale"):
+        G.prepare(lc.Model.from_text(W.model_text({"iscale": 1}, golden_model_text)), t)
+    assert "iscale" in _native.last_error()
