@@ -53,27 +53,38 @@ __device__ double block_sum(double v, double* red) {
   return t;
 }
 
-// One workgroup per row: the autoscale factor, then the chi-squared.
+// One workgroup per row: re_scale (orchestration.rs:361-388) of the row's
+// curve against the data, then -chi^2 / 2. Points with weight <= 0 take no
+// part; without any (or with sum w m^2 / ferr^2 = 0) the reference returns
+// scale 1 and chi-squared 0.
 __global__ __launch_bounds__(256) void chisq_kernel(const double* __restrict__ curve, const double* __restrict__ y,
                                                     const double* __restrict__ ferr, const double* __restrict__ w,
                                                     int nb, double* __restrict__ out) {
   __shared__ double red[4];
   const double* m = curve + static_cast<size_t>(blockIdx.x) * nb;
-  double sdy = 0.0, syy = 0.0;
+  double sdy = 0.0, syy = 0.0, wn = 0.0;
   for (int j = threadIdx.x; j < nb; j += 256) {
-    const double wt = w[j] / (ferr[j] * ferr[j]);
-    sdy += m[j] * (wt * y[j]);
-    syy += m[j] * m[j] * wt;
+    if (w[j] > 0.0) {
+      const double wt = w[j] / (ferr[j] * ferr[j]);
+      sdy += wt * y[j] * m[j];
+      syy += wt * m[j] * m[j];
+      wn += w[j];
+    }
   }
   sdy = block_sum(sdy, red);
   syy = block_sum(syy, red);
-  const double s = syy > 0.0 ? sdy / syy : 1.0;
+  wn = block_sum(wn, red);
   double chi = 0.0;
-  for (int j = threadIdx.x; j < nb; j += 256) {
-    const double r = (y[j] - s * m[j]) / ferr[j];
-    chi += w[j] * r * r;
+  if (wn > 0.0 && syy > 0.0) {
+    const double s = sdy / syy;
+    for (int j = threadIdx.x; j < nb; j += 256) {
+      if (w[j] > 0.0) {
+        const double r = (y[j] - m[j] * s) / ferr[j];
+        chi += w[j] * (r * r);
+      }
+    }
+    chi = block_sum(chi, red);
   }
-  chi = block_sum(chi, red);
   if (threadIdx.x == 0) out[blockIdx.x] = -0.5 * chi;
 }
 

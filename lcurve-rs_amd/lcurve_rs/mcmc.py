@@ -216,16 +216,18 @@ class Surrogate:
     ``frame`` is passed."""
 
     def __init__(self, nbins=256, hidden=(512, 512, 512, 512), seed=1, max_batch=4096, box=BOX, phases=None,
-                 frame=None, times=None, degree=2, tail=False):
+                 frame=None, times=None, degree=2, tail=None):
         self.frame = Frame.from_box(box) if frame is None else frame
         if tail and self.frame.kind != "ball":
             raise ValueError("a Gaussian tail needs a ball frame")
-        # Outside a ball frame's domain the log-likelihood is -inf, or with
-        # ``tail`` the Gaussian of the frame's covariance: the value at the
-        # centre minus half the squared Mahalanobis distance. A positive
-        # surrogate density everywhere keeps delayed acceptance exact on
-        # the whole prior, not only inside the ball.
-        self.tail = bool(tail)
+        # Outside a ball frame's domain the log-likelihood is, with ``tail``
+        # (the default for ball frames), the Gaussian of the frame's
+        # covariance: the value at the centre minus half the squared
+        # Mahalanobis distance. A positive surrogate density everywhere keeps
+        # delayed acceptance exact on the whole prior. Without a tail (box
+        # frames, or tail=False) it is -inf there, and the sampler then
+        # targets the posterior restricted to the surrogate's domain.
+        self.tail = (self.frame.kind == "ball") if tail is None else bool(tail)
         self.degree = int(degree)
         self.times = None if times is None else np.array(times, dtype=np.float64)
         if self.times is not None:
@@ -315,13 +317,20 @@ class Surrogate:
 
 
 def autoscaled_chisq(model_flux, y, ferr, w):
-    """re_scale (orchestration.rs:361-388) on rows of model curves."""
-    wt = w / (ferr * ferr)
-    sdy = (model_flux * (wt * y)).sum(1)
-    syy = (model_flux * model_flux * wt).sum(1)
-    s = np.where(syy > 0, sdy / np.where(syy > 0, syy, 1.0), 1.0)
-    r = (y - s[:, None] * model_flux) / ferr
-    return (w * r * r).sum(1)
+    """re_scale (orchestration.rs:361-388) on rows of model curves: points
+    with weight <= 0 take no part, and a row with no such point or with
+    sum w m^2 / ferr^2 = 0 gets chi-squared 0 (the reference's (1, 0, 0))."""
+    model_flux = np.atleast_2d(model_flux)
+    ok = w > 0
+    yo, eo, wo = y[ok], ferr[ok], w[ok]
+    mo = model_flux[:, ok]
+    wt = wo / (eo * eo)
+    sdy = (wt * yo * mo).sum(1)
+    syy = (wt * mo * mo).sum(1)
+    good = (wo.sum() > 0) & (syy > 0)
+    s = np.where(good, sdy / np.where(good, syy, 1.0), 1.0)
+    r = (yo - mo * s[:, None]) / eo
+    return np.where(good, (wo * (r * r)).sum(1), 0.0)
 
 
 class Target:
@@ -404,7 +413,13 @@ def ensemble_mcmc(target, p0, nsteps, seed=0, surrogate=None, a=2.0, surr_tol=25
     ``surrogate`` is given. Returns dict with chain [nsteps, nwalkers,
     ndim], acceptance counts, exact-evaluation count and wall time.
     ``log`` (a callable) receives a progress line every ``log_every``
-    steps."""
+    steps. With a surrogate, every initial walker must have a finite
+    surrogate log-likelihood (inside its domain, or any point for a
+    surrogate with a tail); a surrogate without a tail confines the chain to
+    its domain, so the chain then samples the posterior restricted to it.
+    Delayed acceptance falls back to exact steps for the rest of the run when
+    the spread of surrogate - exact log-likelihood over the walkers exceeds
+    ``surr_tol`` or is not finite."""
     rng = np.random.default_rng(seed)
     x = np.array(p0, dtype=np.float64)
     nw, nd = x.shape
@@ -415,6 +430,9 @@ def ensemble_mcmc(target, p0, nsteps, seed=0, surrogate=None, a=2.0, surr_tol=25
     t0 = time.perf_counter()
     lp = target.logp_exact(x)
     ls = target.logp_surrogate(surrogate, x) if surrogate is not None else None
+    if ls is not None and not np.all(np.isfinite(ls)):
+        raise ValueError(f"{int(np.sum(~np.isfinite(ls)))} initial walkers have no finite surrogate "
+                         "log-likelihood (outside the surrogate's domain, which has no tail)")
     chain = np.zeros((nsteps, nw, nd))
     acc = 0
     stage1 = 0
@@ -464,7 +482,7 @@ def ensemble_mcmc(target, p0, nsteps, seed=0, surrogate=None, a=2.0, surr_tol=25
             lp[k[ok]] = lpy[ok]
             acc += int(ok.sum())
         chain[step] = x
-        if use_surr and np.all(np.isfinite(lp)) and np.std(ls - lp) > surr_tol:
+        if use_surr and np.all(np.isfinite(lp)) and not (np.std(ls - lp) <= surr_tol):
             use_surr = False
             fallback_at = step
         if log and (step + 1) % log_every == 0:

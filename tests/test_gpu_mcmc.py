@@ -40,8 +40,11 @@ def test_surrogate_learns_a_local_box(mc):
     s.fit(p[500:], lg[500:], epochs=150, batch=512)
     err = s.predict(p[:500]) - lg[:500]
     quad = mc.quad_features(mc.to_unit(p[:500], box)) @ s.coef - lg[:500]
-    assert np.sqrt(np.mean(err ** 2)) < 2e-3
-    assert np.sqrt(np.mean(err ** 2)) <= np.sqrt(np.mean(quad ** 2)) * 1.05
+    rms_net, rms_quad = np.sqrt(np.mean(err ** 2)), np.sqrt(np.mean(quad ** 2))
+    assert rms_net < 2e-3
+    # the network's gain over the method without it: the quadratic baseline
+    # alone leaves at least twice the held-out error
+    assert rms_net <= 0.5 * rms_quad, (rms_net, rms_quad)
 
 
 def test_delayed_acceptance_runs_on_the_exact_likelihood(mc):
@@ -81,13 +84,16 @@ def test_time_output_surrogate_in_a_local_frame(lcurve_rs, mc):
     m = lcurve_rs.Model.from_text(W.model_text(dict(zip(mc.PARAM_NAMES, p[0])), base))
     r = m.light_curve(times=times, expose=expose, ndiv=ndiv)
     assert np.allclose(lg[0], np.log(r.calc / r.wdwarf), rtol=0, atol=1e-12)
-    s = mc.Surrogate(hidden=(256, 256), seed=2, frame=frame, times=times, degree=3)
+    s = mc.Surrogate(hidden=(256, 256), seed=2, frame=frame, times=times, degree=3, tail=False)
     assert s.nout == 320 and s.nbins == 300
     s.fit(p[400:], lg[400:], epochs=150, batch=512)
     err = s.predict(p[:400]) - lg[:400]
     poly = s.baseline(p[:400]) - lg[:400]
-    assert np.sqrt(np.mean(err ** 2)) < 1e-3
-    assert np.sqrt(np.mean(err ** 2)) <= np.sqrt(np.mean(poly ** 2)) * 1.05
+    rms_net, rms_poly = np.sqrt(np.mean(err ** 2)), np.sqrt(np.mean(poly ** 2))
+    assert rms_net < 1e-3
+    # the network's gain: the cubic baseline alone leaves at least 1.5 times
+    # the held-out error
+    assert rms_net <= rms_poly / 1.5, (rms_net, rms_poly)
     tg = mc.Target(base, truth, times, 0.02, seed=1, expose=expose, ndiv=ndiv)
     far = truth + 20.0 * sd
     ls = tg.logp_surrogate(s, np.stack([truth, far]))
@@ -114,14 +120,19 @@ def test_gpu_surrogate_likelihood_matches_host(mc):
     frame = mc.Frame.local(truth + rng.standard_normal((300, 5)) * sd, radius=5.0)
     p = frame.sample(640, 8)
     lg = mc.make_training_set(base, p, times=times, expose=expose, ndiv=ndiv)
-    s = mc.Surrogate(hidden=(128,), seed=4, frame=frame, times=times, degree=3)
+    s = mc.Surrogate(hidden=(128,), seed=4, frame=frame, times=times, degree=3, tail=False)
     assert s.nout == 512
     s.fit(p, lg, epochs=5, batch=128)
     tg = mc.Target(base, truth, times, 0.01, seed=2, expose=expose, ndiv=ndiv)
+    # masked points (re_scale drops weight <= 0), some with zero error
+    tg.w[::7] = 0.0
+    tg.ferr[::14] = 0.0
     q = np.concatenate([frame.sample(299, 9), (truth + 30.0 * sd)[None]])
     g = tg.logp_surrogate(s, q)
     h = tg.logp_surrogate_host(s, q)
     assert np.isinf(g[-1]) and np.isinf(h[-1])
+    # the same masked points in the exact likelihood: finite, same scale
+    assert np.all(np.isfinite(tg.logp_exact(q[:3])))
     assert np.all(np.isfinite(g[:-1])) and np.all(np.isfinite(h[:-1]))
     assert np.allclose(g[:-1], h[:-1], rtol=1e-9, atol=1e-7)
     assert np.all(np.isinf(tg.logp_surrogate(s, q[-1:])))

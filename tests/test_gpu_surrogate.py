@@ -96,3 +96,27 @@ def test_step_timing_reports_gemm_flops(surr):
     assert t["ms_step"] > 0 and t["ms_gemm"] > 0
     # 2*B*(64*512*2 + 3*(512*512*3) + 512*256*3)
     assert t["gemm_flop"] == pytest.approx(2 * 4096 * (64 * 512 * 2 + 3 * 512 * 512 * 3 + 512 * 256 * 3))
+
+
+def test_full_network_matches_oracle(surr):
+    """The configured network, 5-512x4-256 at batch 4096 (the weight
+    gradients then split K = 4096 across slices): forward outputs and
+    gradients against the BF16-rounding oracle and exact FP64
+    backpropagation, at the bars above."""
+    dims = (5, 512, 512, 512, 512, 256)
+    net = surr.Mlp(dims, seed=9, max_batch=4096)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, (4096, 5)).astype(np.float32)
+    t = (np.sin(np.outer(x @ np.array([1.0, -0.5, 0.3, 0.8, -1.2]), np.linspace(0.5, 3, 256)))
+         + 0.1 * rng.standard_normal((4096, 256))).astype(np.float32)
+    p = net.get_params().astype(np.float64)
+    y = net.predict(x)
+    ref, _ = M.forward(p, dims, x, bf16_mode=True)
+    assert np.max(np.abs(y - ref)) <= 2e-3 * np.abs(ref).max()
+    loss, g = net.loss_grad(x, t)
+    lo, go = M.loss_and_grad(p, dims, x, t, bf16_mode=True)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    assert np.linalg.norm(g - go) <= 2e-2 * np.linalg.norm(go)
+    assert np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)) > 0.9999
+    _, ge = M.loss_and_grad(p, dims, x, t, bf16_mode=False)
+    assert np.linalg.norm(g - ge) <= 5e-2 * np.linalg.norm(ge)

@@ -270,3 +270,26 @@ def test_surrogate_gaussian_tail_outside_the_ball():
     assert np.all(tailed[~ins] < l0 - 0.5 * 9.0 + 1e-9)
     with pytest.raises(ValueError):
         mcmc.Surrogate.__init__(mcmc.Surrogate.__new__(mcmc.Surrogate), tail=True)
+
+
+def test_autoscaled_chisq_is_re_scale_with_masked_points(golden_model_text):
+    """mcmc.autoscaled_chisq (the host twin of the GPU surrogate likelihood)
+    follows re_scale (orchestration.rs:361-388) as the oracle restates it:
+    points with weight <= 0 take no part even with ferr = 0 or a negative
+    weight, and with no usable point the chi-squared is 0."""
+    import pyoracle
+    from lcurve_rs import workloads as W
+    t = W.linspace_times(-0.2, 1.2, 90)
+    base, _ = pyoracle.light_curve(golden_model_text, t)
+    rng = np.random.default_rng(5)
+    y = 1.7 * base * (1 + 0.01 * rng.standard_normal(len(t)))
+    ferr = 0.01 * np.abs(y)
+    w = np.ones(len(t))
+    w[::5] = 0.0
+    ferr[::10] = 0.0          # zero error on points of zero weight
+    w[3] = -1.0               # a negative weight is excluded too
+    _, sc = pyoracle.light_curve(golden_model_text, t, flux=y, ferr=ferr, weight=w, scale=True, rdata=True)
+    got = mcmc.autoscaled_chisq(base[None, :], y, ferr, w)[0]
+    assert np.isfinite(got) and got == pytest.approx(sc["chisq"], rel=1e-12)
+    assert mcmc.autoscaled_chisq(base[None, :], y, ferr, np.zeros(len(t)))[0] == 0.0
+    assert mcmc.autoscaled_chisq(np.zeros((1, len(t))), y, ferr, w)[0] == 0.0
