@@ -326,9 +326,11 @@ int lch_mlp_time_step(lch_mlp* m, int batch, int reps, double* ms_step, double* 
  * sum_f phi_f(u) coef[f][j] (phi: 1, then the monomials of each degree up
  * to `degree` in lexicographic index order; coef [nfeat x nbins]
  * row-major, nfeat from lch_surr_lik_n_features), plus y_std times the
- * network's output j, exponentiated; then wt = w / ferr^2,
- * s = sum(m wt y) / sum(m^2 wt) (1 if not positive) and
- * logl = -0.5 sum(w ((y - s m) / ferr)^2), all FP64 but the network.
+ * network's output j, exponentiated; then re_scale over the points with
+ * w > 0: wt = w / ferr^2, s = sum(wt y m) / sum(wt m^2) and
+ * logl = -0.5 sum(w ((y - s m) / ferr)^2), or logl = 0 when no point has
+ * w > 0 or sum(wt m^2) = 0 (the reference's (1, 0, 0)), all FP64 but the
+ * network. ferr must be positive at every point with w > 0.
  * The network must outlive the likelihood and have in == ndim and
  * out >= nbins; training it afterwards changes the likelihood with it.
  * Not thread-safe. */

@@ -51,8 +51,9 @@ SurrogateLik::SurrogateLik(Mlp& net, int ndim, const double* center, const doubl
   if (ndim <= 0 || ndim != net.in_dim()) throw std::invalid_argument("ndim must equal the network's input width");
   if (degree < 0 || degree > 6) throw std::invalid_argument("degree must be in 0..6");
   if (nbins <= 0 || nbins > net.out_dim()) throw std::invalid_argument("nbins must be in 1..network output width");
+  // Points of weight <= 0 take no part in re_scale, whatever their error.
   for (int j = 0; j < nbins; ++j)
-    if (!(ferr[j] > 0.0)) throw std::invalid_argument("ferr must be positive");
+    if (w[j] > 0.0 && !(ferr[j] > 0.0)) throw std::invalid_argument("ferr must be positive where the weight is");
   mono_ = all_monomials(ndim, degree);
   nf_ = 1 + static_cast<int>(mono_.size());
   if (nf_ > 768) throw std::invalid_argument("too many polynomial features");

@@ -31,20 +31,25 @@ def test_training_curves_equal_light_curves(lcurve_rs, mc):
 
 
 def test_surrogate_learns_a_local_box(mc):
+    """In a box of +-5% about the centre the quadratic baseline misses
+    structure the network learns: the network cuts the quadratic's held-out
+    error at least fourfold (measured: 8.8e-3 -> 9.5e-4). In much smaller
+    boxes what the polynomial leaves is the surface grids' discretisation
+    noise, which no smooth regressor can learn."""
     base = mc.surrogate_base_text(NLAT)
     center = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
-    box = mc.local_box(center, center * 0.005, 1.0)
+    box = mc.local_box(center, center * 0.05, 1.0)
     p = mc.sample_box(3000, 5, box)
     lg = mc.make_training_set(base, p, 128)
     s = mc.Surrogate(128, hidden=(256, 256), seed=2, box=box)
-    s.fit(p[500:], lg[500:], epochs=150, batch=512)
+    s.fit(p[500:], lg[500:], epochs=400, batch=512)
     err = s.predict(p[:500]) - lg[:500]
     quad = mc.quad_features(mc.to_unit(p[:500], box)) @ s.coef - lg[:500]
     rms_net, rms_quad = np.sqrt(np.mean(err ** 2)), np.sqrt(np.mean(quad ** 2))
-    assert rms_net < 2e-3
+    assert rms_net < 1.5e-3
     # the network's gain over the method without it: the quadratic baseline
-    # alone leaves at least twice the held-out error
-    assert rms_net <= 0.5 * rms_quad, (rms_net, rms_quad)
+    # alone leaves at least four times the held-out error
+    assert rms_net <= rms_quad / 4.0, (rms_net, rms_quad)
 
 
 def test_delayed_acceptance_runs_on_the_exact_likelihood(mc):
@@ -78,7 +83,7 @@ def test_time_output_surrogate_in_a_local_frame(lcurve_rs, mc):
     corr = np.eye(5)
     corr[0, 1] = corr[1, 0] = -0.8
     pilot = rng.multivariate_normal(truth, corr * np.outer(sd, sd), size=400)
-    frame = mc.Frame.local(pilot, radius=5.0)
+    frame = mc.Frame.local(pilot, radius=20.0)
     p = frame.sample(2500, 6)
     lg = mc.make_training_set(base, p, times=times, expose=expose, ndiv=ndiv)
     m = lcurve_rs.Model.from_text(W.model_text(dict(zip(mc.PARAM_NAMES, p[0])), base))
@@ -86,14 +91,15 @@ def test_time_output_surrogate_in_a_local_frame(lcurve_rs, mc):
     assert np.allclose(lg[0], np.log(r.calc / r.wdwarf), rtol=0, atol=1e-12)
     s = mc.Surrogate(hidden=(256, 256), seed=2, frame=frame, times=times, degree=3, tail=False)
     assert s.nout == 320 and s.nbins == 300
-    s.fit(p[400:], lg[400:], epochs=150, batch=512)
+    s.fit(p[400:], lg[400:], epochs=400, batch=512, lr=3e-3)
     err = s.predict(p[:400]) - lg[:400]
     poly = s.baseline(p[:400]) - lg[:400]
     rms_net, rms_poly = np.sqrt(np.mean(err ** 2)), np.sqrt(np.mean(poly ** 2))
     assert rms_net < 1e-3
-    # the network's gain: the cubic baseline alone leaves at least 1.5 times
-    # the held-out error
-    assert rms_net <= rms_poly / 1.5, (rms_net, rms_poly)
+    # the network's gain: over a ball of 20 pilot deviations the cubic
+    # baseline alone leaves at least 1.6 times the held-out error (measured:
+    # 1.58e-3 -> 7.5e-4)
+    assert rms_net <= rms_poly / 1.6, (rms_net, rms_poly)
     tg = mc.Target(base, truth, times, 0.02, seed=1, expose=expose, ndiv=ndiv)
     far = truth + 20.0 * sd
     ls = tg.logp_surrogate(s, np.stack([truth, far]))
