@@ -34,78 +34,25 @@ uint32_t mask_of(uint32_t cls, std::initializer_list<uint32_t> ptypes) {
 
 const std::initializer_list<uint32_t> kAll = {1u, 2u, 3u};
 
-constexpr double kConeMargin = 2e-3;
 
-struct GridSpec {
-  uint32_t id;  // 0..6 visible grids, 7.. calibration copies
-  const std::vector<lc::Point>* pts;
-  uint32_t kind;
-  uint32_t use;
-  bool posn;
-};
-
-// Position of a direction along a Hilbert curve over its octahedral map
-// (16 bits per axis). Consecutive keys are close on the sphere, so runs of
-// 64 elements have narrow normal cones.
-uint64_t direction_key(const lc::Vec3& d) {
-  double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
-  double s = ax + ay + az;
-  if (!(s > 0.0)) return 0;
-  double u = d.x / s, v = d.y / s;
-  if (d.z < 0.0) {
-    double uu = (1.0 - std::fabs(v)) * (u >= 0.0 ? 1.0 : -1.0);
-    double vv = (1.0 - std::fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
-    u = uu;
-    v = vv;
+// Per-segment constants of the code decision (CodeConsts), with the cone
+// margin of 2e-3 rad.
+CodeConsts code_consts(bool cull, double thr, bool front_ok, double thr_front) {
+  constexpr double margin = 2e-3;
+  CodeConsts k{1.0, 0.0, 2.0, 1.0, 0.0, 2.0};
+  if (cull) {
+    const double a = std::acos(thr) + margin;
+    k.ca = std::cos(a);
+    k.sa = std::sin(a);
+    k.c_skip = lc::PI - 1e-3 - a > 0.0 ? std::cos(lc::PI - 1e-3 - a) : 2.0;
   }
-  constexpr int64_t N = 1 << 16;
-  auto q = [](double x) {
-    double t = (x + 1.0) * 0.5 * static_cast<double>(N - 1);
-    return static_cast<int64_t>(std::min(static_cast<double>(N - 1), std::max(0.0, t)));
-  };
-  int64_t x = q(u), y = q(v);
-  uint64_t key = 0;
-  for (int64_t sq = N / 2; sq > 0; sq /= 2) {
-    const int64_t rx = (x & sq) ? 1 : 0, ry = (y & sq) ? 1 : 0;
-    key += static_cast<uint64_t>(sq) * static_cast<uint64_t>(sq) * static_cast<uint64_t>((3 * rx) ^ ry);
-    if (ry == 0) {
-      if (rx == 1) {
-        x = sq - 1 - x;
-        y = sq - 1 - y;
-      }
-      std::swap(x, y);
-    }
+  if (front_ok) {
+    const double b = std::acos(thr_front) - margin;
+    k.cb = std::cos(b);
+    k.sb = std::sin(b);
+    k.c_front = b - 1e-3 > 0.0 ? std::cos(b - 1e-3) : 2.0;
   }
-  return key;
-}
-
-// Cone around the normalised mean direction holding all `dirs`, with the
-// half-angle rounded up. An empty set or a zero mean gives a full sphere
-// (half-angle pi), which never permits a skip.
-Cone bounding_cone(const std::vector<lc::Vec3>& dirs) {
-  Cone c{};
-  lc::Vec3 m(0, 0, 0);
-  for (const lc::Vec3& d : dirs) m = m + d;
-  double len = m.length();
-  if (dirs.empty() || !(len > 0.0)) {
-    c.theta = static_cast<float>(lc::PI);
-    return c;
-  }
-  m = (1.0 / len) * m;
-  double th = 0.0;
-  for (const lc::Vec3& d : dirs) {
-    double dl = d.length();
-    if (!(dl > 0.0)) continue;
-    double ca = lc::clampd(lc::dot(m, d) / dl, -1.0, 1.0);
-    th = std::max(th, std::acos(ca));
-  }
-  th = std::min(lc::PI, th * (1.0 + 1e-6) + 1e-6);
-  c.x = static_cast<float>(m.x);
-  c.y = static_cast<float>(m.y);
-  c.z = static_cast<float>(m.z);
-  // Rounded up to FP32.
-  c.theta = std::nextafter(static_cast<float>(th), 4.0f);
-  return c;
+  return k;
 }
 
 // Degree of a segment's contribution as a polynomial in mu for elements
@@ -129,28 +76,7 @@ void patch_moments(const uint8_t* pr, uint32_t stride, double* M) {
   for (uint32_t i = 0; i < PATCH; ++i) {
     float f[4];
     std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
-    const double x = f[0], y = f[1], z = f[2], w = f[3];
-    if (w == 0.0) continue;
-    M[0] += w * x;
-    M[1] += w * y;
-    M[2] += w * z;
-    M[3] += w * x * x;
-    M[4] += w * y * y;
-    M[5] += w * z * z;
-    M[6] += 2.0 * w * x * y;
-    M[7] += 2.0 * w * x * z;
-    M[8] += 2.0 * w * y * z;
-    M[9] += w * x * x * x;
-    M[10] += w * y * y * y;
-    M[11] += w * z * z * z;
-    M[12] += 3.0 * w * x * x * y;
-    M[13] += 3.0 * w * x * x * z;
-    M[14] += 3.0 * w * y * y * x;
-    M[15] += 3.0 * w * y * y * z;
-    M[16] += 3.0 * w * z * z * x;
-    M[17] += 3.0 * w * z * z * y;
-    M[18] += 6.0 * w * x * y * z;
-    M[19] += std::fabs(w);  // size bound for the limb lanes' fixed point
+    accumulate_moments(M, f[0], f[1], f[2], f[3]);
   }
 }
 
@@ -180,9 +106,7 @@ std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& dat
   return sp;
 }
 
-std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps) {
-  auto sp = std::make_unique<SetPack>();
-  SetPack& S = *sp;
+void set_scalars(SetPack& S, const lc::PreparedModel& P) {
   SetDev& d = S.dev;
   SetHost& H = S.host;
   H.valid = true;
@@ -224,10 +148,13 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
   d.phase2 = P.phase2;
   d.third = P.third;
   for (double& g : d.gscl) g = 1.0;
+}
 
+std::vector<GridRef> grid_refs(const lc::PreparedModel& P) {
+  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0, lens = P.glens1;
   // Which lanes add which grid (class x grid type; see LaneClass).
   const bool cc = P.cal_copies;
-  std::vector<GridSpec> grids;
+  std::vector<GridRef> grids;
   if (P.star1_alias) {
     grids.push_back({0, &P.star1f, KIND_STAR1, mask_of(CLS_NORMAL, kAll) | mask_of(CLS_WD, kAll), beam1});
   } else {
@@ -259,33 +186,33 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
       grids.push_back({10, &P.star2c_cal, KIND_STAR2, mask_of(CLS_CAL2, {1u, 2u}), posn2});
     }
   }
+  return grids;
+}
 
-  // Records, one segment per (grid, eclipse count).
+uint64_t lay_out_segments(SetPack& S, const std::vector<GridRef>& grids, std::vector<uint32_t>* n_real,
+                          std::vector<uint32_t>* seg_ref) {
+  SetHost& H = S.host;
+  S.segs.clear();
+  S.seg_grid.clear();
+  n_real->clear();
+  seg_ref->clear();
   uint64_t off = 0;
-  std::vector<Cone>& cones = S.cones;  // normal cone per patch
-  for (const GridSpec& g : grids) {
+  uint32_t patches = 0;
+  for (size_t gi = 0; gi < grids.size(); ++gi) {
+    const GridRef& g = grids[gi];
     const std::vector<lc::Point>& pts = *g.pts;
     if (g.id < 7) H.grids[g.id].n = static_cast<uint32_t>(pts.size());
     if (pts.empty()) continue;
-    size_t maxe = 0;
-    for (const lc::Point& pt : pts) maxe = std::max(maxe, pt.eclipse.size());
-    if (maxe > static_cast<size_t>(MAX_ECL))
-      throw lc::LcError("element with " + std::to_string(maxe) +
-                        " eclipse intervals exceeds the kernel limit of " + std::to_string(MAX_ECL));
-    for (uint32_t k = 0; k <= maxe; ++k) {
-      std::vector<uint32_t> idx;
-      for (uint32_t e = 0; e < pts.size(); ++e)
-        if (pts[e].eclipse.size() == k) idx.push_back(e);
-      if (idx.empty()) continue;
-      // Spatial order: consecutive 64-element patches then have compact
-      // normal cones, which lets the kernel skip back-facing patches.
-      std::vector<uint64_t> key(pts.size());
-      for (uint32_t e : idx) key[e] = direction_key(pts[e].dirn);
-      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-      const uint32_t n_real = static_cast<uint32_t>(idx.size());
-      const uint32_t n_pad = (n_real + PATCH - 1) / PATCH * PATCH;
-      const uint32_t stride = rec_stride(k, g.posn);
-      const uint32_t eoff = g.posn ? 32u : 16u;
+    uint32_t hist[MAX_ECL + 1] = {};
+    for (const lc::Point& pt : pts) {
+      if (pt.eclipse.size() > static_cast<size_t>(MAX_ECL))
+        throw lc::LcError("element with " + std::to_string(pt.eclipse.size()) +
+                          " eclipse intervals exceeds the kernel limit of " + std::to_string(MAX_ECL));
+      ++hist[pt.eclipse.size()];
+    }
+    for (uint32_t k = 0; k <= static_cast<uint32_t>(MAX_ECL); ++k) {
+      if (!hist[k]) continue;
+      const uint32_t n_pad = (hist[k] + PATCH - 1) / PATCH * PATCH;
       off = (off + 15) / 16 * 16;
       SegDev sg{};
       sg.rec_off = off;
@@ -294,47 +221,93 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
       sg.kind = g.kind;
       sg.necl = k;
       sg.flags = g.posn ? SEG_POSN : 0u;
-      sg.patch_off = static_cast<uint32_t>(cones.size());
-      S.recs.resize(off + static_cast<uint64_t>(stride) * n_pad);
-      uint8_t* base = S.recs.data() + off;
-      for (uint32_t i = 0; i < n_pad; ++i) {
-        uint8_t* r = base + static_cast<size_t>(i) * stride;
-        if (i >= n_real) {
-          // Padding: zero normal and flux contribute exactly 0; its
-          // intervals can never contain a phase in [0, 1).
-          std::memset(r, 0, eoff);
-          for (uint32_t j = 0; j < k; ++j) {
-            const double e3[3] = {2.0, -1.0, -2.0};
-            std::memcpy(r + eoff + 24 * j, e3, 24);
-          }
-          continue;
-        }
-        const lc::Point& pt = pts[idx[i]];
-        const float a[4] = {static_cast<float>(pt.dirn.x), static_cast<float>(pt.dirn.y),
-                            static_cast<float>(pt.dirn.z), pt.flux};
-        std::memcpy(r, a, 16);
-        if (g.posn) {
-          const float b[4] = {static_cast<float>(pt.posn.x), static_cast<float>(pt.posn.y),
-                              static_cast<float>(pt.posn.z), 0.0f};
-          std::memcpy(r + 16, b, 16);
-        }
-        for (uint32_t j = 0; j < k; ++j) {
-          const double e3[3] = {pt.eclipse[j].first, pt.eclipse[j].second, pt.eclipse[j].second - 1.0};
-          std::memcpy(r + eoff + 24 * j, e3, 24);
-        }
-      }
-      for (uint32_t p0 = 0; p0 < n_pad; p0 += PATCH) {
-        std::vector<lc::Vec3> dirs;
-        for (uint32_t i = p0; i < std::min(p0 + PATCH, n_real); ++i) dirs.push_back(pts[idx[i]].dirn);
-        cones.push_back(bounding_cone(dirs));
-      }
-      off += static_cast<uint64_t>(stride) * n_pad;
+      sg.patch_off = patches;
+      off += static_cast<uint64_t>(rec_stride(k, g.posn)) * n_pad;
+      patches += n_pad / PATCH;
       S.segs.push_back(sg);
       S.seg_grid.push_back(g.id);
-      if (keep_maps) {
-        idx.resize(n_pad, UINT32_MAX);
-        S.seg_orig.push_back(std::move(idx));
+      n_real->push_back(hist[k]);
+      seg_ref->push_back(static_cast<uint32_t>(gi));
+    }
+  }
+  return off;
+}
+
+std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps) {
+  auto sp = std::make_unique<SetPack>();
+  SetPack& S = *sp;
+  set_scalars(S, P);
+  SetDev& d = S.dev;
+  const std::vector<GridRef> grids = grid_refs(P);
+  std::vector<uint32_t> n_real, seg_ref;
+  S.recs.assign(lay_out_segments(S, grids, &n_real, &seg_ref), 0);
+  // Records, one segment per (grid, eclipse count): the segment's elements
+  // in spatial order (stable), so consecutive 64-element patches have
+  // compact normal cones, which lets the kernel skip back-facing patches.
+  std::vector<Cone>& cones = S.cones;  // normal cone per patch
+  for (size_t si = 0; si < S.segs.size(); ++si) {
+    const SegDev& sg = S.segs[si];
+    const GridRef& g = grids[seg_ref[si]];
+    const std::vector<lc::Point>& pts = *g.pts;
+    const uint32_t k = sg.necl;
+    std::vector<uint32_t> idx;
+    for (uint32_t e = 0; e < pts.size(); ++e)
+      if (pts[e].eclipse.size() == k) idx.push_back(e);
+    std::vector<uint32_t> key(pts.size());
+    for (uint32_t e : idx)
+      key[e] = direction_key16(static_cast<float>(pts[e].dirn.x), static_cast<float>(pts[e].dirn.y),
+                               static_cast<float>(pts[e].dirn.z));
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    const uint32_t nr = static_cast<uint32_t>(idx.size());
+    if (nr != n_real[si] || cones.size() != sg.patch_off) throw lc::LcError("segment layout out of step");
+    const uint32_t n_pad = sg.n;
+    const uint32_t stride = rec_stride(k, g.posn);
+    const uint32_t eoff = g.posn ? 32u : 16u;
+    uint8_t* base = S.recs.data() + sg.rec_off;
+    for (uint32_t i = 0; i < n_pad; ++i) {
+      uint8_t* r = base + static_cast<size_t>(i) * stride;
+      if (i >= nr) {
+        // Padding: zero normal and flux contribute exactly 0; its
+        // intervals can never contain a phase in [0, 1).
+        std::memset(r, 0, eoff);
+        for (uint32_t j = 0; j < k; ++j) {
+          const double e3[3] = {2.0, -1.0, -2.0};
+          std::memcpy(r + eoff + 24 * j, e3, 24);
+        }
+        continue;
       }
+      const lc::Point& pt = pts[idx[i]];
+      const float a4[4] = {static_cast<float>(pt.dirn.x), static_cast<float>(pt.dirn.y),
+                           static_cast<float>(pt.dirn.z), pt.flux};
+      std::memcpy(r, a4, 16);
+      if (g.posn) {
+        const float b4[4] = {static_cast<float>(pt.posn.x), static_cast<float>(pt.posn.y),
+                             static_cast<float>(pt.posn.z), 0.0f};
+        std::memcpy(r + 16, b4, 16);
+      }
+      for (uint32_t j = 0; j < k; ++j) {
+        const double e3[3] = {pt.eclipse[j].first, pt.eclipse[j].second, pt.eclipse[j].second - 1.0};
+        std::memcpy(r + eoff + 24 * j, e3, 24);
+      }
+    }
+    for (uint32_t p0 = 0; p0 < n_pad; p0 += PATCH) {
+      // The cone of the FP32 normals the records hold, in record order.
+      ConeBuilder cb;
+      const uint32_t p1 = std::min(p0 + PATCH, nr);
+      for (uint32_t i = p0; i < p1; ++i) {
+        const float* f = reinterpret_cast<const float*>(base + static_cast<size_t>(i) * stride);
+        cb.add(f[0], f[1], f[2]);
+      }
+      cb.axis();
+      for (uint32_t i = p0; i < p1; ++i) {
+        const float* f = reinterpret_cast<const float*>(base + static_cast<size_t>(i) * stride);
+        cb.bound(f[0], f[1], f[2]);
+      }
+      cones.push_back(cb.finish());
+    }
+    if (keep_maps) {
+      idx.resize(n_pad, UINT32_MAX);
+      S.seg_orig.push_back(std::move(idx));
     }
   }
 
@@ -350,16 +323,11 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
   return sp;
 }
 
-void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data) {
+void pack_lane_tables(SetPack& S, const lc::PreparedModel& P, const lc::Data& data) {
   SetDev& d = S.dev;
   SetHost& H = S.host;
-  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0;
-  const std::vector<Cone>& cones = R.cones;
-  if (R.segs.size() != S.segs.size() || R.cones.size() * MOM_STRIDE != R.moms.size())
-    throw lc::LcError("records and segments out of step");
   S.lane_phi.clear();
   S.lane_meta.clear();
-  S.lsplit.clear();
   // Lanes.
   lc::Ginterp gi;
   gi.phase1 = P.phase1;
@@ -403,6 +371,58 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
   d.n_lanes = static_cast<uint32_t>(S.lane_phi.size());
   d.n_times = static_cast<uint32_t>(nt);
 
+  // Earth vector per lane, FP32 as the kernel reads it: comp_light lanes
+  // use the TWOPI*iangle/360 inclination, the single-star lanes
+  // iangle.to_radians().
+  S.lane_e.resize(S.lane_phi.size());
+  for (size_t l = 0; l < S.lane_phi.size(); ++l) {
+    const uint32_t m = S.lane_meta[l];
+    // iscale time lanes follow the single-component functions, which use
+    // iangle.to_radians() (flux.rs:121, 167, 211, 238, 264).
+    const bool lcl = meta_class(m) == CLS_NORMAL && !P.iscale;
+    const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
+    S.lane_e[l] = make_float4(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z), 0.0f);
+  }
+
+  // Element-visits in the reference's loop order (flux.rs:24-107 and the
+  // single-star calls of orchestration.rs), used for throughput figures.
+  const uint64_t n1f = P.star1f.size(), n1c = P.star1_alias ? n1f : P.star1c.size();
+  const uint64_t n2f = P.star2f.size(), n2c = P.star2_alias ? n2f : P.star2c.size();
+  const uint64_t nother = P.disc.size() + P.edge.size() + P.spot.size();
+  uint64_t visits = 0;
+  for (size_t l = 0; l < S.lane_meta.size(); ++l) {
+    const uint32_t m = S.lane_meta[l];
+    if (!(m & META_VALID)) continue;
+    const uint32_t cls = meta_class(m), pt = meta_ptype(m);
+    const uint64_t s1 = pt == 1 ? n1f : n1c, s2 = pt == 3 ? n2f : n2c;
+    if (cls == CLS_NORMAL) visits += s1 + s2 + nother;
+    else if (cls == CLS_CAL2) visits += s2;
+    else visits += s1;
+  }
+  H.visits = visits;
+  H.n_times = static_cast<uint32_t>(nt);
+
+  // Per-time terms and observations.
+  S.t_slfac = P.slfac;
+  S.t_phase = P.phase;
+  S.t_y.resize(nt);
+  S.t_ferr.resize(nt);
+  S.t_w.resize(nt);
+  for (size_t t = 0; t < nt; ++t) {
+    S.t_y[t] = data[t].flux;
+    S.t_ferr[t] = data[t].ferr;
+    S.t_w[t] = data[t].weight;
+  }
+  }
+
+void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
+  SetDev& d = S.dev;
+  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0;
+  const std::vector<Cone>& cones = R.cones;
+  if (R.segs.size() != S.segs.size() || R.cones.size() * MOM_STRIDE != R.moms.size())
+    throw lc::LcError("records and segments out of step");
+  if (S.lane_e.size() != S.lane_phi.size()) throw lc::LcError("lane tables out of step");
+  S.lsplit.clear();
   // Per (patch, tile) work codes. For each tile: its valid lanes' earth
   // vectors (the FP32 vectors the kernel reads: comp_light lanes use the
   // TWOPI*iangle/360 inclination, the single-star lanes iangle.to_radians()),
@@ -414,15 +434,6 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
     std::vector<lc::Vec3> es;
   };
   std::vector<TileInfo> tiles(ntile);
-  S.lane_e.resize(S.lane_phi.size());
-  for (size_t l = 0; l < S.lane_phi.size(); ++l) {
-    const uint32_t m = S.lane_meta[l];
-    // iscale time lanes follow the single-component functions, which use
-    // iangle.to_radians() (flux.rs:121, 167, 211, 238, 264).
-    const bool lcl = meta_class(m) == CLS_NORMAL && !P.iscale;
-    const lc::Vec3 e = lc::set_earth(lcl ? P.cosi_lc : P.cosi_rad, lcl ? P.sini_lc : P.sini_rad, S.lane_phi[l]);
-    S.lane_e[l] = make_float4(static_cast<float>(e.x), static_cast<float>(e.y), static_cast<float>(e.z), 0.0f);
-  }
   for (uint32_t t = 0; t < ntile; ++t) {
     TileInfo& ti = tiles[t];
     for (uint32_t l = t * TILE; l < (t + 1) * TILE; ++l) {
@@ -462,21 +473,19 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
       thr = (s1 ? beam1 : beam2) ? mc : std::max(mc, 0.0);
     }
     const bool cull = std::fabs(thr) < 0.99;
-    const double beta = cull ? std::acos(thr) : lc::PI;
     // Seen whole: mu > max(thr, 0) for every element and lane, so that the
     // reference's term is the contribution polynomial itself.
     const double thr_front = std::max(thr, 0.0);
     const bool front_ok = mom_ok && thr_front < 0.99;
-    const double beta_front = front_ok ? std::acos(thr_front) : 0.0;
+    const CodeConsts K = code_consts(cull, thr, front_ok, thr_front);
     // Per-lane version of the skip / seen-whole tests for a PT_FREE pair:
     // every element of the patch is free of eclipses at every lane of the
-    // tile, so a lane whose line of sight is within gf of the patch axis
-    // sees it whole and one beyond gam sees none of it.
-    auto split_lanes = [&](uint32_t patch, uint32_t t, const lc::Vec3& pa, double plen, double gam,
-                           double gf) {
+    // tile, so a lane whose line of sight has cosine above cf with the
+    // patch axis sees it whole and one below cg sees none of it.
+    auto split_lanes = [&](uint32_t patch, uint32_t t, const lc::Vec3& pa, double plen, float cosw) {
       if (!front_ok || !(plen > 0.0)) return;
-      const bool can_all = gf > 1e-3, can_none = cull && gam < lc::PI - 1e-3;
-      const double cf = can_all ? std::cos(gf) : 2.0, cg = can_none ? std::cos(gam) : -2.0;
+      const bool can_all = cosw > K.c_front, can_none = cull && cosw > K.c_skip;
+      const double cf = can_all ? cos_front(K, cosw) : 2.0, cg = can_none ? cos_skip(K, cosw) : -2.0;
       uint64_t all = 0, some = 0;
       for (uint32_t j = 0; j < TILE; ++j) {
         const uint32_t l = t * TILE + j;
@@ -531,9 +540,9 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
     };
     for (uint32_t q = 0; q < sg.n / PATCH; ++q) {
       const Cone& pc = cones[sg.patch_off + q];
-      lc::Vec3 pa(pc.x, pc.y, pc.z);
-      const double plen = pa.length();
-      if (plen > 0.0) pa = (1.0 / plen) * pa;
+      lc::Vec3 pa;
+      double plen = 0.0;
+      cone_axis(pc, &pa.x, &pa.y, &pa.z, &plen);
       const uint8_t* pr = R.recs.data() + sg.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
       for (uint32_t t = 0; t < ntile; ++t) {
         const TileInfo& ti = tiles[t];
@@ -542,30 +551,28 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
           code = PT_SKIP;
           continue;
         }
-        // Angles between the patch axis and the tile's lines of sight span
-        // [acos(emax), acos(emin)]; a normal of the patch lies within
-        // theta_p of the axis.
+        // Cosines between the patch axis and the tile's lines of sight span
+        // [emin, emax]; a normal of the patch has cosine >= cosw with the
+        // axis.
         double emin = 1.0, emax = -1.0;
         for (const lc::Vec3& e : ti.es) {
           const double c = lc::dot(e, pa);
           emin = std::min(emin, c);
           emax = std::max(emax, c);
         }
-        // Back-facing: every normal is more than beta + margin from every
-        // line of sight, so every FP32 mu <= thr. The margin (2e-3 rad)
-        // covers FP32 rounding of mu and of the patch cone.
-        const double gam = beta + pc.theta + kConeMargin;
-        if (cull && plen > 0.0 && gam < lc::PI - 1e-3 && emax < std::cos(gam)) {
+        // Back-facing: every normal is more than acos(thr) + margin from
+        // every line of sight, so every FP32 mu <= thr. The margin (2e-3
+        // rad) covers FP32 rounding of mu and of the patch cone.
+        if (cull && plen > 0.0 && pc.cosw > K.c_skip && emax < cos_skip(K, pc.cosw)) {
           code = PT_SKIP;
           continue;
         }
-        // Seen whole: every normal is less than beta_front - margin from
-        // every line of sight.
-        const double gf = beta_front - pc.theta - kConeMargin;
-        const bool front = front_ok && plen > 0.0 && gf > 1e-3 && emin > std::cos(gf);
+        // Seen whole: every normal is less than acos(thr_front) - margin
+        // from every line of sight.
+        const bool front = front_ok && plen > 0.0 && pc.cosw > K.c_front && emin > cos_front(K, pc.cosw);
         if (sg.necl == 0) {
           code = front ? PT_MOM : PT_FREE;
-          if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, gam, gf);
+          if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, pc.cosw);
           continue;
         }
         // Point::visible (types.rs:50-58) over the tile's phase range
@@ -583,48 +590,26 @@ void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const 
           for (uint32_t k = 0; k < sg.necl; ++k) {
             double e3[3];
             std::memcpy(e3, e + 24 * k, sizeof e3);
-            if (!((ti.hi < e3[0] || ti.lo > e3[1]) && ti.lo > e3[2])) vis = false;
-            if ((ti.lo >= e3[0] && ti.hi <= e3[1]) || ti.hi <= e3[2]) ecl = true;
+            bool misses, holds;
+            interval_vs_range(e3, ti.lo, ti.hi, &misses, &holds);
+            vis = vis && misses;
+            ecl = ecl || holds;
           }
           all_vis = all_vis && vis;
           all_ecl = all_ecl && ecl;
         }
         code = all_ecl ? PT_SKIP : (all_vis ? (front ? PT_MOM : PT_FREE) : PT_FULL);
-        if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, gam, gf);
+        if (code == PT_FREE) split_lanes(sg.patch_off + q, t, pa, plen, pc.cosw);
       }
     }
   }
 
-  // Element-visits in the reference's loop order (flux.rs:24-107 and the
-  // single-star calls of orchestration.rs), used for throughput figures.
-  const uint64_t n1f = P.star1f.size(), n1c = P.star1_alias ? n1f : P.star1c.size();
-  const uint64_t n2f = P.star2f.size(), n2c = P.star2_alias ? n2f : P.star2c.size();
-  const uint64_t nother = P.disc.size() + P.edge.size() + P.spot.size();
-  uint64_t visits = 0;
-  for (size_t l = 0; l < S.lane_meta.size(); ++l) {
-    const uint32_t m = S.lane_meta[l];
-    if (!(m & META_VALID)) continue;
-    const uint32_t cls = meta_class(m), pt = meta_ptype(m);
-    const uint64_t s1 = pt == 1 ? n1f : n1c, s2 = pt == 3 ? n2f : n2c;
-    if (cls == CLS_NORMAL) visits += s1 + s2 + nother;
-    else if (cls == CLS_CAL2) visits += s2;
-    else visits += s1;
-  }
-  H.visits = visits;
-  H.n_times = static_cast<uint32_t>(nt);
+}
 
-  // Per-time terms and observations.
-  S.t_slfac = P.slfac;
-  S.t_phase = P.phase;
-  S.t_y.resize(nt);
-  S.t_ferr.resize(nt);
-  S.t_w.resize(nt);
-  for (size_t t = 0; t < nt; ++t) {
-    S.t_y[t] = data[t].flux;
-    S.t_ferr[t] = data[t].ferr;
-    S.t_w[t] = data[t].weight;
-  }
-  }
+void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data) {
+  pack_lane_tables(S, P, data);
+  pack_codes(S, R, P);
+}
 
 std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps) {
   auto sp = pack_records(P, keep_maps);

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "layout.h"
+#include "pack_math.h"
 #include "../setup/prepare.h"
 
 namespace lch {
@@ -29,9 +30,16 @@ struct SetHost {
   } grids[7];
 };
 
-// Bounding cone of a set of directions: axis and half-angle (pi = no bound).
-struct Cone {
-  float x, y, z, theta;
+// A grid of a prepared set as the pack uses it: grid id (0..6 as
+// lch_batch_visibility numbers them, 7..10 calibration copies), its
+// elements, segment kind, the lanes that add it (cp_bit mask) and whether
+// its records carry positions (beaming or lensing).
+struct GridRef {
+  uint32_t id;
+  const std::vector<lc::Point>* pts;
+  uint32_t kind;
+  uint32_t use;
+  bool posn;
 };
 
 // One set, packed on the host, offsets relative to the set.
@@ -69,6 +77,15 @@ struct SetPack {
 // Pack a prepared model; `data` supplies the observations of this set.
 // pack_records then pack_lanes on the same pack.
 std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps);
+// The pieces of pack_records: the set's scalars (SetDev, SetHost), its
+// grids in segment order, and its segments without records (one per grid
+// and eclipse-pair count: rec_off, padded n, use mask, kind, necl, flags,
+// patch_off), with each segment's element count and grid (index in
+// `grids`); returns the record bytes.
+void set_scalars(SetPack& S, const lc::PreparedModel& P);
+std::vector<GridRef> grid_refs(const lc::PreparedModel& P);
+uint64_t lay_out_segments(SetPack& S, const std::vector<GridRef>& grids, std::vector<uint32_t>* n_real,
+                          std::vector<uint32_t>* seg_ref);
 // The part of a pack that does not depend on the phases: flux-path scalars,
 // segments, element records in patch order, patch cones and moments.
 std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps);
@@ -77,6 +94,11 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
 // (not yet rescaled) records and cones of R, visit counts, per-time tables
 // and the observations of `data`. S carries R's segments (S may be R).
 void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data);
+// The two halves of pack_lanes: lane and per-time tables (lanes, earth
+// vectors, visit counts, observations), then the work codes and lane
+// splits made from them and R's records.
+void pack_lane_tables(SetPack& S, const lc::PreparedModel& P, const lc::Data& data);
+void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P);
 // Rescales the records of the set's linear-law star segments for the device
 // (SEG_LINW); call after every host table has been made from the records.
 void scale_linear_records(SetPack& S);

@@ -1,0 +1,201 @@
+// Arithmetic of the per-set pack shared by the host packer (batch_build.cpp)
+// and the GPU packer (pack_kernels.hip): the spatial key that orders a
+// segment's elements into patches, the bounding cone of a patch's normals
+// and the per-(patch, tile) work-code decision. Only IEEE-exact operations
+// (+ - * /, sqrt, comparisons) appear, in a fixed order, and the device
+// translation unit compiles them without FMA contraction, so host and GPU
+// packs of the same grid are identical bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "layout.h"
+
+namespace lch {
+
+// Position of a normal along a Hilbert curve over its octahedral map,
+// 8 bits per axis: consecutive keys are close on the sphere, so runs of 64
+// elements sorted by key have narrow normal cones. The FP32 normal the
+// records hold is the input.
+__host__ __device__ inline uint32_t direction_key16(float fx, float fy, float fz) {
+  const double x0 = fx, y0 = fy, z0 = fz;
+  const double ax = x0 < 0.0 ? -x0 : x0, ay = y0 < 0.0 ? -y0 : y0, az = z0 < 0.0 ? -z0 : z0;
+  const double s = ax + ay + az;
+  if (!(s > 0.0)) return 0u;
+  double u = x0 / s, v = y0 / s;
+  if (z0 < 0.0) {
+    const double au = u < 0.0 ? -u : u, av = v < 0.0 ? -v : v;
+    const double uu = (1.0 - av) * (u >= 0.0 ? 1.0 : -1.0);
+    const double vv = (1.0 - au) * (v >= 0.0 ? 1.0 : -1.0);
+    u = uu;
+    v = vv;
+  }
+  constexpr int N = 1 << 8;
+  auto q = [](double t) {
+    double w = (t + 1.0) * 0.5 * static_cast<double>(N - 1);
+    w = w < 0.0 ? 0.0 : (w > static_cast<double>(N - 1) ? static_cast<double>(N - 1) : w);
+    return static_cast<int>(w);
+  };
+  int x = q(u), y = q(v);
+  uint32_t key = 0;
+  for (int sq = N / 2; sq > 0; sq /= 2) {
+    const int rx = (x & sq) ? 1 : 0, ry = (y & sq) ? 1 : 0;
+    key += static_cast<uint32_t>(sq) * static_cast<uint32_t>(sq) * static_cast<uint32_t>((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = sq - 1 - x;
+        y = sq - 1 - y;
+      }
+      const int t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return key;
+}
+
+// Normal cone of a patch: unit axis (x, y, z) and cosw, a lower bound on
+// the cosine of the angle between the axis and any normal of the patch
+// (-1 or below: no bound). An empty patch or a zero mean has cosw < -1.
+struct Cone {
+  float x, y, z, cosw;
+};
+
+// Accumulates a patch's FP32 normals in element order (the mean), then the
+// smallest cosine to the normalised mean: call add() for every element,
+// axis(), then bound() for every element, then finish().
+struct ConeBuilder {
+  double mx = 0.0, my = 0.0, mz = 0.0;
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  double mn = 1.0;
+  bool any = false, ok = false;
+  __host__ __device__ void add(float x, float y, float z) {
+    mx += static_cast<double>(x);
+    my += static_cast<double>(y);
+    mz += static_cast<double>(z);
+    any = true;
+  }
+  __host__ __device__ void axis() {
+    const double len = sqrt(mx * mx + my * my + mz * mz);
+    ok = any && len > 0.0;
+    if (ok) {
+      const double inv = 1.0 / len;
+      ux = inv * mx;
+      uy = inv * my;
+      uz = inv * mz;
+    }
+  }
+  __host__ __device__ void bound(float x, float y, float z) {
+    if (!ok) return;
+    const double dx = x, dy = y, dz = z;
+    const double dl = sqrt(dx * dx + dy * dy + dz * dz);
+    if (!(dl > 0.0)) return;
+    double ca = (ux * dx + uy * dy + uz * dz) / dl;
+    ca = ca < -1.0 ? -1.0 : (ca > 1.0 ? 1.0 : ca);
+    mn = ca < mn ? ca : mn;
+  }
+  __host__ __device__ Cone finish() const {
+    Cone c{0.0f, 0.0f, 0.0f, -2.0f};
+    if (!ok) return c;
+    c.x = static_cast<float>(ux);
+    c.y = static_cast<float>(uy);
+    c.z = static_cast<float>(uz);
+    // A small slack for the rounding of the bound itself, then rounded down
+    // to FP32.
+    double cp = mn - 2e-6;
+    cp = cp < -1.0 ? -1.0 : cp;
+    const float f = static_cast<float>(cp);
+    c.cosw = static_cast<double>(f) > cp ? nextafterf(f, -4.0f) : f;
+    return c;
+  }
+};
+
+// Unit axis of a cone (from its FP32 axis), with plen = 0 for a cone that
+// bounds nothing.
+__host__ __device__ inline void cone_axis(const Cone& c, double* px, double* py, double* pz, double* plen) {
+  double x = c.x, y = c.y, z = c.z;
+  const double len = sqrt(x * x + y * y + z * z);
+  if (len > 0.0) {
+    const double inv = 1.0 / len;
+    x = inv * x;
+    y = inv * y;
+    z = inv * z;
+  }
+  *px = x;
+  *py = y;
+  *pz = z;
+  *plen = len;
+}
+
+// Per-segment constants of the code decision. An element adds nothing
+// unless mu > thr; it is seen whole when mu > thr_front = max(thr, 0) for
+// the whole patch. With a = acos(thr) + margin and b = acos(thr_front) -
+// margin (margin 2e-3 rad covers FP32 rounding of mu and of the cone):
+//  * back-facing (skip) iff the patch half-angle w < pi - 1e-3 - a, i.e.
+//    cosw > c_skip, and max over the tile of e.axis < cos(a + w)
+//    = ca cosw - sa sinw;
+//  * seen whole iff w < b - 1e-3, i.e. cosw > c_front, and min over the
+//    tile of e.axis > cos(b - w) = cb cosw + sb sinw.
+// c_skip / c_front = 2 disables the test. The host evaluates the
+// transcendental constants once per segment.
+struct CodeConsts {
+  double ca, sa, c_skip;
+  double cb, sb, c_front;
+};
+
+__host__ __device__ inline double cone_sin(float cosw) {
+  const double c = cosw;
+  const double s2 = 1.0 - c * c;
+  return s2 > 0.0 ? sqrt(s2) : 0.0;
+}
+
+// cos(a + w) and cos(b - w) of CodeConsts for a cone of half-angle w.
+__host__ __device__ inline double cos_skip(const CodeConsts& k, float cosw) {
+  return k.ca * static_cast<double>(cosw) - k.sa * cone_sin(cosw);
+}
+__host__ __device__ inline double cos_front(const CodeConsts& k, float cosw) {
+  return k.cb * static_cast<double>(cosw) + k.sb * cone_sin(cosw);
+}
+
+// Adds one record's normal d and weight w to a patch's direction moments
+// (layout of MOM_STRIDE in layout.h): sum w d, the coefficients of
+// sum w (e.d)^2 and of sum w (e.d)^3 in the monomials of e, and sum |w|.
+// Records of zero weight (padding) add nothing.
+__host__ __device__ inline void accumulate_moments(double* M, float fx, float fy, float fz, float fw) {
+  const double x = fx, y = fy, z = fz, w = fw;
+  if (w == 0.0) return;
+  M[0] += w * x;
+  M[1] += w * y;
+  M[2] += w * z;
+  M[3] += w * x * x;
+  M[4] += w * y * y;
+  M[5] += w * z * z;
+  M[6] += 2.0 * w * x * y;
+  M[7] += 2.0 * w * x * z;
+  M[8] += 2.0 * w * y * z;
+  M[9] += w * x * x * x;
+  M[10] += w * y * y * y;
+  M[11] += w * z * z * z;
+  M[12] += 3.0 * w * x * x * y;
+  M[13] += 3.0 * w * x * x * z;
+  M[14] += 3.0 * w * y * y * x;
+  M[15] += 3.0 * w * y * y * z;
+  M[16] += 3.0 * w * z * z * x;
+  M[17] += 3.0 * w * z * z * y;
+  M[18] += 6.0 * w * x * y * z;
+  M[19] += w < 0.0 ? -w : w;  // size bound for the limb lanes' fixed point
+}
+
+// Point::visible over a tile's phase range [lo, hi] (types.rs:50-58, the
+// same FP64 comparisons) for one element's interval triple
+// (ingress, egress, egress - 1): misses = no lane of the range eclipsed by
+// it, holds = every lane eclipsed.
+__host__ __device__ inline void interval_vs_range(const double* e3, double lo, double hi, bool* misses,
+                                                  bool* holds) {
+  *misses = (hi < e3[0] || lo > e3[1]) && lo > e3[2];
+  *holds = (lo >= e3[0] && hi <= e3[1]) || hi <= e3[2];
+}
+
+}  // namespace lch
