@@ -268,6 +268,24 @@ void lch_grids_free(lch_grids* g);
 int lch_flux_eval(lch_ctx* ctx, lch_grids* g, const double* phase, const double* expose, const int* ndiv,
                   const double* slfac, double third, double* out, int n);
 
+/* No counterpart in the reference: lch_flux_eval for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The call's work tables (lanes,
+ * work items, partial sums, per-time tables, results) are carved from a
+ * static device array of LCH_STATIC_SCRATCH_BYTES per GPU instead of being
+ * allocated: no hipMalloc / hipFree, no stream, event or graph is created
+ * (the grids were allocated once by lch_model_upload). A call whose tables
+ * exceed the array fails with LCH_ERR_INVALID before any kernel runs (about
+ * 512 B per partial row, one row per tile of 64 sub-phases and work chunk
+ * of at most 1024 elements, plus ~100 B per time and per sub-phase). All
+ * calls on one GPU share the array and the last call's contents stay in it;
+ * this process serialises them with a lock per GPU, so they do not race,
+ * and each call waits for its own kernels before it returns. Results equal
+ * lch_flux_eval's bit for bit. */
+#define LCH_STATIC_SCRATCH_BYTES (64ll << 20)
+int lch_flux_eval_static(lch_ctx* ctx, lch_grids* g, const double* phase, const double* expose, const int* ndiv,
+                         const double* slfac, double third, double* out, int n);
+
 /* The engine's CPU setup of a model (the reference's method: grids,
  * eclipses, brightness; orchestration.rs:36-179) in the form
  * lch_model_upload takes, with the per-time terms of `time`

@@ -10,6 +10,7 @@
 
 #include "engine.h"
 #include "grid_model.h"
+#include "scratch.h"
 #include "selfcheck.h"
 #include "../surrogate/mlp.h"
 #include "../surrogate/surr_lik.h"
@@ -694,6 +695,27 @@ int lch_flux_eval(lch_ctx* ctx, lch_grids* g, const double* phase, const double*
     g->impl->eval(phase, expose, ndiv, slfac, third, out, n, nullptr);
     return LCH_OK;
   });
+}
+
+int lch_flux_eval_static(lch_ctx* ctx, lch_grids* g, const double* phase, const double* expose, const int* ndiv,
+                         const double* slfac, double third, double* out, int n) {
+  CtxScope scope(ctx);
+  if (!ctx || !g) return fail(LCH_ERR_INVALID, "null argument");
+  if (g->device != ctx->impl.device()) return fail(LCH_ERR_INVALID, "grids were uploaded on another device");
+  void* ptr = ctx->impl.scratch();
+  if (!ptr) return fail(LCH_ERR_RUNTIME, std::string("no static scratch on this device: ") + ctx->impl.scratch_error());
+  std::lock_guard<std::mutex> lk(lch::scratch_lock(ctx->impl.device()));
+  try {
+    lch::BatchMemory mem;
+    mem.arena = static_cast<char*>(ptr);
+    mem.arena_bytes = lch::kScratchBytes;
+    g->impl->eval(phase, expose, ndiv, slfac, third, out, n, &mem);
+    return LCH_OK;
+  } catch (const lch::ArenaFull& e) {
+    return fail(LCH_ERR_INVALID, std::string(e.what()) + " (LCH_STATIC_SCRATCH_BYTES)");
+  } catch (...) {
+    return guarded([&]() -> int { throw; });
+  }
 }
 
 int lch_model_prepare(const lch_model* m, const double* time, const double* expose, const int* ndiv, int n,

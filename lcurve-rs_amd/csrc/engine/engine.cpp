@@ -12,6 +12,7 @@
 #include <thread>
 
 #include "batch_build.h"
+#include "scratch.h"
 
 namespace lch {
 
@@ -37,6 +38,8 @@ Context::Context(int device) : device_(device) {
   cu_count_ = prop.multiProcessorCount;
   // Default priority on purpose: streams of other priorities are avoided.
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  // The static scratch's address on this device, with the device current.
+  scratch_ = scratch_device_ptr(device, &scratch_err_);
 }
 
 Context::~Context() {

@@ -39,12 +39,18 @@ class Context {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   int cu_count() const { return cu_count_; }
+  // This device's static scratch (lch_flux_eval_static), or nullptr with
+  // the reason in scratch_error().
+  void* scratch() const { return scratch_; }
+  const char* scratch_error() const { return scratch_err_; }
   Tuning tuning;
 
  private:
   int device_ = 0;
   int cu_count_ = 0;
   hipStream_t stream_ = nullptr;
+  void* scratch_ = nullptr;
+  const char* scratch_err_ = "";
 };
 
 // Device memory a batch may use instead of allocating its own.

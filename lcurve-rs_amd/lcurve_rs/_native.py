@@ -131,6 +131,7 @@ SIGNATURES = {
     "lch_model_upload": (INT, [VP, ctypes.POINTER(GridSoaC), ctypes.POINTER(FluxParamsC), ctypes.POINTER(VP)]),
     "lch_grids_free": (None, [VP]),
     "lch_flux_eval": (INT, [VP, VP, D, D, I, D, ctypes.c_double, D, INT]),
+    "lch_flux_eval_static": (INT, [VP, VP, D, D, I, D, ctypes.c_double, D, INT]),
     "lch_model_prepare": (INT, [VP, D, D, I, INT, ctypes.POINTER(VP)]),
     "lch_setup_grid": (INT, [VP, INT, ctypes.POINTER(GridSoaC)]),
     "lch_setup_params": (INT, [VP, ctypes.POINTER(FluxParamsC)]),
