@@ -193,6 +193,61 @@ int lch_batch_set_info(const lch_batch* b, int set, int* valid, double* logg1, d
 int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const double* phases,
                          int nphase, unsigned char* mask, int* nelem);
 
+/* ---- device-packed batches: the per-set pack on the GPU ----
+ * chisq_batch_gpu (lcurve-cuda/src/batch.rs:73-136) sets up and flattens
+ * every model on the CPU, then uploads each flattened set and runs the flux
+ * kernel on it. Here the CPU does the setup (prepare_model, the reference's
+ * method) and lays out each set (segments, lanes, per-time tables, raw
+ * element grids in struct-of-arrays form, staged in pinned memory); every
+ * launch then packs all sets on the GPU from those grids (element records
+ * sorted into patches, patch cones and moments, work codes, work items and
+ * their moment sums) and runs the flux pass and the assembly on them.
+ * Tables and results equal those of lch_batch_build_models with limb lanes
+ * off and the same chunk, bit for bit. Arguments as lch_batch_build_models;
+ * chunk: elements per work item, a multiple of 64 in 64..4096. At most
+ * 65536 segments per batch (a set has one per grid and eclipse-pair count,
+ * typically 5-12). */
+typedef struct lch_dpack lch_dpack;
+typedef struct lch_dpack_stats {
+  long long n_sets, n_valid, n_segs, n_elements, n_pairs, n_patches, n_tiles, n_chunks, n_codes;
+  long long n_cand;       /* (chunk, tile) pairs the item kernel considers */
+  long long n_items, n_items_plain, n_items_ecl; /* work items of the last pack, by kernel form */
+  long long rec_bytes;    /* element records the pack writes */
+  long long input_bytes;  /* raw grids and layout tables one lch_dpack_upload copies */
+  long long work_bytes;   /* device tables the pack and the flux pass write */
+  long long visits;       /* element-visits in the reference's loop order per launch */
+  double setup_core_s;    /* CPU setup, summed over the setup threads */
+  double layout_core_s;   /* host layout and grid flattening, summed over the setup threads */
+  double build_wall_s;    /* both, wall time */
+  double concat_s;        /* batch-wide tables from the per-set layouts */
+  double upload_s;        /* the last lch_dpack_upload */
+  int chunk, threads;
+} lch_dpack_stats;
+int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, const double* time,
+                    const double* expose, const double* flux, const double* ferr, const double* weight,
+                    const int* ndiv, int n, const double* flux2d, const double* ferr2d, const double* weight2d,
+                    int chunk, lch_dpack** out);
+void lch_dpack_free(lch_dpack* b);
+/* Copy the staged grids and tables to the device again (one copy from
+ * pinned host memory) and wait; the build already did it once. */
+int lch_dpack_upload(lch_ctx* ctx, lch_dpack* b);
+/* Enqueue the pack, wait for its work-item counts, enqueue the flux pass
+ * and the assembly; no copy back. */
+int lch_dpack_launch(lch_ctx* ctx, lch_dpack* b, int scale);
+/* One launch, wait, copy back (any output may be null; flux [nsets][n]). */
+int lch_dpack_run(lch_ctx* ctx, lch_dpack* b, int scale, double* chisq, double* wnok, double* sfac,
+                  double* wdwarf, double* flux);
+/* Mean milliseconds per launch over `reps` launches: of the pack kernels
+ * alone and of the whole launch, from HIP events on the context stream. */
+int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_total);
+int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out);
+/* Pack one set on the host (the same setup; the host packer of
+ * lch_batch_build_models, limb lanes off, this batch's chunk) and compare
+ * it with a fresh device pack: diff[0] differing record bytes, [1] patch
+ * cones, [2] patch moment values, [3] work codes, [4] segment flags,
+ * [5] work items, [6] item moment values; diff[7] = the host's item count. */
+int lch_dpack_check(lch_ctx* ctx, lch_dpack* b, int set, long long* diff);
+
 /* ---- grid-level boundary: the caller's own surface-element grids ----
  * The reference's GPU backend flattens lcurve's Point grids on the host and
  * sums the flux over them on the device. These entry points take such grids

@@ -8,6 +8,7 @@
 #include <new>
 #include <string>
 
+#include "device_pack.h"
 #include "engine.h"
 #include "grid_model.h"
 #include "scratch.h"
@@ -46,6 +47,9 @@ struct lch_setup {
   std::vector<double> phase, expose, slfac;
   std::vector<int> ndiv;
   double third = 0.0;
+};
+struct lch_dpack {
+  std::unique_ptr<lch::DevicePackBatch> impl;
 };
 struct lch_batch {
   std::unique_ptr<lch::Batch> impl;
@@ -350,23 +354,32 @@ int lch_chisq_batch(lch_ctx* ctx, const lch_model* base, const double* time, con
 }
 
 namespace {
+// One data set shared by every set, or one per set from the [nsets][n]
+// arrays given.
+std::vector<lc::Data> obs_sets(int nsets, const double* time, const double* expose, const double* flux,
+                               const double* ferr, const double* weight, const int* ndiv, int n,
+                               const double* flux2d, const double* ferr2d, const double* weight2d) {
+  std::vector<lc::Data> data;
+  if (flux2d || ferr2d || weight2d) {
+    data.resize(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) {
+      size_t o = static_cast<size_t>(s) * n;
+      data[s] = make_data(time, expose, flux2d ? flux2d + o : flux, ferr2d ? ferr2d + o : ferr,
+                          weight2d ? weight2d + o : weight, ndiv, n);
+    }
+  } else {
+    data.push_back(make_data(time, expose, flux, ferr, weight, ndiv, n));
+  }
+  return data;
+}
+
 lch_batch* build_with_obs(lch_ctx* ctx, const std::vector<lc::Model>& models, const double* time,
                           const double* expose, const double* flux, const double* ferr,
                           const double* weight, const int* ndiv, int n, const double* flux2d,
                           const double* ferr2d, const double* weight2d, int flags) {
-    const int nsets = static_cast<int>(models.size());
-    std::vector<lc::Data> data;
-    if (flux2d || ferr2d || weight2d) {
-      data.resize(static_cast<size_t>(nsets));
-      for (int s = 0; s < nsets; ++s) {
-        size_t o = static_cast<size_t>(s) * n;
-        data[s] = make_data(time, expose, flux2d ? flux2d + o : flux, ferr2d ? ferr2d + o : ferr,
-                            weight2d ? weight2d + o : weight, ndiv, n);
-      }
-    } else {
-      data.push_back(make_data(time, expose, flux, ferr, weight, ndiv, n));
-    }
-    return build_batch(ctx, models, data, (flags & LCH_BATCH_KEEP_MAPS) != 0);
+  const std::vector<lc::Data> data = obs_sets(static_cast<int>(models.size()), time, expose, flux, ferr, weight,
+                                              ndiv, n, flux2d, ferr2d, weight2d);
+  return build_batch(ctx, models, data, (flags & LCH_BATCH_KEEP_MAPS) != 0);
 }
 }  // namespace
 
@@ -410,6 +423,114 @@ int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nse
 }
 
 void lch_batch_free(lch_batch* b) { delete b; }
+
+int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, const double* time,
+                    const double* expose, const double* flux, const double* ferr, const double* weight,
+                    const int* ndiv, int n, const double* flux2d, const double* ferr2d, const double* weight2d,
+                    int chunk, lch_dpack** out) {
+  CtxScope scope(ctx);
+  if (!ctx || !models || !out) return fail(LCH_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    if (nsets <= 0) throw std::invalid_argument("nsets must be > 0");
+    if (chunk <= 0) throw std::invalid_argument("chunk must be > 0");
+    std::vector<lc::Model> ms;
+    ms.reserve(static_cast<size_t>(nsets));
+    for (int s = 0; s < nsets; ++s) {
+      if (!models[s]) throw std::invalid_argument("null model in batch");
+      ms.push_back(models[s]->impl);
+    }
+    const std::vector<lc::Data> data = obs_sets(nsets, time, expose, flux, ferr, weight, ndiv, n, flux2d, ferr2d,
+                                                weight2d);
+    auto b = std::make_unique<lch_dpack>();
+    b->impl = std::make_unique<lch::DevicePackBatch>(ctx->impl, ms, data, static_cast<uint32_t>(chunk));
+    *out = b.release();
+    return LCH_OK;
+  });
+}
+
+void lch_dpack_free(lch_dpack* b) { delete b; }
+
+int lch_dpack_upload(lch_ctx* ctx, lch_dpack* b) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->upload();
+    return LCH_OK;
+  });
+}
+
+int lch_dpack_launch(lch_ctx* ctx, lch_dpack* b, int scale) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->launch(scale ? 1 : 0);
+    return LCH_OK;
+  });
+}
+
+int lch_dpack_run(lch_ctx* ctx, lch_dpack* b, int scale, double* chisq, double* wnok, double* sfac,
+                  double* wdwarf, double* flux) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->run(scale ? 1 : 0, chisq, wnok, sfac, wdwarf, flux);
+    return LCH_OK;
+  });
+}
+
+int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_total) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  if (reps <= 0) return fail(LCH_ERR_INVALID, "reps must be > 0");
+  return guarded([&] {
+    b->impl->time(scale ? 1 : 0, reps, ms_pack, ms_total);
+    return LCH_OK;
+  });
+}
+
+int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out) {
+  if (!b || !out) return fail(LCH_ERR_INVALID, "null argument");
+  const lch::DevicePackStats& s = b->impl->stats();
+  lch_dpack_stats r{};
+  r.n_sets = static_cast<long long>(s.n_sets);
+  r.n_valid = static_cast<long long>(s.n_valid);
+  r.n_segs = static_cast<long long>(s.n_segs);
+  r.n_elements = static_cast<long long>(s.n_elements);
+  r.n_pairs = static_cast<long long>(s.n_pairs);
+  r.n_patches = static_cast<long long>(s.n_patches);
+  r.n_tiles = static_cast<long long>(s.n_tiles);
+  r.n_chunks = static_cast<long long>(s.n_chunks);
+  r.n_codes = static_cast<long long>(s.n_codes);
+  r.n_cand = static_cast<long long>(s.n_cand);
+  r.n_items = static_cast<long long>(s.n_items);
+  r.n_items_plain = static_cast<long long>(s.n_items_plain);
+  r.n_items_ecl = static_cast<long long>(s.n_items_ecl);
+  r.rec_bytes = static_cast<long long>(s.rec_bytes);
+  r.input_bytes = static_cast<long long>(s.input_bytes);
+  r.work_bytes = static_cast<long long>(s.work_bytes);
+  r.visits = static_cast<long long>(s.visits);
+  r.setup_core_s = s.setup_core_s;
+  r.layout_core_s = s.layout_core_s;
+  r.build_wall_s = s.build_wall_s;
+  r.concat_s = s.concat_s;
+  r.upload_s = s.upload_s;
+  r.chunk = static_cast<int>(s.chunk);
+  r.threads = s.threads;
+  *out = r;
+  return LCH_OK;
+}
+
+int lch_dpack_check(lch_ctx* ctx, lch_dpack* b, int set, long long* diff) {
+  CtxScope scope(ctx);
+  if (!ctx || !b || !diff) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    uint64_t d[8];
+    b->impl->check(set, d);
+    for (int i = 0; i < 8; ++i) diff[i] = static_cast<long long>(d[i]);
+    return LCH_OK;
+  });
+}
 
 int lch_batch_set_obs(lch_ctx* ctx, lch_batch* b, const double* flux2d, const double* ferr2d,
                       const double* weight2d) {

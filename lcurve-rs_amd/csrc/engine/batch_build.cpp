@@ -35,6 +35,8 @@ uint32_t mask_of(uint32_t cls, std::initializer_list<uint32_t> ptypes) {
 const std::initializer_list<uint32_t> kAll = {1u, 2u, 3u};
 
 
+}  // namespace
+
 // Per-segment constants of the code decision (CodeConsts), with the cone
 // margin of 2e-3 rad.
 CodeConsts code_consts(bool cull, double thr, bool front_ok, double thr_front) {
@@ -55,10 +57,6 @@ CodeConsts code_consts(bool cull, double thr, bool front_ok, double thr_front) {
   return k;
 }
 
-// Degree of a segment's contribution as a polynomial in mu for elements
-// that are visible with mu above the cut-off, or 0 where it is none
-// (beaming, lensing, Claret or cubic / quartic Poly limb darkening). The
-// FP32 coefficients are the ones the kernel uses.
 int moment_degree(const SegDev& sg, const SetDev& d) {
   if (sg.kind == KIND_SPOT) return 1;
   if (sg.kind == KIND_DISC || sg.kind == KIND_EDGE) return 3;
@@ -69,6 +67,38 @@ int moment_degree(const SegDev& sg, const SetDev& d) {
   if (a[2] != 0.0f || a[3] != 0.0f) return 0;
   return a[1] != 0.0f ? 3 : 2;
 }
+
+SegCodeRule seg_code_rule(const SegDev& sg, const SetDev& d, const lc::PreparedModel& P) {
+  SegCodeRule r;
+  r.mom_ok = moment_degree(sg, d) > 0;
+  // An element adds nothing unless mu > thr: the star's mucrit (mu <= 0
+  // also gives 0 through imu unless beaming evaluates imu at another mu),
+  // or 0 for disc, edge and spot (flux.rs:36-106).
+  r.thr = 0.0;
+  if (sg.kind <= KIND_STAR2) {
+    const bool s1 = sg.kind == KIND_STAR1;
+    const double mc = s1 ? P.ldc1.mucrit : P.ldc2.mucrit;
+    const bool beam = s1 ? P.beam_factor1 != 0.0 : P.beam_factor2 != 0.0;
+    r.thr = beam ? mc : std::max(mc, 0.0);
+  }
+  r.cull = std::fabs(r.thr) < 0.99;
+  // Seen whole: mu > max(thr, 0) for every element and lane, so that the
+  // reference's term is the contribution polynomial itself.
+  r.thr_front = std::max(r.thr, 0.0);
+  r.front_ok = r.mom_ok && r.thr_front < 0.99;
+  r.K = code_consts(r.cull, r.thr, r.front_ok, r.thr_front);
+  return r;
+}
+
+bool linear_law_segment(const SegDev& sg, const SetDev& d) {
+  if (sg.kind > KIND_STAR2 || (sg.flags & (SEG_POSN | SEG_LINW))) return false;
+  const bool s1 = sg.kind == KIND_STAR1;
+  if ((s1 ? d.ltype1 : d.ltype2) != 0u) return false;
+  const float* a = s1 ? d.ldc1 : d.ldc2;
+  return !(a[1] != 0.0f || a[2] != 0.0f || a[3] != 0.0f || (s1 ? d.mucrit1 : d.mucrit2) > 0.0f);
+}
+
+namespace {
 
 // Direction moments of one patch (layout of MOM_STRIDE) from its records.
 void patch_moments(const uint8_t* pr, uint32_t stride, double* M) {
@@ -417,7 +447,6 @@ void pack_lane_tables(SetPack& S, const lc::PreparedModel& P, const lc::Data& da
 
 void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
   SetDev& d = S.dev;
-  const bool beam1 = P.beam_factor1 != 0.0, beam2 = P.beam_factor2 != 0.0;
   const std::vector<Cone>& cones = R.cones;
   if (R.segs.size() != S.segs.size() || R.cones.size() * MOM_STRIDE != R.moms.size())
     throw lc::LcError("records and segments out of step");
@@ -462,22 +491,10 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
     const SegDev& sg = S.segs[g];
     const uint32_t stride = rec_stride(sg.necl, (sg.flags & SEG_POSN) != 0);
     const uint32_t eoff = (sg.flags & SEG_POSN) ? 32u : 16u;
-    const bool mom_ok = moment_degree(sg, d) > 0;
-    // An element adds nothing unless mu > thr: the star's mucrit (mu <= 0
-    // also gives 0 through imu unless beaming evaluates imu at another mu),
-    // or 0 for disc, edge and spot (flux.rs:36-106).
-    double thr = 0.0;
-    if (sg.kind <= KIND_STAR2) {
-      const bool s1 = sg.kind == KIND_STAR1;
-      const double mc = s1 ? P.ldc1.mucrit : P.ldc2.mucrit;
-      thr = (s1 ? beam1 : beam2) ? mc : std::max(mc, 0.0);
-    }
-    const bool cull = std::fabs(thr) < 0.99;
-    // Seen whole: mu > max(thr, 0) for every element and lane, so that the
-    // reference's term is the contribution polynomial itself.
-    const double thr_front = std::max(thr, 0.0);
-    const bool front_ok = mom_ok && thr_front < 0.99;
-    const CodeConsts K = code_consts(cull, thr, front_ok, thr_front);
+    const SegCodeRule rule = seg_code_rule(sg, d, P);
+    const double thr = rule.thr, thr_front = rule.thr_front;
+    const bool cull = rule.cull, front_ok = rule.front_ok;
+    const CodeConsts& K = rule.K;
     // Per-lane version of the skip / seen-whole tests for a PT_FREE pair:
     // every element of the patch is free of eclipses at every lane of the
     // tile, so a lane whose line of sight has cosine above cf with the
@@ -621,13 +638,7 @@ void scale_linear_records(SetPack& S) {
   const SetDev& d = S.dev;
   if (!(d.flags & SET_VALID)) return;
   for (SegDev& sg : S.segs) {
-    // The kernel's linear form (seg_law): Poly law, a1 = a2 = a3 = 0 and
-    // mucrit <= 0 in the FP32 values it reads; no beaming or lensing.
-    if (sg.kind > KIND_STAR2 || (sg.flags & (SEG_POSN | SEG_LINW))) continue;
-    const bool s1 = sg.kind == KIND_STAR1;
-    if ((s1 ? d.ltype1 : d.ltype2) != 0u) continue;
-    const float* a = s1 ? d.ldc1 : d.ldc2;
-    if (a[1] != 0.0f || a[2] != 0.0f || a[3] != 0.0f || (s1 ? d.mucrit1 : d.mucrit2) > 0.0f) continue;
+    if (!linear_law_segment(sg, d)) continue;
     const uint32_t stride = rec_stride(sg.necl, false);
     uint8_t* base = S.recs.data() + sg.rec_off;
     if (sg.rec_off + static_cast<uint64_t>(stride) * sg.n > S.recs.size()) throw lc::LcError("segment records out of range");

@@ -102,6 +102,27 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P);
 // Rescales the records of the set's linear-law star segments for the device
 // (SEG_LINW); call after every host table has been made from the records.
 void scale_linear_records(SetPack& S);
+// The work-code rule of one segment of P: an element adds nothing unless
+// mu > thr, and is seen whole when mu > thr_front for the whole patch;
+// cull / front_ok say whether the skip / seen-whole tests apply, mom_ok
+// whether the segment's contribution is a polynomial in mu of degree <= 3
+// (patch moments), K the constants of the cone tests.
+struct SegCodeRule {
+  bool cull = false, front_ok = false, mom_ok = false;
+  double thr = 0.0, thr_front = 0.0;
+  CodeConsts K{};
+};
+SegCodeRule seg_code_rule(const SegDev& sg, const SetDev& d, const lc::PreparedModel& P);
+CodeConsts code_consts(bool cull, double thr, bool front_ok, double thr_front);
+// Degree of a segment's contribution as a polynomial in mu for elements
+// that are visible with mu above the cut-off, or 0 where it is none
+// (beaming, lensing, Claret or cubic / quartic Poly limb darkening). The
+// FP32 coefficients are the ones the kernel uses.
+int moment_degree(const SegDev& sg, const SetDev& d);
+// Whether a star segment takes the kernel's linear form (SEG_LINW): Poly
+// law, a1 = a2 = a3 = 0 and mucrit <= 0 in the FP32 values the kernel
+// reads, no beaming or lensing.
+bool linear_law_segment(const SegDev& sg, const SetDev& d);
 // A set whose setup failed: contributes NaN results, no work.
 std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& data);
 

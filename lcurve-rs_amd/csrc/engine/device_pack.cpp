@@ -1,0 +1,788 @@
+// Host side of the device-packed batch (device_pack.h): CPU setup, each
+// set's layout and raw grids on the setup threads, the batch-wide tables
+// in one pinned block, and the pack + flux + assembly step.
+#include "device_pack.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
+
+namespace lch {
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// Byte offsets of the arrays of one block, each 256-byte aligned.
+struct BlockLayout {
+  size_t total = 0;
+  size_t add(uint64_t bytes) {
+    const size_t at = total;
+    total += align256(std::max<size_t>(static_cast<size_t>(bytes), 1));
+    return at;
+  }
+};
+
+int thread_count(int nthreads, size_t ns) {
+  int nt = nthreads;
+  if (nt <= 0) {
+    const char* env = std::getenv("LCURVE_HIP_THREADS");
+    nt = env ? std::atoi(env) : 0;
+  }
+  if (nt <= 0) nt = static_cast<int>(std::thread::hardware_concurrency());
+  nt = std::max(1, nt);
+  return static_cast<int>(std::min<size_t>(static_cast<size_t>(nt), std::max<size_t>(ns, 1)));
+}
+
+uint32_t bit_width(uint64_t v) {
+  uint32_t b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+}  // namespace
+
+struct SetLayout {
+  SetPack S;                     // scalars, segments (set-relative), lanes, per-time tables; no records
+  std::vector<uint32_t> n_real;  // elements per segment
+  std::vector<uint32_t> pflags;  // PSEG_* per segment
+  std::vector<CodeConsts> K;     // code constants per segment
+  struct Grid {
+    uint32_t e0 = 0, n = 0, pos0 = 0;  // set-relative first raw element / position
+    uint32_t seg_of[MAX_ECL + 1];      // set-relative segment per eclipse-pair count
+  };
+  std::vector<Grid> grids;             // the set's non-empty grids in segment order
+  std::vector<uint32_t> seg_grid;      // per segment: its entry of `grids`
+  std::vector<float4> df, pos;         // raw elements: (dirn, flux) and (posn, 0) in FP32
+  std::vector<uint32_t> ecnt;          // eclipse pairs per raw element
+  std::vector<double2> ecl;            // (ingress, egress) pairs, element by element
+  uint64_t rec_bytes = 0;
+  double setup_s = 0, layout_s = 0;
+};
+
+namespace {
+
+// Everything of one set's device pack that the host makes: the pack's
+// own segment layout, lanes and per-time tables (batch_build.cpp), the
+// segments' code rules and the grids' raw elements in their own order.
+std::unique_ptr<SetLayout> lay_out_set(const lc::PreparedModel& P, const lc::Data& data) {
+  auto L = std::make_unique<SetLayout>();
+  SetPack& S = L->S;
+  set_scalars(S, P);
+  const std::vector<GridRef> grids = grid_refs(P);
+  std::vector<uint32_t> seg_ref;
+  L->rec_bytes = lay_out_segments(S, grids, &L->n_real, &seg_ref);
+  pack_lane_tables(S, P, data);
+  S.n_tiles = static_cast<uint32_t>(S.lane_phi.size() / TILE);
+  for (const SegDev& sg : S.segs) {
+    const SegCodeRule r = seg_code_rule(sg, S.dev, P);
+    L->pflags.push_back((r.cull ? PSEG_CULL : 0u) | (r.front_ok ? PSEG_FRONT : 0u) | (r.mom_ok ? PSEG_MOM : 0u) |
+                        (linear_law_segment(sg, S.dev) ? PSEG_LINW : 0u));
+    L->K.push_back(r.K);
+  }
+  size_t ne = 0, np = 0, nq = 0;
+  for (const GridRef& g : grids) {
+    ne += g.pts->size();
+    if (g.posn) np += g.pts->size();
+    for (const lc::Point& pt : *g.pts) nq += pt.eclipse.size();
+  }
+  if (ne >= UINT32_MAX) throw lc::LcError("too many elements in one set");
+  L->df.reserve(ne);
+  L->pos.reserve(np);
+  L->ecnt.reserve(ne);
+  L->ecl.reserve(nq);
+  std::vector<int> slot(grids.size(), -1);
+  for (size_t gi = 0; gi < grids.size(); ++gi) {
+    const std::vector<lc::Point>& pts = *grids[gi].pts;
+    if (pts.empty()) continue;
+    SetLayout::Grid g;
+    g.e0 = static_cast<uint32_t>(L->df.size());
+    g.n = static_cast<uint32_t>(pts.size());
+    g.pos0 = grids[gi].posn ? static_cast<uint32_t>(L->pos.size()) : 0u;
+    for (uint32_t& s : g.seg_of) s = UINT32_MAX;
+    slot[gi] = static_cast<int>(L->grids.size());
+    L->grids.push_back(g);
+    for (const lc::Point& pt : pts) {
+      L->df.push_back(make_float4(static_cast<float>(pt.dirn.x), static_cast<float>(pt.dirn.y),
+                                  static_cast<float>(pt.dirn.z), pt.flux));
+      if (grids[gi].posn)
+        L->pos.push_back(make_float4(static_cast<float>(pt.posn.x), static_cast<float>(pt.posn.y),
+                                     static_cast<float>(pt.posn.z), 0.0f));
+      L->ecnt.push_back(static_cast<uint32_t>(pt.eclipse.size()));
+      for (const auto& e : pt.eclipse) L->ecl.push_back(make_double2(e.first, e.second));
+    }
+  }
+  for (size_t g = 0; g < S.segs.size(); ++g) {
+    const int k = slot[seg_ref[g]];
+    if (k < 0) throw lc::LcError("segment of an empty grid");
+    L->grids[static_cast<size_t>(k)].seg_of[S.segs[g].necl] = static_cast<uint32_t>(g);
+    L->seg_grid.push_back(static_cast<uint32_t>(k));
+  }
+  return L;
+}
+
+}  // namespace
+
+DevicePackBatch::DevicePackBatch(Context& ctx, const std::vector<lc::Model>& models,
+                                 const std::vector<lc::Data>& data, uint32_t chunk, int nthreads)
+    : ctx_(ctx), models_(models), data_(data) {
+  if (models.empty()) throw std::invalid_argument("no parameter sets");
+  if (data.empty() || (data.size() != 1 && data.size() != models.size()))
+    throw std::invalid_argument("give one data set, or one per parameter set");
+  if (chunk < PATCH || chunk % PATCH || chunk > MAX_ITEM_PATCHES * PATCH)
+    throw std::invalid_argument("chunk must be a multiple of 64 in 64..4096");
+  chunk_ = chunk;
+  n_times_ = data[0].size();
+  for (const lc::Data& d : data)
+    if (d.size() != n_times_) throw std::invalid_argument("every data set needs the same number of times");
+  const size_t ns = models.size();
+  std::vector<std::unique_ptr<SetLayout>> lay(ns);
+  const int nt = thread_count(nthreads, ns);
+  st_.threads = nt;
+  const double t0 = now_s();
+  std::atomic<size_t> next{0};
+  std::exception_ptr fatal;
+  std::mutex mu;
+  auto worker = [&]() {
+    for (;;) {
+      const size_t s = next.fetch_add(1);
+      if (s >= ns) break;
+      try {
+        const double a = now_s();
+        std::unique_ptr<lc::PreparedModel> P;
+        std::string err;
+        try {
+          P = lc::prepare_model(models_[s], data_of(s));
+        } catch (const lc::LcError& e) {
+          err = e.what();
+        }
+        const double b = now_s();
+        std::unique_ptr<SetLayout> L;
+        if (P) {
+          try {
+            L = lay_out_set(*P, data_of(s));
+          } catch (const lc::LcError& e) {
+            err = e.what();
+          }
+        }
+        P.reset();
+        if (!L) {
+          L = std::make_unique<SetLayout>();
+          L->S = std::move(*invalid_set(err, data_of(s)));
+        }
+        L->setup_s = b - a;
+        L->layout_s = now_s() - b;
+        lay[s] = std::move(L);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!fatal) fatal = std::current_exception();
+        next.store(ns);
+      }
+    }
+  };
+  if (nt == 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  if (fatal) std::rethrow_exception(fatal);
+  st_.build_wall_s = now_s() - t0;
+  for (const auto& L : lay) {
+    st_.setup_core_s += L->setup_s;
+    st_.layout_core_s += L->layout_s;
+  }
+  try {
+    build(std::move(lay));
+    upload();
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+DevicePackBatch::~DevicePackBatch() { release(); }
+
+void DevicePackBatch::release() {
+  (void)hipSetDevice(ctx_.device());
+  if (side_.side) (void)hipStreamSynchronize(side_.side);
+  (void)hipStreamSynchronize(ctx_.stream());
+  for (hipEvent_t& e : ev_) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (side_.fork) (void)hipEventDestroy(side_.fork);
+  if (side_.join) (void)hipEventDestroy(side_.join);
+  if (side_.side) (void)hipStreamDestroy(side_.side);
+  side_ = SideStream{};
+  if (h_in_) (void)hipHostFree(h_in_);
+  if (h_counts_) (void)hipHostFree(h_counts_);
+  if (d_in_) (void)hipFree(d_in_);
+  if (d_work_) (void)hipFree(d_work_);
+  if (sort_temp_) (void)hipFree(sort_temp_);
+  if (scan_temp_) (void)hipFree(scan_temp_);
+  h_in_ = nullptr;
+  h_counts_ = nullptr;
+  d_in_ = d_work_ = nullptr;
+  sort_temp_ = scan_temp_ = nullptr;
+}
+
+void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
+  const double t0 = now_s();
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  const size_t ns = lay.size();
+  // Pass 1: sizes.
+  uint64_t n_raw = 0, n_pos = 0, n_pairs = 0, n_segs = 0, n_grids = 0, n_patches = 0, n_tiles = 0;
+  uint64_t n_chunks = 0, n_codes = 0, n_prows = 0, n_lanes = 0, n_times = 0, rec = 0, n_pblocks = 0, n_eclp = 0;
+  uint32_t max_tiles = 0;
+  bool any_iscale = false;
+  for (const auto& Lp : lay) {
+    const SetLayout& L = *Lp;
+    n_raw += L.df.size();
+    n_pos += L.pos.size();
+    n_pairs += L.ecl.size();
+    n_segs += L.S.segs.size();
+    n_grids += L.grids.size();
+    uint64_t np = 0, nch = 0;
+    for (const SegDev& sg : L.S.segs) {
+      np += sg.n / PATCH;
+      nch += (sg.n + chunk_ - 1) / chunk_;
+      if (sg.necl) n_eclp += sg.n / PATCH;
+    }
+    const uint32_t nt = static_cast<uint32_t>(L.S.lane_phi.size() / TILE);
+    n_patches += np;
+    n_tiles += nt;
+    max_tiles = std::max(max_tiles, nt);
+    n_chunks += nch;
+    n_codes += np * nt;
+    n_prows += nch * nt;
+    n_lanes += L.S.lane_phi.size();
+    n_times += L.S.t_lane0.size();
+    rec += align256(L.rec_bytes);
+    n_pblocks += (np + 255) / 256;
+    if (L.S.dev.flags & SET_ISCALE) any_iscale = true;
+  }
+  // Sort keys are (segment << 16 | direction key) in 32 bits.
+  if (n_segs > 65536) throw std::invalid_argument("a device-packed batch holds at most 65536 segments: split the sets");
+  if (n_raw >= UINT32_MAX || n_pairs >= UINT32_MAX || n_patches >= UINT32_MAX || n_tiles >= UINT32_MAX ||
+      n_lanes >= UINT32_MAX || n_chunks >= UINT32_MAX || n_pblocks >= UINT32_MAX)
+    throw std::invalid_argument("device-packed batch too large: split the sets");
+  if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
+  const uint64_t n_cand = n_chunks * max_tiles;
+  if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
+  key_bits_ = 16 + bit_width(n_segs > 0 ? n_segs - 1 : 0);
+  st_.n_sets = ns;
+  st_.n_segs = n_segs;
+  st_.n_grids = n_grids;
+  st_.n_elements = n_raw;
+  st_.n_pairs = n_pairs;
+  st_.n_patches = n_patches;
+  st_.n_tiles = n_tiles;
+  st_.n_chunks = n_chunks;
+  st_.n_codes = n_codes;
+  st_.n_cand = n_cand;
+  st_.n_prows = n_prows;
+  st_.n_lanes = n_lanes;
+  st_.n_times = n_times;
+  st_.rec_bytes = rec;
+  st_.chunk = chunk_;
+
+  // Input block.
+  BlockLayout in;
+  const size_t o_df = in.add(n_raw * sizeof(float4)), o_pos = in.add(n_pos * sizeof(float4));
+  const size_t o_eoff = in.add((n_raw + 1) * sizeof(uint32_t)), o_ecl = in.add(n_pairs * sizeof(double2));
+  const size_t o_grids = in.add(n_grids * sizeof(PackGrid)), o_psegs = in.add(n_segs * sizeof(PackSeg));
+  const size_t o_segs = in.add(n_segs * sizeof(SegDev)), o_psets = in.add(ns * sizeof(PackSet));
+  const size_t o_sets = in.add(ns * sizeof(SetDev)), o_pseg = in.add(n_patches * sizeof(uint32_t));
+  const size_t o_pblk = in.add(n_pblocks * sizeof(PatchBlock)), o_eclp = in.add(n_eclp * sizeof(uint32_t));
+  const size_t o_chunks = in.add(n_chunks * sizeof(PackChunk)), o_tset = in.add(n_tiles * sizeof(uint32_t));
+  const size_t o_cinfo = in.add(n_chunks * sizeof(uint32_t)), o_lphi = in.add(n_lanes * sizeof(double));
+  const size_t o_le = in.add(n_lanes * sizeof(float4)), o_lm = in.add(n_lanes * sizeof(uint32_t));
+  const size_t o_tl0 = in.add(n_times * sizeof(uint32_t)), o_tnd = in.add(n_times * sizeof(int));
+  const size_t o_tsl = in.add(n_times * sizeof(double)), o_ty = in.add(n_times * sizeof(double));
+  const size_t o_tfe = in.add(n_times * sizeof(double)), o_tw = in.add(n_times * sizeof(double));
+  const size_t o_tph = in.add(n_times * sizeof(double));
+  in_bytes_ = in.total;
+  st_.input_bytes = in_bytes_;
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_in_), in_bytes_, hipHostMallocDefault), "hipHostMalloc inputs");
+  hip_check(hipMalloc(reinterpret_cast<void**>(&d_in_), in_bytes_), "hipMalloc inputs");
+  std::memset(h_in_, 0, in_bytes_);
+  auto at = [&](size_t off) { return h_in_ + off; };
+  float4* df = reinterpret_cast<float4*>(at(o_df));
+  float4* pos = reinterpret_cast<float4*>(at(o_pos));
+  uint32_t* eoff = reinterpret_cast<uint32_t*>(at(o_eoff));
+  double2* ecl = reinterpret_cast<double2*>(at(o_ecl));
+  PackGrid* pgrids = reinterpret_cast<PackGrid*>(at(o_grids));
+  PackSeg* psegs = reinterpret_cast<PackSeg*>(at(o_psegs));
+  SegDev* segs = reinterpret_cast<SegDev*>(at(o_segs));
+  PackSet* psets = reinterpret_cast<PackSet*>(at(o_psets));
+  SetDev* sets = reinterpret_cast<SetDev*>(at(o_sets));
+  uint32_t* patch_seg = reinterpret_cast<uint32_t*>(at(o_pseg));
+  PatchBlock* pblocks = reinterpret_cast<PatchBlock*>(at(o_pblk));
+  uint32_t* eclp = reinterpret_cast<uint32_t*>(at(o_eclp));
+  PackChunk* chunks = reinterpret_cast<PackChunk*>(at(o_chunks));
+  uint32_t* tile_set = reinterpret_cast<uint32_t*>(at(o_tset));
+  uint32_t* cinfo = reinterpret_cast<uint32_t*>(at(o_cinfo));
+  double* lane_phi = reinterpret_cast<double*>(at(o_lphi));
+  float4* lane_e = reinterpret_cast<float4*>(at(o_le));
+  uint32_t* lane_meta = reinterpret_cast<uint32_t*>(at(o_lm));
+  uint32_t* t_lane0 = reinterpret_cast<uint32_t*>(at(o_tl0));
+  int* t_ndiv = reinterpret_cast<int*>(at(o_tnd));
+  double* t_slfac = reinterpret_cast<double*>(at(o_tsl));
+  double* t_y = reinterpret_cast<double*>(at(o_ty));
+  double* t_ferr = reinterpret_cast<double*>(at(o_tfe));
+  double* t_w = reinterpret_cast<double*>(at(o_tw));
+  double* t_phase = reinterpret_cast<double*>(at(o_tph));
+
+  // Pass 2: batch-wide tables.
+  host_.resize(ns);
+  seg0_.resize(ns);
+  nseg_.resize(ns);
+  psets_.resize(ns);
+  sets_.resize(ns);
+  rec0_.resize(ns);
+  uint64_t e = 0, pq = 0, pr = 0, code = 0, prow = 0, recb = 0;
+  uint32_t seg = 0, grid = 0, patch = 0, tile = 0, chunk = 0, lane = 0, time = 0, sorted = 0, pblk = 0, ne_cl = 0;
+  uint32_t max_lanes = 0;
+  eoff[0] = 0;
+  for (size_t s = 0; s < ns; ++s) {
+    SetLayout& L = *lay[s];
+    SetPack& S = L.S;
+    const uint32_t nt = static_cast<uint32_t>(S.lane_phi.size() / TILE);
+    SetDev sd = S.dev;
+    sd.lane_off = lane;
+    sd.time_off = time;
+    sd.chunk_off = chunk;
+    PackSet ps{};
+    ps.lane_off = lane;
+    ps.n_tiles = nt;
+    ps.tile0 = tile;
+    ps.patch0 = patch;
+    ps.code0 = code;
+    ps.prow_base = prow;
+    host_[s] = S.host;
+    seg0_[s] = seg;
+    nseg_[s] = static_cast<uint32_t>(S.segs.size());
+    rec0_[s] = recb;
+    const uint64_t e_set = e, q_set = pq;
+    for (const SetLayout::Grid& g : L.grids) {
+      PackGrid G{};
+      G.e0 = e_set + g.e0;
+      G.n = g.n;
+      for (int k = 0; k <= MAX_ECL; ++k) G.seg_of[k] = g.seg_of[k] == UINT32_MAX ? UINT32_MAX : seg + g.seg_of[k];
+      pgrids[grid++] = G;
+    }
+    if (!L.df.empty()) std::memcpy(df + e, L.df.data(), L.df.size() * sizeof(float4));
+    if (!L.pos.empty()) std::memcpy(pos + pq, L.pos.data(), L.pos.size() * sizeof(float4));
+    for (size_t i = 0; i < L.ecnt.size(); ++i) eoff[e + i + 1] = eoff[e + i] + L.ecnt[i];
+    if (!L.ecl.empty()) std::memcpy(ecl + pr, L.ecl.data(), L.ecl.size() * sizeof(double2));
+    e += L.df.size();
+    pq += L.pos.size();
+    pr += L.ecl.size();
+    uint32_t np_set = 0, nch = 0;
+    for (size_t g = 0; g < S.segs.size(); ++g) {
+      const SegDev& sg = S.segs[g];
+      const SetLayout::Grid& gr = L.grids[L.seg_grid[g]];
+      PackSeg q{};
+      q.rec_off = recb + sg.rec_off;
+      q.e0 = e_set + gr.e0;
+      q.pos0 = static_cast<uint32_t>(q_set + gr.pos0);
+      q.n_real = L.n_real[g];
+      q.n_pad = sg.n;
+      q.necl = sg.necl;
+      q.posn = (sg.flags & SEG_POSN) ? 1u : 0u;
+      q.patch0 = patch + sg.patch_off;
+      q.sorted0 = sorted;
+      q.set = static_cast<uint32_t>(s);
+      q.flags = L.pflags[g];
+      q.base_flags = sg.flags;
+      q.K = L.K[g];
+      SegDev d = sg;
+      d.rec_off = q.rec_off;
+      d.set = static_cast<uint32_t>(s);
+      d.mom_off = q.patch0;
+      psegs[seg] = q;
+      segs[seg] = d;
+      sorted += q.n_real;
+      for (uint32_t p = 0; p < sg.n / PATCH; ++p) {
+        patch_seg[q.patch0 + p] = seg;
+        if (sg.necl) eclp[ne_cl++] = q.patch0 + p;
+      }
+      const uint32_t c = (sg.n + chunk_ - 1) / chunk_;
+      for (uint32_t k = 0; k < c; ++k) {
+        chunks[chunk + nch + k] = PackChunk{static_cast<uint32_t>(s), seg, k * chunk_,
+                                            std::min(chunk_, sg.n - k * chunk_), nch + k};
+        cinfo[chunk + nch + k] = sg.use_mask | (sg.kind << 16);
+      }
+      nch += c;
+      np_set += sg.n / PATCH;
+      ++seg;
+    }
+    ps.n_patches = np_set;
+    ps.n_chunks = nch;
+    sd.n_chunks = nch;
+    sd.prow_base = prow;
+    for (uint32_t p0 = 0; p0 < np_set; p0 += 256)
+      pblocks[pblk++] = PatchBlock{static_cast<uint32_t>(s), patch + p0, std::min<uint32_t>(256, np_set - p0)};
+    for (uint32_t t = 0; t < nt; ++t) tile_set[tile + t] = static_cast<uint32_t>(s);
+    const size_t nl = S.lane_phi.size(), ntm = S.t_lane0.size();
+    if (S.lane_e.size() != nl || S.lane_meta.size() != nl) throw DeviceError("lane tables out of step");
+    if (S.t_ndiv.size() != ntm || S.t_slfac.size() != ntm || S.t_y.size() != ntm || S.t_ferr.size() != ntm ||
+        S.t_w.size() != ntm || S.t_phase.size() != ntm)
+      throw DeviceError("time tables out of step");
+    std::copy(S.lane_phi.begin(), S.lane_phi.end(), lane_phi + lane);
+    std::copy(S.lane_e.begin(), S.lane_e.end(), lane_e + lane);
+    std::copy(S.lane_meta.begin(), S.lane_meta.end(), lane_meta + lane);
+    std::copy(S.t_lane0.begin(), S.t_lane0.end(), t_lane0 + time);
+    std::copy(S.t_ndiv.begin(), S.t_ndiv.end(), t_ndiv + time);
+    std::copy(S.t_slfac.begin(), S.t_slfac.end(), t_slfac + time);
+    std::copy(S.t_y.begin(), S.t_y.end(), t_y + time);
+    std::copy(S.t_ferr.begin(), S.t_ferr.end(), t_ferr + time);
+    std::copy(S.t_w.begin(), S.t_w.end(), t_w + time);
+    std::copy(S.t_phase.begin(), S.t_phase.end(), t_phase + time);
+    // The flux kernels' bounds, as the host-packed batch checks them.
+    if ((sd.flags & SET_VALID) && sd.special0 >= sd.n_lanes) throw DeviceError("special lanes out of range");
+    for (size_t t = 0; t < ntm; ++t)
+      if ((sd.flags & SET_VALID) &&
+          static_cast<uint64_t>(S.t_lane0[t]) + static_cast<uint64_t>(std::max(0, S.t_ndiv[t])) > sd.special0)
+        throw DeviceError("time lanes out of range");
+    if (sd.flags & SET_VALID) max_lanes = std::max(max_lanes, sd.special0);
+    psets[s] = ps;
+    sets[s] = sd;
+    psets_[s] = ps;
+    sets_[s] = sd;
+    st_.visits += S.host.visits;
+    if (S.host.valid) ++st_.n_valid;
+    lane += static_cast<uint32_t>(nl);
+    time += static_cast<uint32_t>(ntm);
+    tile += nt;
+    patch += np_set;
+    chunk += nch;
+    code += static_cast<uint64_t>(np_set) * nt;
+    prow += static_cast<uint64_t>(nch) * nt;
+    recb += align256(L.rec_bytes);
+    lay[s].reset();
+  }
+  if (e != n_raw || seg != n_segs || patch != n_patches || tile != n_tiles || chunk != n_chunks || code != n_codes ||
+      prow != n_prows || ne_cl != n_eclp || pblk != n_pblocks || sorted != n_raw || eoff[n_raw] != n_pairs)
+    throw DeviceError("device-pack layout out of step");
+
+  // Work block.
+  BlockLayout w;
+  const size_t w_keys = w.add(n_raw * 4), w_vals = w.add(n_raw * 4), w_keys_s = w.add(n_raw * 4),
+               w_vals_s = w.add(n_raw * 4);
+  const size_t w_recs = w.add(rec), w_cones = w.add(n_patches * sizeof(Cone));
+  const size_t w_moms = w.add(n_patches * MOM_STRIDE * sizeof(double)), w_wmax = w.add(n_segs * 8 + 8);
+  const size_t w_ten = w.add(n_tiles * TILE * sizeof(double4)), w_tval = w.add(n_tiles * 8),
+               w_trng = w.add(n_tiles * sizeof(double2)), w_tcp = w.add(n_tiles * 4);
+  const size_t w_codes = w.add(n_codes), w_efl = w.add(n_codes);
+  const size_t w_cand = w.add(n_cand * sizeof(ItemDev)), w_ccls = w.add(n_cand),
+               w_cpos = w.add(n_cand * sizeof(ItemCounts));
+  const size_t w_items = w.add(n_cand * sizeof(ItemDev)), w_imoms = w.add(n_cand * MOM_STRIDE * sizeof(double));
+  const size_t w_counts = w.add(sizeof(ItemCounts));
+  const size_t w_part = w.add(n_prows * TILE * sizeof(double)), w_ltot = w.add(n_lanes * sizeof(double));
+  const size_t w_flux = w.add(n_times * sizeof(double)), w_out = w.add(ns * sizeof(SetOut));
+  const size_t w_lcomp = w.add(any_iscale ? n_lanes * 5 * sizeof(double) : 0),
+               w_comp = w.add(any_iscale ? n_times * 5 * sizeof(double) : 0),
+               w_lsw = w.add(any_iscale ? n_times * 6 * sizeof(double) : 0);
+  work_bytes_ = w.total;
+  st_.work_bytes = work_bytes_;
+  hip_check(hipMalloc(reinterpret_cast<void**>(&d_work_), work_bytes_), "hipMalloc pack tables");
+  sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(n_raw), 16);
+  scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand), 16);
+  hip_check(hipMalloc(&sort_temp_, sort_temp_bytes_), "hipMalloc sort storage");
+  hip_check(hipMalloc(&scan_temp_, scan_temp_bytes_), "hipMalloc scan storage");
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_counts_), sizeof(ItemCounts), hipHostMallocDefault),
+            "hipHostMalloc counts");
+  auto din = [&](size_t off) { return d_in_ + off; };
+  auto dw = [&](size_t off) { return d_work_ + off; };
+
+  PackDev& p = pd_;
+  p.raw_df = reinterpret_cast<const float4*>(din(o_df));
+  p.raw_pos = reinterpret_cast<const float4*>(din(o_pos));
+  p.raw_eoff = reinterpret_cast<const uint32_t*>(din(o_eoff));
+  p.raw_ecl = reinterpret_cast<const double2*>(din(o_ecl));
+  p.n_raw = n_raw;
+  p.grids = reinterpret_cast<const PackGrid*>(din(o_grids));
+  p.n_grids = static_cast<uint32_t>(n_grids);
+  p.psegs = reinterpret_cast<const PackSeg*>(din(o_psegs));
+  p.segs = reinterpret_cast<SegDev*>(din(o_segs));
+  p.n_segs = static_cast<uint32_t>(n_segs);
+  p.sets = reinterpret_cast<const PackSet*>(din(o_psets));
+  p.n_sets = static_cast<uint32_t>(ns);
+  p.patch_seg = reinterpret_cast<const uint32_t*>(din(o_pseg));
+  p.n_patches = static_cast<uint32_t>(n_patches);
+  p.pblocks = reinterpret_cast<const PatchBlock*>(din(o_pblk));
+  p.n_pblocks = static_cast<uint32_t>(n_pblocks);
+  p.ecl_patches = reinterpret_cast<const uint32_t*>(din(o_eclp));
+  p.n_ecl_patches = static_cast<uint32_t>(n_eclp);
+  p.chunks = reinterpret_cast<const PackChunk*>(din(o_chunks));
+  p.n_chunks = static_cast<uint32_t>(n_chunks);
+  p.max_tiles = max_tiles;
+  p.n_tiles = static_cast<uint32_t>(n_tiles);
+  p.tile_set = reinterpret_cast<const uint32_t*>(din(o_tset));
+  p.lane_phi = reinterpret_cast<const double*>(din(o_lphi));
+  p.lane_e = reinterpret_cast<const float4*>(din(o_le));
+  p.lane_meta = reinterpret_cast<const uint32_t*>(din(o_lm));
+  p.keys = reinterpret_cast<uint32_t*>(dw(w_keys));
+  p.vals = reinterpret_cast<uint32_t*>(dw(w_vals));
+  p.keys_s = reinterpret_cast<uint32_t*>(dw(w_keys_s));
+  p.vals_s = reinterpret_cast<uint32_t*>(dw(w_vals_s));
+  p.recs = dw(w_recs);
+  p.cones = reinterpret_cast<Cone*>(dw(w_cones));
+  p.moms = reinterpret_cast<double*>(dw(w_moms));
+  p.wmax = reinterpret_cast<uint32_t*>(dw(w_wmax));
+  p.tile_en = reinterpret_cast<double4*>(dw(w_ten));
+  p.tile_valid = reinterpret_cast<uint64_t*>(dw(w_tval));
+  p.tile_range = reinterpret_cast<double2*>(dw(w_trng));
+  p.tile_cp = reinterpret_cast<uint32_t*>(dw(w_tcp));
+  p.codes = reinterpret_cast<uint8_t*>(dw(w_codes));
+  p.eflags = reinterpret_cast<uint8_t*>(dw(w_efl));
+  p.cand = reinterpret_cast<ItemDev*>(dw(w_cand));
+  p.cand_class = reinterpret_cast<uint8_t*>(dw(w_ccls));
+  p.cand_pos = reinterpret_cast<ItemCounts*>(dw(w_cpos));
+  p.items = reinterpret_cast<ItemDev*>(dw(w_items));
+  p.imoms = reinterpret_cast<double*>(dw(w_imoms));
+  p.counts = reinterpret_cast<ItemCounts*>(dw(w_counts));
+  p.partials = reinterpret_cast<double*>(dw(w_part));
+
+  DeviceBatch& d = d_;
+  d.n_sets = static_cast<uint32_t>(ns);
+  d.max_set_lanes = max_lanes;
+  d.sets = reinterpret_cast<const SetDev*>(din(o_sets));
+  d.segs = p.segs;
+  d.items = p.items;
+  d.recs = p.recs;
+  d.lane_phi = p.lane_phi;
+  d.lane_e = p.lane_e;
+  d.lane_meta = p.lane_meta;
+  d.chunk_info = reinterpret_cast<const uint32_t*>(din(o_cinfo));
+  d.moms = p.moms;
+  d.imoms = p.imoms;
+  d.t_lane0 = reinterpret_cast<const uint32_t*>(din(o_tl0));
+  d.t_ndiv = reinterpret_cast<const int*>(din(o_tnd));
+  d.t_slfac = reinterpret_cast<const double*>(din(o_tsl));
+  d.t_y = reinterpret_cast<const double*>(din(o_ty));
+  d.t_ferr = reinterpret_cast<const double*>(din(o_tfe));
+  d.t_w = reinterpret_cast<const double*>(din(o_tw));
+  d.t_phase = reinterpret_cast<const double*>(din(o_tph));
+  if (any_iscale) {
+    d.lane_comp = reinterpret_cast<double*>(dw(w_lcomp));
+    d.comp = reinterpret_cast<double*>(dw(w_comp));
+    d.lsw = reinterpret_cast<double*>(dw(w_lsw));
+  }
+  d.partials = p.partials;
+  d.lane_tot = reinterpret_cast<double*>(dw(w_ltot));
+  d.flux = reinterpret_cast<double*>(dw(w_flux));
+  d.out = reinterpret_cast<SetOut*>(dw(w_out));
+  d.n_limb_chunks = 0;
+
+  hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
+  hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+  hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+  st_.concat_s = now_s() - t0;
+}
+
+void DevicePackBatch::upload() {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hipStream_t st = ctx_.stream();
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  const double t0 = now_s();
+  hip_check(hipMemcpyAsync(d_in_, h_in_, in_bytes_, hipMemcpyHostToDevice, st), "hipMemcpyAsync inputs");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  st_.upload_s = now_s() - t0;
+}
+
+void DevicePackBatch::launch(int scale) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hipStream_t st = ctx_.stream();
+  hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
+  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
+            "hipMemcpyAsync counts");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  const ItemCounts c = *h_counts_;
+  const uint64_t n = static_cast<uint64_t>(c.a) + c.b + c.c;
+  if (n > st_.n_cand) throw DeviceError("work item count out of range");
+  d_.n_items = static_cast<uint32_t>(n);
+  d_.n_items_plain = c.a;
+  d_.n_items_ecl = c.b;
+  st_.n_items = n;
+  st_.n_items_plain = c.a;
+  st_.n_items_ecl = c.b;
+  hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
+  hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+}
+
+void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux) {
+  launch(scale);
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  last_out_.resize(sets_.size());
+  hip_check(hipMemcpy(last_out_.data(), d_.out, last_out_.size() * sizeof(SetOut), hipMemcpyDeviceToHost),
+            "hipMemcpy out");
+  for (size_t s = 0; s < last_out_.size(); ++s) {
+    if (chisq) chisq[s] = last_out_[s].chisq;
+    if (wnok) wnok[s] = last_out_[s].wnok;
+    if (sfac) sfac[s] = last_out_[s].scale;
+    if (wdwarf) wdwarf[s] = last_out_[s].wdwarf;
+  }
+  if (flux && st_.n_times)
+    hip_check(hipMemcpy(flux, d_.flux, st_.n_times * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
+}
+
+void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_total) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hipStream_t st = ctx_.stream();
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  double tp = 0.0;
+  hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
+  for (int r = 0; r < reps; ++r) {
+    hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
+    hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
+    hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
+    hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
+              "hipMemcpyAsync counts");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    tp += ms;
+    const ItemCounts c = *h_counts_;
+    const uint64_t n = static_cast<uint64_t>(c.a) + c.b + c.c;
+    if (n > st_.n_cand) throw DeviceError("work item count out of range");
+    d_.n_items = static_cast<uint32_t>(n);
+    d_.n_items_plain = c.a;
+    d_.n_items_ecl = c.b;
+    hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
+    hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+  }
+  hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
+  hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
+  float total = 0.0f;
+  hip_check(hipEventElapsedTime(&total, ev_[2], ev_[1]), "hipEventElapsedTime");
+  if (ms_pack) *ms_pack = reps > 0 ? tp / reps : 0.0;
+  if (ms_total) *ms_total = reps > 0 ? total / reps : 0.0;
+}
+
+void DevicePackBatch::check(int set, uint64_t diff[8]) {
+  for (int i = 0; i < 8; ++i) diff[i] = 0;
+  if (set < 0 || static_cast<size_t>(set) >= sets_.size()) throw std::invalid_argument("set index out of range");
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hipStream_t st = ctx_.stream();
+  // The device tables of a fresh pack.
+  hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
+  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
+            "hipMemcpyAsync counts");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  const ItemCounts cnt = *h_counts_;
+  const uint64_t n_items = static_cast<uint64_t>(cnt.a) + cnt.b + cnt.c;
+  const size_t s = static_cast<size_t>(set);
+  const lc::Data& data = data_of(s);
+  // The host pack of the same set.
+  std::unique_ptr<lc::PreparedModel> P;
+  try {
+    P = lc::prepare_model(models_[s], data);
+  } catch (const lc::LcError&) {
+    if (sets_[s].flags & SET_VALID) diff[3] = 1;  // the device set is valid, the host's is not
+    return;
+  }
+  std::unique_ptr<SetPack> pack;
+  try {
+    pack = pack_set(*P, data, false);
+  } catch (const lc::LcError&) {
+    if (sets_[s].flags & SET_VALID) diff[3] = 1;
+    return;
+  }
+  P.reset();
+  const PackSet& ps = psets_[s];
+  const uint32_t seg0 = seg0_[s], nseg = nseg_[s];
+  if (pack->segs.size() != nseg || pack->n_tiles != ps.n_tiles || pack->cones.size() != ps.n_patches)
+    throw DeviceError("host and device layouts of the set differ");
+  auto get = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy check");
+  };
+  // Cones, moments and codes come from the plain records.
+  std::vector<Cone> cones(ps.n_patches);
+  get(cones.data(), pd_.cones + ps.patch0, cones.size() * sizeof(Cone));
+  for (size_t i = 0; i < cones.size(); ++i)
+    if (std::memcmp(&cones[i], &pack->cones[i], sizeof(Cone)) != 0) ++diff[1];
+  std::vector<double> moms(static_cast<size_t>(ps.n_patches) * MOM_STRIDE);
+  get(moms.data(), pd_.moms + static_cast<size_t>(ps.patch0) * MOM_STRIDE, moms.size() * sizeof(double));
+  for (size_t i = 0; i < moms.size(); ++i)
+    if (std::memcmp(&moms[i], &pack->moms[i], sizeof(double)) != 0) ++diff[2];
+  std::vector<uint8_t> codes(static_cast<size_t>(ps.n_patches) * ps.n_tiles);
+  get(codes.data(), pd_.codes + ps.code0, codes.size());
+  for (size_t i = 0; i < codes.size(); ++i)
+    if (codes[i] != pack->ptcode[i]) ++diff[3];
+  // Records and segment flags after the linear-law rescaling.
+  scale_linear_records(*pack);
+  std::vector<char> recs(pack->recs.size());
+  get(recs.data(), pd_.recs + rec0_[s], recs.size());
+  for (size_t i = 0; i < recs.size(); ++i)
+    if (recs[i] != static_cast<char>(pack->recs[i])) ++diff[0];
+  std::vector<SegDev> segs(nseg);
+  get(segs.data(), pd_.segs + seg0, segs.size() * sizeof(SegDev));
+  for (uint32_t g = 0; g < nseg; ++g)
+    if (segs[g].flags != pack->segs[g].flags) ++diff[4];
+  // Work items of a host-packed batch of this set alone, limb lanes off.
+  struct TuningGuard {
+    Context& c;
+    Tuning saved;
+    ~TuningGuard() { c.tuning = saved; }
+  } guard{ctx_, ctx_.tuning};
+  ctx_.tuning.chunk = chunk_;
+  ctx_.tuning.limb = 0;
+  std::vector<std::unique_ptr<SetPack>> one;
+  one.push_back(std::move(pack));
+  Batch B(ctx_, std::move(one), data.size());
+  const Batch::Tables T = B.tables();
+  std::vector<ItemDev> all(n_items);
+  get(all.data(), pd_.items, all.size() * sizeof(ItemDev));
+  std::vector<ItemDev> mine;
+  for (const ItemDev& it : all)
+    if (it.seg >= seg0 && it.seg < seg0 + nseg) mine.push_back(it);
+  diff[7] = T.items.size();
+  if (mine.size() != T.items.size()) {
+    diff[5] = std::max(mine.size(), T.items.size());
+    return;
+  }
+  std::vector<double> im(MOM_STRIDE);
+  for (size_t i = 0; i < mine.size(); ++i) {
+    ItemDev a = mine[i];
+    const ItemDev& b = T.items[i];
+    a.seg -= seg0;
+    a.lane0 -= ps.lane_off;
+    a.prow -= ps.prow_base;
+    const bool same = a.seg == b.seg && a.e0 == b.e0 && a.ne == b.ne && a.lane0 == b.lane0 && a.prow == b.prow &&
+                      a.todo == b.todo && a.free == b.free && a.mom == b.mom && a.pmom == b.pmom &&
+                      a.lm_off == b.lm_off && a.ngrp == b.ngrp && a.pad == b.pad;
+    if (!same) ++diff[5];
+    if (same && a.mom) {
+      get(im.data(), pd_.imoms + static_cast<size_t>(a.imom) * MOM_STRIDE, MOM_STRIDE * sizeof(double));
+      for (uint32_t c = 0; c < MOM_STRIDE; ++c)
+        if (std::memcmp(&im[c], &T.imoms[static_cast<size_t>(b.imom) * MOM_STRIDE + c], sizeof(double)) != 0)
+          ++diff[6];
+    }
+  }
+}
+
+}  // namespace lch

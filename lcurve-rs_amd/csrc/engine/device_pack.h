@@ -1,0 +1,123 @@
+// A batch whose per-set pack runs on the GPU. The host keeps only what the
+// pack does not compute: the CPU setup (the reference's grids, eclipses and
+// brightness), each set's segment layout, lanes and per-time tables, and
+// the raw element grids in struct-of-arrays form. Every step then builds,
+// on the device, the element records in patch order, patch cones and
+// moments, the per-(patch, tile) work codes, the work items and their
+// moment sums (pack_kernels.hip), and runs the flux pass and the assembly
+// on them. The tables equal the host pack's bit for bit for the same chunk
+// size with limb lanes off (check()).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_build.h"
+#include "engine.h"
+#include "kernels.h"
+#include "pack_kernels.h"
+
+namespace lch {
+
+struct DevicePackStats {
+  uint64_t n_sets = 0, n_valid = 0, n_segs = 0, n_grids = 0, n_elements = 0, n_pairs = 0, n_patches = 0;
+  uint64_t n_tiles = 0, n_chunks = 0, n_codes = 0, n_cand = 0, n_prows = 0, n_lanes = 0, n_times = 0;
+  uint64_t rec_bytes = 0;     // element records the pack writes
+  uint64_t input_bytes = 0;   // raw grids and layout tables copied by upload()
+  uint64_t work_bytes = 0;    // device tables the pack and the flux pass write
+  uint64_t visits = 0;        // element-visits in the reference's loop order per pass
+  uint64_t n_items = 0, n_items_plain = 0, n_items_ecl = 0;  // of the last pack
+  double setup_core_s = 0;    // CPU setup (prepare_model), summed over threads
+  double layout_core_s = 0;   // host layout and grid flattening, summed over threads
+  double build_wall_s = 0;    // both, on the setup threads
+  double concat_s = 0;        // batch-wide tables from the per-set layouts
+  double upload_s = 0;        // the last upload()
+  uint32_t chunk = 0;
+  int threads = 0;
+};
+
+// The host layout of one set, before the batch-wide offsets.
+struct SetLayout;
+
+class DevicePackBatch {
+ public:
+  // CPU setup and host layout of every set on `nthreads` threads (0: all
+  // cores or LCURVE_HIP_THREADS), batch-wide tables, device memory, then
+  // upload(). `data` holds one data set shared by all models or one per
+  // model. Work items cover `chunk` elements (a multiple of 64, at most
+  // 4096).
+  DevicePackBatch(Context& ctx, const std::vector<lc::Model>& models, const std::vector<lc::Data>& data,
+                  uint32_t chunk, int nthreads = 0);
+  ~DevicePackBatch();
+  DevicePackBatch(const DevicePackBatch&) = delete;
+  DevicePackBatch& operator=(const DevicePackBatch&) = delete;
+
+  // Copies the host-staged raw grids and layout tables to the device (one
+  // copy from pinned memory) and waits.
+  void upload();
+  // Enqueues the pack on the context stream, waits for its item counts,
+  // then enqueues the flux pass and the assembly (no copy back).
+  void launch(int scale);
+  // launch(), wait, copy back (any pointer may be null; flux [n_sets][n]).
+  void run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux);
+  // Mean milliseconds per step over `reps` launches: of the pack alone
+  // (HIP events around its kernels) and of the whole step.
+  void time(int scale, int reps, double* ms_pack, double* ms_total);
+  // Packs set `set` on the host (the same CPU setup, pack_set, limb lanes
+  // off, this batch's chunk) and compares it with the device tables of the
+  // last pack. diff[0] differing record bytes, [1] patch cones, [2] patch
+  // moment values, [3] work codes, [4] segment flags, [5] work items
+  // (a count mismatch counts every item), [6] item moment values, [7] the
+  // host's item count.
+  void check(int set, uint64_t diff[8]);
+
+  const DevicePackStats& stats() const { return st_; }
+  uint32_t n_sets() const { return static_cast<uint32_t>(host_.size()); }
+  const SetHost& set(uint32_t i) const { return host_[i]; }
+  size_t n_times() const { return n_times_; }
+  const std::vector<SetOut>& last_results() const { return last_out_; }
+
+ private:
+  void release();
+  void build(std::vector<std::unique_ptr<SetLayout>> lay);
+  const lc::Data& data_of(size_t s) const { return data_.size() == 1 ? data_[0] : data_[s]; }
+
+  Context& ctx_;
+  std::vector<lc::Model> models_;
+  std::vector<lc::Data> data_;
+  size_t n_times_ = 0;
+  uint32_t chunk_ = 0;
+  DevicePackStats st_;
+  std::vector<SetHost> host_;
+  std::vector<SetOut> last_out_;
+  // Host mirrors used by check(): per set its first segment, patch, tile,
+  // code and chunk; its PackSet.
+  std::vector<uint32_t> seg0_, nseg_;
+  std::vector<uint64_t> rec0_;  // first record byte of each set
+  std::vector<PackSet> psets_;
+  std::vector<SetDev> sets_;
+
+  // One pinned host block and one device block of the same layout hold
+  // every input; a second device block holds the pack's work tables.
+  char* h_in_ = nullptr;
+  char* d_in_ = nullptr;
+  char* d_work_ = nullptr;
+  size_t in_bytes_ = 0, work_bytes_ = 0;
+  void* sort_temp_ = nullptr;
+  size_t sort_temp_bytes_ = 0;
+  void* scan_temp_ = nullptr;
+  size_t scan_temp_bytes_ = 0;
+  uint32_t key_bits_ = 32;
+  ItemCounts* h_counts_ = nullptr;  // pinned
+
+  PackDev pd_;
+  DeviceBatch d_;
+  hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+  SideStream side_;
+};
+
+}  // namespace lch

@@ -820,6 +820,27 @@ uint64_t Batch::visits() const {
   return v;
 }
 
+Batch::Tables Batch::tables() const {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  Tables t;
+  t.recs.resize(rec_bytes_);
+  if (rec_bytes_)
+    hip_check(hipMemcpy(t.recs.data(), d_.recs, rec_bytes_, hipMemcpyDeviceToHost), "hipMemcpy records");
+  t.segs.resize(segs_.size());
+  if (!segs_.empty())
+    hip_check(hipMemcpy(t.segs.data(), d_.segs, segs_.size() * sizeof(SegDev), hipMemcpyDeviceToHost),
+              "hipMemcpy segments");
+  t.moms = moms_;
+  t.imoms = item_moms_;
+  t.codes = ptcode_;
+  t.items.resize(n_items_);
+  if (n_items_)
+    hip_check(hipMemcpy(t.items.data(), d_items_, n_items_ * sizeof(ItemDev), hipMemcpyDeviceToHost),
+              "hipMemcpy items");
+  return t;
+}
+
 void Batch::launch(int scale) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();

@@ -98,6 +98,17 @@ class Batch {
   void set_curve_terms(const double* slfac, double third);
   // Device bytes taken from the arena so far (0 without one).
   size_t arena_used() const { return arena_used_; }
+  // Host copies of the batch's tables, for parity checks of other packers:
+  // element records, segments, patch moments, work codes as packed, and the
+  // work items and item moments of the current item build.
+  struct Tables {
+    std::vector<char> recs;
+    std::vector<SegDev> segs;
+    std::vector<double> moms, imoms;
+    std::vector<uint8_t> codes;
+    std::vector<ItemDev> items;
+  };
+  Tables tables() const;
 
   uint32_t n_sets() const { return static_cast<uint32_t>(sets_.size()); }
   // Per-set results of the last run() (empty before the first).

@@ -22,7 +22,7 @@ import numpy as np
 from . import _native
 from ._native import LcurveError, check
 
-__all__ = ["Model", "LcResult", "Batch", "set_device", "get_device", "has_hip", "device_count",
+__all__ = ["Model", "LcResult", "Batch", "DevicePackBatch", "set_device", "get_device", "has_hip", "device_count",
            "LcurveError", "read_data"]
 
 _state = {"device": 0}
@@ -443,6 +443,81 @@ class Model:
     def __repr__(self):
         return (f"Model(q={self.q}, iangle={self.iangle}, r1={self.r1}, r2={self.r2}, "
                 f"t1={self.t1}, t2={self.t2})")
+
+
+class DevicePackBatch:
+    """A batch of parameter sets whose per-set pack runs on the GPU
+    (lch_dpack_*): the constructor does the CPU setup and each set's host
+    layout and stages the raw element grids; every ``run`` / ``time`` packs
+    all sets on the GPU from those grids (element records, patch cones and
+    moments, work codes, work items) and runs the flux pass and the
+    assembly. Results equal a ``Batch`` of the same models with limb lanes
+    off and the same chunk, bit for bit. The stand-in for the reference's
+    chisq_batch_gpu (lcurve-cuda/src/batch.rs:73-136) after prepare_model."""
+
+    def __init__(self, models, times, expose, ndiv, *, flux=None, ferr=None, weight=None, flux2d=None,
+                 ferr2d=None, weight2d=None, chunk=2048):
+        L = _native.lib()
+        t = _f64(times)
+        n = len(t)
+        self.n_times = n
+        self.n_sets = len(models)
+        ex = _f64(np.broadcast_to(expose, (n,)))
+        nd = np.ascontiguousarray(np.broadcast_to(ndiv, (n,)), dtype=np.int32)
+        fl = _f64(np.zeros(n) if flux is None else flux)
+        fe = _f64(np.ones(n) if ferr is None else ferr)
+        wt = _f64(np.ones(n) if weight is None else weight)
+        f2, e2, w2 = (None if a is None else _f64(a) for a in (flux2d, ferr2d, weight2d))
+        harr = (ctypes.c_void_p * self.n_sets)(*[m._h.value for m in models])
+        out = ctypes.c_void_p()
+        check(L.lch_dpack_build(_ctx(), harr, self.n_sets, _dptr(t), _dptr(ex), _dptr(fl), _dptr(fe), _dptr(wt),
+                                _iptr(nd), n, *[None if a is None else _dptr(a) for a in (f2, e2, w2)],
+                                int(chunk), ctypes.byref(out)))
+        self._h = out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _native._lib is not None:
+            _native._lib.lch_dpack_free(h)
+            self._h = None
+
+    def run(self, scale=True, want_flux=True):
+        """One pack + pass; dict with chisq, wnok, sfac, wdwarf (per set)
+        and flux [n_sets, n_times]."""
+        ns, nt = self.n_sets, self.n_times
+        chisq, wnok, sfac, wd = (np.zeros(ns) for _ in range(4))
+        flux = np.zeros((ns, nt)) if want_flux else None
+        check(_native.lib().lch_dpack_run(_ctx(), self._h, int(bool(scale)), _dptr(chisq), _dptr(wnok),
+                                          _dptr(sfac), _dptr(wd), None if flux is None else _dptr(flux)))
+        return {"chisq": chisq, "wnok": wnok, "sfac": sfac, "wdwarf": wd, "flux": flux}
+
+    def launch(self, scale=True):
+        check(_native.lib().lch_dpack_launch(_ctx(), self._h, int(bool(scale))))
+
+    def upload(self):
+        """Copy the staged grids and tables to the device again."""
+        check(_native.lib().lch_dpack_upload(_ctx(), self._h))
+
+    def time(self, scale=True, reps=5):
+        """Mean ms per launch of the pack alone and of the whole launch."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(_native.lib().lch_dpack_time(_ctx(), self._h, int(bool(scale)), int(reps),
+                                           ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def stats(self):
+        s = _native.DPackStatsC()
+        check(_native.lib().lch_dpack_stats_get(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def check(self, set_index):
+        """Host pack of one set against a fresh device pack: dict of
+        mismatch counts (records bytes, cones, moments, codes, segment
+        flags, items, item moments) and the host's item count."""
+        d = (ctypes.c_longlong * 8)()
+        check(_native.lib().lch_dpack_check(_ctx(), self._h, int(set_index), d))
+        keys = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments", "host_items")
+        return dict(zip(keys, (int(v) for v in d)))
 
 
 class Batch:

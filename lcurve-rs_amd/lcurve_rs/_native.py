@@ -47,6 +47,14 @@ class BatchStatsC(ctypes.Structure):
                 ("limb_items", ctypes.c_longlong), ("limb_slots", ctypes.c_longlong)]
 
 
+class DPackStatsC(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in (
+        "n_sets", "n_valid", "n_segs", "n_elements", "n_pairs", "n_patches", "n_tiles", "n_chunks", "n_codes",
+        "n_cand", "n_items", "n_items_plain", "n_items_ecl", "rec_bytes", "input_bytes", "work_bytes", "visits")] + \
+        [(k, ctypes.c_double) for k in ("setup_core_s", "layout_core_s", "build_wall_s", "concat_s", "upload_s")] + \
+        [("chunk", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
 LL = ctypes.POINTER(ctypes.c_longlong)
 LCH_NGRID = 11
 GRID_NAMES = ("star1f", "star1c", "star2f", "star2c", "disc", "edge", "spot",
@@ -106,6 +114,14 @@ SIGNATURES = {
     "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
     "lch_batch_autotune": (INT, [VP, VP, INT]),
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
+    "lch_dpack_build": (INT, [VP, ctypes.POINTER(VP), INT, D, D, D, D, D, I, INT, D, D, D, INT, ctypes.POINTER(VP)]),
+    "lch_dpack_free": (None, [VP]),
+    "lch_dpack_upload": (INT, [VP, VP]),
+    "lch_dpack_launch": (INT, [VP, VP, INT]),
+    "lch_dpack_run": (INT, [VP, VP, INT, D, D, D, D, D]),
+    "lch_dpack_time": (INT, [VP, VP, INT, INT, D, D]),
+    "lch_dpack_stats_get": (INT, [VP, ctypes.POINTER(DPackStatsC)]),
+    "lch_dpack_check": (INT, [VP, VP, INT, LL]),
     "lch_batch_sfac": (INT, [VP, D]),
     "lch_selfcheck_patch_codes": (INT, [VP, D, D, I, INT, D, ctypes.POINTER(ctypes.c_longlong),
                                         ctypes.POINTER(ctypes.c_longlong)]),

@@ -1,0 +1,135 @@
+"""GPU checks of the device-packed batch (lch_dpack_*, lcurve_rs.DevicePackBatch).
+
+For each model variant of the flux path, the per-set pack the GPU makes from
+the raw grids must equal the host packer's bit for bit: element records in
+patch order, patch cones and moments, work codes, segment flags (linear-law
+rescaling), work items and their moment sums (lch_dpack_check). Its curves and
+chi-squared values must equal those of a host-packed Batch of the same models
+(limb lanes off, the same chunk) bit for bit, and through it the FP64 oracle
+within the parity bar. A set whose setup fails gives NaN, as on the host path.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from lcurve_rs import workloads as W
+from variants import ISCALE_DISC, VARIANTS
+
+pytestmark = pytest.mark.gpu
+
+CHECK_KEYS = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments")
+
+
+@pytest.fixture
+def host_tuning(lcurve_rs):
+    """Host-packed batches built in the test use limb lanes off; the
+    context's defaults come back afterwards."""
+    yield lcurve_rs
+    lcurve_rs.set_limb(True)
+    lcurve_rs.set_tuning(1024)
+
+
+def _models(lc, text, n, seed):
+    """n models around `text`: mass ratio and inclination varied."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = lc.Model.from_text(text)
+        m.set_param("q", m.get_param("q") * (1.0 + 0.1 * (rng.random() - 0.5)))
+        m.set_param("iangle", m.get_param("iangle") + 2.0 * (rng.random() - 0.5))
+        out.append(m)
+    return out
+
+
+def _obs(n, seed):
+    rng = np.random.default_rng(seed)
+    return {"flux": 1.0 + 0.01 * rng.standard_normal(n), "ferr": np.full(n, 0.01), "weight": np.ones(n)}
+
+
+def _compare(lc, models, times, expose, ndiv, chunk, obs):
+    dp = lc.DevicePackBatch(models, times, expose, ndiv, chunk=chunk, **obs)
+    for s in range(len(models)):
+        d = dp.check(s)
+        assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
+        assert d["host_items"] > 0, (s, d)
+    rd = dp.run()
+    lc.set_limb(False)
+    lc.set_tuning(chunk)
+    hb = lc.Batch(models, times, expose, ndiv, **obs)
+    rh = hb.run()
+    assert np.array_equal(rd["flux"], rh["flux"]), np.nanmax(np.abs(rd["flux"] / rh["flux"] - 1))
+    for k in ("chisq", "wnok", "sfac", "wdwarf"):
+        assert np.array_equal(rd[k], rh[k]), (k, rd[k], rh[k])
+    return dp, rd
+
+
+@pytest.mark.parametrize("name", ["base"] + sorted(VARIANTS))
+def test_device_pack_equals_host_pack(host_tuning, golden_model_text, name):
+    lc = host_tuning
+    text = golden_model_text if name == "base" else W.model_text(VARIANTS[name], golden_model_text)
+    times = W.linspace_times(-0.2, 1.2, 300)
+    _compare(lc, _models(lc, text, 3, 1), times, 0.001, 1, 1024, _obs(len(times), 2))
+
+
+def test_device_pack_smearing_small_chunks_and_iscale(host_tuning, golden_model_text):
+    """Exposure sub-phases, several chunks per segment, the disc, rim and
+    spot, and the per-component fits."""
+    lc = host_tuning
+    times = W.linspace_times(-0.3, 1.3, 257)
+    text = W.model_text(VARIANTS["disc_edge_spot"], golden_model_text)
+    _compare(lc, _models(lc, text, 2, 3), times, 0.004, 3, 256, _obs(len(times), 4))
+    text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
+    _compare(lc, _models(lc, text, 2, 5), times, 0.002, 2, 512, _obs(len(times), 6))
+
+
+def test_device_pack_against_oracle(host_tuning, golden_model_text):
+    """The curves of a device-packed batch against the FP64 oracle (bar:
+    1e-6 relative), for a disc and bright-spot model with smearing."""
+    lc = host_tuning
+    text = W.model_text(VARIANTS["opaque_disc_calibration"], golden_model_text)
+    times = W.linspace_times(-0.2, 1.2, 200)
+    m = lc.Model.from_text(text)
+    dp = lc.DevicePackBatch([m], times, 0.002, 3, chunk=2048)
+    r = dp.run(scale=False)
+    ref, _ = pyoracle.light_curve(text, times, expose=0.002, ndiv=3, nthreads=8)
+    assert np.max(np.abs(r["flux"][0] / ref - 1)) <= 1e-6
+
+
+def test_device_pack_many_sets_and_repeat(host_tuning):
+    """48 sets of the chi-squared workload's geometry at reduced grids, one
+    of them unphysical: codes and items equal the host's on a sample of sets,
+    results equal the host-packed batch's, the failed set is NaN, and a
+    re-upload and second launch give the same results."""
+    lc = host_tuning
+    base = W.chisq_base_text()
+    p = W.chisq_sets(48, seed=7, nlat_lo=20, nlat_hi=60)
+    models = []
+    for i, s in enumerate(p):
+        ov = {"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4], "nlat1f": 24, "nlat1c": 24,
+              "nlat2f": int(s[5]), "nlat2c": int(s[5])}
+        if i == 17:
+            ov["r2"] = 0.99  # overfills the donor's Roche lobe: setup fails
+        models.append(lc.Model.from_text(W.model_text(ov, base)))
+    times = W.linspace_times(-0.2, 1.2, 400)
+    obs = _obs(len(times), 9)
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=2048, **obs)
+    for s in (0, 5, 17, 31, 47):
+        d = dp.check(s)
+        assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
+    r1 = dp.run()
+    assert np.isnan(r1["chisq"][17]) and np.all(np.isfinite(np.delete(r1["chisq"], 17)))
+    lc.set_limb(False)
+    lc.set_tuning(2048)
+    rh = lc.Batch(models, times, 0.001, 1, **obs).run()
+    ok = np.isfinite(rh["chisq"])
+    assert not ok[17] and ok.sum() == 47
+    assert np.array_equal(r1["chisq"][ok], rh["chisq"][ok])
+    assert np.array_equal(r1["flux"][ok], rh["flux"][ok])
+    dp.upload()
+    r2 = dp.run()
+    assert np.array_equal(r2["chisq"][ok], r1["chisq"][ok]) and np.array_equal(r2["flux"][ok], r1["flux"][ok])
+    ms_pack, ms_total = dp.time(reps=3)
+    assert 0.0 < ms_pack <= ms_total
+    st = dp.stats()
+    assert st["n_sets"] == 48 and st["n_valid"] == 47 and st["n_items"] > 0
+    assert st["input_bytes"] > 0 and st["upload_s"] > 0.0
