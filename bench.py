@@ -4,11 +4,14 @@
 Default workload: chisq_4096 — 4096 seeded parameter sets of the white-dwarf
 + Roche-donor binary with per-set star-2 resolution (10k-150k elements),
 each scored against its own 2000-time curve with 1% Gaussian noise,
-autoscaled. A step is one pass of the resident batch (flux kernel +
-FP64 assembly/chi-squared, one captured HIP graph); the CPU setup that
-builds the grids runs once before timing (it is the reference's CPU method
-and is reported separately). The metric is chi-squared evaluations per
-second.
+autoscaled. The CPU setup that builds the grids (the reference's CPU
+method) and each set's host layout run once before timing and are reported
+separately. A step evaluates every set afresh from its raw element grids
+resident in HBM: the per-set pack on the GPU (records sorted into patches,
+cones, moments, work codes, work items), the flux pass and the FP64
+assembly / chi-squared. The metric is chi-squared evaluations per second.
+The same line reports the rate with the raw grids copied over PCIe every
+step, and the replay rate of a host-packed resident batch.
 
   python bench.py [--steps K] [--warmup W] [--workload chisq_4096|wd_donor_45k|disc_spot_300k]
   python bench.py --workload surrogate_200k   # learned surrogate: BF16 MFMA training
@@ -56,6 +59,33 @@ def chisq_models(nsets):
     texts = [W.model_text({"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4],
                            "nlat2f": int(s[5]), "nlat2c": int(s[5])}, base) for s in sets]
     return texts, sets
+
+
+def build_chisq_dpack(nsets, chunk):
+    """The device-packed chisq_4096 batch with its synthetic observations:
+    each set's true curve (unscaled) with 1% Gaussian noise."""
+    texts, sets = chisq_models(nsets)
+    times = W.linspace_times(-0.2, 1.2, 2000)
+    log(f"CPU setup and host layout of {nsets} sets (grids, eclipses, brightness; raw grids staged) ...")
+    t0 = time.time()
+    models = [lc.Model.from_text(x) for x in texts]
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=chunk)
+    t_build = time.time() - t0
+    st = dp.stats()
+    log(f"setup {st['setup_core_s']:.1f} core-s, layout {st['layout_core_s']:.1f} core-s, wall "
+        f"{st['build_wall_s']:.1f} s on {st['threads']} threads; batch tables {st['concat_s']:.1f} s, upload "
+        f"{st['upload_s']:.2f} s ({st['input_bytes'] / 1e9:.2f} GB); {st['n_valid']}/{st['n_sets']} valid")
+    true = dp.run(scale=False)["flux"]
+    noise = W.chisq_noise(nsets, len(times))
+    obs = true * (1.0 + 0.01 * noise)
+    sig = 0.01 * np.abs(true)
+    dp.set_obs(obs, sig, np.ones_like(obs))
+    info = {"texts": texts, "times": times, "expose": 0.001, "ndiv": 1, "obs": obs, "sig": sig,
+            "scale": True, "unit_per_step": nsets, "build_s": t_build,
+            "config": {"workload": "chisq_4096", "sets": nsets, "times": len(times),
+                       "star2_nlat": "U{89..344}", "star2_elements": "10k-150k", "noise": 0.01,
+                       "autoscale": True, "seed": 42}}
+    return dp, models, info
 
 
 def build_workload(name, nsets):
@@ -222,6 +252,165 @@ def roofline(workload, st, ms_flux, limb):
             "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
             "hbm_algorithmic_bytes": alg_bytes,
             "hbm_frac": round((traffic or alg_bytes) / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+
+
+def roofline_dpack(st, ms_flux, ms_pack, pm):
+    """Roofline of the flux pass of a device-packed step (its dominant
+    part): FP32 work as in roofline(), against the VALU peak; the pack's
+    compulsory bytes beside it."""
+    pairs, mom_lanes = st["pairs_evaluated"], st["moment_evals"]
+    flop = pairs * FLOP_PER_PAIR + mom_lanes * FLOP_PER_MOMENT_LANE
+    tflops = flop / (ms_flux * 1e-3) / 1e12
+    moms = st["n_patches"] * 20 * 8
+    alg_bytes = st["rec_bytes"] + moms + st["n_tiles"] * 64 * 12 + 8 * 64 * st["n_items"]
+    traffic = pm["hbm_bytes_per_launch"] if pm else None
+    ref_flop = st["visits"] * FLOP_PER_VISIT
+    # The pack reads the raw grids and writes the records, cones, moments,
+    # codes and items once (the radix sort's passes over the 8-byte
+    # key/value pairs on top).
+    pack_bytes = st["input_bytes"] + st["rec_bytes"] + moms + st["n_codes"] + 64 * st["n_items"]
+    return {"bound": "valu", "achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": traffic,
+            "traffic_source": ("profiles/flux_pass_chisq_4096_dpack_pmc.json (rocprofv3 --pmc FETCH_SIZE, "
+                               "WRITE_SIZE at the same work-item size)") if pm else None,
+            "reference_formulation": {
+                "flop_per_visit": FLOP_PER_VISIT, "visits_per_launch": st["visits"],
+                "equivalent_tflop_s": round(ref_flop / (ms_flux * 1e-3) / 1e12, 2),
+                "equivalent_frac_of_valu_peak": round(ref_flop / (ms_flux * 1e-3) / 1e12 / VALU_PEAK_TFLOPS, 3),
+                "executed_work_frac": round(flop / ref_flop, 5),
+                "visits_evaluated_frac": round(pairs / st["visits"], 5),
+                "visits_by_moments_frac": round(st["pairs_by_moments"] / st["visits"], 5)},
+            "kernel": "flux_items_kernel", "kernel_ms": round(ms_flux, 4),
+            "flop_per_evaluated_pair": FLOP_PER_PAIR, "flop_per_moment_patch_lane": FLOP_PER_MOMENT_LANE,
+            "flop_counts_exact": True,
+            "pairs_evaluated_per_launch": pairs, "moment_evals_per_launch": mom_lanes,
+            "pairs_by_moments_per_launch": st["pairs_by_moments"],
+            "reference_visits_per_launch": st["visits"],
+            "valu_issue_frac_pmc": pm.get("valu_issue_frac") if pm else None,
+            "hbm_algorithmic_bytes": alg_bytes,
+            "hbm_frac": round((traffic or alg_bytes) / (ms_flux * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "pack": {"ms": round(ms_pack, 4), "compulsory_bytes": pack_bytes,
+                     "gb_s": round(pack_bytes / (ms_pack * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(pack_bytes / (ms_pack * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+
+
+def bench_chisq(args):
+    """chisq_4096: fresh sets from resident raw grids (the value), the
+    PCIe-inclusive rate and the host-packed replay beside it."""
+    chunk = args.chunk if args.chunk > 0 else 2048
+    dp, models, info = build_chisq_dpack(args.nsets, chunk)
+    scale = info["scale"]
+    nsets = info["unit_per_step"]
+    out = dp.run(scale=scale)
+    st = dp.stats()
+    log(f"device pack: {st['n_items']} work items, {st['rec_bytes'] / 1e9:.2f} GB records")
+    log("parity sample against the FP64 oracle ...")
+    flux_err, chi_err, nchk = parity_sample(None, info, out, args.parity_sets)
+    log(f"max rel flux err {flux_err:.3e} over {nchk} sets")
+    chk_sets = sorted({0, nsets // 2, nsets - 1})
+    mism = 0
+    for s_ in chk_sets:
+        d = dp.check(s_)
+        mism += sum(v for k, v in d.items() if k != "host_items")
+    log(f"device pack vs host pack on sets {chk_sets}: {mism} differing values")
+
+    for _ in range(args.warmup):
+        dp.launch(scale=scale)
+    log(f"timing {args.steps} steps (pack + flux + assembly) ...")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dp.launch(scale=scale)
+    lc.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / args.steps
+    value = nsets * args.steps / elapsed
+    reps = max(3, min(args.steps, 10))
+    ms_pack, ms_flux, ms_total = dp.time(scale=scale, reps=reps)
+    # The same step with the raw grids and layout tables copied from pinned
+    # host memory first (the boundary hands over host buffers).
+    npc = max(2, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(npc):
+        dp.upload()
+        dp.launch(scale=scale)
+    lc.sync()
+    pcie = nsets * npc / (time.perf_counter() - t0)
+    st = dp.stats()
+    pm = None
+    path = os.path.join(PROFILES, "flux_pass_chisq_4096_dpack_pmc.json")
+    try:
+        with open(path) as f:
+            pm = json.load(f)
+        if pm.get("chunk") != chunk:
+            pm = None
+    except (OSError, ValueError):
+        pm = None
+    roof = roofline_dpack(st, ms_flux, ms_pack, pm)
+    rec = {"metric": "chi2 evals/sec (1 GPU) at <=1e-6 rel flux err vs FP64",
+           "value": round(value, 3), "unit": "chi2/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+           "data": "synthetic (seeded parameter sets and noise; no external data)",
+           "config": info["config"],
+           "step": "every set evaluated afresh from its raw element grids resident in HBM: GPU pack, flux pass, "
+                   "FP64 assembly and chi^2",
+           "element_visits_per_step": st["visits"],
+           "visits_per_s": round(st["visits"] * args.steps / elapsed, 1),
+           "reference_equivalent_tflop_s": round(st["visits"] * FLOP_PER_VISIT / (ms_flux * 1e-3) / 1e12, 2),
+           "max_rel_flux_err_vs_fp64": flux_err, "parity_sets": nchk, "max_rel_chisq_err_vs_fp64": chi_err,
+           "device_pack_vs_host_pack": {"sets": chk_sets, "differing_values": int(mism)},
+           "fresh_sets": {"ms_pack": round(ms_pack, 4), "ms_flux": round(ms_flux, 4),
+                          "ms_launch_total": round(ms_total, 4),
+                          "pcie_inclusive_chi2_per_s": round(pcie, 1),
+                          "upload_bytes": st["input_bytes"], "upload_s": round(st["upload_s"], 4)},
+           "host_side": {"threads": st["threads"], "setup_core_s": round(st["setup_core_s"], 2),
+                         "layout_core_s": round(st["layout_core_s"], 2),
+                         "setup_core_s_per_set": round(st["setup_core_s"] / nsets, 5),
+                         "layout_core_s_per_set": round(st["layout_core_s"] / nsets, 5),
+                         "build_wall_s": round(st["build_wall_s"], 2), "batch_tables_s": round(st["concat_s"], 2)},
+           "tuning": {"chunk": chunk, "items": st["n_items"], "limb": 0},
+           "roofline": roof}
+    if not args.no_replay:
+        log("host-packed resident batch (CPU pack, replayed) ...")
+        lc.set_limb(not args.no_limb, args.limb_max_part)
+        t0 = time.time()
+        batch = lc.Batch(models, info["times"], 0.001, 1)
+        rb_build = time.time() - t0
+        batch.set_obs(info["obs"], info["sig"], np.ones_like(info["obs"]))
+        if not args.no_autotune:
+            batch.autotune(scale=scale)
+        bst = batch.stats()
+        for _ in range(args.warmup):
+            batch.launch(scale=scale)
+        batch.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.launch(scale=scale)
+        batch.sync()
+        rel = time.perf_counter() - t0
+        rv = nsets * args.steps / rel
+        rec["replay_host_packed"] = {
+            "value": round(rv, 1), "ms_per_step": round(1e3 * rel / args.steps, 3),
+            "step": "flux pass + assembly over tables packed once on the CPU (the same sets replayed)",
+            "cpu_setup_and_pack_s": round(bst["setup_seconds"], 2), "build_wall_s": round(rb_build, 2),
+            "tuning": {"chunk": bst["chunk"], "items": bst["n_items"],
+                       "limb": (0 if args.no_limb else args.limb_max_part) if bst["limb_chunks"] else 0}}
+        del batch
+    if not args.no_cpu_baseline:
+        log("CPU baseline (FP64 oracle, bounded sample) ...")
+        rec["cpu_baseline"] = cpu_baseline(info, budget_s=30.0)
+        rec["cpu_baseline"]["unit"] = rec["unit"]
+        rec["speedup_vs_cpu_flux"] = round(value / rec["cpu_baseline"]["value"], 1)
+        rec["speedup_vs_cpu_flux_pcie_inclusive"] = round(pcie / rec["cpu_baseline"]["value"], 1)
+        s = rec["cpu_baseline"]["setup_core_seconds_per_set"]
+        # End to end, setup included on both sides (HOST_THREADS cores): the
+        # GPU path's measured build (parsing, CPU setup, host layout, batch
+        # tables, upload) plus one step; the CPU path's setup plus its flux.
+        rec["end_to_end"] = {
+            "gpu_chi2_per_s_incl_setup_16_cores": round(nsets / (info["build_s"] + nsets / value), 2),
+            "cpu_chi2_per_s_incl_setup_16_cores": round(
+                1.0 / (s / HOST_THREADS + 1.0 / rec["cpu_baseline"]["value"]), 2)}
+    print(json.dumps(rec), flush=True)
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16
@@ -447,6 +636,8 @@ def main():
     ap.add_argument("--limb-max-part", type=int, default=48,
                     help="split a no-eclipse patch per lane when at most this many lanes see it in part")
     ap.add_argument("--parity-sets", type=int, default=8)
+    ap.add_argument("--no-replay", action="store_true",
+                    help="chisq_4096: skip the host-packed replay batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a streaming kernel of known traffic (for PMC scaling)")
     args = ap.parse_args()
@@ -472,6 +663,9 @@ def main():
         if args.steps == 10:
             args.steps = 400
         bench_da_mcmc(args)
+        return
+    if args.workload == "chisq_4096":
+        bench_chisq(args)
         return
     if args.chunk > 0:
         lc.set_tuning(args.chunk, 0)

@@ -29,6 +29,8 @@
  *                                                 crates/lcurve-cuda/src/batch.rs:198-231
  *   lch_flux_eval      <- lcuda_flux_eval         crates/lcurve-cuda/kernels/flux_kernel.cu:534-555
  *   lch_model_prepare  <- prepare_model (host)    crates/lcurve-cuda/src/batch.rs:233-501
+ *   lch_dpack_*        <- chisq_batch_gpu's flatten + upload + kernel per set, after prepare_model
+ *                                                 crates/lcurve-cuda/src/batch.rs:73-136
  * The reference's lcuda_malloc / memcpy / stream / host_alloc exports exist
  * because its Rust host manages device buffers; here the library owns all
  * device memory, so they have no counterpart.
@@ -206,7 +208,8 @@ int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const do
  * off and the same chunk, bit for bit. Arguments as lch_batch_build_models;
  * chunk: elements per work item, a multiple of 64 in 64..4096. At most
  * 65536 segments per batch (a set has one per grid and eclipse-pair count,
- * typically 5-12). */
+ * typically 5-12) and under 2^32 elements and work-item candidates
+ * (chunks x lane tiles). */
 typedef struct lch_dpack lch_dpack;
 typedef struct lch_dpack_stats {
   long long n_sets, n_valid, n_segs, n_elements, n_pairs, n_patches, n_tiles, n_chunks, n_codes;
@@ -216,6 +219,8 @@ typedef struct lch_dpack_stats {
   long long input_bytes;  /* raw grids and layout tables one lch_dpack_upload copies */
   long long work_bytes;   /* device tables the pack and the flux pass write */
   long long visits;       /* element-visits in the reference's loop order per launch */
+  long long pairs_evaluated, pairs_by_moments, moment_evals; /* the flux pass's work at the last
+                             lch_dpack_run, as lch_batch_stats counts it */
   double setup_core_s;    /* CPU setup, summed over the setup threads */
   double layout_core_s;   /* host layout and grid flattening, summed over the setup threads */
   double build_wall_s;    /* both, wall time */
@@ -238,8 +243,14 @@ int lch_dpack_launch(lch_ctx* ctx, lch_dpack* b, int scale);
 int lch_dpack_run(lch_ctx* ctx, lch_dpack* b, int scale, double* chisq, double* wnok, double* sfac,
                   double* wdwarf, double* flux);
 /* Mean milliseconds per launch over `reps` launches: of the pack kernels
- * alone and of the whole launch, from HIP events on the context stream. */
-int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_total);
+ * alone, of the flux pass alone and of the whole launch, from HIP events on
+ * the context stream. */
+int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_flux,
+                   double* ms_total);
+/* Replace the observations of every set: [nsets][n] each, a null array
+ * keeps the current values. */
+int lch_dpack_set_obs(lch_ctx* ctx, lch_dpack* b, const double* flux2d, const double* ferr2d,
+                      const double* weight2d);
 int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out);
 /* Pack one set on the host (the same setup; the host packer of
  * lch_batch_build_models, limb lanes off, this batch's chunk) and compare

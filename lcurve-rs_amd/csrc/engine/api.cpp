@@ -479,12 +479,23 @@ int lch_dpack_run(lch_ctx* ctx, lch_dpack* b, int scale, double* chisq, double* 
   });
 }
 
-int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_total) {
+int lch_dpack_time(lch_ctx* ctx, lch_dpack* b, int scale, int reps, double* ms_pack, double* ms_flux,
+                   double* ms_total) {
   CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
   if (reps <= 0) return fail(LCH_ERR_INVALID, "reps must be > 0");
   return guarded([&] {
-    b->impl->time(scale ? 1 : 0, reps, ms_pack, ms_total);
+    b->impl->time(scale ? 1 : 0, reps, ms_pack, ms_flux, ms_total);
+    return LCH_OK;
+  });
+}
+
+int lch_dpack_set_obs(lch_ctx* ctx, lch_dpack* b, const double* flux2d, const double* ferr2d,
+                      const double* weight2d) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  return guarded([&] {
+    b->impl->set_obs(flux2d, ferr2d, weight2d);
     return LCH_OK;
   });
 }
@@ -510,6 +521,9 @@ int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out) {
   r.input_bytes = static_cast<long long>(s.input_bytes);
   r.work_bytes = static_cast<long long>(s.work_bytes);
   r.visits = static_cast<long long>(s.visits);
+  r.pairs_evaluated = static_cast<long long>(s.pairs_evaluated);
+  r.pairs_by_moments = static_cast<long long>(s.pairs_by_moments);
+  r.moment_evals = static_cast<long long>(s.moment_evals);
   r.setup_core_s = s.setup_core_s;
   r.layout_core_s = s.layout_core_s;
   r.build_wall_s = s.build_wall_s;

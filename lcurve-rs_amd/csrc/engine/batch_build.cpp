@@ -100,14 +100,41 @@ bool linear_law_segment(const SegDev& sg, const SetDev& d) {
 
 namespace {
 
-// Direction moments of one patch (layout of MOM_STRIDE) from its records.
+// Direction moments of one patch (layout of MOM_STRIDE) from its records:
+// each component the tree_sum64 sum of the records' terms.
 void patch_moments(const uint8_t* pr, uint32_t stride, double* M) {
-  for (uint32_t i = 0; i < MOM_STRIDE; ++i) M[i] = 0.0;
+  double T[PATCH][MOM_STRIDE];
   for (uint32_t i = 0; i < PATCH; ++i) {
     float f[4];
     std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
-    accumulate_moments(M, f[0], f[1], f[2], f[3]);
+    moment_terms(T[i], f[0], f[1], f[2], f[3]);
   }
+  double a[PATCH];
+  for (uint32_t c = 0; c < MOM_STRIDE; ++c) {
+    for (uint32_t i = 0; i < PATCH; ++i) a[i] = T[i][c];
+    M[c] = tree_sum64(a);
+  }
+}
+
+// Cone of one patch of records, nr of them real (the rest padding).
+Cone patch_cone(const uint8_t* pr, uint32_t stride, uint32_t nr) {
+  double sx[PATCH], sy[PATCH], sz[PATCH];
+  for (uint32_t i = 0; i < PATCH; ++i) {
+    float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (i < nr) std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
+    sx[i] = i < nr ? static_cast<double>(f[0]) : 0.0;
+    sy[i] = i < nr ? static_cast<double>(f[1]) : 0.0;
+    sz[i] = i < nr ? static_cast<double>(f[2]) : 0.0;
+  }
+  ConeBuilder cb;
+  cb.set_sum(tree_sum64(sx), tree_sum64(sy), tree_sum64(sz), nr > 0);
+  cb.axis();
+  for (uint32_t i = 0; i < nr; ++i) {
+    float f[4];
+    std::memcpy(f, pr + static_cast<size_t>(i) * stride, sizeof f);
+    cb.bound(f[0], f[1], f[2]);
+  }
+  return cb.finish();
 }
 
 void push_lane(SetPack& S, double phi, uint32_t cls, const lc::Ginterp& gi) {
@@ -321,19 +348,9 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
       }
     }
     for (uint32_t p0 = 0; p0 < n_pad; p0 += PATCH) {
-      // The cone of the FP32 normals the records hold, in record order.
-      ConeBuilder cb;
-      const uint32_t p1 = std::min(p0 + PATCH, nr);
-      for (uint32_t i = p0; i < p1; ++i) {
-        const float* f = reinterpret_cast<const float*>(base + static_cast<size_t>(i) * stride);
-        cb.add(f[0], f[1], f[2]);
-      }
-      cb.axis();
-      for (uint32_t i = p0; i < p1; ++i) {
-        const float* f = reinterpret_cast<const float*>(base + static_cast<size_t>(i) * stride);
-        cb.bound(f[0], f[1], f[2]);
-      }
-      cones.push_back(cb.finish());
+      // The cone of the FP32 normals the records hold.
+      const uint32_t pn = nr > p0 ? std::min(PATCH, nr - p0) : 0u;
+      cones.push_back(patch_cone(base + static_cast<size_t>(p0) * stride, stride, pn));
     }
     if (keep_maps) {
       idx.resize(n_pad, UINT32_MAX);

@@ -48,15 +48,6 @@ int thread_count(int nthreads, size_t ns) {
   return static_cast<int>(std::min<size_t>(static_cast<size_t>(nt), std::max<size_t>(ns, 1)));
 }
 
-uint32_t bit_width(uint64_t v) {
-  uint32_t b = 0;
-  while (v) {
-    ++b;
-    v >>= 1;
-  }
-  return b;
-}
-
 }  // namespace
 
 struct SetLayout {
@@ -288,7 +279,11 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
   const uint64_t n_cand = n_chunks * max_tiles;
   if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
-  key_bits_ = 16 + bit_width(n_segs > 0 ? n_segs - 1 : 0);
+  key_bits_ = 16;
+  for (uint64_t v = n_segs > 0 ? n_segs - 1 : 0; v; v >>= 1) ++key_bits_;
+  uint64_t n_ktiles = 0;
+  for (const auto& Lp : lay)
+    for (const SetLayout::Grid& g : Lp->grids) n_ktiles += (g.n + KEY_TILE - 1) / KEY_TILE;
   st_.n_sets = ns;
   st_.n_segs = n_segs;
   st_.n_grids = n_grids;
@@ -310,6 +305,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   const size_t o_df = in.add(n_raw * sizeof(float4)), o_pos = in.add(n_pos * sizeof(float4));
   const size_t o_eoff = in.add((n_raw + 1) * sizeof(uint32_t)), o_ecl = in.add(n_pairs * sizeof(double2));
   const size_t o_grids = in.add(n_grids * sizeof(PackGrid)), o_psegs = in.add(n_segs * sizeof(PackSeg));
+  const size_t o_kt = in.add(n_ktiles * sizeof(KeyTile));
   const size_t o_segs = in.add(n_segs * sizeof(SegDev)), o_psets = in.add(ns * sizeof(PackSet));
   const size_t o_sets = in.add(ns * sizeof(SetDev)), o_pseg = in.add(n_patches * sizeof(uint32_t));
   const size_t o_pblk = in.add(n_pblocks * sizeof(PatchBlock)), o_eclp = in.add(n_eclp * sizeof(uint32_t));
@@ -320,6 +316,9 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   const size_t o_tsl = in.add(n_times * sizeof(double)), o_ty = in.add(n_times * sizeof(double));
   const size_t o_tfe = in.add(n_times * sizeof(double)), o_tw = in.add(n_times * sizeof(double));
   const size_t o_tph = in.add(n_times * sizeof(double));
+  o_ty_ = o_ty;
+  o_tfe_ = o_tfe;
+  o_tw_ = o_tw;
   in_bytes_ = in.total;
   st_.input_bytes = in_bytes_;
   hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_in_), in_bytes_, hipHostMallocDefault), "hipHostMalloc inputs");
@@ -331,6 +330,8 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   uint32_t* eoff = reinterpret_cast<uint32_t*>(at(o_eoff));
   double2* ecl = reinterpret_cast<double2*>(at(o_ecl));
   PackGrid* pgrids = reinterpret_cast<PackGrid*>(at(o_grids));
+  KeyTile* ktiles = reinterpret_cast<KeyTile*>(at(o_kt));
+  uint32_t kt = 0;
   PackSeg* psegs = reinterpret_cast<PackSeg*>(at(o_psegs));
   SegDev* segs = reinterpret_cast<SegDev*>(at(o_segs));
   PackSet* psets = reinterpret_cast<PackSet*>(at(o_psets));
@@ -388,6 +389,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
       G.e0 = e_set + g.e0;
       G.n = g.n;
       for (int k = 0; k <= MAX_ECL; ++k) G.seg_of[k] = g.seg_of[k] == UINT32_MAX ? UINT32_MAX : seg + g.seg_of[k];
+      for (uint32_t i0 = 0; i0 < g.n; i0 += KEY_TILE) ktiles[kt++] = KeyTile{grid, i0, std::min(KEY_TILE, g.n - i0)};
       pgrids[grid++] = G;
     }
     if (!L.df.empty()) std::memcpy(df + e, L.df.data(), L.df.size() * sizeof(float4));
@@ -481,7 +483,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
     recb += align256(L.rec_bytes);
     lay[s].reset();
   }
-  if (e != n_raw || seg != n_segs || patch != n_patches || tile != n_tiles || chunk != n_chunks || code != n_codes ||
+  if (kt != n_ktiles || e != n_raw || seg != n_segs || patch != n_patches || tile != n_tiles || chunk != n_chunks || code != n_codes ||
       prow != n_prows || ne_cl != n_eclp || pblk != n_pblocks || sorted != n_raw || eoff[n_raw] != n_pairs)
     throw DeviceError("device-pack layout out of step");
 
@@ -490,7 +492,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   const size_t w_keys = w.add(n_raw * 4), w_vals = w.add(n_raw * 4), w_keys_s = w.add(n_raw * 4),
                w_vals_s = w.add(n_raw * 4);
   const size_t w_recs = w.add(rec), w_cones = w.add(n_patches * sizeof(Cone));
-  const size_t w_moms = w.add(n_patches * MOM_STRIDE * sizeof(double)), w_wmax = w.add(n_segs * 8 + 8);
+  const size_t w_moms = w.add(n_patches * MOM_STRIDE * sizeof(double)), w_wmax = w.add(n_segs * 8);
   const size_t w_ten = w.add(n_tiles * TILE * sizeof(double4)), w_tval = w.add(n_tiles * 8),
                w_trng = w.add(n_tiles * sizeof(double2)), w_tcp = w.add(n_tiles * 4);
   const size_t w_codes = w.add(n_codes), w_efl = w.add(n_codes);
@@ -523,6 +525,8 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.n_raw = n_raw;
   p.grids = reinterpret_cast<const PackGrid*>(din(o_grids));
   p.n_grids = static_cast<uint32_t>(n_grids);
+  p.ktiles = reinterpret_cast<const KeyTile*>(din(o_kt));
+  p.n_ktiles = static_cast<uint32_t>(n_ktiles);
   p.psegs = reinterpret_cast<const PackSeg*>(din(o_psegs));
   p.segs = reinterpret_cast<SegDev*>(din(o_segs));
   p.n_segs = static_cast<uint32_t>(n_segs);
@@ -598,6 +602,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[3]), "hipEventCreate");
   hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
   hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
   hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
@@ -648,13 +653,43 @@ void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, 
   }
   if (flux && st_.n_times)
     hip_check(hipMemcpy(flux, d_.flux, st_.n_times * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
+  // The flux pass's work, counted from the items as the host-packed batch
+  // counts it.
+  std::vector<ItemDev> items(d_.n_items);
+  if (!items.empty())
+    hip_check(hipMemcpy(items.data(), pd_.items, items.size() * sizeof(ItemDev), hipMemcpyDeviceToHost),
+              "hipMemcpy items");
+  uint64_t pe = 0, pm = 0, me = 0;
+  for (const ItemDev& it : items) {
+    pe += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
+    pm += static_cast<uint64_t>(__builtin_popcountll(it.mom)) * PATCH * TILE;
+    me += it.mom ? TILE : 0u;
+  }
+  st_.pairs_evaluated = pe;
+  st_.pairs_by_moments = pm;
+  st_.moment_evals = me;
 }
 
-void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_total) {
+void DevicePackBatch::set_obs(const double* flux2d, const double* ferr2d, const double* weight2d) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  double tp = 0.0;
+  const size_t n = st_.n_times * sizeof(double);
+  const std::pair<const double*, size_t> obs[3] = {{flux2d, o_ty_}, {ferr2d, o_tfe_}, {weight2d, o_tw_}};
+  for (const auto& o : obs) {
+    if (!o.first || !n) continue;
+    std::memcpy(h_in_ + o.second, o.first, n);
+    hip_check(hipMemcpyAsync(d_in_ + o.second, h_in_ + o.second, n, hipMemcpyHostToDevice, st),
+              "hipMemcpyAsync observations");
+  }
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_flux, double* ms_total) {
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  hipStream_t st = ctx_.stream();
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  double tp = 0.0, tf = 0.0;
   hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
   for (int r = 0; r < reps; ++r) {
     hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
@@ -672,14 +707,20 @@ void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_tota
     d_.n_items = static_cast<uint32_t>(n);
     d_.n_items_plain = c.a;
     d_.n_items_ecl = c.b;
+    hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
     hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
+    hip_check(hipEventRecord(ev_[3], st), "hipEventRecord");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+    hip_check(hipEventSynchronize(ev_[3]), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[3]), "hipEventElapsedTime");
+    tf += ms;
   }
   hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
   hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
   float total = 0.0f;
   hip_check(hipEventElapsedTime(&total, ev_[2], ev_[1]), "hipEventElapsedTime");
   if (ms_pack) *ms_pack = reps > 0 ? tp / reps : 0.0;
+  if (ms_flux) *ms_flux = reps > 0 ? tf / reps : 0.0;
   if (ms_total) *ms_total = reps > 0 ? total / reps : 0.0;
 }
 

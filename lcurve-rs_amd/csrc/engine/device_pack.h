@@ -31,6 +31,10 @@ struct DevicePackStats {
   uint64_t work_bytes = 0;    // device tables the pack and the flux pass write
   uint64_t visits = 0;        // element-visits in the reference's loop order per pass
   uint64_t n_items = 0, n_items_plain = 0, n_items_ecl = 0;  // of the last pack
+  // Work of the flux pass of the last run(): (element, lane) pairs
+  // evaluated one by one, pairs summed from patch moments, moment sums
+  // evaluated (per lane).
+  uint64_t pairs_evaluated = 0, pairs_by_moments = 0, moment_evals = 0;
   double setup_core_s = 0;    // CPU setup (prepare_model), summed over threads
   double layout_core_s = 0;   // host layout and grid flattening, summed over threads
   double build_wall_s = 0;    // both, on the setup threads
@@ -64,9 +68,13 @@ class DevicePackBatch {
   void launch(int scale);
   // launch(), wait, copy back (any pointer may be null; flux [n_sets][n]).
   void run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux);
-  // Mean milliseconds per step over `reps` launches: of the pack alone
-  // (HIP events around its kernels) and of the whole step.
-  void time(int scale, int reps, double* ms_pack, double* ms_total);
+  // Mean milliseconds per step over `reps` launches: of the pack alone and
+  // of the flux pass alone (HIP events around their kernels), and of the
+  // whole step.
+  void time(int scale, int reps, double* ms_pack, double* ms_flux, double* ms_total);
+  // Replace the observations of every set ([n_sets][n_times] each; a null
+  // array keeps the current values), in the staging block and on the device.
+  void set_obs(const double* flux2d, const double* ferr2d, const double* weight2d);
   // Packs set `set` on the host (the same CPU setup, pack_set, limb lanes
   // off, this batch's chunk) and compares it with the device tables of the
   // last pack. diff[0] differing record bytes, [1] patch cones, [2] patch
@@ -111,12 +119,13 @@ class DevicePackBatch {
   size_t sort_temp_bytes_ = 0;
   void* scan_temp_ = nullptr;
   size_t scan_temp_bytes_ = 0;
-  uint32_t key_bits_ = 32;
+  uint32_t key_bits_ = 32;          // significant bits of the sort keys
   ItemCounts* h_counts_ = nullptr;  // pinned
 
   PackDev pd_;
   DeviceBatch d_;
-  hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+  size_t o_ty_ = 0, o_tfe_ = 0, o_tw_ = 0;  // observation tables in the input block
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   SideStream side_;
 };
 

@@ -54,6 +54,13 @@ struct PackChunk {
   uint32_t set, seg, e0, ne, col;  // col: chunk index within the set
 };
 
+// A run of up to KEY_TILE elements of one grid (the key and record
+// kernels' workgroups).
+constexpr uint32_t KEY_TILE = 4096;
+struct KeyTile {
+  uint32_t grid, i0, n;
+};
+
 // A block of up to 256 patches of one set (the code kernel's workgroups).
 struct PatchBlock {
   uint32_t set, patch0, n;
@@ -73,6 +80,8 @@ struct PackDev {
   uint64_t n_raw = 0;
   const PackGrid* grids = nullptr;
   uint32_t n_grids = 0;
+  const KeyTile* ktiles = nullptr;
+  uint32_t n_ktiles = 0;
   const PackSeg* psegs = nullptr;
   SegDev* segs = nullptr;            // the flux kernels' segment table (flags set here)
   uint32_t n_segs = 0;
@@ -93,6 +102,7 @@ struct PackDev {
   const float4* lane_e = nullptr;
   const uint32_t* lane_meta = nullptr;
   // work and outputs
+  uint32_t* wmax = nullptr;          // per segment: largest weight (FP32 bits), then a bad-weight flag
   uint32_t* keys = nullptr;          // [n_raw] x 2 (sort in / out)
   uint32_t* vals = nullptr;
   uint32_t* keys_s = nullptr;
@@ -100,7 +110,6 @@ struct PackDev {
   char* recs = nullptr;
   Cone* cones = nullptr;
   double* moms = nullptr;
-  uint32_t* wmax = nullptr;          // per segment: max weight bits, then bad flag
   double4* tile_en = nullptr;        // [n_tiles * TILE] normalised earth vectors
   uint64_t* tile_valid = nullptr;
   double2* tile_range = nullptr;     // (lo, hi) of phi - floor(phi) over valid lanes
@@ -118,7 +127,8 @@ struct PackDev {
   double* partials = nullptr;
 };
 
-// Bytes of the radix sort's temporary storage for n keys.
+// Bytes of the radix sort's temporary storage for n keys. The keys are
+// (segment << 16 | direction key): at most 65536 segments.
 size_t pack_sort_temp_bytes(uint64_t n);
 size_t pack_scan_temp_bytes(uint64_t n);
 // Every step of the pack on `st`; the item counts land in d.counts.

@@ -32,37 +32,100 @@ __device__ inline float4 load_f4(const char* r) {
   return make_float4(a.x, a.y, b.x, b.y);
 }
 
-// Sort key of every raw element: segment (high bits) and spatial key.
+// Sort key of every raw element: its segment above the spatial key. With
+// it, per segment, the largest weight and whether a weight is negative or
+// not finite, for the linear-law rescaling. One workgroup per key tile.
 __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (e >= d.n_raw) return;
-  uint32_t lo = 0, hi = d.n_grids;
-  while (hi - lo > 1u) {
-    const uint32_t mid = (lo + hi) / 2u;
-    if (d.grids[mid].e0 <= e) lo = mid;
-    else hi = mid;
+  constexpr int NK = MAX_ECL + 1;
+  __shared__ float red_w[kBlock / TILE][NK];
+  __shared__ uint32_t red_b[kBlock / TILE];
+  const KeyTile kt = d.ktiles[blockIdx.x];
+  const PackGrid G = d.grids[kt.grid];
+  float mx[NK];
+#pragma unroll
+  for (int c = 0; c < NK; ++c) mx[c] = 0.0f;
+  uint32_t bad = 0;  // bit k: a weight of eclipse-pair count k is negative or not finite
+  for (uint32_t i = threadIdx.x; i < kt.n; i += kBlock) {
+    const uint64_t e = G.e0 + kt.i0 + i;
+    uint32_t k = d.raw_eoff[e + 1] - d.raw_eoff[e];
+    k = k < static_cast<uint32_t>(MAX_ECL) ? k : static_cast<uint32_t>(MAX_ECL);
+    const float4 v = d.raw_df[e];
+    d.keys[e] = (G.seg_of[k] << 16) | direction_key16(v.x, v.y, v.z);
+    d.vals[e] = static_cast<uint32_t>(e);
+    const float w = v.w;
+    if (!(w >= 0.0f) || !isfinite(w)) {
+      bad |= 1u << k;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NK; ++c) mx[c] = (static_cast<uint32_t>(c) == k && w > mx[c]) ? w : mx[c];
+    }
   }
-  uint32_t k = d.raw_eoff[e + 1] - d.raw_eoff[e];
-  k = k < static_cast<uint32_t>(MAX_ECL) ? k : static_cast<uint32_t>(MAX_ECL);
-  const uint32_t seg = d.grids[lo].seg_of[k];
-  const float4 v = d.raw_df[e];
-  d.keys[e] = (seg << 16) | direction_key16(v.x, v.y, v.z);
-  d.vals[e] = static_cast<uint32_t>(e);
+#pragma unroll
+  for (int c = 0; c < NK; ++c)
+    for (int off = 32; off > 0; off >>= 1) {
+      const float o = __shfl_xor(mx[c], off);
+      mx[c] = o > mx[c] ? o : mx[c];
+    }
+  for (int off = 32; off > 0; off >>= 1) bad |= __shfl_xor(bad, off);
+  const uint32_t wv = threadIdx.x / TILE;
+  if (threadIdx.x % TILE == 0) {
+#pragma unroll
+    for (int c = 0; c < NK; ++c) red_w[wv][c] = mx[c];
+    red_b[wv] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x < NK) {
+    const uint32_t c = threadIdx.x;
+    const uint32_t seg = G.seg_of[c];
+    if (seg != 0xFFFFFFFFu) {
+      float wm = 0.0f;
+      uint32_t b = 0;
+      for (uint32_t i = 0; i < kBlock / TILE; ++i) {
+        wm = red_w[i][c] > wm ? red_w[i][c] : wm;
+        b |= red_b[i];
+      }
+      // Non-negative FP32 values order as their bit patterns.
+      if (wm > 0.0f) atomicMax(&d.wmax[2u * seg], __float_as_uint(wm));
+      if ((b >> c) & 1u) atomicOr(&d.wmax[2u * seg + 1u], 1u);
+    }
+  }
 }
 
-// Element records in sorted order: each segment's elements by spatial key,
-// ties in raw (grid) order.
-__global__ __launch_bounds__(kBlock) void records_kernel(PackDev d) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= d.n_raw) return;
-  const uint32_t seg = d.keys_s[p] >> 16;
+// Flags of each segment for the flux kernels: the base flags, plus the
+// linear-law rescaling with its exponent k (sqrt(w max) < 2^k) where the
+// segment allows it and all its weights are finite and >= 0, the largest
+// > 0.
+__global__ __launch_bounds__(kBlock) void seg_flags_kernel(PackDev d) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= d.n_segs) return;
+  const PackSeg S = d.psegs[s];
+  uint32_t flags = S.base_flags;
+  const float wm = __uint_as_float(d.wmax[2u * s]);
+  if ((S.flags & PSEG_LINW) && d.wmax[2u * s + 1u] == 0u && wm != 0.0f) {
+    int k = 0;
+    (void)frexp(sqrt(static_cast<double>(wm)), &k);
+    if (k >= -60 && k <= 60) flags |= SEG_LINW | (static_cast<uint32_t>(k + 128) << 8);
+  }
+  d.segs[s].flags = flags;
+}
+
+// The record of sorted position p, an element of segment seg.
+__device__ inline void record_at(const PackDev& d, uint64_t p, uint32_t seg) {
   const uint32_t e = d.vals_s[p];
   const PackSeg S = d.psegs[seg];
+  const uint32_t flags = d.segs[seg].flags;
   const uint32_t rank = static_cast<uint32_t>(p) - S.sorted0;
   const uint32_t stride = stride_of(S);
   const uint32_t eoff = S.posn ? 32u : 16u;
   char* r = d.recs + S.rec_off + static_cast<uint64_t>(rank) * stride;
-  store_f4(r, d.raw_df[e]);
+  float4 f = d.raw_df[e];
+  if (flags & SEG_LINW) {
+    const double inv = ldexp(1.0, -seg_linw_exp(flags));
+    const double sc = sqrt(static_cast<double>(f.w)) * inv;
+    f = make_float4(static_cast<float>(static_cast<double>(f.x) * sc), static_cast<float>(static_cast<double>(f.y) * sc),
+                    static_cast<float>(static_cast<double>(f.z) * sc), static_cast<float>(sc));
+  }
+  store_f4(r, f);
   if (S.posn) {
     float4 q = d.raw_pos[S.pos0 + (e - S.e0)];
     q.w = 0.0f;
@@ -76,6 +139,26 @@ __global__ __launch_bounds__(kBlock) void records_kernel(PackDev d) {
     o[1] = pr.y;
     o[2] = pr.y - 1.0;
   }
+}
+
+// Element records in sorted order: each segment's elements by spatial key,
+// ties in raw (grid) order. Linear-law segments (SEG_LINW) take (s d, s),
+// s = sqrt(w) / 2^k, in place of (d, w).
+__global__ __launch_bounds__(kBlock) void records_kernel(PackDev d) {
+  const KeyTile kt = d.ktiles[blockIdx.x];
+  const PackGrid G = d.grids[kt.grid];
+  for (uint32_t i = threadIdx.x; i < kt.n; i += kBlock) {
+    const uint64_t p = G.e0 + kt.i0 + i;
+    record_at(d, p, d.keys_s[p] >> 16);
+  }
+}
+
+// The plain (dirn, flux) of slot j of patch q of segment S (zeros for a
+// padding slot), from the raw grid through the sorted order.
+__device__ inline float4 plain_slot(const PackDev& d, const PackSeg& S, uint32_t q, uint32_t j) {
+  const uint32_t slot = q * PATCH + j;
+  if (slot >= S.n_real) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  return d.raw_df[d.vals_s[S.sorted0 + slot]];
 }
 
 // Padding records of each segment: zero normal and weight; intervals that
@@ -96,40 +179,6 @@ __global__ __launch_bounds__(PATCH) void pad_kernel(PackDev d) {
   }
 }
 
-// Cone and (for moment-capable segments) direction moments of each patch,
-// from the plain records, in record order.
-__global__ __launch_bounds__(kBlock) void patches_kernel(PackDev d) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= d.n_patches) return;
-  const PackSeg S = d.psegs[d.patch_seg[p]];
-  const uint32_t q = p - S.patch0;
-  const uint32_t stride = stride_of(S);
-  const char* base = d.recs + S.rec_off + static_cast<uint64_t>(q) * PATCH * stride;
-  const uint32_t first = q * PATCH;
-  const uint32_t nr = S.n_real > first ? (S.n_real - first < PATCH ? S.n_real - first : PATCH) : 0u;
-  ConeBuilder cb;
-  for (uint32_t i = 0; i < nr; ++i) {
-    const float4 f = load_f4(base + static_cast<size_t>(i) * stride);
-    cb.add(f.x, f.y, f.z);
-  }
-  cb.axis();
-  for (uint32_t i = 0; i < nr; ++i) {
-    const float4 f = load_f4(base + static_cast<size_t>(i) * stride);
-    cb.bound(f.x, f.y, f.z);
-  }
-  d.cones[p] = cb.finish();
-  double M[MOM_STRIDE];
-  for (uint32_t c = 0; c < MOM_STRIDE; ++c) M[c] = 0.0;
-  if (S.flags & PSEG_MOM) {
-    for (uint32_t i = 0; i < PATCH; ++i) {
-      const float4 f = load_f4(base + static_cast<size_t>(i) * stride);
-      accumulate_moments(M, f.x, f.y, f.z, f.w);
-    }
-  }
-  double* o = d.moms + static_cast<uint64_t>(p) * MOM_STRIDE;
-  for (uint32_t c = 0; c < MOM_STRIDE; ++c) o[c] = M[c];
-}
-
 __device__ inline double wave_min(double v) {
   for (int off = 32; off > 0; off >>= 1) {
     const double o = __shfl_xor(v, off);
@@ -147,6 +196,57 @@ __device__ inline double wave_max(double v) {
 __device__ inline uint32_t wave_or(uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
   return v;
+}
+
+// Cone and (for moment-capable segments) direction moments of each patch,
+// from the plain values: one wavefront per patch, a lane per slot. Each sum
+// is tree_sum64's: its first level (lane i + lane i + 32) by a shuffle, the
+// rest by one lane per sum over the 32 partial sums in LDS.
+__global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
+  constexpr uint32_t NC = MOM_STRIDE + 3;  // cone sums x, y, z, then the moments
+  // Row stride of 23 doubles: conflict-free row writes and column reads.
+  __shared__ double part[PATCH / 2][NC];
+  const uint32_t p = blockIdx.x;
+  const uint32_t j = threadIdx.x;
+  const PackSeg S = d.psegs[d.patch_seg[p]];
+  const uint32_t q = p - S.patch0;
+  const uint32_t first = q * PATCH;
+  const uint32_t nr = S.n_real > first ? (S.n_real - first < PATCH ? S.n_real - first : PATCH) : 0u;
+  const bool real = j < nr;
+  const float4 f = plain_slot(d, S, q, j);
+  const bool mom = (S.flags & PSEG_MOM) != 0;
+  auto level1 = [&](double v, uint32_t c) {
+    const double o = __shfl_xor(v, 32);
+    if (j < PATCH / 2) part[j][c] = v + o;
+  };
+  level1(real ? static_cast<double>(f.x) : 0.0, 0);
+  level1(real ? static_cast<double>(f.y) : 0.0, 1);
+  level1(real ? static_cast<double>(f.z) : 0.0, 2);
+  if (mom) {
+    double T[MOM_STRIDE];
+    moment_terms(T, f.x, f.y, f.z, f.w);
+#pragma unroll
+    for (uint32_t c = 0; c < MOM_STRIDE; ++c) level1(T[c], 3 + c);
+  }
+  __syncthreads();
+  double sum = 0.0;
+  if (j < (mom ? NC : 3u)) {
+    double a[PATCH / 2];
+#pragma unroll
+    for (uint32_t i = 0; i < PATCH / 2; ++i) a[i] = part[i][j];
+#pragma unroll
+    for (uint32_t off = PATCH / 4; off > 0; off >>= 1)
+#pragma unroll
+      for (uint32_t i = 0; i < off; ++i) a[i] = a[i] + a[i + off];
+    sum = a[0];
+  }
+  ConeBuilder cb;
+  cb.set_sum(__shfl(sum, 0), __shfl(sum, 1), __shfl(sum, 2), nr > 0);
+  cb.axis();
+  cb.mn = wave_min(real ? cb.cosine(f.x, f.y, f.z) : 1.0);
+  const Cone c = cb.finish();
+  if (j == 0) d.cones[p] = c;
+  if (j >= 3 && j < NC) d.moms[static_cast<uint64_t>(p) * MOM_STRIDE + (j - 3)] = mom ? sum : 0.0;
 }
 
 // Per lane tile: normalised FP32 earth vectors, valid lanes, the range of
@@ -190,7 +290,7 @@ __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
   const uint32_t stride = stride_of(S);
   const uint32_t eoff = S.posn ? 32u : 16u;
   const char* r = d.recs + S.rec_off + (static_cast<uint64_t>(q) * PATCH + threadIdx.x) * stride;
-  const float w = load_f4(r).w;
+  const float w = plain_slot(d, S, q, threadIdx.x).w;
   double e3[MAX_ECL][3];
   for (uint32_t j = 0; j < S.necl; ++j) {
     const double* s = reinterpret_cast<const double*>(r + eoff + 24u * j);
@@ -264,56 +364,6 @@ __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
     }
   }
   d.codes[ci] = code;
-}
-
-// Largest weight of each linear-law segment (non-negative FP32 values
-// compare as their bit patterns), and whether any weight is negative or not
-// finite.
-__global__ __launch_bounds__(kBlock) void linw_max_kernel(PackDev d) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= d.n_raw) return;
-  const uint32_t seg = d.keys_s[p] >> 16;
-  if (!(d.psegs[seg].flags & PSEG_LINW)) return;
-  const float w = d.raw_df[d.vals_s[p]].w;
-  if (!(w >= 0.0f) || !isfinite(w)) atomicOr(&d.wmax[2u * seg + 1u], 1u);
-  else if (w > 0.0f) atomicMax(&d.wmax[2u * seg], __float_as_uint(w));  // zeros (and -0) leave the max
-}
-
-// Segment flags of the flux kernels: the base flags, plus the linear-law
-// rescaling with its exponent k (sqrt(w max) < 2^k) where it applies.
-__global__ __launch_bounds__(kBlock) void linw_flags_kernel(PackDev d) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= d.n_segs) return;
-  const PackSeg S = d.psegs[s];
-  uint32_t flags = S.base_flags;
-  if (S.flags & PSEG_LINW) {
-    const float wm = __uint_as_float(d.wmax[2u * s]);
-    if (d.wmax[2u * s + 1u] == 0u && wm != 0.0f) {
-      int k = 0;
-      (void)frexp(sqrt(static_cast<double>(wm)), &k);
-      if (k >= -60 && k <= 60) flags |= SEG_LINW | (static_cast<uint32_t>(k + 128) << 8);
-    }
-  }
-  d.segs[s].flags = flags;
-}
-
-// (s d, s) in place of (d, w), s = sqrt(w) / 2^k, in the records of
-// rescaled segments (the padding records hold zeros and stay so).
-__global__ __launch_bounds__(kBlock) void linw_scale_kernel(PackDev d) {
-  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= d.n_raw) return;
-  const uint32_t seg = d.keys_s[p] >> 16;
-  const uint32_t flags = d.segs[seg].flags;
-  if (!(flags & SEG_LINW)) return;
-  const PackSeg S = d.psegs[seg];
-  const uint32_t rank = static_cast<uint32_t>(p) - S.sorted0;
-  char* r = d.recs + S.rec_off + static_cast<uint64_t>(rank) * stride_of(S);
-  const float4 f = load_f4(r);
-  const double inv = ldexp(1.0, -seg_linw_exp(flags));
-  const double sc = sqrt(static_cast<double>(f.w)) * inv;
-  store_f4(r, make_float4(static_cast<float>(static_cast<double>(f.x) * sc),
-                          static_cast<float>(static_cast<double>(f.y) * sc),
-                          static_cast<float>(static_cast<double>(f.z) * sc), static_cast<float>(sc)));
 }
 
 // Candidate work item of every (chunk, tile): its patch masks and kernel
@@ -394,20 +444,31 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(PackDev d) {
   d.items[out] = it;
 }
 
-// Summed moments of each item's PT_MOM patches, in patch order.
+// Summed moments of each item's PT_MOM patches, in patch order: a thread
+// per item.
 __global__ __launch_bounds__(kBlock) void imoms_kernel(PackDev d, uint64_t capacity) {
-  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const uint64_t i = g / MOM_STRIDE;
-  const uint32_t c = static_cast<uint32_t>(g % MOM_STRIDE);
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= capacity) return;
   const ItemCounts n = *d.counts;
   if (i >= static_cast<uint64_t>(n.a) + n.b + n.c) return;
   const ItemDev it = d.items[i];
   if (!it.mom) return;
   const double* m0 = d.moms + (static_cast<uint64_t>(d.segs[it.seg].mom_off) + it.e0 / PATCH) * MOM_STRIDE;
-  double s = 0.0;
-  for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) s += m0[static_cast<uint64_t>(__builtin_ctzll(mm)) * MOM_STRIDE + c];
-  d.imoms[i * MOM_STRIDE + c] = s;
+  double s[MOM_STRIDE];
+#pragma unroll
+  for (uint32_t c = 0; c < MOM_STRIDE; ++c) s[c] = 0.0;
+  for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) {
+    const double2* a = reinterpret_cast<const double2*>(m0 + static_cast<uint64_t>(__builtin_ctzll(mm)) * MOM_STRIDE);
+#pragma unroll
+    for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) {
+      const double2 v = a[c];
+      s[2 * c] = s[2 * c] + v.x;
+      s[2 * c + 1] = s[2 * c + 1] + v.y;
+    }
+  }
+  double2* o = reinterpret_cast<double2*>(d.imoms + i * MOM_STRIDE);
+#pragma unroll
+  for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) o[c] = make_double2(s[2 * c], s[2 * c + 1]);
 }
 
 uint32_t blocks(uint64_t n, uint32_t b = kBlock) { return static_cast<uint32_t>((n + b - 1) / b); }
@@ -433,27 +494,25 @@ size_t pack_scan_temp_bytes(uint64_t n) {
 hipError_t launch_pack(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
                        size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st) {
   hipError_t e = hipSuccess;
+  if (d.n_segs) {
+    e = hipMemsetAsync(d.wmax, 0, static_cast<size_t>(d.n_segs) * 2u * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+  }
   if (d.n_raw) {
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks(d.n_raw)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(keys_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
     size_t tb = sort_temp_bytes;
     e = rocprim::radix_sort_pairs(sort_temp, tb, d.keys, d.keys_s, d.vals, d.vals_s, static_cast<size_t>(d.n_raw),
                                   0, key_bits, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(records_kernel, dim3(blocks(d.n_raw)), dim3(kBlock), 0, st, d);
   }
+  if (d.n_segs) hipLaunchKernelGGL(seg_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
+  if (d.n_raw) hipLaunchKernelGGL(records_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
   if (d.n_segs) hipLaunchKernelGGL(pad_kernel, dim3(d.n_segs), dim3(PATCH), 0, st, d);
-  if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(blocks(d.n_patches)), dim3(kBlock), 0, st, d);
+  if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(d.n_patches), dim3(TILE), 0, st, d);
   if (d.n_tiles) hipLaunchKernelGGL(tiles_kernel, dim3(d.n_tiles), dim3(TILE), 0, st, d);
   if (d.n_ecl_patches) hipLaunchKernelGGL(eclflags_kernel, dim3(d.n_ecl_patches), dim3(TILE), 0, st, d);
   if (d.n_pblocks && d.max_tiles)
     hipLaunchKernelGGL(codes_kernel, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
-  // The linear-law rescaling after the cones, moments and codes, which are
-  // made from the plain records.
-  e = hipMemsetAsync(d.wmax, 0, static_cast<size_t>(d.n_segs) * 2u * sizeof(uint32_t) + 8u, st);
-  if (e != hipSuccess) return e;
-  if (d.n_raw) hipLaunchKernelGGL(linw_max_kernel, dim3(blocks(d.n_raw)), dim3(kBlock), 0, st, d);
-  if (d.n_segs) hipLaunchKernelGGL(linw_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
-  if (d.n_raw) hipLaunchKernelGGL(linw_scale_kernel, dim3(blocks(d.n_raw)), dim3(kBlock), 0, st, d);
   const uint64_t nc = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
   if (nc) {
     hipLaunchKernelGGL(items_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
@@ -463,7 +522,7 @@ hipError_t launch_pack(const PackDev& d, void* sort_temp, size_t sort_temp_bytes
                                 AddCounts(), st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
-    hipLaunchKernelGGL(imoms_kernel, dim3(blocks(nc * MOM_STRIDE)), dim3(kBlock), 0, st, d, nc);
+    hipLaunchKernelGGL(imoms_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d, nc);
   } else {
     e = hipMemsetAsync(d.counts, 0, sizeof(ItemCounts), st);
     if (e != hipSuccess) return e;

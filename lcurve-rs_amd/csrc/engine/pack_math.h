@@ -56,6 +56,18 @@ __host__ __device__ inline uint32_t direction_key16(float fx, float fy, float fz
   return key;
 }
 
+// Sums over the 64 slots of a patch are taken in one fixed order: the
+// xor butterfly of a wavefront (offsets 32, 16, ..., 1, each lane adding
+// its partner's value), after which every lane holds the same total. Lane
+// 0's chain alone is the tree below: 63 additions, the lower half at each
+// offset (a[i] + a[i + off] is lane i's addition; the upper lanes repeat
+// the lower ones' values). `a` is overwritten.
+__host__ __device__ inline double tree_sum64(double* a) {
+  for (int off = 32; off > 0; off >>= 1)
+    for (int i = 0; i < off; ++i) a[i] = a[i] + a[i + off];
+  return a[0];
+}
+
 // Normal cone of a patch: unit axis (x, y, z) and cosw, a lower bound on
 // the cosine of the angle between the axis and any normal of the patch
 // (-1 or below: no bound). An empty patch or a zero mean has cosw < -1.
@@ -63,19 +75,19 @@ struct Cone {
   float x, y, z, cosw;
 };
 
-// Accumulates a patch's FP32 normals in element order (the mean), then the
-// smallest cosine to the normalised mean: call add() for every element,
-// axis(), then bound() for every element, then finish().
+// A patch's cone from the sum of its FP32 normals (tree_sum64 order over
+// the 64 slots, padding slots adding 0): set_sum(), axis(), then bound()
+// for every element (a minimum, in any order), then finish().
 struct ConeBuilder {
   double mx = 0.0, my = 0.0, mz = 0.0;
   double ux = 0.0, uy = 0.0, uz = 0.0;
   double mn = 1.0;
   bool any = false, ok = false;
-  __host__ __device__ void add(float x, float y, float z) {
-    mx += static_cast<double>(x);
-    my += static_cast<double>(y);
-    mz += static_cast<double>(z);
-    any = true;
+  __host__ __device__ void set_sum(double sx, double sy, double sz, bool nonempty) {
+    mx = sx;
+    my = sy;
+    mz = sz;
+    any = nonempty;
   }
   __host__ __device__ void axis() {
     const double len = sqrt(mx * mx + my * my + mz * mz);
@@ -87,13 +99,18 @@ struct ConeBuilder {
       uz = inv * mz;
     }
   }
-  __host__ __device__ void bound(float x, float y, float z) {
-    if (!ok) return;
+  // Cosine between the axis and one normal (clamped to [-1, 1]), or 1
+  // (no constraint) for a zero normal or a cone without an axis.
+  __host__ __device__ double cosine(float x, float y, float z) const {
+    if (!ok) return 1.0;
     const double dx = x, dy = y, dz = z;
     const double dl = sqrt(dx * dx + dy * dy + dz * dz);
-    if (!(dl > 0.0)) return;
-    double ca = (ux * dx + uy * dy + uz * dz) / dl;
-    ca = ca < -1.0 ? -1.0 : (ca > 1.0 ? 1.0 : ca);
+    if (!(dl > 0.0)) return 1.0;
+    const double ca = (ux * dx + uy * dy + uz * dz) / dl;
+    return ca < -1.0 ? -1.0 : (ca > 1.0 ? 1.0 : ca);
+  }
+  __host__ __device__ void bound(float x, float y, float z) {
+    const double ca = cosine(x, y, z);
     mn = ca < mn ? ca : mn;
   }
   __host__ __device__ Cone finish() const {
@@ -159,33 +176,37 @@ __host__ __device__ inline double cos_front(const CodeConsts& k, float cosw) {
   return k.cb * static_cast<double>(cosw) + k.sb * cone_sin(cosw);
 }
 
-// Adds one record's normal d and weight w to a patch's direction moments
-// (layout of MOM_STRIDE in layout.h): sum w d, the coefficients of
-// sum w (e.d)^2 and of sum w (e.d)^3 in the monomials of e, and sum |w|.
-// Records of zero weight (padding) add nothing.
-__host__ __device__ inline void accumulate_moments(double* M, float fx, float fy, float fz, float fw) {
+// One record's terms of a patch's direction moments (layout of MOM_STRIDE
+// in layout.h) from its normal d and weight w: w d, the coefficients of
+// w (e.d)^2 and of w (e.d)^3 in the monomials of e, and |w|. A record of
+// zero weight (padding) has all terms +0. The moments are the tree_sum64
+// sums of the terms over a patch's 64 slots.
+__host__ __device__ inline void moment_terms(double* T, float fx, float fy, float fz, float fw) {
   const double x = fx, y = fy, z = fz, w = fw;
-  if (w == 0.0) return;
-  M[0] += w * x;
-  M[1] += w * y;
-  M[2] += w * z;
-  M[3] += w * x * x;
-  M[4] += w * y * y;
-  M[5] += w * z * z;
-  M[6] += 2.0 * w * x * y;
-  M[7] += 2.0 * w * x * z;
-  M[8] += 2.0 * w * y * z;
-  M[9] += w * x * x * x;
-  M[10] += w * y * y * y;
-  M[11] += w * z * z * z;
-  M[12] += 3.0 * w * x * x * y;
-  M[13] += 3.0 * w * x * x * z;
-  M[14] += 3.0 * w * y * y * x;
-  M[15] += 3.0 * w * y * y * z;
-  M[16] += 3.0 * w * z * z * x;
-  M[17] += 3.0 * w * z * z * y;
-  M[18] += 6.0 * w * x * y * z;
-  M[19] += w < 0.0 ? -w : w;  // size bound for the limb lanes' fixed point
+  if (w == 0.0) {
+    for (uint32_t c = 0; c < MOM_STRIDE; ++c) T[c] = 0.0;
+    return;
+  }
+  T[0] = w * x;
+  T[1] = w * y;
+  T[2] = w * z;
+  T[3] = w * x * x;
+  T[4] = w * y * y;
+  T[5] = w * z * z;
+  T[6] = 2.0 * w * x * y;
+  T[7] = 2.0 * w * x * z;
+  T[8] = 2.0 * w * y * z;
+  T[9] = w * x * x * x;
+  T[10] = w * y * y * y;
+  T[11] = w * z * z * z;
+  T[12] = 3.0 * w * x * x * y;
+  T[13] = 3.0 * w * x * x * z;
+  T[14] = 3.0 * w * y * y * x;
+  T[15] = 3.0 * w * y * y * z;
+  T[16] = 3.0 * w * z * z * x;
+  T[17] = 3.0 * w * z * z * y;
+  T[18] = 6.0 * w * x * y * z;
+  T[19] = w < 0.0 ? -w : w;  // size bound for the limb lanes' fixed point
 }
 
 // Point::visible over a tile's phase range [lo, hi] (types.rs:50-58, the

@@ -103,6 +103,11 @@ def set_limb(on, max_part=48):
     check(_native.lib().lch_ctx_set_limb(_ctx(), int(max_part) if on else 0))
 
 
+def sync():
+    """Wait for the work enqueued on the current device's context stream."""
+    check(_native.lib().lch_ctx_sync(_ctx()))
+
+
 def _dptr(a):
     return a.ctypes.data_as(_native.D)
 
@@ -498,12 +503,23 @@ class DevicePackBatch:
         """Copy the staged grids and tables to the device again."""
         check(_native.lib().lch_dpack_upload(_ctx(), self._h))
 
+    def sync(self):
+        sync()
+
     def time(self, scale=True, reps=5):
-        """Mean ms per launch of the pack alone and of the whole launch."""
-        a, b = ctypes.c_double(), ctypes.c_double()
+        """Mean ms per launch of the pack alone, the flux pass alone and the
+        whole launch."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         check(_native.lib().lch_dpack_time(_ctx(), self._h, int(bool(scale)), int(reps),
-                                           ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+                                           ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_obs(self, flux2d=None, ferr2d=None, weight2d=None):
+        """Replace the observations of all sets ([n_sets, n_times] arrays)."""
+        shp = (self.n_sets, self.n_times)
+        arrs = [None if a is None else np.ascontiguousarray(np.broadcast_to(a, shp), dtype=np.float64)
+                for a in (flux2d, ferr2d, weight2d)]
+        check(_native.lib().lch_dpack_set_obs(_ctx(), self._h, *[None if a is None else _dptr(a) for a in arrs]))
 
     def stats(self):
         s = _native.DPackStatsC()

@@ -50,7 +50,8 @@ class BatchStatsC(ctypes.Structure):
 class DPackStatsC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in (
         "n_sets", "n_valid", "n_segs", "n_elements", "n_pairs", "n_patches", "n_tiles", "n_chunks", "n_codes",
-        "n_cand", "n_items", "n_items_plain", "n_items_ecl", "rec_bytes", "input_bytes", "work_bytes", "visits")] + \
+        "n_cand", "n_items", "n_items_plain", "n_items_ecl", "rec_bytes", "input_bytes", "work_bytes", "visits",
+        "pairs_evaluated", "pairs_by_moments", "moment_evals")] + \
         [(k, ctypes.c_double) for k in ("setup_core_s", "layout_core_s", "build_wall_s", "concat_s", "upload_s")] + \
         [("chunk", ctypes.c_int), ("threads", ctypes.c_int)]
 
@@ -119,7 +120,8 @@ SIGNATURES = {
     "lch_dpack_upload": (INT, [VP, VP]),
     "lch_dpack_launch": (INT, [VP, VP, INT]),
     "lch_dpack_run": (INT, [VP, VP, INT, D, D, D, D, D]),
-    "lch_dpack_time": (INT, [VP, VP, INT, INT, D, D]),
+    "lch_dpack_time": (INT, [VP, VP, INT, INT, D, D, D]),
+    "lch_dpack_set_obs": (INT, [VP, VP, D, D, D]),
     "lch_dpack_stats_get": (INT, [VP, ctypes.POINTER(DPackStatsC)]),
     "lch_dpack_check": (INT, [VP, VP, INT, LL]),
     "lch_batch_sfac": (INT, [VP, D]),
