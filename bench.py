@@ -297,7 +297,7 @@ def roofline_dpack(st, ms_flux, ms_pack, pm):
 def bench_chisq(args):
     """chisq_4096: fresh sets from resident raw grids (the value), the
     PCIe-inclusive rate and the host-packed replay beside it."""
-    chunk = args.chunk if args.chunk > 0 else 2048
+    chunk = args.chunk if args.chunk > 0 else 4096
     dp, models, info = build_chisq_dpack(args.nsets, chunk)
     scale = info["scale"]
     nsets = info["unit_per_step"]

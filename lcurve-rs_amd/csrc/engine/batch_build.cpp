@@ -477,7 +477,7 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
   struct TileInfo {
     bool any = false;
     double lo = 0.0, hi = 0.0;  // min / max of phi - floor(phi)
-    std::vector<lc::Vec3> es;
+    std::vector<float> ex, ey, ez;  // unit lines of sight of the valid lanes, FP32
   };
   std::vector<TileInfo> tiles(ntile);
   for (uint32_t t = 0; t < ntile; ++t) {
@@ -488,7 +488,10 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
       const float4 e = S.lane_e[l];
       lc::Vec3 v(e.x, e.y, e.z);
       const double len = v.length();
-      ti.es.push_back(len > 0.0 ? (1.0 / len) * v : v);
+      if (len > 0.0) v = (1.0 / len) * v;
+      ti.ex.push_back(static_cast<float>(v.x));
+      ti.ey.push_back(static_cast<float>(v.y));
+      ti.ez.push_back(static_cast<float>(v.z));
       const double phif = S.lane_phi[l] - std::floor(S.lane_phi[l]);
       ti.lo = ti.any ? std::min(ti.lo, phif) : phif;
       ti.hi = ti.any ? std::max(ti.hi, phif) : phif;
@@ -588,11 +591,12 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
         // Cosines between the patch axis and the tile's lines of sight span
         // [emin, emax]; a normal of the patch has cosine >= cosw with the
         // axis.
-        double emin = 1.0, emax = -1.0;
-        for (const lc::Vec3& e : ti.es) {
-          const double c = lc::dot(e, pa);
-          emin = std::min(emin, c);
-          emax = std::max(emax, c);
+        float emin = 1.0f, emax = -1.0f;
+        const float fx = static_cast<float>(pa.x), fy = static_cast<float>(pa.y), fz = static_cast<float>(pa.z);
+        for (size_t k = 0; k < ti.ex.size(); ++k) {
+          const float c = los_cosine(ti.ex[k], ti.ey[k], ti.ez[k], fx, fy, fz);
+          emin = c < emin ? c : emin;
+          emax = c > emax ? c : emax;
         }
         // Back-facing: every normal is more than acos(thr) + margin from
         // every line of sight, so every FP32 mu <= thr. The margin (2e-3

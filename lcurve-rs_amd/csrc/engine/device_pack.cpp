@@ -493,7 +493,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
                w_vals_s = w.add(n_raw * 4);
   const size_t w_recs = w.add(rec), w_cones = w.add(n_patches * sizeof(Cone));
   const size_t w_moms = w.add(n_patches * MOM_STRIDE * sizeof(double)), w_wmax = w.add(n_segs * 8);
-  const size_t w_ten = w.add(n_tiles * TILE * sizeof(double4)), w_tval = w.add(n_tiles * 8),
+  const size_t w_ten = w.add(n_tiles * TILE * sizeof(float4)), w_tval = w.add(n_tiles * 8),
                w_trng = w.add(n_tiles * sizeof(double2)), w_tcp = w.add(n_tiles * 4);
   const size_t w_codes = w.add(n_codes), w_efl = w.add(n_codes);
   const size_t w_cand = w.add(n_cand * sizeof(ItemDev)), w_ccls = w.add(n_cand),
@@ -554,7 +554,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.cones = reinterpret_cast<Cone*>(dw(w_cones));
   p.moms = reinterpret_cast<double*>(dw(w_moms));
   p.wmax = reinterpret_cast<uint32_t*>(dw(w_wmax));
-  p.tile_en = reinterpret_cast<double4*>(dw(w_ten));
+  p.tile_en = reinterpret_cast<float4*>(dw(w_ten));
   p.tile_valid = reinterpret_cast<uint64_t*>(dw(w_tval));
   p.tile_range = reinterpret_cast<double2*>(dw(w_trng));
   p.tile_cp = reinterpret_cast<uint32_t*>(dw(w_tcp));
@@ -770,10 +770,13 @@ void DevicePackBatch::check(int set, uint64_t diff[8]) {
   get(moms.data(), pd_.moms + static_cast<size_t>(ps.patch0) * MOM_STRIDE, moms.size() * sizeof(double));
   for (size_t i = 0; i < moms.size(); ++i)
     if (std::memcmp(&moms[i], &pack->moms[i], sizeof(double)) != 0) ++diff[2];
+  // The device's codes are tile-major, the host's patch-major.
   std::vector<uint8_t> codes(static_cast<size_t>(ps.n_patches) * ps.n_tiles);
   get(codes.data(), pd_.codes + ps.code0, codes.size());
-  for (size_t i = 0; i < codes.size(); ++i)
-    if (codes[i] != pack->ptcode[i]) ++diff[3];
+  for (uint32_t t = 0; t < ps.n_tiles; ++t)
+    for (uint32_t q = 0; q < ps.n_patches; ++q)
+      if (codes[static_cast<size_t>(t) * ps.n_patches + q] != pack->ptcode[static_cast<size_t>(q) * ps.n_tiles + t])
+        ++diff[3];
   // Records and segment flags after the linear-law rescaling.
   scale_linear_records(*pack);
   std::vector<char> recs(pack->recs.size());

@@ -45,7 +45,7 @@ struct PackSet {
   uint32_t tile0;              // first tile (batch-wide)
   uint32_t patch0, n_patches;  // first patch, patches
   uint32_t n_chunks;           // work chunks (partial-row columns)
-  uint64_t code0;              // first code: code0 + local patch * n_tiles + tile
+  uint64_t code0;              // first code: code0 + tile * n_patches + local patch (tile-major)
   uint64_t prow_base;          // first partial row
 };
 
@@ -110,12 +110,14 @@ struct PackDev {
   char* recs = nullptr;
   Cone* cones = nullptr;
   double* moms = nullptr;
-  double4* tile_en = nullptr;        // [n_tiles * TILE] normalised earth vectors
+  float4* tile_en = nullptr;         // [n_tiles * TILE] normalised earth vectors, rounded to FP32
   uint64_t* tile_valid = nullptr;
   double2* tile_range = nullptr;     // (lo, hi) of phi - floor(phi) over valid lanes
   uint32_t* tile_cp = nullptr;       // cp_bit mask of the valid lanes
-  uint8_t* codes = nullptr;
-  uint8_t* eflags = nullptr;         // per code: bit 0 no lane eclipsed, bit 1 all eclipsed
+  uint8_t* codes = nullptr;          // tile-major, see PackSet::code0
+  // Per (patch, tile), patch-major (code0 + local patch * n_tiles + tile):
+  // bit 0 no lane eclipsed, bit 1 every lane eclipsed.
+  uint8_t* eflags = nullptr;
   ItemDev* cand = nullptr;           // [n_chunks * max_tiles]
   uint8_t* cand_class = nullptr;
   ItemCounts* cand_pos = nullptr;

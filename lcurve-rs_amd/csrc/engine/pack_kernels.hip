@@ -266,7 +266,8 @@ __global__ __launch_bounds__(TILE) void tiles_kernel(PackDev d) {
     y = inv * y;
     z = inv * z;
   }
-  d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x] = make_double4(x, y, z, 0.0);
+  d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x] =
+      make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.0f);
   const double phi = d.lane_phi[l];
   const double phif = phi - floor(phi);
   const double lo = wave_min(valid ? phif : 4.0), hi = wave_max(valid ? phif : -4.0);
@@ -284,13 +285,14 @@ __global__ __launch_bounds__(TILE) void tiles_kernel(PackDev d) {
 // element; zero-weight records count as both clear and eclipsed.
 __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
   const uint32_t p = d.ecl_patches[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
   const PackSeg S = d.psegs[d.patch_seg[p]];
   const PackSet set = d.sets[S.set];
   const uint32_t q = p - S.patch0;
   const uint32_t stride = stride_of(S);
   const uint32_t eoff = S.posn ? 32u : 16u;
-  const char* r = d.recs + S.rec_off + (static_cast<uint64_t>(q) * PATCH + threadIdx.x) * stride;
-  const float w = plain_slot(d, S, q, threadIdx.x).w;
+  const char* r = d.recs + S.rec_off + (static_cast<uint64_t>(q) * PATCH + lane) * stride;
+  const float w = plain_slot(d, S, q, lane).w;
   double e3[MAX_ECL][3];
   for (uint32_t j = 0; j < S.necl; ++j) {
     const double* s = reinterpret_cast<const double*>(r + eoff + 24u * j);
@@ -299,35 +301,43 @@ __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
     e3[j][2] = s[2];
   }
   const uint64_t cbase = set.code0 + static_cast<uint64_t>(p - set.patch0) * set.n_tiles;
+  // Lane t % 64 keeps tile t's flags; each run of 64 tiles is written at once.
+  uint8_t mine = 0;
   for (uint32_t t = 0; t < set.n_tiles; ++t) {
     const uint32_t tg = set.tile0 + t;
-    if (d.tile_valid[tg] == 0ull) continue;  // coded skip without the test
-    const double2 rg = d.tile_range[tg];
-    bool vis = true, ecl = false;
-    for (uint32_t j = 0; j < S.necl; ++j) {
-      bool misses, holds;
-      interval_vs_range(e3[j], rg.x, rg.y, &misses, &holds);
-      vis = vis && misses;
-      ecl = ecl || holds;
+    uint8_t f = 0;
+    if (d.tile_valid[tg] != 0ull) {  // else coded skip without the test
+      const double2 rg = d.tile_range[tg];
+      bool vis = true, ecl = false;
+      for (uint32_t j = 0; j < S.necl; ++j) {
+        bool misses, holds;
+        interval_vs_range(e3[j], rg.x, rg.y, &misses, &holds);
+        vis = vis && misses;
+        ecl = ecl || holds;
+      }
+      if (w == 0.0f) vis = ecl = true;
+      const uint64_t bv = __ballot(vis), be = __ballot(ecl);
+      f = static_cast<uint8_t>((bv == ~0ull ? 1u : 0u) | (be == ~0ull ? 2u : 0u));
     }
-    if (w == 0.0f) vis = ecl = true;
-    const uint64_t bv = __ballot(vis), be = __ballot(ecl);
-    if (threadIdx.x == 0)
-      d.eflags[cbase + t] = static_cast<uint8_t>((bv == ~0ull ? 1u : 0u) | (be == ~0ull ? 2u : 0u));
+    if (lane == t % TILE) mine = f;
+    if (t % TILE == TILE - 1 || t + 1 == set.n_tiles) {
+      const uint32_t t0 = t - t % TILE;
+      if (lane <= t - t0) d.eflags[cbase + t0 + lane] = mine;
+    }
   }
 }
 
 // Work code of every (patch, tile) pair: a workgroup per (block of 256
 // patches of one set, tile), the tile's lanes in LDS, a thread per patch.
 __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
-  __shared__ double ex[TILE], ey[TILE], ez[TILE];
+  __shared__ float ex[TILE], ey[TILE], ez[TILE];
   const PatchBlock pb = d.pblocks[blockIdx.x];
   const PackSet set = d.sets[pb.set];
   const uint32_t t = blockIdx.y;
   if (t >= set.n_tiles) return;  // uniform over the workgroup, before the barrier
   const uint32_t tg = set.tile0 + t;
   if (threadIdx.x < TILE) {
-    const double4 v = d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x];
+    const float4 v = d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x];
     ex[threadIdx.x] = v.x;
     ey[threadIdx.x] = v.y;
     ez[threadIdx.x] = v.z;
@@ -336,17 +346,19 @@ __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
   if (threadIdx.x >= pb.n) return;
   const uint32_t p = pb.patch0 + threadIdx.x;
   const PackSeg S = d.psegs[d.patch_seg[p]];
-  const uint64_t ci = set.code0 + static_cast<uint64_t>(p - set.patch0) * set.n_tiles + t;
+  const uint32_t lp = p - set.patch0;
+  const uint64_t ci = set.code0 + static_cast<uint64_t>(lp) * set.n_tiles + t;  // eclipse flags, patch-major
   const uint64_t vm = d.tile_valid[tg];
   uint8_t code = PT_SKIP;
   if (vm != 0ull) {
     const Cone c = d.cones[p];
     double px, py, pz, plen;
     cone_axis(c, &px, &py, &pz, &plen);
-    double emin = 1.0, emax = -1.0;
+    const float fx = static_cast<float>(px), fy = static_cast<float>(py), fz = static_cast<float>(pz);
+    float emin = 1.0f, emax = -1.0f;
     for (uint32_t j = 0; j < TILE; ++j) {
       if (!((vm >> j) & 1ull)) continue;
-      const double v = ex[j] * px + ey[j] * py + ez[j] * pz;
+      const float v = los_cosine(ex[j], ey[j], ez[j], fx, fy, fz);
       emin = v < emin ? v : emin;
       emax = v > emax ? v : emax;
     }
@@ -363,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
       }
     }
   }
-  d.codes[ci] = code;
+  d.codes[set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp] = code;
 }
 
 // Candidate work item of every (chunk, tile): its patch masks and kernel
@@ -390,8 +402,9 @@ __global__ __launch_bounds__(kBlock) void items_kernel(PackDev d) {
       it.prow = row;
       const uint32_t q0 = (S.patch0 - set.patch0) + C.e0 / PATCH;
       const uint32_t np = C.ne / PATCH;
+      const uint8_t* codes = d.codes + set.code0 + static_cast<uint64_t>(t) * set.n_patches + q0;
       for (uint32_t q = 0; q < np; ++q) {
-        const uint8_t code = d.codes[set.code0 + static_cast<uint64_t>(q0 + q) * set.n_tiles + t];
+        const uint8_t code = codes[q];
         if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
         if (code == PT_FREE) it.free |= 1ull << q;
         if (code == PT_MOM) it.mom |= 1ull << q;

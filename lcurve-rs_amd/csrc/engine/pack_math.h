@@ -129,6 +129,14 @@ struct ConeBuilder {
   }
 };
 
+// Cosine between a tile's line of sight e (unit, rounded to FP32) and a
+// patch axis p (unit, rounded to FP32) for the code decision: FP32
+// products and sums in this order. Its rounding (a few 1e-7) is far inside
+// the decision's 2e-3 rad margin.
+__host__ __device__ inline float los_cosine(float ex, float ey, float ez, float px, float py, float pz) {
+  return ex * px + ey * py + ez * pz;
+}
+
 // Unit axis of a cone (from its FP32 axis), with plen = 0 for a cone that
 // bounds nothing.
 __host__ __device__ inline void cone_axis(const Cone& c, double* px, double* py, double* pz, double* plen) {
