@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liblcurve_oracle.so")
+# LCURVE_ORACLE_LIB: another build of the same sources (the sanitizer build).
+LIB_PATH = os.environ.get("LCURVE_ORACLE_LIB", os.path.join(HERE, "_build", "liblcurve_oracle.so"))
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
