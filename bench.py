@@ -61,7 +61,7 @@ def chisq_models(nsets):
     return texts, sets
 
 
-def build_chisq_dpack(nsets, chunk):
+def build_chisq_dpack(nsets, chunk, limb):
     """The device-packed chisq_4096 batch with its synthetic observations:
     each set's true curve (unscaled) with 1% Gaussian noise."""
     texts, sets = chisq_models(nsets)
@@ -69,7 +69,7 @@ def build_chisq_dpack(nsets, chunk):
     log(f"CPU setup and host layout of {nsets} sets (grids, eclipses, brightness; raw grids staged) ...")
     t0 = time.time()
     models = [lc.Model.from_text(x) for x in texts]
-    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=chunk)
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=chunk, limb=limb)
     t_build = time.time() - t0
     st = dp.stats()
     log(f"setup {st['setup_core_s']:.1f} core-s, layout {st['layout_core_s']:.1f} core-s, wall "
@@ -298,7 +298,8 @@ def bench_chisq(args):
     """chisq_4096: fresh sets from resident raw grids (the value), the
     PCIe-inclusive rate and the host-packed replay beside it."""
     chunk = args.chunk if args.chunk > 0 else 4096
-    dp, models, info = build_chisq_dpack(args.nsets, chunk)
+    limb = args.device_limb
+    dp, models, info = build_chisq_dpack(args.nsets, chunk, limb)
     scale = info["scale"]
     nsets = info["unit_per_step"]
     out = dp.run(scale=scale)
@@ -311,7 +312,7 @@ def bench_chisq(args):
     mism = 0
     for s_ in chk_sets:
         d = dp.check(s_)
-        mism += sum(v for k, v in d.items() if k != "host_items")
+        mism += sum(v for k, v in d.items() if not k.startswith("host_"))
     log(f"device pack vs host pack on sets {chk_sets}: {mism} differing values")
 
     for _ in range(args.warmup):
@@ -341,7 +342,7 @@ def bench_chisq(args):
     try:
         with open(path) as f:
             pm = json.load(f)
-        if pm.get("chunk") != chunk:
+        if pm.get("chunk") != chunk or pm.get("limb", 0) != limb:
             pm = None
     except (OSError, ValueError):
         pm = None
@@ -368,7 +369,9 @@ def bench_chisq(args):
                          "setup_core_s_per_set": round(st["setup_core_s"] / nsets, 5),
                          "layout_core_s_per_set": round(st["layout_core_s"] / nsets, 5),
                          "build_wall_s": round(st["build_wall_s"], 2), "batch_tables_s": round(st["concat_s"], 2)},
-           "tuning": {"chunk": chunk, "items": st["n_items"], "limb": 0},
+           "tuning": {"chunk": chunk, "items": st["n_items"], "limb": limb,
+                      "limb_chunks": st["n_limb_chunks"], "limb_items": st["n_limb_items"],
+                      "split_pairs": st["split_pairs"], "split_limb_lanes": st["split_limb_lanes"]},
            "roofline": roof}
     if not args.no_replay:
         log("host-packed resident batch (CPU pack, replayed) ...")
@@ -633,6 +636,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-limb", action="store_true",
                     help="evaluate no-eclipse patches on all lanes of a tile (no limb lanes)")
+    ap.add_argument("--device-limb", type=int, default=0,
+                    help="chisq_4096 fresh sets: limb lanes in the device pack (0 off, else the split threshold)")
     ap.add_argument("--limb-max-part", type=int, default=48,
                     help="split a no-eclipse patch per lane when at most this many lanes see it in part")
     ap.add_argument("--parity-sets", type=int, default=8)

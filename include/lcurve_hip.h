@@ -105,6 +105,12 @@ int lch_ctx_set_tuning(lch_ctx* ctx, int chunk, int persistent);
  * the split off (all 64 lanes of the tile run the element loop). Results
  * agree to FP32 rounding either way. No counterpart in the reference. */
 int lch_ctx_set_limb(lch_ctx* ctx, int max_part);
+/* Host-packed batches built from now on: 1 (default) decides the lanes the
+ * split's cone test leaves undecided from the patch's elements (a lane that
+ * sees every element, or none, leaves the limb lanes); 0 leaves them limb
+ * lanes, the rule of the device pack (lch_dpack_*), whose tables a host
+ * batch then equals bit for bit. No counterpart in the reference. */
+int lch_ctx_set_limb_check(lch_ctx* ctx, int on);
 int lch_ctx_sync(lch_ctx* ctx);
 
 /* ---- model parameter files ---- */
@@ -204,12 +210,14 @@ int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const do
  * launch then packs all sets on the GPU from those grids (element records
  * sorted into patches, patch cones and moments, work codes, work items and
  * their moment sums) and runs the flux pass and the assembly on them.
- * Tables and results equal those of lch_batch_build_models with limb lanes
- * off and the same chunk, bit for bit. Arguments as lch_batch_build_models;
- * chunk: elements per work item, a multiple of 64 in 64..4096. At most
- * 65536 segments per batch (a set has one per grid and eclipse-pair count,
- * typically 5-12) and under 2^32 elements and work-item candidates
- * (chunks x lane tiles). */
+ * Tables and results equal those of lch_batch_build_models with the same
+ * chunk and limb-lane threshold, bit for bit. Arguments as
+ * lch_batch_build_models; chunk: elements per work item, a multiple of 64
+ * in 64..4096; limb: 0 no limb lanes, else the largest number of lanes of a
+ * tile (1..64) that may see a no-eclipse patch in part for the pair to be
+ * split per lane (lch_ctx_set_limb). At most 65536 segments per batch (a set
+ * has one per grid and eclipse-pair count, typically 5-12) and under 2^32
+ * elements and work-item candidates (chunks x lane tiles). */
 typedef struct lch_dpack lch_dpack;
 typedef struct lch_dpack_stats {
   long long n_sets, n_valid, n_segs, n_elements, n_pairs, n_patches, n_tiles, n_chunks, n_codes;
@@ -221,17 +229,21 @@ typedef struct lch_dpack_stats {
   long long visits;       /* element-visits in the reference's loop order per launch */
   long long pairs_evaluated, pairs_by_moments, moment_evals; /* the flux pass's work at the last
                              lch_dpack_run, as lch_batch_stats counts it */
+  long long pairs_limb;   /* of pairs_evaluated: the pairs of limb lanes */
+  long long n_limb_chunks, n_limb_items; /* limb tables of the last pack */
+  long long split_pairs, split_limb_lanes; /* per-lane split at the last lch_dpack_run: (patch,
+                             tile) pairs split, and their lanes that see the patch in part */
   double setup_core_s;    /* CPU setup, summed over the setup threads */
   double layout_core_s;   /* host layout and grid flattening, summed over the setup threads */
   double build_wall_s;    /* both, wall time */
   double concat_s;        /* batch-wide tables from the per-set layouts */
   double upload_s;        /* the last lch_dpack_upload */
-  int chunk, threads;
+  int chunk, limb, threads;
 } lch_dpack_stats;
 int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, const double* time,
                     const double* expose, const double* flux, const double* ferr, const double* weight,
                     const int* ndiv, int n, const double* flux2d, const double* ferr2d, const double* weight2d,
-                    int chunk, lch_dpack** out);
+                    int chunk, int limb, lch_dpack** out);
 void lch_dpack_free(lch_dpack* b);
 /* Copy the staged grids and tables to the device again (one copy from
  * pinned host memory) and wait; the build already did it once. */
@@ -253,10 +265,13 @@ int lch_dpack_set_obs(lch_ctx* ctx, lch_dpack* b, const double* flux2d, const do
                       const double* weight2d);
 int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out);
 /* Pack one set on the host (the same setup; the host packer of
- * lch_batch_build_models, limb lanes off, this batch's chunk) and compare
- * it with a fresh device pack: diff[0] differing record bytes, [1] patch
- * cones, [2] patch moment values, [3] work codes, [4] segment flags,
- * [5] work items, [6] item moment values; diff[7] = the host's item count. */
+ * lch_batch_build_models at this batch's chunk and limb threshold) and
+ * compare it with a fresh device pack: diff[0] differing record bytes,
+ * [1] patch cones, [2] patch moment values, [3] work codes, [4] segment
+ * flags, [5] work items, [6] item moment values and lane-mask groups;
+ * diff[7] = the host's item count; [8] differing limb chunks, limb items,
+ * lane indices and lane-list entries; diff[9] = the host's limb item count.
+ * diff holds 10 entries. */
 int lch_dpack_check(lch_ctx* ctx, lch_dpack* b, int set, long long* diff);
 
 /* ---- grid-level boundary: the caller's own surface-element grids ----

@@ -128,7 +128,7 @@ lch_batch* build_batch(lch_ctx* ctx, const std::vector<lc::Model>& models,
                        const std::vector<lc::Data>& data, bool keep_maps) {
   auto b = std::make_unique<lch_batch>();
   double t0 = now_s();
-  auto packs = lch::build_packs(models, data, 0, keep_maps);
+  auto packs = lch::build_packs(models, data, 0, keep_maps, ctx->impl.tuning.limb_check);
   double t1 = now_s();
   for (const auto& p : packs) b->n_valid += p->host.valid ? 1 : 0;
   size_t nt = data[0].size();
@@ -193,6 +193,13 @@ int lch_ctx_set_limb(lch_ctx* ctx, int max_part) {
   CtxScope scope(ctx);
   if (!ctx || max_part < 0 || max_part > 64) return fail(LCH_ERR_INVALID, "bad limb setting");
   ctx->impl.tuning.limb = max_part;
+  return LCH_OK;
+}
+
+int lch_ctx_set_limb_check(lch_ctx* ctx, int on) {
+  CtxScope scope(ctx);
+  if (!ctx) return fail(LCH_ERR_INVALID, "null context");
+  ctx->impl.tuning.limb_check = on != 0;
   return LCH_OK;
 }
 
@@ -427,7 +434,7 @@ void lch_batch_free(lch_batch* b) { delete b; }
 int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, const double* time,
                     const double* expose, const double* flux, const double* ferr, const double* weight,
                     const int* ndiv, int n, const double* flux2d, const double* ferr2d, const double* weight2d,
-                    int chunk, lch_dpack** out) {
+                    int chunk, int limb, lch_dpack** out) {
   CtxScope scope(ctx);
   if (!ctx || !models || !out) return fail(LCH_ERR_INVALID, "null argument");
   *out = nullptr;
@@ -443,7 +450,7 @@ int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, con
     const std::vector<lc::Data> data = obs_sets(nsets, time, expose, flux, ferr, weight, ndiv, n, flux2d, ferr2d,
                                                 weight2d);
     auto b = std::make_unique<lch_dpack>();
-    b->impl = std::make_unique<lch::DevicePackBatch>(ctx->impl, ms, data, static_cast<uint32_t>(chunk));
+    b->impl = std::make_unique<lch::DevicePackBatch>(ctx->impl, ms, data, static_cast<uint32_t>(chunk), limb);
     *out = b.release();
     return LCH_OK;
   });
@@ -524,12 +531,18 @@ int lch_dpack_stats_get(const lch_dpack* b, lch_dpack_stats* out) {
   r.pairs_evaluated = static_cast<long long>(s.pairs_evaluated);
   r.pairs_by_moments = static_cast<long long>(s.pairs_by_moments);
   r.moment_evals = static_cast<long long>(s.moment_evals);
+  r.pairs_limb = static_cast<long long>(s.pairs_limb);
+  r.n_limb_chunks = static_cast<long long>(s.n_limb_chunks);
+  r.n_limb_items = static_cast<long long>(s.n_limb_items);
+  r.split_pairs = static_cast<long long>(s.split_pairs);
+  r.split_limb_lanes = static_cast<long long>(s.split_limb_lanes);
   r.setup_core_s = s.setup_core_s;
   r.layout_core_s = s.layout_core_s;
   r.build_wall_s = s.build_wall_s;
   r.concat_s = s.concat_s;
   r.upload_s = s.upload_s;
   r.chunk = static_cast<int>(s.chunk);
+  r.limb = static_cast<int>(s.limb);
   r.threads = s.threads;
   *out = r;
   return LCH_OK;
@@ -539,9 +552,9 @@ int lch_dpack_check(lch_ctx* ctx, lch_dpack* b, int set, long long* diff) {
   CtxScope scope(ctx);
   if (!ctx || !b || !diff) return fail(LCH_ERR_INVALID, "null argument");
   return guarded([&] {
-    uint64_t d[8];
+    uint64_t d[10];
     b->impl->check(set, d);
-    for (int i = 0; i < 8; ++i) diff[i] = static_cast<long long>(d[i]);
+    for (int i = 0; i < 10; ++i) diff[i] = static_cast<long long>(d[i]);
     return LCH_OK;
   });
 }

@@ -462,7 +462,7 @@ void pack_lane_tables(SetPack& S, const lc::PreparedModel& P, const lc::Data& da
   }
   }
 
-void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
+void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, bool elem_check) {
   SetDev& d = S.dev;
   const std::vector<Cone>& cones = R.cones;
   if (R.segs.size() != S.segs.size() || R.cones.size() * MOM_STRIDE != R.moms.size())
@@ -499,12 +499,16 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
     }
   }
   S.n_tiles = ntile;
-  // Normalised FP32 earth vector per lane, for the per-lane split.
-  std::vector<lc::Vec3> lane_en(S.lane_e.size());
+  // Unit line of sight per lane rounded to FP32 (the vectors of the code
+  // decision), for the per-lane split.
+  std::vector<float> lex(S.lane_e.size()), ley(S.lane_e.size()), lez(S.lane_e.size());
   for (size_t l = 0; l < S.lane_e.size(); ++l) {
     lc::Vec3 v(S.lane_e[l].x, S.lane_e[l].y, S.lane_e[l].z);
     const double len = v.length();
-    lane_en[l] = len > 0.0 ? (1.0 / len) * v : v;
+    if (len > 0.0) v = (1.0 / len) * v;
+    lex[l] = static_cast<float>(v.x);
+    ley[l] = static_cast<float>(v.y);
+    lez[l] = static_cast<float>(v.z);
   }
   S.ptcode.assign(cones.size() * ntile, PT_FULL);
   for (size_t g = 0; g < S.segs.size(); ++g) {
@@ -518,25 +522,29 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
     // Per-lane version of the skip / seen-whole tests for a PT_FREE pair:
     // every element of the patch is free of eclipses at every lane of the
     // tile, so a lane whose line of sight has cosine above cf with the
-    // patch axis sees it whole and one below cg sees none of it.
+    // patch axis sees it whole and one below cg sees none of it (the FP32
+    // cosine of the code decision; the cone constants' margin covers its
+    // rounding).
     auto split_lanes = [&](uint32_t patch, uint32_t t, const lc::Vec3& pa, double plen, float cosw) {
       if (!front_ok || !(plen > 0.0)) return;
       const bool can_all = cosw > K.c_front, can_none = cull && cosw > K.c_skip;
       const double cf = can_all ? cos_front(K, cosw) : 2.0, cg = can_none ? cos_skip(K, cosw) : -2.0;
       uint64_t all = 0, some = 0;
+      const float fx = static_cast<float>(pa.x), fy = static_cast<float>(pa.y), fz = static_cast<float>(pa.z);
       for (uint32_t j = 0; j < TILE; ++j) {
         const uint32_t l = t * TILE + j;
         const uint32_t m = S.lane_meta[l];
         if (!(m & META_VALID) || !(sg.use_mask & cp_bit(meta_class(m), meta_ptype(m)))) continue;
-        const double c = lc::dot(lane_en[l], pa);
+        const double c = los_cosine(lex[l], ley[l], lez[l], fx, fy, fz);
         if (c > cf) all |= 1ull << j;
         else if (!(c < cg)) some |= 1ull << j;
       }
-      // The cone leaves a band of lanes undecided; decide them from the
-      // elements themselves: mu of every element against the cut-offs with
-      // a 1e-6 margin, which covers the FP32 rounding of mu here and in the
-      // kernel (a few 1e-7).
-      if (some) {
+      // The cone leaves a band of lanes undecided; with the element-level
+      // check, decide them from the elements themselves: mu of every
+      // element against the cut-offs with a 1e-6 margin, which covers the
+      // FP32 rounding of mu here and in the kernel (a few 1e-7). Without it
+      // they stay limb lanes (evaluated element by element).
+      if (some && elem_check) {
         float lx[TILE], ly[TILE], lz[TILE], mn[TILE], mx[TILE];
         uint32_t js[TILE];
         uint32_t n = 0;
@@ -644,14 +652,15 @@ void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P) {
 
 }
 
-void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data) {
+void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data, bool elem_check) {
   pack_lane_tables(S, P, data);
-  pack_codes(S, R, P);
+  pack_codes(S, R, P, elem_check);
 }
 
-std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps) {
+std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps,
+                                  bool elem_check) {
   auto sp = pack_records(P, keep_maps);
-  pack_lanes(*sp, *sp, P, data);
+  pack_lanes(*sp, *sp, P, data, elem_check);
   return sp;
 }
 
@@ -691,7 +700,7 @@ void scale_linear_records(SetPack& S) {
 
 std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& models,
                                                   const std::vector<lc::Data>& data, int nthreads,
-                                                  bool keep_maps) {
+                                                  bool keep_maps, bool elem_check) {
   const size_t ns = models.size();
   std::vector<std::unique_ptr<SetPack>> packs(ns);
   if (data.empty()) throw lc::LcError("no data");
@@ -716,7 +725,7 @@ std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& 
       try {
         try {
           auto P = lc::prepare_model(models[s], data_of(s));
-          packs[s] = pack_set(*P, data_of(s), keep_maps);
+          packs[s] = pack_set(*P, data_of(s), keep_maps, elem_check);
           scale_linear_records(*packs[s]);
         } catch (const lc::LcError& e) {
           packs[s] = invalid_set(e.what(), data_of(s));

@@ -75,8 +75,12 @@ struct SetPack {
 };
 
 // Pack a prepared model; `data` supplies the observations of this set.
-// pack_records then pack_lanes on the same pack.
-std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps);
+// pack_records then pack_lanes on the same pack. elem_check: the lane
+// split decides the lanes its cone test leaves undecided from the patch's
+// elements (the host default); without it they stay limb lanes (the
+// device pack's rule).
+std::unique_ptr<SetPack> pack_set(const lc::PreparedModel& P, const lc::Data& data, bool keep_maps,
+                                  bool elem_check = true);
 // The pieces of pack_records: the set's scalars (SetDev, SetHost), its
 // grids in segment order, and its segments without records (one per grid
 // and eclipse-pair count: rec_off, padded n, use mask, kind, necl, flags,
@@ -93,12 +97,13 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
 // the per-(patch, tile) work codes and lane splits made from the plain
 // (not yet rescaled) records and cones of R, visit counts, per-time tables
 // and the observations of `data`. S carries R's segments (S may be R).
-void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data);
+void pack_lanes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, const lc::Data& data,
+                bool elem_check = true);
 // The two halves of pack_lanes: lane and per-time tables (lanes, earth
 // vectors, visit counts, observations), then the work codes and lane
 // splits made from them and R's records.
 void pack_lane_tables(SetPack& S, const lc::PreparedModel& P, const lc::Data& data);
-void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P);
+void pack_codes(SetPack& S, const SetPack& R, const lc::PreparedModel& P, bool elem_check = true);
 // Rescales the records of the set's linear-law star segments for the device
 // (SEG_LINW); call after every host table has been made from the records.
 void scale_linear_records(SetPack& S);
@@ -130,6 +135,6 @@ std::unique_ptr<SetPack> invalid_set(const std::string& err, const lc::Data& dat
 // `data` holds either one data set shared by all models or one per model.
 std::vector<std::unique_ptr<SetPack>> build_packs(const std::vector<lc::Model>& models,
                                                   const std::vector<lc::Data>& data, int nthreads,
-                                                  bool keep_maps);
+                                                  bool keep_maps, bool elem_check = true);
 
 }  // namespace lch

@@ -55,6 +55,7 @@ struct SetLayout {
   std::vector<uint32_t> n_real;  // elements per segment
   std::vector<uint32_t> pflags;  // PSEG_* per segment
   std::vector<CodeConsts> K;     // code constants per segment
+  std::vector<float> whole, none;  // the per-lane split's element cut-offs per segment
   struct Grid {
     uint32_t e0 = 0, n = 0, pos0 = 0;  // set-relative first raw element / position
     uint32_t seg_of[MAX_ECL + 1];      // set-relative segment per eclipse-pair count
@@ -87,6 +88,8 @@ std::unique_ptr<SetLayout> lay_out_set(const lc::PreparedModel& P, const lc::Dat
     L->pflags.push_back((r.cull ? PSEG_CULL : 0u) | (r.front_ok ? PSEG_FRONT : 0u) | (r.mom_ok ? PSEG_MOM : 0u) |
                         (linear_law_segment(sg, S.dev) ? PSEG_LINW : 0u));
     L->K.push_back(r.K);
+    L->whole.push_back(static_cast<float>(r.thr_front + 1e-6));
+    L->none.push_back(static_cast<float>(r.thr - 1e-6));
   }
   size_t ne = 0, np = 0, nq = 0;
   for (const GridRef& g : grids) {
@@ -132,14 +135,17 @@ std::unique_ptr<SetLayout> lay_out_set(const lc::PreparedModel& P, const lc::Dat
 }  // namespace
 
 DevicePackBatch::DevicePackBatch(Context& ctx, const std::vector<lc::Model>& models,
-                                 const std::vector<lc::Data>& data, uint32_t chunk, int nthreads)
+                                 const std::vector<lc::Data>& data, uint32_t chunk, int limb, int nthreads)
     : ctx_(ctx), models_(models), data_(data) {
+  if (limb < 0 || limb > static_cast<int>(TILE)) throw std::invalid_argument("limb must be in 0..64");
+  limb_ = static_cast<uint32_t>(limb);
   if (models.empty()) throw std::invalid_argument("no parameter sets");
   if (data.empty() || (data.size() != 1 && data.size() != models.size()))
     throw std::invalid_argument("give one data set, or one per parameter set");
   if (chunk < PATCH || chunk % PATCH || chunk > MAX_ITEM_PATCHES * PATCH)
     throw std::invalid_argument("chunk must be a multiple of 64 in 64..4096");
   chunk_ = chunk;
+  st_.limb = limb_;
   n_times_ = data[0].size();
   for (const lc::Data& d : data)
     if (d.size() != n_times_) throw std::invalid_argument("every data set needs the same number of times");
@@ -230,6 +236,10 @@ void DevicePackBatch::release() {
   if (d_work_) (void)hipFree(d_work_);
   if (sort_temp_) (void)hipFree(sort_temp_);
   if (scan_temp_) (void)hipFree(scan_temp_);
+  for (void*& g : grow_) {
+    if (g) (void)hipFree(g);
+    g = nullptr;
+  }
   h_in_ = nullptr;
   h_counts_ = nullptr;
   d_in_ = d_work_ = nullptr;
@@ -279,6 +289,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
   const uint64_t n_cand = n_chunks * max_tiles;
   if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
+  if (n_codes >= UINT32_MAX) throw std::invalid_argument("too many (patch, tile) pairs: split the sets");
   key_bits_ = 16;
   for (uint64_t v = n_segs > 0 ? n_segs - 1 : 0; v; v >>= 1) ++key_bits_;
   uint64_t n_ktiles = 0;
@@ -378,6 +389,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
     ps.tile0 = tile;
     ps.patch0 = patch;
     ps.code0 = code;
+    ps.chunk0 = chunk;
     ps.prow_base = prow;
     host_[s] = S.host;
     seg0_[s] = seg;
@@ -417,6 +429,8 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
       q.flags = L.pflags[g];
       q.base_flags = sg.flags;
       q.K = L.K[g];
+      q.whole = L.whole[g];
+      q.none = L.none[g];
       SegDev d = sg;
       d.rec_off = q.rec_off;
       d.set = static_cast<uint32_t>(s);
@@ -496,10 +510,17 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   const size_t w_ten = w.add(n_tiles * TILE * sizeof(float4)), w_tval = w.add(n_tiles * 8),
                w_trng = w.add(n_tiles * sizeof(double2)), w_tcp = w.add(n_tiles * 4);
   const size_t w_codes = w.add(n_codes), w_efl = w.add(n_codes);
-  const size_t w_cand = w.add(n_cand * sizeof(ItemDev)), w_ccls = w.add(n_cand),
+  // The per-lane split and per-chunk limb tables (limb lanes only).
+  const uint64_t n_split = limb_ ? n_codes : 0, n_lch = limb_ ? n_chunks : 0;
+  const size_t w_hsp = w.add(n_split), w_sal = w.add(n_split * 8), w_ssm = w.add(n_split * 8),
+               w_gln = w.add(limb_ ? 4 : 0), w_gl = w.add(limb_ ? n_cand * 4 : 0), w_sst = w.add(limb_ ? 32 : 0);
+  const size_t w_crev = w.add(n_lch), w_csh = w.add(n_lch * 4), w_cnq = w.add(n_lch * MAX_ITEM_PATCHES * 4),
+               w_clc = w.add(n_lch * sizeof(LimbCounts)), w_clp = w.add(n_lch * sizeof(LimbCounts)),
+               w_lch = w.add(n_lch * sizeof(LimbChunk));
+  const size_t w_cand = w.add(n_cand * sizeof(ItemDev)), w_cinf = w.add(n_cand * 4),
                w_cpos = w.add(n_cand * sizeof(ItemCounts));
-  const size_t w_items = w.add(n_cand * sizeof(ItemDev)), w_imoms = w.add(n_cand * MOM_STRIDE * sizeof(double));
-  const size_t w_counts = w.add(sizeof(ItemCounts));
+  const size_t w_items = w.add(n_cand * sizeof(ItemDev));
+  const size_t w_counts = w.add(sizeof(PackCounts));
   const size_t w_part = w.add(n_prows * TILE * sizeof(double)), w_ltot = w.add(n_lanes * sizeof(double));
   const size_t w_flux = w.add(n_times * sizeof(double)), w_out = w.add(ns * sizeof(SetOut));
   const size_t w_lcomp = w.add(any_iscale ? n_lanes * 5 * sizeof(double) : 0),
@@ -509,10 +530,10 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   st_.work_bytes = work_bytes_;
   hip_check(hipMalloc(reinterpret_cast<void**>(&d_work_), work_bytes_), "hipMalloc pack tables");
   sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(n_raw), 16);
-  scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand), 16);
+  scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand, n_chunks), 16);
   hip_check(hipMalloc(&sort_temp_, sort_temp_bytes_), "hipMalloc sort storage");
   hip_check(hipMalloc(&scan_temp_, scan_temp_bytes_), "hipMalloc scan storage");
-  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_counts_), sizeof(ItemCounts), hipHostMallocDefault),
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_counts_), sizeof(PackCounts), hipHostMallocDefault),
             "hipHostMalloc counts");
   auto din = [&](size_t off) { return d_in_ + off; };
   auto dw = [&](size_t off) { return d_work_ + off; };
@@ -546,6 +567,8 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.lane_phi = reinterpret_cast<const double*>(din(o_lphi));
   p.lane_e = reinterpret_cast<const float4*>(din(o_le));
   p.lane_meta = reinterpret_cast<const uint32_t*>(din(o_lm));
+  p.setdev = reinterpret_cast<const SetDev*>(din(o_sets));
+  p.limb = limb_;
   p.keys = reinterpret_cast<uint32_t*>(dw(w_keys));
   p.vals = reinterpret_cast<uint32_t*>(dw(w_vals));
   p.keys_s = reinterpret_cast<uint32_t*>(dw(w_keys_s));
@@ -560,12 +583,25 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.tile_cp = reinterpret_cast<uint32_t*>(dw(w_tcp));
   p.codes = reinterpret_cast<uint8_t*>(dw(w_codes));
   p.eflags = reinterpret_cast<uint8_t*>(dw(w_efl));
+  if (limb_) {
+    p.has_split = reinterpret_cast<uint8_t*>(dw(w_hsp));
+    p.split_all = reinterpret_cast<uint64_t*>(dw(w_sal));
+    p.split_some = reinterpret_cast<uint64_t*>(dw(w_ssm));
+    p.glist_n = reinterpret_cast<uint32_t*>(dw(w_gln));
+    p.glist = reinterpret_cast<uint32_t*>(dw(w_gl));
+    p.split_stats = reinterpret_cast<unsigned long long*>(dw(w_sst));
+    p.chunk_rev = reinterpret_cast<uint8_t*>(dw(w_crev));
+    p.chunk_shift = reinterpret_cast<int32_t*>(dw(w_csh));
+    p.chunk_nq = reinterpret_cast<uint32_t*>(dw(w_cnq));
+    p.chunk_lcnt = reinterpret_cast<LimbCounts*>(dw(w_clc));
+    p.chunk_lpos = reinterpret_cast<LimbCounts*>(dw(w_clp));
+    p.lchunks = reinterpret_cast<LimbChunk*>(dw(w_lch));
+  }
   p.cand = reinterpret_cast<ItemDev*>(dw(w_cand));
-  p.cand_class = reinterpret_cast<uint8_t*>(dw(w_ccls));
+  p.cand_info = reinterpret_cast<uint32_t*>(dw(w_cinf));
   p.cand_pos = reinterpret_cast<ItemCounts*>(dw(w_cpos));
   p.items = reinterpret_cast<ItemDev*>(dw(w_items));
-  p.imoms = reinterpret_cast<double*>(dw(w_imoms));
-  p.counts = reinterpret_cast<ItemCounts*>(dw(w_counts));
+  p.counts = reinterpret_cast<PackCounts*>(dw(w_counts));
   p.partials = reinterpret_cast<double*>(dw(w_part));
 
   DeviceBatch& d = d_;
@@ -580,7 +616,6 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   d.lane_meta = p.lane_meta;
   d.chunk_info = reinterpret_cast<const uint32_t*>(din(o_cinfo));
   d.moms = p.moms;
-  d.imoms = p.imoms;
   d.t_lane0 = reinterpret_cast<const uint32_t*>(din(o_tl0));
   d.t_ndiv = reinterpret_cast<const int*>(din(o_tnd));
   d.t_slfac = reinterpret_cast<const double*>(din(o_tsl));
@@ -598,6 +633,12 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   d.flux = reinterpret_cast<double*>(dw(w_flux));
   d.out = reinterpret_cast<SetOut*>(dw(w_out));
   d.n_limb_chunks = 0;
+  d.limb_chunks = p.lchunks;
+  // The second phase's tables, sized for one moment entry per candidate to
+  // begin with (the most a pack without limb lanes writes).
+  PackCounts c0{};
+  c0.items.moms = static_cast<uint32_t>(n_cand);
+  ensure_capacity(c0);
 
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
@@ -619,22 +660,73 @@ void DevicePackBatch::upload() {
   st_.upload_s = now_s() - t0;
 }
 
+void DevicePackBatch::ensure_capacity(const PackCounts& c) {
+  PackDev& p = pd_;
+  // A table too small for `need` entries of `elem` bytes is replaced by one
+  // with a quarter of headroom, so that packs of similar sets reuse it.
+  // Returns the capacity in entries.
+  auto grow = [&](int slot, uint64_t need, uint64_t have, size_t elem, const char* what) -> uint64_t {
+    if (need <= have && grow_[slot]) return have;
+    const uint64_t cap = std::max<uint64_t>(need + need / 4, 1024);
+    if (grow_[slot]) hip_check(hipFree(grow_[slot]), "hipFree pack table");
+    grow_[slot] = nullptr;
+    hip_check(hipMalloc(&grow_[slot], static_cast<size_t>(cap) * elem), what);
+    return cap;
+  };
+  const uint64_t ce = grow(0, c.items.moms, p.cap_imoms, MOM_STRIDE * sizeof(double), "hipMalloc item moments");
+  (void)grow(5, c.items.moms, p.cap_imoms, sizeof(uint32_t), "hipMalloc moment entries");
+  (void)grow(6, c.items.moms, p.cap_imoms, sizeof(uint64_t), "hipMalloc moment entries");
+  p.cap_imoms = ce;
+  p.imoms = static_cast<double*>(grow_[0]);
+  p.ment_base = static_cast<uint32_t*>(grow_[5]);
+  p.ment_mask = static_cast<uint64_t*>(grow_[6]);
+  p.cap_masks = grow(1, c.items.masks, p.cap_masks, sizeof(uint64_t), "hipMalloc lane masks");
+  p.lane_masks = static_cast<uint64_t*>(grow_[1]);
+  if (limb_) {
+    const uint64_t ci = grow(2, c.limb.items, p.cap_litems, sizeof(LimbItem), "hipMalloc limb items");
+    (void)grow(3, c.limb.items, p.cap_litems, TILE * sizeof(uint16_t), "hipMalloc limb lane indices");
+    p.cap_litems = ci;
+    p.litems = static_cast<LimbItem*>(grow_[2]);
+    p.lidx = static_cast<uint16_t*>(grow_[3]);
+    p.cap_llanes = grow(4, c.limb.lanes, p.cap_llanes, sizeof(uint32_t), "hipMalloc limb lanes");
+    p.llanes = static_cast<uint32_t*>(grow_[4]);
+  }
+  d_.imoms = p.imoms;
+  d_.lane_masks = p.lane_masks;
+  d_.limb_items = p.litems;
+  d_.limb_idx = p.lidx;
+  d_.limb_lanes = p.llanes;
+}
+
+void DevicePackBatch::pack(hipStream_t st) {
+  hip_check(launch_pack_count(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st),
+            "pack");
+  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(PackCounts), hipMemcpyDeviceToHost, st),
+            "hipMemcpyAsync counts");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  const PackCounts c = *h_counts_;
+  const uint64_t n = static_cast<uint64_t>(c.items.a) + c.items.b + c.items.c;
+  if (n > st_.n_cand || c.limb.chunks > st_.n_chunks || (!limb_ && (c.limb.chunks || c.items.masks)))
+    throw DeviceError("pack counts out of range");
+  // Between the phases the stream is idle: buffers may be replaced.
+  ensure_capacity(c);
+  hip_check(launch_pack_fill(pd_, c.items.moms, st), "pack tables");
+  last_counts_ = c;
+  d_.n_items = static_cast<uint32_t>(n);
+  d_.n_items_plain = c.items.a;
+  d_.n_items_ecl = c.items.b;
+  d_.n_limb_chunks = c.limb.chunks;
+  st_.n_items = n;
+  st_.n_items_plain = c.items.a;
+  st_.n_items_ecl = c.items.b;
+  st_.n_limb_chunks = c.limb.chunks;
+  st_.n_limb_items = c.limb.items;
+}
+
 void DevicePackBatch::launch(int scale) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();
-  hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
-  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
-            "hipMemcpyAsync counts");
-  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  const ItemCounts c = *h_counts_;
-  const uint64_t n = static_cast<uint64_t>(c.a) + c.b + c.c;
-  if (n > st_.n_cand) throw DeviceError("work item count out of range");
-  d_.n_items = static_cast<uint32_t>(n);
-  d_.n_items_plain = c.a;
-  d_.n_items_ecl = c.b;
-  st_.n_items = n;
-  st_.n_items_plain = c.a;
-  st_.n_items_ecl = c.b;
+  pack(st);
   hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
   hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
 }
@@ -653,21 +745,39 @@ void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, 
   }
   if (flux && st_.n_times)
     hip_check(hipMemcpy(flux, d_.flux, st_.n_times * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
-  // The flux pass's work, counted from the items as the host-packed batch
-  // counts it.
+  // The flux pass's work, counted from the items and limb items as the
+  // host-packed batch counts it.
   std::vector<ItemDev> items(d_.n_items);
   if (!items.empty())
     hip_check(hipMemcpy(items.data(), pd_.items, items.size() * sizeof(ItemDev), hipMemcpyDeviceToHost),
               "hipMemcpy items");
-  uint64_t pe = 0, pm = 0, me = 0;
+  uint64_t pe = 0, pm = 0, me = 0, pl = 0;
   for (const ItemDev& it : items) {
     pe += static_cast<uint64_t>(__builtin_popcountll(it.todo)) * PATCH * TILE;
-    pm += static_cast<uint64_t>(__builtin_popcountll(it.mom)) * PATCH * TILE;
-    me += it.mom ? TILE : 0u;
+    pm += static_cast<uint64_t>(__builtin_popcountll(it.mom) + __builtin_popcountll(it.pmom)) * PATCH * TILE;
+    me += ((it.mom ? 1u : 0u) + static_cast<uint64_t>(it.ngrp)) * TILE;
   }
-  st_.pairs_evaluated = pe;
+  if (limb_) {
+    unsigned long long ss[2];
+    hip_check(hipMemcpy(ss, pd_.split_stats, sizeof ss, hipMemcpyDeviceToHost), "hipMemcpy split stats");
+    st_.split_pairs = ss[0];
+    st_.split_limb_lanes = ss[1];
+  }
+  if (limb_ && st_.n_chunks) {
+    std::vector<LimbCounts> lc(st_.n_chunks);
+    std::vector<uint32_t> nq(st_.n_chunks * MAX_ITEM_PATCHES);
+    hip_check(hipMemcpy(lc.data(), pd_.chunk_lcnt, lc.size() * sizeof(LimbCounts), hipMemcpyDeviceToHost),
+              "hipMemcpy limb counts");
+    hip_check(hipMemcpy(nq.data(), pd_.chunk_nq, nq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+              "hipMemcpy limb lanes");
+    for (size_t c = 0; c < lc.size(); ++c)
+      if (lc[c].chunks)
+        for (uint32_t q = 0; q < MAX_ITEM_PATCHES; ++q) pl += static_cast<uint64_t>(nq[c * MAX_ITEM_PATCHES + q]) * PATCH;
+  }
+  st_.pairs_evaluated = pe + pl;
   st_.pairs_by_moments = pm;
   st_.moment_evals = me;
+  st_.pairs_limb = pl;
 }
 
 void DevicePackBatch::set_obs(const double* flux2d, const double* ferr2d, const double* weight2d) {
@@ -692,27 +802,19 @@ void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_flux
   double tp = 0.0, tf = 0.0;
   hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
   for (int r = 0; r < reps; ++r) {
+    // The pack's time runs from its first kernel to its last, the host's
+    // wait for the counts between its phases included.
     hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
-    hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
+    pack(st);
     hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
-    hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
-              "hipMemcpyAsync counts");
-    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    float ms = 0.0f;
-    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
-    tp += ms;
-    const ItemCounts c = *h_counts_;
-    const uint64_t n = static_cast<uint64_t>(c.a) + c.b + c.c;
-    if (n > st_.n_cand) throw DeviceError("work item count out of range");
-    d_.n_items = static_cast<uint32_t>(n);
-    d_.n_items_plain = c.a;
-    d_.n_items_ecl = c.b;
-    hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
     hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
     hip_check(hipEventRecord(ev_[3], st), "hipEventRecord");
     hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
     hip_check(hipEventSynchronize(ev_[3]), "hipEventSynchronize");
-    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[3]), "hipEventElapsedTime");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    tp += ms;
+    hip_check(hipEventElapsedTime(&ms, ev_[1], ev_[3]), "hipEventElapsedTime");
     tf += ms;
   }
   hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
@@ -724,18 +826,16 @@ void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_flux
   if (ms_total) *ms_total = reps > 0 ? total / reps : 0.0;
 }
 
-void DevicePackBatch::check(int set, uint64_t diff[8]) {
-  for (int i = 0; i < 8; ++i) diff[i] = 0;
+void DevicePackBatch::check(int set, uint64_t diff[10]) {
+  for (int i = 0; i < 10; ++i) diff[i] = 0;
   if (set < 0 || static_cast<size_t>(set) >= sets_.size()) throw std::invalid_argument("set index out of range");
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();
   // The device tables of a fresh pack.
-  hip_check(launch_pack(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st), "pack");
-  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(ItemCounts), hipMemcpyDeviceToHost, st),
-            "hipMemcpyAsync counts");
+  pack(st);
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  const ItemCounts cnt = *h_counts_;
-  const uint64_t n_items = static_cast<uint64_t>(cnt.a) + cnt.b + cnt.c;
+  const PackCounts cnt = last_counts_;
+  const uint64_t n_items = static_cast<uint64_t>(cnt.items.a) + cnt.items.b + cnt.items.c;
   const size_t s = static_cast<size_t>(set);
   const lc::Data& data = data_of(s);
   // The host pack of the same set.
@@ -748,7 +848,7 @@ void DevicePackBatch::check(int set, uint64_t diff[8]) {
   }
   std::unique_ptr<SetPack> pack;
   try {
-    pack = pack_set(*P, data, false);
+    pack = pack_set(*P, data, false, /*elem_check=*/false);
   } catch (const lc::LcError&) {
     if (sets_[s].flags & SET_VALID) diff[3] = 1;
     return;
@@ -787,14 +887,15 @@ void DevicePackBatch::check(int set, uint64_t diff[8]) {
   get(segs.data(), pd_.segs + seg0, segs.size() * sizeof(SegDev));
   for (uint32_t g = 0; g < nseg; ++g)
     if (segs[g].flags != pack->segs[g].flags) ++diff[4];
-  // Work items of a host-packed batch of this set alone, limb lanes off.
+  // Work items and limb tables of a host-packed batch of this set alone,
+  // at this batch's chunk and limb threshold.
   struct TuningGuard {
     Context& c;
     Tuning saved;
     ~TuningGuard() { c.tuning = saved; }
   } guard{ctx_, ctx_.tuning};
   ctx_.tuning.chunk = chunk_;
-  ctx_.tuning.limb = 0;
+  ctx_.tuning.limb = static_cast<int>(limb_);
   std::vector<std::unique_ptr<SetPack>> one;
   one.push_back(std::move(pack));
   Batch B(ctx_, std::move(one), data.size());
@@ -805,27 +906,74 @@ void DevicePackBatch::check(int set, uint64_t diff[8]) {
   for (const ItemDev& it : all)
     if (it.seg >= seg0 && it.seg < seg0 + nseg) mine.push_back(it);
   diff[7] = T.items.size();
+  diff[9] = T.limb_items.size();
   if (mine.size() != T.items.size()) {
     diff[5] = std::max(mine.size(), T.items.size());
+  } else {
+    std::vector<double> im(MOM_STRIDE);
+    std::vector<uint64_t> gm;
+    for (size_t i = 0; i < mine.size(); ++i) {
+      ItemDev a = mine[i];
+      const ItemDev& b = T.items[i];
+      a.seg -= seg0;
+      a.lane0 -= ps.lane_off;
+      a.prow -= ps.prow_base;
+      const bool same = a.seg == b.seg && a.e0 == b.e0 && a.ne == b.ne && a.lane0 == b.lane0 && a.prow == b.prow &&
+                        a.todo == b.todo && a.free == b.free && a.mom == b.mom && a.pmom == b.pmom &&
+                        a.ngrp == b.ngrp && a.pad == b.pad;
+      if (!same) {
+        ++diff[5];
+        continue;
+      }
+      // Moment entries: the PT_MOM sum, then one per lane-mask group.
+      const uint32_t ne = (a.mom ? 1u : 0u) + a.ngrp;
+      for (uint32_t k = 0; k < ne; ++k) {
+        get(im.data(), pd_.imoms + (static_cast<size_t>(a.imom) + k) * MOM_STRIDE, MOM_STRIDE * sizeof(double));
+        for (uint32_t c = 0; c < MOM_STRIDE; ++c)
+          if (std::memcmp(&im[c], &T.imoms[(static_cast<size_t>(b.imom) + k) * MOM_STRIDE + c], sizeof(double)) != 0)
+            ++diff[6];
+      }
+      if (a.ngrp) {
+        gm.resize(a.ngrp);
+        get(gm.data(), pd_.lane_masks + a.lm_off, gm.size() * sizeof(uint64_t));
+        for (uint32_t k = 0; k < a.ngrp; ++k)
+          if (gm[k] != T.lane_masks[static_cast<size_t>(b.lm_off) + k]) ++diff[6];
+      }
+    }
+  }
+  // Limb chunks of this set, in order, with their items, lane indices and
+  // lane lists.
+  std::vector<LimbChunk> lch(last_counts_.limb.chunks), lmine;
+  get(lch.data(), pd_.lchunks, lch.size() * sizeof(LimbChunk));
+  for (const LimbChunk& c : lch)
+    if (c.seg >= seg0 && c.seg < seg0 + nseg) lmine.push_back(c);
+  if (lmine.size() != T.limb_chunks.size()) {
+    diff[8] = std::max(lmine.size(), T.limb_chunks.size());
     return;
   }
-  std::vector<double> im(MOM_STRIDE);
-  for (size_t i = 0; i < mine.size(); ++i) {
-    ItemDev a = mine[i];
-    const ItemDev& b = T.items[i];
-    a.seg -= seg0;
-    a.lane0 -= ps.lane_off;
-    a.prow -= ps.prow_base;
-    const bool same = a.seg == b.seg && a.e0 == b.e0 && a.ne == b.ne && a.lane0 == b.lane0 && a.prow == b.prow &&
-                      a.todo == b.todo && a.free == b.free && a.mom == b.mom && a.pmom == b.pmom &&
-                      a.lm_off == b.lm_off && a.ngrp == b.ngrp && a.pad == b.pad;
-    if (!same) ++diff[5];
-    if (same && a.mom) {
-      get(im.data(), pd_.imoms + static_cast<size_t>(a.imom) * MOM_STRIDE, MOM_STRIDE * sizeof(double));
-      for (uint32_t c = 0; c < MOM_STRIDE; ++c)
-        if (std::memcmp(&im[c], &T.imoms[static_cast<size_t>(b.imom) * MOM_STRIDE + c], sizeof(double)) != 0)
-          ++diff[6];
+  for (size_t i = 0; i < lmine.size(); ++i) {
+    const LimbChunk& a = lmine[i];
+    const LimbChunk& b = T.limb_chunks[i];
+    if (a.seg - seg0 != b.seg || a.chunk != b.chunk || a.n_items != b.n_items || a.n_lanes != b.n_lanes ||
+        a.shift != b.shift) {
+      ++diff[8];
+      continue;
     }
+    std::vector<LimbItem> li(a.n_items);
+    std::vector<uint16_t> lx(static_cast<size_t>(a.n_items) * TILE);
+    std::vector<uint32_t> ll(a.n_lanes);
+    get(li.data(), pd_.litems + a.item0, li.size() * sizeof(LimbItem));
+    get(lx.data(), pd_.lidx + static_cast<size_t>(a.item0) * TILE, lx.size() * sizeof(uint16_t));
+    get(ll.data(), pd_.llanes + a.lane_off, ll.size() * sizeof(uint32_t));
+    for (uint32_t k = 0; k < a.n_items; ++k) {
+      const LimbItem& h = T.limb_items[b.item0 + k];
+      if (li[k].e0 != h.e0 || li[k].split != h.split) ++diff[8];
+      for (uint32_t j = 0; j < TILE; ++j)
+        if (lx[static_cast<size_t>(k) * TILE + j] != T.limb_idx[(static_cast<size_t>(b.item0) + k) * TILE + j])
+          ++diff[8];
+    }
+    for (uint32_t u = 0; u < a.n_lanes; ++u)
+      if (ll[u] - ps.lane_off != T.limb_lanes[b.lane_off + u]) ++diff[8];
   }
 }
 

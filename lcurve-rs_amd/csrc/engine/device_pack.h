@@ -6,7 +6,7 @@
 // moments, the per-(patch, tile) work codes, the work items and their
 // moment sums (pack_kernels.hip), and runs the flux pass and the assembly
 // on them. The tables equal the host pack's bit for bit for the same chunk
-// size with limb lanes off (check()).
+// size and limb-lane threshold (check()).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,16 +31,21 @@ struct DevicePackStats {
   uint64_t work_bytes = 0;    // device tables the pack and the flux pass write
   uint64_t visits = 0;        // element-visits in the reference's loop order per pass
   uint64_t n_items = 0, n_items_plain = 0, n_items_ecl = 0;  // of the last pack
+  uint64_t n_limb_chunks = 0, n_limb_items = 0;              // of the last pack
+  // The per-lane split of the last run(): (patch, tile) pairs split and
+  // their limb lanes (lanes that see the patch in part).
+  uint64_t split_pairs = 0, split_limb_lanes = 0;
   // Work of the flux pass of the last run(): (element, lane) pairs
   // evaluated one by one, pairs summed from patch moments, moment sums
-  // evaluated (per lane).
-  uint64_t pairs_evaluated = 0, pairs_by_moments = 0, moment_evals = 0;
+  // evaluated (per lane); of the first, the pairs of limb lanes.
+  uint64_t pairs_evaluated = 0, pairs_by_moments = 0, moment_evals = 0, pairs_limb = 0;
   double setup_core_s = 0;    // CPU setup (prepare_model), summed over threads
   double layout_core_s = 0;   // host layout and grid flattening, summed over threads
   double build_wall_s = 0;    // both, on the setup threads
   double concat_s = 0;        // batch-wide tables from the per-set layouts
   double upload_s = 0;        // the last upload()
   uint32_t chunk = 0;
+  uint32_t limb = 0;
   int threads = 0;
 };
 
@@ -53,9 +58,11 @@ class DevicePackBatch {
   // cores or LCURVE_HIP_THREADS), batch-wide tables, device memory, then
   // upload(). `data` holds one data set shared by all models or one per
   // model. Work items cover `chunk` elements (a multiple of 64, at most
-  // 4096).
+  // 4096). limb: 0 no limb lanes; otherwise a no-eclipse (patch, tile) pair
+  // is split per lane when at most `limb` (<= 64) of its lanes see the
+  // patch in part (Tuning::limb).
   DevicePackBatch(Context& ctx, const std::vector<lc::Model>& models, const std::vector<lc::Data>& data,
-                  uint32_t chunk, int nthreads = 0);
+                  uint32_t chunk, int limb, int nthreads = 0);
   ~DevicePackBatch();
   DevicePackBatch(const DevicePackBatch&) = delete;
   DevicePackBatch& operator=(const DevicePackBatch&) = delete;
@@ -75,13 +82,15 @@ class DevicePackBatch {
   // Replace the observations of every set ([n_sets][n_times] each; a null
   // array keeps the current values), in the staging block and on the device.
   void set_obs(const double* flux2d, const double* ferr2d, const double* weight2d);
-  // Packs set `set` on the host (the same CPU setup, pack_set, limb lanes
-  // off, this batch's chunk) and compares it with the device tables of the
-  // last pack. diff[0] differing record bytes, [1] patch cones, [2] patch
-  // moment values, [3] work codes, [4] segment flags, [5] work items
-  // (a count mismatch counts every item), [6] item moment values, [7] the
-  // host's item count.
-  void check(int set, uint64_t diff[8]);
+  // Packs set `set` on the host (the same CPU setup, pack_set, this
+  // batch's chunk and limb threshold) and compares it with the tables of a
+  // fresh device pack. diff[0] differing record bytes, [1] patch cones,
+  // [2] patch moment values, [3] work codes, [4] segment flags, [5] work
+  // items (a count mismatch counts every item), [6] item moment values and
+  // lane-mask groups, [7] the host's item count, [8] limb chunks, limb
+  // items, their lane indices and lane lists that differ (a count mismatch
+  // counts every entry), [9] the host's limb item count.
+  void check(int set, uint64_t diff[10]);
 
   const DevicePackStats& stats() const { return st_; }
   uint32_t n_sets() const { return static_cast<uint32_t>(host_.size()); }
@@ -92,6 +101,10 @@ class DevicePackBatch {
  private:
   void release();
   void build(std::vector<std::unique_ptr<SetLayout>> lay);
+  // The pack's first phase, its counts read back, the second phase's tables
+  // grown to them, then the second phase; leaves d_ ready for the flux pass.
+  void pack(hipStream_t st);
+  void ensure_capacity(const PackCounts& c);
   const lc::Data& data_of(size_t s) const { return data_.size() == 1 ? data_[0] : data_[s]; }
 
   Context& ctx_;
@@ -99,6 +112,7 @@ class DevicePackBatch {
   std::vector<lc::Data> data_;
   size_t n_times_ = 0;
   uint32_t chunk_ = 0;
+  uint32_t limb_ = 0;
   DevicePackStats st_;
   std::vector<SetHost> host_;
   std::vector<SetOut> last_out_;
@@ -120,7 +134,12 @@ class DevicePackBatch {
   void* scan_temp_ = nullptr;
   size_t scan_temp_bytes_ = 0;
   uint32_t key_bits_ = 32;          // significant bits of the sort keys
-  ItemCounts* h_counts_ = nullptr;  // pinned
+  PackCounts* h_counts_ = nullptr;  // pinned
+  PackCounts last_counts_{};
+  // Tables of the pack's second phase, each its own allocation so it can
+  // grow: item moments, lane-mask groups, limb items, their lane indices,
+  // limb lane lists, the moment entries' first patches and patch masks.
+  void* grow_[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
   PackDev pd_;
   DeviceBatch d_;

@@ -834,6 +834,11 @@ Batch::Tables Batch::tables() const {
   t.moms = moms_;
   t.imoms = item_moms_;
   t.codes = ptcode_;
+  t.lane_masks = lane_masks_;
+  t.limb_chunks = limb_chunks_;
+  t.limb_items = limb_items_;
+  t.limb_idx = limb_idx_;
+  t.limb_lanes = limb_lanes_;
   t.items.resize(n_items_);
   if (n_items_)
     hip_check(hipMemcpy(t.items.data(), d_items_, n_items_ * sizeof(ItemDev), hipMemcpyDeviceToHost),

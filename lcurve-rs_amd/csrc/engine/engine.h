@@ -29,6 +29,10 @@ struct Tuning {
   // Limb lanes: 0 off; otherwise a no-eclipse (patch, tile) pair is split
   // per lane when at most `limb` of its lanes see the patch in part.
   int limb = 48;
+  // The host packer's element-level check of the lanes the split's cone
+  // test leaves undecided (on), or none: they stay limb lanes (the device
+  // pack's rule).
+  bool limb_check = true;
   bool tuned = false;
 };
 
@@ -107,6 +111,11 @@ class Batch {
     std::vector<double> moms, imoms;
     std::vector<uint8_t> codes;
     std::vector<ItemDev> items;
+    std::vector<uint64_t> lane_masks;
+    std::vector<LimbChunk> limb_chunks;
+    std::vector<LimbItem> limb_items;
+    std::vector<uint16_t> limb_idx;
+    std::vector<uint32_t> limb_lanes;
   };
   Tables tables() const;
 

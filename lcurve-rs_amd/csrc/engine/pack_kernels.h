@@ -30,6 +30,11 @@ struct PackSeg {
   uint32_t flags;     // PSEG_* below
   uint32_t base_flags;  // SegDev::flags before the linear-law rescaling
   CodeConsts K;
+  // The per-lane split's element-level cut-offs: a lane sees the patch
+  // whole when every element's FP32 mu exceeds `whole` (= the seen-whole
+  // threshold + 1e-6) and none of it when every mu is below `none` (= the
+  // cut-off - 1e-6).
+  float whole, none;
 };
 constexpr uint32_t PSEG_CULL = 1u, PSEG_FRONT = 2u, PSEG_MOM = 4u, PSEG_LINW = 8u;
 
@@ -45,6 +50,7 @@ struct PackSet {
   uint32_t tile0;              // first tile (batch-wide)
   uint32_t patch0, n_patches;  // first patch, patches
   uint32_t n_chunks;           // work chunks (partial-row columns)
+  uint32_t chunk0;             // first chunk (batch-wide)
   uint64_t code0;              // first code: code0 + tile * n_patches + local patch (tile-major)
   uint64_t prow_base;          // first partial row
 };
@@ -66,9 +72,22 @@ struct PatchBlock {
   uint32_t set, patch0, n;
 };
 
-// Item counts per kernel class (plain, staged intervals, beaming / lensing).
+// Per work-item candidate: items per kernel class (plain, staged
+// intervals, beaming / lensing), lane-mask groups and item-moment entries.
 struct ItemCounts {
-  uint32_t a, b, c;
+  uint32_t a, b, c, masks, moms;
+};
+
+// Per chunk: limb chunks (0 or 1), limb items and limb lanes.
+struct LimbCounts {
+  uint32_t chunks, items, lanes;
+};
+
+// What the pack's first phase counted (totals of the scans), read by the
+// host before the second phase writes the tables these size.
+struct PackCounts {
+  ItemCounts items;
+  LimbCounts limb;
 };
 
 struct PackDev {
@@ -101,6 +120,10 @@ struct PackDev {
   const double* lane_phi = nullptr;
   const float4* lane_e = nullptr;
   const uint32_t* lane_meta = nullptr;
+  const SetDev* setdev = nullptr;      // the flux kernels' set table (limb-darkening laws)
+  // Limb lanes: 0 off; otherwise a PT_FREE (patch, tile) pair is split per
+  // lane when at most `limb` of its lanes see the patch in part.
+  uint32_t limb = 0;
   // work and outputs
   uint32_t* wmax = nullptr;          // per segment: largest weight (FP32 bits), then a bad-weight flag
   uint32_t* keys = nullptr;          // [n_raw] x 2 (sort in / out)
@@ -118,12 +141,49 @@ struct PackDev {
   // Per (patch, tile), patch-major (code0 + local patch * n_tiles + tile):
   // bit 0 no lane eclipsed, bit 1 every lane eclipsed.
   uint8_t* eflags = nullptr;
+  // Per-lane split of each PT_FREE pair (tile-major, as the codes):
+  // has_split 1 where the pair was split, then the lanes of the tile that
+  // see the patch whole (split_all) and in part (split_some).
+  uint8_t* has_split = nullptr;
+  uint64_t* split_all = nullptr;
+  uint64_t* split_some = nullptr;
+  // The split's work: pairs split and their limb lanes.
+  unsigned long long* split_stats = nullptr;
+
+  // Per chunk: 1 when its limb lanes exceed LIMB_UNION_MAX (its split
+  // patches go back to the element loops), the fixed-point scale, the
+  // limb lanes of each of its patches ([n_chunks][MAX_ITEM_PATCHES]), and
+  // its limb counts with their exclusive scan.
+  uint8_t* chunk_rev = nullptr;
+  int32_t* chunk_shift = nullptr;
+  uint32_t* chunk_nq = nullptr;
+  LimbCounts* chunk_lcnt = nullptr;
+  LimbCounts* chunk_lpos = nullptr;
   ItemDev* cand = nullptr;           // [n_chunks * max_tiles]
-  uint8_t* cand_class = nullptr;
+  // class | ngrp << 8 | (item has PT_MOM patches) << 16
+  uint32_t* cand_info = nullptr;
+  // Candidates with split patches (their lane-mask groups are counted and
+  // written a wavefront each): the list's length, then the list.
+  uint32_t* glist_n = nullptr;
+  uint32_t* glist = nullptr;
   ItemCounts* cand_pos = nullptr;
   ItemDev* items = nullptr;
-  double* imoms = nullptr;
-  ItemCounts* counts = nullptr;      // one: items per class
+  // Tables the second phase writes, sized by the counts of the first.
+  double* imoms = nullptr;           // item-moment entries
+  // Per entry: the moments of the item's first patch, and the mask of the
+  // item's patches it sums.
+  uint32_t* ment_base = nullptr;
+  uint64_t* ment_mask = nullptr;
+  uint64_t cap_imoms = 0;
+  uint64_t* lane_masks = nullptr;    // lane-mask groups of the items' split patches
+  uint64_t cap_masks = 0;
+  LimbChunk* lchunks = nullptr;      // capacity n_chunks
+  LimbItem* litems = nullptr;
+  uint16_t* lidx = nullptr;          // TILE per limb item
+  uint64_t cap_litems = 0;
+  uint32_t* llanes = nullptr;
+  uint64_t cap_llanes = 0;
+  PackCounts* counts = nullptr;
   // Partial rows of the flux pass: the row of a (chunk, tile) pair without
   // an item is zeroed here; the rows with one are written by the item.
   double* partials = nullptr;
@@ -132,9 +192,14 @@ struct PackDev {
 // Bytes of the radix sort's temporary storage for n keys. The keys are
 // (segment << 16 | direction key): at most 65536 segments.
 size_t pack_sort_temp_bytes(uint64_t n);
-size_t pack_scan_temp_bytes(uint64_t n);
-// Every step of the pack on `st`; the item counts land in d.counts.
-hipError_t launch_pack(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                       size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st);
+size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks);
+// The pack in two phases on `st`. The first makes the records, cones,
+// moments, codes, lane splits and item candidates and leaves the counts of
+// what the second writes in d.counts; the host reads them and sizes the
+// second phase's tables. The second writes the work items, their lane-mask
+// groups and moment sums, and the limb tables.
+hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
+                             size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st);
+hipError_t launch_pack_fill(const PackDev& d, uint64_t n_entries, hipStream_t st);
 
 }  // namespace lch

@@ -16,6 +16,9 @@ namespace lch {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// Workgroups of the kernels that stride over a list whose length is on the
+// device.
+constexpr uint32_t kListBlocks = 2048;
 
 __device__ inline uint32_t stride_of(const PackSeg& S) { return rec_stride(S.necl, S.posn != 0); }
 
@@ -329,40 +332,72 @@ __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
 
 // Work code of every (patch, tile) pair: a workgroup per (block of 256
 // patches of one set, tile), the tile's lanes in LDS, a thread per patch.
+// With limb lanes, each PT_FREE pair of a segment that allows moments is
+// also split per lane (the host packer's split_lanes, batch_build.cpp, with
+// its element-level check off): a lane of the tile whose FP32 cosine with
+// the patch axis (the one the code decision takes) is above cos_front sees
+// it whole, one below cos_skip sees none of it, and one between sees it in
+// part (a limb lane).
+template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
   __shared__ float ex[TILE], ey[TILE], ez[TILE];
+  __shared__ uint32_t lcp[TILE];
+  __shared__ unsigned long long cnt[2];
   const PatchBlock pb = d.pblocks[blockIdx.x];
   const PackSet set = d.sets[pb.set];
   const uint32_t t = blockIdx.y;
-  if (t >= set.n_tiles) return;  // uniform over the workgroup, before the barrier
+  if (t >= set.n_tiles) return;  // uniform over the workgroup, before the barriers
   const uint32_t tg = set.tile0 + t;
   if (threadIdx.x < TILE) {
     const float4 v = d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x];
     ex[threadIdx.x] = v.x;
     ey[threadIdx.x] = v.y;
     ez[threadIdx.x] = v.z;
+    if (SPLIT) {
+      // The lane's (class, grid type) bit: the lanes a segment's split covers.
+      const uint32_t m = d.lane_meta[set.lane_off + t * TILE + threadIdx.x];
+      lcp[threadIdx.x] = (m & META_VALID) ? cp_bit(meta_class(m), meta_ptype(m)) : 0u;
+      if (threadIdx.x < 2) cnt[threadIdx.x] = 0ull;
+    }
   }
   __syncthreads();
-  if (threadIdx.x >= pb.n) return;
-  const uint32_t p = pb.patch0 + threadIdx.x;
-  const PackSeg S = d.psegs[d.patch_seg[p]];
+  const bool live = threadIdx.x < pb.n;
+  const uint32_t p = pb.patch0 + (live ? threadIdx.x : 0u);
+  const uint32_t seg = d.patch_seg[p];
+  const PackSeg& S = d.psegs[seg];
   const uint32_t lp = p - set.patch0;
   const uint64_t ci = set.code0 + static_cast<uint64_t>(lp) * set.n_tiles + t;  // eclipse flags, patch-major
+  const uint64_t co = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;  // codes, tile-major
   const uint64_t vm = d.tile_valid[tg];
+  const uint32_t sflags = S.flags;
+  const bool cull = (sflags & PSEG_CULL) != 0, front_ok = (sflags & PSEG_FRONT) != 0;
   uint8_t code = PT_SKIP;
-  if (vm != 0ull) {
+  bool has = false;
+  uint64_t all = 0, some = 0;
+  if (live && vm != 0ull) {
     const Cone c = d.cones[p];
     double px, py, pz, plen;
     cone_axis(c, &px, &py, &pz, &plen);
     const float fx = static_cast<float>(px), fy = static_cast<float>(py), fz = static_cast<float>(pz);
+    // The split's per-lane bounds (split_lanes): cos_front / cos_skip of
+    // the cone where it allows them.
+    const bool split = SPLIT && front_ok && plen > 0.0;
+    const bool can_all = c.cosw > S.K.c_front, can_none = cull && c.cosw > S.K.c_skip;
+    const double cf = can_all ? cos_front(S.K, c.cosw) : 2.0;
+    const double cg = can_none ? cos_skip(S.K, c.cosw) : -2.0;
+    const uint32_t use = split ? d.segs[seg].use_mask : 0u;
     float emin = 1.0f, emax = -1.0f;
     for (uint32_t j = 0; j < TILE; ++j) {
       if (!((vm >> j) & 1ull)) continue;
       const float v = los_cosine(ex[j], ey[j], ez[j], fx, fy, fz);
       emin = v < emin ? v : emin;
       emax = v > emax ? v : emax;
+      if (SPLIT && (use & lcp[j])) {
+        const double cv = v;
+        if (cv > cf) all |= 1ull << j;
+        else if (!(cv < cg)) some |= 1ull << j;
+      }
     }
-    const bool cull = (S.flags & PSEG_CULL) != 0, front_ok = (S.flags & PSEG_FRONT) != 0;
     if (cull && plen > 0.0 && c.cosw > S.K.c_skip && emax < cos_skip(S.K, c.cosw)) {
       code = PT_SKIP;
     } else {
@@ -374,103 +409,351 @@ __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
         code = (ef & 2u) ? PT_SKIP : ((ef & 1u) ? (front ? PT_MOM : PT_FREE) : PT_FULL);
       }
     }
+    has = split && code == PT_FREE;
   }
-  d.codes[set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp] = code;
+  if (live) d.codes[co] = code;
+  if (!SPLIT) return;  // uniform
+  if (!has) all = some = 0;
+  if (live) {
+    d.has_split[co] = has ? 1u : 0u;
+    d.split_all[co] = all;
+    d.split_some[co] = some;
+  }
+  // Counts per wavefront, then one LDS add per wavefront.
+  const uint32_t nh = static_cast<uint32_t>(__popcll(__ballot(has)));
+  uint32_t nl = static_cast<uint32_t>(__popcll(some));
+  for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
+  if (threadIdx.x % TILE == 0 && nh) {
+    atomicAdd(&cnt[0], static_cast<unsigned long long>(nh));
+    atomicAdd(&cnt[1], static_cast<unsigned long long>(nl));
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 && cnt[threadIdx.x]) atomicAdd(&d.split_stats[threadIdx.x], cnt[threadIdx.x]);
 }
 
-// Candidate work item of every (chunk, tile): its patch masks and kernel
-// class (0: adds nothing, not launched; 1 plain; 2 staged intervals; 3
-// beaming / lensing).
-__global__ __launch_bounds__(kBlock) void items_kernel(PackDev d) {
-  const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (idx >= static_cast<uint64_t>(d.n_chunks) * d.max_tiles) return;
-  const uint32_t ch = static_cast<uint32_t>(idx / d.max_tiles), t = static_cast<uint32_t>(idx % d.max_tiles);
+// Whether (patch, tile) pair `ci` is taken as a split patch: PT_FREE, split,
+// with at most `limb` lanes that see it in part.
+__device__ inline bool take_split(const PackDev& d, uint64_t ci) {
+  return d.codes[ci] == PT_FREE && d.has_split[ci] && static_cast<uint32_t>(__popcll(d.split_some[ci])) <= d.limb;
+}
+
+__device__ inline uint64_t wave_or64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+__device__ inline uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+// Sum of v over the lanes below this one.
+__device__ inline uint32_t wave_excl(uint32_t v, uint32_t lane) {
+  uint32_t x = v;
+  for (int off = 1; off < TILE; off <<= 1) {
+    const uint32_t o = __shfl_up(x, off);
+    if (lane >= static_cast<uint32_t>(off)) x += o;
+  }
+  return x - v;
+}
+
+// Limb items of a patch with n limb lanes: 64 lanes each, the last one
+// holding the rest; and the lane-group split of item k (the host packer's
+// rule: a remainder of at most 16 (32) lanes is shared by 4 (2) groups).
+__device__ inline uint32_t limb_items_of(uint32_t n) { return n / TILE + (n % TILE ? 1u : 0u); }
+__device__ inline uint32_t limb_split_of(uint32_t n, uint32_t k) {
+  const uint32_t rem = n - k * TILE;
+  return rem <= TILE / 4 ? 2u : (rem <= TILE / 2 ? 1u : 0u);
+}
+
+// Per chunk (a wavefront, a lane per patch): the limb lanes of each patch
+// (lanes of its split pairs that see it in part), the union of limb lanes
+// over the chunk's tiles, whether the union exceeds LIMB_UNION_MAX (then
+// the chunk's split patches go back to the element loops), the number of
+// limb items and the fixed-point scale of its sums (engine.cpp's
+// build_items, in the same order).
+__global__ __launch_bounds__(kBlock) void limb_chunk_kernel(PackDev d) {
+  const uint32_t ch = blockIdx.x * (kBlock / TILE) + threadIdx.x / TILE;
+  if (ch >= d.n_chunks) return;  // whole wavefront
+  const uint32_t q = threadIdx.x % TILE;
   const PackChunk C = d.chunks[ch];
   const PackSet set = d.sets[C.set];
-  uint8_t cls = 0;
-  uint64_t row = ~0ull;
-  if (t < set.n_tiles) {
-    row = set.prow_base + static_cast<uint64_t>(t) * set.n_chunks + C.col;
-    const PackSeg S = d.psegs[C.seg];
-    const SegDev sg = d.segs[C.seg];
-    if (d.tile_cp[set.tile0 + t] & sg.use_mask) {
-      ItemDev it{};
-      it.seg = C.seg;
-      it.e0 = C.e0;
-      it.ne = C.ne;
-      it.lane0 = set.lane_off + t * TILE;
-      it.prow = row;
-      const uint32_t q0 = (S.patch0 - set.patch0) + C.e0 / PATCH;
-      const uint32_t np = C.ne / PATCH;
-      const uint8_t* codes = d.codes + set.code0 + static_cast<uint64_t>(t) * set.n_patches + q0;
-      for (uint32_t q = 0; q < np; ++q) {
-        const uint8_t code = codes[q];
-        if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
-        if (code == PT_FREE) it.free |= 1ull << q;
-        if (code == PT_MOM) it.mom |= 1ull << q;
-      }
-      if (it.todo || it.mom) {
-        const bool fx = sg.kind <= KIND_STAR2 && S.posn;
-        const bool staged = (it.todo & ~it.free) != 0ull && S.necl >= 1 && S.necl <= LDS_ECL_MAX;
-        cls = fx ? 3 : (staged ? 2 : 1);
-        d.cand[idx] = it;
+  const PackSeg& S = d.psegs[C.seg];
+  const SegDev sg = d.segs[C.seg];
+  const uint32_t np = C.ne / PATCH;
+  const uint32_t lp = (S.patch0 - set.patch0) + C.e0 / PATCH + q;
+  uint32_t nq = 0, nu = 0;
+  for (uint32_t t = 0; t < set.n_tiles; ++t) {
+    if (!(d.tile_cp[set.tile0 + t] & sg.use_mask)) continue;  // uniform
+    uint64_t s = 0;
+    if (q < np) {
+      const uint64_t ci = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;
+      if (take_split(d, ci)) s = d.split_some[ci];
+    }
+    nq += static_cast<uint32_t>(__popcll(s));
+    nu += static_cast<uint32_t>(__popcll(wave_or64(s)));
+  }
+  const bool rev = nu > LIMB_UNION_MAX;
+  const bool has = !rev && nu > 0;
+  if (!has) nq = 0;
+  const uint32_t n_items = wave_sum(limb_items_of(nq));
+  // A lane's total over the chunk's limb patches is at most
+  // sum |w| * max |I(m)|, m in [0, 1]; sum |w| in patch order.
+  const double mw = nq ? d.moms[static_cast<uint64_t>(S.patch0 + C.e0 / PATCH + q) * MOM_STRIDE + 19] : 0.0;
+  double wsum = 0.0;
+  for (uint32_t k = 0; k < np; ++k) {
+    const double v = __shfl(mw, static_cast<int>(k));
+    if (__shfl(nq, static_cast<int>(k))) wsum += v;
+  }
+  if (q == 0) {
+    double imax = 1.0;
+    const SetDev& sd = d.setdev[C.set];
+    if (sg.kind <= KIND_STAR2) {
+      const float* a = sg.kind == KIND_STAR1 ? sd.ldc1 : sd.ldc2;
+      imax += fabsf(a[0]) + fabsf(a[1]) + fabsf(a[2]) + fabsf(a[3]);
+    } else if (sg.kind != KIND_SPOT) {
+      imax += fabsf(sd.lin_disc) + fabsf(sd.quad_disc);
+    }
+    const double bound = 1.01 * imax * wsum;
+    int ex = 0;
+    if (bound > 0.0) (void)frexp(bound, &ex);  // bound < 2^ex
+    d.chunk_shift[ch] = bound > 0.0 ? 61 - ex : 0;
+    d.chunk_rev[ch] = rev ? 1u : 0u;
+    d.chunk_lcnt[ch] = LimbCounts{has ? 1u : 0u, has ? n_items : 0u, has ? nu : 0u};
+  }
+  d.chunk_nq[static_cast<uint64_t>(ch) * MAX_ITEM_PATCHES + q] = nq;
+}
+
+// The patch masks of candidate (chunk, tile) in a wavefront, a lane per
+// patch, and per lane of a split patch: its lane mask (split_all) and its
+// group, the first split patch with the same mask.
+struct CandMasks {
+  bool live = false;  // the tile uses the segment
+  uint64_t todo = 0, free = 0, mom = 0, pmom = 0;
+  uint64_t mask = 0;  // this lane's split mask (pmom lanes)
+  uint64_t lead = 0;  // lanes whose mask no lower pmom lane has
+  uint32_t ngrp = 0;
+  uint32_t grp = 0;   // this lane's group: its leader's rank among the leaders
+};
+__device__ inline CandMasks cand_masks(const PackDev& d, const PackChunk& C, const PackSet& set, uint32_t ch,
+                                       uint32_t t, uint32_t q) {
+  CandMasks r;
+  const PackSeg& S = d.psegs[C.seg];
+  const uint32_t use = d.segs[C.seg].use_mask;
+  if (!(d.tile_cp[set.tile0 + t] & use)) return r;
+  r.live = true;
+  const uint32_t np = C.ne / PATCH;
+  const uint32_t lp = (S.patch0 - set.patch0) + C.e0 / PATCH + q;
+  const uint64_t ci = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;
+  const uint8_t code = q < np ? d.codes[ci] : PT_SKIP;
+  const bool split = q < np && d.limb && !d.chunk_rev[ch] && take_split(d, ci);
+  r.todo = __ballot(!split && (code == PT_FULL || code == PT_FREE));
+  r.free = __ballot(!split && code == PT_FREE);
+  r.mom = __ballot(code == PT_MOM);
+  r.pmom = __ballot(split);
+  if (r.pmom) {
+    r.mask = split ? d.split_all[ci] : 0ull;
+    // A pmom lane leads its group unless a lower pmom lane has its mask;
+    // its leader is the lowest pmom lane with its mask.
+    bool lead = split;
+    uint32_t first = q;
+    for (uint64_t b = r.pmom; b; b &= b - 1ull) {
+      const uint32_t k = static_cast<uint32_t>(__builtin_ctzll(b));
+      const uint64_t mk = __shfl(r.mask, static_cast<int>(k));
+      if (k < q && mk == r.mask) {
+        lead = false;
+        first = k < first ? k : first;
       }
     }
+    r.lead = __ballot(lead);
+    r.ngrp = static_cast<uint32_t>(__popcll(r.lead));
+    r.grp = split ? static_cast<uint32_t>(__popcll(r.lead & ((1ull << first) - 1ull))) : 0u;
   }
-  d.cand_class[idx] = cls;
-  if (cls == 0 && row != ~0ull) {
-    double2* o = reinterpret_cast<double2*>(d.partials + row * TILE);
-    for (int j = 0; j < TILE / 2; ++j) o[j] = make_double2(0.0, 0.0);
+  return r;
+}
+
+// Candidate work item of every (chunk, tile), a thread each: its patch
+// masks and kernel class (0: adds nothing, not launched; 1 plain; 2 staged
+// intervals; 3 beaming / lensing). Candidates with split patches are listed
+// for groups_kernel, which counts their lane-mask groups. The partial row
+// of a candidate without an item is zeroed here.
+__global__ __launch_bounds__(kBlock) void items_kernel(PackDev d) {
+  const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const uint64_t n = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
+  uint32_t cls = 0;
+  bool listed = false;
+  if (idx < n) {
+    const uint32_t ch = static_cast<uint32_t>(idx / d.max_tiles), t = static_cast<uint32_t>(idx % d.max_tiles);
+    const PackChunk C = d.chunks[ch];
+    const PackSet set = d.sets[C.set];
+    uint64_t row = ~0ull;
+    uint64_t mom = 0;
+    if (t < set.n_tiles) {
+      row = set.prow_base + static_cast<uint64_t>(t) * set.n_chunks + C.col;
+      const PackSeg& S = d.psegs[C.seg];
+      const SegDev sg = d.segs[C.seg];
+      if (d.tile_cp[set.tile0 + t] & sg.use_mask) {
+        ItemDev it{};
+        it.seg = C.seg;
+        it.e0 = C.e0;
+        it.ne = C.ne;
+        it.lane0 = set.lane_off + t * TILE;
+        it.prow = row;
+        const uint32_t np = C.ne / PATCH;
+        const uint64_t co0 = set.code0 + static_cast<uint64_t>(t) * set.n_patches + (S.patch0 - set.patch0) + C.e0 / PATCH;
+        const bool can_split = d.limb && !d.chunk_rev[ch];
+        for (uint32_t q = 0; q < np; ++q) {
+          const uint8_t code = d.codes[co0 + q];
+          if (code == PT_FREE && can_split && take_split(d, co0 + q)) {
+            it.pmom |= 1ull << q;
+            continue;
+          }
+          if (code == PT_FULL || code == PT_FREE) it.todo |= 1ull << q;
+          if (code == PT_FREE) it.free |= 1ull << q;
+          if (code == PT_MOM) it.mom |= 1ull << q;
+        }
+        if (it.todo || it.mom || it.pmom) {
+          const bool fx = sg.kind <= KIND_STAR2 && S.posn;
+          const bool staged = (it.todo & ~it.free) != 0ull && S.necl >= 1 && S.necl <= LDS_ECL_MAX;
+          cls = fx ? 3u : (staged ? 2u : 1u);
+          mom = it.mom;
+          listed = it.pmom != 0ull;
+          d.cand[idx] = it;
+        }
+      }
+    }
+    d.cand_info[idx] = cls | ((mom ? 1u : 0u) << 16);
+    if (cls == 0 && row != ~0ull) {
+      double2* o = reinterpret_cast<double2*>(d.partials + row * TILE);
+      for (int j = 0; j < TILE / 2; ++j) o[j] = make_double2(0.0, 0.0);
+    }
+  }
+  // The candidates with split patches, appended by wavefront.
+  if (d.limb) {
+    const uint64_t b = __ballot(listed);
+    if (b) {
+      uint32_t base = 0;
+      const uint32_t lane = threadIdx.x % TILE;
+      if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) base = atomicAdd(d.glist_n, static_cast<uint32_t>(__popcll(b)));
+      base = static_cast<uint32_t>(__shfl(static_cast<int>(base), __builtin_ctzll(b)));
+      if (listed) d.glist[base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)))] = static_cast<uint32_t>(idx);
+    }
   }
 }
 
-struct ClassCount {
-  __device__ ItemCounts operator()(uint8_t c) const {
-    return ItemCounts{c == 1 ? 1u : 0u, c == 2 ? 1u : 0u, c == 3 ? 1u : 0u};
+// Lane-mask groups of each listed candidate, a wavefront each (over the
+// list): their number goes into the candidate's item and info.
+__global__ __launch_bounds__(kBlock) void groups_kernel(PackDev d) {
+  const uint32_t n = *d.glist_n;
+  const uint32_t lane = threadIdx.x % TILE;
+  const uint32_t nw = gridDim.x * (kBlock / TILE);
+  for (uint32_t i = blockIdx.x * (kBlock / TILE) + threadIdx.x / TILE; i < n; i += nw) {
+    const uint32_t idx = d.glist[i];
+    const uint32_t ch = idx / d.max_tiles, t = idx % d.max_tiles;
+    const PackChunk C = d.chunks[ch];
+    const CandMasks r = cand_masks(d, C, d.sets[C.set], ch, t, lane);
+    if (lane == 0) {
+      d.cand[idx].ngrp = r.ngrp;
+      d.cand_info[idx] |= r.ngrp << 8;
+    }
+  }
+}
+
+struct CandCount {
+  __device__ ItemCounts operator()(uint32_t v) const {
+    const uint32_t c = v & 0xffu, g = (v >> 8) & 0xffu, hm = (v >> 16) & 1u;
+    return ItemCounts{c == 1 ? 1u : 0u, c == 2 ? 1u : 0u, c == 3 ? 1u : 0u, g, hm + g};
   }
 };
 struct AddCounts {
   __device__ ItemCounts operator()(const ItemCounts& a, const ItemCounts& b) const {
-    return ItemCounts{a.a + b.a, a.b + b.b, a.c + b.c};
+    return ItemCounts{a.a + b.a, a.b + b.b, a.c + b.c, a.masks + b.masks, a.moms + b.moms};
+  }
+};
+struct AddLimb {
+  __device__ LimbCounts operator()(const LimbCounts& a, const LimbCounts& b) const {
+    return LimbCounts{a.chunks + b.chunks, a.items + b.items, a.lanes + b.lanes};
   }
 };
 
-// Items in launch order (plain, then staged intervals, then beaming /
-// lensing), each class in candidate order; each item's moment entry is its
-// own index.
+// Totals of the two scans: the last exclusive prefix plus the last entry.
+__global__ void totals_kernel(PackDev d) {
+  PackCounts c{};
+  const uint64_t n = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
+  if (n) {
+    const ItemCounts last = CandCount()(d.cand_info[n - 1]);
+    c.items = AddCounts()(d.cand_pos[n - 1], last);
+  }
+  if (d.n_chunks && d.limb) c.limb = AddLimb()(d.chunk_lpos[d.n_chunks - 1], d.chunk_lcnt[d.n_chunks - 1]);
+  *d.counts = c;
+}
+
+// The launch position of a candidate's item: plain, then staged intervals,
+// then beaming / lensing, each class in candidate order.
+__device__ inline uint32_t item_slot(uint32_t cls, const ItemCounts& pos, const ItemCounts& tot) {
+  return cls == 1 ? pos.a : (cls == 2 ? tot.a + pos.b : tot.a + tot.b + pos.c);
+}
+
+// Items in launch order, a thread per candidate, with their first moment
+// entry and lane-mask group; the entry of its PT_MOM patches (moment sums
+// are made per entry: the patch mask and the item's first patch).
 __global__ __launch_bounds__(kBlock) void scatter_kernel(PackDev d) {
   const uint64_t n = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
   const uint64_t idx = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= n) return;
-  const uint8_t lc = d.cand_class[n - 1];
-  ItemCounts tot = d.cand_pos[n - 1];
-  tot.a += lc == 1 ? 1u : 0u;
-  tot.b += lc == 2 ? 1u : 0u;
-  tot.c += lc == 3 ? 1u : 0u;
-  if (idx == 0) *d.counts = tot;
-  const uint8_t cls = d.cand_class[idx];
+  const uint32_t cls = d.cand_info[idx] & 0xffu;
   if (!cls) return;
+  const ItemCounts tot = d.counts->items;
   const ItemCounts pos = d.cand_pos[idx];
-  const uint32_t out = cls == 1 ? pos.a : (cls == 2 ? tot.a + pos.b : tot.a + tot.b + pos.c);
   ItemDev it = d.cand[idx];
-  it.imom = out;
-  d.items[out] = it;
+  it.lm_off = pos.masks;
+  it.imom = pos.moms;
+  d.items[item_slot(cls, pos, tot)] = it;
+  if (it.mom && pos.moms < d.cap_imoms) {
+    d.ment_base[pos.moms] = d.segs[it.seg].mom_off + it.e0 / PATCH;
+    d.ment_mask[pos.moms] = it.mom;
+  }
 }
 
-// Summed moments of each item's PT_MOM patches, in patch order: a thread
-// per item.
-__global__ __launch_bounds__(kBlock) void imoms_kernel(PackDev d, uint64_t capacity) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= capacity) return;
-  const ItemCounts n = *d.counts;
-  if (i >= static_cast<uint64_t>(n.a) + n.b + n.c) return;
-  const ItemDev it = d.items[i];
-  if (!it.mom) return;
-  const double* m0 = d.moms + (static_cast<uint64_t>(d.segs[it.seg].mom_off) + it.e0 / PATCH) * MOM_STRIDE;
+// The lane-mask groups of each listed candidate's item, a wavefront each:
+// the group masks (split_all of each group's first split patch, in order)
+// and one moment entry per group (its split patches).
+__global__ __launch_bounds__(kBlock) void scatter_groups_kernel(PackDev d) {
+  const uint32_t n = *d.glist_n;
+  const uint32_t lane = threadIdx.x % TILE;
+  const uint32_t nw = gridDim.x * (kBlock / TILE);
+  for (uint32_t i = blockIdx.x * (kBlock / TILE) + threadIdx.x / TILE; i < n; i += nw) {
+    const uint32_t idx = d.glist[i];
+    const uint32_t ch = idx / d.max_tiles, t = idx % d.max_tiles;
+    const PackChunk C = d.chunks[ch];
+    const CandMasks r = cand_masks(d, C, d.sets[C.set], ch, t, lane);
+    const ItemCounts pos = d.cand_pos[idx];
+    const bool split = (r.pmom >> lane) & 1ull;
+    if ((r.lead >> lane) & 1ull) {
+      const uint64_t g = pos.masks + static_cast<uint64_t>(__popcll(r.lead & ((1ull << lane) - 1ull)));
+      if (g < d.cap_masks) d.lane_masks[g] = r.mask;
+    }
+    const uint32_t base = d.segs[C.seg].mom_off + C.e0 / PATCH;
+    const uint64_t e0 = static_cast<uint64_t>(pos.moms) + ((d.cand_info[idx] >> 16) & 1u);
+    for (uint32_t k = 0; k < r.ngrp; ++k) {
+      const uint64_t gm = __ballot(split && r.grp == k);
+      if (lane == 0 && e0 + k < d.cap_imoms) {
+        d.ment_base[e0 + k] = base;
+        d.ment_mask[e0 + k] = gm;
+      }
+    }
+  }
+}
+
+// Moment entries, a thread each: the sum, in patch order, of the moments
+// of the patches in its mask.
+__global__ __launch_bounds__(kBlock) void imoms_kernel(PackDev d) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const uint64_t n = d.counts->items.moms;
+  if (e >= n || e >= d.cap_imoms) return;
+  const double* m0 = d.moms + static_cast<uint64_t>(d.ment_base[e]) * MOM_STRIDE;
   double s[MOM_STRIDE];
 #pragma unroll
   for (uint32_t c = 0; c < MOM_STRIDE; ++c) s[c] = 0.0;
-  for (uint64_t mm = it.mom; mm; mm &= mm - 1ull) {
+  for (uint64_t mm = d.ment_mask[e]; mm; mm &= mm - 1ull) {
     const double2* a = reinterpret_cast<const double2*>(m0 + static_cast<uint64_t>(__builtin_ctzll(mm)) * MOM_STRIDE);
 #pragma unroll
     for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) {
@@ -479,9 +762,90 @@ __global__ __launch_bounds__(kBlock) void imoms_kernel(PackDev d, uint64_t capac
       s[2 * c + 1] = s[2 * c + 1] + v.y;
     }
   }
-  double2* o = reinterpret_cast<double2*>(d.imoms + i * MOM_STRIDE);
+  double2* o = reinterpret_cast<double2*>(d.imoms + e * MOM_STRIDE);
 #pragma unroll
   for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) o[c] = make_double2(s[2 * c], s[2 * c + 1]);
+}
+
+// The limb tables of each chunk with limb lanes, a wavefront (a workgroup)
+// per chunk and a lane per patch: its LimbChunk, its lane list (the union
+// of its limb lanes, by tile and then lane), and per patch its limb items
+// (64 lanes each, the last one split into lane groups) with each item's 64
+// union indices (0xFFFF unused), in the host packer's order. A chunk's
+// union indices are assembled in LDS when they fit and then copied out
+// whole; a larger chunk writes them in place.
+constexpr uint32_t LIMB_FILL_LDS = 16384;  // union indices (256 limb items)
+__global__ __launch_bounds__(TILE) void limb_fill_kernel(PackDev d) {
+  __shared__ uint64_t lbuf8[LIMB_FILL_LDS / 4];
+  uint16_t* lbuf = reinterpret_cast<uint16_t*>(lbuf8);
+  const uint32_t ch = blockIdx.x;
+  if (ch >= d.n_chunks) return;  // whole workgroup
+  const LimbCounts cnt = d.chunk_lcnt[ch];
+  if (!cnt.chunks) return;
+  const LimbCounts pos = d.chunk_lpos[ch];
+  if (pos.chunks >= d.n_chunks || static_cast<uint64_t>(pos.items) + cnt.items > d.cap_litems ||
+      static_cast<uint64_t>(pos.lanes) + cnt.lanes > d.cap_llanes)
+    return;
+  const uint32_t q = threadIdx.x;
+  const PackChunk C = d.chunks[ch];
+  const PackSet set = d.sets[C.set];
+  const PackSeg& S = d.psegs[C.seg];
+  const uint32_t use = d.segs[C.seg].use_mask;
+  const uint32_t np = C.ne / PATCH;
+  const uint32_t nq = d.chunk_nq[static_cast<uint64_t>(ch) * MAX_ITEM_PATCHES + q];
+  const uint32_t ni = limb_items_of(nq);
+  const uint32_t ib = wave_excl(ni, q);  // this patch's first item, relative to the chunk's
+  const bool staged = static_cast<uint64_t>(cnt.items) * TILE <= LIMB_FILL_LDS;
+  uint16_t* out = staged ? lbuf : d.lidx + static_cast<uint64_t>(pos.items) * TILE;
+  if (q == 0) {
+    LimbChunk lc{};
+    lc.seg = C.seg;
+    lc.chunk = C.col;
+    lc.item0 = pos.items;
+    lc.n_items = cnt.items;
+    lc.lane_off = pos.lanes;
+    lc.n_lanes = cnt.lanes;
+    lc.shift = d.chunk_shift[ch];
+    d.lchunks[pos.chunks] = lc;
+  }
+  for (uint32_t k = 0; k < ni; ++k) d.litems[pos.items + ib + k] = LimbItem{C.e0 + q * PATCH, limb_split_of(nq, k)};
+  // Unused slots: those past the last item's lanes, in each of its groups.
+  if (ni) {
+    const uint32_t k = ni - 1, sp = limb_split_of(nq, k), gl = TILE >> sp, rem = nq - k * TILE;
+    for (uint32_t g = 0; g < (1u << sp); ++g)
+      for (uint32_t slot = rem; slot < gl; ++slot) out[(ib + k) * TILE + g * gl + slot] = 0xFFFFu;
+  }
+  const uint32_t lp = (S.patch0 - set.patch0) + C.e0 / PATCH + q;
+  uint32_t run = 0, posq = 0;
+  for (uint32_t t = 0; t < set.n_tiles; ++t) {
+    if (!(d.tile_cp[set.tile0 + t] & use)) continue;  // uniform
+    uint64_t sm = 0;
+    if (q < np) {
+      const uint64_t ci = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;
+      if (take_split(d, ci)) sm = d.split_some[ci];
+    }
+    const uint64_t u = wave_or64(sm);
+    // Lane q lists lane q of the tile if it is a limb lane.
+    if ((u >> q) & 1ull)
+      d.llanes[pos.lanes + run + static_cast<uint32_t>(__popcll(u & ((1ull << q) - 1ull)))] = set.lane_off + t * TILE + q;
+    for (uint64_t b = sm; b; b &= b - 1ull, ++posq) {
+      const uint32_t jb = static_cast<uint32_t>(__builtin_ctzll(b));
+      const uint16_t ui = static_cast<uint16_t>(run + static_cast<uint32_t>(__popcll(u & ((1ull << jb) - 1ull))));
+      const uint32_t k = posq / TILE, slot = posq % TILE;
+      const uint32_t sp = limb_split_of(nq, k), gl = TILE >> sp;
+      for (uint32_t g = 0; g < (1u << sp); ++g) out[(ib + k) * TILE + g * gl + slot] = ui;
+    }
+    run += static_cast<uint32_t>(__popcll(u));
+  }
+  if (staged) {
+    __syncthreads();
+    // Copied out 8 bytes per lane and step (the chunk's indices start at a
+    // multiple of 64 entries).
+    const uint64_t* src = lbuf8;
+    uint64_t* dst = reinterpret_cast<uint64_t*>(d.lidx + static_cast<uint64_t>(pos.items) * TILE);
+    const uint32_t n8 = cnt.items * TILE / 4;
+    for (uint32_t i = q; i < n8; i += TILE) dst[i] = src[i];
+  }
 }
 
 uint32_t blocks(uint64_t n, uint32_t b = kBlock) { return static_cast<uint32_t>((n + b - 1) / b); }
@@ -496,17 +860,23 @@ size_t pack_sort_temp_bytes(uint64_t n) {
   return bytes;
 }
 
-size_t pack_scan_temp_bytes(uint64_t n) {
-  size_t bytes = 0;
-  auto in = rocprim::make_transform_iterator(static_cast<const uint8_t*>(nullptr), ClassCount());
-  (void)rocprim::exclusive_scan(nullptr, bytes, in, static_cast<ItemCounts*>(nullptr), ItemCounts{0, 0, 0},
-                                static_cast<size_t>(n), AddCounts());
-  return bytes;
+size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks) {
+  size_t a = 0, b = 0;
+  auto in = rocprim::make_transform_iterator(static_cast<const uint32_t*>(nullptr), CandCount());
+  (void)rocprim::exclusive_scan(nullptr, a, in, static_cast<ItemCounts*>(nullptr), ItemCounts{0, 0, 0, 0, 0},
+                                static_cast<size_t>(n_cand), AddCounts());
+  (void)rocprim::exclusive_scan(nullptr, b, static_cast<const LimbCounts*>(nullptr), static_cast<LimbCounts*>(nullptr),
+                                LimbCounts{0, 0, 0}, static_cast<size_t>(n_chunks), AddLimb());
+  return a > b ? a : b;
 }
 
-hipError_t launch_pack(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                       size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st) {
+hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
+                             size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st) {
   hipError_t e = hipSuccess;
+  if (d.limb) {
+    e = hipMemsetAsync(d.split_stats, 0, 2 * sizeof(unsigned long long), st);
+    if (e != hipSuccess) return e;
+  }
   if (d.n_segs) {
     e = hipMemsetAsync(d.wmax, 0, static_cast<size_t>(d.n_segs) * 2u * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
@@ -524,22 +894,45 @@ hipError_t launch_pack(const PackDev& d, void* sort_temp, size_t sort_temp_bytes
   if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(d.n_patches), dim3(TILE), 0, st, d);
   if (d.n_tiles) hipLaunchKernelGGL(tiles_kernel, dim3(d.n_tiles), dim3(TILE), 0, st, d);
   if (d.n_ecl_patches) hipLaunchKernelGGL(eclflags_kernel, dim3(d.n_ecl_patches), dim3(TILE), 0, st, d);
-  if (d.n_pblocks && d.max_tiles)
-    hipLaunchKernelGGL(codes_kernel, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
-  const uint64_t nc = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
-  if (nc) {
-    hipLaunchKernelGGL(items_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
-    auto in = rocprim::make_transform_iterator(static_cast<const uint8_t*>(d.cand_class), ClassCount());
+  if (d.n_pblocks && d.max_tiles) {
+    if (d.limb) hipLaunchKernelGGL(codes_kernel<true>, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
+    else hipLaunchKernelGGL(codes_kernel<false>, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
+  }
+  const uint32_t wpb = kBlock / TILE;  // wavefronts per workgroup
+  if (d.limb && d.n_chunks) {
+    hipLaunchKernelGGL(limb_chunk_kernel, dim3(blocks(d.n_chunks, wpb)), dim3(kBlock), 0, st, d);
     size_t tb = scan_temp_bytes;
-    e = rocprim::exclusive_scan(scan_temp, tb, in, d.cand_pos, ItemCounts{0, 0, 0}, static_cast<size_t>(nc),
-                                AddCounts(), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
-    hipLaunchKernelGGL(imoms_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d, nc);
-  } else {
-    e = hipMemsetAsync(d.counts, 0, sizeof(ItemCounts), st);
+    e = rocprim::exclusive_scan(scan_temp, tb, d.chunk_lcnt, d.chunk_lpos, LimbCounts{0, 0, 0},
+                                static_cast<size_t>(d.n_chunks), AddLimb(), st);
     if (e != hipSuccess) return e;
   }
+  const uint64_t nc = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
+  if (nc) {
+    if (d.limb) {
+      e = hipMemsetAsync(d.glist_n, 0, sizeof(uint32_t), st);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(items_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
+    // The list's length stays on the device: a fixed grid strides over it.
+    if (d.limb) hipLaunchKernelGGL(groups_kernel, dim3(kListBlocks), dim3(kBlock), 0, st, d);
+    auto in = rocprim::make_transform_iterator(static_cast<const uint32_t*>(d.cand_info), CandCount());
+    size_t tb = scan_temp_bytes;
+    e = rocprim::exclusive_scan(scan_temp, tb, in, d.cand_pos, ItemCounts{0, 0, 0, 0, 0}, static_cast<size_t>(nc),
+                                AddCounts(), st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_fill(const PackDev& d, uint64_t n_entries, hipStream_t st) {
+  const uint64_t nc = static_cast<uint64_t>(d.n_chunks) * d.max_tiles;
+  if (nc) {
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
+    if (d.limb) hipLaunchKernelGGL(scatter_groups_kernel, dim3(kListBlocks), dim3(kBlock), 0, st, d);
+  }
+  if (n_entries) hipLaunchKernelGGL(imoms_kernel, dim3(blocks(n_entries)), dim3(kBlock), 0, st, d);
+  if (d.limb && d.n_chunks) hipLaunchKernelGGL(limb_fill_kernel, dim3(d.n_chunks), dim3(TILE), 0, st, d);
   return hipGetLastError();
 }
 

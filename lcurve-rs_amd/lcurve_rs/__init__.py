@@ -94,13 +94,17 @@ def set_tuning(chunk, persistent=0):
     check(_native.lib().lch_ctx_set_tuning(_ctx(), int(chunk), int(persistent)))
 
 
-def set_limb(on, max_part=48):
+def set_limb(on, max_part=48, check_elements=True):
     """Limb lanes for batches built from now on (default on): no-eclipse
     patches are split per lane into closed-form moments, limb lanes evaluated
     element by element (regrouped across tiles) and lanes that see none of
     the patch, where at most `max_part` of a tile's lanes see it in part.
-    Off: every lane of the tile runs the element loop."""
+    Off: every lane of the tile runs the element loop. check_elements: the
+    host packer decides the lanes the split's cone test leaves undecided from
+    the patch's elements (default); False keeps them limb lanes, the device
+    pack's rule (``DevicePackBatch``)."""
     check(_native.lib().lch_ctx_set_limb(_ctx(), int(max_part) if on else 0))
+    check(_native.lib().lch_ctx_set_limb_check(_ctx(), int(bool(check_elements))))
 
 
 def sync():
@@ -455,13 +459,15 @@ class DevicePackBatch:
     (lch_dpack_*): the constructor does the CPU setup and each set's host
     layout and stages the raw element grids; every ``run`` / ``time`` packs
     all sets on the GPU from those grids (element records, patch cones and
-    moments, work codes, work items) and runs the flux pass and the
-    assembly. Results equal a ``Batch`` of the same models with limb lanes
-    off and the same chunk, bit for bit. The stand-in for the reference's
-    chisq_batch_gpu (lcurve-cuda/src/batch.rs:73-136) after prepare_model."""
+    moments, work codes, per-lane splits, work items, limb tables) and runs
+    the flux pass and the assembly. Results equal a ``Batch`` of the same
+    models built with the same chunk and limb threshold (``limb``: 0 off,
+    else ``set_limb(True, limb)``), bit for bit. The stand-in for the
+    reference's chisq_batch_gpu (lcurve-cuda/src/batch.rs:73-136) after
+    prepare_model."""
 
     def __init__(self, models, times, expose, ndiv, *, flux=None, ferr=None, weight=None, flux2d=None,
-                 ferr2d=None, weight2d=None, chunk=2048):
+                 ferr2d=None, weight2d=None, chunk=2048, limb=0):
         L = _native.lib()
         t = _f64(times)
         n = len(t)
@@ -477,7 +483,7 @@ class DevicePackBatch:
         out = ctypes.c_void_p()
         check(L.lch_dpack_build(_ctx(), harr, self.n_sets, _dptr(t), _dptr(ex), _dptr(fl), _dptr(fe), _dptr(wt),
                                 _iptr(nd), n, *[None if a is None else _dptr(a) for a in (f2, e2, w2)],
-                                int(chunk), ctypes.byref(out)))
+                                int(chunk), int(limb), ctypes.byref(out)))
         self._h = out
 
     def __del__(self):
@@ -529,10 +535,12 @@ class DevicePackBatch:
     def check(self, set_index):
         """Host pack of one set against a fresh device pack: dict of
         mismatch counts (records bytes, cones, moments, codes, segment
-        flags, items, item moments) and the host's item count."""
-        d = (ctypes.c_longlong * 8)()
+        flags, items, item moments and lane-mask groups, limb tables) and
+        the host's item and limb item counts."""
+        d = (ctypes.c_longlong * 10)()
         check(_native.lib().lch_dpack_check(_ctx(), self._h, int(set_index), d))
-        keys = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments", "host_items")
+        keys = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments", "host_items",
+                "limb", "host_limb_items")
         return dict(zip(keys, (int(v) for v in d)))
 
 

@@ -51,9 +51,10 @@ class DPackStatsC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in (
         "n_sets", "n_valid", "n_segs", "n_elements", "n_pairs", "n_patches", "n_tiles", "n_chunks", "n_codes",
         "n_cand", "n_items", "n_items_plain", "n_items_ecl", "rec_bytes", "input_bytes", "work_bytes", "visits",
-        "pairs_evaluated", "pairs_by_moments", "moment_evals")] + \
+        "pairs_evaluated", "pairs_by_moments", "moment_evals", "pairs_limb", "n_limb_chunks", "n_limb_items",
+        "split_pairs", "split_limb_lanes")] + \
         [(k, ctypes.c_double) for k in ("setup_core_s", "layout_core_s", "build_wall_s", "concat_s", "upload_s")] + \
-        [("chunk", ctypes.c_int), ("threads", ctypes.c_int)]
+        [("chunk", ctypes.c_int), ("limb", ctypes.c_int), ("threads", ctypes.c_int)]
 
 
 LL = ctypes.POINTER(ctypes.c_longlong)
@@ -88,6 +89,7 @@ SIGNATURES = {
     "lch_ctx_create": (INT, [INT, ctypes.POINTER(VP)]),
     "lch_ctx_destroy": (None, [VP]),
     "lch_ctx_set_limb": (INT, [VP, INT]),
+    "lch_ctx_set_limb_check": (INT, [VP, INT]),
     "lch_ctx_set_tuning": (INT, [VP, INT, INT]),
     "lch_ctx_sync": (INT, [VP]),
     "lch_model_load_file": (INT, [CP, ctypes.POINTER(VP)]),
@@ -115,7 +117,8 @@ SIGNATURES = {
     "lch_batch_time": (INT, [VP, VP, INT, INT, D, D]),
     "lch_batch_autotune": (INT, [VP, VP, INT]),
     "lch_batch_stats_get": (INT, [VP, ctypes.POINTER(BatchStatsC)]),
-    "lch_dpack_build": (INT, [VP, ctypes.POINTER(VP), INT, D, D, D, D, D, I, INT, D, D, D, INT, ctypes.POINTER(VP)]),
+    "lch_dpack_build": (INT, [VP, ctypes.POINTER(VP), INT, D, D, D, D, D, I, INT, D, D, D, INT, INT,
+                              ctypes.POINTER(VP)]),
     "lch_dpack_free": (None, [VP]),
     "lch_dpack_upload": (INT, [VP, VP]),
     "lch_dpack_launch": (INT, [VP, VP, INT]),

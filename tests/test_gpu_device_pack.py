@@ -3,10 +3,11 @@
 For each model variant of the flux path, the per-set pack the GPU makes from
 the raw grids must equal the host packer's bit for bit: element records in
 patch order, patch cones and moments, work codes, segment flags (linear-law
-rescaling), work items and their moment sums (lch_dpack_check). Its curves and
-chi-squared values must equal those of a host-packed Batch of the same models
-(limb lanes off, the same chunk) bit for bit, and through it the FP64 oracle
-within the parity bar. A set whose setup fails gives NaN, as on the host path.
+rescaling), per-lane splits, work items with their lane-mask groups and moment
+sums, and the limb tables (lch_dpack_check). Its curves and chi-squared values
+must equal those of a host-packed Batch of the same models (the same chunk and
+limb threshold) bit for bit, and through it the FP64 oracle within the parity
+bar. A set whose setup fails gives NaN, as on the host path.
 """
 import numpy as np
 import pytest
@@ -17,13 +18,13 @@ from variants import ISCALE_DISC, VARIANTS
 
 pytestmark = pytest.mark.gpu
 
-CHECK_KEYS = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments")
+CHECK_KEYS = ("record_bytes", "cones", "moments", "codes", "seg_flags", "items", "item_moments", "limb")
 
 
 @pytest.fixture
 def host_tuning(lcurve_rs):
-    """Host-packed batches built in the test use limb lanes off; the
-    context's defaults come back afterwards."""
+    """Host-packed batches built in the test take the device pack's limb
+    threshold; the context's defaults come back afterwards."""
     yield lcurve_rs
     lcurve_rs.set_limb(True)
     lcurve_rs.set_tuning(1024)
@@ -46,14 +47,20 @@ def _obs(n, seed):
     return {"flux": 1.0 + 0.01 * rng.standard_normal(n), "ferr": np.full(n, 0.01), "weight": np.ones(n)}
 
 
-def _compare(lc, models, times, expose, ndiv, chunk, obs):
-    dp = lc.DevicePackBatch(models, times, expose, ndiv, chunk=chunk, **obs)
+def _host_limb(lc, limb):
+    """Host-packed batches under the device pack's split rule (no
+    element-level check of the cone-undecided lanes)."""
+    lc.set_limb(limb > 0, max(limb, 1), check_elements=False)
+
+
+def _compare(lc, models, times, expose, ndiv, chunk, obs, limb=0):
+    dp = lc.DevicePackBatch(models, times, expose, ndiv, chunk=chunk, limb=limb, **obs)
     for s in range(len(models)):
         d = dp.check(s)
         assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
         assert d["host_items"] > 0, (s, d)
     rd = dp.run()
-    lc.set_limb(False)
+    _host_limb(lc, limb)
     lc.set_tuning(chunk)
     hb = lc.Batch(models, times, expose, ndiv, **obs)
     rh = hb.run()
@@ -63,23 +70,26 @@ def _compare(lc, models, times, expose, ndiv, chunk, obs):
     return dp, rd
 
 
+@pytest.mark.parametrize("limb", [0, 48])
 @pytest.mark.parametrize("name", ["base"] + sorted(VARIANTS))
-def test_device_pack_equals_host_pack(host_tuning, golden_model_text, name):
+def test_device_pack_equals_host_pack(host_tuning, golden_model_text, name, limb):
     lc = host_tuning
     text = golden_model_text if name == "base" else W.model_text(VARIANTS[name], golden_model_text)
     times = W.linspace_times(-0.2, 1.2, 300)
-    _compare(lc, _models(lc, text, 3, 1), times, 0.001, 1, 1024, _obs(len(times), 2))
+    _compare(lc, _models(lc, text, 3, 1), times, 0.001, 1, 1024, _obs(len(times), 2), limb=limb)
 
 
-def test_device_pack_smearing_small_chunks_and_iscale(host_tuning, golden_model_text):
+@pytest.mark.parametrize("limb", [0, 16, 64])
+def test_device_pack_smearing_small_chunks_and_iscale(host_tuning, golden_model_text, limb):
     """Exposure sub-phases, several chunks per segment, the disc, rim and
-    spot, and the per-component fits."""
+    spot, and the per-component fits; limb thresholds from few split pairs
+    to every no-eclipse pair."""
     lc = host_tuning
     times = W.linspace_times(-0.3, 1.3, 257)
     text = W.model_text(VARIANTS["disc_edge_spot"], golden_model_text)
-    _compare(lc, _models(lc, text, 2, 3), times, 0.004, 3, 256, _obs(len(times), 4))
+    _compare(lc, _models(lc, text, 2, 3), times, 0.004, 3, 256, _obs(len(times), 4), limb=limb)
     text = W.model_text(dict(ISCALE_DISC, iscale=1), golden_model_text)
-    _compare(lc, _models(lc, text, 2, 5), times, 0.002, 2, 512, _obs(len(times), 6))
+    _compare(lc, _models(lc, text, 2, 5), times, 0.002, 2, 512, _obs(len(times), 6), limb=limb)
 
 
 def test_device_pack_against_oracle(host_tuning, golden_model_text):
@@ -89,17 +99,19 @@ def test_device_pack_against_oracle(host_tuning, golden_model_text):
     text = W.model_text(VARIANTS["opaque_disc_calibration"], golden_model_text)
     times = W.linspace_times(-0.2, 1.2, 200)
     m = lc.Model.from_text(text)
-    dp = lc.DevicePackBatch([m], times, 0.002, 3, chunk=2048)
-    r = dp.run(scale=False)
     ref, _ = pyoracle.light_curve(text, times, expose=0.002, ndiv=3, nthreads=8)
-    assert np.max(np.abs(r["flux"][0] / ref - 1)) <= 1e-6
+    for limb in (0, 48):
+        dp = lc.DevicePackBatch([m], times, 0.002, 3, chunk=2048, limb=limb)
+        r = dp.run(scale=False)
+        assert np.max(np.abs(r["flux"][0] / ref - 1)) <= 1e-6, limb
 
 
-def test_device_pack_many_sets_and_repeat(host_tuning):
+@pytest.mark.parametrize("limb", [0, 48])
+def test_device_pack_many_sets_and_repeat(host_tuning, limb):
     """48 sets of the chi-squared workload's geometry at reduced grids, one
-    of them unphysical: codes and items equal the host's on a sample of sets,
-    results equal the host-packed batch's, the failed set is NaN, and a
-    re-upload and second launch give the same results."""
+    of them unphysical: codes, items and limb tables equal the host's on a
+    sample of sets, results equal the host-packed batch's, the failed set is
+    NaN, and a re-upload and second launch give the same results."""
     lc = host_tuning
     base = W.chisq_base_text()
     p = W.chisq_sets(48, seed=7, nlat_lo=20, nlat_hi=60)
@@ -112,13 +124,15 @@ def test_device_pack_many_sets_and_repeat(host_tuning):
         models.append(lc.Model.from_text(W.model_text(ov, base)))
     times = W.linspace_times(-0.2, 1.2, 400)
     obs = _obs(len(times), 9)
-    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=2048, **obs)
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=2048, limb=limb, **obs)
     for s in (0, 5, 17, 31, 47):
         d = dp.check(s)
         assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
+        if limb and s != 17:
+            assert d["host_limb_items"] > 0, (s, d)
     r1 = dp.run()
     assert np.isnan(r1["chisq"][17]) and np.all(np.isfinite(np.delete(r1["chisq"], 17)))
-    lc.set_limb(False)
+    _host_limb(lc, limb)
     lc.set_tuning(2048)
     rh = lc.Batch(models, times, 0.001, 1, **obs).run()
     ok = np.isfinite(rh["chisq"])
@@ -134,8 +148,10 @@ def test_device_pack_many_sets_and_repeat(host_tuning):
     assert st["n_sets"] == 48 and st["n_valid"] == 47 and st["n_items"] > 0
     assert st["input_bytes"] > 0 and st["upload_s"] > 0.0
     hs = lc.Batch(models, times, 0.001, 1, **obs).stats()
-    for k in ("pairs_evaluated", "pairs_by_moments", "moment_evals", "visits"):
+    for k in ("pairs_evaluated", "pairs_by_moments", "moment_evals", "visits", "pairs_limb"):
         assert st[k] == hs[k], (k, st[k], hs[k])
+    assert st["n_limb_chunks"] == hs["limb_chunks"] and st["n_limb_items"] == hs["limb_items"]
+    assert (st["n_limb_chunks"] > 0) == (limb > 0)
     # new observations: the same as a batch built with them
     obs2 = _obs(len(times), 10)
     f2 = np.tile(obs2["flux"], (48, 1))
