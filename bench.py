@@ -61,7 +61,7 @@ def chisq_models(nsets):
     return texts, sets
 
 
-def build_chisq_dpack(nsets, chunk, limb):
+def build_chisq_dpack(nsets, chunk, limb, slots=1):
     """The device-packed chisq_4096 batch with its synthetic observations:
     each set's true curve (unscaled) with 1% Gaussian noise."""
     texts, sets = chisq_models(nsets)
@@ -69,7 +69,7 @@ def build_chisq_dpack(nsets, chunk, limb):
     log(f"CPU setup and host layout of {nsets} sets (grids, eclipses, brightness; raw grids staged) ...")
     t0 = time.time()
     models = [lc.Model.from_text(x) for x in texts]
-    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=chunk, limb=limb)
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=chunk, limb=limb, slots=slots)
     t_build = time.time() - t0
     st = dp.stats()
     log(f"setup {st['setup_core_s']:.1f} core-s, layout {st['layout_core_s']:.1f} core-s, wall "
@@ -299,7 +299,7 @@ def bench_chisq(args):
     PCIe-inclusive rate and the host-packed replay beside it."""
     chunk = args.chunk if args.chunk > 0 else 4096
     limb = args.device_limb
-    dp, models, info = build_chisq_dpack(args.nsets, chunk, limb)
+    dp, models, info = build_chisq_dpack(args.nsets, chunk, limb, args.dpack_slots)
     scale = info["scale"]
     nsets = info["unit_per_step"]
     out = dp.run(scale=scale)
@@ -354,7 +354,9 @@ def bench_chisq(args):
            "data": "synthetic (seeded parameter sets and noise; no external data)",
            "config": info["config"],
            "step": "every set evaluated afresh from its raw element grids resident in HBM: GPU pack, flux pass, "
-                   "FP64 assembly and chi^2",
+                   "FP64 assembly and chi^2" + ("; consecutive steps alternate between two work-table sets on two "
+                                               "streams, so one step's pack runs beside the previous step's flux pass"
+                                               if args.dpack_slots > 1 else ""),
            "element_visits_per_step": st["visits"],
            "visits_per_s": round(st["visits"] * args.steps / elapsed, 1),
            "reference_equivalent_tflop_s": round(st["visits"] * FLOP_PER_VISIT / (ms_flux * 1e-3) / 1e12, 2),
@@ -369,7 +371,7 @@ def bench_chisq(args):
                          "setup_core_s_per_set": round(st["setup_core_s"] / nsets, 5),
                          "layout_core_s_per_set": round(st["layout_core_s"] / nsets, 5),
                          "build_wall_s": round(st["build_wall_s"], 2), "batch_tables_s": round(st["concat_s"], 2)},
-           "tuning": {"chunk": chunk, "items": st["n_items"], "limb": limb,
+           "tuning": {"chunk": chunk, "items": st["n_items"], "limb": limb, "slots": args.dpack_slots,
                       "limb_chunks": st["n_limb_chunks"], "limb_items": st["n_limb_items"],
                       "split_pairs": st["split_pairs"], "split_limb_lanes": st["split_limb_lanes"]},
            "roofline": roof}
@@ -636,6 +638,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-limb", action="store_true",
                     help="evaluate no-eclipse patches on all lanes of a tile (no limb lanes)")
+    ap.add_argument("--dpack-slots", type=int, default=1,
+                    help="chisq_4096 fresh sets: work-table sets launches alternate between (2: the pack of one "
+                         "launch runs beside the flux pass of the one before)")
     ap.add_argument("--device-limb", type=int, default=0,
                     help="chisq_4096 fresh sets: limb lanes in the device pack (0 off, else the split threshold)")
     ap.add_argument("--limb-max-part", type=int, default=48,

@@ -215,9 +215,10 @@ int lch_batch_visibility(lch_ctx* ctx, lch_batch* b, int set, int grid, const do
  * lch_batch_build_models; chunk: elements per work item, a multiple of 64
  * in 64..4096; limb: 0 no limb lanes, else the largest number of lanes of a
  * tile (1..64) that may see a no-eclipse patch in part for the pair to be
- * split per lane (lch_ctx_set_limb). At most 65536 segments per batch (a set
- * has one per grid and eclipse-pair count, typically 5-12) and under 2^32
- * elements and work-item candidates (chunks x lane tiles). */
+ * split per lane (lch_ctx_set_limb). At most 65536 segments per batch (a
+ * set has one per grid and eclipse-pair count, typically 5-12) and under
+ * 2^32 elements, (patch, tile) pairs and work-item candidates (chunks x lane
+ * tiles). */
 typedef struct lch_dpack lch_dpack;
 typedef struct lch_dpack_stats {
   long long n_sets, n_valid, n_segs, n_elements, n_pairs, n_patches, n_tiles, n_chunks, n_codes;
@@ -225,7 +226,7 @@ typedef struct lch_dpack_stats {
   long long n_items, n_items_plain, n_items_ecl; /* work items of the last pack, by kernel form */
   long long rec_bytes;    /* element records the pack writes */
   long long input_bytes;  /* raw grids and layout tables one lch_dpack_upload copies */
-  long long work_bytes;   /* device tables the pack and the flux pass write */
+  long long work_bytes;   /* device tables the pack and the flux pass write, per work-table set */
   long long visits;       /* element-visits in the reference's loop order per launch */
   long long pairs_evaluated, pairs_by_moments, moment_evals; /* the flux pass's work at the last
                              lch_dpack_run, as lch_batch_stats counts it */
@@ -248,6 +249,11 @@ void lch_dpack_free(lch_dpack* b);
 /* Copy the staged grids and tables to the device again (one copy from
  * pinned host memory) and wait; the build already did it once. */
 int lch_dpack_upload(lch_ctx* ctx, lch_dpack* b);
+/* Work-table sets that launches alternate between: 1 (as built) or 2. With
+ * 2, each launch runs on its own stream and its pack runs beside the flux
+ * pass of the launch before, at twice the work memory (lch_dpack_stats
+ * work_bytes per set). lch_ctx_sync waits for every launch either way. */
+int lch_dpack_set_slots(lch_ctx* ctx, lch_dpack* b, int slots);
 /* Enqueue the pack, wait for its work-item counts, enqueue the flux pass
  * and the assembly; no copy back. */
 int lch_dpack_launch(lch_ctx* ctx, lch_dpack* b, int scale);

@@ -467,6 +467,16 @@ int lch_dpack_upload(lch_ctx* ctx, lch_dpack* b) {
   });
 }
 
+int lch_dpack_set_slots(lch_ctx* ctx, lch_dpack* b, int slots) {
+  CtxScope scope(ctx);
+  if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");
+  if (slots < 1 || slots > 2) return fail(LCH_ERR_INVALID, "slots must be 1 or 2");
+  return guarded([&] {
+    b->impl->set_slots(slots);
+    return LCH_OK;
+  });
+}
+
 int lch_dpack_launch(lch_ctx* ctx, lch_dpack* b, int scale) {
   CtxScope scope(ctx);
   if (!ctx || !b) return fail(LCH_ERR_INVALID, "null argument");

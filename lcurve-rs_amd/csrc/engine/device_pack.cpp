@@ -218,32 +218,38 @@ DevicePackBatch::DevicePackBatch(Context& ctx, const std::vector<lc::Model>& mod
 
 DevicePackBatch::~DevicePackBatch() { release(); }
 
-void DevicePackBatch::release() {
-  (void)hipSetDevice(ctx_.device());
-  if (side_.side) (void)hipStreamSynchronize(side_.side);
-  (void)hipStreamSynchronize(ctx_.stream());
-  for (hipEvent_t& e : ev_) {
+void DevicePackBatch::release_slot(Slot& s) {
+  if (s.side.side) (void)hipStreamSynchronize(s.side.side);
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  for (hipEvent_t& e : s.ev) {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
   }
-  if (side_.fork) (void)hipEventDestroy(side_.fork);
-  if (side_.join) (void)hipEventDestroy(side_.join);
-  if (side_.side) (void)hipStreamDestroy(side_.side);
-  side_ = SideStream{};
-  if (h_in_) (void)hipHostFree(h_in_);
-  if (h_counts_) (void)hipHostFree(h_counts_);
-  if (d_in_) (void)hipFree(d_in_);
-  if (d_work_) (void)hipFree(d_work_);
-  if (sort_temp_) (void)hipFree(sort_temp_);
-  if (scan_temp_) (void)hipFree(scan_temp_);
-  for (void*& g : grow_) {
+  if (s.done) (void)hipEventDestroy(s.done);
+  if (s.side.fork) (void)hipEventDestroy(s.side.fork);
+  if (s.side.join) (void)hipEventDestroy(s.side.join);
+  if (s.side.side) (void)hipStreamDestroy(s.side.side);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  if (s.h_counts) (void)hipHostFree(s.h_counts);
+  if (s.work) (void)hipFree(s.work);
+  if (s.sort_temp) (void)hipFree(s.sort_temp);
+  if (s.scan_temp) (void)hipFree(s.scan_temp);
+  for (void*& g : s.grow) {
     if (g) (void)hipFree(g);
     g = nullptr;
   }
+  s = Slot{};
+}
+
+void DevicePackBatch::release() {
+  (void)hipSetDevice(ctx_.device());
+  (void)hipStreamSynchronize(ctx_.stream());
+  for (auto& s : slots_) release_slot(*s);
+  slots_.clear();
+  if (h_in_) (void)hipHostFree(h_in_);
+  if (d_in_) (void)hipFree(d_in_);
   h_in_ = nullptr;
-  h_counts_ = nullptr;
-  d_in_ = d_work_ = nullptr;
-  sort_temp_ = scan_temp_ = nullptr;
+  d_in_ = nullptr;
 }
 
 void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
@@ -282,16 +288,17 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
     if (L.S.dev.flags & SET_ISCALE) any_iscale = true;
   }
   // Sort keys are (segment << 16 | direction key) in 32 bits.
-  if (n_segs > 65536) throw std::invalid_argument("a device-packed batch holds at most 65536 segments: split the sets");
   if (n_raw >= UINT32_MAX || n_pairs >= UINT32_MAX || n_patches >= UINT32_MAX || n_tiles >= UINT32_MAX ||
       n_lanes >= UINT32_MAX || n_chunks >= UINT32_MAX || n_pblocks >= UINT32_MAX)
     throw std::invalid_argument("device-packed batch too large: split the sets");
   if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
   const uint64_t n_cand = n_chunks * max_tiles;
   if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
-  if (n_codes >= UINT32_MAX) throw std::invalid_argument("too many (patch, tile) pairs: split the sets");
+  // Sort keys are (segment << 16 | direction key) in 32 bits.
+  if (n_segs > 65536) throw std::invalid_argument("a device-packed batch holds at most 65536 segments: split the sets");
   key_bits_ = 16;
   for (uint64_t v = n_segs > 0 ? n_segs - 1 : 0; v; v >>= 1) ++key_bits_;
+  if (n_codes >= UINT32_MAX) throw std::invalid_argument("too many (patch, tile) pairs: split the sets");
   uint64_t n_ktiles = 0;
   for (const auto& Lp : lay)
     for (const SetLayout::Grid& g : Lp->grids) n_ktiles += (g.n + KEY_TILE - 1) / KEY_TILE;
@@ -501,44 +508,59 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
       prow != n_prows || ne_cl != n_eclp || pblk != n_pblocks || sorted != n_raw || eoff[n_raw] != n_pairs)
     throw DeviceError("device-pack layout out of step");
 
-  // Work block.
+  // Work block layout (each slot allocates one).
   BlockLayout w;
-  const size_t w_keys = w.add(n_raw * 4), w_vals = w.add(n_raw * 4), w_keys_s = w.add(n_raw * 4),
-               w_vals_s = w.add(n_raw * 4);
-  const size_t w_recs = w.add(rec), w_cones = w.add(n_patches * sizeof(Cone));
-  const size_t w_moms = w.add(n_patches * MOM_STRIDE * sizeof(double)), w_wmax = w.add(n_segs * 8);
-  const size_t w_ten = w.add(n_tiles * TILE * sizeof(float4)), w_tval = w.add(n_tiles * 8),
-               w_trng = w.add(n_tiles * sizeof(double2)), w_tcp = w.add(n_tiles * 4);
-  const size_t w_codes = w.add(n_codes), w_efl = w.add(n_codes);
+  WorkLayout& L = wl_;
+  L.keys = w.add(n_raw * 4);
+  L.vals = w.add(n_raw * 4);
+  L.keys_s = w.add(n_raw * 4);
+  L.vals_s = w.add(n_raw * 4);
+  L.recs = w.add(rec);
+  L.cones = w.add(n_patches * sizeof(Cone));
+  L.moms = w.add(n_patches * MOM_STRIDE * sizeof(double));
+  L.wmax = w.add(n_segs * 8);
+  L.ten = w.add(n_tiles * TILE * sizeof(float4));
+  L.tval = w.add(n_tiles * 8);
+  L.trng = w.add(n_tiles * sizeof(double2));
+  L.tcp = w.add(n_tiles * 4);
+  L.codes = w.add(n_codes);
+  L.efl = w.add(n_codes);
   // The per-lane split and per-chunk limb tables (limb lanes only).
   const uint64_t n_split = limb_ ? n_codes : 0, n_lch = limb_ ? n_chunks : 0;
-  const size_t w_hsp = w.add(n_split), w_sal = w.add(n_split * 8), w_ssm = w.add(n_split * 8),
-               w_gln = w.add(limb_ ? 4 : 0), w_gl = w.add(limb_ ? n_cand * 4 : 0), w_sst = w.add(limb_ ? 32 : 0);
-  const size_t w_crev = w.add(n_lch), w_csh = w.add(n_lch * 4), w_cnq = w.add(n_lch * MAX_ITEM_PATCHES * 4),
-               w_clc = w.add(n_lch * sizeof(LimbCounts)), w_clp = w.add(n_lch * sizeof(LimbCounts)),
-               w_lch = w.add(n_lch * sizeof(LimbChunk));
-  const size_t w_cand = w.add(n_cand * sizeof(ItemDev)), w_cinf = w.add(n_cand * 4),
-               w_cpos = w.add(n_cand * sizeof(ItemCounts));
-  const size_t w_items = w.add(n_cand * sizeof(ItemDev));
-  const size_t w_counts = w.add(sizeof(PackCounts));
-  const size_t w_part = w.add(n_prows * TILE * sizeof(double)), w_ltot = w.add(n_lanes * sizeof(double));
-  const size_t w_flux = w.add(n_times * sizeof(double)), w_out = w.add(ns * sizeof(SetOut));
-  const size_t w_lcomp = w.add(any_iscale ? n_lanes * 5 * sizeof(double) : 0),
-               w_comp = w.add(any_iscale ? n_times * 5 * sizeof(double) : 0),
-               w_lsw = w.add(any_iscale ? n_times * 6 * sizeof(double) : 0);
-  work_bytes_ = w.total;
-  st_.work_bytes = work_bytes_;
-  hip_check(hipMalloc(reinterpret_cast<void**>(&d_work_), work_bytes_), "hipMalloc pack tables");
+  L.hsp = w.add(n_split);
+  L.sal = w.add(n_split * 8);
+  L.ssm = w.add(n_split * 8);
+  L.gln = w.add(limb_ ? 4 : 0);
+  L.gl = w.add(limb_ ? n_cand * 4 : 0);
+  L.sst = w.add(limb_ ? 32 : 0);
+  L.crev = w.add(n_lch);
+  L.csh = w.add(n_lch * 4);
+  L.cnq = w.add(n_lch * MAX_ITEM_PATCHES * 4);
+  L.clc = w.add(n_lch * sizeof(LimbCounts));
+  L.clp = w.add(n_lch * sizeof(LimbCounts));
+  L.lch = w.add(n_lch * sizeof(LimbChunk));
+  L.cand = w.add(n_cand * sizeof(ItemDev));
+  L.cinf = w.add(n_cand * 4);
+  L.cpos = w.add(n_cand * sizeof(ItemCounts));
+  L.items = w.add(n_cand * sizeof(ItemDev));
+  L.counts = w.add(sizeof(PackCounts));
+  L.part = w.add(n_prows * TILE * sizeof(double));
+  L.ltot = w.add(n_lanes * sizeof(double));
+  L.flux = w.add(n_times * sizeof(double));
+  L.out = w.add(ns * sizeof(SetOut));
+  L.iscale = any_iscale;
+  L.lcomp = w.add(any_iscale ? n_lanes * 5 * sizeof(double) : 0);
+  L.comp = w.add(any_iscale ? n_times * 5 * sizeof(double) : 0);
+  L.lsw = w.add(any_iscale ? n_times * 6 * sizeof(double) : 0);
+  L.total = w.total;
+  st_.work_bytes = L.total;
   sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(n_raw), 16);
   scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand, n_chunks), 16);
-  hip_check(hipMalloc(&sort_temp_, sort_temp_bytes_), "hipMalloc sort storage");
-  hip_check(hipMalloc(&scan_temp_, scan_temp_bytes_), "hipMalloc scan storage");
-  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_counts_), sizeof(PackCounts), hipHostMallocDefault),
-            "hipHostMalloc counts");
+  n_cand_ = n_cand;
   auto din = [&](size_t off) { return d_in_ + off; };
-  auto dw = [&](size_t off) { return d_work_ + off; };
 
-  PackDev& p = pd_;
+  // The inputs and sizes every slot shares.
+  PackDev& p = pd0_;
   p.raw_df = reinterpret_cast<const float4*>(din(o_df));
   p.raw_pos = reinterpret_cast<const float4*>(din(o_pos));
   p.raw_eoff = reinterpret_cast<const uint32_t*>(din(o_eoff));
@@ -549,7 +571,6 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.ktiles = reinterpret_cast<const KeyTile*>(din(o_kt));
   p.n_ktiles = static_cast<uint32_t>(n_ktiles);
   p.psegs = reinterpret_cast<const PackSeg*>(din(o_psegs));
-  p.segs = reinterpret_cast<SegDev*>(din(o_segs));
   p.n_segs = static_cast<uint32_t>(n_segs);
   p.sets = reinterpret_cast<const PackSet*>(din(o_psets));
   p.n_sets = static_cast<uint32_t>(ns);
@@ -569,53 +590,15 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.lane_meta = reinterpret_cast<const uint32_t*>(din(o_lm));
   p.setdev = reinterpret_cast<const SetDev*>(din(o_sets));
   p.limb = limb_;
-  p.keys = reinterpret_cast<uint32_t*>(dw(w_keys));
-  p.vals = reinterpret_cast<uint32_t*>(dw(w_vals));
-  p.keys_s = reinterpret_cast<uint32_t*>(dw(w_keys_s));
-  p.vals_s = reinterpret_cast<uint32_t*>(dw(w_vals_s));
-  p.recs = dw(w_recs);
-  p.cones = reinterpret_cast<Cone*>(dw(w_cones));
-  p.moms = reinterpret_cast<double*>(dw(w_moms));
-  p.wmax = reinterpret_cast<uint32_t*>(dw(w_wmax));
-  p.tile_en = reinterpret_cast<float4*>(dw(w_ten));
-  p.tile_valid = reinterpret_cast<uint64_t*>(dw(w_tval));
-  p.tile_range = reinterpret_cast<double2*>(dw(w_trng));
-  p.tile_cp = reinterpret_cast<uint32_t*>(dw(w_tcp));
-  p.codes = reinterpret_cast<uint8_t*>(dw(w_codes));
-  p.eflags = reinterpret_cast<uint8_t*>(dw(w_efl));
-  if (limb_) {
-    p.has_split = reinterpret_cast<uint8_t*>(dw(w_hsp));
-    p.split_all = reinterpret_cast<uint64_t*>(dw(w_sal));
-    p.split_some = reinterpret_cast<uint64_t*>(dw(w_ssm));
-    p.glist_n = reinterpret_cast<uint32_t*>(dw(w_gln));
-    p.glist = reinterpret_cast<uint32_t*>(dw(w_gl));
-    p.split_stats = reinterpret_cast<unsigned long long*>(dw(w_sst));
-    p.chunk_rev = reinterpret_cast<uint8_t*>(dw(w_crev));
-    p.chunk_shift = reinterpret_cast<int32_t*>(dw(w_csh));
-    p.chunk_nq = reinterpret_cast<uint32_t*>(dw(w_cnq));
-    p.chunk_lcnt = reinterpret_cast<LimbCounts*>(dw(w_clc));
-    p.chunk_lpos = reinterpret_cast<LimbCounts*>(dw(w_clp));
-    p.lchunks = reinterpret_cast<LimbChunk*>(dw(w_lch));
-  }
-  p.cand = reinterpret_cast<ItemDev*>(dw(w_cand));
-  p.cand_info = reinterpret_cast<uint32_t*>(dw(w_cinf));
-  p.cand_pos = reinterpret_cast<ItemCounts*>(dw(w_cpos));
-  p.items = reinterpret_cast<ItemDev*>(dw(w_items));
-  p.counts = reinterpret_cast<PackCounts*>(dw(w_counts));
-  p.partials = reinterpret_cast<double*>(dw(w_part));
 
-  DeviceBatch& d = d_;
+  DeviceBatch& d = d0_;
   d.n_sets = static_cast<uint32_t>(ns);
   d.max_set_lanes = max_lanes;
   d.sets = reinterpret_cast<const SetDev*>(din(o_sets));
-  d.segs = p.segs;
-  d.items = p.items;
-  d.recs = p.recs;
   d.lane_phi = p.lane_phi;
   d.lane_e = p.lane_e;
   d.lane_meta = p.lane_meta;
   d.chunk_info = reinterpret_cast<const uint32_t*>(din(o_cinfo));
-  d.moms = p.moms;
   d.t_lane0 = reinterpret_cast<const uint32_t*>(din(o_tl0));
   d.t_ndiv = reinterpret_cast<const int*>(din(o_tnd));
   d.t_slfac = reinterpret_cast<const double*>(din(o_tsl));
@@ -623,45 +606,149 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   d.t_ferr = reinterpret_cast<const double*>(din(o_tfe));
   d.t_w = reinterpret_cast<const double*>(din(o_tw));
   d.t_phase = reinterpret_cast<const double*>(din(o_tph));
-  if (any_iscale) {
-    d.lane_comp = reinterpret_cast<double*>(dw(w_lcomp));
-    d.comp = reinterpret_cast<double*>(dw(w_comp));
-    d.lsw = reinterpret_cast<double*>(dw(w_lsw));
-  }
-  d.partials = p.partials;
-  d.lane_tot = reinterpret_cast<double*>(dw(w_ltot));
-  d.flux = reinterpret_cast<double*>(dw(w_flux));
-  d.out = reinterpret_cast<SetOut*>(dw(w_out));
   d.n_limb_chunks = 0;
-  d.limb_chunks = p.lchunks;
-  // The second phase's tables, sized for one moment entry per candidate to
-  // begin with (the most a pack without limb lanes writes).
-  PackCounts c0{};
-  c0.items.moms = static_cast<uint32_t>(n_cand);
-  ensure_capacity(c0);
-
-  hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
-  hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
-  hip_check(hipEventCreate(&ev_[2]), "hipEventCreate");
-  hip_check(hipEventCreate(&ev_[3]), "hipEventCreate");
-  hip_check(hipStreamCreateWithFlags(&side_.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-  hip_check(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
-  hip_check(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+  // The segment table is written by the pack (its flags): each slot keeps
+  // its own copy of the uploaded one.
+  segs_off_ = o_segs;
+  segs_bytes_ = n_segs * sizeof(SegDev);
+  add_slot();
   st_.concat_s = now_s() - t0;
+}
+
+void DevicePackBatch::add_slot() {
+  auto sp = std::make_unique<Slot>();
+  Slot& s = *sp;
+  try {
+    hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreateWithFlags");
+    for (hipEvent_t& e : s.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipStreamCreateWithFlags(&s.side.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&s.side.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipEventCreateWithFlags(&s.side.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hip_check(hipMalloc(reinterpret_cast<void**>(&s.work), wl_.total + segs_bytes_), "hipMalloc pack tables");
+    hip_check(hipMalloc(&s.sort_temp, sort_temp_bytes_), "hipMalloc sort storage");
+    hip_check(hipMalloc(&s.scan_temp, scan_temp_bytes_), "hipMalloc scan storage");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&s.h_counts), sizeof(PackCounts), hipHostMallocDefault),
+              "hipHostMalloc counts");
+    // The slot's copy of the segment table, after its work tables, from
+    // the staged inputs.
+    SegDev* segs = reinterpret_cast<SegDev*>(s.work + wl_.total);
+    if (segs_bytes_)
+      hip_check(hipMemcpyAsync(segs, h_in_ + segs_off_, segs_bytes_, hipMemcpyHostToDevice, s.stream),
+                "hipMemcpyAsync segments");
+    hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+    const WorkLayout& L = wl_;
+    auto dw = [&](size_t off) { return s.work + off; };
+    s.pd = pd0_;
+    PackDev& p = s.pd;
+    p.segs = segs;
+    p.keys = reinterpret_cast<uint32_t*>(dw(L.keys));
+    p.vals = reinterpret_cast<uint32_t*>(dw(L.vals));
+    p.keys_s = reinterpret_cast<uint32_t*>(dw(L.keys_s));
+    p.vals_s = reinterpret_cast<uint32_t*>(dw(L.vals_s));
+    p.recs = dw(L.recs);
+    p.cones = reinterpret_cast<Cone*>(dw(L.cones));
+    p.moms = reinterpret_cast<double*>(dw(L.moms));
+    p.wmax = reinterpret_cast<uint32_t*>(dw(L.wmax));
+    p.tile_en = reinterpret_cast<float4*>(dw(L.ten));
+    p.tile_valid = reinterpret_cast<uint64_t*>(dw(L.tval));
+    p.tile_range = reinterpret_cast<double2*>(dw(L.trng));
+    p.tile_cp = reinterpret_cast<uint32_t*>(dw(L.tcp));
+    p.codes = reinterpret_cast<uint8_t*>(dw(L.codes));
+    p.eflags = reinterpret_cast<uint8_t*>(dw(L.efl));
+    if (limb_) {
+      p.has_split = reinterpret_cast<uint8_t*>(dw(L.hsp));
+      p.split_all = reinterpret_cast<uint64_t*>(dw(L.sal));
+      p.split_some = reinterpret_cast<uint64_t*>(dw(L.ssm));
+      p.glist_n = reinterpret_cast<uint32_t*>(dw(L.gln));
+      p.glist = reinterpret_cast<uint32_t*>(dw(L.gl));
+      p.split_stats = reinterpret_cast<unsigned long long*>(dw(L.sst));
+      p.chunk_rev = reinterpret_cast<uint8_t*>(dw(L.crev));
+      p.chunk_shift = reinterpret_cast<int32_t*>(dw(L.csh));
+      p.chunk_nq = reinterpret_cast<uint32_t*>(dw(L.cnq));
+      p.chunk_lcnt = reinterpret_cast<LimbCounts*>(dw(L.clc));
+      p.chunk_lpos = reinterpret_cast<LimbCounts*>(dw(L.clp));
+      p.lchunks = reinterpret_cast<LimbChunk*>(dw(L.lch));
+    }
+    p.cand = reinterpret_cast<ItemDev*>(dw(L.cand));
+    p.cand_info = reinterpret_cast<uint32_t*>(dw(L.cinf));
+    p.cand_pos = reinterpret_cast<ItemCounts*>(dw(L.cpos));
+    p.items = reinterpret_cast<ItemDev*>(dw(L.items));
+    p.counts = reinterpret_cast<PackCounts*>(dw(L.counts));
+    p.partials = reinterpret_cast<double*>(dw(L.part));
+    s.d = d0_;
+    DeviceBatch& d = s.d;
+    d.segs = p.segs;
+    d.items = p.items;
+    d.recs = p.recs;
+    d.moms = p.moms;
+    if (L.iscale) {
+      d.lane_comp = reinterpret_cast<double*>(dw(L.lcomp));
+      d.comp = reinterpret_cast<double*>(dw(L.comp));
+      d.lsw = reinterpret_cast<double*>(dw(L.lsw));
+    }
+    d.partials = p.partials;
+    d.lane_tot = reinterpret_cast<double*>(dw(L.ltot));
+    d.flux = reinterpret_cast<double*>(dw(L.flux));
+    d.out = reinterpret_cast<SetOut*>(dw(L.out));
+    d.limb_chunks = p.lchunks;
+    // The second phase's tables, sized for one moment entry per candidate
+    // to begin with (the most a pack without limb lanes writes).
+    PackCounts c0{};
+    c0.items.moms = static_cast<uint32_t>(n_cand_);
+    ensure_capacity(s, c0);
+  } catch (...) {
+    release_slot(s);
+    throw;
+  }
+  slots_.push_back(std::move(sp));
+}
+
+void DevicePackBatch::set_slots(int n) {
+  if (n < 1 || n > 2) throw std::invalid_argument("slots must be 1 or 2");
+  hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
+  sync_all();
+  while (slots_.size() > static_cast<size_t>(n)) {
+    release_slot(*slots_.back());
+    slots_.pop_back();
+  }
+  while (slots_.size() < static_cast<size_t>(n)) add_slot();
+  cur_ = last_ = 0;
+}
+
+void DevicePackBatch::sync_all() {
+  for (auto& s : slots_) {
+    if (s->side.side) hip_check(hipStreamSynchronize(s->side.side), "hipStreamSynchronize");
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  }
+  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+}
+
+DevicePackBatch::Slot& DevicePackBatch::next_slot() {
+  Slot& s = *slots_[cur_];
+  last_ = cur_;
+  cur_ = (cur_ + 1) % slots_.size();
+  return s;
 }
 
 void DevicePackBatch::upload() {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();
-  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  sync_all();
   const double t0 = now_s();
   hip_check(hipMemcpyAsync(d_in_, h_in_, in_bytes_, hipMemcpyHostToDevice, st), "hipMemcpyAsync inputs");
+  // Each slot's copy of the segment table too.
+  for (auto& sp : slots_)
+    if (segs_bytes_)
+      hip_check(hipMemcpyAsync(sp->pd.segs, h_in_ + segs_off_, segs_bytes_, hipMemcpyHostToDevice, st),
+                "hipMemcpyAsync segments");
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   st_.upload_s = now_s() - t0;
 }
 
-void DevicePackBatch::ensure_capacity(const PackCounts& c) {
-  PackDev& p = pd_;
+void DevicePackBatch::ensure_capacity(Slot& sl, const PackCounts& c) {
+  PackDev& p = sl.pd;
+  void** grow_ = sl.grow;
   // A table too small for `need` entries of `elem` bytes is replaced by one
   // with a quarter of headroom, so that packs of similar sets reuse it.
   // Returns the capacity in entries.
@@ -691,31 +778,32 @@ void DevicePackBatch::ensure_capacity(const PackCounts& c) {
     p.cap_llanes = grow(4, c.limb.lanes, p.cap_llanes, sizeof(uint32_t), "hipMalloc limb lanes");
     p.llanes = static_cast<uint32_t*>(grow_[4]);
   }
-  d_.imoms = p.imoms;
-  d_.lane_masks = p.lane_masks;
-  d_.limb_items = p.litems;
-  d_.limb_idx = p.lidx;
-  d_.limb_lanes = p.llanes;
+  sl.d.imoms = p.imoms;
+  sl.d.lane_masks = p.lane_masks;
+  sl.d.limb_items = p.litems;
+  sl.d.limb_idx = p.lidx;
+  sl.d.limb_lanes = p.llanes;
 }
 
-void DevicePackBatch::pack(hipStream_t st) {
-  hip_check(launch_pack_count(pd_, sort_temp_, sort_temp_bytes_, scan_temp_, scan_temp_bytes_, key_bits_, st),
+void DevicePackBatch::pack(Slot& sl) {
+  hipStream_t st = sl.stream;
+  hip_check(launch_pack_count(sl.pd, sl.sort_temp, sort_temp_bytes_, sl.scan_temp, scan_temp_bytes_, key_bits_, st),
             "pack");
-  hip_check(hipMemcpyAsync(h_counts_, pd_.counts, sizeof(PackCounts), hipMemcpyDeviceToHost, st),
+  hip_check(hipMemcpyAsync(sl.h_counts, sl.pd.counts, sizeof(PackCounts), hipMemcpyDeviceToHost, st),
             "hipMemcpyAsync counts");
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  const PackCounts c = *h_counts_;
+  const PackCounts c = *sl.h_counts;
   const uint64_t n = static_cast<uint64_t>(c.items.a) + c.items.b + c.items.c;
   if (n > st_.n_cand || c.limb.chunks > st_.n_chunks || (!limb_ && (c.limb.chunks || c.items.masks)))
     throw DeviceError("pack counts out of range");
-  // Between the phases the stream is idle: buffers may be replaced.
-  ensure_capacity(c);
-  hip_check(launch_pack_fill(pd_, c.items.moms, st), "pack tables");
-  last_counts_ = c;
-  d_.n_items = static_cast<uint32_t>(n);
-  d_.n_items_plain = c.items.a;
-  d_.n_items_ecl = c.items.b;
-  d_.n_limb_chunks = c.limb.chunks;
+  // Between the phases this slot's stream is idle: its tables may be replaced.
+  ensure_capacity(sl, c);
+  hip_check(launch_pack_fill(sl.pd, c.items.moms, st), "pack tables");
+  sl.last_counts = c;
+  sl.d.n_items = static_cast<uint32_t>(n);
+  sl.d.n_items_plain = c.items.a;
+  sl.d.n_items_ecl = c.items.b;
+  sl.d.n_limb_chunks = c.limb.chunks;
   st_.n_items = n;
   st_.n_items_plain = c.items.a;
   st_.n_items_ecl = c.items.b;
@@ -725,17 +813,24 @@ void DevicePackBatch::pack(hipStream_t st) {
 
 void DevicePackBatch::launch(int scale) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
-  hipStream_t st = ctx_.stream();
-  pack(st);
-  hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
-  hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
+  Slot& sl = next_slot();
+  pack(sl);
+  hip_check(launch_flux_items(sl.d, sl.stream, &sl.side), "flux kernel launch");
+  hip_check(launch_assemble(sl.d, scale, sl.stream), "assemble kernel launch");
+  // The context stream waits for this launch: syncing it waits for every
+  // launch so far, and it runs nothing itself.
+  hip_check(hipEventRecord(sl.done, sl.stream), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(ctx_.stream(), sl.done, 0), "hipStreamWaitEvent");
 }
 
 void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, double* wdwarf, double* flux) {
   launch(scale);
-  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  sync_all();
+  const Slot& sl = *slots_[last_];
+  const DeviceBatch& db = sl.d;
+  const PackDev& pd = sl.pd;
   last_out_.resize(sets_.size());
-  hip_check(hipMemcpy(last_out_.data(), d_.out, last_out_.size() * sizeof(SetOut), hipMemcpyDeviceToHost),
+  hip_check(hipMemcpy(last_out_.data(), db.out, last_out_.size() * sizeof(SetOut), hipMemcpyDeviceToHost),
             "hipMemcpy out");
   for (size_t s = 0; s < last_out_.size(); ++s) {
     if (chisq) chisq[s] = last_out_[s].chisq;
@@ -744,12 +839,12 @@ void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, 
     if (wdwarf) wdwarf[s] = last_out_[s].wdwarf;
   }
   if (flux && st_.n_times)
-    hip_check(hipMemcpy(flux, d_.flux, st_.n_times * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
+    hip_check(hipMemcpy(flux, db.flux, st_.n_times * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy flux");
   // The flux pass's work, counted from the items and limb items as the
   // host-packed batch counts it.
-  std::vector<ItemDev> items(d_.n_items);
+  std::vector<ItemDev> items(db.n_items);
   if (!items.empty())
-    hip_check(hipMemcpy(items.data(), pd_.items, items.size() * sizeof(ItemDev), hipMemcpyDeviceToHost),
+    hip_check(hipMemcpy(items.data(), pd.items, items.size() * sizeof(ItemDev), hipMemcpyDeviceToHost),
               "hipMemcpy items");
   uint64_t pe = 0, pm = 0, me = 0, pl = 0;
   for (const ItemDev& it : items) {
@@ -759,16 +854,16 @@ void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, 
   }
   if (limb_) {
     unsigned long long ss[2];
-    hip_check(hipMemcpy(ss, pd_.split_stats, sizeof ss, hipMemcpyDeviceToHost), "hipMemcpy split stats");
+    hip_check(hipMemcpy(ss, pd.split_stats, sizeof ss, hipMemcpyDeviceToHost), "hipMemcpy split stats");
     st_.split_pairs = ss[0];
     st_.split_limb_lanes = ss[1];
   }
   if (limb_ && st_.n_chunks) {
     std::vector<LimbCounts> lc(st_.n_chunks);
     std::vector<uint32_t> nq(st_.n_chunks * MAX_ITEM_PATCHES);
-    hip_check(hipMemcpy(lc.data(), pd_.chunk_lcnt, lc.size() * sizeof(LimbCounts), hipMemcpyDeviceToHost),
+    hip_check(hipMemcpy(lc.data(), pd.chunk_lcnt, lc.size() * sizeof(LimbCounts), hipMemcpyDeviceToHost),
               "hipMemcpy limb counts");
-    hip_check(hipMemcpy(nq.data(), pd_.chunk_nq, nq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+    hip_check(hipMemcpy(nq.data(), pd.chunk_nq, nq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
               "hipMemcpy limb lanes");
     for (size_t c = 0; c < lc.size(); ++c)
       if (lc[c].chunks)
@@ -783,7 +878,7 @@ void DevicePackBatch::run(int scale, double* chisq, double* wnok, double* sfac, 
 void DevicePackBatch::set_obs(const double* flux2d, const double* ferr2d, const double* weight2d) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
   hipStream_t st = ctx_.stream();
-  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  sync_all();
   const size_t n = st_.n_times * sizeof(double);
   const std::pair<const double*, size_t> obs[3] = {{flux2d, o_ty_}, {ferr2d, o_tfe_}, {weight2d, o_tw_}};
   for (const auto& o : obs) {
@@ -797,30 +892,39 @@ void DevicePackBatch::set_obs(const double* flux2d, const double* ferr2d, const 
 
 void DevicePackBatch::time(int scale, int reps, double* ms_pack, double* ms_flux, double* ms_total) {
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
-  hipStream_t st = ctx_.stream();
-  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  sync_all();
   double tp = 0.0, tf = 0.0;
-  hip_check(hipEventRecord(ev_[2], st), "hipEventRecord");
+  // The whole: `reps` launches as launch() makes them (alternating slots),
+  // from the first event to the context stream's join of the last.
+  hipEvent_t t0 = slots_[0]->ev[2], t1 = slots_[0]->ev[3];
+  hip_check(hipEventRecord(t0, ctx_.stream()), "hipEventRecord");
+  for (auto& sp : slots_) hip_check(hipStreamWaitEvent(sp->stream, t0, 0), "hipStreamWaitEvent");
+  for (int r = 0; r < reps; ++r) launch(scale);
+  hip_check(hipEventRecord(t1, ctx_.stream()), "hipEventRecord");
+  hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+  float total = 0.0f;
+  hip_check(hipEventElapsedTime(&total, t0, t1), "hipEventElapsedTime");
+  // The parts, one launch at a time on slot 0: the pack from its first
+  // kernel to its last (the host's wait for the counts between its phases
+  // included) and the flux pass.
+  Slot& sl = *slots_[0];
   for (int r = 0; r < reps; ++r) {
-    // The pack's time runs from its first kernel to its last, the host's
-    // wait for the counts between its phases included.
-    hip_check(hipEventRecord(ev_[0], st), "hipEventRecord");
-    pack(st);
-    hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
-    hip_check(launch_flux_items(d_, st, &side_), "flux kernel launch");
-    hip_check(hipEventRecord(ev_[3], st), "hipEventRecord");
-    hip_check(launch_assemble(d_, scale, st), "assemble kernel launch");
-    hip_check(hipEventSynchronize(ev_[3]), "hipEventSynchronize");
+    hip_check(hipEventRecord(sl.ev[0], sl.stream), "hipEventRecord");
+    pack(sl);
+    hip_check(hipEventRecord(sl.ev[1], sl.stream), "hipEventRecord");
+    hip_check(launch_flux_items(sl.d, sl.stream, &sl.side), "flux kernel launch");
+    hip_check(hipEventRecord(sl.ev[3], sl.stream), "hipEventRecord");
+    hip_check(launch_assemble(sl.d, scale, sl.stream), "assemble kernel launch");
+    hip_check(hipEventSynchronize(sl.ev[3]), "hipEventSynchronize");
     float ms = 0.0f;
-    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[1]), "hipEventElapsedTime");
     tp += ms;
-    hip_check(hipEventElapsedTime(&ms, ev_[1], ev_[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&ms, sl.ev[1], sl.ev[3]), "hipEventElapsedTime");
     tf += ms;
   }
-  hip_check(hipEventRecord(ev_[1], st), "hipEventRecord");
-  hip_check(hipEventSynchronize(ev_[1]), "hipEventSynchronize");
-  float total = 0.0f;
-  hip_check(hipEventElapsedTime(&total, ev_[2], ev_[1]), "hipEventElapsedTime");
+  sync_all();
+  last_ = 0;
+  cur_ = slots_.size() > 1 ? 1 : 0;
   if (ms_pack) *ms_pack = reps > 0 ? tp / reps : 0.0;
   if (ms_flux) *ms_flux = reps > 0 ? tf / reps : 0.0;
   if (ms_total) *ms_total = reps > 0 ? total / reps : 0.0;
@@ -830,11 +934,13 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
   for (int i = 0; i < 10; ++i) diff[i] = 0;
   if (set < 0 || static_cast<size_t>(set) >= sets_.size()) throw std::invalid_argument("set index out of range");
   hip_check(hipSetDevice(ctx_.device()), "hipSetDevice");
-  hipStream_t st = ctx_.stream();
-  // The device tables of a fresh pack.
-  pack(st);
-  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  const PackCounts cnt = last_counts_;
+  sync_all();
+  // The device tables of a fresh pack (on the first slot).
+  Slot& sl = *slots_[0];
+  pack(sl);
+  hip_check(hipStreamSynchronize(sl.stream), "hipStreamSynchronize");
+  const PackCounts cnt = sl.last_counts;
+  const PackDev& pd = sl.pd;
   const uint64_t n_items = static_cast<uint64_t>(cnt.items.a) + cnt.items.b + cnt.items.c;
   const size_t s = static_cast<size_t>(set);
   const lc::Data& data = data_of(s);
@@ -863,16 +969,16 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
   };
   // Cones, moments and codes come from the plain records.
   std::vector<Cone> cones(ps.n_patches);
-  get(cones.data(), pd_.cones + ps.patch0, cones.size() * sizeof(Cone));
+  get(cones.data(), pd.cones + ps.patch0, cones.size() * sizeof(Cone));
   for (size_t i = 0; i < cones.size(); ++i)
     if (std::memcmp(&cones[i], &pack->cones[i], sizeof(Cone)) != 0) ++diff[1];
   std::vector<double> moms(static_cast<size_t>(ps.n_patches) * MOM_STRIDE);
-  get(moms.data(), pd_.moms + static_cast<size_t>(ps.patch0) * MOM_STRIDE, moms.size() * sizeof(double));
+  get(moms.data(), pd.moms + static_cast<size_t>(ps.patch0) * MOM_STRIDE, moms.size() * sizeof(double));
   for (size_t i = 0; i < moms.size(); ++i)
     if (std::memcmp(&moms[i], &pack->moms[i], sizeof(double)) != 0) ++diff[2];
   // The device's codes are tile-major, the host's patch-major.
   std::vector<uint8_t> codes(static_cast<size_t>(ps.n_patches) * ps.n_tiles);
-  get(codes.data(), pd_.codes + ps.code0, codes.size());
+  get(codes.data(), pd.codes + ps.code0, codes.size());
   for (uint32_t t = 0; t < ps.n_tiles; ++t)
     for (uint32_t q = 0; q < ps.n_patches; ++q)
       if (codes[static_cast<size_t>(t) * ps.n_patches + q] != pack->ptcode[static_cast<size_t>(q) * ps.n_tiles + t])
@@ -880,11 +986,11 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
   // Records and segment flags after the linear-law rescaling.
   scale_linear_records(*pack);
   std::vector<char> recs(pack->recs.size());
-  get(recs.data(), pd_.recs + rec0_[s], recs.size());
+  get(recs.data(), pd.recs + rec0_[s], recs.size());
   for (size_t i = 0; i < recs.size(); ++i)
     if (recs[i] != static_cast<char>(pack->recs[i])) ++diff[0];
   std::vector<SegDev> segs(nseg);
-  get(segs.data(), pd_.segs + seg0, segs.size() * sizeof(SegDev));
+  get(segs.data(), pd.segs + seg0, segs.size() * sizeof(SegDev));
   for (uint32_t g = 0; g < nseg; ++g)
     if (segs[g].flags != pack->segs[g].flags) ++diff[4];
   // Work items and limb tables of a host-packed batch of this set alone,
@@ -901,7 +1007,7 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
   Batch B(ctx_, std::move(one), data.size());
   const Batch::Tables T = B.tables();
   std::vector<ItemDev> all(n_items);
-  get(all.data(), pd_.items, all.size() * sizeof(ItemDev));
+  get(all.data(), pd.items, all.size() * sizeof(ItemDev));
   std::vector<ItemDev> mine;
   for (const ItemDev& it : all)
     if (it.seg >= seg0 && it.seg < seg0 + nseg) mine.push_back(it);
@@ -928,14 +1034,14 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
       // Moment entries: the PT_MOM sum, then one per lane-mask group.
       const uint32_t ne = (a.mom ? 1u : 0u) + a.ngrp;
       for (uint32_t k = 0; k < ne; ++k) {
-        get(im.data(), pd_.imoms + (static_cast<size_t>(a.imom) + k) * MOM_STRIDE, MOM_STRIDE * sizeof(double));
+        get(im.data(), pd.imoms + (static_cast<size_t>(a.imom) + k) * MOM_STRIDE, MOM_STRIDE * sizeof(double));
         for (uint32_t c = 0; c < MOM_STRIDE; ++c)
           if (std::memcmp(&im[c], &T.imoms[(static_cast<size_t>(b.imom) + k) * MOM_STRIDE + c], sizeof(double)) != 0)
             ++diff[6];
       }
       if (a.ngrp) {
         gm.resize(a.ngrp);
-        get(gm.data(), pd_.lane_masks + a.lm_off, gm.size() * sizeof(uint64_t));
+        get(gm.data(), pd.lane_masks + a.lm_off, gm.size() * sizeof(uint64_t));
         for (uint32_t k = 0; k < a.ngrp; ++k)
           if (gm[k] != T.lane_masks[static_cast<size_t>(b.lm_off) + k]) ++diff[6];
       }
@@ -943,8 +1049,8 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
   }
   // Limb chunks of this set, in order, with their items, lane indices and
   // lane lists.
-  std::vector<LimbChunk> lch(last_counts_.limb.chunks), lmine;
-  get(lch.data(), pd_.lchunks, lch.size() * sizeof(LimbChunk));
+  std::vector<LimbChunk> lch(cnt.limb.chunks), lmine;
+  get(lch.data(), pd.lchunks, lch.size() * sizeof(LimbChunk));
   for (const LimbChunk& c : lch)
     if (c.seg >= seg0 && c.seg < seg0 + nseg) lmine.push_back(c);
   if (lmine.size() != T.limb_chunks.size()) {
@@ -962,9 +1068,9 @@ void DevicePackBatch::check(int set, uint64_t diff[10]) {
     std::vector<LimbItem> li(a.n_items);
     std::vector<uint16_t> lx(static_cast<size_t>(a.n_items) * TILE);
     std::vector<uint32_t> ll(a.n_lanes);
-    get(li.data(), pd_.litems + a.item0, li.size() * sizeof(LimbItem));
-    get(lx.data(), pd_.lidx + static_cast<size_t>(a.item0) * TILE, lx.size() * sizeof(uint16_t));
-    get(ll.data(), pd_.llanes + a.lane_off, ll.size() * sizeof(uint32_t));
+    get(li.data(), pd.litems + a.item0, li.size() * sizeof(LimbItem));
+    get(lx.data(), pd.lidx + static_cast<size_t>(a.item0) * TILE, lx.size() * sizeof(uint16_t));
+    get(ll.data(), pd.llanes + a.lane_off, ll.size() * sizeof(uint32_t));
     for (uint32_t k = 0; k < a.n_items; ++k) {
       const LimbItem& h = T.limb_items[b.item0 + k];
       if (li[k].e0 != h.e0 || li[k].split != h.split) ++diff[8];

@@ -28,7 +28,7 @@ struct DevicePackStats {
   uint64_t n_tiles = 0, n_chunks = 0, n_codes = 0, n_cand = 0, n_prows = 0, n_lanes = 0, n_times = 0;
   uint64_t rec_bytes = 0;     // element records the pack writes
   uint64_t input_bytes = 0;   // raw grids and layout tables copied by upload()
-  uint64_t work_bytes = 0;    // device tables the pack and the flux pass write
+  uint64_t work_bytes = 0;    // device tables the pack and the flux pass write, per slot
   uint64_t visits = 0;        // element-visits in the reference's loop order per pass
   uint64_t n_items = 0, n_items_plain = 0, n_items_ecl = 0;  // of the last pack
   uint64_t n_limb_chunks = 0, n_limb_items = 0;              // of the last pack
@@ -92,6 +92,11 @@ class DevicePackBatch {
   // counts every entry), [9] the host's limb item count.
   void check(int set, uint64_t diff[10]);
 
+  // Work-table sets that launches alternate between (1 or 2; built with
+  // 1). With 2, the pack of a launch runs beside the flux pass of the one
+  // before, at twice the work memory.
+  void set_slots(int n);
+  int slots() const { return static_cast<int>(slots_.size()); }
   const DevicePackStats& stats() const { return st_; }
   uint32_t n_sets() const { return static_cast<uint32_t>(host_.size()); }
   const SetHost& set(uint32_t i) const { return host_[i]; }
@@ -99,12 +104,48 @@ class DevicePackBatch {
   const std::vector<SetOut>& last_results() const { return last_out_; }
 
  private:
+  // Offsets of the work tables in a slot's work block.
+  struct WorkLayout {
+    size_t keys, vals, keys_s, vals_s, recs, cones, moms, wmax, ten, tval, trng, tcp, codes, efl;
+    size_t hsp, sal, ssm, gln, gl, sst, crev, csh, cnq, clc, clp, lch;
+    size_t cand, cinf, cpos, items, counts, part, ltot, flux, out, lcomp, comp, lsw;
+    size_t total;
+    bool iscale;
+  };
+  // One set of the pack's and the flux pass's work tables with its own
+  // streams: launches alternate between slots, so that the pack of one
+  // launch can run beside the flux pass of the one before.
+  struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each launch; the context stream waits on it
+    char* work = nullptr;
+    void* sort_temp = nullptr;
+    void* scan_temp = nullptr;
+    PackCounts* h_counts = nullptr;  // pinned
+    PackCounts last_counts{};
+    // Tables of the pack's second phase, each its own allocation so it can
+    // grow: item moments, lane-mask groups, limb items, their lane indices,
+    // limb lane lists, the moment entries' first patches and patch masks.
+    void* grow[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    PackDev pd;
+    DeviceBatch d;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    SideStream side;
+  };
+
   void release();
+  void release_slot(Slot& s);
   void build(std::vector<std::unique_ptr<SetLayout>> lay);
+  void add_slot();
   // The pack's first phase, its counts read back, the second phase's tables
-  // grown to them, then the second phase; leaves d_ ready for the flux pass.
-  void pack(hipStream_t st);
-  void ensure_capacity(const PackCounts& c);
+  // grown to them, then the second phase; leaves s.d ready for the flux
+  // pass. All on s.stream.
+  void pack(Slot& s);
+  void ensure_capacity(Slot& s, const PackCounts& c);
+  // The next launch's slot, after its previous launch (in its stream order).
+  Slot& next_slot();
+  // Waits for every slot.
+  void sync_all();
   const lc::Data& data_of(size_t s) const { return data_.size() == 1 ? data_[0] : data_[s]; }
 
   Context& ctx_;
@@ -124,28 +165,22 @@ class DevicePackBatch {
   std::vector<SetDev> sets_;
 
   // One pinned host block and one device block of the same layout hold
-  // every input; a second device block holds the pack's work tables.
+  // every input; each slot's device block holds its work tables.
   char* h_in_ = nullptr;
   char* d_in_ = nullptr;
-  char* d_work_ = nullptr;
-  size_t in_bytes_ = 0, work_bytes_ = 0;
-  void* sort_temp_ = nullptr;
-  size_t sort_temp_bytes_ = 0;
-  void* scan_temp_ = nullptr;
-  size_t scan_temp_bytes_ = 0;
-  uint32_t key_bits_ = 32;          // significant bits of the sort keys
-  PackCounts* h_counts_ = nullptr;  // pinned
-  PackCounts last_counts_{};
-  // Tables of the pack's second phase, each its own allocation so it can
-  // grow: item moments, lane-mask groups, limb items, their lane indices,
-  // limb lane lists, the moment entries' first patches and patch masks.
-  void* grow_[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-
-  PackDev pd_;
-  DeviceBatch d_;
+  size_t in_bytes_ = 0;
+  WorkLayout wl_{};
+  size_t sort_temp_bytes_ = 0, scan_temp_bytes_ = 0;
+  uint32_t key_bits_ = 32;  // significant bits of the sort keys
+  uint64_t n_cand_ = 0;
+  // The input pointers and sizes every slot starts from.
+  PackDev pd0_;
+  DeviceBatch d0_;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  size_t cur_ = 0;   // the slot of the next launch
+  size_t last_ = 0;  // the slot of the last launch
   size_t o_ty_ = 0, o_tfe_ = 0, o_tw_ = 0;  // observation tables in the input block
-  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
-  SideStream side_;
+  size_t segs_off_ = 0, segs_bytes_ = 0;     // the segment table in the input block
 };
 
 }  // namespace lch

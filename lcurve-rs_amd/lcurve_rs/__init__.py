@@ -467,7 +467,7 @@ class DevicePackBatch:
     prepare_model."""
 
     def __init__(self, models, times, expose, ndiv, *, flux=None, ferr=None, weight=None, flux2d=None,
-                 ferr2d=None, weight2d=None, chunk=2048, limb=0):
+                 ferr2d=None, weight2d=None, chunk=2048, limb=0, slots=1):
         L = _native.lib()
         t = _f64(times)
         n = len(t)
@@ -485,6 +485,13 @@ class DevicePackBatch:
                                 _iptr(nd), n, *[None if a is None else _dptr(a) for a in (f2, e2, w2)],
                                 int(chunk), int(limb), ctypes.byref(out)))
         self._h = out
+        if slots != 1:
+            self.set_slots(slots)
+
+    def set_slots(self, slots):
+        """Work-table sets launches alternate between (1 or 2): with 2 the
+        pack of one launch runs beside the flux pass of the one before."""
+        check(_native.lib().lch_dpack_set_slots(_ctx(), self._h, int(slots)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -121,6 +121,7 @@ SIGNATURES = {
                               ctypes.POINTER(VP)]),
     "lch_dpack_free": (None, [VP]),
     "lch_dpack_upload": (INT, [VP, VP]),
+    "lch_dpack_set_slots": (INT, [VP, VP, INT]),
     "lch_dpack_launch": (INT, [VP, VP, INT]),
     "lch_dpack_run": (INT, [VP, VP, INT, D, D, D, D, D]),
     "lch_dpack_time": (INT, [VP, VP, INT, INT, D, D, D]),

@@ -144,6 +144,19 @@ def test_device_pack_many_sets_and_repeat(host_tuning, limb):
     assert np.array_equal(r2["chisq"][ok], r1["chisq"][ok]) and np.array_equal(r2["flux"][ok], r1["flux"][ok])
     ms_pack, ms_flux, ms_total = dp.time(reps=3)
     assert 0.0 < ms_pack and 0.0 < ms_flux and ms_pack + ms_flux <= ms_total
+    # Two work-table sets: consecutive launches alternate between them (the
+    # pack of one beside the flux pass of the other); each gives the same.
+    dp.set_slots(2)
+    for _ in range(3):
+        dp.launch()
+    dp.sync()
+    for _ in range(2):
+        r5 = dp.run()
+        assert np.array_equal(r5["chisq"][ok], r1["chisq"][ok]) and np.array_equal(r5["flux"][ok], r1["flux"][ok])
+    d = dp.check(5)
+    assert all(d[k] == 0 for k in CHECK_KEYS), d
+    ms_pack2, ms_flux2, ms_total2 = dp.time(reps=3)
+    assert 0.0 < ms_pack2 and 0.0 < ms_flux2 and 0.0 < ms_total2
     st = dp.stats()
     assert st["n_sets"] == 48 and st["n_valid"] == 47 and st["n_items"] > 0
     assert st["input_bytes"] > 0 and st["upload_s"] > 0.0
