@@ -3,10 +3,10 @@
 // gemm_kernel — C[M x N] = A[M x K] . B[K x N] on BF16 MFMA
 //   (v_mfma_f32_32x32x16_bf16) with FP32 accumulation. A workgroup of four
 //   wavefronts computes a 64 x 64 tile, each wave a 32 x 32 quarter; K is
-//   staged through two LDS buffers in steps of 64 with both operands stored
-//   K-contiguous there, so every lane's MFMA operand is one 16-byte LDS
-//   read, and the next step's global loads are in flight (in registers)
-//   while the current step's MFMAs run. Operands
+//   staged through two LDS buffers in steps of 64 with both operands
+//   stored K-contiguous there, so every lane's MFMA operand is one 16-byte
+//   LDS read, and the global loads of the next two steps are in flight (in
+//   registers) while the current step's MFMAs run. Operands
 //   may arrive K-contiguous or transposed in global memory (the three GEMMs
 //   of a layer's forward and backward passes need both), and the epilogue is
 //   fused: bias + GELU for hidden layers, bias only for the output layer,
@@ -31,8 +31,11 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 64;
-constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row; the pad breaks bank conflicts
+// K step BK (64 as launched). The LDS row stride is BK + 8 bf16; the pad
+// breaks bank conflicts.
+constexpr int BM = 64, BN = 64, BK_MIN = 64;
+template <int BK>
+constexpr int lds_stride() { return BK + 8; }
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
@@ -51,19 +54,20 @@ __device__ __forceinline__ float gelu_grad(float z) {
 }
 
 // One BM x BK (or BN x BK) operand tile: each of the 256 threads fetches
-// two 16-byte pieces into registers (fetch), later written to LDS as
+// BK / 32 16-byte pieces into registers (fetch), later written to LDS as
 // [row][k] (put). KC: global element (row, k) at g[row * ld + k]; else at
 // g[k * ld + row].
+template <int BK>
 struct Piece {
-  uint4 v[2];
+  uint4 v[BK / 32];
 };
 
-template <bool KC>
-__device__ __forceinline__ Piece fetch(const uint16_t* __restrict__ g, int ld, int row0, int k0) {
+template <bool KC, int BK>
+__device__ __forceinline__ Piece<BK> fetch(const uint16_t* __restrict__ g, int ld, int row0, int k0) {
   const int t = threadIdx.x;
-  Piece p;
+  Piece<BK> p;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < BK / 32; ++h) {
     if (KC) {
       const int r = t >> 2, kk = (t & 3) * 8 + h * 32;
       p.v[h] = *reinterpret_cast<const uint4*>(g + static_cast<size_t>(row0 + r) * ld + k0 + kk);
@@ -75,60 +79,107 @@ __device__ __forceinline__ Piece fetch(const uint16_t* __restrict__ g, int ld, i
   return p;
 }
 
-template <bool KC>
-__device__ __forceinline__ void put(uint16_t* __restrict__ s, const Piece& p) {
+template <bool KC, int BK>
+__device__ __forceinline__ void put(uint16_t* __restrict__ s, const Piece<BK>& p) {
+  constexpr int LS = lds_stride<BK>();
   const int t = threadIdx.x;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < BK / 32; ++h) {
     if (KC) {
       const int r = t >> 2, kk = (t & 3) * 8 + h * 32;
-      *reinterpret_cast<uint4*>(s + r * LDS_STRIDE + kk) = p.v[h];
+      *reinterpret_cast<uint4*>(s + r * LS + kk) = p.v[h];
     } else {
       const int kk = (t >> 3) + h * 32, r = (t & 7) * 8;
       const uint16_t* e = reinterpret_cast<const uint16_t*>(&p.v[h]);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) s[(r + i) * LDS_STRIDE + kk] = e[i];
+      for (int i = 0; i < 8; ++i) s[(r + i) * LS + kk] = e[i];
     }
   }
 }
 
 }  // namespace
 
-template <bool AKC, bool BKC, int EPI>
+// Workgroups are dispatched to the eight XCDs (each with its own L2) in
+// turn: workgroup b runs on XCD b % 8. The tile of workgroup b is numbered
+// so that XCD x takes a contiguous run of the tiles in (slice, row panel,
+// column) order, i.e. whole row panels of A (and whole K slices), read
+// into one L2 instead of eight.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t q = n / 8u, r = n % 8u;
+  const uint32_t x = b % 8u, i = b / 8u;
+  return x * q + (x < r ? x : r) + i;
+}
+
+template <bool AKC, bool BKC, int EPI, int BK>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
+  constexpr int LDS_STRIDE = lds_stride<BK>();
   __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const uint32_t nx = gridDim.x, ny = gridDim.y;
+  const uint32_t tile =
+      xcd_tile((blockIdx.z * ny + blockIdx.y) * nx + blockIdx.x, nx * ny * gridDim.z);
+  const int bx = static_cast<int>(tile % nx), by = static_cast<int>((tile / nx) % ny),
+            bz = static_cast<int>(tile / (nx * ny));
+  const int m0 = by * BM, n0 = bx * BN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  // Split K: slice blockIdx.z covers [kb, kb + K / gridDim.z).
+  // Split K: slice bz covers [kb, kb + K / gridDim.z).
   const int kspan = a.K / static_cast<int>(gridDim.z);
-  const int kb = static_cast<int>(blockIdx.z) * kspan;
+  const int kb = bz * kspan;
   f32x16 acc = {};
-  Piece pa = fetch<AKC>(a.A, a.lda, m0, kb);
-  Piece pb = fetch<BKC>(a.B, a.ldb, n0, kb);
-  put<AKC>(As[0], pa);
-  put<BKC>(Bs[0], pb);
-  __syncthreads();
   const int nk = kspan / BK;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) {
-      pa = fetch<AKC>(a.A, a.lda, m0, kb + (kt + 1) * BK);
-      pb = fetch<BKC>(a.B, a.ldb, n0, kb + (kt + 1) * BK);
-    }
+  // Operand loads run two K steps ahead: two register sets alternate (the
+  // loop is unrolled by two so that no register set is copied while its
+  // loads are in flight); each step's set is written to the LDS buffer the
+  // step before last finished reading.
+  auto mma = [&](int buf) {
 #pragma unroll
     for (int s = 0; s < BK / 16; ++s) {
       const int kk = s * 16 + (l >> 5) * 8;
-      const bf16x8 av = *reinterpret_cast<const bf16x8*>(As[cur] + (wm + (l & 31)) * LDS_STRIDE + kk);
-      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs[cur] + (wn + (l & 31)) * LDS_STRIDE + kk);
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(As[buf] + (wm + (l & 31)) * LDS_STRIDE + kk);
+      const bf16x8 bv = *reinterpret_cast<const bf16x8*>(Bs[buf] + (wn + (l & 31)) * LDS_STRIDE + kk);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
     }
+  };
+  {
+    const Piece<BK> pa = fetch<AKC, BK>(a.A, a.lda, m0, kb);
+    const Piece<BK> pb = fetch<BKC, BK>(a.B, a.ldb, n0, kb);
+    put<AKC, BK>(As[0], pa);
+    put<BKC, BK>(Bs[0], pb);
+  }
+  Piece<BK> a1, b1, a2, b2;  // the sets of steps kt + 1 and kt + 2
+  if (nk > 1) {
+    a1 = fetch<AKC, BK>(a.A, a.lda, m0, kb + BK);
+    b1 = fetch<BKC, BK>(a.B, a.ldb, n0, kb + BK);
+  }
+  if (nk > 2) {
+    a2 = fetch<AKC, BK>(a.A, a.lda, m0, kb + 2 * BK);
+    b2 = fetch<BKC, BK>(a.B, a.ldb, n0, kb + 2 * BK);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += 2) {
+    // Step kt from buffer 0; set 1 holds step kt + 1, set 2 step kt + 2.
+    mma(0);
     if (kt + 1 < nk) {
-      // The other buffer was last read in step kt - 1, before the barrier
-      // that ended it.
-      put<AKC>(As[cur ^ 1], pa);
-      put<BKC>(Bs[cur ^ 1], pb);
+      // Buffer 1 was last read in step kt - 1, before the barrier that ended it.
+      put<AKC, BK>(As[1], a1);
+      put<BKC, BK>(Bs[1], b1);
+    }
+    if (kt + 3 < nk) {
+      a1 = fetch<AKC, BK>(a.A, a.lda, m0, kb + (kt + 3) * BK);
+      b1 = fetch<BKC, BK>(a.B, a.ldb, n0, kb + (kt + 3) * BK);
+    }
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    // Step kt + 1 from buffer 1; set 2 holds step kt + 2, set 1 step kt + 3.
+    mma(1);
+    if (kt + 2 < nk) {
+      put<AKC, BK>(As[0], a2);
+      put<BKC, BK>(Bs[0], b2);
+    }
+    if (kt + 4 < nk) {
+      a2 = fetch<AKC, BK>(a.A, a.lda, m0, kb + (kt + 4) * BK);
+      b2 = fetch<BKC, BK>(a.B, a.ldb, n0, kb + (kt + 4) * BK);
     }
     __syncthreads();
   }
@@ -136,7 +187,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs a) {
   // 32 x 32 quarter in acc[r].
   const int n = n0 + wn + (l & 31);
   // Split-K slices write FP32 partial products to consecutive M x ldc planes.
-  const size_t plane = static_cast<size_t>(blockIdx.z) * a.M * a.ldc;
+  const size_t plane = static_cast<size_t>(bz) * a.M * a.ldc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int m = m0 + wm + (r >> 2) * 8 + (l >> 5) * 4 + (r & 3);
@@ -285,13 +336,15 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
 // ------------------------------------------------------------ launchers
 template <bool AKC, bool BKC, int EPI>
 static hipError_t launch_gemm_t(const GemmArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL((gemm_kernel<AKC, BKC, EPI>), dim3(a.N / BN, a.M / BM, a.ksplit), dim3(256), 0, st, a);
+  // K steps of 64: 37 KB of LDS, four workgroups per CU (steps of 128 halve
+  // that and measured slower at every batch).
+  hipLaunchKernelGGL((gemm_kernel<AKC, BKC, EPI, 64>), dim3(a.N / BN, a.M / BM, a.ksplit), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_gemm(const GemmArgs& a, bool a_kc, bool b_kc, int epi, hipStream_t st) {
-  if (a.M % BM || a.N % BN || a.K % BK || a.M <= 0 || a.N <= 0 || a.K <= 0) return hipErrorInvalidValue;
-  if (a.ksplit < 1 || (a.K / BK) % a.ksplit || (a.ksplit > 1 && epi != EPI_PLAIN)) return hipErrorInvalidValue;
+  if (a.M % BM || a.N % BN || a.K % BK_MIN || a.M <= 0 || a.N <= 0 || a.K <= 0) return hipErrorInvalidValue;
+  if (a.ksplit < 1 || (a.K / BK_MIN) % a.ksplit || (a.ksplit > 1 && epi != EPI_PLAIN)) return hipErrorInvalidValue;
   if (a.lda % 8 || a.ldb % 8) return hipErrorInvalidValue;
 #define LCH_GEMM_CASE(AK, BK_, E) \
   if (a_kc == AK && b_kc == BK_ && epi == E) return launch_gemm_t<AK, BK_, E>(a, st);
