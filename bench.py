@@ -428,7 +428,10 @@ def bench_surrogate(args):
     against the exact method; MFMA utilisation of the training GEMMs."""
     from lcurve_rs import mcmc
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    base = mcmc.surrogate_base_text()
+    # The disc model's parameter box: q, iangle, r1, t1, t2 over the
+    # batched-chi-squared ranges, everything else as in disc_spot_300k, at
+    # reduced grids (the CPU setup of every training curve is the cost).
+    base = W.disc_box_text()
     n = args.ncurves
     params = mcmc.sample_box(n, 42)
     log(f"generating {n} curves with the exact GPU method ...")
@@ -438,11 +441,19 @@ def bench_surrogate(args):
     ok = np.all(np.isfinite(curves), axis=1)
     P, C = params[ok], curves[ok]
     nh = max(1024, len(P) // 20)
-    s = mcmc.Surrogate(256, seed=1)
-    log(f"training on {len(P) - nh} curves, {args.epochs} epochs ...")
-    fit = s.fit(P[nh:], C[nh:], epochs=args.epochs, log=log)
-    err = s.predict(P[:nh]) - C[:nh]
-    ae = np.abs(err)
+    # One fit per epoch count on the same curves; the record is the last.
+    fits = []
+    for ep in [int(v) for v in str(args.epochs).split(",")]:
+        s = mcmc.Surrogate(256, seed=1)
+        log(f"training on {len(P) - nh} curves, {ep} epochs ...")
+        fit = s.fit(P[nh:], C[nh:], epochs=ep, log=log)
+        err = s.predict(P[:nh]) - C[:nh]
+        ae = np.abs(err)
+        fits.append({"epochs": ep, "train_s": round(fit["seconds"], 2), "final_train_loss": float(fit["loss"][-1]),
+                     "heldout_rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
+                     "heldout_median_abs": float(np.median(ae)), "heldout_p99_abs": float(np.quantile(ae, 0.99)),
+                     "heldout_max_abs": float(ae.max())})
+    err_poly = s.baseline(P[:nh]) - C[:nh]
     tm = s.net.time_step(4096, reps=50)
     tflops = tm["gemm_flop"] / (tm["ms_gemm"] * 1e-3) / 1e12
     value = 4096.0 / (tm["ms_step"] * 1e-3)
@@ -462,16 +473,19 @@ def bench_surrogate(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f32",
            "data": "synthetic: curves from the exact GPU method over the parameter box",
            "config": {"workload": "surrogate_200k", "curves": n, "bins": 256, "mlp": "5-512x4-256 GELU",
-                      "batch": 4096, "epochs": args.epochs,
+                      "batch": 4096, "epochs": fits[-1]["epochs"],
                       "box": {k: list(v) for k, v in zip(mcmc.PARAM_NAMES, mcmc.BOX.tolist())},
-                      "grid_nlat": [20, 20, 40, 40]},
+                      "model": "disc_spot_300k (disc, rim, bright spot, irradiation, Claret) at reduced grids",
+                      "grids": W.DISC_BOX_GRIDS},
            "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None,
                         "kernel": "gemm_kernel", "gemm_ms_per_step": round(tm["ms_gemm"], 4),
                         "gemm_flop_per_step": tm["gemm_flop"], "larger_batches": scaling},
            "heldout_vs_exact": {"curves": int(nh), "rms_log_flux": float(np.sqrt(np.mean(err ** 2))),
                                 "median_abs": float(np.median(ae)), "p99_abs": float(np.percentile(ae, 99)),
-                                "max_abs": float(ae.max())},
+                                "max_abs": float(ae.max()),
+                                "quadratic_baseline_alone_rms": float(np.sqrt(np.mean(err_poly ** 2)))},
+           "epoch_sweep": fits,
            "data_generation_s": round(t_gen, 1), "train_s": round(fit["seconds"], 2),
            "final_train_loss": fit["loss"][-1], "failed_setups": int((~ok).sum())}
     if not args.no_cpu_baseline:
@@ -509,12 +523,51 @@ def bench_da_mcmc(args):
     truth = np.array([0.5, 82.0, 0.015, 15000.0, 3500.0])
     times = np.linspace(-0.2, 1.2, 3000)
     tg = mcmc.Target(base, truth, times, args.noise, seed=42, expose=DA_EXPOSE, ndiv=DA_NDIV)
-    rng = np.random.default_rng(1)
     nw = args.walkers
-    p0 = truth * (1.0 + 1e-4 * rng.standard_normal((nw, 5)))
-    log(f"pilot: {nw} walkers x {args.pilot_steps} exact steps ...")
-    pilot = mcmc.ensemble_mcmc(tg, p0, args.pilot_steps, seed=1, log=log)
-    tail = pilot["chain"][args.pilot_steps // 2:].reshape(-1, 5)
+    # A run can be split over several processes: --mcmc-save writes the
+    # pilot and the plain chain so far, --mcmc-load resumes from them (the
+    # plain chain continues from its last walkers with a fresh seed, which is
+    # the same Markov chain).
+    if args.mcmc_load:
+        st = np.load(args.mcmc_load)
+        if st["pilot_tail"].shape[-1] != 5 or st["start"].shape[0] != nw:
+            raise SystemExit(f"{args.mcmc_load} holds another walker count")
+        tail, start = st["pilot_tail"], st["start"]
+        pilot = {"seconds": float(st["pilot_seconds"]), "n_exact": int(st["pilot_n_exact"])}
+        pchain = st["plain_chain"]
+        pinfo = {k: float(st["plain_" + k]) for k in ("seconds", "n_exact", "accepted", "proposals")}
+        log(f"resumed: pilot {tail.shape[0]} states, plain chain {pchain.shape[0]} steps")
+    else:
+        rng = np.random.default_rng(1)
+        p0 = truth * (1.0 + 1e-4 * rng.standard_normal((nw, 5)))
+        log(f"pilot: {nw} walkers x {args.pilot_steps} exact steps ...")
+        pr = mcmc.ensemble_mcmc(tg, p0, args.pilot_steps, seed=1, log=log)
+        tail = pr["chain"][args.pilot_steps // 2:].reshape(-1, 5)
+        start = pr["chain"][-1].copy()
+        pilot = {"seconds": pr["seconds"], "n_exact": pr["n_exact"]}
+        pchain = np.zeros((0, nw, 5))
+        pinfo = {"seconds": 0.0, "n_exact": 0.0, "accepted": 0.0, "proposals": 0.0}
+    plain_steps = args.steps if args.plain_steps < 0 else args.plain_steps
+    if plain_steps:
+        x0 = pchain[-1] if len(pchain) else start
+        log(f"plain exact MCMC: {plain_steps} steps (after {len(pchain)}) ...")
+        r = mcmc.ensemble_mcmc(tg, x0, plain_steps, seed=2 + len(pchain), log=log)
+        pchain = np.concatenate([pchain, r["chain"]])
+        for k in ("seconds", "n_exact", "accepted", "proposals"):
+            pinfo[k] += float(r[k])
+    plain = dict(pinfo, chain=pchain)
+    if args.mcmc_save:
+        np.savez(args.mcmc_save, pilot_tail=tail, start=start, pilot_seconds=pilot["seconds"],
+                 pilot_n_exact=pilot["n_exact"], plain_chain=pchain,
+                 **{"plain_" + k: v for k, v in pinfo.items()})
+        log(f"state saved to {args.mcmc_save}")
+    if args.skip_da:
+        e, t = mcmc.ess(pchain, burn=0.1)
+        print(json.dumps({"da_mcmc_part": "plain", "walkers": nw, "plain_steps": int(len(pchain)),
+                          "ess": round(float(e), 1), "tau": t.round(2).tolist(),
+                          "seconds": round(pinfo["seconds"], 1), "ess_per_s": round(e / pinfo["seconds"], 3),
+                          "pilot_seconds": round(pilot["seconds"], 1)}), flush=True)
+        return
     frame = mcmc.Frame.local(tail, args.radius)
     ncurves = args.ncurves
     log(f"training set: {ncurves} curves over the pilot posterior ball of radius {args.radius} sigma ...")
@@ -530,9 +583,6 @@ def bench_da_mcmc(args):
     fit = s.fit(P[nh:], C[nh:], epochs=args.epochs, log=log)
     err = s.predict(P[:nh]) - C[:nh]
     err_poly = s.baseline(P[:nh]) - C[:nh]
-    start = pilot["chain"][-1].copy()
-    log(f"plain exact MCMC: {args.steps} steps ...")
-    plain = mcmc.ensemble_mcmc(tg, start, args.steps, seed=2, log=log)
     inners = [int(v) for v in str(args.inner).split(",")]
     sweep = []
     for inner in inners:
@@ -560,7 +610,7 @@ def bench_da_mcmc(args):
     u_plain = np.sqrt(np.max(np.sum(frame.to_u(plain["chain"][len(plain["chain"]) // 10:].reshape(-1, 5)) ** 2, 1)))
     out_da = float(np.mean(~frame.contains(da["chain"][len(da["chain"]) // 10:].reshape(-1, 5))))
     rec = {"metric": "MCMC effective samples/s (1 GPU), delayed acceptance", "value": round(value, 3),
-           "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.pilot_steps,
+           "unit": "ESS/s", "n_gpus": 1, "steps": args.steps, "warmup": int(len(tail) * 2 // nw),
            "ms_per_step": round(1e3 * da["seconds"] / args.steps, 2), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32/f64 exact, bf16 surrogate",
            "data": f"synthetic target curve (3000 times, {args.noise:g} relative noise) of the one-binary geometry",
@@ -571,7 +621,8 @@ def bench_da_mcmc(args):
                       "frame": f"ball of {args.radius} pilot sigma, whitened; Gaussian tail outside",
                       "baseline_degree": args.degree, "outputs": "the 3000 data times (smeared)"},
            "plain": {"ess_per_s": round(base_v, 3), "ess": round(float(ess_p), 1), "tau": tau_p.round(2).tolist(),
-                     "seconds": round(plain["seconds"], 1), "n_exact": plain["n_exact"],
+                     "steps": int(len(plain["chain"])),
+                     "seconds": round(plain["seconds"], 1), "n_exact": int(plain["n_exact"]),
                      "acceptance": round(plain["accepted"] / plain["proposals"], 3),
                      "max_radius_in_frame": round(float(u_plain), 3)},
            "delayed_acceptance": {"ess_per_s": round(value, 3), "ess": round(float(ess_d), 1),
@@ -621,7 +672,8 @@ def main():
                     choices=["chisq_4096", "wd_donor_45k", "disc_spot_300k", "surrogate_200k", "da_mcmc"])
     ap.add_argument("--ncurves", type=int, default=None,
                     help="training curves (surrogate_200k: 200000; da_mcmc: 16000)")
-    ap.add_argument("--epochs", type=int, default=None, help="surrogate training epochs")
+    ap.add_argument("--epochs", default=None,
+                    help="surrogate training epochs (surrogate_200k: a comma-separated list, one fit each)")
     ap.add_argument("--walkers", type=int, default=64)
     ap.add_argument("--pilot-steps", type=int, default=200)
     ap.add_argument("--inner", default="100,50,25",
@@ -631,6 +683,11 @@ def main():
                     help="da_mcmc surrogate domain: pilot posterior standard deviations (whitened ball)")
     ap.add_argument("--degree", type=int, default=3, help="da_mcmc surrogate polynomial baseline degree")
     ap.add_argument("--noise", type=float, default=0.005, help="relative noise of the da_mcmc target curve")
+    ap.add_argument("--plain-steps", type=int, default=-1,
+                    help="da_mcmc: plain exact steps in this process (default --steps)")
+    ap.add_argument("--skip-da", action="store_true", help="da_mcmc: plain chain only (with --mcmc-save)")
+    ap.add_argument("--mcmc-save", default=None, help="da_mcmc: write the pilot and plain chain here (.npz)")
+    ap.add_argument("--mcmc-load", default=None, help="da_mcmc: resume from a --mcmc-save file")
     ap.add_argument("--nsets", type=int, default=4096, help="sets of chisq_4096 (4096 is the workload)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--chunk", type=int, default=0,
@@ -664,12 +721,12 @@ def main():
     lc.set_device("hip:0")
     if args.workload == "surrogate_200k":
         args.ncurves = args.ncurves or 200000
-        args.epochs = args.epochs or 60
+        args.epochs = args.epochs or "60,600"
         bench_surrogate(args)
         return
     if args.workload == "da_mcmc":
         args.ncurves = args.ncurves or 16000
-        args.epochs = args.epochs or 200
+        args.epochs = int(args.epochs or 200)
         if args.steps == 10:
             args.steps = 400
         bench_da_mcmc(args)

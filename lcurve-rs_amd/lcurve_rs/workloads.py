@@ -193,6 +193,24 @@ def disc_spot(scale=1.0, ntime=5000):
     return model_text(ov), linspace_times(-0.5, 1.5, ntime), 0.005, 5
 
 
+# Grids of the surrogate's training set over the disc model's parameter box:
+# 200,000 CPU setups at these resolutions take ~0.05 core-s each (the disc's
+# radial count dominates the cost; at C2's full grids a set takes seconds).
+DISC_BOX_GRIDS = {"nlat1f": 8, "nlat1c": 8, "nlat2f": 40, "nlat2c": 40, "nrad": 16}
+
+
+def disc_box_text(grids=None):
+    """The disc_spot_300k model (white dwarf, Roche-lobe-filling donor,
+    opaque flared disc with its rim, bright spot, irradiation, Claret limb
+    darkening on both stars) at reduced grids, with ephemeris t0 = 0 and
+    period = 1 so that time is phase: the model whose parameter box (q,
+    iangle, r1, t1, t2) the surrogate is trained over."""
+    ov = dict(DISC_SPOT_OVERRIDES)
+    ov.update(DISC_BOX_GRIDS if grids is None else grids)
+    ov.update({"t0": 0.0, "period": 1.0, "pdot": 0.0, "deltat": 0.0})
+    return model_text(ov)
+
+
 def _splitmix64(state):
     """splitmix64 stream (Steele, Lea, Flood 2014): portable seeded uniforms."""
     mask = (1 << 64) - 1
