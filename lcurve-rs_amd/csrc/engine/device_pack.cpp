@@ -243,7 +243,9 @@ void DevicePackBatch::release_slot(Slot& s) {
 
 void DevicePackBatch::release() {
   (void)hipSetDevice(ctx_.device());
-  (void)hipStreamSynchronize(ctx_.stream());
+  // A device-wide wait before the frees: after a stream wait alone the
+  // runtime can keep freed memory allocated (Batch::release_all).
+  (void)hipDeviceSynchronize();
   for (auto& s : slots_) release_slot(*s);
   slots_.clear();
   if (h_in_) (void)hipHostFree(h_in_);
@@ -755,7 +757,10 @@ void DevicePackBatch::ensure_capacity(Slot& sl, const PackCounts& c) {
   auto grow = [&](int slot, uint64_t need, uint64_t have, size_t elem, const char* what) -> uint64_t {
     if (need <= have && grow_[slot]) return have;
     const uint64_t cap = std::max<uint64_t>(need + need / 4, 1024);
-    if (grow_[slot]) hip_check(hipFree(grow_[slot]), "hipFree pack table");
+    if (grow_[slot]) {
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");  // see DevicePackBatch::release
+      hip_check(hipFree(grow_[slot]), "hipFree pack table");
+    }
     grow_[slot] = nullptr;
     hip_check(hipMalloc(&grow_[slot], static_cast<size_t>(cap) * elem), what);
     return cap;

@@ -227,6 +227,12 @@ Batch::~Batch() { release_all(); }
 
 void Batch::release_all() {
   (void)hipSetDevice(ctx_.device());
+  // Measured on ROCm 7.2: device memory freed after a graph launch with only
+  // the launching stream synchronised stays allocated for the rest of the
+  // process (about 230 MB of a 128-set batch's 270 MB, every batch); after a
+  // device-wide synchronisation it is released. A batch in given memory
+  // frees nothing and skips the wait.
+  if (!mem_.arena) (void)hipDeviceSynchronize();
   release_graph();
   for (hipEvent_t& e : ev_) {
     if (e) (void)hipEventDestroy(e);
@@ -314,8 +320,9 @@ void Batch::build_items(uint32_t chunk) {
   chunk = std::min<uint32_t>(chunk, MAX_ITEM_PATCHES * PATCH);
   if (mem_.arena && items_built_) throw DeviceError("the work items of a batch in given memory are built once");
   items_built_ = true;
-  // Launches in flight still read the tables replaced below.
-  hip_check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  // Launches in flight still read the tables replaced below; the device-wide
+  // wait is the one that lets the runtime release them (Batch::release_all).
+  hip_check(mem_.arena ? hipStreamSynchronize(ctx_.stream()) : hipDeviceSynchronize(), "synchronize");
   release_graph();
   chunk_ = chunk;
   chunk_info_.clear();

@@ -150,6 +150,7 @@ GridModel::~GridModel() {
   cache_.reset();
   if (d_recs_) {
     (void)hipSetDevice(ctx_.device());
+    (void)hipDeviceSynchronize();  // before the free, as in Batch::release_all
     (void)hipFree(d_recs_);
   }
 }

@@ -135,7 +135,9 @@ Mlp::~Mlp() { release(); }
 
 void Mlp::release() {
   (void)hipSetDevice(ctx_.device());
-  (void)hipStreamSynchronize(ctx_.stream());
+  // A device-wide wait before the frees: after a stream wait alone the
+  // runtime can keep freed memory allocated (lch::Batch::release_all).
+  (void)hipDeviceSynchronize();
   for (hipEvent_t& e : ev_) {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
@@ -187,7 +189,7 @@ void Mlp::set_params(const float* in) {
 
 void Mlp::set_data(const float* x, const float* y, int n) {
   if (n <= 0) throw std::invalid_argument("empty training set");
-  check(hipStreamSynchronize(ctx_.stream()), "hipStreamSynchronize");
+  check(hipDeviceSynchronize(), "hipDeviceSynchronize");  // before the frees, as in release()
   if (X_) (void)hipFree(X_);
   if (Y_) (void)hipFree(Y_);
   X_ = Y_ = nullptr;

@@ -101,7 +101,9 @@ SurrogateLik::~SurrogateLik() { release(); }
 void SurrogateLik::release() {
   Context& ctx = net_.context();
   (void)hipSetDevice(ctx.device());
-  (void)hipStreamSynchronize(ctx.stream());
+  // A device-wide wait before the frees: after a stream wait alone the
+  // runtime can keep freed memory allocated (lch::Batch::release_all).
+  (void)hipDeviceSynchronize();
   for (void* p : {static_cast<void*>(coef_), static_cast<void*>(y_), static_cast<void*>(ferr_),
                   static_cast<void*>(w_), static_cast<void*>(curve_), static_cast<void*>(out_),
                   static_cast<void*>(stage_)})
