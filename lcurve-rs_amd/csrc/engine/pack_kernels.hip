@@ -201,55 +201,110 @@ __device__ inline uint32_t wave_or(uint32_t v) {
   return v;
 }
 
+// Patches per wavefront of the patch kernel (consecutive patch numbers).
+constexpr uint32_t PATCH_RUN = 32;
+
+// Lane x of a wavefront's value at lane x + 32, for lanes below 32 (lanes
+// 32-63 get their own value back): v_permlane32_swap on each 32-bit half.
+__device__ inline double upper_half(double v) {
+  const uint32_t lo = __double2loint(v), hi = __double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(static_cast<int>(h[1]), static_cast<int>(l[1]));
+}
+
 // Cone and (for moment-capable segments) direction moments of each patch,
-// from the plain values: one wavefront per patch, a lane per slot. Each sum
-// is tree_sum64's: its first level (lane i + lane i + 32) by a shuffle, the
-// rest by one lane per sum over the 32 partial sums in LDS.
+// from the plain values: a wavefront per run of PATCH_RUN patches, a lane
+// per slot. Each sum is tree_sum64's: its first level (lane i + lane i +
+// 32) by a lane swap, the rest by one lane per sum over the 32 partial sums
+// in LDS. The gather is pipelined: while a patch is summed, the plain
+// values of the next one and the sorted positions of the one after it are
+// in flight. The segment's fields are re-read only when a run crosses into
+// the next segment.
 __global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
   constexpr uint32_t NC = MOM_STRIDE + 3;  // cone sums x, y, z, then the moments
   // Row stride of 23 doubles: conflict-free row writes and column reads.
   __shared__ double part[PATCH / 2][NC];
-  const uint32_t p = blockIdx.x;
   const uint32_t j = threadIdx.x;
-  const PackSeg S = d.psegs[d.patch_seg[p]];
-  const uint32_t q = p - S.patch0;
-  const uint32_t first = q * PATCH;
-  const uint32_t nr = S.n_real > first ? (S.n_real - first < PATCH ? S.n_real - first : PATCH) : 0u;
-  const bool real = j < nr;
-  const float4 f = plain_slot(d, S, q, j);
-  const bool mom = (S.flags & PSEG_MOM) != 0;
-  auto level1 = [&](double v, uint32_t c) {
-    const double o = __shfl_xor(v, 32);
-    if (j < PATCH / 2) part[j][c] = v + o;
+  const uint32_t p0 = blockIdx.x * PATCH_RUN;
+  if (p0 >= d.n_patches) return;
+  const uint32_t pend = d.n_patches - p0 < PATCH_RUN ? d.n_patches : p0 + PATCH_RUN;
+  // The segment of the patch whose positions are read next.
+  uint32_t s_patch0 = 0, s_end = 0, s_nreal = 0, s_sorted0 = 0, s_mom = 0;
+  // Real slots, moment flag and this lane's raw element of a patch.
+  struct Ahead {
+    uint32_t nr, mom, e;
   };
-  level1(real ? static_cast<double>(f.x) : 0.0, 0);
-  level1(real ? static_cast<double>(f.y) : 0.0, 1);
-  level1(real ? static_cast<double>(f.z) : 0.0, 2);
-  if (mom) {
-    double T[MOM_STRIDE];
-    moment_terms(T, f.x, f.y, f.z, f.w);
+  auto positions = [&](uint32_t pp) {
+    if (pp >= s_end) {
+      const PackSeg& S = d.psegs[d.patch_seg[pp]];
+      s_patch0 = S.patch0;
+      s_end = S.patch0 + S.n_pad / PATCH;
+      s_nreal = S.n_real;
+      s_sorted0 = S.sorted0;
+      s_mom = (S.flags & PSEG_MOM) != 0u;
+    }
+    const uint32_t first = (pp - s_patch0) * PATCH;
+    Ahead a;
+    a.nr = s_nreal > first ? (s_nreal - first < PATCH ? s_nreal - first : PATCH) : 0u;
+    a.mom = s_mom;
+    a.e = j < a.nr ? d.vals_s[s_sorted0 + first + j] : 0xFFFFFFFFu;
+    return a;
+  };
+  auto plain = [&](const Ahead& a) {
+    return a.e != 0xFFFFFFFFu ? d.raw_df[a.e] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  };
+  Ahead cur = positions(p0);
+  float4 f = plain(cur);
+  Ahead nxt = cur;
+  if (p0 + 1 < pend) nxt = positions(p0 + 1);
+  for (uint32_t p = p0; p < pend; ++p) {
+    Ahead after = nxt;
+    float4 fn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (p + 1 < pend) fn = plain(nxt);
+    if (p + 2 < pend) after = positions(p + 2);
+
+    const uint32_t nr = cur.nr;
+    const bool real = j < nr;
+    const bool mom = cur.mom != 0u;
+    auto level1 = [&](double v, uint32_t c) {
+      const double o = upper_half(v);
+      if (j < PATCH / 2) part[j][c] = v + o;
+    };
+    level1(real ? static_cast<double>(f.x) : 0.0, 0);
+    level1(real ? static_cast<double>(f.y) : 0.0, 1);
+    level1(real ? static_cast<double>(f.z) : 0.0, 2);
+    if (mom) {
+      double T[MOM_STRIDE];
+      moment_terms(T, f.x, f.y, f.z, f.w);
 #pragma unroll
-    for (uint32_t c = 0; c < MOM_STRIDE; ++c) level1(T[c], 3 + c);
+      for (uint32_t c = 0; c < MOM_STRIDE; ++c) level1(T[c], 3 + c);
+    }
+    __syncthreads();
+    double sum = 0.0;
+    if (j < (mom ? NC : 3u)) {
+      double a[PATCH / 2];
+#pragma unroll
+      for (uint32_t i = 0; i < PATCH / 2; ++i) a[i] = part[i][j];
+#pragma unroll
+      for (uint32_t off = PATCH / 4; off > 0; off >>= 1)
+#pragma unroll
+        for (uint32_t i = 0; i < off; ++i) a[i] = a[i] + a[i + off];
+      sum = a[0];
+    }
+    __syncthreads();  // the partial sums are read before the next patch writes them
+    ConeBuilder cb;
+    cb.set_sum(__shfl(sum, 0), __shfl(sum, 1), __shfl(sum, 2), nr > 0);
+    cb.axis();
+    cb.mn = wave_min(real ? cb.cosine(f.x, f.y, f.z) : 1.0);
+    const Cone c = cb.finish();
+    if (j == 0) d.cones[p] = c;
+    if (j >= 3 && j < NC) d.moms[static_cast<uint64_t>(p) * MOM_STRIDE + (j - 3)] = mom ? sum : 0.0;
+
+    cur = nxt;
+    f = fn;
+    nxt = after;
   }
-  __syncthreads();
-  double sum = 0.0;
-  if (j < (mom ? NC : 3u)) {
-    double a[PATCH / 2];
-#pragma unroll
-    for (uint32_t i = 0; i < PATCH / 2; ++i) a[i] = part[i][j];
-#pragma unroll
-    for (uint32_t off = PATCH / 4; off > 0; off >>= 1)
-#pragma unroll
-      for (uint32_t i = 0; i < off; ++i) a[i] = a[i] + a[i + off];
-    sum = a[0];
-  }
-  ConeBuilder cb;
-  cb.set_sum(__shfl(sum, 0), __shfl(sum, 1), __shfl(sum, 2), nr > 0);
-  cb.axis();
-  cb.mn = wave_min(real ? cb.cosine(f.x, f.y, f.z) : 1.0);
-  const Cone c = cb.finish();
-  if (j == 0) d.cones[p] = c;
-  if (j >= 3 && j < NC) d.moms[static_cast<uint64_t>(p) * MOM_STRIDE + (j - 3)] = mom ? sum : 0.0;
 }
 
 // Per lane tile: normalised FP32 earth vectors, valid lanes, the range of
@@ -283,6 +338,16 @@ __global__ __launch_bounds__(TILE) void tiles_kernel(PackDev d) {
   }
 }
 
+// Lane k's value (k wave-uniform), by v_readlane on each 32-bit half.
+__device__ inline uint64_t readlane_u64(uint64_t v, uint32_t k) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), k));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), k));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ inline double readlane_f64(double v, uint32_t k) {
+  return __longlong_as_double(static_cast<long long>(readlane_u64(static_cast<uint64_t>(__double_as_longlong(v)), k)));
+}
+
 // Eclipse interval test of every patch of a segment with eclipse pairs
 // against every tile of its set: one wavefront per patch, a lane per
 // element; zero-weight records count as both clear and eclipsed.
@@ -304,29 +369,33 @@ __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
     e3[j][2] = s[2];
   }
   const uint64_t cbase = set.code0 + static_cast<uint64_t>(p - set.patch0) * set.n_tiles;
-  // Lane t % 64 keeps tile t's flags; each run of 64 tiles is written at once.
-  uint8_t mine = 0;
-  for (uint32_t t = 0; t < set.n_tiles; ++t) {
-    const uint32_t tg = set.tile0 + t;
-    uint8_t f = 0;
-    if (d.tile_valid[tg] != 0ull) {  // else coded skip without the test
-      const double2 rg = d.tile_range[tg];
-      bool vis = true, ecl = false;
-      for (uint32_t j = 0; j < S.necl; ++j) {
-        bool misses, holds;
-        interval_vs_range(e3[j], rg.x, rg.y, &misses, &holds);
-        vis = vis && misses;
-        ecl = ecl || holds;
+  // Per run of 64 tiles: lane k loads tile t0 + k's valid mask and phase
+  // range, the loop reads them with readlane (no scalar load per tile), and
+  // lane k keeps tile t0 + k's flags, written at once.
+  for (uint32_t t0 = 0; t0 < set.n_tiles; t0 += TILE) {
+    const uint32_t nt = set.n_tiles - t0 < TILE ? set.n_tiles - t0 : TILE;
+    const bool in = lane < nt;
+    const uint64_t vl = in ? d.tile_valid[set.tile0 + t0 + lane] : 0ull;
+    const double2 rl = in ? d.tile_range[set.tile0 + t0 + lane] : make_double2(0.0, 0.0);
+    uint8_t mine = 0;
+    for (uint32_t k = 0; k < nt; ++k) {
+      uint8_t f = 0;
+      if (readlane_u64(vl, k) != 0ull) {  // else coded skip without the test
+        const double lo = readlane_f64(rl.x, k), hi = readlane_f64(rl.y, k);
+        bool vis = true, ecl = false;
+        for (uint32_t j = 0; j < S.necl; ++j) {
+          bool misses, holds;
+          interval_vs_range(e3[j], lo, hi, &misses, &holds);
+          vis = vis && misses;
+          ecl = ecl || holds;
+        }
+        if (w == 0.0f) vis = ecl = true;
+        const uint64_t bv = __ballot(vis), be = __ballot(ecl);
+        f = static_cast<uint8_t>((bv == ~0ull ? 1u : 0u) | (be == ~0ull ? 2u : 0u));
       }
-      if (w == 0.0f) vis = ecl = true;
-      const uint64_t bv = __ballot(vis), be = __ballot(ecl);
-      f = static_cast<uint8_t>((bv == ~0ull ? 1u : 0u) | (be == ~0ull ? 2u : 0u));
+      if (lane == k) mine = f;
     }
-    if (lane == t % TILE) mine = f;
-    if (t % TILE == TILE - 1 || t + 1 == set.n_tiles) {
-      const uint32_t t0 = t - t % TILE;
-      if (lane <= t - t0) d.eflags[cbase + t0 + lane] = mine;
-    }
+    if (in) d.eflags[cbase + t0 + lane] = mine;
   }
 }
 
@@ -891,7 +960,7 @@ hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp
   if (d.n_segs) hipLaunchKernelGGL(seg_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
   if (d.n_raw) hipLaunchKernelGGL(records_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
   if (d.n_segs) hipLaunchKernelGGL(pad_kernel, dim3(d.n_segs), dim3(PATCH), 0, st, d);
-  if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(d.n_patches), dim3(TILE), 0, st, d);
+  if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(blocks(d.n_patches, PATCH_RUN)), dim3(TILE), 0, st, d);
   if (d.n_tiles) hipLaunchKernelGGL(tiles_kernel, dim3(d.n_tiles), dim3(TILE), 0, st, d);
   if (d.n_ecl_patches) hipLaunchKernelGGL(eclflags_kernel, dim3(d.n_ecl_patches), dim3(TILE), 0, st, d);
   if (d.n_pblocks && d.max_tiles) {
