@@ -282,6 +282,93 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
+// ------------------------------------- linear-law star patches on MFMA
+// The cosines of a rescaled (LD_LINW) patch against a tile: a 64 x 64
+// product of the patch's directions (64 x 3) and the lanes' earth vectors
+// (3 x 64), as 16 v_mfma_f32_16x16x4_f32 (K = 4, the fourth term zero).
+// Operand layouts of one 16 x 16 block: lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16], and D[4 (l / 16) + i][l % 16] in result element i.
+// Lane l then owns column (time lane) 16 cb + l % 16 of column block cb
+// for the 4 rows 4 (l / 16) + i of each row block, and sums v s and v^2
+// over them (v = clamp(mu), s the record's weight scale); the four lanes
+// of a column meet in FP64 once per item.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Lane l's value at lane l ^ 16 and at lane l ^ 32 (v_permlane16_swap /
+// v_permlane32_swap: rows of 16 lanes, halves of 32).
+__device__ __forceinline__ uint32_t xor16_u32(uint32_t v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return ((__lane_id() >> 4) & 1u) ? r[0] : r[1];
+}
+__device__ __forceinline__ uint32_t xor32_u32(uint32_t v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (__lane_id() >> 5) ? r[0] : r[1];
+}
+__device__ __forceinline__ double xor16_f64(double v) {
+  return __hiloint2double(static_cast<int>(xor16_u32(static_cast<uint32_t>(__double2hiint(v)))),
+                          static_cast<int>(xor16_u32(static_cast<uint32_t>(__double2loint(v)))));
+}
+__device__ __forceinline__ double xor32_f64(double v) {
+  return __hiloint2double(static_cast<int>(xor32_u32(static_cast<uint32_t>(__double2hiint(v)))),
+                          static_cast<int>(xor32_u32(static_cast<uint32_t>(__double2loint(v)))));
+}
+
+// The B operands (earth vectors of the tile's lanes) of the four column
+// blocks, from every lane's LaneCtx.
+__device__ __forceinline__ void linw_mfma_b(const LaneCtx& L, float b[4]) {
+  const uint32_t lane = __lane_id(), r = lane % 16u, k = lane / 16u;
+#pragma unroll
+  for (uint32_t cb = 0; cb < 4; ++cb) {
+    const int src = static_cast<int>(cb * 16u + r);
+    const float x = __shfl(L.ex, src), y = __shfl(L.ey, src), z = __shfl(L.ez, src);
+    b[cb] = k == 0u ? x : (k == 1u ? y : (k == 2u ? z : 0.0f));
+  }
+}
+
+// One staged patch: adds w2 (c0 S1 + a0 S2) over this lane's rows of each
+// column block to acc[cb] (S1 = sum v s, S2 = sum v^2 in FP32 per patch).
+__device__ __forceinline__ void linw_mfma_patch(const float4* lp, const float b[4], const StarPar& P,
+                                                double acc[4]) {
+  const uint32_t lane = __lane_id(), r = lane % 16u, k = lane / 16u;
+  const float* lpf = reinterpret_cast<const float*>(lp);
+  float s1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, s2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (uint32_t rb = 0; rb < 4; ++rb) {
+    const float a = lpf[(rb * 16u + r) * 4u + k];  // k = 3: the weight scale, times B's zero row
+    float sw[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) sw[i] = lp[rb * 16u + 4u * k + i].w;
+    // The four blocks' products first, so they overlap in the matrix unit.
+    f32x4 mu[4];
+#pragma unroll
+    for (uint32_t cb = 0; cb < 4; ++cb)
+      mu[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[cb], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+    for (uint32_t cb = 0; cb < 4; ++cb) {
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) {
+        const float v = clamp01(mu[cb][i]);
+        s1[cb] = fmaf(v, sw[i], s1[cb]);
+        s2[cb] = fmaf(v, v, s2[cb]);
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t cb = 0; cb < 4; ++cb) acc[cb] += static_cast<double>(P.w2 * fmaf(P.c0, s1[cb], P.a0 * s2[cb]));
+}
+
+// The item's total for this lane's own time lane: the four lanes of each
+// column meet, and lane l takes column block l / 16.
+__device__ __forceinline__ double linw_mfma_total(double acc[4]) {
+#pragma unroll
+  for (uint32_t cb = 0; cb < 4; ++cb) {
+    acc[cb] += xor16_f64(acc[cb]);
+    acc[cb] += xor32_f64(acc[cb]);
+  }
+  const uint32_t k = __lane_id() / 16u;
+  return k == 0u ? acc[0] : (k == 1u ? acc[1] : (k == 2u ? acc[2] : acc[3]));
+}
+
 // ----------------------------------------------------- one work item
 // Visits the patches in `todo` (wave-uniform), choosing the patch body by
 // the `fr` bit; each patch's FP32 sum is added in FP64.
@@ -309,6 +396,21 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
       return *reinterpret_cast<const float4*>(p + (static_cast<size_t>(__builtin_ctzll(t)) * PATCH + lane) * STRIDE);
     };
     float4 nxt = rec(todo);
+    if constexpr (PK == PK_STAR && LD == LD_LINW) {
+      // Linear-law star patches: the cosines on MFMA (linw_mfma_patch).
+      float b[4];
+      linw_mfma_b(L, b);
+      double part[4] = {0.0, 0.0, 0.0, 0.0};
+      while (todo) {
+        todo &= todo - 1ull;
+        lp[lane] = nxt;
+        __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+        if (todo) nxt = rec(todo);
+        linw_mfma_patch(lp, b, P, part);
+        __builtin_amdgcn_wave_barrier();  // reads done before the next write
+      }
+      return linw_mfma_total(part);
+    }
     while (todo) {
       todo &= todo - 1ull;
       lp[lane] = nxt;

@@ -112,47 +112,50 @@ __global__ __launch_bounds__(kBlock) void seg_flags_kernel(PackDev d) {
   d.segs[s].flags = flags;
 }
 
-// The record of sorted position p, an element of segment seg.
-__device__ inline void record_at(const PackDev& d, uint64_t p, uint32_t seg) {
-  const uint32_t e = d.vals_s[p];
-  const PackSeg S = d.psegs[seg];
-  const uint32_t flags = d.segs[seg].flags;
-  const uint32_t rank = static_cast<uint32_t>(p) - S.sorted0;
-  const uint32_t stride = stride_of(S);
-  const uint32_t eoff = S.posn ? 32u : 16u;
-  char* r = d.recs + S.rec_off + static_cast<uint64_t>(rank) * stride;
-  float4 f = d.raw_df[e];
-  if (flags & SEG_LINW) {
-    const double inv = ldexp(1.0, -seg_linw_exp(flags));
+// What writing a segment's element records needs.
+struct RecSeg {
+  uint64_t rec_off;
+  uint32_t stride, eoff, necl, posn, pos0, flags;
+  uint64_t e0;
+};
+__device__ inline RecSeg rec_seg(const PackDev& d, uint32_t seg) {
+  const PackSeg& S = d.psegs[seg];
+  RecSeg R;
+  R.rec_off = S.rec_off;
+  R.stride = stride_of(S);
+  R.eoff = S.posn ? 32u : 16u;
+  R.necl = S.necl;
+  R.posn = S.posn;
+  R.pos0 = S.pos0;
+  R.e0 = S.e0;
+  R.flags = d.segs[seg].flags;
+  return R;
+}
+
+// The record of the element of rank `rank` in its segment's sorted order,
+// raw element e with plain values f. Linear-law segments (SEG_LINW) take
+// (s d, s), s = sqrt(w) / 2^k, in place of (d, w).
+__device__ inline void write_record(const PackDev& d, const RecSeg& R, uint32_t rank, uint32_t e, float4 f) {
+  char* r = d.recs + R.rec_off + static_cast<uint64_t>(rank) * R.stride;
+  if (R.flags & SEG_LINW) {
+    const double inv = ldexp(1.0, -seg_linw_exp(R.flags));
     const double sc = sqrt(static_cast<double>(f.w)) * inv;
     f = make_float4(static_cast<float>(static_cast<double>(f.x) * sc), static_cast<float>(static_cast<double>(f.y) * sc),
                     static_cast<float>(static_cast<double>(f.z) * sc), static_cast<float>(sc));
   }
   store_f4(r, f);
-  if (S.posn) {
-    float4 q = d.raw_pos[S.pos0 + (e - S.e0)];
+  if (R.posn) {
+    float4 q = d.raw_pos[R.pos0 + (e - R.e0)];
     q.w = 0.0f;
     store_f4(r + 16, q);
   }
   const uint32_t i0 = d.raw_eoff[e];
-  for (uint32_t j = 0; j < S.necl; ++j) {
+  for (uint32_t j = 0; j < R.necl; ++j) {
     const double2 pr = d.raw_ecl[i0 + j];
-    double* o = reinterpret_cast<double*>(r + eoff + 24u * j);
+    double* o = reinterpret_cast<double*>(r + R.eoff + 24u * j);
     o[0] = pr.x;
     o[1] = pr.y;
     o[2] = pr.y - 1.0;
-  }
-}
-
-// Element records in sorted order: each segment's elements by spatial key,
-// ties in raw (grid) order. Linear-law segments (SEG_LINW) take (s d, s),
-// s = sqrt(w) / 2^k, in place of (d, w).
-__global__ __launch_bounds__(kBlock) void records_kernel(PackDev d) {
-  const KeyTile kt = d.ktiles[blockIdx.x];
-  const PackGrid G = d.grids[kt.grid];
-  for (uint32_t i = threadIdx.x; i < kt.n; i += kBlock) {
-    const uint64_t p = G.e0 + kt.i0 + i;
-    record_at(d, p, d.keys_s[p] >> 16);
   }
 }
 
@@ -217,7 +220,9 @@ __device__ inline double upper_half(double v) {
 // from the plain values: a wavefront per run of PATCH_RUN patches, a lane
 // per slot. Each sum is tree_sum64's: its first level (lane i + lane i +
 // 32) by a lane swap, the rest by one lane per sum over the 32 partial sums
-// in LDS. The gather is pipelined: while a patch is summed, the plain
+// in LDS. The same pass writes the patch's element records (sorted order,
+// each segment's elements by spatial key, ties in raw order) from the values
+// it gathered. The gather is pipelined: while a patch is summed, the plain
 // values of the next one and the sorted positions of the one after it are
 // in flight. The segment's fields are re-read only when a run crosses into
 // the next segment.
@@ -230,14 +235,16 @@ __global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
   if (p0 >= d.n_patches) return;
   const uint32_t pend = d.n_patches - p0 < PATCH_RUN ? d.n_patches : p0 + PATCH_RUN;
   // The segment of the patch whose positions are read next.
-  uint32_t s_patch0 = 0, s_end = 0, s_nreal = 0, s_sorted0 = 0, s_mom = 0;
-  // Real slots, moment flag and this lane's raw element of a patch.
+  uint32_t s_seg = 0, s_patch0 = 0, s_end = 0, s_nreal = 0, s_sorted0 = 0, s_mom = 0;
+  // Segment, first slot, real slots, moment flag and this lane's raw
+  // element of a patch.
   struct Ahead {
-    uint32_t nr, mom, e;
+    uint32_t seg, first, nr, mom, e;
   };
   auto positions = [&](uint32_t pp) {
     if (pp >= s_end) {
-      const PackSeg& S = d.psegs[d.patch_seg[pp]];
+      s_seg = d.patch_seg[pp];
+      const PackSeg& S = d.psegs[s_seg];
       s_patch0 = S.patch0;
       s_end = S.patch0 + S.n_pad / PATCH;
       s_nreal = S.n_real;
@@ -246,6 +253,8 @@ __global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
     }
     const uint32_t first = (pp - s_patch0) * PATCH;
     Ahead a;
+    a.seg = s_seg;
+    a.first = first;
     a.nr = s_nreal > first ? (s_nreal - first < PATCH ? s_nreal - first : PATCH) : 0u;
     a.mom = s_mom;
     a.e = j < a.nr ? d.vals_s[s_sorted0 + first + j] : 0xFFFFFFFFu;
@@ -256,6 +265,9 @@ __global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
   };
   Ahead cur = positions(p0);
   float4 f = plain(cur);
+  // The segment whose records are written.
+  uint32_t r_seg = 0xFFFFFFFFu;
+  RecSeg R{};
   Ahead nxt = cur;
   if (p0 + 1 < pend) nxt = positions(p0 + 1);
   for (uint32_t p = p0; p < pend; ++p) {
@@ -267,6 +279,11 @@ __global__ __launch_bounds__(TILE) void patches_kernel(PackDev d) {
     const uint32_t nr = cur.nr;
     const bool real = j < nr;
     const bool mom = cur.mom != 0u;
+    if (cur.seg != r_seg) {
+      r_seg = cur.seg;
+      R = rec_seg(d, r_seg);
+    }
+    if (real) write_record(d, R, cur.first + j, cur.e, f);
     auto level1 = [&](double v, uint32_t c) {
       const double o = upper_half(v);
       if (j < PATCH / 2) part[j][c] = v + o;
@@ -958,7 +975,6 @@ hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp
     if (e != hipSuccess) return e;
   }
   if (d.n_segs) hipLaunchKernelGGL(seg_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
-  if (d.n_raw) hipLaunchKernelGGL(records_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
   if (d.n_segs) hipLaunchKernelGGL(pad_kernel, dim3(d.n_segs), dim3(PATCH), 0, st, d);
   if (d.n_patches) hipLaunchKernelGGL(patches_kernel, dim3(blocks(d.n_patches, PATCH_RUN)), dim3(TILE), 0, st, d);
   if (d.n_tiles) hipLaunchKernelGGL(tiles_kernel, dim3(d.n_tiles), dim3(TILE), 0, st, d);
