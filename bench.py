@@ -317,6 +317,7 @@ def bench_chisq(args):
 
     for _ in range(args.warmup):
         dp.launch(scale=scale)
+    lc.sync()  # the warm-up's last flux pass is still running: outside the timed region
     log(f"timing {args.steps} steps (pack + flux + assembly) ...")
     t0 = time.perf_counter()
     for _ in range(args.steps):

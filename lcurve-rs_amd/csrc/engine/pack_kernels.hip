@@ -416,8 +416,10 @@ __global__ __launch_bounds__(TILE) void eclflags_kernel(PackDev d) {
   }
 }
 
-// Work code of every (patch, tile) pair: a workgroup per (block of 256
-// patches of one set, tile), the tile's lanes in LDS, a thread per patch.
+// Work code of every (patch, tile) pair: a workgroup per block of 256
+// patches of one set, a thread per patch, looping over the set's tiles with
+// each tile's lanes in LDS. The patch's axis and cone bounds (FP64) are
+// made once, before the tile loop.
 // With limb lanes, each PT_FREE pair of a segment that allows moments is
 // also split per lane (the host packer's split_lanes, batch_build.cpp, with
 // its element-level check off): a lane of the tile whose FP32 cosine with
@@ -431,88 +433,101 @@ __global__ __launch_bounds__(kBlock) void codes_kernel(PackDev d) {
   __shared__ unsigned long long cnt[2];
   const PatchBlock pb = d.pblocks[blockIdx.x];
   const PackSet set = d.sets[pb.set];
-  const uint32_t t = blockIdx.y;
-  if (t >= set.n_tiles) return;  // uniform over the workgroup, before the barriers
-  const uint32_t tg = set.tile0 + t;
-  if (threadIdx.x < TILE) {
-    const float4 v = d.tile_en[static_cast<uint64_t>(tg) * TILE + threadIdx.x];
-    ex[threadIdx.x] = v.x;
-    ey[threadIdx.x] = v.y;
-    ez[threadIdx.x] = v.z;
-    if (SPLIT) {
-      // The lane's (class, grid type) bit: the lanes a segment's split covers.
-      const uint32_t m = d.lane_meta[set.lane_off + t * TILE + threadIdx.x];
-      lcp[threadIdx.x] = (m & META_VALID) ? cp_bit(meta_class(m), meta_ptype(m)) : 0u;
-      if (threadIdx.x < 2) cnt[threadIdx.x] = 0ull;
-    }
-  }
-  __syncthreads();
   const bool live = threadIdx.x < pb.n;
   const uint32_t p = pb.patch0 + (live ? threadIdx.x : 0u);
   const uint32_t seg = d.patch_seg[p];
   const PackSeg& S = d.psegs[seg];
   const uint32_t lp = p - set.patch0;
-  const uint64_t ci = set.code0 + static_cast<uint64_t>(lp) * set.n_tiles + t;  // eclipse flags, patch-major
-  const uint64_t co = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;  // codes, tile-major
-  const uint64_t vm = d.tile_valid[tg];
   const uint32_t sflags = S.flags;
   const bool cull = (sflags & PSEG_CULL) != 0, front_ok = (sflags & PSEG_FRONT) != 0;
-  uint8_t code = PT_SKIP;
-  bool has = false;
-  uint64_t all = 0, some = 0;
-  if (live && vm != 0ull) {
-    const Cone c = d.cones[p];
-    double px, py, pz, plen;
-    cone_axis(c, &px, &py, &pz, &plen);
-    const float fx = static_cast<float>(px), fy = static_cast<float>(py), fz = static_cast<float>(pz);
-    // The split's per-lane bounds (split_lanes): cos_front / cos_skip of
-    // the cone where it allows them.
-    const bool split = SPLIT && front_ok && plen > 0.0;
-    const bool can_all = c.cosw > S.K.c_front, can_none = cull && c.cosw > S.K.c_skip;
-    const double cf = can_all ? cos_front(S.K, c.cosw) : 2.0;
-    const double cg = can_none ? cos_skip(S.K, c.cosw) : -2.0;
-    const uint32_t use = split ? d.segs[seg].use_mask : 0u;
-    float emin = 1.0f, emax = -1.0f;
-    for (uint32_t j = 0; j < TILE; ++j) {
-      if (!((vm >> j) & 1ull)) continue;
-      const float v = los_cosine(ex[j], ey[j], ez[j], fx, fy, fz);
-      emin = v < emin ? v : emin;
-      emax = v > emax ? v : emax;
-      if (SPLIT && (use & lcp[j])) {
-        const double cv = v;
-        if (cv > cf) all |= 1ull << j;
-        else if (!(cv < cg)) some |= 1ull << j;
+  const uint32_t necl = S.necl;
+  // Per patch: the axis in FP32 and the cone's bounds.
+  const Cone c = d.cones[p];
+  double px, py, pz, plen;
+  cone_axis(c, &px, &py, &pz, &plen);
+  const float fx = static_cast<float>(px), fy = static_cast<float>(py), fz = static_cast<float>(pz);
+  const bool can_all = c.cosw > S.K.c_front, can_none = cull && c.cosw > S.K.c_skip;
+  const double cf = can_all ? cos_front(S.K, c.cosw) : 2.0;
+  const double cg = can_none ? cos_skip(S.K, c.cosw) : -2.0;
+  // Skip when every lane is beyond cos_skip; moments when every lane is
+  // inside cos_front (both need a cone that bounds something).
+  const bool may_skip = can_none && plen > 0.0, may_front = front_ok && can_all && plen > 0.0;
+  // The split's per-lane bounds (split_lanes): cos_front / cos_skip of the
+  // cone where it allows them.
+  const bool split = SPLIT && front_ok && plen > 0.0;
+  const uint32_t use = split ? d.segs[seg].use_mask : 0u;
+  if (SPLIT && threadIdx.x < 2) cnt[threadIdx.x] = 0ull;
+  for (uint32_t t = 0; t < set.n_tiles; ++t) {
+    const uint32_t tg = set.tile0 + t;
+    const uint64_t vm = d.tile_valid[tg];
+    __syncthreads();  // the previous tile's lanes are read
+    if (threadIdx.x < TILE) {
+      // A lane outside the tile's valid lanes takes the first valid lane's
+      // line of sight: the extremes over the tile are the same, and the
+      // lane loop below needs no test per lane. (A tile without valid
+      // lanes is coded skip without the loop.)
+      const uint32_t src = ((vm >> threadIdx.x) & 1ull) || vm == 0ull
+                               ? threadIdx.x
+                               : static_cast<uint32_t>(__builtin_ctzll(vm));
+      const float4 v = d.tile_en[static_cast<uint64_t>(tg) * TILE + src];
+      ex[threadIdx.x] = v.x;
+      ey[threadIdx.x] = v.y;
+      ez[threadIdx.x] = v.z;
+      if (SPLIT) {
+        // The lane's (class, grid type) bit: the lanes a segment's split covers.
+        const uint32_t m = d.lane_meta[set.lane_off + t * TILE + threadIdx.x];
+        lcp[threadIdx.x] = (m & META_VALID) ? cp_bit(meta_class(m), meta_ptype(m)) : 0u;
       }
     }
-    if (cull && plen > 0.0 && c.cosw > S.K.c_skip && emax < cos_skip(S.K, c.cosw)) {
-      code = PT_SKIP;
-    } else {
-      const bool front = front_ok && plen > 0.0 && c.cosw > S.K.c_front && emin > cos_front(S.K, c.cosw);
-      if (S.necl == 0) {
-        code = front ? PT_MOM : PT_FREE;
+    __syncthreads();
+    const uint64_t co = set.code0 + static_cast<uint64_t>(t) * set.n_patches + lp;  // codes, tile-major
+    uint8_t code = PT_SKIP;
+    bool has = false;
+    uint64_t all = 0, some = 0;
+    if (live && vm != 0ull) {
+      float emin = 1.0f, emax = -1.0f;
+      for (uint32_t j = 0; j < TILE; ++j) {
+        const float v = los_cosine(ex[j], ey[j], ez[j], fx, fy, fz);
+        emin = fminf(v, emin);  // v is finite: the same as v < emin ? v : emin
+        emax = fmaxf(v, emax);
+        if (SPLIT && (use & lcp[j])) {
+          const double cv = v;
+          if (cv > cf) all |= 1ull << j;
+          else if (!(cv < cg)) some |= 1ull << j;
+        }
+      }
+      if (may_skip && emax < cg) {
+        code = PT_SKIP;
       } else {
-        const uint8_t ef = d.eflags[ci];
-        code = (ef & 2u) ? PT_SKIP : ((ef & 1u) ? (front ? PT_MOM : PT_FREE) : PT_FULL);
+        const bool front = may_front && emin > cf;
+        if (necl == 0) {
+          code = front ? PT_MOM : PT_FREE;
+        } else {
+          const uint8_t ef = d.eflags[set.code0 + static_cast<uint64_t>(lp) * set.n_tiles + t];  // patch-major
+          code = (ef & 2u) ? PT_SKIP : ((ef & 1u) ? (front ? PT_MOM : PT_FREE) : PT_FULL);
+        }
+      }
+      has = split && code == PT_FREE;
+    }
+    if (live) d.codes[co] = code;
+    if (SPLIT) {
+      if (!has) all = some = 0;
+      if (live) {
+        d.has_split[co] = has ? 1u : 0u;
+        d.split_all[co] = all;
+        d.split_some[co] = some;
+      }
+      // Counts per wavefront, then one LDS add per wavefront.
+      const uint32_t nh = static_cast<uint32_t>(__popcll(__ballot(has)));
+      uint32_t nl = static_cast<uint32_t>(__popcll(some));
+      for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
+      if (threadIdx.x % TILE == 0 && nh) {
+        atomicAdd(&cnt[0], static_cast<unsigned long long>(nh));
+        atomicAdd(&cnt[1], static_cast<unsigned long long>(nl));
       }
     }
-    has = split && code == PT_FREE;
   }
-  if (live) d.codes[co] = code;
   if (!SPLIT) return;  // uniform
-  if (!has) all = some = 0;
-  if (live) {
-    d.has_split[co] = has ? 1u : 0u;
-    d.split_all[co] = all;
-    d.split_some[co] = some;
-  }
-  // Counts per wavefront, then one LDS add per wavefront.
-  const uint32_t nh = static_cast<uint32_t>(__popcll(__ballot(has)));
-  uint32_t nl = static_cast<uint32_t>(__popcll(some));
-  for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
-  if (threadIdx.x % TILE == 0 && nh) {
-    atomicAdd(&cnt[0], static_cast<unsigned long long>(nh));
-    atomicAdd(&cnt[1], static_cast<unsigned long long>(nl));
-  }
   __syncthreads();
   if (threadIdx.x < 2 && cnt[threadIdx.x]) atomicAdd(&d.split_stats[threadIdx.x], cnt[threadIdx.x]);
 }
@@ -980,8 +995,8 @@ hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp
   if (d.n_tiles) hipLaunchKernelGGL(tiles_kernel, dim3(d.n_tiles), dim3(TILE), 0, st, d);
   if (d.n_ecl_patches) hipLaunchKernelGGL(eclflags_kernel, dim3(d.n_ecl_patches), dim3(TILE), 0, st, d);
   if (d.n_pblocks && d.max_tiles) {
-    if (d.limb) hipLaunchKernelGGL(codes_kernel<true>, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
-    else hipLaunchKernelGGL(codes_kernel<false>, dim3(d.n_pblocks, d.max_tiles), dim3(kBlock), 0, st, d);
+    if (d.limb) hipLaunchKernelGGL(codes_kernel<true>, dim3(d.n_pblocks), dim3(kBlock), 0, st, d);
+    else hipLaunchKernelGGL(codes_kernel<false>, dim3(d.n_pblocks), dim3(kBlock), 0, st, d);
   }
   const uint32_t wpb = kBlock / TILE;  // wavefronts per workgroup
   if (d.limb && d.n_chunks) {
