@@ -312,8 +312,8 @@ std::unique_ptr<SetPack> pack_records(const lc::PreparedModel& P, bool keep_maps
       if (pts[e].eclipse.size() == k) idx.push_back(e);
     std::vector<uint32_t> key(pts.size());
     for (uint32_t e : idx)
-      key[e] = direction_key16(static_cast<float>(pts[e].dirn.x), static_cast<float>(pts[e].dirn.y),
-                               static_cast<float>(pts[e].dirn.z));
+      key[e] = direction_key(static_cast<float>(pts[e].dirn.x), static_cast<float>(pts[e].dirn.y),
+                             static_cast<float>(pts[e].dirn.z));
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
     const uint32_t nr = static_cast<uint32_t>(idx.size());
     if (nr != n_real[si] || cones.size() != sg.patch_off) throw lc::LcError("segment layout out of step");

@@ -296,10 +296,34 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
   const uint64_t n_cand = n_chunks * max_tiles;
   if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
-  // Sort keys are (segment << 16 | direction key) in 32 bits.
-  if (n_segs > 65536) throw std::invalid_argument("a device-packed batch holds at most 65536 segments: split the sets");
-  key_bits_ = 16;
-  for (uint64_t v = n_segs > 0 ? n_segs - 1 : 0; v; v >>= 1) ++key_bits_;
+  // Sort pieces: runs of whole sets of at most SORT_PIECE_SEGS segments,
+  // each sorted in its own range on (segment within the piece, direction
+  // key) in at most 24 bits.
+  pieces_.clear();
+  std::vector<uint32_t> piece_seg0(ns);
+  {
+    uint64_t r = 0, r0 = 0;
+    uint32_t sg = 0, sg0 = 0;
+    auto close = [&]() {
+      if (sg == sg0) return;
+      uint32_t bits = DIR_KEY_BITS;
+      for (uint32_t v = sg - sg0 - 1u; v; v >>= 1) ++bits;
+      pieces_.push_back(SortPiece{r0, r - r0, bits});
+    };
+    for (size_t s = 0; s < ns; ++s) {
+      const uint32_t nseg = static_cast<uint32_t>(lay[s]->S.segs.size());
+      if (nseg > SORT_PIECE_SEGS) throw std::invalid_argument("a set with too many segments");
+      if (sg - sg0 + nseg > SORT_PIECE_SEGS) {
+        close();
+        r0 = r;
+        sg0 = sg;
+      }
+      piece_seg0[s] = sg0;
+      r += lay[s]->df.size();
+      sg += nseg;
+    }
+    close();
+  }
   if (n_codes >= UINT32_MAX) throw std::invalid_argument("too many (patch, tile) pairs: split the sets");
   uint64_t n_ktiles = 0;
   for (const auto& Lp : lay)
@@ -410,6 +434,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
       G.e0 = e_set + g.e0;
       G.n = g.n;
       for (int k = 0; k <= MAX_ECL; ++k) G.seg_of[k] = g.seg_of[k] == UINT32_MAX ? UINT32_MAX : seg + g.seg_of[k];
+      G.seg_base = piece_seg0[s];
       for (uint32_t i0 = 0; i0 < g.n; i0 += KEY_TILE) ktiles[kt++] = KeyTile{grid, i0, std::min(KEY_TILE, g.n - i0)};
       pgrids[grid++] = G;
     }
@@ -556,7 +581,9 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   L.lsw = w.add(any_iscale ? n_times * 6 * sizeof(double) : 0);
   L.total = w.total;
   st_.work_bytes = L.total;
-  sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(n_raw), 16);
+  uint64_t max_piece = 0;
+  for (const SortPiece& P : pieces_) max_piece = std::max(max_piece, P.n);
+  sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(max_piece), 16);
   scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand, n_chunks), 16);
   n_cand_ = n_cand;
   auto din = [&](size_t off) { return d_in_ + off; };
@@ -792,7 +819,8 @@ void DevicePackBatch::ensure_capacity(Slot& sl, const PackCounts& c) {
 
 void DevicePackBatch::pack(Slot& sl) {
   hipStream_t st = sl.stream;
-  hip_check(launch_pack_count(sl.pd, sl.sort_temp, sort_temp_bytes_, sl.scan_temp, scan_temp_bytes_, key_bits_, st),
+  hip_check(launch_pack_count(sl.pd, sl.sort_temp, sort_temp_bytes_, sl.scan_temp, scan_temp_bytes_, pieces_.data(),
+                              pieces_.size(), st),
             "pack");
   hip_check(hipMemcpyAsync(sl.h_counts, sl.pd.counts, sizeof(PackCounts), hipMemcpyDeviceToHost, st),
             "hipMemcpyAsync counts");

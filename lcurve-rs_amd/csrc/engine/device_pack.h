@@ -171,7 +171,7 @@ class DevicePackBatch {
   size_t in_bytes_ = 0;
   WorkLayout wl_{};
   size_t sort_temp_bytes_ = 0, scan_temp_bytes_ = 0;
-  uint32_t key_bits_ = 32;  // significant bits of the sort keys
+  std::vector<SortPiece> pieces_;  // the sort's pieces (whole sets each)
   uint64_t n_cand_ = 0;
   // The input pointers and sizes every slot starts from.
   PackDev pd0_;

@@ -43,7 +43,18 @@ struct PackGrid {
   uint64_t e0;
   uint32_t n;
   uint32_t seg_of[MAX_ECL + 1];  // segment of each eclipse-pair count (0xFFFFFFFF: none)
+  uint32_t seg_base;             // first segment of the grid's sort piece
 };
+
+// A run of whole sets sorted on its own: raw elements [r0, r0 + n) are
+// also its sorted positions, and its keys are (segment - the piece's first
+// segment) << DIR_KEY_BITS | direction key, in `bits` bits.
+struct SortPiece {
+  uint64_t r0, n;
+  uint32_t bits;
+};
+// Segments of one sort piece at most (a 24-bit key with DIR_KEY_BITS = 12).
+constexpr uint32_t SORT_PIECE_SEGS = 4096;
 
 struct PackSet {
   uint32_t lane_off, n_tiles;  // first lane (batch-wide, multiple of TILE), lane tiles
@@ -189,8 +200,7 @@ struct PackDev {
   double* partials = nullptr;
 };
 
-// Bytes of the radix sort's temporary storage for n keys. The keys are
-// (segment << 16 | direction key): at most 65536 segments.
+// Bytes of the radix sort's temporary storage for a piece of n keys.
 size_t pack_sort_temp_bytes(uint64_t n);
 size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks);
 // The pack in two phases on `st`. The first makes the records, cones,
@@ -199,7 +209,7 @@ size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks);
 // second phase's tables. The second writes the work items, their lane-mask
 // groups and moment sums, and the limb tables.
 hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                             size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st);
+                             size_t scan_temp_bytes, const SortPiece* pieces, size_t n_pieces, hipStream_t st);
 hipError_t launch_pack_fill(const PackDev& d, uint64_t n_entries, hipStream_t st);
 
 }  // namespace lch

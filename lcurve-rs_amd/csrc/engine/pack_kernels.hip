@@ -35,7 +35,8 @@ __device__ inline float4 load_f4(const char* r) {
   return make_float4(a.x, a.y, b.x, b.y);
 }
 
-// Sort key of every raw element: its segment above the spatial key. With
+// Sort key of every raw element: its segment (within its sort piece) above
+// the spatial key. With
 // it, per segment, the largest weight and whether a weight is negative or
 // not finite, for the linear-law rescaling. One workgroup per key tile.
 __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
     uint32_t k = d.raw_eoff[e + 1] - d.raw_eoff[e];
     k = k < static_cast<uint32_t>(MAX_ECL) ? k : static_cast<uint32_t>(MAX_ECL);
     const float4 v = d.raw_df[e];
-    d.keys[e] = (G.seg_of[k] << 16) | direction_key16(v.x, v.y, v.z);
+    d.keys[e] = ((G.seg_of[k] - G.seg_base) << DIR_KEY_BITS) | direction_key(v.x, v.y, v.z);
     d.vals[e] = static_cast<uint32_t>(e);
     const float w = v.w;
     if (!(w >= 0.0f) || !isfinite(w)) {
@@ -972,7 +973,7 @@ size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks) {
 }
 
 hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                             size_t scan_temp_bytes, uint32_t key_bits, hipStream_t st) {
+                             size_t scan_temp_bytes, const SortPiece* pieces, size_t n_pieces, hipStream_t st) {
   hipError_t e = hipSuccess;
   if (d.limb) {
     e = hipMemsetAsync(d.split_stats, 0, 2 * sizeof(unsigned long long), st);
@@ -984,10 +985,16 @@ hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp
   }
   if (d.n_raw) {
     hipLaunchKernelGGL(keys_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
-    size_t tb = sort_temp_bytes;
-    e = rocprim::radix_sort_pairs(sort_temp, tb, d.keys, d.keys_s, d.vals, d.vals_s, static_cast<size_t>(d.n_raw),
-                                  0, key_bits, st);
-    if (e != hipSuccess) return e;
+    // Each piece (whole sets, at most SORT_PIECE_SEGS segments) is sorted
+    // in place in its own range: three 8-bit passes instead of four.
+    for (size_t i = 0; i < n_pieces; ++i) {
+      const SortPiece& P = pieces[i];
+      if (!P.n) continue;
+      size_t tb = sort_temp_bytes;
+      e = rocprim::radix_sort_pairs(sort_temp, tb, d.keys + P.r0, d.keys_s + P.r0, d.vals + P.r0, d.vals_s + P.r0,
+                                    static_cast<size_t>(P.n), 0, P.bits, st);
+      if (e != hipSuccess) return e;
+    }
   }
   if (d.n_segs) hipLaunchKernelGGL(seg_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
   if (d.n_segs) hipLaunchKernelGGL(pad_kernel, dim3(d.n_segs), dim3(PATCH), 0, st, d);

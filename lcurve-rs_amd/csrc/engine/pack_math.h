@@ -184,6 +184,16 @@ __host__ __device__ inline double cos_front(const CodeConsts& k, float cosw) {
   return k.cb * static_cast<double>(cosw) + k.sb * cone_sin(cosw);
 }
 
+// The key that orders a segment's elements into patches: the top
+// DIR_KEY_BITS bits of direction_key16, a Hilbert curve over 64 x 64 cells
+// of the octahedral map. Ties keep the grid order. Twelve bits leave the
+// segment index 12 bits of a 24-bit sort key (three 8-bit radix passes)
+// for up to 4096 segments.
+constexpr uint32_t DIR_KEY_BITS = 12;
+__host__ __device__ inline uint32_t direction_key(float fx, float fy, float fz) {
+  return direction_key16(fx, fy, fz) >> (16u - DIR_KEY_BITS);
+}
+
 // One record's terms of a patch's direction moments (layout of MOM_STRIDE
 // in layout.h) from its normal d and weight w: w d, the coefficients of
 // w (e.d)^2 and of w (e.d)^3 in the monomials of e, and |w|. A record of
