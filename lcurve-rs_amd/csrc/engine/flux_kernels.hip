@@ -294,35 +294,17 @@ __device__ __forceinline__ float spot_patch(const char* __restrict__ p, const La
 // of a column meet in FP64 once per item.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Lane l's value at lane l ^ 16 and at lane l ^ 32 (v_permlane16_swap /
-// v_permlane32_swap: rows of 16 lanes, halves of 32).
-__device__ __forceinline__ uint32_t xor16_u32(uint32_t v) {
-  const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-  return ((__lane_id() >> 4) & 1u) ? r[0] : r[1];
-}
-__device__ __forceinline__ uint32_t xor32_u32(uint32_t v) {
-  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  return (__lane_id() >> 5) ? r[0] : r[1];
-}
-__device__ __forceinline__ double xor16_f64(double v) {
-  return __hiloint2double(static_cast<int>(xor16_u32(static_cast<uint32_t>(__double2hiint(v)))),
-                          static_cast<int>(xor16_u32(static_cast<uint32_t>(__double2loint(v)))));
-}
-__device__ __forceinline__ double xor32_f64(double v) {
-  return __hiloint2double(static_cast<int>(xor32_u32(static_cast<uint32_t>(__double2hiint(v)))),
-                          static_cast<int>(xor32_u32(static_cast<uint32_t>(__double2loint(v)))));
-}
-
 // The B operands (earth vectors of the tile's lanes) of the four column
-// blocks, from every lane's LaneCtx.
-__device__ __forceinline__ void linw_mfma_b(const LaneCtx& L, float b[4]) {
+// blocks: every lane puts its earth vector in `eb` (64 float4 of LDS), then
+// reads its four operands.
+__device__ __forceinline__ void linw_mfma_b(const LaneCtx& L, float4* eb, float b[4]) {
   const uint32_t lane = __lane_id(), r = lane % 16u, k = lane / 16u;
+  eb[lane] = make_float4(L.ex, L.ey, L.ez, 0.0f);
+  __builtin_amdgcn_wave_barrier();  // every lane's vector is in LDS
+  const float* ef = reinterpret_cast<const float*>(eb);
 #pragma unroll
-  for (uint32_t cb = 0; cb < 4; ++cb) {
-    const int src = static_cast<int>(cb * 16u + r);
-    const float x = __shfl(L.ex, src), y = __shfl(L.ey, src), z = __shfl(L.ez, src);
-    b[cb] = k == 0u ? x : (k == 1u ? y : (k == 2u ? z : 0.0f));
-  }
+  for (uint32_t cb = 0; cb < 4; ++cb) b[cb] = ef[(cb * 16u + r) * 4u + k];  // k = 3: the zero row
+  __builtin_amdgcn_wave_barrier();  // reads done before the LDS is reused
 }
 
 // One staged patch: adds w2 (c0 S1 + a0 S2) over this lane's rows of each
@@ -357,16 +339,20 @@ __device__ __forceinline__ void linw_mfma_patch(const float4* lp, const float b[
   for (uint32_t cb = 0; cb < 4; ++cb) acc[cb] += static_cast<double>(P.w2 * fmaf(P.c0, s1[cb], P.a0 * s2[cb]));
 }
 
-// The item's total for this lane's own time lane: the four lanes of each
-// column meet, and lane l takes column block l / 16.
-__device__ __forceinline__ double linw_mfma_total(double acc[4]) {
+// The item's total for this lane's own time lane: lane l owns column
+// 16 cb + l % 16 of block cb for the rows 4 (l / 16) + i, so the four lanes
+// l % 16 + 16 g (g = 0..3) of a column meet through LDS (`red`, 256
+// doubles), in the order g = 0, 1, 2, 3, and lane l takes block l / 16.
+__device__ __forceinline__ double linw_mfma_total(const double acc[4], double* red) {
+  const uint32_t lane = __lane_id(), c = lane % 16u, cb = lane / 16u;
 #pragma unroll
-  for (uint32_t cb = 0; cb < 4; ++cb) {
-    acc[cb] += xor16_f64(acc[cb]);
-    acc[cb] += xor32_f64(acc[cb]);
-  }
-  const uint32_t k = __lane_id() / 16u;
-  return k == 0u ? acc[0] : (k == 1u ? acc[1] : (k == 2u ? acc[2] : acc[3]));
+  for (uint32_t j = 0; j < 4; ++j) red[lane * 4u + j] = acc[j];
+  __builtin_amdgcn_wave_barrier();  // every lane's partials are in LDS
+  double s = red[c * 4u + cb] + red[(16u + c) * 4u + cb];
+  s += red[(32u + c) * 4u + cb];
+  s += red[(48u + c) * 4u + cb];
+  __builtin_amdgcn_wave_barrier();  // reads done before the LDS is reused
+  return s;
 }
 
 // ----------------------------------------------------- one work item
@@ -397,19 +383,25 @@ __device__ __forceinline__ double item_sum(const char* __restrict__ p, uint64_t 
     };
     float4 nxt = rec(todo);
     if constexpr (PK == PK_STAR && LD == LD_LINW) {
-      // Linear-law star patches: the cosines on MFMA (linw_mfma_patch).
-      float b[4];
-      linw_mfma_b(L, b);
-      double part[4] = {0.0, 0.0, 0.0, 0.0};
-      while (todo) {
-        todo &= todo - 1ull;
-        lp[lane] = nxt;
-        __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
-        if (todo) nxt = rec(todo);
-        linw_mfma_patch(lp, b, P, part);
-        __builtin_amdgcn_wave_barrier();  // reads done before the next write
+      // Linear-law star patches: the cosines on MFMA (linw_mfma_patch),
+      // for items of two patches or more (the set-up and the final sum
+      // cost about what one patch saves). lp holds 2 PATCH float4 here:
+      // the earth vectors go to its second half before the first patch,
+      // and the final sum uses all of it after the last.
+      if (__builtin_popcountll(todo) >= 2) {
+        float b[4];
+        linw_mfma_b(L, lp + PATCH, b);
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        while (todo) {
+          todo &= todo - 1ull;
+          lp[lane] = nxt;
+          __builtin_amdgcn_wave_barrier();  // the patch is in LDS for every lane
+          if (todo) nxt = rec(todo);
+          linw_mfma_patch(lp, b, P, part);
+          __builtin_amdgcn_wave_barrier();  // reads done before the next write
+        }
+        return linw_mfma_total(part, reinterpret_cast<double*>(lp));
       }
-      return linw_mfma_total(part);
     }
     while (todo) {
       todo &= todo - 1ull;
@@ -728,7 +720,9 @@ __global__ __launch_bounds__(TILE, 5) void flux_items_kernel(
     const double* __restrict__ moms, const double* __restrict__ imoms, const double* __restrict__ lane_phi,
     const float4* __restrict__ lane_e, const uint32_t* __restrict__ lane_meta,
     const uint64_t* __restrict__ lane_masks, double* __restrict__ partials) {
-  __shared__ float4 pbuf[ECLS ? PBUF_F4 : PATCH];  // one patch of records (one wavefront per workgroup)
+  // One patch of records (one wavefront per workgroup); the linear-law
+  // MFMA path also uses a second PATCH float4 (ECLS: within PBUF_F4).
+  __shared__ float4 pbuf[ECLS ? PBUF_F4 : 2 * PATCH];
   const uint32_t id = xcd_map(blockIdx.x, n_items);
   if (id >= n_items) return;
   process_item<FX, ECLS>(id, items, segs, sets, recs, moms, imoms, lane_phi, lane_e, lane_meta, lane_masks, partials,

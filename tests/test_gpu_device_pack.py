@@ -173,3 +173,34 @@ def test_device_pack_many_sets_and_repeat(host_tuning, limb):
     r4 = lc.Batch(models, times, 0.001, 1, flux2d=f2, ferr2d=np.tile(obs["ferr"], (48, 1)),
                   weight2d=np.tile(obs["weight"], (48, 1))).run()
     assert np.array_equal(r3["chisq"][ok], r4["chisq"][ok])
+
+
+def test_device_pack_sort_pieces(host_tuning):
+    """A batch of more segments than one sort piece holds (4096): the sort
+    runs piece by piece on keys local to each piece. Tables equal the
+    host's on sets of the first and last pieces, and every result equals
+    the host-packed batch's."""
+    lc = host_tuning
+    base = W.chisq_base_text()
+    n = 1600
+    p = W.chisq_sets(n, seed=11, nlat_lo=12, nlat_hi=20)
+    models = [lc.Model.from_text(W.model_text(
+        {"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4], "nlat1f": 8, "nlat1c": 8,
+         "nlat2f": int(s[5]), "nlat2c": int(s[5])}, base)) for s in p]
+    times = W.linspace_times(-0.2, 1.2, 128)
+    obs = _obs(len(times), 12)
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=1024, **obs)
+    st = dp.stats()
+    assert st["n_segs"] > 4096, st["n_segs"]
+    for s in (0, 1, n // 2, n - 2, n - 1):
+        d = dp.check(s)
+        assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
+    rd = dp.run()
+    _host_limb(lc, 0)
+    lc.set_tuning(1024)
+    rh = lc.Batch(models, times, 0.001, 1, **obs).run()
+    ok = np.isfinite(rh["chisq"])
+    assert ok.sum() > n // 2
+    assert np.array_equal(np.isfinite(rd["chisq"]), ok)
+    assert np.array_equal(rd["chisq"][ok], rh["chisq"][ok])
+    assert np.array_equal(rd["flux"][ok], rh["flux"][ok])
