@@ -232,7 +232,6 @@ void DevicePackBatch::release_slot(Slot& s) {
   if (s.stream) (void)hipStreamDestroy(s.stream);
   if (s.h_counts) (void)hipHostFree(s.h_counts);
   if (s.work) (void)hipFree(s.work);
-  if (s.sort_temp) (void)hipFree(s.sort_temp);
   if (s.scan_temp) (void)hipFree(s.scan_temp);
   for (void*& g : s.grow) {
     if (g) (void)hipFree(g);
@@ -296,34 +295,6 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   if (max_tiles > 65535) throw std::invalid_argument("too many lane tiles in one set");
   const uint64_t n_cand = n_chunks * max_tiles;
   if (n_cand >= UINT32_MAX) throw std::invalid_argument("too many work items: split the sets");
-  // Sort pieces: runs of whole sets of at most SORT_PIECE_SEGS segments,
-  // each sorted in its own range on (segment within the piece, direction
-  // key) in at most 24 bits.
-  pieces_.clear();
-  std::vector<uint32_t> piece_seg0(ns);
-  {
-    uint64_t r = 0, r0 = 0;
-    uint32_t sg = 0, sg0 = 0;
-    auto close = [&]() {
-      if (sg == sg0) return;
-      uint32_t bits = DIR_KEY_BITS;
-      for (uint32_t v = sg - sg0 - 1u; v; v >>= 1) ++bits;
-      pieces_.push_back(SortPiece{r0, r - r0, bits});
-    };
-    for (size_t s = 0; s < ns; ++s) {
-      const uint32_t nseg = static_cast<uint32_t>(lay[s]->S.segs.size());
-      if (nseg > SORT_PIECE_SEGS) throw std::invalid_argument("a set with too many segments");
-      if (sg - sg0 + nseg > SORT_PIECE_SEGS) {
-        close();
-        r0 = r;
-        sg0 = sg;
-      }
-      piece_seg0[s] = sg0;
-      r += lay[s]->df.size();
-      sg += nseg;
-    }
-    close();
-  }
   if (n_codes >= UINT32_MAX) throw std::invalid_argument("too many (patch, tile) pairs: split the sets");
   uint64_t n_ktiles = 0;
   for (const auto& Lp : lay)
@@ -349,6 +320,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   const size_t o_df = in.add(n_raw * sizeof(float4)), o_pos = in.add(n_pos * sizeof(float4));
   const size_t o_eoff = in.add((n_raw + 1) * sizeof(uint32_t)), o_ecl = in.add(n_pairs * sizeof(double2));
   const size_t o_grids = in.add(n_grids * sizeof(PackGrid)), o_psegs = in.add(n_segs * sizeof(PackSeg));
+  const size_t o_gorder = in.add(n_grids * sizeof(uint32_t));
   const size_t o_kt = in.add(n_ktiles * sizeof(KeyTile));
   const size_t o_segs = in.add(n_segs * sizeof(SegDev)), o_psets = in.add(ns * sizeof(PackSet));
   const size_t o_sets = in.add(ns * sizeof(SetDev)), o_pseg = in.add(n_patches * sizeof(uint32_t));
@@ -434,7 +406,6 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
       G.e0 = e_set + g.e0;
       G.n = g.n;
       for (int k = 0; k <= MAX_ECL; ++k) G.seg_of[k] = g.seg_of[k] == UINT32_MAX ? UINT32_MAX : seg + g.seg_of[k];
-      G.seg_base = piece_seg0[s];
       for (uint32_t i0 = 0; i0 < g.n; i0 += KEY_TILE) ktiles[kt++] = KeyTile{grid, i0, std::min(KEY_TILE, g.n - i0)};
       pgrids[grid++] = G;
     }
@@ -581,12 +552,16 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   L.lsw = w.add(any_iscale ? n_times * 6 * sizeof(double) : 0);
   L.total = w.total;
   st_.work_bytes = L.total;
-  uint64_t max_piece = 0;
-  for (const SortPiece& P : pieces_) max_piece = std::max(max_piece, P.n);
-  sort_temp_bytes_ = std::max<size_t>(pack_sort_temp_bytes(max_piece), 16);
   scan_temp_bytes_ = std::max<size_t>(pack_scan_temp_bytes(n_cand, n_chunks), 16);
   n_cand_ = n_cand;
   auto din = [&](size_t off) { return d_in_ + off; };
+
+  // The sort's workgroups take the grids largest first.
+  {
+    uint32_t* gorder = reinterpret_cast<uint32_t*>(at(o_gorder));
+    for (uint32_t g = 0; g < n_grids; ++g) gorder[g] = g;
+    std::stable_sort(gorder, gorder + n_grids, [&](uint32_t a, uint32_t b) { return pgrids[a].n > pgrids[b].n; });
+  }
 
   // The inputs and sizes every slot shares.
   PackDev& p = pd0_;
@@ -597,6 +572,7 @@ void DevicePackBatch::build(std::vector<std::unique_ptr<SetLayout>> lay) {
   p.n_raw = n_raw;
   p.grids = reinterpret_cast<const PackGrid*>(din(o_grids));
   p.n_grids = static_cast<uint32_t>(n_grids);
+  p.grid_order = reinterpret_cast<const uint32_t*>(din(o_gorder));
   p.ktiles = reinterpret_cast<const KeyTile*>(din(o_kt));
   p.n_ktiles = static_cast<uint32_t>(n_ktiles);
   p.psegs = reinterpret_cast<const PackSeg*>(din(o_psegs));
@@ -655,7 +631,6 @@ void DevicePackBatch::add_slot() {
     hip_check(hipEventCreateWithFlags(&s.side.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
     hip_check(hipEventCreateWithFlags(&s.side.join, hipEventDisableTiming), "hipEventCreateWithFlags");
     hip_check(hipMalloc(reinterpret_cast<void**>(&s.work), wl_.total + segs_bytes_), "hipMalloc pack tables");
-    hip_check(hipMalloc(&s.sort_temp, sort_temp_bytes_), "hipMalloc sort storage");
     hip_check(hipMalloc(&s.scan_temp, scan_temp_bytes_), "hipMalloc scan storage");
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&s.h_counts), sizeof(PackCounts), hipHostMallocDefault),
               "hipHostMalloc counts");
@@ -819,8 +794,7 @@ void DevicePackBatch::ensure_capacity(Slot& sl, const PackCounts& c) {
 
 void DevicePackBatch::pack(Slot& sl) {
   hipStream_t st = sl.stream;
-  hip_check(launch_pack_count(sl.pd, sl.sort_temp, sort_temp_bytes_, sl.scan_temp, scan_temp_bytes_, pieces_.data(),
-                              pieces_.size(), st),
+  hip_check(launch_pack_count(sl.pd, sl.scan_temp, scan_temp_bytes_, st),
             "pack");
   hip_check(hipMemcpyAsync(sl.h_counts, sl.pd.counts, sizeof(PackCounts), hipMemcpyDeviceToHost, st),
             "hipMemcpyAsync counts");

@@ -119,7 +119,6 @@ class DevicePackBatch {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;  // recorded after each launch; the context stream waits on it
     char* work = nullptr;
-    void* sort_temp = nullptr;
     void* scan_temp = nullptr;
     PackCounts* h_counts = nullptr;  // pinned
     PackCounts last_counts{};
@@ -170,8 +169,7 @@ class DevicePackBatch {
   char* d_in_ = nullptr;
   size_t in_bytes_ = 0;
   WorkLayout wl_{};
-  size_t sort_temp_bytes_ = 0, scan_temp_bytes_ = 0;
-  std::vector<SortPiece> pieces_;  // the sort's pieces (whole sets each)
+  size_t scan_temp_bytes_ = 0;
   uint64_t n_cand_ = 0;
   // The input pointers and sizes every slot starts from.
   PackDev pd0_;

@@ -43,18 +43,7 @@ struct PackGrid {
   uint64_t e0;
   uint32_t n;
   uint32_t seg_of[MAX_ECL + 1];  // segment of each eclipse-pair count (0xFFFFFFFF: none)
-  uint32_t seg_base;             // first segment of the grid's sort piece
 };
-
-// A run of whole sets sorted on its own: raw elements [r0, r0 + n) are
-// also its sorted positions, and its keys are (segment - the piece's first
-// segment) << DIR_KEY_BITS | direction key, in `bits` bits.
-struct SortPiece {
-  uint64_t r0, n;
-  uint32_t bits;
-};
-// Segments of one sort piece at most (a 24-bit key with DIR_KEY_BITS = 12).
-constexpr uint32_t SORT_PIECE_SEGS = 4096;
 
 struct PackSet {
   uint32_t lane_off, n_tiles;  // first lane (batch-wide, multiple of TILE), lane tiles
@@ -110,6 +99,7 @@ struct PackDev {
   uint64_t n_raw = 0;
   const PackGrid* grids = nullptr;
   uint32_t n_grids = 0;
+  const uint32_t* grid_order = nullptr;  // grids by size, largest first (the sort's workgroups)
   const KeyTile* ktiles = nullptr;
   uint32_t n_ktiles = 0;
   const PackSeg* psegs = nullptr;
@@ -137,7 +127,11 @@ struct PackDev {
   uint32_t limb = 0;
   // work and outputs
   uint32_t* wmax = nullptr;          // per segment: largest weight (FP32 bits), then a bad-weight flag
-  uint32_t* keys = nullptr;          // [n_raw] x 2 (sort in / out)
+  // Sort keys and raw element indices, [n_raw] each. The sort's first pass
+  // goes from keys / vals to keys_s / vals_s, its second back: the kernels
+  // after it read the sorted order through vals_s, which launch_pack_count
+  // points at vals for them.
+  uint32_t* keys = nullptr;
   uint32_t* vals = nullptr;
   uint32_t* keys_s = nullptr;
   uint32_t* vals_s = nullptr;
@@ -200,16 +194,13 @@ struct PackDev {
   double* partials = nullptr;
 };
 
-// Bytes of the radix sort's temporary storage for a piece of n keys.
-size_t pack_sort_temp_bytes(uint64_t n);
 size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks);
 // The pack in two phases on `st`. The first makes the records, cones,
 // moments, codes, lane splits and item candidates and leaves the counts of
 // what the second writes in d.counts; the host reads them and sizes the
 // second phase's tables. The second writes the work items, their lane-mask
 // groups and moment sums, and the limb tables.
-hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                             size_t scan_temp_bytes, const SortPiece* pieces, size_t n_pieces, hipStream_t st);
+hipError_t launch_pack_count(const PackDev& d, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
 hipError_t launch_pack_fill(const PackDev& d, uint64_t n_entries, hipStream_t st);
 
 }  // namespace lch

@@ -5,7 +5,8 @@
 
 #pragma clang fp contract(off)
 
-#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/block/block_radix_rank.hpp>
+#include <rocprim/block/block_scan.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
@@ -35,8 +36,9 @@ __device__ inline float4 load_f4(const char* r) {
   return make_float4(a.x, a.y, b.x, b.y);
 }
 
-// Sort key of every raw element: its segment (within its sort piece) above
-// the spatial key. With
+// Sort key of every raw element: its eclipse-pair count (the segment
+// within its grid: a grid's segments are numbered by that count, in order)
+// above the spatial key. With
 // it, per segment, the largest weight and whether a weight is negative or
 // not finite, for the linear-law rescaling. One workgroup per key tile.
 __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
     uint32_t k = d.raw_eoff[e + 1] - d.raw_eoff[e];
     k = k < static_cast<uint32_t>(MAX_ECL) ? k : static_cast<uint32_t>(MAX_ECL);
     const float4 v = d.raw_df[e];
-    d.keys[e] = ((G.seg_of[k] - G.seg_base) << DIR_KEY_BITS) | direction_key(v.x, v.y, v.z);
+    d.keys[e] = (k << DIR_KEY_BITS) | direction_key(v.x, v.y, v.z);
     d.vals[e] = static_cast<uint32_t>(e);
     const float w = v.w;
     if (!(w >= 0.0f) || !isfinite(w)) {
@@ -91,6 +93,75 @@ __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
       // Non-negative FP32 values order as their bit patterns.
       if (wm > 0.0f) atomicMax(&d.wmax[2u * seg], __float_as_uint(wm));
       if ((b >> c) & 1u) atomicOr(&d.wmax[2u * seg + 1u], 1u);
+    }
+  }
+}
+
+// The sort: each grid's elements in place in their own range (a grid's
+// raw elements are also its sorted positions: its segments are numbered in
+// eclipse-pair order, after the previous grid's), by (eclipse-pair count,
+// direction key), a 15-bit key, stable. A workgroup per grid, largest grids
+// first, two 8-bit LSD passes: a histogram of the grid's digits in LDS,
+// its exclusive scan, then the grid in tiles of SS_TILE elements in order,
+// each ranked by rocPRIM's block radix rank and scattered to its digit's
+// running offset. The rank's match form keeps LDS small (per-wavefront
+// digit counters) and is stable in warp-striped order: wavefront w holds
+// elements w * 64 * SS_ITEMS + 64 i + lane of the tile, which is also the
+// coalesced way to load them.
+constexpr uint32_t SS_BLOCK = 256, SS_ITEMS = 16, SS_TILE = SS_BLOCK * SS_ITEMS;
+using SsRank = rocprim::block_radix_rank<SS_BLOCK, 8, rocprim::block_radix_rank_algorithm::match>;
+using SsScan = rocprim::block_scan<uint32_t, SS_BLOCK>;
+static_assert(SsRank::digits_per_thread == 1, "one digit per thread");
+static_assert(MAX_ECL < 8 && DIR_KEY_BITS + 3 <= 16, "keys of at most 16 bits: two 8-bit passes");
+
+__global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
+  __shared__ typename SsRank::storage_type rank_storage;
+  __shared__ typename SsScan::storage_type scan_storage;
+  __shared__ uint32_t hist[256], offs[256], pref[256];
+  const PackGrid G = d.grids[d.grid_order[blockIdx.x]];
+  const uint64_t e0 = G.e0;
+  const uint32_t n = G.n, t = threadIdx.x;
+  for (uint32_t pass = 0; pass < 2; ++pass) {
+    const uint32_t* kin = (pass == 0 ? d.keys : d.keys_s) + e0;
+    const uint32_t* vin = (pass == 0 ? d.vals : d.vals_s) + e0;
+    uint32_t* kout = (pass == 0 ? d.keys_s : d.keys) + e0;
+    uint32_t* vout = (pass == 0 ? d.vals_s : d.vals) + e0;
+    const uint32_t shift = 8u * pass;
+    hist[t] = 0u;
+    __syncthreads();  // also: the previous pass's writes are done
+    for (uint32_t i = t; i < n; i += SS_BLOCK) atomicAdd(&hist[(kin[i] >> shift) & 0xffu], 1u);
+    __syncthreads();
+    uint32_t start = 0u;
+    SsScan().exclusive_scan(hist[t], start, 0u, scan_storage);
+    offs[t] = start;
+    for (uint32_t base = 0; base < n; base += SS_TILE) {
+      uint32_t key[SS_ITEMS], val[SS_ITEMS], rank[SS_ITEMS];
+      const uint32_t first = base + (t / TILE) * (TILE * SS_ITEMS) + t % TILE;  // warp-striped
+#pragma unroll
+      for (uint32_t i = 0; i < SS_ITEMS; ++i) {
+        const uint32_t idx = first + i * TILE;
+        // Past the grid's end: digit 255, ranked after every element of
+        // the tile (they come last in its order), never written.
+        key[i] = idx < n ? kin[idx] : 0xffffffffu;
+        val[i] = idx < n ? vin[idx] : 0u;
+      }
+      uint32_t prefix[1], counts[1];
+      SsRank().rank_keys(
+          key, rank, rank_storage, [shift](const uint32_t& k) { return (k >> shift) & 0xffu; }, prefix, counts);
+      pref[t] = prefix[0];
+      __syncthreads();  // pref, and offs of the previous tile
+#pragma unroll
+      for (uint32_t i = 0; i < SS_ITEMS; ++i) {
+        const uint32_t idx = first + i * TILE;
+        if (idx < n) {
+          const uint32_t dg = (key[i] >> shift) & 0xffu;
+          const uint32_t pos = offs[dg] + rank[i] - pref[dg];
+          kout[pos] = key[i];
+          vout[pos] = val[i];
+        }
+      }
+      __syncthreads();  // offs and pref are read
+      offs[t] += counts[0];
     }
   }
 }
@@ -954,13 +1025,6 @@ uint32_t blocks(uint64_t n, uint32_t b = kBlock) { return static_cast<uint32_t>(
 
 }  // namespace
 
-size_t pack_sort_temp_bytes(uint64_t n) {
-  size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, bytes, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                                  static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
-                                  static_cast<size_t>(n), 0, 32);
-  return bytes;
-}
 
 size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks) {
   size_t a = 0, b = 0;
@@ -972,8 +1036,11 @@ size_t pack_scan_temp_bytes(uint64_t n_cand, uint64_t n_chunks) {
   return a > b ? a : b;
 }
 
-hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp_bytes, void* scan_temp,
-                             size_t scan_temp_bytes, const SortPiece* pieces, size_t n_pieces, hipStream_t st) {
+hipError_t launch_pack_count(const PackDev& d0, void* scan_temp, size_t scan_temp_bytes, hipStream_t st) {
+  // After the sort's two passes the sorted order is in keys / vals.
+  PackDev d = d0;
+  d.keys_s = d0.keys;
+  d.vals_s = d0.vals;
   hipError_t e = hipSuccess;
   if (d.limb) {
     e = hipMemsetAsync(d.split_stats, 0, 2 * sizeof(unsigned long long), st);
@@ -984,17 +1051,8 @@ hipError_t launch_pack_count(const PackDev& d, void* sort_temp, size_t sort_temp
     if (e != hipSuccess) return e;
   }
   if (d.n_raw) {
-    hipLaunchKernelGGL(keys_kernel, dim3(d.n_ktiles), dim3(kBlock), 0, st, d);
-    // Each piece (whole sets, at most SORT_PIECE_SEGS segments) is sorted
-    // in place in its own range: three 8-bit passes instead of four.
-    for (size_t i = 0; i < n_pieces; ++i) {
-      const SortPiece& P = pieces[i];
-      if (!P.n) continue;
-      size_t tb = sort_temp_bytes;
-      e = rocprim::radix_sort_pairs(sort_temp, tb, d.keys + P.r0, d.keys_s + P.r0, d.vals + P.r0, d.vals_s + P.r0,
-                                    static_cast<size_t>(P.n), 0, P.bits, st);
-      if (e != hipSuccess) return e;
-    }
+    hipLaunchKernelGGL(keys_kernel, dim3(d0.n_ktiles), dim3(kBlock), 0, st, d0);
+    if (d0.n_grids) hipLaunchKernelGGL(segsort_kernel, dim3(d0.n_grids), dim3(SS_BLOCK), 0, st, d0);
   }
   if (d.n_segs) hipLaunchKernelGGL(seg_flags_kernel, dim3(blocks(d.n_segs)), dim3(kBlock), 0, st, d);
   if (d.n_segs) hipLaunchKernelGGL(pad_kernel, dim3(d.n_segs), dim3(PATCH), 0, st, d);
