@@ -117,22 +117,25 @@ static_assert(MAX_ECL < 8 && DIR_KEY_BITS + 3 <= 16, "keys of at most 16 bits: t
 __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
   __shared__ typename SsRank::storage_type rank_storage;
   __shared__ typename SsScan::storage_type scan_storage;
-  __shared__ uint32_t hist[256], offs[256], pref[256];
+  // hist[0]: the first pass's digit counts (a loop over the grid); hist[1]:
+  // the second's, counted while the first pass scatters.
+  __shared__ uint32_t hist[2][256], offs[256], pref[256];
   const PackGrid G = d.grids[d.grid_order[blockIdx.x]];
   const uint64_t e0 = G.e0;
   const uint32_t n = G.n, t = threadIdx.x;
+  hist[0][t] = 0u;
+  hist[1][t] = 0u;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += SS_BLOCK) atomicAdd(&hist[0][d.keys[e0 + i] & 0xffu], 1u);
   for (uint32_t pass = 0; pass < 2; ++pass) {
     const uint32_t* kin = (pass == 0 ? d.keys : d.keys_s) + e0;
     const uint32_t* vin = (pass == 0 ? d.vals : d.vals_s) + e0;
     uint32_t* kout = (pass == 0 ? d.keys_s : d.keys) + e0;
     uint32_t* vout = (pass == 0 ? d.vals_s : d.vals) + e0;
     const uint32_t shift = 8u * pass;
-    hist[t] = 0u;
-    __syncthreads();  // also: the previous pass's writes are done
-    for (uint32_t i = t; i < n; i += SS_BLOCK) atomicAdd(&hist[(kin[i] >> shift) & 0xffu], 1u);
-    __syncthreads();
+    __syncthreads();  // the pass's counts are complete; the previous pass's writes are done
     uint32_t start = 0u;
-    SsScan().exclusive_scan(hist[t], start, 0u, scan_storage);
+    SsScan().exclusive_scan(hist[pass][t], start, 0u, scan_storage);
     offs[t] = start;
     for (uint32_t base = 0; base < n; base += SS_TILE) {
       uint32_t key[SS_ITEMS], val[SS_ITEMS], rank[SS_ITEMS];
@@ -158,6 +161,7 @@ __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
           const uint32_t pos = offs[dg] + rank[i] - pref[dg];
           kout[pos] = key[i];
           vout[pos] = val[i];
+          if (pass == 0) atomicAdd(&hist[1][(key[i] >> 8) & 0xffu], 1u);
         }
       }
       __syncthreads();  // offs and pref are read

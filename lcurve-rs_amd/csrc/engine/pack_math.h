@@ -186,9 +186,8 @@ __host__ __device__ inline double cos_front(const CodeConsts& k, float cosw) {
 
 // The key that orders a segment's elements into patches: the top
 // DIR_KEY_BITS bits of direction_key16, a Hilbert curve over 64 x 64 cells
-// of the octahedral map. Ties keep the grid order. Twelve bits leave the
-// segment index 12 bits of a 24-bit sort key (three 8-bit radix passes)
-// for up to 4096 segments.
+// of the octahedral map. Ties keep the grid order. With the eclipse-pair
+// count above it, a grid's sort key has 15 bits: two 8-bit radix passes.
 constexpr uint32_t DIR_KEY_BITS = 12;
 __host__ __device__ inline uint32_t direction_key(float fx, float fy, float fz) {
   return direction_key16(fx, fy, fz) >> (16u - DIR_KEY_BITS);

@@ -175,11 +175,11 @@ def test_device_pack_many_sets_and_repeat(host_tuning, limb):
     assert np.array_equal(r3["chisq"][ok], r4["chisq"][ok])
 
 
-def test_device_pack_sort_pieces(host_tuning):
-    """A batch of more segments than one sort piece holds (4096): the sort
-    runs piece by piece on keys local to each piece. Tables equal the
-    host's on sets of the first and last pieces, and every result equals
-    the host-packed batch's."""
+def test_device_pack_many_grids(host_tuning):
+    """1600 sets, over 4096 segments: each grid is sorted by its own
+    workgroup, the largest first, so grids finish out of order. Tables
+    equal the host's on sets at both ends and in the middle, and every
+    result equals the host-packed batch's."""
     lc = host_tuning
     base = W.chisq_base_text()
     n = 1600
