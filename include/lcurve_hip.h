@@ -169,6 +169,9 @@ int lch_batch_build_models(lch_ctx* ctx, const lch_model* const* models, int nse
                            const double* ferr, const double* weight, const int* ndiv, int n,
                            const double* flux2d, const double* ferr2d, const double* weight2d,
                            int flags, lch_batch** out);
+/* Frees the batch's device tables after waiting on the whole device (a
+ * wait on the launching stream alone left freed memory allocated on
+ * ROCm 7.2). */
 void lch_batch_free(lch_batch* b);
 /* Replace the observations of every set: flux2d / ferr2d / weight2d are
  * [nsets][n] (a null array keeps the current values). */
@@ -245,6 +248,7 @@ int lch_dpack_build(lch_ctx* ctx, const lch_model* const* models, int nsets, con
                     const double* expose, const double* flux, const double* ferr, const double* weight,
                     const int* ndiv, int n, const double* flux2d, const double* ferr2d, const double* weight2d,
                     int chunk, int limb, lch_dpack** out);
+/* Frees the batch's device tables after waiting on the whole device. */
 void lch_dpack_free(lch_dpack* b);
 /* Copy the staged grids and tables to the device again (one copy from
  * pinned host memory) and wait; the build already did it once. */
