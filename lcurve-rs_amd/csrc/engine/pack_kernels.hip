@@ -108,13 +108,13 @@ __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
 // digit counters) and is stable in warp-striped order: wavefront w holds
 // elements w * 64 * SS_ITEMS + 64 i + lane of the tile, which is also the
 // coalesced way to load them.
-constexpr uint32_t SS_BLOCK = 256, SS_ITEMS = 16, SS_TILE = SS_BLOCK * SS_ITEMS;
+constexpr uint32_t SS_BLOCK = 512, SS_ITEMS = 16, SS_TILE = SS_BLOCK * SS_ITEMS;
 using SsRank = rocprim::block_radix_rank<SS_BLOCK, 8, rocprim::block_radix_rank_algorithm::match>;
 using SsScan = rocprim::block_scan<uint32_t, SS_BLOCK>;
-static_assert(SsRank::digits_per_thread == 1, "one digit per thread");
+static_assert(SsRank::digits_per_thread == 1 && SS_BLOCK >= 256, "digit t on thread t");
 static_assert(MAX_ECL < 8 && DIR_KEY_BITS + 3 <= 16, "keys of at most 16 bits: two 8-bit passes");
 
-__global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
+__global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void segsort_kernel(PackDev d) {
   __shared__ typename SsRank::storage_type rank_storage;
   __shared__ typename SsScan::storage_type scan_storage;
   // hist[0]: the first pass's digit counts (a loop over the grid); hist[1]:
@@ -123,8 +123,10 @@ __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
   const PackGrid G = d.grids[d.grid_order[blockIdx.x]];
   const uint64_t e0 = G.e0;
   const uint32_t n = G.n, t = threadIdx.x;
-  hist[0][t] = 0u;
-  hist[1][t] = 0u;
+  if (t < 256) {
+    hist[0][t] = 0u;
+    hist[1][t] = 0u;
+  }
   __syncthreads();
   for (uint32_t i = t; i < n; i += SS_BLOCK) atomicAdd(&hist[0][d.keys[e0 + i] & 0xffu], 1u);
   for (uint32_t pass = 0; pass < 2; ++pass) {
@@ -135,8 +137,8 @@ __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
     const uint32_t shift = 8u * pass;
     __syncthreads();  // the pass's counts are complete; the previous pass's writes are done
     uint32_t start = 0u;
-    SsScan().exclusive_scan(hist[pass][t], start, 0u, scan_storage);
-    offs[t] = start;
+    SsScan().exclusive_scan(t < 256 ? hist[pass][t] : 0u, start, 0u, scan_storage);
+    if (t < 256) offs[t] = start;
     for (uint32_t base = 0; base < n; base += SS_TILE) {
       uint32_t key[SS_ITEMS], val[SS_ITEMS], rank[SS_ITEMS];
       const uint32_t first = base + (t / TILE) * (TILE * SS_ITEMS) + t % TILE;  // warp-striped
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
       uint32_t prefix[1], counts[1];
       SsRank().rank_keys(
           key, rank, rank_storage, [shift](const uint32_t& k) { return (k >> shift) & 0xffu; }, prefix, counts);
-      pref[t] = prefix[0];
+      if (t < 256) pref[t] = prefix[0];
       __syncthreads();  // pref, and offs of the previous tile
 #pragma unroll
       for (uint32_t i = 0; i < SS_ITEMS; ++i) {
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(SS_BLOCK) void segsort_kernel(PackDev d) {
         }
       }
       __syncthreads();  // offs and pref are read
-      offs[t] += counts[0];
+      if (t < 256) offs[t] += counts[0];
     }
   }
 }
