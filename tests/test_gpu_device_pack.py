@@ -204,3 +204,43 @@ def test_device_pack_many_grids(host_tuning):
     assert np.array_equal(np.isfinite(rd["chisq"]), ok)
     assert np.array_equal(rd["chisq"][ok], rh["chisq"][ok])
     assert np.array_equal(rd["flux"][ok], rh["flux"][ok])
+
+
+@pytest.mark.timeout(600)
+def test_device_pack_chisq_4096_full_size(host_tuning):
+    """The chi-squared workload at its full size: 4096 sets, 2000 times,
+    star-2 grids of 10k-150k elements, chunk 4096. Tables equal the host's
+    on sets at both ends and in the middle; a second launch gives the same
+    values bit for bit; curves and autoscaled chi-squared on a sample of
+    sets meet the parity bar against the FP64 oracle (1e-6 relative on the
+    flux, 1e-5 on chi-squared)."""
+    lc = host_tuning
+    n = 4096
+    base = W.chisq_base_text()
+    texts = [W.model_text({"q": s[0], "iangle": s[1], "r1": s[2], "t1": s[3], "t2": s[4],
+                           "nlat2f": int(s[5]), "nlat2c": int(s[5])}, base) for s in W.chisq_sets(n)]
+    times = W.linspace_times(-0.2, 1.2, 2000)
+    models = [lc.Model.from_text(x) for x in texts]
+    dp = lc.DevicePackBatch(models, times, 0.001, 1, chunk=4096)
+    st = dp.stats()
+    assert st["n_sets"] == n and st["n_valid"] == n, st
+    print(f"chisq_4096: {st['input_bytes'] / 1e9:.2f} GB of raw grids", flush=True)
+    true = dp.run(scale=False)["flux"]
+    assert np.all(np.isfinite(true)) and np.all(true > 0.0)
+    rng = np.random.default_rng(4242)
+    obs = true * (1.0 + 0.01 * rng.standard_normal(true.shape))
+    sig = 0.01 * np.abs(true)
+    dp.set_obs(obs, sig, np.ones_like(obs))
+    for s in (0, n // 2, n - 1):
+        d = dp.check(s)
+        assert all(d[k] == 0 for k in CHECK_KEYS), (s, d)
+    r1 = dp.run()
+    r2 = dp.run()
+    assert np.array_equal(r1["flux"], r2["flux"]) and np.array_equal(r1["chisq"], r2["chisq"])
+    assert np.all(np.isfinite(r1["chisq"])) and np.all(r1["chisq"] > 0.0)
+    idx = np.array([0, 1365, 2730, n - 1])
+    ob = pyoracle.OracleBatch([texts[i] for i in idx], times, 0.001, 1, nthreads=16,
+                              flux2d=obs[idx], ferr2d=sig[idx], weight2d=np.ones((len(idx), len(times))))
+    chi, flux = ob.flux(scale=True, nthreads=16)
+    assert np.max(np.abs(r1["flux"][idx] / flux - 1)) <= 1e-6
+    assert np.max(np.abs(r1["chisq"][idx] / chi - 1)) <= 1e-5
