@@ -103,11 +103,14 @@ __global__ __launch_bounds__(kBlock) void keys_kernel(PackDev d) {
 // direction key), a 15-bit key, stable. A workgroup per grid, largest grids
 // first, two 8-bit LSD passes: a histogram of the grid's digits in LDS,
 // its exclusive scan, then the grid in tiles of SS_TILE elements in order,
-// each ranked by rocPRIM's block radix rank and scattered to its digit's
-// running offset. The rank's match form keeps LDS small (per-wavefront
-// digit counters) and is stable in warp-striped order: wavefront w holds
-// elements w * 64 * SS_ITEMS + 64 i + lane of the tile, which is also the
-// coalesced way to load them.
+// each ranked by rocPRIM's block radix rank, placed in LDS in its ranked
+// order and written out from there, consecutive threads to consecutive
+// positions of each digit's run (from registers, a wavefront's 64 stores
+// went to up to 64 runs and left partly written cache lines). The rank's
+// match form keeps its own LDS small (per-wavefront digit counters) and is
+// stable in warp-striped order: wavefront w holds elements
+// w * 64 * SS_ITEMS + 64 i + lane of the tile, which is also the coalesced
+// way to load them.
 constexpr uint32_t SS_BLOCK = 512, SS_ITEMS = 16, SS_TILE = SS_BLOCK * SS_ITEMS;
 using SsRank = rocprim::block_radix_rank<SS_BLOCK, 8, rocprim::block_radix_rank_algorithm::match>;
 using SsScan = rocprim::block_scan<uint32_t, SS_BLOCK>;
@@ -120,6 +123,8 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
   // hist[0]: the first pass's digit counts (a loop over the grid); hist[1]:
   // the second's, counted while the first pass scatters.
   __shared__ uint32_t hist[2][256], offs[256], pref[256];
+  // The tile in its ranked order: keys, then values.
+  __shared__ uint32_t xk[SS_TILE], xv[SS_TILE];
   const PackGrid G = d.grids[d.grid_order[blockIdx.x]];
   const uint64_t e0 = G.e0;
   const uint32_t n = G.n, t = threadIdx.x;
@@ -154,19 +159,24 @@ __global__ __launch_bounds__(SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) v
       SsRank().rank_keys(
           key, rank, rank_storage, [shift](const uint32_t& k) { return (k >> shift) & 0xffu; }, prefix, counts);
       if (t < 256) pref[t] = prefix[0];
-      __syncthreads();  // pref, and offs of the previous tile
 #pragma unroll
       for (uint32_t i = 0; i < SS_ITEMS; ++i) {
-        const uint32_t idx = first + i * TILE;
-        if (idx < n) {
-          const uint32_t dg = (key[i] >> shift) & 0xffu;
-          const uint32_t pos = offs[dg] + rank[i] - pref[dg];
-          kout[pos] = key[i];
-          vout[pos] = val[i];
-          if (pass == 0) atomicAdd(&hist[1][(key[i] >> 8) & 0xffu], 1u);
-        }
+        xk[rank[i]] = key[i];
+        xv[rank[i]] = val[i];
       }
-      __syncthreads();  // offs and pref are read
+      __syncthreads();  // the ranked tile, pref, and offs of the previous tile
+      // The tile's m elements of the grid hold ranks 0..m-1: the ones past
+      // the grid's end have the last digit and come last in the tile's order.
+      const uint32_t m = n - base < SS_TILE ? n - base : SS_TILE;
+      for (uint32_t j = t; j < m; j += SS_BLOCK) {
+        const uint32_t k = xk[j];
+        const uint32_t dg = (k >> shift) & 0xffu;
+        const uint32_t pos = offs[dg] + j - pref[dg];
+        kout[pos] = k;
+        vout[pos] = xv[j];
+        if (pass == 0) atomicAdd(&hist[1][(k >> 8) & 0xffu], 1u);
+      }
+      __syncthreads();  // offs, pref and the ranked tile are read
       if (t < 256) offs[t] += counts[0];
     }
   }
