@@ -932,28 +932,28 @@ __global__ __launch_bounds__(kBlock) void scatter_groups_kernel(PackDev d) {
   }
 }
 
-// Moment entries, a thread each: the sum, in patch order, of the moments
-// of the patches in its mask.
+// Moment entries: the sum, in patch order, of the moments of the patches
+// in each entry's mask. Ten lanes per entry, one pair of the 20 components
+// each, six entries per wavefront (lanes 60-63 idle): an entry's patch rows
+// are read, and its sums written, as whole 160-byte runs.
+constexpr uint32_t IM_LANES = MOM_STRIDE / 2, IM_PER_WAVE = TILE / IM_LANES;
+constexpr uint32_t IM_PER_BLOCK = IM_PER_WAVE * (kBlock / TILE);
+static_assert(MOM_STRIDE % 2 == 0 && IM_PER_WAVE >= 1, "component pairs per lane");
 __global__ __launch_bounds__(kBlock) void imoms_kernel(PackDev d) {
-  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const uint32_t wave = threadIdx.x / TILE, lane = threadIdx.x % TILE;
+  const uint32_t sub = lane / IM_LANES, c = lane % IM_LANES;
+  if (sub >= IM_PER_WAVE) return;
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * IM_PER_BLOCK + wave * IM_PER_WAVE + sub;
   const uint64_t n = d.counts->items.moms;
   if (e >= n || e >= d.cap_imoms) return;
-  const double* m0 = d.moms + static_cast<uint64_t>(d.ment_base[e]) * MOM_STRIDE;
-  double s[MOM_STRIDE];
-#pragma unroll
-  for (uint32_t c = 0; c < MOM_STRIDE; ++c) s[c] = 0.0;
+  const double* m0 = d.moms + static_cast<uint64_t>(d.ment_base[e]) * MOM_STRIDE + 2u * c;
+  double sx = 0.0, sy = 0.0;
   for (uint64_t mm = d.ment_mask[e]; mm; mm &= mm - 1ull) {
-    const double2* a = reinterpret_cast<const double2*>(m0 + static_cast<uint64_t>(__builtin_ctzll(mm)) * MOM_STRIDE);
-#pragma unroll
-    for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) {
-      const double2 v = a[c];
-      s[2 * c] = s[2 * c] + v.x;
-      s[2 * c + 1] = s[2 * c + 1] + v.y;
-    }
+    const double2 v = *reinterpret_cast<const double2*>(m0 + static_cast<uint64_t>(__builtin_ctzll(mm)) * MOM_STRIDE);
+    sx = sx + v.x;
+    sy = sy + v.y;
   }
-  double2* o = reinterpret_cast<double2*>(d.imoms + e * MOM_STRIDE);
-#pragma unroll
-  for (uint32_t c = 0; c < MOM_STRIDE / 2; ++c) o[c] = make_double2(s[2 * c], s[2 * c + 1]);
+  reinterpret_cast<double2*>(d.imoms + e * MOM_STRIDE)[c] = make_double2(sx, sy);
 }
 
 // The limb tables of each chunk with limb lanes, a wavefront (a workgroup)
@@ -1112,7 +1112,7 @@ hipError_t launch_pack_fill(const PackDev& d, uint64_t n_entries, hipStream_t st
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nc)), dim3(kBlock), 0, st, d);
     if (d.limb) hipLaunchKernelGGL(scatter_groups_kernel, dim3(kListBlocks), dim3(kBlock), 0, st, d);
   }
-  if (n_entries) hipLaunchKernelGGL(imoms_kernel, dim3(blocks(n_entries)), dim3(kBlock), 0, st, d);
+  if (n_entries) hipLaunchKernelGGL(imoms_kernel, dim3(blocks(n_entries, IM_PER_BLOCK)), dim3(kBlock), 0, st, d);
   if (d.limb && d.n_chunks) hipLaunchKernelGGL(limb_fill_kernel, dim3(d.n_chunks), dim3(TILE), 0, st, d);
   return hipGetLastError();
 }
